@@ -1,0 +1,250 @@
+/*
+ * paves_gpu.h — C ABI of the MI355X query engine (libpaves_amd.so).
+ *
+ * This library is a drop-in for PAVES's CUDA object: it exports the same
+ * nine entry points with the same argument meaning, return codes and byte
+ * layouts, so engine.cpp (built with -DFV_GPU) links against it unchanged.
+ *
+ *   reference interface                                this header
+ *   extension/paves/src/engine/gpu_engine.h:24      -> FvGpuIndex (opaque)
+ *   extension/paves/src/engine/gpu_engine.h:52-60   -> FvGpuQueryDesc
+ *   extension/paves/src/engine/engine_api.h:102-117 -> FvGpuPredNode, FvGpuSpan
+ *   extension/paves/src/engine/engine_api.h:145-148 -> FvHit
+ *   extension/paves/src/engine/gpu_engine.h:63-109  -> fv_gpu_* entry points
+ *   extension/paves/src/engine/engine_api.h:194-195 -> fv_gpu_tile_variant_qt,
+ *                                                     fv_gpu_mem_probe
+ *
+ * When compiled together with the reference headers (engine_api.h already
+ * included) the shared types are taken from there; the static size checks
+ * below hold either way.
+ *
+ * Contract (as in the reference): every entry point is synchronous; errors
+ * return -1 (NULL for upload) with a message in errbuf; no C++ exception
+ * crosses the ABI; device initialisation is lazy (first upload), so it
+ * happens after fork().
+ *
+ * Entry points under "extensions" have no counterpart in the reference.
+ */
+#ifndef PAVES_AMD_GPU_H
+#define PAVES_AMD_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#ifndef PAVES_ENGINE_API_H
+/* ---- shared types, byte-identical to engine_api.h ---- */
+typedef enum { FV_CMP_LT, FV_CMP_LE, FV_CMP_EQ, FV_CMP_GE, FV_CMP_GT, FV_CMP_NE } FvCmpOp;
+typedef enum { FV_NODE_CMP, FV_NODE_IN, FV_NODE_AND, FV_NODE_OR, FV_NODE_NOT } FvNodeKind;
+
+/* engine_api.h:102-112 — one predicate node; children/IN values are the
+ * ranges [child0, child0+nchildren) / [val0, val0+nvalues) of side arrays */
+typedef struct FvGpuPredNode {
+    int32_t kind;
+    int32_t col;
+    int32_t op;
+    int32_t nchildren;
+    int32_t child0;
+    int32_t nvalues;
+    int32_t val0;
+    int32_t pad;
+    double value;
+} FvGpuPredNode;
+
+/* engine_api.h:114-117 — [lo, hi) positions in post_rows(col) */
+typedef struct FvGpuSpan {
+    int32_t col;
+    uint32_t lo, hi;
+} FvGpuSpan;
+
+/* engine_api.h:145-148 */
+typedef struct FvHit {
+    uint64_t node;
+    float dist; /* squared L2 */
+} FvHit;
+#endif /* PAVES_ENGINE_API_H */
+
+#ifndef PAVES_GPU_ENGINE_H
+typedef struct FvGpuIndex FvGpuIndex;
+
+/* gpu_engine.h:33-34. FP32: exact, bit-identical ids and distances to the
+ * CPU cursors. SQ8 (the reference's reduced-precision request) is served
+ * here by the BF16 path below, which keeps the exact contract. */
+#define FV_GPU_PREC_FP32 0
+#define FV_GPU_PREC_SQ8 1
+
+#define FV_GPU_MAX_TOPB 4096   /* gpu_engine.h:37 */
+#define FV_GPU_MAX_SPANS 4096  /* gpu_engine.h:39 */
+#define FV_GPU_MAX_EF 512      /* gpu_engine.h:42 */
+#define FV_GPU_MAX_BATCH 1024  /* gpu_engine.h:44 */
+
+/* gpu_engine.h:52-60 */
+typedef struct FvGpuQueryDesc {
+    int32_t proot;
+    int32_t need_eval;
+    int32_t span0;
+    int32_t nspans;
+    int32_t soff0;
+    uint32_t entry;
+    uint64_t total;
+} FvGpuQueryDesc;
+#endif /* PAVES_GPU_ENGINE_H */
+
+/* BF16 MFMA distance scan + exact FP32 re-rank of an over-fetched,
+ * error-bounded candidate set; ids and distances equal the FP32 path. */
+#define FV_GPU_PREC_BF16 2
+
+#ifdef __cplusplus
+#define PV_STATIC_ASSERT static_assert
+#else
+#define PV_STATIC_ASSERT _Static_assert
+#endif
+PV_STATIC_ASSERT(sizeof(FvGpuPredNode) == 40, "FvGpuPredNode layout");
+PV_STATIC_ASSERT(sizeof(FvGpuSpan) == 12, "FvGpuSpan layout");
+PV_STATIC_ASSERT(sizeof(FvGpuQueryDesc) == 32, "FvGpuQueryDesc layout");
+PV_STATIC_ASSERT(sizeof(FvHit) == 16, "FvHit layout");
+
+/* ================= reference entry points ================= */
+
+/* gpu_engine.h:63 — 1 if a usable HIP device exists (initialises HIP). */
+int fv_gpu_available(void);
+
+/* gpu_engine.h:69-73 — copy vectors (count x dim fp32, row-major), scalar
+ * columns (ncols x count f64, column-major, NULL = NaN), posting rows
+ * (ncols x count u32) and level-0 adjacency (count x l0_stride u32,
+ * [cnt, neigh...]; may be NULL) to one GPU. device -1 picks the device
+ * with the most free memory. NULL + errbuf on failure. */
+FvGpuIndex *fv_gpu_index_upload(const float *vectors, const double *cols,
+                                const uint32_t *post_rows,
+                                const uint32_t *l0, uint32_t l0_stride,
+                                uint64_t count, int dim, int ncols, int device,
+                                int precision, char *errbuf, size_t errlen);
+/* gpu_engine.h:74-76 */
+void fv_gpu_index_free(FvGpuIndex *g);
+int fv_gpu_index_device(const FvGpuIndex *g);
+int fv_gpu_index_precision(const FvGpuIndex *g);
+
+/* gpu_engine.h:86-94 — batched exact filtered top-topb. Hits of query q go
+ * to out[q*topb ..] ascending by (dist, node); out_cnt[q] entries;
+ * truncated[q] = 1 when out_cnt[q] == topb (more passing rows may exist,
+ * gpu_engine.cu:1059-1064). dedup, spans and spanoff are accepted for ABI
+ * compatibility and used by the span-gather route. 0 / -1 + errbuf. */
+int fv_gpu_brute_batch(FvGpuIndex *g, int nq, const float *queries,
+                       const FvGpuQueryDesc *qd, const uint8_t *dedup,
+                       const FvGpuPredNode *nodes, int nnodes,
+                       const int32_t *children, int nchildren,
+                       const double *invals, int ninvals,
+                       const FvGpuSpan *spans, int nspans_total,
+                       const uint64_t *spanoff, int nspanoff, uint32_t topb,
+                       FvHit *out, uint32_t *out_cnt, uint8_t *truncated,
+                       char *errbuf, size_t errlen);
+
+/* gpu_engine.h:104-109 — batched filtered level-0 beam search from
+ * qd[q].entry. Traversal is unrestricted; only predicate-passing nodes enter
+ * the result list (capacity ef <= FV_GPU_MAX_EF). Results ascending by
+ * (dist, node). exhausted[q] = 1 when a wider ef cannot add results
+ * (frontier drained or max_visited reached). 0 / -1 + errbuf. */
+int fv_gpu_graph_batch(FvGpuIndex *g, int nq, const float *queries,
+                       const FvGpuQueryDesc *qd, const FvGpuPredNode *nodes,
+                       int nnodes, const int32_t *children, int nchildren,
+                       const double *invals, int ninvals, uint32_t ef,
+                       uint64_t max_visited, FvHit *out, uint32_t *out_cnt,
+                       uint8_t *exhausted, char *errbuf, size_t errlen);
+
+/* engine_api.h:194 — query-tile width of the fast brute path for dim
+ * (0 = no fast path). */
+int fv_gpu_tile_variant_qt(int dim);
+/* engine_api.h:195 — free/total bytes of the chosen device; returns the
+ * device id or -1. */
+int fv_gpu_mem_probe(uint64_t *free_bytes, uint64_t *total_bytes);
+
+/* ================= extensions (no reference counterpart) ================= */
+
+/* Per-call options for the brute path. Zero-initialised = defaults. */
+typedef struct FvGpuBruteOpts {
+    int32_t precision;   /* -1/0 = index default, else FV_GPU_PREC_* */
+    int32_t route;       /* 0 auto, 1 force full scan, 2 force span gather */
+    uint32_t overfetch;  /* re-rank candidates per query (0 = auto) */
+    uint32_t sample_stride; /* threshold pre-pass stride (0 = auto) */
+} FvGpuBruteOpts;
+
+/* Per-call statistics of the last batch run on this index. Kernel times are
+ * measured with HIP events on the index's stream when timing is enabled. */
+typedef struct FvGpuStats {
+    double t_bitmap_us, t_prepass_us, t_scan_us, t_merge_us, t_rerank_us;
+    double t_gather_us, t_graph_us, t_total_us;
+    uint64_t scan_rows, scan_bytes;   /* algorithmic bytes of the scan */
+    uint64_t candidates;              /* keys handed to the merge */
+    int32_t route;                    /* 1 scan, 2 gather, 3 graph */
+    int32_t precision;
+    uint32_t certified_fail;          /* queries re-run for certification */
+    uint32_t pad;
+} FvGpuStats;
+
+/* Enable (1) / disable (0) per-kernel event timing on an index. */
+void fv_gpu_set_timing(FvGpuIndex *g, int on);
+void fv_gpu_last_stats(const FvGpuIndex *g, FvGpuStats *st);
+
+/* fv_gpu_brute_batch with options (same arguments otherwise). */
+int fv_gpu_brute_batch_ex(FvGpuIndex *g, int nq, const float *queries,
+                          const FvGpuQueryDesc *qd, const uint8_t *dedup,
+                          const FvGpuPredNode *nodes, int nnodes,
+                          const int32_t *children, int nchildren,
+                          const double *invals, int ninvals,
+                          const FvGpuSpan *spans, int nspans_total,
+                          const uint64_t *spanoff, int nspanoff, uint32_t topb,
+                          const FvGpuBruteOpts *opts, FvHit *out,
+                          uint32_t *out_cnt, uint8_t *truncated, char *errbuf,
+                          size_t errlen);
+
+/* fv_gpu_graph_batch with search knobs: wmax = frontier heads expanded per
+ * iteration (1 = the CPU cursor's one-at-a-time order), fcap = frontier
+ * capacity (64..1024), dbg = optional host array [nq][4] receiving
+ * iterations, visited, results and hash-overflow counts per query. */
+int fv_gpu_graph_batch_ex(FvGpuIndex *g, int nq, const float *queries,
+                          const FvGpuQueryDesc *qd, const FvGpuPredNode *nodes,
+                          int nnodes, const int32_t *children, int nchildren,
+                          const double *invals, int ninvals, uint32_t ef,
+                          uint64_t max_visited, int wmax, int fcap, FvHit *out,
+                          uint32_t *out_cnt, uint8_t *exhausted, uint32_t *dbg,
+                          char *errbuf, size_t errlen);
+
+/*
+ * Device-resident batches: stage a batch's inputs in HBM once, then run it
+ * any number of times with no host<->device copy inside the run. Outputs
+ * stay on the device until fv_gpu_batch_fetch. A batch belongs to one
+ * index. Used by throughput measurement and by callers that keep queries
+ * on the GPU.
+ */
+typedef struct FvGpuBatch FvGpuBatch;
+FvGpuBatch *fv_gpu_batch_stage(FvGpuIndex *g, int nq, const float *queries,
+                               const FvGpuQueryDesc *qd, const uint8_t *dedup,
+                               const FvGpuPredNode *nodes, int nnodes,
+                               const int32_t *children, int nchildren,
+                               const double *invals, int ninvals,
+                               const FvGpuSpan *spans, int nspans_total,
+                               const uint64_t *spanoff, int nspanoff,
+                               uint32_t topb, char *errbuf, size_t errlen);
+/* Enqueue the brute pipeline for a staged batch on the index's stream
+ * (asynchronous; 0 / -1 + errbuf). */
+int fv_gpu_batch_run_brute(FvGpuIndex *g, FvGpuBatch *b,
+                           const FvGpuBruteOpts *opts, char *errbuf,
+                           size_t errlen);
+/* Enqueue the graph pipeline for a staged batch (asynchronous). */
+int fv_gpu_batch_run_graph(FvGpuIndex *g, FvGpuBatch *b, uint32_t ef,
+                           uint64_t max_visited, char *errbuf, size_t errlen);
+/* Wait for the index's stream. */
+int fv_gpu_sync(FvGpuIndex *g, char *errbuf, size_t errlen);
+/* Copy a batch's results to the host (synchronous). */
+int fv_gpu_batch_fetch(FvGpuIndex *g, FvGpuBatch *b, FvHit *out,
+                       uint32_t *out_cnt, uint8_t *flags, char *errbuf,
+                       size_t errlen);
+void fv_gpu_batch_free(FvGpuBatch *b);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PAVES_AMD_GPU_H */

@@ -1,0 +1,438 @@
+/*
+ * paves_oracle.c — CPU restatement of PAVES's query arithmetic.
+ *
+ * TEST INFRASTRUCTURE ONLY. This file is the checker for the HIP engine in
+ * paves_amd/: only tests/, __graft_entry__.smoke() and bench.py's CPU
+ * baseline leg may load it. The product never links or calls it.
+ *
+ * Each function cites the reference code it restates (paths relative to the
+ * PAVES source tree). Built with -ffp-contract=off so every rounding step is
+ * the one written here (fused multiply-adds are explicit fmaf calls).
+ *
+ * Pinning (see tests/test_oracle.py):
+ *   or_l2sq          — bit-identical to the compiled reference l2sq (via
+ *                      oracle/_ref) on random non-integer data; exact against
+ *                      FP64 on integer data (closed form).
+ *   or_pred_eval     — known-answer tests (NaN/NULL, NE, IN, AND/OR/NOT) and
+ *                      the compiled reference's filtered cursors.
+ *   or_brute_topb    — compiled reference BruteCursor (same ids, distances)
+ *                      and an FP64 brute force on tiny inputs.
+ *   or_hnsw_search   — compiled reference HnswCursor on the same graph file.
+ *   or_graph_entry   — compiled reference fv_index_graph_entry.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "paves_oracle.h"
+
+/* ---------------------------------------------------------------------
+ * Distance. engine.cpp:43-63 (AVX-512F branch): two 16-lane accumulators
+ * over 32-dim blocks (d = a - b, acc = fma(d, d, acc)), one more 16-lane
+ * block into acc0, then _mm512_reduce_add_ps(acc0 + acc1) — GCC's tree:
+ * 8 adds (hi half + lo half), 4 adds, 2 adds, 1 add — then a scalar tail
+ * sum = fma(d, d, sum) (GCC contracts the reference loop at -O3).
+ * ------------------------------------------------------------------- */
+float or_l2sq(const float *a, const float *b, int dim) {
+    float acc0[16], acc1[16];
+    int i = 0, j;
+    for (j = 0; j < 16; j++) acc0[j] = acc1[j] = 0.f;
+    for (; i + 32 <= dim; i += 32) {
+        for (j = 0; j < 16; j++) {
+            float d0 = a[i + j] - b[i + j];
+            float d1 = a[i + 16 + j] - b[i + 16 + j];
+            acc0[j] = fmaf(d0, d0, acc0[j]);
+            acc1[j] = fmaf(d1, d1, acc1[j]);
+        }
+    }
+    for (; i + 16 <= dim; i += 16) {
+        for (j = 0; j < 16; j++) {
+            float d = a[i + j] - b[i + j];
+            acc0[j] = fmaf(d, d, acc0[j]);
+        }
+    }
+    float t[16], t8[8], t4[4];
+    for (j = 0; j < 16; j++) t[j] = acc0[j] + acc1[j];
+    for (j = 0; j < 8; j++) t8[j] = t[8 + j] + t[j];
+    for (j = 0; j < 4; j++) t4[j] = t8[4 + j] + t8[j];
+    float s0 = t4[0] + t4[2];
+    float s1 = t4[1] + t4[3];
+    float sum = s0 + s1;
+    /* The reference's scalar tail `sum += d * d` as its compiler emits it
+     * (GCC 11, -O3, AVX-512 target): when >= 8 elements remain, one 8-wide
+     * in-order (fold-left) block — products rounded separately, added in
+     * order — then a fused multiply-add per remaining element (the
+     * scalar epilogue is contracted). Pinned against oracle/_ref for tail
+     * lengths 1..15. */
+    if (dim - i >= 8) {
+        for (j = 0; j < 8; j++, i++) {
+            float d = a[i] - b[i];
+            float p = d * d;
+            sum = sum + p;
+        }
+    }
+    for (; i < dim; i++) {
+        float d = a[i] - b[i];
+        sum = fmaf(d, d, sum);
+    }
+    return sum;
+}
+
+/* FP64 recomputation (the stated reference for distance tolerances). */
+double or_l2sq_f64(const float *a, const float *b, int dim) {
+    double s = 0.0;
+    for (int i = 0; i < dim; i++) {
+        double d = (double)a[i] - (double)b[i];
+        s += d * d;
+    }
+    return s;
+}
+
+/* Round-to-nearest-even float -> bfloat16 bits (NaN kept quiet). */
+uint16_t or_bf16_bits(float x) {
+    uint32_t u;
+    memcpy(&u, &x, 4);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40);
+    uint32_t lsb = (u >> 16) & 1u;
+    u += 0x7fffu + lsb;
+    return (uint16_t)(u >> 16);
+}
+
+/* ---------------------------------------------------------------------
+ * Predicate. engine.cpp:397-430 (FvPred::eval_node) over the serialized
+ * arrays of engine_api.h:95-117 (as interpreted by gpu_engine.cu:238-279).
+ * NULL is stored as NaN: every comparison with NaN is false; NE is
+ * v == v && v != c; IN is true when v equals any listed value.
+ * ------------------------------------------------------------------- */
+int or_pred_eval(const OrPredNode *nodes, const int32_t *children,
+                 const double *invals, const double *cols, uint64_t count,
+                 int32_t ni, uint32_t row) {
+    const OrPredNode *n = &nodes[ni];
+    switch (n->kind) {
+        case OR_NODE_CMP: {
+            double v = cols[(uint64_t)n->col * count + row];
+            switch (n->op) {
+                case OR_CMP_LT: return v < n->value;
+                case OR_CMP_LE: return v <= n->value;
+                case OR_CMP_EQ: return v == n->value;
+                case OR_CMP_GE: return v >= n->value;
+                case OR_CMP_GT: return v > n->value;
+                case OR_CMP_NE: return v == v && v != n->value;
+            }
+            return 0;
+        }
+        case OR_NODE_IN: {
+            double v = cols[(uint64_t)n->col * count + row];
+            for (int i = 0; i < n->nvalues; i++)
+                if (v == invals[n->val0 + i]) return 1;
+            return 0;
+        }
+        case OR_NODE_AND:
+            for (int i = 0; i < n->nchildren; i++)
+                if (!or_pred_eval(nodes, children, invals, cols, count,
+                                  children[n->child0 + i], row))
+                    return 0;
+            return 1;
+        case OR_NODE_OR:
+            for (int i = 0; i < n->nchildren; i++)
+                if (or_pred_eval(nodes, children, invals, cols, count,
+                                 children[n->child0 + i], row))
+                    return 1;
+            return 0;
+        case OR_NODE_NOT:
+            return !or_pred_eval(nodes, children, invals, cols, count,
+                                 children[n->child0], row);
+    }
+    return 0;
+}
+
+/* ---------------------------------------------------------------------
+ * Exact filtered top-B. Order is (dist, row) lexicographic — the order of
+ * BruteCursor's final sort (engine.cpp:1114-1116) and of the GPU packed
+ * keys (gpu_engine.cu:435-438). A max-heap on (dist, row) over rows in
+ * increasing order keeps exactly the B smallest pairs. out_cnt = min(B,
+ * passing); truncated = (out_cnt == B), as the reference's tiled GPU path
+ * reports it (gpu_engine.cu:1059-1064).
+ * ------------------------------------------------------------------- */
+static inline int pair_less(float da, uint64_t na, float db, uint64_t nb) {
+    return da < db || (da == db && na < nb);
+}
+
+static void heap_sift_up(OrHit *h, uint32_t i) {
+    while (i) {
+        uint32_t p = (i - 1) / 2;
+        if (!pair_less(h[p].dist, h[p].node, h[i].dist, h[i].node)) break;
+        OrHit t = h[p]; h[p] = h[i]; h[i] = t;
+        i = p;
+    }
+}
+
+static void heap_sift_down(OrHit *h, uint32_t n) {
+    uint32_t i = 0;
+    for (;;) {
+        uint32_t l = 2 * i + 1, r = l + 1, m = i;
+        if (l < n && pair_less(h[m].dist, h[m].node, h[l].dist, h[l].node)) m = l;
+        if (r < n && pair_less(h[m].dist, h[m].node, h[r].dist, h[r].node)) m = r;
+        if (m == i) break;
+        OrHit t = h[i]; h[i] = h[m]; h[m] = t;
+        i = m;
+    }
+}
+
+static int hit_cmp(const void *pa, const void *pb) {
+    const OrHit *a = (const OrHit *)pa, *b = (const OrHit *)pb;
+    if (pair_less(a->dist, a->node, b->dist, b->node)) return -1;
+    if (pair_less(b->dist, b->node, a->dist, a->node)) return 1;
+    return 0;
+}
+
+void or_sort_hits(OrHit *h, uint32_t n) { qsort(h, n, sizeof(OrHit), hit_cmp); }
+
+uint64_t or_brute_topb(const float *vectors, uint64_t count, int dim,
+                       const double *cols, const float *query,
+                       const OrPredNode *nodes, const int32_t *children,
+                       const double *invals, int32_t proot, uint32_t topb,
+                       OrHit *out, uint32_t *out_cnt, uint8_t *truncated) {
+    uint32_t n = 0;
+    uint64_t passing = 0;
+    for (uint64_t r = 0; r < count; r++) {
+        if (proot >= 0 &&
+            !or_pred_eval(nodes, children, invals, cols, count, proot, (uint32_t)r))
+            continue;
+        passing++;
+        float d = or_l2sq(query, vectors + r * (uint64_t)dim, dim);
+        if (n < topb) {
+            out[n].node = r;
+            out[n].dist = d;
+            heap_sift_up(out, n);
+            n++;
+        } else if (pair_less(d, r, out[0].dist, out[0].node)) {
+            out[0].node = r;
+            out[0].dist = d;
+            heap_sift_down(out, n);
+        }
+    }
+    or_sort_hits(out, n);
+    *out_cnt = n;
+    *truncated = (n == topb) ? 1 : 0;
+    return passing;
+}
+
+/* Batch form over the ABI's per-query descriptors (gpu_engine.h:52-60),
+ * parallel over queries. Returns 0. */
+int or_brute_batch(const float *vectors, uint64_t count, int dim,
+                   const double *cols, int nq, const float *queries,
+                   const int32_t *proots, const OrPredNode *nodes,
+                   const int32_t *children, const double *invals,
+                   uint32_t topb, OrHit *out, uint32_t *out_cnt,
+                   uint8_t *truncated, uint64_t *passing) {
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int q = 0; q < nq; q++) {
+        uint64_t p = or_brute_topb(vectors, count, dim, cols,
+                                   queries + (uint64_t)q * dim, nodes, children,
+                                   invals, proots[q], topb,
+                                   out + (uint64_t)q * topb, &out_cnt[q],
+                                   &truncated[q]);
+        if (passing) passing[q] = p;
+    }
+    return 0;
+}
+
+/* Exact top-B over an explicit row list (the span-enumeration semantics of
+ * BruteCursor::compute, engine.cpp:1082-1130): rows may repeat when spans
+ * overlap; duplicates are removed after the sort when dedup is set. */
+uint32_t or_brute_rows(const float *vectors, int dim, const double *cols,
+                       uint64_t count, const float *query, const uint32_t *rows,
+                       uint64_t nrows, const OrPredNode *nodes,
+                       const int32_t *children, const double *invals,
+                       int32_t proot, int need_eval, int dedup, uint32_t topb,
+                       OrHit *scratch, OrHit *out) {
+    uint64_t n = 0;
+    for (uint64_t i = 0; i < nrows; i++) {
+        uint32_t r = rows[i];
+        if (need_eval && proot >= 0 &&
+            !or_pred_eval(nodes, children, invals, cols, count, proot, r))
+            continue;
+        scratch[n].node = r;
+        scratch[n].dist = or_l2sq(query, vectors + (uint64_t)r * dim, dim);
+        n++;
+    }
+    qsort(scratch, n, sizeof(OrHit), hit_cmp);
+    uint64_t m = 0;
+    for (uint64_t i = 0; i < n; i++) {
+        if (dedup && m > 0 && scratch[m - 1].node == scratch[i].node) continue;
+        scratch[m++] = scratch[i];
+    }
+    uint32_t k = (uint32_t)(m < topb ? m : topb);
+    memcpy(out, scratch, k * sizeof(OrHit));
+    return k;
+}
+
+/* ---------------------------------------------------------------------
+ * Filtered HNSW level-0 search: HnswCursor::start + run_phase
+ * (engine.cpp:1631-1712). cand is a min-heap on (dist, node), res a
+ * max-heap on (dist, node) holding predicate-passing nodes only, capacity
+ * ef. Stop when the best candidate is worse than the worst result with res
+ * full, or when visited >= max_visited (exhausted), or when cand drains
+ * (exhausted). Neighbours are admitted to cand when res is not full or
+ * they beat the worst result; only passing ones enter res. The graph is
+ * the reference's level-0 layout: l0[node * stride] = count, followed by
+ * the neighbour ids.
+ * ------------------------------------------------------------------- */
+typedef struct { float d; uint32_t n; } Pair;
+
+static inline int p_less(Pair a, Pair b) {
+    return a.d < b.d || (a.d == b.d && a.n < b.n);
+}
+
+/* heap with comparator sign: minheap (sign=+1 -> smallest on top) */
+static void ph_push(Pair *h, uint64_t *n, Pair v, int maxheap) {
+    uint64_t i = (*n)++;
+    h[i] = v;
+    while (i) {
+        uint64_t p = (i - 1) / 2;
+        int better = maxheap ? p_less(h[p], h[i]) : p_less(h[i], h[p]);
+        if (!better) break;
+        Pair t = h[p]; h[p] = h[i]; h[i] = t;
+        i = p;
+    }
+}
+
+static Pair ph_pop(Pair *h, uint64_t *n, int maxheap) {
+    Pair top = h[0];
+    uint64_t m = --(*n);
+    h[0] = h[m];
+    uint64_t i = 0;
+    for (;;) {
+        uint64_t l = 2 * i + 1, r = l + 1, b = i;
+        if (l < m && (maxheap ? p_less(h[b], h[l]) : p_less(h[l], h[b]))) b = l;
+        if (r < m && (maxheap ? p_less(h[b], h[r]) : p_less(h[r], h[b]))) b = r;
+        if (b == i) break;
+        Pair t = h[i]; h[i] = h[b]; h[b] = t;
+        i = b;
+    }
+    return top;
+}
+
+uint32_t or_hnsw_search(const float *vectors, uint64_t count, int dim,
+                        const uint32_t *l0, uint32_t l0_stride,
+                        const double *cols, const OrPredNode *nodes,
+                        const int32_t *children, const double *invals,
+                        int32_t proot, const float *query, uint32_t entry,
+                        uint32_t ef, uint64_t max_visited, OrHit *out,
+                        uint8_t *exhausted, uint64_t *visited_out) {
+    uint8_t *seen = (uint8_t *)calloc(count, 1);
+    Pair *cand = (Pair *)malloc(sizeof(Pair) * (count + 1));
+    Pair *res = (Pair *)malloc(sizeof(Pair) * (ef + 2));
+    uint64_t nc = 0, nr = 0, visited = 1;
+    int exh = 0;
+
+    float epd = or_l2sq(query, vectors + (uint64_t)entry * dim, dim);
+    seen[entry] = 1;
+    ph_push(cand, &nc, (Pair){epd, entry}, 0);
+    if (proot < 0 || or_pred_eval(nodes, children, invals, cols, count, proot, entry))
+        ph_push(res, &nr, (Pair){epd, entry}, 1);
+
+    while (nc > 0) {
+        Pair c = cand[0];
+        if (nr >= ef && c.d > res[0].d) break;
+        if (visited >= max_visited) {
+            exh = 1;
+            break;
+        }
+        ph_pop(cand, &nc, 0);
+        const uint32_t *lst = l0 + (uint64_t)c.n * l0_stride;
+        uint32_t cnt = lst[0];
+        for (uint32_t i = 0; i < cnt; i++) {
+            uint32_t n2 = lst[1 + i];
+            if (seen[n2]) continue;
+            seen[n2] = 1;
+            visited++;
+            float d = or_l2sq(query, vectors + (uint64_t)n2 * dim, dim);
+            if (nr < ef || d < res[0].d) {
+                ph_push(cand, &nc, (Pair){d, n2}, 0);
+                if (proot < 0 ||
+                    or_pred_eval(nodes, children, invals, cols, count, proot, n2)) {
+                    ph_push(res, &nr, (Pair){d, n2}, 1);
+                    if (nr > ef) ph_pop(res, &nr, 1);
+                }
+            }
+        }
+    }
+    if (nc == 0) exh = 1;
+    uint32_t k = (uint32_t)nr;
+    for (uint32_t i = 0; i < k; i++) {
+        out[i].node = res[i].n;
+        out[i].dist = res[i].d;
+    }
+    or_sort_hits(out, k);
+    *exhausted = (uint8_t)exh;
+    if (visited_out) *visited_out = visited;
+    free(seen);
+    free(cand);
+    free(res);
+    return k;
+}
+
+/* Parallel batch over queries (one search per query). */
+int or_hnsw_batch(const float *vectors, uint64_t count, int dim,
+                  const uint32_t *l0, uint32_t l0_stride, const double *cols,
+                  const OrPredNode *nodes, const int32_t *children,
+                  const double *invals, int nq, const int32_t *proots,
+                  const float *queries, const uint32_t *entries, uint32_t ef,
+                  uint64_t max_visited, OrHit *out, uint32_t *out_cnt,
+                  uint8_t *exhausted) {
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int q = 0; q < nq; q++)
+        out_cnt[q] = or_hnsw_search(vectors, count, dim, l0, l0_stride, cols,
+                                    nodes, children, invals, proots[q],
+                                    queries + (uint64_t)q * dim, entries[q], ef,
+                                    max_visited, out + (uint64_t)q * ef,
+                                    &exhausted[q], NULL);
+    return 0;
+}
+
+/* ---------------------------------------------------------------------
+ * Upper-level greedy descent: fv_index_graph_entry (engine.cpp:1539-1559).
+ * upper[upper_offs[node] + (level-1)*(M+1)] = count, then neighbour ids.
+ * ------------------------------------------------------------------- */
+uint32_t or_graph_entry(const float *vectors, int dim,
+                        const uint64_t *upper_offs, const uint32_t *upper,
+                        uint32_t M, uint64_t entry_node, int max_level,
+                        const float *query) {
+    uint64_t ep = entry_node;
+    for (int lc = max_level; lc >= 1; lc--) {
+        float d = or_l2sq(query, vectors + ep * (uint64_t)dim, dim);
+        int changed = 1;
+        while (changed) {
+            changed = 0;
+            const uint32_t *lst = upper + upper_offs[ep] + (uint64_t)(lc - 1) * (M + 1);
+            uint32_t cnt = lst[0];
+            for (uint32_t i = 0; i < cnt; i++) {
+                float nd = or_l2sq(query, vectors + (uint64_t)lst[1 + i] * dim, dim);
+                if (nd < d) {
+                    d = nd;
+                    ep = lst[1 + i];
+                    changed = 1;
+                }
+            }
+        }
+    }
+    return (uint32_t)ep;
+}
+
+/* recall@k of got against truth (|ids ∩ truth| / |truth|), the definition
+ * of bench/benchmark.py:53-64 for one query. */
+double or_recall(const uint64_t *got, uint32_t ngot, const uint64_t *truth,
+                 uint32_t ntruth) {
+    if (ntruth == 0) return 1.0;
+    uint32_t hit = 0;
+    for (uint32_t i = 0; i < ntruth; i++)
+        for (uint32_t j = 0; j < ngot; j++)
+            if (got[j] == truth[i]) {
+                hit++;
+                break;
+            }
+    return (double)hit / (double)ntruth;
+}

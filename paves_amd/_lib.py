@@ -1,0 +1,126 @@
+"""ctypes bindings of libpaves_amd.so (include/paves_gpu.h).
+
+The shared library is built in-tree by ``paves_amd/Makefile``; importing
+this module never falls back to anything else — a missing library raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpaves_amd.so")
+
+# byte layouts of engine_api.h / gpu_engine.h
+PRED_DT = np.dtype([("kind", "<i4"), ("col", "<i4"), ("op", "<i4"), ("nchildren", "<i4"),
+                    ("child0", "<i4"), ("nvalues", "<i4"), ("val0", "<i4"), ("pad", "<i4"),
+                    ("value", "<f8")])
+SPAN_DT = np.dtype([("col", "<i4"), ("lo", "<u4"), ("hi", "<u4")])
+QD_DT = np.dtype([("proot", "<i4"), ("need_eval", "<i4"), ("span0", "<i4"), ("nspans", "<i4"),
+                  ("soff0", "<i4"), ("entry", "<u4"), ("total", "<u8")])
+HIT_DT = np.dtype({"names": ["node", "dist"], "formats": ["<u8", "<f4"],
+                   "offsets": [0, 8], "itemsize": 16})
+assert PRED_DT.itemsize == 40 and SPAN_DT.itemsize == 12
+assert QD_DT.itemsize == 32 and HIT_DT.itemsize == 16
+
+FV_GPU_PREC_FP32 = 0
+FV_GPU_PREC_SQ8 = 1
+FV_GPU_PREC_BF16 = 2
+FV_GPU_MAX_TOPB = 4096
+FV_GPU_MAX_SPANS = 4096
+FV_GPU_MAX_EF = 512
+FV_GPU_MAX_BATCH = 1024
+
+# FvCmpOp / FvNodeKind (engine_api.h:69-70)
+CMP_LT, CMP_LE, CMP_EQ, CMP_GE, CMP_GT, CMP_NE = range(6)
+NODE_CMP, NODE_IN, NODE_AND, NODE_OR, NODE_NOT = range(5)
+
+REF_SYMBOLS = ["fv_gpu_available", "fv_gpu_index_upload", "fv_gpu_index_free",
+               "fv_gpu_index_device", "fv_gpu_index_precision", "fv_gpu_brute_batch",
+               "fv_gpu_graph_batch", "fv_gpu_tile_variant_qt", "fv_gpu_mem_probe"]
+EXT_SYMBOLS = ["fv_gpu_set_timing", "fv_gpu_last_stats", "fv_gpu_brute_batch_ex",
+               "fv_gpu_graph_batch_ex", "fv_gpu_batch_stage", "fv_gpu_batch_run_brute",
+               "fv_gpu_batch_run_graph", "fv_gpu_sync", "fv_gpu_batch_fetch",
+               "fv_gpu_batch_free"]
+
+
+class BruteOpts(C.Structure):
+    _fields_ = [("precision", C.c_int32), ("route", C.c_int32),
+                ("overfetch", C.c_uint32), ("sample_stride", C.c_uint32)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("t_bitmap_us", C.c_double), ("t_prepass_us", C.c_double),
+                ("t_scan_us", C.c_double), ("t_merge_us", C.c_double),
+                ("t_rerank_us", C.c_double), ("t_gather_us", C.c_double),
+                ("t_graph_us", C.c_double), ("t_total_us", C.c_double),
+                ("scan_rows", C.c_uint64), ("scan_bytes", C.c_uint64),
+                ("candidates", C.c_uint64), ("route", C.c_int32),
+                ("precision", C.c_int32), ("certified_fail", C.c_uint32),
+                ("pad", C.c_uint32)]
+
+
+_P = C.c_void_p
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load the engine library (raises when it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"paves_amd: {LIB_PATH} not built (run __graft_entry__.build())")
+    lib = C.CDLL(LIB_PATH)
+    sz = C.c_size_t
+    lib.fv_gpu_available.restype = C.c_int
+    lib.fv_gpu_index_upload.restype = _P
+    lib.fv_gpu_index_upload.argtypes = [_P, _P, _P, _P, C.c_uint32, C.c_uint64, C.c_int,
+                                        C.c_int, C.c_int, C.c_int, C.c_char_p, sz]
+    lib.fv_gpu_index_free.argtypes = [_P]
+    lib.fv_gpu_index_device.argtypes = [_P]
+    lib.fv_gpu_index_precision.argtypes = [_P]
+    lib.fv_gpu_brute_batch.restype = C.c_int
+    lib.fv_gpu_brute_batch.argtypes = [_P, C.c_int, _P, _P, _P, _P, C.c_int, _P, C.c_int,
+                                       _P, C.c_int, _P, C.c_int, _P, C.c_int, C.c_uint32,
+                                       _P, _P, _P, C.c_char_p, sz]
+    lib.fv_gpu_brute_batch_ex.restype = C.c_int
+    lib.fv_gpu_brute_batch_ex.argtypes = [_P, C.c_int, _P, _P, _P, _P, C.c_int, _P, C.c_int,
+                                          _P, C.c_int, _P, C.c_int, _P, C.c_int, C.c_uint32,
+                                          C.POINTER(BruteOpts), _P, _P, _P, C.c_char_p, sz]
+    lib.fv_gpu_graph_batch.restype = C.c_int
+    lib.fv_gpu_graph_batch.argtypes = [_P, C.c_int, _P, _P, _P, C.c_int, _P, C.c_int, _P,
+                                       C.c_int, C.c_uint32, C.c_uint64, _P, _P, _P,
+                                       C.c_char_p, sz]
+    lib.fv_gpu_graph_batch_ex.restype = C.c_int
+    lib.fv_gpu_graph_batch_ex.argtypes = [_P, C.c_int, _P, _P, _P, C.c_int, _P, C.c_int, _P,
+                                          C.c_int, C.c_uint32, C.c_uint64, C.c_int, C.c_int,
+                                          _P, _P, _P, _P, C.c_char_p, sz]
+    lib.fv_gpu_tile_variant_qt.argtypes = [C.c_int]
+    lib.fv_gpu_mem_probe.argtypes = [_P, _P]
+    lib.fv_gpu_set_timing.argtypes = [_P, C.c_int]
+    lib.fv_gpu_last_stats.argtypes = [_P, C.POINTER(Stats)]
+    lib.fv_gpu_batch_stage.restype = _P
+    lib.fv_gpu_batch_stage.argtypes = [_P, C.c_int, _P, _P, _P, _P, C.c_int, _P, C.c_int,
+                                       _P, C.c_int, _P, C.c_int, _P, C.c_int, C.c_uint32,
+                                       C.c_char_p, sz]
+    lib.fv_gpu_batch_run_brute.argtypes = [_P, _P, C.POINTER(BruteOpts), C.c_char_p, sz]
+    lib.fv_gpu_batch_run_graph.argtypes = [_P, _P, C.c_uint32, C.c_uint64, C.c_char_p, sz]
+    lib.fv_gpu_sync.argtypes = [_P, C.c_char_p, sz]
+    lib.fv_gpu_batch_fetch.argtypes = [_P, _P, _P, _P, _P, C.c_char_p, sz]
+    lib.fv_gpu_batch_free.argtypes = [_P]
+    _lib = lib
+    return lib
+
+
+def ptr(a) -> int | None:
+    """Data pointer of a numpy array (None for None / empty)."""
+    if a is None:
+        return None
+    return a.ctypes.data if a.size else None
+
+
+def errbuf():
+    return C.create_string_buffer(512)

@@ -1,0 +1,1226 @@
+/*
+ * pv_brute.hip — batched exact filtered top-B on MI355X.
+ *
+ * Replaces the reference's brute path (gpu_engine.cu:348-418 span kernel,
+ * 474-610 tiled kernel, 1003-1065 merge kernel, 2206-2336 router) with a
+ * different pipeline built for CDNA4 (one stream, no host round trip):
+ *
+ *   1. k_prep_queries   fp32 queries -> bf16 copy, ||q||^2, error bound eps
+ *   2. k_bitmap         predicate programs -> row bitmaps (pv_pred.hip)
+ *   3. k_scan<PREPASS>  distances of every stride-th row; per 16-row group
+ *                       the minimum passing distance
+ *   4. k_tau            tau[q] = B-th smallest group minimum (+2 eps): at
+ *                       least B passing rows lie at or below it, so every
+ *                       row of the exact top-B does too
+ *   5. k_scan<MAIN>     full scan; rows with dist <= tau enter a per-query
+ *                       candidate pool (LDS first, spill to HBM)
+ *   6. k_select         radix-select the B-th smallest pool key, collect the
+ *                       keys within 2 eps of it, re-rank them with exact FP32
+ *                       in the CPU cursor's summation order, bitonic-sort by
+ *                       (dist, row), emit FvHit + count + truncated
+ *
+ * Distances in the scans: BF16 MFMA (v_mfma_f32_32x32x16_bf16) computing
+ * x.q with the ||x||^2 + ||q||^2 - 2 x.q epilogue, or plain FP32 VALU
+ * sum (x - q)^2. Both are approximations of the reference l2sq value with a
+ * per-query bound eps (0 when every value is an integer small enough that
+ * all arithmetic is exact); the re-rank restores the exact FP32 value, so
+ * ids and distances equal the CPU cursor's (BruteCursor, engine.cpp:1050-
+ * 1130) whatever scan precision is used.
+ *
+ * Span-gather route: for batches whose driver spans (engine.cpp:464-599)
+ * enumerate few rows, k_gather computes exact distances for the enumerated
+ * rows only (the analogue of fv_brute_kernel) and k_select finishes.
+ */
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+
+#include "pv_device.h"
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+namespace pv {
+
+/* per-query auxiliary array layout inside d_qaux (floats) */
+struct QAux {
+    float *qnorm, *eps, *tau;
+    uint32_t *status;
+};
+static inline QAux qaux_of(FvGpuBatch *b) {
+    const int nqp = b->nqt * kQT;
+    float *base = b->d_qaux.as<float>();
+    QAux a;
+    a.qnorm = base;
+    a.eps = base + nqp;
+    a.tau = base + 2 * nqp;
+    a.status = (uint32_t *)(base + 3 * nqp);
+    return a;
+}
+
+/* ---------------- 1. query preparation ---------------- */
+
+/* One wave per (padded) query. mode: 0 = FP32 scan bound, 1 = BF16 bound. */
+__global__ __launch_bounds__(64) void k_prep_queries(
+    const float *__restrict__ q, int nq, int dim, uint16_t *__restrict__ qbf,
+    float *__restrict__ qnorm, float *__restrict__ eps, float xmax_norm,
+    float vmax_abs, int all_int, int mode) {
+    const int qi = blockIdx.x;
+    const int l = threadIdx.x;
+    const float *src = q + (uint64_t)qi * dim;
+    float ss = 0.f, amax = 0.f;
+    int nonint = 0;
+    for (int j = l; j < dim; j += 64) {
+        float v = src[j];
+        qbf[(uint64_t)qi * dim + j] = f2bf(v);
+        ss = fmaf(v, v, ss);
+        amax = fmaxf(amax, fabsf(v));
+        nonint |= (v != rintf(v)) ? 1 : 0;
+    }
+    for (int o = 32; o; o >>= 1) {
+        ss += __shfl_xor(ss, o);
+        amax = fmaxf(amax, __shfl_xor(amax, o));
+        nonint |= __shfl_xor(nonint, o);
+    }
+    if (l == 0) {
+        qnorm[qi] = ss;
+        double e = 0.0;
+        const double D = (double)dim;
+        const double s = (double)amax + (double)vmax_abs;
+        const bool exact = all_int && !nonint && amax <= 256.f &&
+                           vmax_abs <= 256.f && D * s * s < 16777216.0;
+        if (!exact && qi < nq) {
+            const double qn = sqrt((double)ss), xn = (double)xmax_norm;
+            const double g = (D + 8.0) * ldexp(1.0, -24);
+            const double big = (xn + qn) * (xn + qn);
+            if (mode == 1) {
+                const double u = ldexp(1.0, -8);
+                e = (4.0 * u + 2.0 * u * u) * xn * qn + 4.0 * g * big;
+            } else {
+                e = 2.0 * g * big;
+            }
+            e = e * 1.25 + 1e-30;
+        }
+        eps[qi] = (float)e;
+    }
+}
+
+/* ---------------- shared scan epilogue pieces ---------------- */
+
+struct ScanArgs {
+    const uint16_t *vbf;
+    const float *vecs;
+    const float *xnorm;
+    uint64_t count, count_pad;
+    int dim;
+    const uint16_t *qbf;
+    const float *qf;
+    const float *qnorm;
+    const float *tau;
+    const int32_t *pofq;
+    const uint32_t *bitmap;
+    uint64_t words;
+    int nq;
+    uint32_t ntiles;      /* 32-row tiles in the row domain */
+    uint32_t stride;      /* sample stride (PREPASS) */
+    uint64_t nsample;     /* number of sampled rows (PREPASS) */
+    uint64_t *pool;
+    uint32_t *pcnt;
+    uint64_t cap_pool;
+    uint64_t *gmin;
+    uint32_t ngroups;
+};
+
+/* row held in accumulator element i by lane l (32x32 MFMA C layout) */
+__device__ __forceinline__ int acc_row(int i, int l) {
+    return (i >> 2) * 8 + (l >> 5) * 4 + (i & 3);
+}
+
+/*
+ * Consume one 32-row x 64-query tile of distances d[qb][i] held in the
+ * MFMA C layout: lane l owns queries (l & 31) + 32*qb and the 16 rows
+ * acc_row(i, l). PREPASS writes the minimum passing distance of the lane's
+ * 16 rows per query (one group); MAIN appends passing rows with
+ * dist <= tau to the candidate pool.
+ */
+template <bool PREPASS>
+__device__ __forceinline__ void scan_epilogue(const ScanArgs &A, int qbase,
+                                              uint32_t t, float (&d)[2][16],
+                                              const float (&tq)[2],
+                                              const int32_t (&pq)[2],
+                                              uint32_t *lcnt, uint64_t *lbuf) {
+    const int l = threadIdx.x & 63;
+#pragma unroll
+    for (int qb = 0; qb < 2; qb++) {
+        const int qi = (l & 31) + 32 * qb;
+        const int q = qbase + qi;
+        if (q >= A.nq) continue;
+        if (PREPASS) {
+            float m = __uint_as_float(0x7f800000u);
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const uint64_t sidx = (uint64_t)t * 32 + acc_row(i, l);
+                if (sidx >= A.nsample) continue;
+                const uint64_t row = sidx * A.stride;
+                bool pass = true;
+                if (pq[qb] >= 0)
+                    pass = (A.bitmap[(uint64_t)pq[qb] * A.words + (row >> 5)] >>
+                            (row & 31)) & 1u;
+                if (pass) m = fminf(m, d[qb][i]);
+            }
+            const uint32_t grp = t * 2 + (l >> 5);
+            A.gmin[(uint64_t)q * A.ngroups + grp] = make_key(m, grp);
+        } else {
+            uint32_t word = 0xFFFFFFFFu;
+            if (pq[qb] >= 0) word = A.bitmap[(uint64_t)pq[qb] * A.words + t];
+            const float tau = tq[qb];
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const int r = acc_row(i, l);
+                const uint64_t row = (uint64_t)t * 32 + r;
+                const bool ok = ((word >> r) & 1u) && row < A.count &&
+                                d[qb][i] <= tau;
+                if (ok) {
+                    const uint64_t key = make_key(d[qb][i], (uint32_t)row);
+                    const uint32_t pos = atomicAdd(&lcnt[qi], 1u);
+                    if (pos < (uint32_t)kLdsCand) {
+                        lbuf[qi * kLdsCand + pos] = key;
+                    } else {
+                        const uint32_t gp = atomicAdd(&A.pcnt[q], 1u);
+                        if (gp < A.cap_pool) A.pool[(uint64_t)q * A.cap_pool + gp] = key;
+                    }
+                }
+            }
+        }
+    }
+}
+
+/* Flush the WG's LDS candidates to the pool (MAIN). */
+__device__ __forceinline__ void flush_candidates(const ScanArgs &A, int qbase,
+                                                 uint32_t *lcnt, uint64_t *lbuf) {
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int nw = blockDim.x >> 6;
+    for (int qi = wave; qi < kQT; qi += nw) {
+        const int q = qbase + qi;
+        if (q >= A.nq) continue;
+        const uint32_t n = min(lcnt[qi], (uint32_t)kLdsCand);
+        if (n == 0) continue;
+        uint32_t base = 0;
+        if (l == 0) base = atomicAdd(&A.pcnt[q], n);
+        base = __shfl(base, 0);
+        if ((uint32_t)l < n && base + l < A.cap_pool)
+            A.pool[(uint64_t)q * A.cap_pool + base + l] = lbuf[qi * kLdsCand + l];
+    }
+}
+
+/* per-lane query state for the two column blocks */
+__device__ __forceinline__ void load_query_state(const ScanArgs &A, int qbase,
+                                                 float (&tq)[2], float (&qn)[2],
+                                                 int32_t (&pq)[2]) {
+    const int l = threadIdx.x & 63;
+#pragma unroll
+    for (int qb = 0; qb < 2; qb++) {
+        const int q = qbase + (l & 31) + 32 * qb;
+        const bool v = q < A.nq;
+        tq[qb] = v ? A.tau[q] : 0.f;
+        qn[qb] = v ? A.qnorm[q] : 0.f;
+        pq[qb] = v ? A.pofq[q] : -1;
+    }
+}
+
+/* ---------------- 3/5. BF16 MFMA scan ---------------- */
+
+/*
+ * grid.x: row chunks, grid.y: query tiles of 64. 256 threads = 4 waves;
+ * wave w processes the chunk's 32-row tiles w, w+4, ... One tile is
+ * 2 x (dim/16) v_mfma_f32_32x32x16_bf16: A = 32 rows (lane l: row l&31,
+ * dims (l>>5)*dim/2 + 8s .. +8, i.e. 128 contiguous bytes per 64-dim chunk),
+ * B = 32 queries from LDS in the same k order. The next tile's first chunk
+ * is loaded before the current tile's MFMAs (software pipeline).
+ * Requires dim % 128 == 0.
+ */
+template <bool PREPASS>
+__global__ __launch_bounds__(256, 2) void k_scan_bf16(ScanArgs A) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    const int S = A.dim / 16;            /* MFMA k-steps per tile */
+    const int NCH = A.dim / 128;         /* 8-step chunks */
+    u32x4 *qsh = (u32x4 *)smem;          /* [2][S][64] x 16 B */
+    uint32_t *lcnt = (uint32_t *)(smem + (size_t)2 * S * 64 * 16);
+    uint64_t *lbuf = (uint64_t *)(lcnt + kQT);
+
+    const int tid = threadIdx.x, l = tid & 63, wave = tid >> 6;
+    const int qbase = blockIdx.y * kQT;
+    const int half = A.dim / 2;
+
+    /* stage B operands: qsh[(qb*S + s)*64 + lane] = 8 bf16 of query
+     * qbase + (lane&31) + 32 qb, dims (lane>>5)*half + 8 s */
+    for (int e = tid; e < 2 * S * 64; e += blockDim.x) {
+        const int lane = e & 63, s = (e >> 6) % S, qb = (e >> 6) / S;
+        const int q = qbase + (lane & 31) + 32 * qb;
+        const u32x4 *src = (const u32x4 *)(A.qbf + (uint64_t)q * A.dim +
+                                           (lane >> 5) * half + 8 * s);
+        qsh[e] = *src; /* query buffer is padded to the tile */
+    }
+    if (!PREPASS)
+        for (int i = tid; i < kQT; i += blockDim.x) lcnt[i] = 0;
+    __syncthreads();
+
+    float tq[2], qn[2];
+    int32_t pq[2];
+    load_query_state(A, qbase, tq, qn, pq);
+
+    const uint32_t nwg = gridDim.x;
+    const uint32_t t0 = (uint32_t)(((uint64_t)A.ntiles * blockIdx.x) / nwg);
+    const uint32_t t1 = (uint32_t)(((uint64_t)A.ntiles * (blockIdx.x + 1)) / nwg);
+
+    auto row_ptr = [&](uint32_t t) -> const u32x4 * {
+        uint64_t row;
+        if (PREPASS) {
+            uint64_t sidx = (uint64_t)t * 32 + (l & 31);
+            row = sidx < A.nsample ? sidx * A.stride : 0;
+        } else {
+            row = (uint64_t)t * 32 + (l & 31);
+        }
+        return (const u32x4 *)(A.vbf + row * A.dim + (l >> 5) * half);
+    };
+
+    u32x4 a[8], an[8];
+    uint32_t t = t0 + wave;
+    if (t < t1) {
+        const u32x4 *p = row_ptr(t);
+#pragma unroll
+        for (int j = 0; j < 8; j++) a[j] = p[j];
+    }
+    for (; t < t1; t += 4) {
+        f32x16 acc0 = {}, acc1 = {};
+        const u32x4 *p = row_ptr(t);
+        for (int c = 0; c < NCH; c++) {
+            /* prefetch: next chunk of this tile, or first chunk of the next */
+            const bool more = (c + 1 < NCH) || (t + 4 < t1);
+            if (more) {
+                const u32x4 *pn = (c + 1 < NCH) ? p + 8 * (c + 1) : row_ptr(t + 4);
+#pragma unroll
+                for (int j = 0; j < 8; j++) an[j] = pn[j];
+            }
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                const int s = c * 8 + j;
+                const bf16x8 av = __builtin_bit_cast(bf16x8, a[j]);
+                const bf16x8 b0 = __builtin_bit_cast(bf16x8, qsh[(0 * S + s) * 64 + l]);
+                const bf16x8 b1 = __builtin_bit_cast(bf16x8, qsh[(1 * S + s) * 64 + l]);
+                acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, b0, acc0, 0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, b1, acc1, 0, 0, 0);
+            }
+            if (more) {
+#pragma unroll
+                for (int j = 0; j < 8; j++) a[j] = an[j];
+            }
+        }
+        /* epilogue: ||x||^2 + ||q||^2 - 2 x.q, clamped at 0 */
+        float d[2][16];
+        float xn[16];
+        if (PREPASS) {
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const uint64_t sidx = (uint64_t)t * 32 + acc_row(i, l);
+                xn[i] = sidx < A.nsample ? A.xnorm[sidx * A.stride] : 0.f;
+            }
+        } else {
+#pragma unroll
+            for (int g4 = 0; g4 < 4; g4++) {
+                const float4 v = *(const float4 *)(A.xnorm + (uint64_t)t * 32 +
+                                                   g4 * 8 + (l >> 5) * 4);
+                xn[g4 * 4 + 0] = v.x;
+                xn[g4 * 4 + 1] = v.y;
+                xn[g4 * 4 + 2] = v.z;
+                xn[g4 * 4 + 3] = v.w;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            d[0][i] = fmaxf(xn[i] + qn[0] - 2.f * acc0[i], 0.f);
+            d[1][i] = fmaxf(xn[i] + qn[1] - 2.f * acc1[i], 0.f);
+        }
+        scan_epilogue<PREPASS>(A, qbase, t, d, tq, pq, lcnt, lbuf);
+    }
+    if (!PREPASS) flush_candidates(A, qbase, lcnt, lbuf);
+}
+
+/* ---------------- 3/5. plain FP32 VALU scan ---------------- */
+
+/*
+ * The plain FP32 path: sum over dims of (x - q)^2 in FP32 (the arithmetic
+ * of the reference tiled kernel, gpu_engine.cu:534-566), producing the same
+ * 32-row x 64-query C layout as the MFMA path so the epilogue is shared.
+ * Per 64-dim chunk the WG stages the query chunk transposed (qT[64][64])
+ * and each wave its 32 rows transposed (xT[64][32+4]); all waves iterate in
+ * lockstep (barriers), idle waves compute on a masked tile.
+ */
+constexpr int kXP = 36; /* padded row stride of xT */
+template <bool PREPASS>
+__global__ __launch_bounds__(256, 2) void k_scan_f32(ScanArgs A) {
+    __shared__ float qT[64 * 64];
+    __shared__ float xT[4][64 * kXP];
+    __shared__ uint32_t lcnt[kQT];
+    __shared__ uint64_t lbuf[kQT * kLdsCand];
+
+    const int tid = threadIdx.x, l = tid & 63, wave = tid >> 6;
+    const int qbase = blockIdx.y * kQT;
+    if (!PREPASS)
+        for (int i = tid; i < kQT; i += blockDim.x) lcnt[i] = 0;
+
+    float tq[2], qn[2];
+    int32_t pq[2];
+    load_query_state(A, qbase, tq, qn, pq);
+
+    const uint32_t nwg = gridDim.x;
+    const uint32_t t0 = (uint32_t)(((uint64_t)A.ntiles * blockIdx.x) / nwg);
+    const uint32_t t1 = (uint32_t)(((uint64_t)A.ntiles * (blockIdx.x + 1)) / nwg);
+    const uint32_t iters = (t1 - t0 + 3) / 4;
+    float *xw = xT[wave];
+
+    for (uint32_t it = 0; it < iters; it++) {
+        const uint32_t t = t0 + it * 4 + wave;
+        const bool live = t < t1;
+        float acc[2][16];
+#pragma unroll
+        for (int qb = 0; qb < 2; qb++)
+#pragma unroll
+            for (int i = 0; i < 16; i++) acc[qb][i] = 0.f;
+        for (int j0 = 0; j0 < A.dim; j0 += 64) {
+            const int nd = min(64, A.dim - j0);
+            __syncthreads();
+            for (int e = tid; e < 64 * 64; e += blockDim.x) {
+                const int qi = e & 63, j = e >> 6;
+                float v = 0.f;
+                if (j < nd) v = A.qf[(uint64_t)(qbase + qi) * A.dim + j0 + j];
+                qT[j * 64 + qi] = v;
+            }
+            for (int e = l; e < 32 * 64; e += 64) {
+                const int r = e >> 6, j = e & 63;
+                float v = 0.f;
+                if (live && j < nd) {
+                    uint64_t row;
+                    if (PREPASS) {
+                        const uint64_t sidx = (uint64_t)t * 32 + r;
+                        row = sidx < A.nsample ? sidx * A.stride : 0;
+                    } else {
+                        row = (uint64_t)t * 32 + r;
+                    }
+                    v = A.vecs[row * A.dim + j0 + j];
+                }
+                xw[j * kXP + r] = v;
+            }
+            __syncthreads();
+            const int h = l >> 5, qi = l & 31;
+            for (int j = 0; j < nd; j++) {
+                const float q0 = qT[j * 64 + qi], q1 = qT[j * 64 + qi + 32];
+#pragma unroll
+                for (int g4 = 0; g4 < 4; g4++) {
+                    const float4 xv = *(const float4 *)&xw[j * kXP + g4 * 8 + h * 4];
+                    const float xs[4] = {xv.x, xv.y, xv.z, xv.w};
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        const float e0 = xs[k] - q0, e1 = xs[k] - q1;
+                        acc[0][g4 * 4 + k] = fmaf(e0, e0, acc[0][g4 * 4 + k]);
+                        acc[1][g4 * 4 + k] = fmaf(e1, e1, acc[1][g4 * 4 + k]);
+                    }
+                }
+            }
+        }
+        if (live) scan_epilogue<PREPASS>(A, qbase, t, acc, tq, pq, lcnt, lbuf);
+    }
+    if (!PREPASS) flush_candidates(A, qbase, lcnt, lbuf);
+}
+
+/* ---------------- block-level helpers ---------------- */
+
+/* inclusive prefix sum over a 256-thread block */
+__device__ __forceinline__ uint32_t block_scan_incl(uint32_t v, uint32_t *tmp) {
+    const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t n = __shfl_up(v, o);
+        if (l >= o) v += n;
+    }
+    if (l == 63) tmp[w] = v;
+    __syncthreads();
+    uint32_t add = 0;
+    for (int i = 0; i < w; i++) add += tmp[i];
+    __syncthreads();
+    return v + add;
+}
+
+/*
+ * k-th smallest (1-based) of n unique u64 keys, 8-bit MSD radix select
+ * over a 256-thread block. Keys are read through `get`.
+ */
+template <typename Get>
+__device__ uint64_t block_kth(Get get, uint32_t n, uint32_t k, uint32_t *hist,
+                              uint32_t *tmp, uint32_t *bc) {
+    uint64_t prefix = 0;
+    const int t = threadIdx.x;
+    for (int shift = 56; shift >= 0; shift -= 8) {
+        hist[t] = 0;
+        __syncthreads();
+        for (uint32_t i = t; i < n; i += blockDim.x) {
+            const uint64_t key = get(i);
+            if (shift == 56 || (key >> (shift + 8)) == prefix)
+                atomicAdd(&hist[(key >> shift) & 255u], 1u);
+        }
+        __syncthreads();
+        const uint32_t c = hist[t];
+        const uint32_t incl = block_scan_incl(c, tmp);
+        const uint32_t excl = incl - c;
+        if (excl < k && k <= incl) {
+            bc[0] = t;
+            bc[1] = excl;
+            bc[2] = c;
+        }
+        __syncthreads();
+        const uint32_t dg = bc[0], ex = bc[1], cnt = bc[2];
+        prefix = (prefix << 8) | dg;
+        k -= ex;
+        __syncthreads();
+        if (cnt == 1 && shift > 0) {
+            /* unique key under this prefix: fetch it */
+            for (uint32_t i = t; i < n; i += blockDim.x) {
+                const uint64_t key = get(i);
+                if ((key >> shift) == prefix) {
+                    bc[4] = (uint32_t)key;
+                    bc[5] = (uint32_t)(key >> 32);
+                }
+            }
+            __syncthreads();
+            const uint64_t r = ((uint64_t)bc[5] << 32) | bc[4];
+            __syncthreads();
+            return r;
+        }
+    }
+    return prefix;
+}
+
+/* ascending bitonic sort of n (power of two) keys in LDS */
+__device__ void block_bitonic(uint64_t *a, uint32_t n) {
+    for (uint32_t k = 2; k <= n; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+                const uint32_t ixj = i ^ j;
+                if (ixj > i) {
+                    const bool up = (i & k) == 0;
+                    const uint64_t x = a[i], y = a[ixj];
+                    if ((x > y) == up) {
+                        a[i] = y;
+                        a[ixj] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+/* ---------------- 4. threshold from the pre-pass ---------------- */
+
+/* One block per query: tau = B-th smallest group-minimum key's distance,
+ * widened by 2 eps and a relative 2^-20 slack; also clears the pool
+ * counter and status (a fill, so the pipeline can be graph-captured). */
+__global__ __launch_bounds__(256) void k_tau(const uint64_t *__restrict__ gmin,
+                                             uint32_t ngroups, uint32_t B,
+                                             const float *__restrict__ eps,
+                                             float *__restrict__ tau,
+                                             uint32_t *__restrict__ pcnt,
+                                             uint32_t *__restrict__ status) {
+    __shared__ uint32_t hist[256], tmp[8], bc[8];
+    const int q = blockIdx.x;
+    const uint64_t *g = gmin + (uint64_t)q * ngroups;
+    float tv = __uint_as_float(0x7f800000u);
+    if (ngroups >= B) {
+        const uint64_t kth =
+            block_kth([&](uint32_t i) { return g[i]; }, ngroups, B, hist, tmp, bc);
+        const float d = key_dist(kth);
+        if (d < __uint_as_float(0x7f800000u)) {
+            const double w = (double)d * (1.0 + ldexp(1.0, -20)) +
+                             2.0 * (double)eps[q] + 1e-30;
+            tv = (float)w;
+            if ((double)tv < w) tv = nextafterf(tv, __uint_as_float(0x7f800000u));
+        }
+    }
+    if (threadIdx.x == 0) {
+        tau[q] = tv;
+        pcnt[q] = 0;
+        status[q] = 0;
+    }
+}
+
+/* clears pool counters and status, tau = +inf (gather route / no pre-pass) */
+__global__ void k_reset(int nq, float *tau, uint32_t *pcnt, uint32_t *status) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < nq) {
+        tau[q] = __uint_as_float(0x7f800000u);
+        pcnt[q] = 0;
+        status[q] = 0;
+    }
+}
+
+/* ---------------- span-gather route ---------------- */
+
+/*
+ * grid (nq, chunks). 16 lanes per enumerated row: enumeration position e ->
+ * row through the query's driver spans (binary search over its spanoff
+ * prefix, as in fv_brute_kernel, gpu_engine.cu:371-384) or e itself when
+ * the query has no spans; predicate evaluated when need_eval; exact
+ * CPU-order FP32 distance; key appended to the pool.
+ */
+__global__ __launch_bounds__(256) void k_gather(
+    const float *__restrict__ vecs, const double *__restrict__ cols,
+    const uint32_t *__restrict__ prows, uint64_t count, int dim,
+    const float *__restrict__ qf, const FvGpuQueryDesc *__restrict__ qds,
+    const FvGpuSpan *__restrict__ spans, const uint64_t *__restrict__ spanoff,
+    const PvOp *__restrict__ ops, const int32_t *__restrict__ poff,
+    const int32_t *__restrict__ plen, const int32_t *__restrict__ pofq,
+    const double *__restrict__ vals, uint64_t *__restrict__ pool,
+    uint32_t *__restrict__ pcnt, uint64_t cap_pool) {
+    const int q = blockIdx.x;
+    const FvGpuQueryDesc qd = qds[q];
+    const int grp = threadIdx.x >> 4, l16 = threadIdx.x & 15;
+    const int ngrp = blockDim.x >> 4;
+    const float *qv = qf + (uint64_t)q * dim;
+    const int p = pofq[q];
+    const uint64_t total = qd.nspans > 0 ? qd.total : count;
+    const uint64_t per = (total + gridDim.y - 1) / gridDim.y;
+    const uint64_t e0 = (uint64_t)blockIdx.y * per;
+    const uint64_t e1 = min(total, e0 + per);
+    const uint64_t *soff = spanoff + (qd.nspans > 0 ? qd.soff0 : 0);
+    for (uint64_t e = e0 + grp; e < e1; e += ngrp) {
+        uint32_t row;
+        if (qd.nspans == 0) {
+            row = (uint32_t)e;
+        } else {
+            int lo = 0, hi = qd.nspans - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (soff[mid] <= e) lo = mid; else hi = mid - 1;
+            }
+            const FvGpuSpan s = spans[qd.span0 + lo];
+            row = prows[(uint64_t)s.col * count + (s.lo + (uint32_t)(e - soff[lo]))];
+        }
+        bool pass = true;
+        if (p >= 0 && (qd.need_eval || qd.nspans == 0))
+            pass = eval_prog(ops + poff[p], plen[p], vals, cols, count, row);
+        float d = 0.f;
+        if (pass && row < count)
+            d = cpu_order_l2(vecs + (uint64_t)row * dim, qv, dim, l16);
+        if (pass && row < count && l16 == 0) {
+            const uint32_t gp = atomicAdd(&pcnt[q], 1u);
+            if (gp < cap_pool) pool[(uint64_t)q * cap_pool + gp] = make_key(d, row);
+        }
+    }
+}
+
+/* ---------------- 6. select + exact re-rank ---------------- */
+
+/*
+ * One 256-thread block per query. Pool keys are unique (each row enters a
+ * query's pool at most once). Finds the B-th smallest key kth (B =
+ * min(topb, pool size)); collects every key whose distance is <= dist(kth)
+ * + 2 eps (just the keys <= kth when eps == 0), re-ranks them with the
+ * exact CPU-order FP32 distance, sorts by (dist, row) and emits the first
+ * B. truncated = (B == topb) (gpu_engine.cu:1059-1064). Status bits flag a
+ * pool overflow or a re-rank set beyond kSelMax; the host then re-runs.
+ */
+__global__ __launch_bounds__(256) void k_select(
+    const uint64_t *__restrict__ pool, const uint32_t *__restrict__ pcnt,
+    uint64_t cap_pool, uint32_t topb, const float *__restrict__ eps,
+    const float *__restrict__ vecs, const float *__restrict__ qf, int dim,
+    FvHit *__restrict__ out, uint32_t out_stride, uint32_t *__restrict__ out_cnt,
+    uint8_t *__restrict__ out_flag, uint32_t *__restrict__ status) {
+    __shared__ uint64_t sel[kSelMax];
+    __shared__ uint32_t hist[256], tmp[8], bc[8];
+    __shared__ uint32_t nsel;
+    extern __shared__ float qsh[];
+    const int q = blockIdx.x, t = threadIdx.x;
+    const uint64_t *pk = pool + (uint64_t)q * cap_pool;
+    const uint32_t ntot = pcnt[q];
+    const uint32_t n = (uint32_t)min((uint64_t)ntot, cap_pool);
+    uint32_t st = 0;
+    if ((uint64_t)ntot > cap_pool) st |= kStOverflow;
+    const uint32_t B = min(topb, n);
+    if (B == 0) {
+        if (t == 0) {
+            out_cnt[q] = 0;
+            out_flag[q] = 0;
+            status[q] |= st;
+        }
+        return;
+    }
+    for (int j = t; j < dim; j += blockDim.x) qsh[j] = qf[(uint64_t)q * dim + j];
+    if (t == 0) nsel = 0;
+    const uint64_t kth =
+        block_kth([&](uint32_t i) { return pk[i]; }, n, B, hist, tmp, bc);
+    const float e = eps ? eps[q] : 0.f;
+    float lim = key_dist(kth);
+    if (e > 0.f) {
+        double w = (double)lim + 2.0 * (double)e;
+        lim = (float)w;
+        if ((double)lim < w) lim = nextafterf(lim, __uint_as_float(0x7f800000u));
+    }
+    __syncthreads();
+    for (uint32_t i = t; i < n; i += blockDim.x) {
+        const uint64_t k = pk[i];
+        const bool take = (e > 0.f) ? (key_dist(k) <= lim || k <= kth) : (k <= kth);
+        if (take) {
+            const uint32_t pos = atomicAdd(&nsel, 1u);
+            if (pos < (uint32_t)kSelMax) sel[pos] = k;
+        }
+    }
+    __syncthreads();
+    const uint32_t m = nsel;
+    if (m > (uint32_t)kSelMax) {
+        if (t == 0) {
+            status[q] |= st | kStCertFail;
+            out_cnt[q] = 0;
+            out_flag[q] = 0;
+        }
+        return;
+    }
+    /* exact re-rank: 16 lanes per key */
+    const int grp = t >> 4, l16 = t & 15;
+    for (uint32_t i = grp; i < m; i += blockDim.x / 16) {
+        const uint32_t row = key_row(sel[i]);
+        const float d = cpu_order_l2(vecs + (uint64_t)row * dim, qsh, dim, l16);
+        if (l16 == 0) sel[i] = make_key(d, row);
+    }
+    uint32_t np2 = 1;
+    while (np2 < m) np2 <<= 1;
+    for (uint32_t i = m + t; i < np2; i += blockDim.x) sel[i] = ~0ull;
+    __syncthreads();
+    block_bitonic(sel, np2);
+    for (uint32_t i = t; i < B; i += blockDim.x) {
+        FvHit h;
+        h.node = key_row(sel[i]);
+        h.dist = key_dist(sel[i]);
+        out[(uint64_t)q * out_stride + i] = h;
+    }
+    if (t == 0) {
+        out_cnt[q] = B;
+        out_flag[q] = (B == topb) ? 1 : 0;
+        status[q] |= st;
+    }
+}
+
+/* ---------------- host side ---------------- */
+
+static inline size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+void free_batch_buffers(FvGpuBatch *b) {
+    b->h_stage.release();
+    b->d_stage.release();
+    b->d_qbf.release();
+    b->d_qaux.release();
+    b->d_bitmap.release();
+    b->d_gmin.release();
+    b->d_pool.release();
+    b->d_pcnt.release();
+    b->d_out.release();
+    b->d_cnt.release();
+    b->d_flag.release();
+    b->h_out.release();
+}
+
+int stage_batch(FvGpuIndex *g, FvGpuBatch *b, int nq, const float *queries,
+                const FvGpuQueryDesc *qd, const uint8_t *dedup,
+                const FvGpuPredNode *nodes, int nnodes,
+                const int32_t *children, int nchildren, const double *invals,
+                int ninvals, const FvGpuSpan *spans, int nspans_total,
+                const uint64_t *spanoff, int nspanoff, uint32_t topb,
+                char *errbuf, size_t errlen) {
+    if (nq < 1 || nq > FV_GPU_MAX_BATCH || topb == 0 || topb > FV_GPU_MAX_TOPB ||
+        nspans_total > FV_GPU_MAX_SPANS * nq)
+        PV_FAIL("gpu: bad brute batch (nq=%d topb=%u)", nq, topb);
+    if (queries == nullptr || qd == nullptr) PV_FAIL("gpu: NULL queries or descriptors");
+    ProgSet ps;
+    if (compile_programs(nq, qd, nodes, nnodes, children, nchildren, invals,
+                         ninvals, g->ncols, &ps, errbuf, errlen) != 0)
+        return -1;
+    /* validate spans (used by the gather route) */
+    b->total.resize(nq);
+    b->nspans.resize(nq);
+    b->all_spans = true;
+    b->any_dedup = false;
+    b->sum_total = 0;
+    b->max_total = 0;
+    for (int q = 0; q < nq; q++) {
+        const FvGpuQueryDesc &d = qd[q];
+        b->total[q] = d.nspans > 0 ? d.total : g->count;
+        b->nspans[q] = d.nspans;
+        if (d.nspans > 0) {
+            if (spans == nullptr || spanoff == nullptr || d.span0 < 0 ||
+                (int64_t)d.span0 + d.nspans > nspans_total || d.soff0 < 0 ||
+                (int64_t)d.soff0 + d.nspans + 1 > nspanoff)
+                PV_FAIL("gpu: query %d: span range out of bounds", q);
+            for (int s = 0; s < d.nspans; s++) {
+                const FvGpuSpan &sp = spans[d.span0 + s];
+                if (sp.col < 0 || sp.col >= g->ncols || sp.lo > sp.hi ||
+                    sp.hi > g->count)
+                    PV_FAIL("gpu: query %d: bad span", q);
+                if (spanoff[d.soff0 + s + 1] - spanoff[d.soff0 + s] != (uint64_t)(sp.hi - sp.lo))
+                    PV_FAIL("gpu: query %d: span prefix mismatch", q);
+            }
+            if (spanoff[d.soff0] != 0 || spanoff[d.soff0 + d.nspans] != d.total)
+                PV_FAIL("gpu: query %d: span total mismatch", q);
+        }
+        if (d.nspans == 0 || (dedup && dedup[q])) b->all_spans = false;
+        if (dedup && dedup[q]) b->any_dedup = true;
+        b->sum_total += b->total[q];
+        b->max_total = std::max(b->max_total, b->total[q]);
+    }
+
+    b->owner = g;
+    b->nq = nq;
+    b->nqt = (nq + kQT - 1) / kQT;
+    b->topb = topb;
+    b->nprog = (int)ps.off.size();
+    const int nqp = b->nqt * kQT;
+    const int dim = g->dim;
+
+    /* packed staging layout */
+    size_t off = 0;
+    const size_t o_q = off; off = al256(off + (size_t)nqp * dim * 4);
+    const size_t o_qd = off; off = al256(off + (size_t)nq * sizeof(FvGpuQueryDesc));
+    const size_t o_ops = off; off = al256(off + ps.ops.size() * sizeof(PvOp));
+    const size_t o_vals = off; off = al256(off + ps.vals.size() * 8);
+    const size_t o_poff = off; off = al256(off + ps.off.size() * 4);
+    const size_t o_plen = off; off = al256(off + ps.len.size() * 4);
+    const size_t o_pofq = off; off = al256(off + (size_t)nqp * 4);
+    const size_t o_sp = off; off = al256(off + (size_t)std::max(nspans_total, 0) * sizeof(FvGpuSpan));
+    const size_t o_so = off; off = al256(off + (size_t)std::max(nspanoff, 0) * 8);
+    const size_t total_bytes = off;
+    PV_TRY(b->h_stage.ensure(total_bytes));
+    PV_TRY(b->d_stage.ensure(total_bytes));
+    unsigned char *h = b->h_stage.as<unsigned char>();
+    memset(h + o_q, 0, (size_t)nqp * dim * 4);
+    memcpy(h + o_q, queries, (size_t)nq * dim * 4);
+    memcpy(h + o_qd, qd, (size_t)nq * sizeof(FvGpuQueryDesc));
+    if (!ps.ops.empty()) memcpy(h + o_ops, ps.ops.data(), ps.ops.size() * sizeof(PvOp));
+    if (!ps.vals.empty()) memcpy(h + o_vals, ps.vals.data(), ps.vals.size() * 8);
+    if (!ps.off.empty()) memcpy(h + o_poff, ps.off.data(), ps.off.size() * 4);
+    if (!ps.len.empty()) memcpy(h + o_plen, ps.len.data(), ps.len.size() * 4);
+    {
+        int32_t *pofq = (int32_t *)(h + o_pofq);
+        for (int q = 0; q < nqp; q++) pofq[q] = q < nq ? ps.of_query[q] : -1;
+    }
+    if (nspans_total > 0) memcpy(h + o_sp, spans, (size_t)nspans_total * sizeof(FvGpuSpan));
+    if (nspanoff > 0) memcpy(h + o_so, spanoff, (size_t)nspanoff * 8);
+    PV_TRY(hipMemcpyAsync(b->d_stage.p, h, total_bytes, hipMemcpyHostToDevice, g->stream));
+    unsigned char *d = b->d_stage.as<unsigned char>();
+    b->d_q = (float *)(d + o_q);
+    b->d_qd = (FvGpuQueryDesc *)(d + o_qd);
+    b->d_ops = (PvOp *)(d + o_ops);
+    b->d_vals = (double *)(d + o_vals);
+    b->d_poff = (int32_t *)(d + o_poff);
+    b->d_plen = (int32_t *)(d + o_plen);
+    b->d_pofq = (int32_t *)(d + o_pofq);
+    b->d_spans = (FvGpuSpan *)(d + o_sp);
+    b->d_spanoff = (uint64_t *)(d + o_so);
+
+    PV_TRY(b->d_qbf.ensure((size_t)nqp * dim * 2));
+    PV_TRY(b->d_qaux.ensure((size_t)nqp * 16));
+    if (b->nprog > 0) PV_TRY(b->d_bitmap.ensure((size_t)b->nprog * (g->count_pad / 32) * 4));
+    b->out_stride = topb;
+    PV_TRY(b->d_out.ensure((size_t)nq * topb * sizeof(FvHit)));
+    PV_TRY(b->d_cnt.ensure((size_t)nq * 4));
+    PV_TRY(b->d_flag.ensure((size_t)nq));
+    PV_TRY(b->d_pcnt.ensure((size_t)nqp * 4));
+    /* the predicate bitmaps depend only on the staged inputs */
+    PV_TRY(launch_bitmap(g, b, g->stream));
+    return 0;
+}
+
+/* scan precision actually used for a request */
+static int scan_precision(const FvGpuIndex *g, const FvGpuBruteOpts *o) {
+    int p = (o && o->precision > 0) ? o->precision : g->precision;
+    if (p == FV_GPU_PREC_SQ8) p = FV_GPU_PREC_BF16;
+    if (p == FV_GPU_PREC_BF16 && (g->d_vbf == nullptr || g->dim % 128 != 0))
+        p = FV_GPU_PREC_FP32;
+    return p;
+}
+
+/* Enqueue the brute pipeline. stride_override > 0 forces the pre-pass
+ * sample stride (1 = every row: the tightest threshold). */
+static int run_brute(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
+                     uint32_t stride_override, char *errbuf, size_t errlen) {
+    hipStream_t s = g->stream;
+    const int prec = scan_precision(g, o);
+    const int nq = b->nq, nqp = b->nqt * kQT, dim = g->dim;
+    const uint32_t B = b->topb;
+    QAux qa = qaux_of(b);
+    FvGpuStats &st = g->last;
+    memset(&st, 0, sizeof(st));
+    st.precision = prec;
+
+    int route = o ? o->route : 0;
+    if (route == 0) {
+        /* gather when every query enumerates a short span list whose
+         * random-row reads cost less than one shared scan pass */
+        const double row_bytes = (double)dim * (prec == FV_GPU_PREC_BF16 ? 2.0 : 4.0);
+        const double scan_bytes = (double)g->count * row_bytes;
+        const double gather_bytes = (double)b->sum_total * dim * 4.0 * 2.0;
+        route = (b->all_spans && b->max_total <= 65536 && gather_bytes < scan_bytes) ? 2 : 1;
+    }
+    b->last_route = route;
+    st.route = route;
+
+    ev_record(g, 0);
+    hipLaunchKernelGGL(k_prep_queries, dim3(nqp), dim3(64), 0, s, b->d_q, nq, dim,
+                       b->d_qbf.as<uint16_t>(), qa.qnorm, qa.eps, g->xmax_norm,
+                       g->vmax_abs, g->all_int ? 1 : 0,
+                       prec == FV_GPU_PREC_BF16 ? 1 : 0);
+    PV_TRY(hipGetLastError());
+    ev_record(g, 1);
+
+    if (route == 2) {
+        if (b->any_dedup) PV_FAIL("gpu: gather route cannot deduplicate spans");
+        if (b->max_total > (uint64_t)(1u << 22))
+            PV_FAIL("gpu: gather route: enumeration of %llu rows too large",
+                    (unsigned long long)b->max_total);
+        b->cap_pool = std::max<uint64_t>(1024, b->max_total);
+        PV_TRY(b->d_pool.ensure((size_t)nq * b->cap_pool * 8));
+        hipLaunchKernelGGL(k_reset, dim3((nq + 255) / 256), dim3(256), 0, s, nq,
+                           qa.tau, b->d_pcnt.as<uint32_t>(), qa.status);
+        const uint32_t chunks = (uint32_t)std::min<uint64_t>(
+            64, std::max<uint64_t>(1, (b->max_total + 4095) / 4096));
+        hipLaunchKernelGGL(k_gather, dim3(nq, chunks), dim3(256), 0, s, g->d_vecs,
+                           g->d_cols, g->d_prows, g->count, dim, b->d_q, b->d_qd,
+                           b->d_spans, b->d_spanoff, b->d_ops, b->d_poff, b->d_plen,
+                           b->d_pofq, b->d_vals, b->d_pool.as<uint64_t>(),
+                           b->d_pcnt.as<uint32_t>(), b->cap_pool);
+        PV_TRY(hipGetLastError());
+        ev_record(g, 4);
+        /* gathered distances are already exact: eps = 0 */
+        hipLaunchKernelGGL(k_select, dim3(nq), dim3(256), dim * 4, s,
+                           b->d_pool.as<uint64_t>(), b->d_pcnt.as<uint32_t>(),
+                           b->cap_pool, B, (const float *)nullptr,
+                           g->d_vecs, b->d_q, dim, b->d_out.as<FvHit>(),
+                           b->out_stride, b->d_cnt.as<uint32_t>(),
+                           b->d_flag.as<uint8_t>(), qa.status);
+        PV_TRY(hipGetLastError());
+        ev_record(g, 5);
+        st.scan_rows = b->sum_total;
+        st.scan_bytes = b->sum_total * (uint64_t)dim * 4;
+        return 0;
+    }
+
+    /* ---- full-scan route ---- */
+    uint32_t stride = stride_override ? stride_override
+                                      : (o && o->sample_stride ? o->sample_stride : 0);
+    if (stride == 0) {
+        stride = B <= 64 ? 64 : std::max<uint32_t>(1, 4096 / B);
+        const uint64_t min_stride = (g->count + 262143) / 262144;
+        stride = (uint32_t)std::max<uint64_t>(stride, min_stride);
+    }
+    const uint64_t nsample = (g->count + stride - 1) / stride;
+    const uint32_t stiles = (uint32_t)((nsample + 31) / 32);
+    b->stride = stride;
+    b->ngroups = stiles * 2;
+    b->cap_pool = std::max<uint64_t>(16384, (uint64_t)8 * B * stride);
+    if (stride_override == 1) b->cap_pool = std::max<uint64_t>(b->cap_pool, (uint64_t)64 * B + 4096);
+    PV_TRY(b->d_gmin.ensure((size_t)nqp * b->ngroups * 8));
+    PV_TRY(b->d_pool.ensure((size_t)nq * b->cap_pool * 8));
+
+    ScanArgs A = {};
+    A.vbf = g->d_vbf;
+    A.vecs = g->d_vecs;
+    A.xnorm = g->d_norm;
+    A.count = g->count;
+    A.count_pad = g->count_pad;
+    A.dim = dim;
+    A.qbf = b->d_qbf.as<uint16_t>();
+    A.qf = b->d_q;
+    A.qnorm = qa.qnorm;
+    A.tau = qa.tau;
+    A.pofq = b->d_pofq;
+    A.bitmap = b->d_bitmap.as<uint32_t>();
+    A.words = g->count_pad / 32;
+    A.nq = nq;
+    A.pool = b->d_pool.as<uint64_t>();
+    A.pcnt = b->d_pcnt.as<uint32_t>();
+    A.cap_pool = b->cap_pool;
+    A.gmin = b->d_gmin.as<uint64_t>();
+    A.ngroups = b->ngroups;
+
+    const bool bf = prec == FV_GPU_PREC_BF16;
+    const size_t shm_bf = (size_t)2 * (dim / 16) * 64 * 16 + kQT * 4 + kQT * kLdsCand * 8;
+    /* pre-pass over sampled rows */
+    {
+        ScanArgs P = A;
+        P.ntiles = stiles;
+        P.stride = stride;
+        P.nsample = nsample;
+        const uint32_t gx = std::max<uint32_t>(1, (stiles + 3) / 4);
+        if (bf)
+            hipLaunchKernelGGL(k_scan_bf16<true>, dim3(gx, b->nqt), dim3(256), shm_bf, s, P);
+        else
+            hipLaunchKernelGGL(k_scan_f32<true>, dim3(gx, b->nqt), dim3(256), 0, s, P);
+        PV_TRY(hipGetLastError());
+    }
+    ev_record(g, 2);
+    hipLaunchKernelGGL(k_tau, dim3(nq), dim3(256), 0, s, b->d_gmin.as<uint64_t>(),
+                       b->ngroups, B, qa.eps, qa.tau, b->d_pcnt.as<uint32_t>(),
+                       qa.status);
+    PV_TRY(hipGetLastError());
+    ev_record(g, 3);
+    {
+        ScanArgs M = A;
+        M.ntiles = (uint32_t)(g->count_pad / 32);
+        M.stride = 1;
+        M.nsample = g->count;
+        uint32_t gx = (uint32_t)std::max(64, (2 * g->ncu) / std::max(1, b->nqt));
+        gx = std::min<uint32_t>(gx, std::max<uint32_t>(1, M.ntiles / 4));
+        if (bf)
+            hipLaunchKernelGGL(k_scan_bf16<false>, dim3(gx, b->nqt), dim3(256), shm_bf, s, M);
+        else
+            hipLaunchKernelGGL(k_scan_f32<false>, dim3(gx, b->nqt), dim3(256), 0, s, M);
+        PV_TRY(hipGetLastError());
+    }
+    ev_record(g, 4);
+    hipLaunchKernelGGL(k_select, dim3(nq), dim3(256), dim * 4, s,
+                       b->d_pool.as<uint64_t>(), b->d_pcnt.as<uint32_t>(),
+                       b->cap_pool, B, qa.eps, g->d_vecs, b->d_q, dim,
+                       b->d_out.as<FvHit>(), b->out_stride,
+                       b->d_cnt.as<uint32_t>(), b->d_flag.as<uint8_t>(), qa.status);
+    PV_TRY(hipGetLastError());
+    ev_record(g, 5);
+    st.scan_rows = g->count;
+    st.scan_bytes = g->count * (uint64_t)dim * (bf ? 2 : 4) * (uint64_t)b->nqt;
+    return 0;
+}
+
+/* copy outputs and per-query status to pinned host memory (synchronous) */
+static int fetch_raw(FvGpuIndex *g, FvGpuBatch *b, char *errbuf, size_t errlen) {
+    const int nq = b->nq, nqp = b->nqt * kQT;
+    const size_t o_out = 0;
+    const size_t o_cnt = al256((size_t)nq * b->out_stride * sizeof(FvHit));
+    const size_t o_flag = al256(o_cnt + (size_t)nq * 4);
+    const size_t o_st = al256(o_flag + (size_t)nq);
+    const size_t tot = o_st + (size_t)nqp * 4;
+    PV_TRY(b->h_out.ensure(tot));
+    unsigned char *h = b->h_out.as<unsigned char>();
+    QAux qa = qaux_of(b);
+    PV_TRY(hipMemcpyAsync(h + o_out, b->d_out.p, (size_t)nq * b->out_stride * sizeof(FvHit),
+                          hipMemcpyDeviceToHost, g->stream));
+    PV_TRY(hipMemcpyAsync(h + o_cnt, b->d_cnt.p, (size_t)nq * 4, hipMemcpyDeviceToHost, g->stream));
+    PV_TRY(hipMemcpyAsync(h + o_flag, b->d_flag.p, (size_t)nq, hipMemcpyDeviceToHost, g->stream));
+    PV_TRY(hipMemcpyAsync(h + o_st, qa.status, (size_t)nq * 4, hipMemcpyDeviceToHost, g->stream));
+    PV_TRY(hipStreamSynchronize(g->stream));
+    return 0;
+}
+
+static void fetch_views(FvGpuBatch *b, const FvHit **out, const uint32_t **cnt,
+                        const uint8_t **flag, const uint32_t **status) {
+    const int nq = b->nq;
+    const size_t o_cnt = al256((size_t)nq * b->out_stride * sizeof(FvHit));
+    const size_t o_flag = al256(o_cnt + (size_t)nq * 4);
+    const size_t o_st = al256(o_flag + (size_t)nq);
+    unsigned char *h = b->h_out.as<unsigned char>();
+    *out = (const FvHit *)h;
+    *cnt = (const uint32_t *)(h + o_cnt);
+    *flag = (const uint8_t *)(h + o_flag);
+    *status = (const uint32_t *)(h + o_st);
+}
+
+static void fill_stats_times(FvGpuIndex *g) {
+    if (!g->timing) return;
+    FvGpuStats &st = g->last;
+    st.t_prepass_us = ev_elapsed_us(g, 1, 2);
+    st.t_scan_us = st.route == 2 ? ev_elapsed_us(g, 1, 4) : ev_elapsed_us(g, 3, 4);
+    if (st.route == 2) st.t_gather_us = st.t_scan_us;
+    st.t_merge_us = ev_elapsed_us(g, 2, 3);
+    st.t_rerank_us = ev_elapsed_us(g, 4, 5);
+    st.t_total_us = ev_elapsed_us(g, 0, 5);
+}
+
+} // namespace pv
+
+using namespace pv;
+
+static int brute_sync(FvGpuIndex *g, int nq, const float *queries,
+                      const FvGpuQueryDesc *qd, const uint8_t *dedup,
+                      const FvGpuPredNode *nodes, int nnodes,
+                      const int32_t *children, int nchildren,
+                      const double *invals, int ninvals, const FvGpuSpan *spans,
+                      int nspans_total, const uint64_t *spanoff, int nspanoff,
+                      uint32_t topb, const FvGpuBruteOpts *opts, FvHit *out,
+                      uint32_t *out_cnt, uint8_t *truncated, char *errbuf,
+                      size_t errlen) {
+    if (g == nullptr) PV_FAIL("gpu: NULL index");
+    PV_TRY(hipSetDevice(g->device));
+    if (out == nullptr || out_cnt == nullptr || truncated == nullptr)
+        PV_FAIL("gpu: NULL output arrays");
+    if (g->sync_batch == nullptr) g->sync_batch = new FvGpuBatch();
+    FvGpuBatch *b = g->sync_batch;
+    if (stage_batch(g, b, nq, queries, qd, dedup, nodes, nnodes, children,
+                    nchildren, invals, ninvals, spans, nspans_total, spanoff,
+                    nspanoff, topb, errbuf, errlen) != 0)
+        return -1;
+    if (run_brute(g, b, opts, 0, errbuf, errlen) != 0) return -1;
+    if (fetch_raw(g, b, errbuf, errlen) != 0) return -1;
+    fill_stats_times(g);
+    const FvHit *ho;
+    const uint32_t *hc, *hs;
+    const uint8_t *hf;
+    fetch_views(b, &ho, &hc, &hf, &hs);
+    bool bad = false;
+    for (int q = 0; q < nq; q++) bad |= hs[q] != 0;
+    if (bad) {
+        /* pool overflow or oversized re-rank set: retry with the tightest
+         * threshold (pre-pass over every row) */
+        FvGpuBruteOpts o2 = opts ? *opts : FvGpuBruteOpts{};
+        if (o2.route == 0) o2.route = 1;
+        g->last.certified_fail = 0;
+        for (int q = 0; q < nq; q++) g->last.certified_fail += hs[q] != 0;
+        const uint32_t nfail = g->last.certified_fail;
+        if (run_brute(g, b, &o2, 1, errbuf, errlen) != 0) return -1;
+        if (fetch_raw(g, b, errbuf, errlen) != 0) return -1;
+        fill_stats_times(g);
+        g->last.certified_fail = nfail;
+        fetch_views(b, &ho, &hc, &hf, &hs);
+        for (int q = 0; q < nq; q++)
+            if (hs[q] != 0)
+                PV_FAIL("gpu: query %d: candidate set exceeds capacity (status %u)",
+                        q, hs[q]);
+    }
+    for (int q = 0; q < nq; q++) {
+        const uint32_t c = hc[q];
+        memcpy(out + (uint64_t)q * topb, ho + (uint64_t)q * b->out_stride,
+               (size_t)c * sizeof(FvHit));
+        out_cnt[q] = c;
+        truncated[q] = hf[q];
+    }
+    return 0;
+}
+
+extern "C" int fv_gpu_brute_batch(FvGpuIndex *g, int nq, const float *queries,
+                                  const FvGpuQueryDesc *qd, const uint8_t *dedup,
+                                  const FvGpuPredNode *nodes, int nnodes,
+                                  const int32_t *children, int nchildren,
+                                  const double *invals, int ninvals,
+                                  const FvGpuSpan *spans, int nspans_total,
+                                  const uint64_t *spanoff, int nspanoff,
+                                  uint32_t topb, FvHit *out, uint32_t *out_cnt,
+                                  uint8_t *truncated, char *errbuf,
+                                  size_t errlen) {
+    try {
+        return brute_sync(g, nq, queries, qd, dedup, nodes, nnodes, children,
+                          nchildren, invals, ninvals, spans, nspans_total,
+                          spanoff, nspanoff, topb, nullptr, out, out_cnt,
+                          truncated, errbuf, errlen);
+    } catch (...) {
+        snprintf(errbuf, errlen, "gpu: internal error");
+        return -1;
+    }
+}
+
+extern "C" int fv_gpu_brute_batch_ex(FvGpuIndex *g, int nq, const float *queries,
+                                     const FvGpuQueryDesc *qd, const uint8_t *dedup,
+                                     const FvGpuPredNode *nodes, int nnodes,
+                                     const int32_t *children, int nchildren,
+                                     const double *invals, int ninvals,
+                                     const FvGpuSpan *spans, int nspans_total,
+                                     const uint64_t *spanoff, int nspanoff,
+                                     uint32_t topb, const FvGpuBruteOpts *opts,
+                                     FvHit *out, uint32_t *out_cnt,
+                                     uint8_t *truncated, char *errbuf,
+                                     size_t errlen) {
+    try {
+        return brute_sync(g, nq, queries, qd, dedup, nodes, nnodes, children,
+                          nchildren, invals, ninvals, spans, nspans_total,
+                          spanoff, nspanoff, topb, opts, out, out_cnt,
+                          truncated, errbuf, errlen);
+    } catch (...) {
+        snprintf(errbuf, errlen, "gpu: internal error");
+        return -1;
+    }
+}
+
+extern "C" FvGpuBatch *fv_gpu_batch_stage(
+    FvGpuIndex *g, int nq, const float *queries, const FvGpuQueryDesc *qd,
+    const uint8_t *dedup, const FvGpuPredNode *nodes, int nnodes,
+    const int32_t *children, int nchildren, const double *invals, int ninvals,
+    const FvGpuSpan *spans, int nspans_total, const uint64_t *spanoff,
+    int nspanoff, uint32_t topb, char *errbuf, size_t errlen) {
+    if (g == nullptr) {
+        snprintf(errbuf, errlen, "gpu: NULL index");
+        return nullptr;
+    }
+    try {
+        if (hipSetDevice(g->device) != hipSuccess) {
+            snprintf(errbuf, errlen, "gpu: hipSetDevice failed");
+            return nullptr;
+        }
+        FvGpuBatch *b = new FvGpuBatch();
+        if (stage_batch(g, b, nq, queries, qd, dedup, nodes, nnodes, children,
+                        nchildren, invals, ninvals, spans, nspans_total, spanoff,
+                        nspanoff, topb, errbuf, errlen) != 0 ||
+            hipStreamSynchronize(g->stream) != hipSuccess) {
+            free_batch_buffers(b);
+            delete b;
+            return nullptr;
+        }
+        return b;
+    } catch (...) {
+        snprintf(errbuf, errlen, "gpu: internal error");
+        return nullptr;
+    }
+}
+
+extern "C" int fv_gpu_batch_run_brute(FvGpuIndex *g, FvGpuBatch *b,
+                                      const FvGpuBruteOpts *opts, char *errbuf,
+                                      size_t errlen) {
+    if (g == nullptr || b == nullptr || b->owner != g) PV_FAIL("gpu: bad batch");
+    try {
+        return run_brute(g, b, opts, 0, errbuf, errlen);
+    } catch (...) {
+        snprintf(errbuf, errlen, "gpu: internal error");
+        return -1;
+    }
+}
+
+extern "C" int fv_gpu_sync(FvGpuIndex *g, char *errbuf, size_t errlen) {
+    if (g == nullptr) PV_FAIL("gpu: NULL index");
+    PV_TRY(hipStreamSynchronize(g->stream));
+    fill_stats_times(g);
+    return 0;
+}
+
+extern "C" int fv_gpu_batch_fetch(FvGpuIndex *g, FvGpuBatch *b, FvHit *out,
+                                  uint32_t *out_cnt, uint8_t *flags,
+                                  char *errbuf, size_t errlen) {
+    if (g == nullptr || b == nullptr || b->owner != g) PV_FAIL("gpu: bad batch");
+    if (fetch_raw(g, b, errbuf, errlen) != 0) return -1;
+    const FvHit *ho;
+    const uint32_t *hc, *hs;
+    const uint8_t *hf;
+    fetch_views(b, &ho, &hc, &hf, &hs);
+    int rc = 0;
+    for (int q = 0; q < b->nq; q++) {
+        const uint32_t c = hc[q];
+        if (out) memcpy(out + (uint64_t)q * b->out_stride, ho + (uint64_t)q * b->out_stride,
+                        (size_t)c * sizeof(FvHit));
+        if (out_cnt) out_cnt[q] = c;
+        if (flags) flags[q] = hf[q];
+        if (hs[q] != 0) rc = -1;
+    }
+    if (rc) snprintf(errbuf, errlen, "gpu: some queries need a re-run (candidate capacity)");
+    return rc;
+}
+
+extern "C" void fv_gpu_batch_free(FvGpuBatch *b) {
+    if (!b) return;
+    if (b->owner) hipSetDevice(b->owner->device);
+    free_batch_buffers(b);
+    delete b;
+}

@@ -1,0 +1,322 @@
+/*
+ * pv_index.hip — GPU-resident index: device choice, upload, free, probes.
+ *
+ * Reference: FvGpuIndex / pick_device / fv_gpu_index_upload /
+ * fv_gpu_index_free (gpu_engine.cu:197-234, 1506-1596, 1692-1773),
+ * fv_gpu_available (1489), fv_gpu_tile_variant_qt and fv_gpu_mem_probe
+ * (2546-2566).
+ *
+ * Upload copies the caller's host arrays (they may live in a read-only
+ * mmap) into HBM and derives the scan layouts on the GPU: a BF16 copy of
+ * the vectors, ||x||^2 per row, the level-0 adjacency in a fixed-stride
+ * 0xFFFFFFFF-padded layout (one 128-byte line per node for M0 = 32), and
+ * the integer/magnitude facts that decide whether BF16 and FP32 scans are
+ * exact. All work is on one stream owned by the index.
+ */
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <mutex>
+
+#include "pv_device.h"
+
+namespace pv {
+
+void ev_record(FvGpuIndex *g, int i) {
+    if (g->timing) hipEventRecord(g->ev[i], g->stream);
+}
+
+double ev_elapsed_us(FvGpuIndex *g, int a, int b) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, g->ev[a], g->ev[b]) != hipSuccess) return 0.0;
+    return (double)ms * 1000.0;
+}
+
+namespace {
+
+constexpr int kMaxDevices = 32;
+int dev_refcount[kMaxDevices];
+std::mutex dev_mu;
+
+/* device with the most free memory above need + 512 MB (gpu_engine.cu:1518-1545).
+ * Unlike the CUDA original this does not reset other devices: HIP contexts
+ * are per-process primary contexts and resetting one a caller still uses
+ * would invalidate its allocations. */
+int pick_device(size_t need) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return -1;
+    const size_t margin = (size_t)512 << 20;
+    int best = -1;
+    size_t bestfree = 0;
+    for (int i = 0; i < n; i++) {
+        if (hipSetDevice(i) != hipSuccess) continue;
+        size_t f = 0, t = 0;
+        if (hipMemGetInfo(&f, &t) != hipSuccess) continue;
+        if (f > bestfree && f > need + margin) {
+            bestfree = f;
+            best = i;
+        }
+    }
+    if (best >= 0) hipSetDevice(best);
+    (void)hipGetLastError();
+    return best;
+}
+
+/* fp32 -> bf16 copy and ||x||^2 (sequential fp32 fma order), plus per-block
+ * max |x|, max ||x|| and "all integers" flags. One wave per row. */
+__global__ __launch_bounds__(256) void k_derive_rows(
+    const float *__restrict__ v, uint64_t count, uint64_t count_pad, int dim,
+    uint16_t *__restrict__ vbf, float *__restrict__ norm,
+    unsigned int *__restrict__ stats /* [0]=max|x| bits, [1]=max||x||^2 bits, [2]=nonint */) {
+    const int l = threadIdx.x & 63;
+    const uint64_t row = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= count_pad) return;
+    float ss = 0.f, amax = 0.f;
+    int nonint = 0;
+    for (int j = l; j < dim; j += 64) {
+        float x = row < count ? v[row * dim + j] : 0.f;
+        if (vbf) vbf[row * dim + j] = f2bf(x);
+        ss = fmaf(x, x, ss);
+        amax = fmaxf(amax, fabsf(x));
+        nonint |= (x != rintf(x)) ? 1 : 0;
+    }
+    for (int o = 32; o; o >>= 1) {
+        ss += __shfl_xor(ss, o);
+        amax = fmaxf(amax, __shfl_xor(amax, o));
+        nonint |= __shfl_xor(nonint, o);
+    }
+    if (l == 0) {
+        norm[row] = ss;
+        atomicMax(&stats[0], __float_as_uint(amax));
+        atomicMax(&stats[1], __float_as_uint(ss));
+        if (nonint) atomicOr(&stats[2], 1u);
+    }
+}
+
+/* level-0 lists [cnt, neigh...] (stride l0_stride) -> padded rows of
+ * nbr_stride ids, 0xFFFFFFFF after the count; ids >= count are dropped. */
+__global__ void k_pack_l0(const uint32_t *__restrict__ l0, uint32_t l0_stride,
+                          uint64_t count, int nbr_stride,
+                          uint32_t *__restrict__ nbr) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count * (uint64_t)nbr_stride) return;
+    const uint64_t node = i / nbr_stride;
+    const uint32_t k = (uint32_t)(i % nbr_stride);
+    const uint32_t *lst = l0 + node * l0_stride;
+    uint32_t cnt = lst[0];
+    if (cnt > l0_stride - 1) cnt = l0_stride - 1;
+    uint32_t v = kNoNode;
+    if (k < cnt) {
+        v = lst[1 + k];
+        if (v >= count) v = kNoNode;
+    }
+    nbr[i] = v;
+}
+
+} // namespace
+} // namespace pv
+
+using namespace pv;
+
+extern "C" int fv_gpu_available(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n > 0;
+}
+
+extern "C" int fv_gpu_index_device(const FvGpuIndex *g) { return g ? g->device : -1; }
+
+extern "C" int fv_gpu_index_precision(const FvGpuIndex *g) {
+    return g ? g->precision : -1;
+}
+
+extern "C" void fv_gpu_index_free(FvGpuIndex *g) {
+    if (!g) return;
+    if (g->device >= 0) hipSetDevice(g->device);
+    if (g->stream) hipStreamSynchronize(g->stream);
+    hipFree(g->d_vecs);
+    hipFree(g->d_vbf);
+    hipFree(g->d_norm);
+    hipFree(g->d_cols);
+    hipFree(g->d_prows);
+    hipFree(g->d_nbr);
+    if (g->sync_batch) {
+        free_batch_buffers(g->sync_batch);
+        delete g->sync_batch;
+    }
+    g->d_visit.release();
+    for (int i = 0; i < 16; i++)
+        if (g->ev[i]) hipEventDestroy(g->ev[i]);
+    if (g->stream) hipStreamDestroy(g->stream);
+    {
+        std::lock_guard<std::mutex> lk(dev_mu);
+        if (g->device >= 0 && g->device < kMaxDevices && dev_refcount[g->device] > 0)
+            dev_refcount[g->device]--;
+    }
+    delete g;
+}
+
+static FvGpuIndex *upload_impl(const float *vectors, const double *cols,
+                               const uint32_t *post_rows, const uint32_t *l0,
+                               uint32_t l0_stride, uint64_t count, int dim,
+                               int ncols, int device, int precision, char *errbuf,
+                               size_t errlen) {
+#define UP_FAIL(...)                                                          \
+    do {                                                                      \
+        snprintf(errbuf, errlen, __VA_ARGS__);                                \
+        fv_gpu_index_free(g);                                                 \
+        return nullptr;                                                       \
+    } while (0)
+#define UP_TRY(call)                                                          \
+    do {                                                                      \
+        hipError_t e__ = (call);                                              \
+        if (e__ != hipSuccess)                                                \
+            UP_FAIL("gpu: index upload failed: %s: %s", #call,                \
+                    hipGetErrorString(e__));                                  \
+    } while (0)
+    FvGpuIndex *g = nullptr;
+    if (vectors == nullptr || count == 0 || dim <= 0 || ncols < 0 ||
+        count >= 0xFFFFFFFFull || (ncols > 0 && (cols == nullptr || post_rows == nullptr))) {
+        snprintf(errbuf, errlen, "gpu: bad upload arguments (count=%llu dim=%d ncols=%d)",
+                 (unsigned long long)count, dim, ncols);
+        return nullptr;
+    }
+    if (precision < FV_GPU_PREC_FP32 || precision > FV_GPU_PREC_BF16) {
+        snprintf(errbuf, errlen, "gpu: unknown precision %d", precision);
+        return nullptr;
+    }
+    const uint64_t count_pad = (count + 63) & ~63ull;
+    const bool want_bf = dim % 16 == 0;
+    const size_t need = count_pad * (size_t)dim * (want_bf ? 6 : 4) + count_pad * 4 +
+                        count * (size_t)ncols * 12 + (l0 ? count * (size_t)l0_stride * 8 : 0);
+    if (device < 0) device = pick_device(need);
+    if (device < 0) {
+        snprintf(errbuf, errlen, "gpu: no HIP device with %.1f GB free",
+                 (need + ((size_t)512 << 20)) / 1e9);
+        return nullptr;
+    }
+    g = new FvGpuIndex();
+    g->device = device;
+    g->precision = precision;
+    g->count = count;
+    g->count_pad = count_pad;
+    g->dim = dim;
+    g->ncols = ncols;
+    g->l0_stride = l0 ? l0_stride : 0;
+    UP_TRY(hipSetDevice(device));
+    {
+        std::lock_guard<std::mutex> lk(dev_mu);
+        if (device < kMaxDevices) dev_refcount[device]++;
+    }
+    {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+            g->ncu = prop.multiProcessorCount;
+    }
+    UP_TRY(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+    for (int i = 0; i < 16; i++) UP_TRY(hipEventCreate(&g->ev[i]));
+    hipStream_t s = g->stream;
+
+    UP_TRY(hipMalloc(&g->d_vecs, count_pad * (size_t)dim * 4));
+    UP_TRY(hipMemsetAsync(g->d_vecs, 0, count_pad * (size_t)dim * 4, s));
+    UP_TRY(hipMemcpyAsync(g->d_vecs, vectors, count * (size_t)dim * 4,
+                          hipMemcpyHostToDevice, s));
+    if (want_bf) UP_TRY(hipMalloc(&g->d_vbf, count_pad * (size_t)dim * 2));
+    UP_TRY(hipMalloc(&g->d_norm, count_pad * 4));
+    unsigned int *d_stats = nullptr;
+    UP_TRY(hipMalloc(&d_stats, 16));
+    UP_TRY(hipMemsetAsync(d_stats, 0, 16, s));
+    hipLaunchKernelGGL(k_derive_rows, dim3((unsigned)((count_pad + 3) / 4)), dim3(256), 0, s,
+                       g->d_vecs, count, count_pad, dim, g->d_vbf, g->d_norm, d_stats);
+    {
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) {
+            hipFree(d_stats);
+            UP_FAIL("gpu: derive kernel: %s", hipGetErrorString(e));
+        }
+    }
+    unsigned int hs[4] = {0, 0, 0, 0};
+    {
+        hipError_t e = hipMemcpyAsync(hs, d_stats, 16, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        hipFree(d_stats);
+        if (e != hipSuccess) UP_FAIL("gpu: derive stats: %s", hipGetErrorString(e));
+    }
+    float amax, n2max;
+    memcpy(&amax, &hs[0], 4);
+    memcpy(&n2max, &hs[1], 4);
+    g->vmax_abs = amax;
+    g->xmax_norm = sqrtf(n2max) * (1.0f + 1e-6f);
+    g->all_int = hs[2] == 0;
+
+    if (ncols > 0) {
+        UP_TRY(hipMalloc(&g->d_cols, (size_t)ncols * count * 8));
+        UP_TRY(hipMemcpyAsync(g->d_cols, cols, (size_t)ncols * count * 8,
+                              hipMemcpyHostToDevice, s));
+        UP_TRY(hipMalloc(&g->d_prows, (size_t)ncols * count * 4));
+        UP_TRY(hipMemcpyAsync(g->d_prows, post_rows, (size_t)ncols * count * 4,
+                              hipMemcpyHostToDevice, s));
+    }
+    if (l0 != nullptr && l0_stride > 1) {
+        const int deg = (int)l0_stride - 1;
+        g->nbr_stride = (deg + 7) & ~7;
+        uint32_t *d_l0 = nullptr;
+        UP_TRY(hipMalloc(&d_l0, count * (size_t)l0_stride * 4));
+        hipError_t e = hipMemcpyAsync(d_l0, l0, count * (size_t)l0_stride * 4,
+                                      hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipMalloc(&g->d_nbr, count * (size_t)g->nbr_stride * 4);
+        if (e == hipSuccess) {
+            const uint64_t n = count * (uint64_t)g->nbr_stride;
+            hipLaunchKernelGGL(k_pack_l0, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                               d_l0, l0_stride, count, g->nbr_stride, g->d_nbr);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        hipFree(d_l0);
+        if (e != hipSuccess) UP_FAIL("gpu: graph upload: %s", hipGetErrorString(e));
+    }
+    UP_TRY(hipStreamSynchronize(s));
+    return g;
+#undef UP_TRY
+#undef UP_FAIL
+}
+
+extern "C" FvGpuIndex *fv_gpu_index_upload(const float *vectors, const double *cols,
+                                           const uint32_t *post_rows,
+                                           const uint32_t *l0, uint32_t l0_stride,
+                                           uint64_t count, int dim, int ncols,
+                                           int device, int precision, char *errbuf,
+                                           size_t errlen) {
+    try {
+        return upload_impl(vectors, cols, post_rows, l0, l0_stride, count, dim, ncols,
+                           device, precision, errbuf, errlen);
+    } catch (...) {
+        snprintf(errbuf, errlen, "gpu: internal error during upload");
+        return nullptr;
+    }
+}
+
+/* Query-tile width of the fast brute path (engine_api.h:190-194): the scan
+ * processes 64 queries per pass for every dimension. */
+extern "C" int fv_gpu_tile_variant_qt(int dim) { return dim > 0 ? kQT : 0; }
+
+extern "C" int fv_gpu_mem_probe(uint64_t *free_b, uint64_t *total_b) {
+    static int probed = -2;
+    if (probed == -2) probed = pick_device(0);
+    if (probed < 0) return -1;
+    size_t f = 0, t = 0;
+    if (hipSetDevice(probed) != hipSuccess || hipMemGetInfo(&f, &t) != hipSuccess)
+        return -1;
+    if (free_b) *free_b = f;
+    if (total_b) *total_b = t;
+    return probed;
+}
+
+extern "C" void fv_gpu_set_timing(FvGpuIndex *g, int on) {
+    if (g) g->timing = on != 0;
+}
+
+extern "C" void fv_gpu_last_stats(const FvGpuIndex *g, FvGpuStats *st) {
+    if (g && st) *st = g->last;
+}

@@ -1,0 +1,227 @@
+/*
+ * pv_internal.h — internal types of the MI355X query engine.
+ *
+ * Device data layout (one index, resident in HBM):
+ *   vecs   fp32  [count_pad][dim]   row-major, rows >= count are zero
+ *   vbf    bf16  [count_pad][dim]   RNE copy for the MFMA scan (dim % 16 == 0)
+ *   norm   fp32  [count_pad]        ||x||^2 (0 for padding rows)
+ *   cols   f64   [ncols][count]     scalar columns, NULL = NaN
+ *   prows  u32   [ncols][count]     posting rows (rows sorted by value)
+ *   nbr    u32   [count][nbr_stride] level-0 adjacency, 0xFFFFFFFF-padded
+ * count_pad is count rounded up to 64 so 32-row tiles never read past the
+ * allocation.
+ */
+#ifndef PV_INTERNAL_H
+#define PV_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+
+#include <vector>
+
+#include "../../include/paves_gpu.h"
+
+#define PV_TRY(call)                                                          \
+    do {                                                                      \
+        hipError_t e__ = (call);                                              \
+        if (e__ != hipSuccess) {                                              \
+            snprintf(errbuf, errlen, "gpu: %s failed: %s", #call,             \
+                     hipGetErrorString(e__));                                 \
+            return -1;                                                        \
+        }                                                                     \
+    } while (0)
+
+#define PV_FAIL(...)                                                          \
+    do {                                                                      \
+        snprintf(errbuf, errlen, __VA_ARGS__);                                \
+        return -1;                                                            \
+    } while (0)
+
+namespace pv {
+
+constexpr uint32_t kNoNode = 0xFFFFFFFFu;
+constexpr int kQT = 64;          /* queries per scan tile (2 MFMA column blocks) */
+constexpr int kRT = 32;          /* rows per wave tile */
+constexpr int kLdsCand = 32;     /* per-query LDS candidate slots in a scan WG */
+constexpr int kSelMax = 4096;    /* max keys sorted in one select WG */
+constexpr int kMaxProgDepth = 64;
+
+/* status bits per query */
+constexpr uint32_t kStOverflow = 1u;   /* candidate pool overflowed */
+constexpr uint32_t kStCertFail = 2u;   /* re-rank set exceeded capacity */
+
+/* predicate program op (postfix; see pv_pred.hip) */
+enum : int32_t { OP_CMP = 0, OP_IN = 1, OP_AND2 = 2, OP_OR2 = 3, OP_NOT = 4,
+                 OP_TRUE = 5, OP_FALSE = 6 };
+struct PvOp {
+    int32_t kind;
+    int32_t col;   /* CMP/IN column */
+    int32_t a;     /* CMP: FvCmpOp; IN: first value index */
+    int32_t b;     /* IN: number of values */
+    double value;  /* CMP constant */
+};
+static_assert(sizeof(PvOp) == 24, "PvOp layout");
+
+/* grow-only device buffer */
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t need) {
+        if (cap >= need && p) return hipSuccess;
+        if (p) hipFree(p);
+        p = nullptr;
+        cap = 0;
+        size_t nc = need + need / 4 + 256;
+        hipError_t e = hipMalloc(&p, nc);
+        if (e == hipSuccess) cap = nc;
+        return e;
+    }
+    void release() {
+        if (p) hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    template <typename T> T *as() const { return (T *)p; }
+};
+
+/* grow-only pinned host buffer */
+struct PinBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t need) {
+        if (cap >= need && p) return hipSuccess;
+        if (p) hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+        size_t nc = need + need / 4 + 256;
+        hipError_t e = hipHostMalloc(&p, nc, hipHostMallocDefault);
+        if (e == hipSuccess) cap = nc;
+        return e;
+    }
+    void release() {
+        if (p) hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    template <typename T> T *as() const { return (T *)p; }
+};
+
+/* compiled predicate programs of one batch (host side) */
+struct ProgSet {
+    std::vector<PvOp> ops;
+    std::vector<double> vals;
+    std::vector<int32_t> off, len;   /* per distinct program */
+    std::vector<int32_t> of_query;   /* per query: program index, -1 = none */
+};
+
+} // namespace pv
+
+/* A staged batch: inputs resident in HBM plus device outputs. */
+struct FvGpuBatch {
+    FvGpuIndex *owner = nullptr;
+    int nq = 0;
+    int nqt = 0;            /* query tiles of kQT */
+    uint32_t topb = 0;
+    int nprog = 0;          /* distinct predicate programs */
+    bool all_spans = false; /* every query has spans and no dedup */
+    bool any_dedup = false; /* some query's spans may repeat rows */
+    uint64_t sum_total = 0; /* sum of span enumeration sizes */
+    uint64_t max_total = 0;
+    bool query_exact = true;
+    std::vector<uint64_t> total;   /* host copy of qd[q].total */
+    std::vector<int32_t> nspans;
+
+    /* packed staging area: one H2D copy per stage */
+    pv::PinBuf h_stage;
+    pv::DevBuf d_stage;
+    float *d_q = nullptr;          /* [nqt*64][dim] fp32 (zero padded) */
+    FvGpuQueryDesc *d_qd = nullptr;
+    pv::PvOp *d_ops = nullptr;
+    double *d_vals = nullptr;
+    int32_t *d_poff = nullptr, *d_plen = nullptr, *d_pofq = nullptr;
+    FvGpuSpan *d_spans = nullptr;
+    uint64_t *d_spanoff = nullptr;
+
+    /* derived per-query data */
+    pv::DevBuf d_qbf;     /* [nqt*64][dim] bf16 */
+    pv::DevBuf d_qaux;    /* qnorm f32 [nq_pad], eps f32 [nq_pad], tau f32 [nq_pad] */
+    pv::DevBuf d_bitmap;  /* [nprog][words] u32 */
+    pv::DevBuf d_gmin;    /* prepass group minima [nq][ngroups] u64 */
+    pv::DevBuf d_pool;    /* candidate pool [nq][cap_pool] u64 */
+    pv::DevBuf d_pcnt;    /* [nq] u32 pool counts + [nq] u32 status */
+    uint64_t cap_pool = 0;
+    uint32_t ngroups = 0;
+    uint32_t stride = 0;
+
+    /* outputs */
+    pv::DevBuf d_out;     /* FvHit [nq][topb] */
+    pv::DevBuf d_cnt;     /* u32 [nq] */
+    pv::DevBuf d_flag;    /* u8 [nq] truncated / exhausted */
+    pv::PinBuf h_out;     /* fetch staging */
+    uint32_t out_stride = 0; /* hits per query in d_out */
+    int last_route = 0;
+};
+
+struct FvGpuIndex {
+    int device = -1;
+    int precision = FV_GPU_PREC_FP32;
+    uint64_t count = 0;
+    uint64_t count_pad = 0;
+    int dim = 0;
+    int ncols = 0;
+    uint32_t l0_stride = 0;
+    int nbr_stride = 0;
+    int ncu = 256;
+
+    float *d_vecs = nullptr;
+    uint16_t *d_vbf = nullptr;
+    float *d_norm = nullptr;
+    double *d_cols = nullptr;
+    uint32_t *d_prows = nullptr;
+    uint32_t *d_nbr = nullptr;
+
+    float xmax_norm = 0.f;   /* max ||x|| */
+    float vmax_abs = 0.f;    /* max |x_i| */
+    bool all_int = false;    /* every vector value is an integer */
+
+    hipStream_t stream = nullptr;
+    bool timing = false;
+    hipEvent_t ev[16] = {};
+    FvGpuStats last = {};
+
+    FvGpuBatch *sync_batch = nullptr; /* reused by the synchronous ABI */
+    pv::DevBuf d_visit;               /* graph overflow visited bitmaps */
+};
+
+namespace pv {
+
+/* pv_pred.hip */
+int compile_programs(int nq, const FvGpuQueryDesc *qd,
+                     const FvGpuPredNode *nodes, int nnodes,
+                     const int32_t *children, int nchildren,
+                     const double *invals, int ninvals, int ncols,
+                     ProgSet *ps, char *errbuf, size_t errlen);
+hipError_t launch_bitmap(FvGpuIndex *g, FvGpuBatch *b, hipStream_t s);
+
+/* pv_brute.hip */
+int stage_batch(FvGpuIndex *g, FvGpuBatch *b, int nq, const float *queries,
+                const FvGpuQueryDesc *qd, const uint8_t *dedup,
+                const FvGpuPredNode *nodes, int nnodes,
+                const int32_t *children, int nchildren, const double *invals,
+                int ninvals, const FvGpuSpan *spans, int nspans_total,
+                const uint64_t *spanoff, int nspanoff, uint32_t topb,
+                char *errbuf, size_t errlen);
+void free_batch_buffers(FvGpuBatch *b);
+
+/* pv_graph.hip */
+int run_graph(FvGpuIndex *g, FvGpuBatch *b, uint32_t ef, uint64_t max_visited,
+              char *errbuf, size_t errlen);
+
+/* timing helpers (pv_index.hip) */
+void ev_record(FvGpuIndex *g, int i);
+double ev_elapsed_us(FvGpuIndex *g, int a, int b);
+
+} // namespace pv
+
+#endif
