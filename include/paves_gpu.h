@@ -163,9 +163,9 @@ int fv_gpu_mem_probe(uint64_t *free_bytes, uint64_t *total_bytes);
 
 /* ================= extensions (no reference counterpart) ================= */
 
-/* Per-call options for the brute path. Zero-initialised = defaults. */
+/* Per-call options for the brute path (a NULL pointer = all defaults). */
 typedef struct FvGpuBruteOpts {
-    int32_t precision;   /* -1/0 = index default, else FV_GPU_PREC_* */
+    int32_t precision;   /* FV_GPU_PREC_* to scan with; -1 = index default */
     int32_t route;       /* 0 auto, 1 force full scan, 2 force span gather */
     uint32_t overfetch;  /* re-rank candidates per query (0 = auto) */
     uint32_t sample_stride; /* threshold pre-pass stride (0 = auto) */
@@ -181,7 +181,7 @@ typedef struct FvGpuStats {
     int32_t route;                    /* 1 scan, 2 gather, 3 graph */
     int32_t precision;
     uint32_t certified_fail;          /* queries re-run for certification */
-    uint32_t pad;
+    uint32_t nprog;                   /* distinct predicate bitmaps */
 } FvGpuStats;
 
 /* Enable (1) / disable (0) per-kernel event timing on an index. */

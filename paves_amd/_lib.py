@@ -47,8 +47,12 @@ EXT_SYMBOLS = ["fv_gpu_set_timing", "fv_gpu_last_stats", "fv_gpu_brute_batch_ex"
 
 
 class BruteOpts(C.Structure):
+    """FvGpuBruteOpts; precision -1 = the index's precision."""
     _fields_ = [("precision", C.c_int32), ("route", C.c_int32),
                 ("overfetch", C.c_uint32), ("sample_stride", C.c_uint32)]
+
+    def __init__(self, precision=-1, route=0, overfetch=0, sample_stride=0):
+        super().__init__(precision, route, overfetch, sample_stride)
 
 
 class Stats(C.Structure):
@@ -59,7 +63,7 @@ class Stats(C.Structure):
                 ("scan_rows", C.c_uint64), ("scan_bytes", C.c_uint64),
                 ("candidates", C.c_uint64), ("route", C.c_int32),
                 ("precision", C.c_int32), ("certified_fail", C.c_uint32),
-                ("pad", C.c_uint32)]
+                ("nprog", C.c_uint32)]
 
 
 _P = C.c_void_p

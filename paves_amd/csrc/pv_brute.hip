@@ -13,7 +13,7 @@
  *                       least B passing rows lie at or below it, so every
  *                       row of the exact top-B does too
  *   5. k_scan<MAIN>     full scan; rows with dist <= tau enter a per-query
- *                       candidate pool (LDS first, spill to HBM)
+ *                       candidate pool (LDS, flushed to HBM per WG)
  *   6. k_select         radix-select the B-th smallest pool key, collect the
  *                       keys within 2 eps of it, re-rank them with exact FP32
  *                       in the CPU cursor's summation order, bitonic-sort by
@@ -62,35 +62,48 @@ static inline QAux qaux_of(FvGpuBatch *b) {
 
 /* ---------------- 1. query preparation ---------------- */
 
-/* One wave per (padded) query. mode: 0 = FP32 scan bound, 1 = BF16 bound. */
+/*
+ * One wave per (padded) query. mode: 0 = FP32 scan bound, 1 = BF16 bound.
+ * The scan is exact (eps = 0) when every vector and query value is an
+ * integer of magnitude <= 256 (exact in BF16, products exact in FP32) and
+ * every partial sum stays below 2^24: sum (x-q)^2 <= D * delta^2 with
+ * delta = max(vmax - qmin, qmax - vmin); ||x||^2 + ||q||^2 <= D (V^2 + Q^2);
+ * |2 x.q| <= 2 D V Q (V, Q = largest magnitudes).
+ */
 __global__ __launch_bounds__(64) void k_prep_queries(
     const float *__restrict__ q, int nq, int dim, uint16_t *__restrict__ qbf,
     float *__restrict__ qnorm, float *__restrict__ eps, float xmax_norm,
-    float vmax_abs, int all_int, int mode) {
+    float vmin, float vmax, int all_int, int mode) {
     const int qi = blockIdx.x;
     const int l = threadIdx.x;
     const float *src = q + (uint64_t)qi * dim;
-    float ss = 0.f, amax = 0.f;
+    float ss = 0.f, pmax = 0.f, nmax = 0.f;
     int nonint = 0;
     for (int j = l; j < dim; j += 64) {
         float v = src[j];
         qbf[(uint64_t)qi * dim + j] = f2bf(v);
         ss = fmaf(v, v, ss);
-        amax = fmaxf(amax, fabsf(v));
+        pmax = fmaxf(pmax, v);
+        nmax = fmaxf(nmax, -v);
         nonint |= (v != rintf(v)) ? 1 : 0;
     }
     for (int o = 32; o; o >>= 1) {
         ss += __shfl_xor(ss, o);
-        amax = fmaxf(amax, __shfl_xor(amax, o));
+        pmax = fmaxf(pmax, __shfl_xor(pmax, o));
+        nmax = fmaxf(nmax, __shfl_xor(nmax, o));
         nonint |= __shfl_xor(nonint, o);
     }
     if (l == 0) {
         qnorm[qi] = ss;
         double e = 0.0;
         const double D = (double)dim;
-        const double s = (double)amax + (double)vmax_abs;
-        const bool exact = all_int && !nonint && amax <= 256.f &&
-                           vmax_abs <= 256.f && D * s * s < 16777216.0;
+        const double V = fmax((double)vmax, -(double)vmin);
+        const double Q = fmax((double)pmax, (double)nmax);
+        const double delta = fmax((double)vmax + (double)nmax, (double)pmax - (double)vmin);
+        const double lim = 16777216.0;
+        const bool exact = all_int && !nonint && V <= 256.0 && Q <= 256.0 &&
+                           D * delta * delta < lim && D * (V * V + Q * Q) < lim &&
+                           2.0 * D * V * Q < lim;
         if (!exact && qi < nq) {
             const double qn = sqrt((double)ss), xn = (double)xmax_norm;
             const double g = (D + 8.0) * ldexp(1.0, -24);
@@ -107,8 +120,16 @@ __global__ __launch_bounds__(64) void k_prep_queries(
     }
 }
 
-/* ---------------- shared scan epilogue pieces ---------------- */
+/* ---------------- shared scan pieces ---------------- */
 
+/*
+ * BF16 copy layout ("MFMA-tiled"): rows are grouped in tiles of 32; a tile
+ * is dim/16 chunks of 1 KB, chunk s holding in 16-byte slot l the 8 bf16
+ * that lane l feeds to k-step s of v_mfma_f32_32x32x16_bf16: row (l & 31),
+ * dims (l >> 5) * dim/2 + 8 s .. +8. A wave therefore loads each k-step with
+ * one fully coalesced 1 KB dwordx4 access and uses it as the A operand with
+ * no shuffles. Written once at upload (pv_index.hip, k_derive_rows).
+ */
 struct ScanArgs {
     const uint16_t *vbf;
     const float *vecs;
@@ -123,103 +144,172 @@ struct ScanArgs {
     const uint32_t *bitmap;
     uint64_t words;
     int nq;
-    uint32_t ntiles;      /* 32-row tiles in the row domain */
-    uint32_t stride;      /* sample stride (PREPASS) */
-    uint64_t nsample;     /* number of sampled rows (PREPASS) */
+    uint32_t ntiles;      /* 32-row tiles visited by this launch */
+    uint32_t stride;      /* tile t of the launch is physical tile t * stride */
     uint64_t *pool;
     uint32_t *pcnt;
     uint64_t cap_pool;
     uint64_t *gmin;
     uint32_t ngroups;
+    const uint32_t *ones;     /* all-ones bitmap row (queries without predicate) */
+    uint32_t *status;         /* per-query status bits */
 };
+
+constexpr uint32_t kSpill = 512; /* per-WG LDS candidate overflow slots */
+/* pool counters sit one 128-byte line apart: every scan WG reserves pool
+ * space with an atomic per query, and counters sharing a line would
+ * serialise all of them in one L2 channel */
+constexpr uint32_t kPcntStride = 32;
+
+/* LDS candidate state of one scan WG */
+struct CandLds {
+    uint32_t cnt[kQT];
+    uint32_t base[kQT];
+    uint32_t nsp, ovf;
+    uint64_t buf[kQT * kLdsCand];
+    uint64_t spk[kSpill];
+    uint32_t spq[kSpill];
+};
+
+/* bitmap row of a query (the all-ones row when it has no predicate) */
+__device__ __forceinline__ const uint32_t *bitmap_row(const ScanArgs &A, int32_t p) {
+    return p >= 0 ? A.bitmap + (uint64_t)p * A.words : A.ones;
+}
+
+/*
+ * Append a candidate: per-query LDS slots first, then a WG-wide LDS spill
+ * area. The scan loop touches only LDS here (no global atomics or stores,
+ * whose completion would gate the next tile's prefetched loads). Spills
+ * past the area set the WG's overflow flag (the host then re-runs with the
+ * tightest threshold).
+ */
+__device__ __forceinline__ void insert_candidate(int qi, uint64_t key, CandLds &C) {
+    const uint32_t pos = atomicAdd(&C.cnt[qi], 1u);
+    if (pos < (uint32_t)kLdsCand) {
+        C.buf[qi * kLdsCand + pos] = key;
+    } else {
+        const uint32_t sp = atomicAdd(&C.nsp, 1u);
+        if (sp < kSpill) {
+            C.spk[sp] = key;
+            C.spq[sp] = (uint32_t)qi;
+        } else {
+            C.ovf = 1u;
+        }
+    }
+}
+
+__device__ __forceinline__ void cand_init(CandLds &C) {
+    for (int i = threadIdx.x; i < kQT; i += blockDim.x) C.cnt[i] = 0;
+    if (threadIdx.x == 0) {
+        C.nsp = 0;
+        C.ovf = 0;
+    }
+}
 
 /* row held in accumulator element i by lane l (32x32 MFMA C layout) */
 __device__ __forceinline__ int acc_row(int i, int l) {
     return (i >> 2) * 8 + (l >> 5) * 4 + (i & 3);
 }
 
+/* per-tile epilogue inputs, loaded together with the tile's rows */
+struct TileIn {
+    float xn[16];      /* ||x||^2 of the lane's 16 rows */
+    uint32_t w[2];     /* predicate bitmap word of the tile, per query block */
+    uint32_t valid;    /* rows < count */
+};
+
+__device__ __forceinline__ void load_tile_in(const ScanArgs &A, uint32_t pt,
+                                             const uint32_t *const (&wr)[2], TileIn &in) {
+    const int l = threadIdx.x & 63;
+#pragma unroll
+    for (int g4 = 0; g4 < 4; g4++) {
+        const float4 v = *(const float4 *)(A.xnorm + (uint64_t)pt * 32 + g4 * 8 + (l >> 5) * 4);
+        in.xn[g4 * 4 + 0] = v.x;
+        in.xn[g4 * 4 + 1] = v.y;
+        in.xn[g4 * 4 + 2] = v.z;
+        in.xn[g4 * 4 + 3] = v.w;
+    }
+    in.w[0] = wr[0][pt];
+    in.w[1] = wr[1][pt];
+    const uint64_t row0 = (uint64_t)pt * 32;
+    in.valid = row0 + 32 <= A.count ? 0xFFFFFFFFu
+               : (A.count > row0 ? (0xFFFFFFFFu >> (32 - (A.count - row0))) : 0u);
+}
+
 /*
- * Consume one 32-row x 64-query tile of distances d[qb][i] held in the
- * MFMA C layout: lane l owns queries (l & 31) + 32*qb and the 16 rows
- * acc_row(i, l). PREPASS writes the minimum passing distance of the lane's
- * 16 rows per query (one group); MAIN appends passing rows with
- * dist <= tau to the candidate pool.
+ * Consume one 32-row x 64-query tile held in the MFMA C layout: lane l owns
+ * queries (l & 31) + 32 qb and the 16 rows acc_row(i, l). BF: distances are
+ * ||x||^2 + ||q||^2 - 2 x.q (clamped at 0) from dot products; otherwise the
+ * accumulators already hold sum (x - q)^2. PREPASS writes the minimum
+ * passing distance of the lane's 16 rows per query (one group); MAIN
+ * appends passing rows with dist <= tau to the WG's candidates.
  */
-template <bool PREPASS>
-__device__ __forceinline__ void scan_epilogue(const ScanArgs &A, int qbase,
-                                              uint32_t t, float (&d)[2][16],
-                                              const float (&tq)[2],
-                                              const int32_t (&pq)[2],
-                                              uint32_t *lcnt, uint64_t *lbuf) {
+template <bool PREPASS, bool BF>
+__device__ __forceinline__ void tile_epilogue(const ScanArgs &A, int qbase, uint32_t t,
+                                              uint32_t pt, const float (&acc)[2][16],
+                                              const TileIn &in, const float (&qn)[2],
+                                              const float (&tq)[2], bool active,
+                                              CandLds &C) {
     const int l = threadIdx.x & 63;
 #pragma unroll
     for (int qb = 0; qb < 2; qb++) {
         const int qi = (l & 31) + 32 * qb;
         const int q = qbase + qi;
-        if (q >= A.nq) continue;
+        if (q >= A.nq || !active) continue;
+        const uint32_t word = in.w[qb] & in.valid;
         if (PREPASS) {
             float m = __uint_as_float(0x7f800000u);
 #pragma unroll
             for (int i = 0; i < 16; i++) {
-                const uint64_t sidx = (uint64_t)t * 32 + acc_row(i, l);
-                if (sidx >= A.nsample) continue;
-                const uint64_t row = sidx * A.stride;
-                bool pass = true;
-                if (pq[qb] >= 0)
-                    pass = (A.bitmap[(uint64_t)pq[qb] * A.words + (row >> 5)] >>
-                            (row & 31)) & 1u;
-                if (pass) m = fminf(m, d[qb][i]);
+                const float d = BF ? fmaxf(in.xn[i] + qn[qb] - 2.f * acc[qb][i], 0.f) : acc[qb][i];
+                if ((word >> acc_row(i, l)) & 1u) m = fminf(m, d);
             }
             const uint32_t grp = t * 2 + (l >> 5);
             A.gmin[(uint64_t)q * A.ngroups + grp] = make_key(m, grp);
         } else {
-            uint32_t word = 0xFFFFFFFFu;
-            if (pq[qb] >= 0) word = A.bitmap[(uint64_t)pq[qb] * A.words + t];
             const float tau = tq[qb];
 #pragma unroll
             for (int i = 0; i < 16; i++) {
+                const float d = BF ? fmaxf(in.xn[i] + qn[qb] - 2.f * acc[qb][i], 0.f) : acc[qb][i];
                 const int r = acc_row(i, l);
-                const uint64_t row = (uint64_t)t * 32 + r;
-                const bool ok = ((word >> r) & 1u) && row < A.count &&
-                                d[qb][i] <= tau;
-                if (ok) {
-                    const uint64_t key = make_key(d[qb][i], (uint32_t)row);
-                    const uint32_t pos = atomicAdd(&lcnt[qi], 1u);
-                    if (pos < (uint32_t)kLdsCand) {
-                        lbuf[qi * kLdsCand + pos] = key;
-                    } else {
-                        const uint32_t gp = atomicAdd(&A.pcnt[q], 1u);
-                        if (gp < A.cap_pool) A.pool[(uint64_t)q * A.cap_pool + gp] = key;
-                    }
-                }
+                if (((word >> r) & 1u) && d <= tau)
+                    insert_candidate(qi, make_key(d, (uint32_t)((uint64_t)pt * 32 + r)), C);
             }
         }
     }
 }
 
-/* Flush the WG's LDS candidates to the pool (MAIN). */
+/* Flush the WG's LDS candidates and spills to the pool (MAIN). */
 __device__ __forceinline__ void flush_candidates(const ScanArgs &A, int qbase,
-                                                 uint32_t *lcnt, uint64_t *lbuf) {
+                                                 CandLds &C) {
     __syncthreads();
-    const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
-    const int nw = blockDim.x >> 6;
-    for (int qi = wave; qi < kQT; qi += nw) {
-        const int q = qbase + qi;
-        if (q >= A.nq) continue;
-        const uint32_t n = min(lcnt[qi], (uint32_t)kLdsCand);
-        if (n == 0) continue;
-        uint32_t base = 0;
-        if (l == 0) base = atomicAdd(&A.pcnt[q], n);
-        base = __shfl(base, 0);
-        if ((uint32_t)l < n && base + l < A.cap_pool)
-            A.pool[(uint64_t)q * A.cap_pool + base + l] = lbuf[qi * kLdsCand + l];
+    /* one reservation per query, all in flight together */
+    if (threadIdx.x < kQT) {
+        const int qi = threadIdx.x, q = qbase + qi;
+        const uint32_t n = min(C.cnt[qi], (uint32_t)kLdsCand);
+        C.base[qi] = (q < A.nq && n > 0) ? atomicAdd(&A.pcnt[(uint64_t)q * kPcntStride], n) : 0u;
     }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < kQT * kLdsCand; idx += blockDim.x) {
+        const int qi = idx / kLdsCand, j = idx % kLdsCand, q = qbase + qi;
+        const uint32_t n = min(C.cnt[qi], (uint32_t)kLdsCand);
+        if (q < A.nq && (uint32_t)j < n && C.base[qi] + j < A.cap_pool)
+            A.pool[(uint64_t)q * A.cap_pool + C.base[qi] + j] = C.buf[idx];
+    }
+    const uint32_t ns = min(C.nsp, kSpill);
+    for (uint32_t i = threadIdx.x; i < ns; i += blockDim.x) {
+        const uint32_t q = (uint32_t)qbase + C.spq[i];
+        const uint32_t gp = atomicAdd(&A.pcnt[(uint64_t)q * kPcntStride], 1u);
+        if (gp < A.cap_pool) A.pool[(uint64_t)q * A.cap_pool + gp] = C.spk[i];
+    }
+    if (C.ovf && threadIdx.x < kQT && qbase + (int)threadIdx.x < A.nq)
+        atomicOr(&A.status[qbase + threadIdx.x], kStOverflow);
 }
 
 /* per-lane query state for the two column blocks */
 __device__ __forceinline__ void load_query_state(const ScanArgs &A, int qbase,
                                                  float (&tq)[2], float (&qn)[2],
-                                                 int32_t (&pq)[2]) {
+                                                 const uint32_t *(&wr)[2]) {
     const int l = threadIdx.x & 63;
 #pragma unroll
     for (int qb = 0; qb < 2; qb++) {
@@ -227,154 +317,132 @@ __device__ __forceinline__ void load_query_state(const ScanArgs &A, int qbase,
         const bool v = q < A.nq;
         tq[qb] = v ? A.tau[q] : 0.f;
         qn[qb] = v ? A.qnorm[q] : 0.f;
-        pq[qb] = v ? A.pofq[q] : -1;
+        wr[qb] = bitmap_row(A, v ? A.pofq[q] : -1);
     }
 }
 
 /* ---------------- 3/5. BF16 MFMA scan ---------------- */
 
 /*
- * grid.x: row chunks, grid.y: query tiles of 64. 256 threads = 4 waves;
- * wave w processes the chunk's 32-row tiles w, w+4, ... One tile is
- * 2 x (dim/16) v_mfma_f32_32x32x16_bf16: A = 32 rows (lane l: row l&31,
- * dims (l>>5)*dim/2 + 8s .. +8, i.e. 128 contiguous bytes per 64-dim chunk),
- * B = 32 queries from LDS in the same k order. The next tile's first chunk
- * is loaded before the current tile's MFMAs (software pipeline).
+ * grid.x: row chunks, grid.y: query tiles of 64; NW waves per WG. A wave
+ * walks its tiles (t0 + wave + NW i) as a stream of "units" of 8 k-steps
+ * (8 KB of the MFMA-tiled copy; dim/128 units per tile). Units alternate
+ * between two register buffers: the loads of unit u+1 — its rows, and the
+ * tile's ||x||^2 and bitmap words — are issued before the 16
+ * v_mfma_f32_32x32x16_bf16 of unit u, and after the last unit of a tile its
+ * epilogue runs while the next tile streams in. Loads are unconditional
+ * (indices clamped to the last unit) so the pipeline is one straight line
+ * for the wait-count analysis. B operands (the 64 queries) sit in LDS.
  * Requires dim % 128 == 0.
  */
-template <bool PREPASS>
-__global__ __launch_bounds__(256, 2) void k_scan_bf16(ScanArgs A) {
+template <bool PREPASS, int NW>
+__global__ __launch_bounds__(NW * 64) void k_scan_bf16(ScanArgs A) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     const int S = A.dim / 16;            /* MFMA k-steps per tile */
-    const int NCH = A.dim / 128;         /* 8-step chunks */
+    const uint32_t NCH = (uint32_t)(S / 8); /* units per tile */
     u32x4 *qsh = (u32x4 *)smem;          /* [2][S][64] x 16 B */
-    uint32_t *lcnt = (uint32_t *)(smem + (size_t)2 * S * 64 * 16);
-    uint64_t *lbuf = (uint64_t *)(lcnt + kQT);
+    CandLds &C = *(CandLds *)(smem + (size_t)2 * S * 64 * 16);
 
     const int tid = threadIdx.x, l = tid & 63, wave = tid >> 6;
     const int qbase = blockIdx.y * kQT;
     const int half = A.dim / 2;
-
-    /* stage B operands: qsh[(qb*S + s)*64 + lane] = 8 bf16 of query
-     * qbase + (lane&31) + 32 qb, dims (lane>>5)*half + 8 s */
     for (int e = tid; e < 2 * S * 64; e += blockDim.x) {
         const int lane = e & 63, s = (e >> 6) % S, qb = (e >> 6) / S;
         const int q = qbase + (lane & 31) + 32 * qb;
-        const u32x4 *src = (const u32x4 *)(A.qbf + (uint64_t)q * A.dim +
-                                           (lane >> 5) * half + 8 * s);
-        qsh[e] = *src; /* query buffer is padded to the tile */
+        qsh[e] = *(const u32x4 *)(A.qbf + (uint64_t)q * A.dim + (lane >> 5) * half + 8 * s);
     }
-    if (!PREPASS)
-        for (int i = tid; i < kQT; i += blockDim.x) lcnt[i] = 0;
+    if (!PREPASS) cand_init(C);
     __syncthreads();
 
     float tq[2], qn[2];
-    int32_t pq[2];
-    load_query_state(A, qbase, tq, qn, pq);
+    const uint32_t *wr[2];
+    load_query_state(A, qbase, tq, qn, wr);
 
     const uint32_t nwg = gridDim.x;
     const uint32_t t0 = (uint32_t)(((uint64_t)A.ntiles * blockIdx.x) / nwg);
     const uint32_t t1 = (uint32_t)(((uint64_t)A.ntiles * (blockIdx.x + 1)) / nwg);
+    const uint32_t first = t0 + wave;
+    const uint32_t nt = first < t1 ? (t1 - first + NW - 1) / NW : 0;
+    const uint32_t nu = nt * NCH;
+    const u32x4 *vb = (const u32x4 *)A.vbf;
 
-    auto row_ptr = [&](uint32_t t) -> const u32x4 * {
-        uint64_t row;
-        if (PREPASS) {
-            uint64_t sidx = (uint64_t)t * 32 + (l & 31);
-            row = sidx < A.nsample ? sidx * A.stride : 0;
-        } else {
-            row = (uint64_t)t * 32 + (l & 31);
-        }
-        return (const u32x4 *)(A.vbf + row * A.dim + (l >> 5) * half);
+    auto tile_of_unit = [&](uint32_t u) { return min(first + NW * (u / NCH), t1 - 1); };
+    auto load_unit = [&](uint32_t u, u32x4 (&a)[8], TileIn &in) {
+        const uint32_t uc = min(u, nu - 1);
+        const uint32_t pt = tile_of_unit(uc) * A.stride;
+        const u32x4 *p = vb + (uint64_t)pt * (4 * A.dim) + (uint64_t)(uc % NCH) * 512 + l;
+#pragma unroll
+        for (int j = 0; j < 8; j++) a[j] = __builtin_nontemporal_load(p + 64 * j);
+        load_tile_in(A, pt, wr, in);
     };
-
-    u32x4 a[8], an[8];
-    uint32_t t = t0 + wave;
-    if (t < t1) {
-        const u32x4 *p = row_ptr(t);
-#pragma unroll
-        for (int j = 0; j < 8; j++) a[j] = p[j];
-    }
-    for (; t < t1; t += 4) {
-        f32x16 acc0 = {}, acc1 = {};
-        const u32x4 *p = row_ptr(t);
-        for (int c = 0; c < NCH; c++) {
-            /* prefetch: next chunk of this tile, or first chunk of the next */
-            const bool more = (c + 1 < NCH) || (t + 4 < t1);
-            if (more) {
-                const u32x4 *pn = (c + 1 < NCH) ? p + 8 * (c + 1) : row_ptr(t + 4);
-#pragma unroll
-                for (int j = 0; j < 8; j++) an[j] = pn[j];
-            }
-#pragma unroll
-            for (int j = 0; j < 8; j++) {
-                const int s = c * 8 + j;
-                const bf16x8 av = __builtin_bit_cast(bf16x8, a[j]);
-                const bf16x8 b0 = __builtin_bit_cast(bf16x8, qsh[(0 * S + s) * 64 + l]);
-                const bf16x8 b1 = __builtin_bit_cast(bf16x8, qsh[(1 * S + s) * 64 + l]);
-                acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, b0, acc0, 0, 0, 0);
-                acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, b1, acc1, 0, 0, 0);
-            }
-            if (more) {
-#pragma unroll
-                for (int j = 0; j < 8; j++) a[j] = an[j];
-            }
+    auto do_unit = [&](uint32_t u, const u32x4 (&a)[8], const TileIn &in, f32x16 &acc0,
+                       f32x16 &acc1, bool active) {
+        const uint32_t c = u % NCH;
+        if (c == 0) {
+            acc0 = f32x16{};
+            acc1 = f32x16{};
         }
-        /* epilogue: ||x||^2 + ||q||^2 - 2 x.q, clamped at 0 */
-        float d[2][16];
-        float xn[16];
-        if (PREPASS) {
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int s = (int)c * 8 + j;
+            const bf16x8 av = __builtin_bit_cast(bf16x8, a[j]);
+            const bf16x8 b0 = __builtin_bit_cast(bf16x8, qsh[(0 * S + s) * 64 + l]);
+            const bf16x8 b1 = __builtin_bit_cast(bf16x8, qsh[(1 * S + s) * 64 + l]);
+            acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, b0, acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, b1, acc1, 0, 0, 0);
+        }
+        if (c == NCH - 1) {
+            float acc[2][16];
 #pragma unroll
             for (int i = 0; i < 16; i++) {
-                const uint64_t sidx = (uint64_t)t * 32 + acc_row(i, l);
-                xn[i] = sidx < A.nsample ? A.xnorm[sidx * A.stride] : 0.f;
+                acc[0][i] = acc0[i];
+                acc[1][i] = acc1[i];
             }
-        } else {
-#pragma unroll
-            for (int g4 = 0; g4 < 4; g4++) {
-                const float4 v = *(const float4 *)(A.xnorm + (uint64_t)t * 32 +
-                                                   g4 * 8 + (l >> 5) * 4);
-                xn[g4 * 4 + 0] = v.x;
-                xn[g4 * 4 + 1] = v.y;
-                xn[g4 * 4 + 2] = v.z;
-                xn[g4 * 4 + 3] = v.w;
-            }
+            const uint32_t t = tile_of_unit(u);
+            tile_epilogue<PREPASS, true>(A, qbase, t, t * A.stride, acc, in, qn, tq, active, C);
         }
-#pragma unroll
-        for (int i = 0; i < 16; i++) {
-            d[0][i] = fmaxf(xn[i] + qn[0] - 2.f * acc0[i], 0.f);
-            d[1][i] = fmaxf(xn[i] + qn[1] - 2.f * acc1[i], 0.f);
+    };
+
+    if (nu > 0) {
+        u32x4 a0[8], a1[8];
+        TileIn in0, in1;
+        f32x16 acc0 = {}, acc1 = {};
+        load_unit(0, a0, in0);
+        for (uint32_t u = 0; u < nu; u += 2) {
+            load_unit(u + 1, a1, in1);
+            do_unit(u, a0, in0, acc0, acc1, true);
+            load_unit(u + 2, a0, in0);
+            do_unit(u + 1, a1, in1, acc0, acc1, u + 1 < nu);
         }
-        scan_epilogue<PREPASS>(A, qbase, t, d, tq, pq, lcnt, lbuf);
     }
-    if (!PREPASS) flush_candidates(A, qbase, lcnt, lbuf);
+    if (!PREPASS) flush_candidates(A, qbase, C);
 }
 
 /* ---------------- 3/5. plain FP32 VALU scan ---------------- */
 
 /*
  * The plain FP32 path: sum over dims of (x - q)^2 in FP32 (the arithmetic
- * of the reference tiled kernel, gpu_engine.cu:534-566), producing the same
- * 32-row x 64-query C layout as the MFMA path so the epilogue is shared.
- * Per 64-dim chunk the WG stages the query chunk transposed (qT[64][64])
- * and each wave its 32 rows transposed (xT[64][32+4]); all waves iterate in
- * lockstep (barriers), idle waves compute on a masked tile.
+ * of the reference tiled kernel, gpu_engine.cu:534-566) on the row-major
+ * fp32 copy, producing the same 32-row x 64-query C layout as the MFMA path
+ * so the epilogue is shared. Per 64-dim chunk the WG stages the query chunk
+ * transposed (qT[64][64]) and each wave its 32 rows transposed
+ * (xT[64][32+4]); all waves iterate in lockstep (barriers), idle waves
+ * compute on a masked tile.
  */
 constexpr int kXP = 36; /* padded row stride of xT */
 template <bool PREPASS>
 __global__ __launch_bounds__(256, 2) void k_scan_f32(ScanArgs A) {
     __shared__ float qT[64 * 64];
     __shared__ float xT[4][64 * kXP];
-    __shared__ uint32_t lcnt[kQT];
-    __shared__ uint64_t lbuf[kQT * kLdsCand];
+    __shared__ CandLds C;
 
     const int tid = threadIdx.x, l = tid & 63, wave = tid >> 6;
     const int qbase = blockIdx.y * kQT;
-    if (!PREPASS)
-        for (int i = tid; i < kQT; i += blockDim.x) lcnt[i] = 0;
+    if (!PREPASS) cand_init(C);
 
     float tq[2], qn[2];
-    int32_t pq[2];
-    load_query_state(A, qbase, tq, qn, pq);
+    const uint32_t *wr[2];
+    load_query_state(A, qbase, tq, qn, wr);
 
     const uint32_t nwg = gridDim.x;
     const uint32_t t0 = (uint32_t)(((uint64_t)A.ntiles * blockIdx.x) / nwg);
@@ -385,6 +453,7 @@ __global__ __launch_bounds__(256, 2) void k_scan_f32(ScanArgs A) {
     for (uint32_t it = 0; it < iters; it++) {
         const uint32_t t = t0 + it * 4 + wave;
         const bool live = t < t1;
+        const uint32_t pt = (live ? t : t0) * A.stride;
         float acc[2][16];
 #pragma unroll
         for (int qb = 0; qb < 2; qb++)
@@ -402,16 +471,7 @@ __global__ __launch_bounds__(256, 2) void k_scan_f32(ScanArgs A) {
             for (int e = l; e < 32 * 64; e += 64) {
                 const int r = e >> 6, j = e & 63;
                 float v = 0.f;
-                if (live && j < nd) {
-                    uint64_t row;
-                    if (PREPASS) {
-                        const uint64_t sidx = (uint64_t)t * 32 + r;
-                        row = sidx < A.nsample ? sidx * A.stride : 0;
-                    } else {
-                        row = (uint64_t)t * 32 + r;
-                    }
-                    v = A.vecs[row * A.dim + j0 + j];
-                }
+                if (live && j < nd) v = A.vecs[((uint64_t)pt * 32 + r) * A.dim + j0 + j];
                 xw[j * kXP + r] = v;
             }
             __syncthreads();
@@ -431,9 +491,13 @@ __global__ __launch_bounds__(256, 2) void k_scan_f32(ScanArgs A) {
                 }
             }
         }
-        if (live) scan_epilogue<PREPASS>(A, qbase, t, acc, tq, pq, lcnt, lbuf);
+        if (live) {
+            TileIn in;
+            load_tile_in(A, pt, wr, in);
+            tile_epilogue<PREPASS, false>(A, qbase, t, pt, acc, in, qn, tq, true, C);
+        }
     }
-    if (!PREPASS) flush_candidates(A, qbase, lcnt, lbuf);
+    if (!PREPASS) flush_candidates(A, qbase, C);
 }
 
 /* ---------------- block-level helpers ---------------- */
@@ -551,7 +615,7 @@ __global__ __launch_bounds__(256) void k_tau(const uint64_t *__restrict__ gmin,
     }
     if (threadIdx.x == 0) {
         tau[q] = tv;
-        pcnt[q] = 0;
+        pcnt[(uint64_t)q * kPcntStride] = 0;
         status[q] = 0;
     }
 }
@@ -561,7 +625,7 @@ __global__ void k_reset(int nq, float *tau, uint32_t *pcnt, uint32_t *status) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q < nq) {
         tau[q] = __uint_as_float(0x7f800000u);
-        pcnt[q] = 0;
+        pcnt[(uint64_t)q * kPcntStride] = 0;
         status[q] = 0;
     }
 }
@@ -615,7 +679,7 @@ __global__ __launch_bounds__(256) void k_gather(
         if (pass && row < count)
             d = cpu_order_l2(vecs + (uint64_t)row * dim, qv, dim, l16);
         if (pass && row < count && l16 == 0) {
-            const uint32_t gp = atomicAdd(&pcnt[q], 1u);
+            const uint32_t gp = atomicAdd(&pcnt[(uint64_t)q * kPcntStride], 1u);
             if (gp < cap_pool) pool[(uint64_t)q * cap_pool + gp] = make_key(d, row);
         }
     }
@@ -644,7 +708,7 @@ __global__ __launch_bounds__(256) void k_select(
     extern __shared__ float qsh[];
     const int q = blockIdx.x, t = threadIdx.x;
     const uint64_t *pk = pool + (uint64_t)q * cap_pool;
-    const uint32_t ntot = pcnt[q];
+    const uint32_t ntot = pcnt[(uint64_t)q * kPcntStride];
     const uint32_t n = (uint32_t)min((uint64_t)ntot, cap_pool);
     uint32_t st = 0;
     if ((uint64_t)ntot > cap_pool) st |= kStOverflow;
@@ -773,7 +837,7 @@ int stage_batch(FvGpuIndex *g, FvGpuBatch *b, int nq, const float *queries,
             if (spanoff[d.soff0] != 0 || spanoff[d.soff0 + d.nspans] != d.total)
                 PV_FAIL("gpu: query %d: span total mismatch", q);
         }
-        if (d.nspans == 0 || (dedup && dedup[q])) b->all_spans = false;
+        if (d.nspans == 0 || (dedup && dedup[q]) || g->d_prows == nullptr) b->all_spans = false;
         if (dedup && dedup[q]) b->any_dedup = true;
         b->sum_total += b->total[q];
         b->max_total = std::max(b->max_total, b->total[q]);
@@ -834,17 +898,15 @@ int stage_batch(FvGpuIndex *g, FvGpuBatch *b, int nq, const float *queries,
     PV_TRY(b->d_out.ensure((size_t)nq * topb * sizeof(FvHit)));
     PV_TRY(b->d_cnt.ensure((size_t)nq * 4));
     PV_TRY(b->d_flag.ensure((size_t)nq));
-    PV_TRY(b->d_pcnt.ensure((size_t)nqp * 4));
-    /* the predicate bitmaps depend only on the staged inputs */
-    PV_TRY(launch_bitmap(g, b, g->stream));
+    PV_TRY(b->d_pcnt.ensure((size_t)nqp * kPcntStride * 4));
     return 0;
 }
 
 /* scan precision actually used for a request */
 static int scan_precision(const FvGpuIndex *g, const FvGpuBruteOpts *o) {
-    int p = (o && o->precision > 0) ? o->precision : g->precision;
+    int p = (o && o->precision >= 0) ? o->precision : g->precision;
     if (p == FV_GPU_PREC_SQ8) p = FV_GPU_PREC_BF16;
-    if (p == FV_GPU_PREC_BF16 && (g->d_vbf == nullptr || g->dim % 128 != 0))
+    if (p == FV_GPU_PREC_BF16 && (g->d_vbf == nullptr || g->dim % 128 != 0 || g->dim > 1024))
         p = FV_GPU_PREC_FP32;
     return p;
 }
@@ -861,6 +923,8 @@ static int run_brute(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
     FvGpuStats &st = g->last;
     memset(&st, 0, sizeof(st));
     st.precision = prec;
+    st.nprog = (uint32_t)b->nprog;
+    g->last_batch = b;
 
     int route = o ? o->route : 0;
     if (route == 0) {
@@ -875,15 +939,22 @@ static int run_brute(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
     st.route = route;
 
     ev_record(g, 0);
+    /* predicate programs -> row bitmaps, every run (the gather route
+     * evaluates the predicate on the enumerated rows instead) */
+    if (route == 1) PV_TRY(launch_bitmap(g, b, s));
+    ev_record(g, 6);
     hipLaunchKernelGGL(k_prep_queries, dim3(nqp), dim3(64), 0, s, b->d_q, nq, dim,
                        b->d_qbf.as<uint16_t>(), qa.qnorm, qa.eps, g->xmax_norm,
-                       g->vmax_abs, g->all_int ? 1 : 0,
+                       g->vmin, g->vmax, g->all_int ? 1 : 0,
                        prec == FV_GPU_PREC_BF16 ? 1 : 0);
     PV_TRY(hipGetLastError());
     ev_record(g, 1);
 
     if (route == 2) {
         if (b->any_dedup) PV_FAIL("gpu: gather route cannot deduplicate spans");
+        if (g->d_prows == nullptr)
+            for (int q = 0; q < nq; q++)
+                if (b->nspans[q] > 0) PV_FAIL("gpu: gather route needs posting rows");
         if (b->max_total > (uint64_t)(1u << 22))
             PV_FAIL("gpu: gather route: enumeration of %llu rows too large",
                     (unsigned long long)b->max_total);
@@ -915,15 +986,17 @@ static int run_brute(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
     }
 
     /* ---- full-scan route ---- */
+    /* pre-pass samples every stride-th 32-row tile */
+    const uint32_t ntiles = (uint32_t)(g->count_pad / 32);
     uint32_t stride = stride_override ? stride_override
                                       : (o && o->sample_stride ? o->sample_stride : 0);
     if (stride == 0) {
         stride = B <= 64 ? 64 : std::max<uint32_t>(1, 4096 / B);
-        const uint64_t min_stride = (g->count + 262143) / 262144;
-        stride = (uint32_t)std::max<uint64_t>(stride, min_stride);
+        const uint32_t min_stride = (ntiles + 131071) / 131072;   /* <= 2^18 groups */
+        stride = std::max(stride, min_stride);
     }
-    const uint64_t nsample = (g->count + stride - 1) / stride;
-    const uint32_t stiles = (uint32_t)((nsample + 31) / 32);
+    stride = std::min(stride, ntiles);
+    const uint32_t stiles = (ntiles + stride - 1) / stride;
     b->stride = stride;
     b->ngroups = stiles * 2;
     b->cap_pool = std::max<uint64_t>(16384, (uint64_t)8 * B * stride);
@@ -951,20 +1024,36 @@ static int run_brute(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
     A.cap_pool = b->cap_pool;
     A.gmin = b->d_gmin.as<uint64_t>();
     A.ngroups = b->ngroups;
+    A.ones = g->d_ones;
+    A.status = qa.status;
 
     const bool bf = prec == FV_GPU_PREC_BF16;
-    const size_t shm_bf = (size_t)2 * (dim / 16) * 64 * 16 + kQT * 4 + kQT * kLdsCand * 8;
-    /* pre-pass over sampled rows */
+    /* 4 waves per WG up to 256 dims (2 WGs per CU), 8 waves above (the
+     * query operands then take most of the LDS) */
+    const int nw = dim <= 256 ? 4 : 8;
+    const size_t shm_bf = (size_t)2 * (dim / 16) * 64 * 16 + sizeof(CandLds);
+    if (bf && shm_bf > 64 * 1024) {
+        PV_TRY(hipFuncSetAttribute((const void *)k_scan_bf16<true, 8>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm_bf));
+        PV_TRY(hipFuncSetAttribute((const void *)k_scan_bf16<false, 8>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm_bf));
+    }
+    if (bf && shm_bf > 160 * 1024) PV_FAIL("gpu: dim %d too large for the BF16 scan", dim);
+    /* pre-pass over sampled tiles */
     {
         ScanArgs P = A;
         P.ntiles = stiles;
         P.stride = stride;
-        P.nsample = nsample;
-        const uint32_t gx = std::max<uint32_t>(1, (stiles + 3) / 4);
-        if (bf)
-            hipLaunchKernelGGL(k_scan_bf16<true>, dim3(gx, b->nqt), dim3(256), shm_bf, s, P);
-        else
+        if (bf) {
+            const uint32_t gx = std::max<uint32_t>(1, (stiles + nw - 1) / nw);
+            if (nw == 4)
+                hipLaunchKernelGGL((k_scan_bf16<true, 4>), dim3(gx, b->nqt), dim3(256), shm_bf, s, P);
+            else
+                hipLaunchKernelGGL((k_scan_bf16<true, 8>), dim3(gx, b->nqt), dim3(512), shm_bf, s, P);
+        } else {
+            const uint32_t gx = std::max<uint32_t>(1, (stiles + 3) / 4);
             hipLaunchKernelGGL(k_scan_f32<true>, dim3(gx, b->nqt), dim3(256), 0, s, P);
+        }
         PV_TRY(hipGetLastError());
     }
     ev_record(g, 2);
@@ -975,13 +1064,15 @@ static int run_brute(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
     ev_record(g, 3);
     {
         ScanArgs M = A;
-        M.ntiles = (uint32_t)(g->count_pad / 32);
+        M.ntiles = ntiles;
         M.stride = 1;
-        M.nsample = g->count;
         uint32_t gx = (uint32_t)std::max(64, (2 * g->ncu) / std::max(1, b->nqt));
-        gx = std::min<uint32_t>(gx, std::max<uint32_t>(1, M.ntiles / 4));
-        if (bf)
-            hipLaunchKernelGGL(k_scan_bf16<false>, dim3(gx, b->nqt), dim3(256), shm_bf, s, M);
+        if (bf && nw == 8) gx = std::max<uint32_t>(64, g->ncu / std::max(1, b->nqt));
+        gx = std::min<uint32_t>(gx, std::max<uint32_t>(1, M.ntiles / nw));
+        if (bf && nw == 4)
+            hipLaunchKernelGGL((k_scan_bf16<false, 4>), dim3(gx, b->nqt), dim3(256), shm_bf, s, M);
+        else if (bf)
+            hipLaunchKernelGGL((k_scan_bf16<false, 8>), dim3(gx, b->nqt), dim3(512), shm_bf, s, M);
         else
             hipLaunchKernelGGL(k_scan_f32<false>, dim3(gx, b->nqt), dim3(256), 0, s, M);
         PV_TRY(hipGetLastError());
@@ -1035,12 +1126,24 @@ static void fetch_views(FvGpuBatch *b, const FvHit **out, const uint32_t **cnt,
 static void fill_stats_times(FvGpuIndex *g) {
     if (!g->timing) return;
     FvGpuStats &st = g->last;
+    st.t_bitmap_us = ev_elapsed_us(g, 0, 6);
     st.t_prepass_us = ev_elapsed_us(g, 1, 2);
     st.t_scan_us = st.route == 2 ? ev_elapsed_us(g, 1, 4) : ev_elapsed_us(g, 3, 4);
     if (st.route == 2) st.t_gather_us = st.t_scan_us;
     st.t_merge_us = ev_elapsed_us(g, 2, 3);
     st.t_rerank_us = ev_elapsed_us(g, 4, 5);
     st.t_total_us = ev_elapsed_us(g, 0, 5);
+    /* candidate-pool sizes of the last batch (timing mode only: a copy) */
+    FvGpuBatch *b = g->last_batch;
+    if (b && b->nq > 0 && b->d_pcnt.p) {
+        std::vector<uint32_t> pc((size_t)b->nq * kPcntStride);
+        if (hipMemcpy(pc.data(), b->d_pcnt.p, pc.size() * 4, hipMemcpyDeviceToHost) ==
+            hipSuccess) {
+            uint64_t s = 0;
+            for (int q = 0; q < b->nq; q++) s += pc[(size_t)q * kPcntStride];
+            st.candidates = s;
+        }
+    }
 }
 
 } // namespace pv
@@ -1057,6 +1160,10 @@ static int brute_sync(FvGpuIndex *g, int nq, const float *queries,
                       uint32_t *out_cnt, uint8_t *truncated, char *errbuf,
                       size_t errlen) {
     if (g == nullptr) PV_FAIL("gpu: NULL index");
+    /* argument checks in the order of gpu_engine.cu:2217-2222 */
+    if (nq < 1 || nq > FV_GPU_MAX_BATCH || topb == 0 || topb > FV_GPU_MAX_TOPB ||
+        nspans_total > FV_GPU_MAX_SPANS * nq)
+        PV_FAIL("gpu: bad brute batch (nq=%d topb=%u)", nq, topb);
     PV_TRY(hipSetDevice(g->device));
     if (out == nullptr || out_cnt == nullptr || truncated == nullptr)
         PV_FAIL("gpu: NULL output arrays");
@@ -1220,7 +1327,10 @@ extern "C" int fv_gpu_batch_fetch(FvGpuIndex *g, FvGpuBatch *b, FvHit *out,
 
 extern "C" void fv_gpu_batch_free(FvGpuBatch *b) {
     if (!b) return;
-    if (b->owner) hipSetDevice(b->owner->device);
+    if (b->owner) {
+        hipSetDevice(b->owner->device);
+        if (b->owner->last_batch == b) b->owner->last_batch = nullptr;
+    }
     free_batch_buffers(b);
     delete b;
 }

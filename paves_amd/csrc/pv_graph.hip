@@ -465,6 +465,7 @@ int run_graph_opts(FvGpuIndex *g, FvGpuBatch *b, uint32_t ef, uint64_t max_visit
         PV_TRY(hipFuncSetAttribute((const void *)k_graph,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
     ev_record(g, 0);
+    PV_TRY(launch_bitmap(g, b, g->stream));
     hipLaunchKernelGGL(k_graph, dim3(b->nq), dim3(64), shm, g->stream, A);
     PV_TRY(hipGetLastError());
     ev_record(g, 5);

@@ -63,8 +63,9 @@ int pick_device(size_t need) {
     return best;
 }
 
-/* fp32 -> bf16 copy and ||x||^2 (sequential fp32 fma order), plus per-block
- * max |x|, max ||x|| and "all integers" flags. One wave per row. */
+/* fp32 -> MFMA-tiled bf16 copy and ||x||^2, plus the largest positive and
+ * negative coordinate, max ||x||^2 and an "all integers" flag. One wave per
+ * row. */
 __global__ __launch_bounds__(256) void k_derive_rows(
     const float *__restrict__ v, uint64_t count, uint64_t count_pad, int dim,
     uint16_t *__restrict__ vbf, float *__restrict__ norm,
@@ -72,25 +73,37 @@ __global__ __launch_bounds__(256) void k_derive_rows(
     const int l = threadIdx.x & 63;
     const uint64_t row = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (row >= count_pad) return;
-    float ss = 0.f, amax = 0.f;
+    float ss = 0.f, pmax = 0.f, nmax = 0.f;
     int nonint = 0;
+    /* MFMA-tiled BF16 position of (row, j): tile row/32, k-step
+     * (j mod dim/2)/8, lane (row mod 32) + 32 (j div dim/2), element j mod 8 */
+    const uint64_t tbase = (row >> 5) * 32ull * dim;
+    const int half = dim / 2;
     for (int j = l; j < dim; j += 64) {
         float x = row < count ? v[row * dim + j] : 0.f;
-        if (vbf) vbf[row * dim + j] = f2bf(x);
+        if (vbf) {
+            const int g = j / half, w = j % half;
+            vbf[tbase + (uint64_t)(w >> 3) * 512 + ((row & 31) + 32 * g) * 8 + (w & 7)] = f2bf(x);
+        }
         ss = fmaf(x, x, ss);
-        amax = fmaxf(amax, fabsf(x));
+        pmax = fmaxf(pmax, x);
+        nmax = fmaxf(nmax, -x);
         nonint |= (x != rintf(x)) ? 1 : 0;
     }
     for (int o = 32; o; o >>= 1) {
         ss += __shfl_xor(ss, o);
-        amax = fmaxf(amax, __shfl_xor(amax, o));
+        pmax = fmaxf(pmax, __shfl_xor(pmax, o));
+        nmax = fmaxf(nmax, __shfl_xor(nmax, o));
         nonint |= __shfl_xor(nonint, o);
     }
-    if (l == 0) {
+    if (l == 0 && row < count) {
         norm[row] = ss;
-        atomicMax(&stats[0], __float_as_uint(amax));
+        atomicMax(&stats[0], __float_as_uint(pmax));
         atomicMax(&stats[1], __float_as_uint(ss));
         if (nonint) atomicOr(&stats[2], 1u);
+        atomicMax(&stats[3], __float_as_uint(nmax));
+    } else if (l == 0) {
+        norm[row] = 0.f;
     }
 }
 
@@ -141,6 +154,7 @@ extern "C" void fv_gpu_index_free(FvGpuIndex *g) {
     hipFree(g->d_cols);
     hipFree(g->d_prows);
     hipFree(g->d_nbr);
+    hipFree(g->d_ones);
     if (g->sync_batch) {
         free_batch_buffers(g->sync_batch);
         delete g->sync_batch;
@@ -177,7 +191,7 @@ static FvGpuIndex *upload_impl(const float *vectors, const double *cols,
     } while (0)
     FvGpuIndex *g = nullptr;
     if (vectors == nullptr || count == 0 || dim <= 0 || ncols < 0 ||
-        count >= 0xFFFFFFFFull || (ncols > 0 && (cols == nullptr || post_rows == nullptr))) {
+        count >= 0xFFFFFFFFull || (ncols > 0 && cols == nullptr)) {
         snprintf(errbuf, errlen, "gpu: bad upload arguments (count=%llu dim=%d ncols=%d)",
                  (unsigned long long)count, dim, ncols);
         return nullptr;
@@ -243,10 +257,13 @@ static FvGpuIndex *upload_impl(const float *vectors, const double *cols,
         hipFree(d_stats);
         if (e != hipSuccess) UP_FAIL("gpu: derive stats: %s", hipGetErrorString(e));
     }
-    float amax, n2max;
-    memcpy(&amax, &hs[0], 4);
+    float pmax, nmax, n2max;
+    memcpy(&pmax, &hs[0], 4);
     memcpy(&n2max, &hs[1], 4);
-    g->vmax_abs = amax;
+    memcpy(&nmax, &hs[3], 4);
+    g->vmax = pmax;
+    g->vmin = -nmax;
+    g->vmax_abs = fmaxf(pmax, nmax);
     g->xmax_norm = sqrtf(n2max) * (1.0f + 1e-6f);
     g->all_int = hs[2] == 0;
 
@@ -254,10 +271,15 @@ static FvGpuIndex *upload_impl(const float *vectors, const double *cols,
         UP_TRY(hipMalloc(&g->d_cols, (size_t)ncols * count * 8));
         UP_TRY(hipMemcpyAsync(g->d_cols, cols, (size_t)ncols * count * 8,
                               hipMemcpyHostToDevice, s));
-        UP_TRY(hipMalloc(&g->d_prows, (size_t)ncols * count * 4));
-        UP_TRY(hipMemcpyAsync(g->d_prows, post_rows, (size_t)ncols * count * 4,
-                              hipMemcpyHostToDevice, s));
+        if (post_rows != nullptr) { /* optional: only the span-gather route reads them */
+            UP_TRY(hipMalloc(&g->d_prows, (size_t)ncols * count * 4));
+            UP_TRY(hipMemcpyAsync(g->d_prows, post_rows, (size_t)ncols * count * 4,
+                                  hipMemcpyHostToDevice, s));
+        }
     }
+    /* bitmap row used by queries without a predicate */
+    UP_TRY(hipMalloc(&g->d_ones, count_pad / 8));
+    UP_TRY(hipMemsetAsync(g->d_ones, 0xFF, count_pad / 8, s));
     if (l0 != nullptr && l0_stride > 1) {
         const int deg = (int)l0_stride - 1;
         g->nbr_stride = (deg + 7) & ~7;

@@ -149,7 +149,7 @@ struct FvGpuBatch {
     pv::DevBuf d_bitmap;  /* [nprog][words] u32 */
     pv::DevBuf d_gmin;    /* prepass group minima [nq][ngroups] u64 */
     pv::DevBuf d_pool;    /* candidate pool [nq][cap_pool] u64 */
-    pv::DevBuf d_pcnt;    /* [nq] u32 pool counts + [nq] u32 status */
+    pv::DevBuf d_pcnt;    /* [nq] u32 pool counts */
     uint64_t cap_pool = 0;
     uint32_t ngroups = 0;
     uint32_t stride = 0;
@@ -180,9 +180,11 @@ struct FvGpuIndex {
     double *d_cols = nullptr;
     uint32_t *d_prows = nullptr;
     uint32_t *d_nbr = nullptr;
+    uint32_t *d_ones = nullptr;  /* all-ones bitmap row [count_pad/32] */
 
     float xmax_norm = 0.f;   /* max ||x|| */
     float vmax_abs = 0.f;    /* max |x_i| */
+    float vmin = 0.f, vmax = 0.f; /* min / max coordinate value (0 included) */
     bool all_int = false;    /* every vector value is an integer */
 
     hipStream_t stream = nullptr;
@@ -191,6 +193,7 @@ struct FvGpuIndex {
     FvGpuStats last = {};
 
     FvGpuBatch *sync_batch = nullptr; /* reused by the synchronous ABI */
+    FvGpuBatch *last_batch = nullptr; /* batch of the last brute run (stats) */
     pv::DevBuf d_visit;               /* graph overflow visited bitmaps */
 };
 
