@@ -70,12 +70,11 @@ static inline QAux qaux_of(FvGpuBatch *b) {
  * delta = max(vmax - qmin, qmax - vmin); ||x||^2 + ||q||^2 <= D (V^2 + Q^2);
  * |2 x.q| <= 2 D V Q (V, Q = largest magnitudes).
  */
-__global__ __launch_bounds__(64) void k_prep_queries(
-    const float *__restrict__ q, int nq, int dim, uint16_t *__restrict__ qbf,
+__device__ __forceinline__ void prep_query(
+    int qi, const float *__restrict__ q, int nq, int dim, uint16_t *__restrict__ qbf,
     float *__restrict__ qnorm, float *__restrict__ eps, float xmax_norm,
     float vmin, float vmax, int all_int, int mode) {
-    const int qi = blockIdx.x;
-    const int l = threadIdx.x;
+    const int l = threadIdx.x & 63;
     const float *src = q + (uint64_t)qi * dim;
     float ss = 0.f, pmax = 0.f, nmax = 0.f;
     int nonint = 0;
@@ -118,6 +117,39 @@ __global__ __launch_bounds__(64) void k_prep_queries(
         }
         eps[qi] = (float)e;
     }
+}
+
+/*
+ * Phase 1 of a run, one launch: blocks [0, nbm) build the predicate
+ * bitmaps (one row per thread, every distinct program per row), the rest
+ * prepare the queries (one wave per padded query, four per block).
+ */
+struct Phase1Args {
+    const PvOp *ops;
+    const int32_t *poff, *plen;
+    const double *vals, *cols;
+    uint64_t count, words;
+    uint32_t *bitmap;
+    int nprog, nops, nvals, ncols;
+    uint32_t nbm;
+    const float *q;
+    int nq, nqp, dim;
+    uint16_t *qbf;
+    float *qnorm, *eps;
+    float xmax_norm, vmin, vmax;
+    int all_int, mode;
+};
+
+__global__ __launch_bounds__(256) void k_phase1(Phase1Args A) {
+    if (blockIdx.x < A.nbm) {
+        bitmap_rows(blockIdx.x, A.nprog, A.nops, A.ops, A.poff, A.plen, A.vals, A.nvals, A.cols, A.count,
+                    A.ncols, A.words, A.bitmap);
+        return;
+    }
+    const int qi = (int)(blockIdx.x - A.nbm) * 4 + (threadIdx.x >> 6);
+    if (qi >= A.nqp) return;
+    prep_query(qi, A.q, A.nq, A.dim, A.qbf, A.qnorm, A.eps, A.xmax_norm, A.vmin, A.vmax,
+               A.all_int, A.mode);
 }
 
 /* ---------------- shared scan pieces ---------------- */
@@ -567,6 +599,64 @@ __device__ uint64_t block_kth(Get get, uint32_t n, uint32_t k, uint32_t *hist,
     return prefix;
 }
 
+/*
+ * Gathers into sel[] every key that is <= an upper bound of the B-th
+ * smallest (B <= 64) of n keys, over a 256-thread block. Bound: each wave
+ * sorts its 64 lane minima; B lanes of a wave hold keys <= the wave's B-th
+ * sorted minimum, so that entry bounds the B-th smallest key from above,
+ * and the least of the four waves' entries is taken. Returns m, the number
+ * of keys under the bound (m >= B). When m <= 64, sel[0..64) holds them
+ * sorted ascending (padded with ~0) and sel[B-1] is the B-th smallest;
+ * when 64 < m <= cap, sel[0..m) holds them unsorted; above cap sel is
+ * partial.
+ */
+template <typename Get>
+__device__ uint32_t block_collect_small(Get get, uint32_t n, uint32_t B, uint64_t *sel,
+                                        uint32_t cap, uint32_t *nsel, uint64_t *wb) {
+    const int t = threadIdx.x, l = t & 63, w = t >> 6;
+    uint64_t mn = ~0ull;
+    for (uint32_t i = t; i < n; i += blockDim.x) {
+        const uint64_t k = get(i);
+        mn = k < mn ? k : mn;
+    }
+    const uint64_t srt = wave_sort64(mn);
+    if (l == (int)B - 1) wb[w] = srt;
+    if (t == 0) *nsel = 0;
+    __syncthreads();
+    uint64_t bound = wb[0];
+    for (int i = 1; i < (int)(blockDim.x >> 6); i++) bound = wb[i] < bound ? wb[i] : bound;
+    for (uint32_t i = t; i < n; i += blockDim.x) {
+        const uint64_t k = get(i);
+        if (k <= bound) {
+            const uint32_t p = atomicAdd(nsel, 1u);
+            if (p < cap) sel[p] = k;
+        }
+    }
+    __syncthreads();
+    const uint32_t m = *nsel;
+    if (m <= 64 && w == 0) {
+        uint64_t x = (uint32_t)l < m ? sel[l] : ~0ull;
+        x = wave_sort64(x);
+        sel[l] = x;
+    }
+    __syncthreads();
+    return m;
+}
+
+/* B-th smallest (B <= 64) of n keys: small-bound gather, then the radix
+ * select over the gathered keys in LDS, or over all keys when too many */
+template <typename Get>
+__device__ uint64_t block_kth_small(Get get, uint32_t n, uint32_t B, uint64_t *sel,
+                                    uint32_t cap, uint32_t *nsel, uint64_t *wb,
+                                    uint32_t *hist, uint32_t *tmp, uint32_t *bc,
+                                    uint32_t *m_out) {
+    const uint32_t m = block_collect_small(get, n, B, sel, cap, nsel, wb);
+    *m_out = m;
+    if (m <= 64) return sel[B - 1];
+    if (m <= cap) return block_kth([&](uint32_t i) { return sel[i]; }, m, B, hist, tmp, bc);
+    return block_kth(get, n, B, hist, tmp, bc);
+}
+
 /* ascending bitonic sort of n (power of two) keys in LDS */
 __device__ void block_bitonic(uint64_t *a, uint32_t n) {
     for (uint32_t k = 2; k <= n; k <<= 1) {
@@ -599,12 +689,17 @@ __global__ __launch_bounds__(256) void k_tau(const uint64_t *__restrict__ gmin,
                                              uint32_t *__restrict__ pcnt,
                                              uint32_t *__restrict__ status) {
     __shared__ uint32_t hist[256], tmp[8], bc[8];
+    __shared__ uint64_t sel[1024], wb[4];
+    __shared__ uint32_t nsel;
     const int q = blockIdx.x;
     const uint64_t *g = gmin + (uint64_t)q * ngroups;
     float tv = __uint_as_float(0x7f800000u);
     if (ngroups >= B) {
+        uint32_t m;
         const uint64_t kth =
-            block_kth([&](uint32_t i) { return g[i]; }, ngroups, B, hist, tmp, bc);
+            B <= 64 ? block_kth_small([&](uint32_t i) { return g[i]; }, ngroups, B, sel, 1024,
+                                      &nsel, wb, hist, tmp, bc, &m)
+                    : block_kth([&](uint32_t i) { return g[i]; }, ngroups, B, hist, tmp, bc);
         const float d = key_dist(kth);
         if (d < __uint_as_float(0x7f800000u)) {
             const double w = (double)d * (1.0 + ldexp(1.0, -20)) +
@@ -702,7 +797,7 @@ __global__ __launch_bounds__(256) void k_select(
     const float *__restrict__ vecs, const float *__restrict__ qf, int dim,
     FvHit *__restrict__ out, uint32_t out_stride, uint32_t *__restrict__ out_cnt,
     uint8_t *__restrict__ out_flag, uint32_t *__restrict__ status) {
-    __shared__ uint64_t sel[kSelMax];
+    __shared__ uint64_t sel[kSelMax], wb[4];
     __shared__ uint32_t hist[256], tmp[8], bc[8];
     __shared__ uint32_t nsel;
     extern __shared__ float qsh[];
@@ -722,9 +817,11 @@ __global__ __launch_bounds__(256) void k_select(
         return;
     }
     for (int j = t; j < dim; j += blockDim.x) qsh[j] = qf[(uint64_t)q * dim + j];
-    if (t == 0) nsel = 0;
+    uint32_t m0 = kSelMax + 1;
     const uint64_t kth =
-        block_kth([&](uint32_t i) { return pk[i]; }, n, B, hist, tmp, bc);
+        B <= 64 ? block_kth_small([&](uint32_t i) { return pk[i]; }, n, B, sel, kSelMax, &nsel,
+                                  wb, hist, tmp, bc, &m0)
+                : block_kth([&](uint32_t i) { return pk[i]; }, n, B, hist, tmp, bc);
     const float e = eps ? eps[q] : 0.f;
     float lim = key_dist(kth);
     if (e > 0.f) {
@@ -732,17 +829,25 @@ __global__ __launch_bounds__(256) void k_select(
         lim = (float)w;
         if ((double)lim < w) lim = nextafterf(lim, __uint_as_float(0x7f800000u));
     }
-    __syncthreads();
-    for (uint32_t i = t; i < n; i += blockDim.x) {
-        const uint64_t k = pk[i];
-        const bool take = (e > 0.f) ? (key_dist(k) <= lim || k <= kth) : (k <= kth);
-        if (take) {
-            const uint32_t pos = atomicAdd(&nsel, 1u);
-            if (pos < (uint32_t)kSelMax) sel[pos] = k;
+    uint32_t m;
+    if (e == 0.f && m0 <= 64) {
+        /* the B smallest keys are already sorted at the front of sel */
+        m = B;
+    } else {
+        __syncthreads();
+        if (t == 0) nsel = 0;
+        __syncthreads();
+        for (uint32_t i = t; i < n; i += blockDim.x) {
+            const uint64_t k = pk[i];
+            const bool take = (e > 0.f) ? (key_dist(k) <= lim || k <= kth) : (k <= kth);
+            if (take) {
+                const uint32_t pos = atomicAdd(&nsel, 1u);
+                if (pos < (uint32_t)kSelMax) sel[pos] = k;
+            }
         }
+        __syncthreads();
+        m = nsel;
     }
-    __syncthreads();
-    const uint32_t m = nsel;
     if (m > (uint32_t)kSelMax) {
         if (t == 0) {
             status[q] |= st | kStCertFail;
@@ -758,11 +863,21 @@ __global__ __launch_bounds__(256) void k_select(
         const float d = cpu_order_l2(vecs + (uint64_t)row * dim, qsh, dim, l16);
         if (l16 == 0) sel[i] = make_key(d, row);
     }
-    uint32_t np2 = 1;
-    while (np2 < m) np2 <<= 1;
-    for (uint32_t i = m + t; i < np2; i += blockDim.x) sel[i] = ~0ull;
     __syncthreads();
-    block_bitonic(sel, np2);
+    if (m <= 64) {
+        if (t < 64) {
+            uint64_t x = (uint32_t)t < m ? sel[t] : ~0ull;
+            x = wave_sort64(x);
+            sel[t] = x;
+        }
+        __syncthreads();
+    } else {
+        uint32_t np2 = 1;
+        while (np2 < m) np2 <<= 1;
+        for (uint32_t i = m + t; i < np2; i += blockDim.x) sel[i] = ~0ull;
+        __syncthreads();
+        block_bitonic(sel, np2);
+    }
     for (uint32_t i = t; i < B; i += blockDim.x) {
         FvHit h;
         h.node = key_row(sel[i]);
@@ -793,6 +908,7 @@ void free_batch_buffers(FvGpuBatch *b) {
     b->d_cnt.release();
     b->d_flag.release();
     b->h_out.release();
+    b->d_visit.release();
 }
 
 int stage_batch(FvGpuIndex *g, FvGpuBatch *b, int nq, const float *queries,
@@ -806,6 +922,10 @@ int stage_batch(FvGpuIndex *g, FvGpuBatch *b, int nq, const float *queries,
         nspans_total > FV_GPU_MAX_SPANS * nq)
         PV_FAIL("gpu: bad brute batch (nq=%d topb=%u)", nq, topb);
     if (queries == nullptr || qd == nullptr) PV_FAIL("gpu: NULL queries or descriptors");
+    if (b->stream == nullptr) {
+        b->stream = g->bstreams[g->next_stream];
+        g->next_stream = (g->next_stream + 1) % FvGpuIndex::kBatchStreams;
+    }
     ProgSet ps;
     if (compile_programs(nq, qd, nodes, nnodes, children, nchildren, invals,
                          ninvals, g->ncols, &ps, errbuf, errlen) != 0)
@@ -848,6 +968,9 @@ int stage_batch(FvGpuIndex *g, FvGpuBatch *b, int nq, const float *queries,
     b->nqt = (nq + kQT - 1) / kQT;
     b->topb = topb;
     b->nprog = (int)ps.off.size();
+    b->nops = (int)ps.ops.size();
+    b->nvals = (int)ps.vals.size();
+    if (b->nprog > kBmProgs) PV_FAIL("gpu: %d distinct predicates exceed %d", b->nprog, kBmProgs);
     const int nqp = b->nqt * kQT;
     const int dim = g->dim;
 
@@ -879,7 +1002,7 @@ int stage_batch(FvGpuIndex *g, FvGpuBatch *b, int nq, const float *queries,
     }
     if (nspans_total > 0) memcpy(h + o_sp, spans, (size_t)nspans_total * sizeof(FvGpuSpan));
     if (nspanoff > 0) memcpy(h + o_so, spanoff, (size_t)nspanoff * 8);
-    PV_TRY(hipMemcpyAsync(b->d_stage.p, h, total_bytes, hipMemcpyHostToDevice, g->stream));
+    PV_TRY(hipMemcpyAsync(b->d_stage.p, h, total_bytes, hipMemcpyHostToDevice, b->stream));
     unsigned char *d = b->d_stage.as<unsigned char>();
     b->d_q = (float *)(d + o_q);
     b->d_qd = (FvGpuQueryDesc *)(d + o_qd);
@@ -915,7 +1038,8 @@ static int scan_precision(const FvGpuIndex *g, const FvGpuBruteOpts *o) {
  * sample stride (1 = every row: the tightest threshold). */
 static int run_brute(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
                      uint32_t stride_override, char *errbuf, size_t errlen) {
-    hipStream_t s = g->stream;
+    hipStream_t s = b->stream;
+    g->ev_stream = s;
     const int prec = scan_precision(g, o);
     const int nq = b->nq, nqp = b->nqt * kQT, dim = g->dim;
     const uint32_t B = b->topb;
@@ -939,15 +1063,40 @@ static int run_brute(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
     st.route = route;
 
     ev_record(g, 0);
-    /* predicate programs -> row bitmaps, every run (the gather route
-     * evaluates the predicate on the enumerated rows instead) */
-    if (route == 1) PV_TRY(launch_bitmap(g, b, s));
+    {
+        /* predicate programs -> row bitmaps, every run (the gather route
+         * evaluates the predicate on the enumerated rows instead), fused
+         * with the query preparation */
+        Phase1Args P = {};
+        P.ops = b->d_ops;
+        P.poff = b->d_poff;
+        P.plen = b->d_plen;
+        P.vals = b->d_vals;
+        P.cols = g->d_cols;
+        P.count = g->count;
+        P.words = g->count_pad / 32;
+        P.bitmap = b->d_bitmap.as<uint32_t>();
+        P.nprog = b->nprog;
+        P.nops = b->nops;
+        P.nvals = b->nvals;
+        P.ncols = g->ncols;
+        P.nbm = (route == 1 && b->nprog > 0) ? (uint32_t)((g->count_pad + kBmRows - 1) / kBmRows) : 0;
+        P.q = b->d_q;
+        P.nq = nq;
+        P.nqp = nqp;
+        P.dim = dim;
+        P.qbf = b->d_qbf.as<uint16_t>();
+        P.qnorm = qa.qnorm;
+        P.eps = qa.eps;
+        P.xmax_norm = g->xmax_norm;
+        P.vmin = g->vmin;
+        P.vmax = g->vmax;
+        P.all_int = g->all_int ? 1 : 0;
+        P.mode = prec == FV_GPU_PREC_BF16 ? 1 : 0;
+        hipLaunchKernelGGL(k_phase1, dim3(P.nbm + (uint32_t)(nqp + 3) / 4), dim3(256), 0, s, P);
+        PV_TRY(hipGetLastError());
+    }
     ev_record(g, 6);
-    hipLaunchKernelGGL(k_prep_queries, dim3(nqp), dim3(64), 0, s, b->d_q, nq, dim,
-                       b->d_qbf.as<uint16_t>(), qa.qnorm, qa.eps, g->xmax_norm,
-                       g->vmin, g->vmax, g->all_int ? 1 : 0,
-                       prec == FV_GPU_PREC_BF16 ? 1 : 0);
-    PV_TRY(hipGetLastError());
     ev_record(g, 1);
 
     if (route == 2) {
@@ -1102,11 +1251,11 @@ static int fetch_raw(FvGpuIndex *g, FvGpuBatch *b, char *errbuf, size_t errlen) 
     unsigned char *h = b->h_out.as<unsigned char>();
     QAux qa = qaux_of(b);
     PV_TRY(hipMemcpyAsync(h + o_out, b->d_out.p, (size_t)nq * b->out_stride * sizeof(FvHit),
-                          hipMemcpyDeviceToHost, g->stream));
-    PV_TRY(hipMemcpyAsync(h + o_cnt, b->d_cnt.p, (size_t)nq * 4, hipMemcpyDeviceToHost, g->stream));
-    PV_TRY(hipMemcpyAsync(h + o_flag, b->d_flag.p, (size_t)nq, hipMemcpyDeviceToHost, g->stream));
-    PV_TRY(hipMemcpyAsync(h + o_st, qa.status, (size_t)nq * 4, hipMemcpyDeviceToHost, g->stream));
-    PV_TRY(hipStreamSynchronize(g->stream));
+                          hipMemcpyDeviceToHost, b->stream));
+    PV_TRY(hipMemcpyAsync(h + o_cnt, b->d_cnt.p, (size_t)nq * 4, hipMemcpyDeviceToHost, b->stream));
+    PV_TRY(hipMemcpyAsync(h + o_flag, b->d_flag.p, (size_t)nq, hipMemcpyDeviceToHost, b->stream));
+    PV_TRY(hipMemcpyAsync(h + o_st, qa.status, (size_t)nq * 4, hipMemcpyDeviceToHost, b->stream));
+    PV_TRY(hipStreamSynchronize(b->stream));
     return 0;
 }
 
@@ -1272,7 +1421,7 @@ extern "C" FvGpuBatch *fv_gpu_batch_stage(
         if (stage_batch(g, b, nq, queries, qd, dedup, nodes, nnodes, children,
                         nchildren, invals, ninvals, spans, nspans_total, spanoff,
                         nspanoff, topb, errbuf, errlen) != 0 ||
-            hipStreamSynchronize(g->stream) != hipSuccess) {
+            hipStreamSynchronize(b->stream) != hipSuccess) {
             free_batch_buffers(b);
             delete b;
             return nullptr;
@@ -1299,6 +1448,7 @@ extern "C" int fv_gpu_batch_run_brute(FvGpuIndex *g, FvGpuBatch *b,
 extern "C" int fv_gpu_sync(FvGpuIndex *g, char *errbuf, size_t errlen) {
     if (g == nullptr) PV_FAIL("gpu: NULL index");
     PV_TRY(hipStreamSynchronize(g->stream));
+    for (hipStream_t bs : g->bstreams) PV_TRY(hipStreamSynchronize(bs));
     fill_stats_times(g);
     return 0;
 }
@@ -1329,6 +1479,7 @@ extern "C" void fv_gpu_batch_free(FvGpuBatch *b) {
     if (!b) return;
     if (b->owner) {
         hipSetDevice(b->owner->device);
+        if (b->stream) hipStreamSynchronize(b->stream);
         if (b->owner->last_batch == b) b->owner->last_batch = nullptr;
     }
     free_batch_buffers(b);

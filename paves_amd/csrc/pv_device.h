@@ -87,32 +87,27 @@ __device__ __forceinline__ float cpu_order_l2(const float *__restrict__ x,
  * FvPred::eval_node (engine.cpp:397-430): comparisons in double, NULL
  * stored as NaN (all comparisons false), NE = v == v && v != c, IN = any
  * equal. The stack is a 64-bit shift register (depth checked on the host).
+ * colv(c) returns the row's value in column c.
  */
-__device__ __forceinline__ bool eval_prog(const PvOp *__restrict__ ops, int n,
-                                          const double *__restrict__ vals,
-                                          const double *__restrict__ cols,
-                                          uint64_t count, uint32_t row) {
+/* comparison through the op's outcome mask (kCmpPassMask), branch-free */
+__device__ __forceinline__ uint32_t cmp_pass(double v, double c, int32_t mask) {
+    const int o = (v != v) ? 3 : (v < c) ? 0 : (v == c) ? 1 : (v > c) ? 2 : 4;
+    return ((uint32_t)mask >> o) & 1u;
+}
+
+template <typename ColFn>
+__device__ __forceinline__ bool eval_prog_fn(const PvOp *__restrict__ ops, int n,
+                                             const double *__restrict__ vals, ColFn colv) {
     uint64_t st = 0;
     for (int i = 0; i < n; i++) {
         const PvOp op = ops[i];
         switch (op.kind) {
             case OP_CMP: {
-                double v = cols[(uint64_t)op.col * count + row];
-                bool r;
-                switch (op.a) {
-                    case FV_CMP_LT: r = v < op.value; break;
-                    case FV_CMP_LE: r = v <= op.value; break;
-                    case FV_CMP_EQ: r = v == op.value; break;
-                    case FV_CMP_GE: r = v >= op.value; break;
-                    case FV_CMP_GT: r = v > op.value; break;
-                    case FV_CMP_NE: r = (v == v) && (v != op.value); break;
-                    default: r = false;
-                }
-                st = (st << 1) | (uint64_t)r;
+                st = (st << 1) | (uint64_t)cmp_pass(colv(op.col), op.value, op.b);
                 break;
             }
             case OP_IN: {
-                double v = cols[(uint64_t)op.col * count + row];
+                const double v = colv(op.col);
                 bool r = false;
                 for (int j = 0; j < op.b; j++) r |= (v == vals[op.a + j]);
                 st = (st << 1) | (uint64_t)r;
@@ -134,6 +129,234 @@ __device__ __forceinline__ bool eval_prog(const PvOp *__restrict__ ops, int n,
         }
     }
     return (st & 1u) != 0;
+}
+
+__device__ __forceinline__ bool eval_prog(const PvOp *__restrict__ ops, int n,
+                                          const double *__restrict__ vals,
+                                          const double *__restrict__ cols,
+                                          uint64_t count, uint32_t row) {
+    return eval_prog_fn(ops, n, vals,
+                        [&](int c) { return cols[(uint64_t)c * count + row]; });
+}
+
+/*
+ * Predicate bitmaps of kBmRows consecutive rows per 256-thread block, row
+ * r = block * kBmRows + j * 256 + threadIdx.x for j < kBmRowsPerThread.
+ * Every distinct program is evaluated for the block's rows with each op
+ * decoded once for all of a thread's rows; the rows' first four column
+ * values stay in registers across programs. Program offsets, a window of
+ * kBmOps ops and (when they fit) the IN values are staged in LDS so that op
+ * decoding costs LDS reads instead of chains of dependent scalar-cache
+ * misses. Result words of kBmGroup programs are collected in LDS and
+ * written as whole 128-byte runs (one run per program per block).
+ * Called by every thread of the block (it synchronises).
+ */
+constexpr int kBmRowsPerThread = 8;
+constexpr int kBmRows = 256 * kBmRowsPerThread;
+constexpr int kBmOps = 512;      /* LDS op window */
+constexpr int kBmVals = 256;     /* IN values staged in LDS when they fit */
+constexpr int kBmProgs = 1024;   /* max distinct programs (one per query) */
+constexpr int kBmGroup = 16;     /* programs per coalesced write-out */
+
+__device__ __forceinline__ void bitmap_rows(uint64_t block, int nprog, int nops,
+                                            const PvOp *__restrict__ ops,
+                                            const int32_t *__restrict__ poff,
+                                            const int32_t *__restrict__ plen,
+                                            const double *__restrict__ vals, int nvals,
+                                            const double *__restrict__ cols, uint64_t count,
+                                            int ncols, uint64_t words,
+                                            uint32_t *__restrict__ bitmap) {
+    constexpr int R = kBmRowsPerThread;
+    __shared__ PvOp s_ops[kBmOps];
+    __shared__ int32_t s_off[kBmProgs], s_len[kBmProgs];
+    __shared__ double s_vals[kBmVals];
+    __shared__ uint64_t s_bm[kBmGroup][4 * R];
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    for (int i = t; i < nprog; i += blockDim.x) {
+        s_off[i] = poff[i];
+        s_len[i] = plen[i];
+    }
+    const bool vals_lds = nvals <= kBmVals;
+    if (vals_lds)
+        for (int i = t; i < nvals; i += blockDim.x) s_vals[i] = vals[i];
+
+    uint64_t row[R];
+    bool live[R];
+    double cv[R][4];
+#pragma unroll
+    for (int j = 0; j < R; j++) {
+        row[j] = block * kBmRows + (uint64_t)j * 256 + t;
+        live[j] = row[j] < count;
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+            cv[j][c] = (c < ncols && live[j]) ? cols[(uint64_t)c * count + row[j]] : 0.0;
+    }
+    /* one column's values for all of the thread's rows (c is uniform) */
+    auto colv = [&](int c, double (&v)[R]) {
+        switch (c) {
+            case 0:
+#pragma unroll
+                for (int j = 0; j < R; j++) v[j] = cv[j][0];
+                break;
+            case 1:
+#pragma unroll
+                for (int j = 0; j < R; j++) v[j] = cv[j][1];
+                break;
+            case 2:
+#pragma unroll
+                for (int j = 0; j < R; j++) v[j] = cv[j][2];
+                break;
+            case 3:
+#pragma unroll
+                for (int j = 0; j < R; j++) v[j] = cv[j][3];
+                break;
+            default:
+#pragma unroll
+                for (int j = 0; j < R; j++)
+                    v[j] = live[j] ? cols[(uint64_t)c * count + row[j]] : 0.0;
+        }
+    };
+    int win_lo = 0, win_hi = 0; /* ops[win_lo, win_hi) are in s_ops */
+    __syncthreads();
+    for (int p = 0; p < nprog; p++) {
+        /* block-uniform: every thread walks the same programs and ops */
+        const int o = __builtin_amdgcn_readfirstlane(s_off[p]);
+        const int n = __builtin_amdgcn_readfirstlane(s_len[p]);
+        uint64_t st[R];
+#pragma unroll
+        for (int j = 0; j < R; j++) st[j] = 0;
+        for (int i = 0; i < n; i++) {
+            const int oi = o + i;
+            if (oi >= win_hi) {
+                __syncthreads();
+                win_lo = oi;
+                win_hi = min(oi + kBmOps, nops);
+                for (int k = t; k < win_hi - win_lo; k += blockDim.x) s_ops[k] = ops[win_lo + k];
+                __syncthreads();
+            }
+            /* op fields into scalar registers: every dispatch below is a
+             * scalar branch taken once per op for all of the thread's rows */
+            const int32_t *ow = (const int32_t *)&s_ops[oi - win_lo];
+            const int kind = __builtin_amdgcn_readfirstlane(ow[0]);
+            const int col = __builtin_amdgcn_readfirstlane(ow[1]);
+            const int oa = __builtin_amdgcn_readfirstlane(ow[2]);
+            const int ob = __builtin_amdgcn_readfirstlane(ow[3]);
+            const double c = __hiloint2double(__builtin_amdgcn_readfirstlane(ow[5]),
+                                              __builtin_amdgcn_readfirstlane(ow[4]));
+            uint32_t r[R];
+            if (kind == OP_CMP) {
+                double v[R];
+                colv(col, v);
+                switch (oa) {
+                    case FV_CMP_LT:
+#pragma unroll
+                        for (int j = 0; j < R; j++) r[j] = v[j] < c;
+                        break;
+                    case FV_CMP_LE:
+#pragma unroll
+                        for (int j = 0; j < R; j++) r[j] = v[j] <= c;
+                        break;
+                    case FV_CMP_EQ:
+#pragma unroll
+                        for (int j = 0; j < R; j++) r[j] = v[j] == c;
+                        break;
+                    case FV_CMP_GE:
+#pragma unroll
+                        for (int j = 0; j < R; j++) r[j] = v[j] >= c;
+                        break;
+                    case FV_CMP_GT:
+#pragma unroll
+                        for (int j = 0; j < R; j++) r[j] = v[j] > c;
+                        break;
+                    default: /* NE: non-NULL and different */
+#pragma unroll
+                        for (int j = 0; j < R; j++) r[j] = (v[j] == v[j]) && (v[j] != c);
+                }
+#pragma unroll
+                for (int j = 0; j < R; j++) st[j] = (st[j] << 1) | r[j];
+            } else if (kind == OP_IN) {
+                double v[R];
+                colv(col, v);
+#pragma unroll
+                for (int j = 0; j < R; j++) r[j] = 0;
+                if (vals_lds) {
+                    for (int k = 0; k < ob; k++) {
+                        const double x = s_vals[oa + k];
+#pragma unroll
+                        for (int j = 0; j < R; j++) r[j] |= (v[j] == x) ? 1u : 0u;
+                    }
+                } else {
+                    for (int k = 0; k < ob; k++) {
+                        const double x = vals[oa + k];
+#pragma unroll
+                        for (int j = 0; j < R; j++) r[j] |= (v[j] == x) ? 1u : 0u;
+                    }
+                }
+#pragma unroll
+                for (int j = 0; j < R; j++) st[j] = (st[j] << 1) | r[j];
+            } else if (kind == OP_AND2) {
+#pragma unroll
+                for (int j = 0; j < R; j++)
+                    st[j] = ((st[j] >> 2) << 1) | (st[j] & (st[j] >> 1) & 1u);
+            } else if (kind == OP_OR2) {
+#pragma unroll
+                for (int j = 0; j < R; j++)
+                    st[j] = ((st[j] >> 2) << 1) | ((st[j] | (st[j] >> 1)) & 1u);
+            } else if (kind == OP_NOT) {
+#pragma unroll
+                for (int j = 0; j < R; j++) st[j] ^= 1u;
+            } else if (kind == OP_TRUE) {
+#pragma unroll
+                for (int j = 0; j < R; j++) st[j] = (st[j] << 1) | 1u;
+            } else { /* OP_FALSE */
+#pragma unroll
+                for (int j = 0; j < R; j++) st[j] <<= 1;
+            }
+        }
+        const int slot = p % kBmGroup;
+#pragma unroll
+        for (int j = 0; j < R; j++) {
+            /* rows j*256 + wv*64 .. +63 of the block: 64-bit word 4j + wv */
+            const uint64_t m = __ballot(live[j] && (st[j] & 1u));
+            if (lane == 0) s_bm[slot][4 * j + wv] = m;
+        }
+        if (slot == kBmGroup - 1 || p == nprog - 1) {
+            __syncthreads();
+            const int pb = p - slot;
+            for (int i = t; i < (slot + 1) * 2 * 4 * R; i += blockDim.x) {
+                const int pp = i / (8 * R), wd = i % (8 * R);
+                const uint64_t w = block * (kBmRows / 32) + wd;
+                if (w < words)
+                    bitmap[(uint64_t)(pb + pp) * words + w] =
+                        (uint32_t)(s_bm[pp][wd >> 1] >> (32 * (wd & 1)));
+            }
+            __syncthreads();
+        }
+    }
+}
+
+/* 64-bit lane exchange (two 32-bit bpermutes) */
+__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t x, int m) {
+    const int lo = __shfl_xor((int)(uint32_t)x, m);
+    const int hi = __shfl_xor((int)(uint32_t)(x >> 32), m);
+    return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
+}
+
+/* ascending bitonic sort of one key per lane across a 64-lane wave
+ * (register-resident, no LDS) */
+__device__ __forceinline__ uint64_t wave_sort64(uint64_t x) {
+    const int l = threadIdx.x & 63;
+#pragma unroll
+    for (int k = 2; k <= 64; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            const uint64_t y = shfl_xor_u64(x, j);
+            const bool keep_min = ((l & j) == 0) == ((l & k) == 0);
+            const uint64_t lo = x < y ? x : y, hi = x < y ? y : x;
+            x = keep_min ? lo : hi;
+        }
+    }
+    return x;
 }
 
 } // namespace pv

@@ -243,7 +243,6 @@ __global__ __launch_bounds__(64) void k_graph(GraphArgs A) {
     uint32_t *nb = H + kHashSize;                                 /* [128] new nodes */
     float *nd = (float *)(nb + kMaxNew);                          /* [128] */
     uint32_t *pops = (uint32_t *)(nd + kMaxNew);                  /* [8] */
-    uint32_t *cnts = pops + 8;                                    /* [4] */
 
     const FvGpuQueryDesc qd = A.qd[q];
     const int prog = A.pofq[q];
@@ -437,7 +436,7 @@ int run_graph_opts(FvGpuIndex *g, FvGpuBatch *b, uint32_t ef, uint64_t max_visit
     if (shm > 160 * 1024) PV_FAIL("gpu: graph search LDS %zu bytes exceeds the CU", shm);
     const uint64_t words = g->count_pad / 32;
     const uint64_t vwords = (words + 3) & ~3ull;
-    PV_TRY(g->d_visit.ensure((size_t)b->nq * vwords * 4));
+    PV_TRY(b->d_visit.ensure((size_t)b->nq * vwords * 4));
     b->out_stride = ef;
     PV_TRY(b->d_out.ensure((size_t)b->nq * ef * sizeof(FvHit)));
     GraphArgs A;
@@ -456,7 +455,7 @@ int run_graph_opts(FvGpuIndex *g, FvGpuBatch *b, uint32_t ef, uint64_t max_visit
     A.max_visited = max_visited;
     A.wmax = wmax;
     A.fcap = fcap;
-    A.visit = g->d_visit.as<uint32_t>();
+    A.visit = b->d_visit.as<uint32_t>();
     A.out = b->d_out.as<FvHit>();
     A.out_cnt = b->d_cnt.as<uint32_t>();
     A.out_exh = b->d_flag.as<uint8_t>();
@@ -464,9 +463,10 @@ int run_graph_opts(FvGpuIndex *g, FvGpuBatch *b, uint32_t ef, uint64_t max_visit
     if (shm > 64 * 1024)
         PV_TRY(hipFuncSetAttribute((const void *)k_graph,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+    g->ev_stream = b->stream;
     ev_record(g, 0);
-    PV_TRY(launch_bitmap(g, b, g->stream));
-    hipLaunchKernelGGL(k_graph, dim3(b->nq), dim3(64), shm, g->stream, A);
+    PV_TRY(launch_bitmap(g, b, b->stream));
+    hipLaunchKernelGGL(k_graph, dim3(b->nq), dim3(64), shm, b->stream, A);
     PV_TRY(hipGetLastError());
     ev_record(g, 5);
     memset(&g->last, 0, sizeof(g->last));
@@ -520,14 +520,14 @@ static int graph_sync(FvGpuIndex *g, int nq, const float *queries,
         b->out_stride = ef;
         std::vector<FvHit> tmp((size_t)nq * ef);
         hipError_t e = hipMemcpyAsync(tmp.data(), b->d_out.p, (size_t)nq * ef * sizeof(FvHit),
-                                      hipMemcpyDeviceToHost, g->stream);
+                                      hipMemcpyDeviceToHost, b->stream);
         if (e == hipSuccess)
-            e = hipMemcpyAsync(out_cnt, b->d_cnt.p, (size_t)nq * 4, hipMemcpyDeviceToHost, g->stream);
+            e = hipMemcpyAsync(out_cnt, b->d_cnt.p, (size_t)nq * 4, hipMemcpyDeviceToHost, b->stream);
         if (e == hipSuccess)
-            e = hipMemcpyAsync(exhausted, b->d_flag.p, (size_t)nq, hipMemcpyDeviceToHost, g->stream);
+            e = hipMemcpyAsync(exhausted, b->d_flag.p, (size_t)nq, hipMemcpyDeviceToHost, b->stream);
         if (e == hipSuccess && dbg_host)
-            e = hipMemcpyAsync(dbg_host, d_dbg, (size_t)nq * 16, hipMemcpyDeviceToHost, g->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(g->stream);
+            e = hipMemcpyAsync(dbg_host, d_dbg, (size_t)nq * 16, hipMemcpyDeviceToHost, b->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
         if (e != hipSuccess) {
             snprintf(errbuf, errlen, "gpu: graph fetch: %s", hipGetErrorString(e));
             rc = -1;

@@ -24,7 +24,7 @@
 namespace pv {
 
 void ev_record(FvGpuIndex *g, int i) {
-    if (g->timing) hipEventRecord(g->ev[i], g->stream);
+    if (g->timing) hipEventRecord(g->ev[i], g->ev_stream ? g->ev_stream : g->stream);
 }
 
 double ev_elapsed_us(FvGpuIndex *g, int a, int b) {
@@ -148,6 +148,8 @@ extern "C" void fv_gpu_index_free(FvGpuIndex *g) {
     if (!g) return;
     if (g->device >= 0) hipSetDevice(g->device);
     if (g->stream) hipStreamSynchronize(g->stream);
+    for (hipStream_t bs : g->bstreams)
+        if (bs) hipStreamSynchronize(bs);
     hipFree(g->d_vecs);
     hipFree(g->d_vbf);
     hipFree(g->d_norm);
@@ -159,10 +161,11 @@ extern "C" void fv_gpu_index_free(FvGpuIndex *g) {
         free_batch_buffers(g->sync_batch);
         delete g->sync_batch;
     }
-    g->d_visit.release();
     for (int i = 0; i < 16; i++)
         if (g->ev[i]) hipEventDestroy(g->ev[i]);
     if (g->stream) hipStreamDestroy(g->stream);
+    for (hipStream_t bs : g->bstreams)
+        if (bs) hipStreamDestroy(bs);
     {
         std::lock_guard<std::mutex> lk(dev_mu);
         if (g->device >= 0 && g->device < kMaxDevices && dev_refcount[g->device] > 0)
@@ -229,6 +232,7 @@ static FvGpuIndex *upload_impl(const float *vectors, const double *cols,
             g->ncu = prop.multiProcessorCount;
     }
     UP_TRY(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+    for (hipStream_t &bs : g->bstreams) UP_TRY(hipStreamCreateWithFlags(&bs, hipStreamNonBlocking));
     for (int i = 0; i < 16; i++) UP_TRY(hipEventCreate(&g->ev[i]));
     hipStream_t s = g->stream;
 

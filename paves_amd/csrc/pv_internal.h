@@ -63,6 +63,14 @@ struct PvOp {
 };
 static_assert(sizeof(PvOp) == 24, "PvOp layout");
 
+/*
+ * A CMP op carries in `b` the set of comparison outcomes that pass, bit o
+ * for outcome o: 0 (v < c), 1 (v == c), 2 (v > c), 3 (v is NULL/NaN),
+ * 4 (c is NaN, v is not). Indexed by FvCmpOp: LT, LE, EQ, GE, GT, NE;
+ * NE passes v == v && v != c (engine.cpp:397-430).
+ */
+constexpr int32_t kCmpPassMask[6] = {0x01, 0x03, 0x02, 0x06, 0x04, 0x15};
+
 /* grow-only device buffer */
 struct DevBuf {
     void *p = nullptr;
@@ -124,6 +132,8 @@ struct FvGpuBatch {
     int nqt = 0;            /* query tiles of kQT */
     uint32_t topb = 0;
     int nprog = 0;          /* distinct predicate programs */
+    int nops = 0;           /* ops of all programs (contiguous, program order) */
+    int nvals = 0;          /* IN values */
     bool all_spans = false; /* every query has spans and no dedup */
     bool any_dedup = false; /* some query's spans may repeat rows */
     uint64_t sum_total = 0; /* sum of span enumeration sizes */
@@ -161,6 +171,8 @@ struct FvGpuBatch {
     pv::PinBuf h_out;     /* fetch staging */
     uint32_t out_stride = 0; /* hits per query in d_out */
     int last_route = 0;
+    hipStream_t stream = nullptr; /* one of the index's batch streams */
+    pv::DevBuf d_visit;   /* graph: overflow visited bitmaps */
 };
 
 struct FvGpuIndex {
@@ -187,14 +199,20 @@ struct FvGpuIndex {
     float vmin = 0.f, vmax = 0.f; /* min / max coordinate value (0 included) */
     bool all_int = false;    /* every vector value is an integer */
 
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;   /* upload and index-wide work */
+    /* Batches are spread over a few streams (fixed per batch, so runs of
+     * one batch stay ordered) so that the latency-bound small kernels of
+     * one batch overlap the HBM-bound scan of another. */
+    static constexpr int kBatchStreams = 3;
+    hipStream_t bstreams[kBatchStreams] = {};
+    int next_stream = 0;
+    hipStream_t ev_stream = nullptr; /* stream of the run being timed */
     bool timing = false;
     hipEvent_t ev[16] = {};
     FvGpuStats last = {};
 
     FvGpuBatch *sync_batch = nullptr; /* reused by the synchronous ABI */
     FvGpuBatch *last_batch = nullptr; /* batch of the last brute run (stats) */
-    pv::DevBuf d_visit;               /* graph overflow visited bitmaps */
 };
 
 namespace pv {

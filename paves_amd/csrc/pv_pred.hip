@@ -49,6 +49,7 @@ struct Emitter {
                 if (n.col < 0 || n.col >= ncols) { err = "predicate column out of range"; return false; }
                 if (n.op < FV_CMP_LT || n.op > FV_CMP_NE) { err = "bad comparison op"; return false; }
                 op.kind = OP_CMP; op.col = n.col; op.a = n.op; op.value = n.value;
+                op.b = kCmpPassMask[n.op];
                 ops->push_back(op);
                 push();
                 return true;
@@ -181,35 +182,23 @@ int compile_programs(int nq, const FvGpuQueryDesc *qd,
     return 0;
 }
 
-/* One wave per 64 rows; grid.y = program. Bits of rows >= count are 0. */
+/* All distinct programs over kBmRows rows per block (bitmap_rows). */
 __global__ __launch_bounds__(256) void k_bitmap(
     const PvOp *__restrict__ ops, const int32_t *__restrict__ poff,
-    const int32_t *__restrict__ plen, const double *__restrict__ vals,
-    const double *__restrict__ cols, uint64_t count, uint64_t words,
+    const int32_t *__restrict__ plen, int nprog, int nops, const double *__restrict__ vals,
+    int nvals, const double *__restrict__ cols, uint64_t count, int ncols, uint64_t words,
     uint32_t *__restrict__ bitmap) {
-    const int p = blockIdx.y;
-    const uint64_t row = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    bool pass = false;
-    if (row < count)
-        pass = eval_prog(ops + poff[p], plen[p], vals, cols, count, (uint32_t)row);
-    uint64_t m = __ballot(pass);
-    if ((threadIdx.x & 63) == 0) {
-        uint64_t w = row >> 5; /* even: row is a multiple of 64 */
-        if (w + 1 < words + 1) {
-            uint64_t *dst = (uint64_t *)(bitmap + (uint64_t)p * words + w);
-            *dst = m;
-        }
-    }
+    bitmap_rows(blockIdx.x, nprog, nops, ops, poff, plen, vals, nvals, cols, count, ncols,
+                words, bitmap);
 }
 
 hipError_t launch_bitmap(FvGpuIndex *g, FvGpuBatch *b, hipStream_t s) {
     if (b->nprog == 0) return hipSuccess;
     const uint64_t words = g->count_pad / 32;
-    dim3 grid((unsigned)(g->count_pad / 256 + ((g->count_pad % 256) ? 1 : 0)),
-              (unsigned)b->nprog);
-    hipLaunchKernelGGL(k_bitmap, grid, dim3(256), 0, s, b->d_ops, b->d_poff,
-                       b->d_plen, b->d_vals, g->d_cols, g->count, words,
-                       b->d_bitmap.as<uint32_t>());
+    const unsigned blocks = (unsigned)((g->count_pad + kBmRows - 1) / kBmRows);
+    hipLaunchKernelGGL(k_bitmap, dim3(blocks), dim3(256), 0, s, b->d_ops, b->d_poff,
+                       b->d_plen, b->nprog, b->nops, b->d_vals, b->nvals, g->d_cols, g->count,
+                       g->ncols, words, b->d_bitmap.as<uint32_t>());
     return hipGetLastError();
 }
 
