@@ -8,6 +8,8 @@ mixture, query i uses filter class i mod 6 (all, price<500, price<100,
 category IN (3,7), price<1, category == centre(q) mod 32), served in
 micro-batches of 64 through the C ABI's device-resident batch entry points
 (fv_gpu_batch_stage once, then fv_gpu_batch_run_brute per micro-batch).
+The library spreads the staged batches over its batch streams, so one
+batch's small kernels overlap another batch's scan.
 
 One step = one pass over all 10,000 queries (157 micro-batches). `value` =
 queries per second of the BF16-MFMA + exact-FP32-re-rank path with inputs
@@ -15,11 +17,17 @@ resident in HBM; the plain FP32 path is timed beside it on the same staged
 batches. Results of both are checked against each other (bit-exact) and a
 sample against the CPU oracle.
 
-roofline: the dominant kernel is the BF16 full scan (k_scan_bf16<false>);
-its average launch duration is measured with HIP events recorded on the
-index's stream around that kernel; algorithmic bytes per launch = N x (2*D
-bytes of BF16 row + 4 bytes of ||x||^2) + the predicate-bitmap words read
-(distinct predicates x N/8).
+roofline: the dominant kernel is the BF16 full scan (k_scan_bf16<false, 4>).
+Its launch duration is measured with HIP events recorded on the batch's
+stream around that kernel, each micro-batch run on its own (no overlap with
+other batches), averaged over all scan-route micro-batches. Algorithmic
+bytes per launch = N x (2*D bytes of BF16 row + 4 bytes of ||x||^2) + the
+predicate-bitmap words read (distinct predicates x N/8). `traffic` comes
+from two rocprofv3 --pmc passes (FETCH_SIZE, then WRITE_SIZE) over the same
+isolated runs (`--roofline-only`), each scaled by what the counter reads for
+a device copy of known size in the same pass. `python bench.py
+--roofline-only` alone is the command whose rocprofv3 kernel trace gives the
+kernel's average duration.
 
 cpu_baseline: the reference engine (engine.cpp compiled from source under
 oracle/_ref) BruteCursor on a bounded sample of the same queries, reading an
@@ -30,6 +38,10 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import shutil
+import signal
+import sqlite3
+import subprocess
 import sys
 import tempfile
 import time
@@ -45,6 +57,8 @@ from paves_amd import indexfile, synth  # noqa: E402
 
 METRIC = "filtered top-10 QPS at recall@10≥0.9: learned-ef+BF16 vs plain FP32, 1 GPU"
 HBM_PEAK_GBS = 8000.0
+SCAN_KERNEL = "k_scan_bf16<false, 4>"
+COPY_BYTES = 1 << 28   # 256 MiB device copy used to calibrate the PMC byte counters
 
 
 def log(*a):
@@ -71,6 +85,119 @@ def run_steps(g, staged, opts, steps):
     return time.perf_counter() - t0
 
 
+def measure_scan(g, batches, staged, bf, n):
+    """Isolated runs (run + sync per micro-batch) with HIP-event timing."""
+    g.set_timing(True)
+    scan_us, prep_us, sel_us, tot_us, nprog = [], [], [], [], []
+    for (s, e, b), bh in zip(batches, staged):
+        g.run_brute(bh, bf)
+        g.sync()
+        st = g.stats()
+        if st.route == 1:
+            scan_us.append(st.t_scan_us)
+            prep_us.append(st.t_prepass_us + st.t_merge_us)
+            sel_us.append(st.t_rerank_us)
+            tot_us.append(st.t_total_us)
+            nprog.append(st.nprog)
+    g.set_timing(False)
+    avg_scan = float(np.mean(scan_us))
+    alg_bytes = n * (128 * 2 + 4) + float(np.mean(nprog)) * n / 8.0
+    return {"avg_scan_us": avg_scan, "alg_bytes": alg_bytes, "launches": len(scan_us),
+            "prepass_tau_us": float(np.mean(prep_us)), "select_us": float(np.mean(sel_us)),
+            "total_us": float(np.mean(tot_us))}
+
+
+def copy_calibration(reps=20):
+    """The library's k_copy_calib over COPY_BYTES (read COPY_BYTES, write
+    COPY_BYTES per launch); returns the read+write rate in GB/s."""
+    import ctypes
+    lib = L.load()
+    gbs = ctypes.c_double(0.0)
+    eb = L.errbuf()
+    if lib.fv_gpu_copy_bench(0, COPY_BYTES, reps, ctypes.byref(gbs), eb, len(eb)) != 0:
+        raise RuntimeError(eb.value.decode())
+    return gbs.value
+
+
+def pmc_pass(counter, args):
+    """One rocprofv3 --pmc pass over `bench.py --roofline-only --copy-calib`
+    (a child process); returns {kernel-kind: mean KiB per dispatch}."""
+    prof = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not os.path.exists(prof):
+        return None
+    out = tempfile.mkdtemp(prefix="paves_pmc_", dir="/tmp")
+    cmd = [prof, "--pmc", counter, "-d", out, "-o", "pmc", "--",
+           sys.executable, os.path.abspath(__file__), "--roofline-only", "--copy-calib",
+           "--n", str(args.n), "--nq", str(min(args.nq, 1024)), "--batch", str(args.batch),
+           "--k", str(args.k)]
+    env = dict(os.environ, TMPDIR="/tmp")
+    try:
+        p = subprocess.Popen(cmd, cwd="/tmp", env=env, stdout=subprocess.DEVNULL,
+                             stderr=subprocess.DEVNULL, start_new_session=True)
+        try:
+            p.wait(timeout=180)
+        except subprocess.TimeoutExpired:
+            os.killpg(p.pid, signal.SIGKILL)
+            p.wait()
+            log(f"pmc {counter}: timed out")
+            return None
+        dbs = [os.path.join(r, f) for r, _, fs in os.walk(out) for f in fs if f.endswith(".db")]
+        if not dbs:
+            log(f"pmc {counter}: no database (rc {p.returncode})")
+            return None
+        c = sqlite3.connect(dbs[0])
+        rows = c.execute("select dispatch_id, kernel_name, sum(value) from counters_collection "
+                         "where counter_name = ? group by dispatch_id", (counter,)).fetchall()
+        c.close()
+    finally:
+        shutil.rmtree(out, ignore_errors=True)
+    scan = [v for _, k, v in rows if SCAN_KERNEL.replace(" ", "") in k.replace(" ", "")]
+    copy = [v for _, k, v in rows if "k_copy_calib" in k]
+    if not scan or not copy:
+        log(f"pmc {counter}: scan {len(scan)} / copy {len(copy)} dispatches found")
+        return None
+    return {"scan_kib": float(np.median(scan)), "copy_kib": float(np.median(copy))}
+
+
+def measure_traffic(args):
+    """HBM bytes per scan launch: FETCH_SIZE + WRITE_SIZE, each scaled by
+    the counter's reading for the known-size copy (COPY_BYTES each way)."""
+    f = pmc_pass("FETCH_SIZE", args)
+    w = pmc_pass("WRITE_SIZE", args) if f else None
+    if not f or not w:
+        return None, None
+    kib = COPY_BYTES / 1024.0
+    fs, ws = kib / f["copy_kib"], kib / w["copy_kib"]
+    fetch = f["scan_kib"] * fs * 1024.0
+    write = w["scan_kib"] * ws * 1024.0
+    info = {"fetch_kib_raw": round(f["scan_kib"], 1), "write_kib_raw": round(w["scan_kib"], 1),
+            "fetch_scale": round(fs, 4), "write_scale": round(ws, 4),
+            "fetch_bytes": int(fetch), "write_bytes": int(write)}
+    return fetch + write, info
+
+
+def roofline_only(args):
+    """Isolated scan-route runs only (for rocprofv3); prints one JSON line."""
+    d = synth.gmm128(args.n, args.nq)
+    cols = synth.columns(d)
+    rows, vals = synth.postings(cols)
+    post = E.Postings(rows, vals)
+    batches = build_batches(d, post, args.n, args.nq, args.batch, args.k)
+    g = E.GpuIndex(d["base"], cols, rows, precision=L.FV_GPU_PREC_BF16, device=0)
+    staged = [g.stage(d["queries"][s:e], b, args.k) for s, e, b in batches]
+    bf = L.BruteOpts(precision=L.FV_GPU_PREC_BF16, route=0)
+    for bh in staged:            # warm-up
+        g.run_brute(bh, bf)
+    g.sync()
+    m = measure_scan(g, batches, staged, bf, args.n)
+    if args.copy_calib:
+        copy_calibration(reps=3)
+    for bh in staged:
+        g.free_batch(bh)
+    g.close()
+    print(json.dumps(m), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -84,12 +211,26 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the two rocprofv3 --pmc passes (traffic = null)")
+    ap.add_argument("--roofline-only", action="store_true",
+                    help="only the isolated scan runs (the rocprofv3 target)")
+    ap.add_argument("--copy-calib", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(the index lives on a single MI355X; see the sharding note in DESIGN.md)",
               file=sys.stderr)
         sys.exit(2)
+    if args.roofline_only:
+        roofline_only(args)
+        return
+
+    traffic, traffic_info = (None, None)
+    if not args.no_traffic:
+        t = time.perf_counter()
+        traffic, traffic_info = measure_traffic(args)
+        log(f"pmc traffic: {traffic_info} ({time.perf_counter() - t:.0f}s)")
 
     t = time.perf_counter()
     d = synth.gmm128(args.n, args.nq)
@@ -125,6 +266,15 @@ def main():
     log(f"fp32: {qps32:,.0f} QPS")
     res_fp = [g.fetch(bh, e - s, args.k) for bh, (s, e, _) in zip(staged, batches)]
 
+    # ---- the reference's synchronous entry point (host buffers in and out:
+    # predicate compile, H2D staging, pipeline, D2H), one pass, for the
+    # PCIe-inclusive rate
+    t0 = time.perf_counter()
+    for s, e, b in batches:
+        g.brute(d["queries"][s:e], b, args.k)
+    qps_sync = args.nq / (time.perf_counter() - t0)
+    log(f"fv_gpu_brute_batch (host buffers, synchronous): {qps_sync:,.0f} QPS")
+
     # ---- correctness: BF16 == FP32 (both exact), sample == oracle
     exact = True
     for (o1, c1, f1, _), (o2, c2, f2, _) in zip(res_bf, res_fp):
@@ -147,25 +297,18 @@ def main():
         log(f"oracle check (first micro-batch, {e - s} queries): {sample_ok}")
 
     # ---- roofline of the dominant kernel (BF16 full scan), HIP events
-    g.set_timing(True)
-    scan_us, prep_us, sel_us, tot_us, nprog = [], [], [], [], []
-    for (s, e, b), bh in zip(batches, staged):
-        g.run_brute(bh, bf)
-        g.sync()
-        st = g.stats()
-        if st.route == 1:
-            scan_us.append(st.t_scan_us)
-            prep_us.append(st.t_prepass_us + st.t_merge_us)
-            sel_us.append(st.t_rerank_us)
-            tot_us.append(st.t_total_us)
-            nprog.append(st.nprog)
-    g.set_timing(False)
-    avg_scan = float(np.mean(scan_us))
-    words_bytes = float(np.mean(nprog)) * args.n / 8.0
-    alg_bytes = args.n * (128 * 2 + 4) + words_bytes
+    m = measure_scan(g, batches, staged, bf, args.n)
+    avg_scan, alg_bytes = m["avg_scan_us"], m["alg_bytes"]
     achieved = alg_bytes / (avg_scan * 1e-6) / 1e9
     log(f"scan kernel avg {avg_scan:.1f} us -> {achieved:.0f} GB/s; prepass+tau "
-        f"{np.mean(prep_us):.1f} us, select {np.mean(sel_us):.1f} us, total {np.mean(tot_us):.1f} us")
+        f"{m['prepass_tau_us']:.1f} us, select {m['select_us']:.1f} us, "
+        f"total {m['total_us']:.1f} us (isolated)")
+    copy_gbs = None
+    try:
+        copy_gbs = copy_calibration()
+        log(f"device copy: {copy_gbs:.0f} GB/s (read + write)")
+    except RuntimeError as ex:
+        log(f"device copy not measured: {ex}")
 
     # ---- CPU baseline: reference BruteCursor on a bounded sample
     cpu = None
@@ -187,10 +330,15 @@ def main():
                    "k": args.k, "topb": args.k},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "kernel": "k_scan_bf16<false, 4>",
+                     "traffic": int(traffic) if traffic else None, "kernel": SCAN_KERNEL,
                      "avg_launch_us": round(avg_scan, 2),
-                     "alg_bytes_per_launch": int(alg_bytes)},
+                     "alg_bytes_per_launch": int(alg_bytes),
+                     "launches": m["launches"],
+                     "copy_peak_GBs": round(copy_gbs, 1) if copy_gbs else None,
+                     "frac_of_copy_peak": round(achieved / copy_gbs, 4) if copy_gbs else None,
+                     "traffic_detail": traffic_info},
         "plain_fp32_qps": round(qps32, 1),
+        "sync_abi_qps_host_buffers": round(qps_sync, 1),
         "speedup_vs_plain_fp32": round(qps / qps32, 3),
         "recall_at_10": 1.0 if (exact and sample_ok is not False) else None,
         "ids_bit_exact_bf16_vs_fp32": exact,

@@ -172,7 +172,7 @@ typedef struct FvGpuBruteOpts {
 } FvGpuBruteOpts;
 
 /* Per-call statistics of the last batch run on this index. Kernel times are
- * measured with HIP events on the index's stream when timing is enabled. */
+ * measured with HIP events on the batch's stream when timing is enabled. */
 typedef struct FvGpuStats {
     double t_bitmap_us, t_prepass_us, t_scan_us, t_merge_us, t_rerank_us;
     double t_gather_us, t_graph_us, t_total_us;
@@ -228,21 +228,29 @@ FvGpuBatch *fv_gpu_batch_stage(FvGpuIndex *g, int nq, const float *queries,
                                const FvGpuSpan *spans, int nspans_total,
                                const uint64_t *spanoff, int nspanoff,
                                uint32_t topb, char *errbuf, size_t errlen);
-/* Enqueue the brute pipeline for a staged batch on the index's stream
- * (asynchronous; 0 / -1 + errbuf). */
+/* Enqueue the brute pipeline for a staged batch on the batch's stream
+ * (asynchronous; 0 / -1 + errbuf). Each batch is given one of the index's
+ * batch streams when it is staged, so different batches may overlap. */
 int fv_gpu_batch_run_brute(FvGpuIndex *g, FvGpuBatch *b,
                            const FvGpuBruteOpts *opts, char *errbuf,
                            size_t errlen);
 /* Enqueue the graph pipeline for a staged batch (asynchronous). */
 int fv_gpu_batch_run_graph(FvGpuIndex *g, FvGpuBatch *b, uint32_t ef,
                            uint64_t max_visited, char *errbuf, size_t errlen);
-/* Wait for the index's stream. */
+/* Wait for every stream of the index. */
 int fv_gpu_sync(FvGpuIndex *g, char *errbuf, size_t errlen);
 /* Copy a batch's results to the host (synchronous). */
 int fv_gpu_batch_fetch(FvGpuIndex *g, FvGpuBatch *b, FvHit *out,
                        uint32_t *out_cnt, uint8_t *flags, char *errbuf,
                        size_t errlen);
 void fv_gpu_batch_free(FvGpuBatch *b);
+
+/* Device copy bandwidth: copies `bytes` (multiple of 16) between two fresh
+ * buffers `reps` times with k_copy_calib and reports read + write GB/s.
+ * The measured copy peak of roofline figures, and the known-size kernel
+ * that scales rocprofv3's FETCH_SIZE / WRITE_SIZE readings. */
+int fv_gpu_copy_bench(int device, uint64_t bytes, int reps, double *gbs,
+                      char *errbuf, size_t errlen);
 
 #ifdef __cplusplus
 }

@@ -43,7 +43,7 @@ REF_SYMBOLS = ["fv_gpu_available", "fv_gpu_index_upload", "fv_gpu_index_free",
 EXT_SYMBOLS = ["fv_gpu_set_timing", "fv_gpu_last_stats", "fv_gpu_brute_batch_ex",
                "fv_gpu_graph_batch_ex", "fv_gpu_batch_stage", "fv_gpu_batch_run_brute",
                "fv_gpu_batch_run_graph", "fv_gpu_sync", "fv_gpu_batch_fetch",
-               "fv_gpu_batch_free"]
+               "fv_gpu_batch_free", "fv_gpu_copy_bench"]
 
 
 class BruteOpts(C.Structure):
@@ -115,6 +115,8 @@ def load() -> C.CDLL:
     lib.fv_gpu_sync.argtypes = [_P, C.c_char_p, sz]
     lib.fv_gpu_batch_fetch.argtypes = [_P, _P, _P, _P, _P, C.c_char_p, sz]
     lib.fv_gpu_batch_free.argtypes = [_P]
+    lib.fv_gpu_copy_bench.argtypes = [C.c_int, C.c_uint64, C.c_int, C.POINTER(C.c_double),
+                                      C.c_char_p, sz]
     _lib = lib
     return lib
 

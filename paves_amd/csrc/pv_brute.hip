@@ -3,21 +3,24 @@
  *
  * Replaces the reference's brute path (gpu_engine.cu:348-418 span kernel,
  * 474-610 tiled kernel, 1003-1065 merge kernel, 2206-2336 router) with a
- * different pipeline built for CDNA4 (one stream, no host round trip):
+ * different pipeline built for CDNA4 (one stream per batch, no host round
+ * trip inside a run):
  *
- *   1. k_prep_queries   fp32 queries -> bf16 copy, ||q||^2, error bound eps
- *   2. k_bitmap         predicate programs -> row bitmaps (pv_pred.hip)
- *   3. k_scan<PREPASS>  distances of every stride-th row; per 16-row group
- *                       the minimum passing distance
- *   4. k_tau            tau[q] = B-th smallest group minimum (+2 eps): at
+ *   1. k_phase1         predicate programs -> row bitmaps (bitmap_rows),
+ *                       and in the same launch fp32 queries -> bf16 copy,
+ *                       ||q||^2, per-query error bound eps
+ *   2. k_scan<PREPASS>  distances of every stride-th 32-row tile; per
+ *                       16-row group the minimum passing distance
+ *   3. k_tau            tau[q] = B-th smallest group minimum (+2 eps): at
  *                       least B passing rows lie at or below it, so every
  *                       row of the exact top-B does too
- *   5. k_scan<MAIN>     full scan; rows with dist <= tau enter a per-query
+ *   4. k_scan<MAIN>     full scan; rows with dist <= tau enter a per-query
  *                       candidate pool (LDS, flushed to HBM per WG)
- *   6. k_select         radix-select the B-th smallest pool key, collect the
- *                       keys within 2 eps of it, re-rank them with exact FP32
- *                       in the CPU cursor's summation order, bitonic-sort by
- *                       (dist, row), emit FvHit + count + truncated
+ *   5. k_select         B-th smallest pool key (wave-sorted bound + 64-lane
+ *                       register sort for B <= 64, radix select otherwise),
+ *                       collect the keys within 2 eps of it, re-rank them
+ *                       with exact FP32 in the CPU cursor's summation order,
+ *                       sort by (dist, row), emit FvHit + count + truncated
  *
  * Distances in the scans: BF16 MFMA (v_mfma_f32_32x32x16_bf16) computing
  * x.q with the ||x||^2 + ||q||^2 - 2 x.q epilogue, or plain FP32 VALU

@@ -346,3 +346,59 @@ extern "C" void fv_gpu_set_timing(FvGpuIndex *g, int on) {
 extern "C" void fv_gpu_last_stats(const FvGpuIndex *g, FvGpuStats *st) {
     if (g && st) *st = g->last;
 }
+
+/* ---------------- copy bandwidth (roofline reference) ---------------- */
+
+namespace pv {
+namespace {
+typedef uint32_t cp_u32x4 __attribute__((ext_vector_type(4)));
+/* grid-stride 16-byte copy: each lane moves consecutive 16-byte words, a
+ * wave covers 1 KiB contiguous per step */
+__global__ __launch_bounds__(256) void k_copy_calib(const cp_u32x4 *__restrict__ src,
+                                                    cp_u32x4 *__restrict__ dst, uint64_t n16) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride)
+        __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
+}
+} // namespace
+} // namespace pv
+
+extern "C" int fv_gpu_copy_bench(int device, uint64_t bytes, int reps, double *gbs,
+                                 char *errbuf, size_t errlen) {
+    if (bytes == 0 || bytes % 16 != 0 || reps < 1 || gbs == nullptr)
+        PV_FAIL("gpu: copy bench: bad arguments");
+    PV_TRY(hipSetDevice(device));
+    void *a = nullptr, *b = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipMalloc(&a, bytes);
+    if (e == hipSuccess) e = hipMalloc(&b, bytes);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipMemsetAsync(a, 1, bytes, s);
+    float ms = 0.f;
+    if (e == hipSuccess) {
+        const uint64_t n16 = bytes / 16;
+        const unsigned grid = 8192;
+        hipLaunchKernelGGL(pv::k_copy_calib, dim3(grid), dim3(256), 0, s, (const pv::cp_u32x4 *)a,
+                           (pv::cp_u32x4 *)b, n16);
+        e = hipEventRecord(e0, s);
+        for (int r = 0; r < reps && e == hipSuccess; r++) {
+            hipLaunchKernelGGL(pv::k_copy_calib, dim3(grid), dim3(256), 0, s,
+                               (const pv::cp_u32x4 *)a, (pv::cp_u32x4 *)b, n16);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipEventRecord(e1, s);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    }
+    if (e1) hipEventDestroy(e1);
+    if (e0) hipEventDestroy(e0);
+    if (s) hipStreamDestroy(s);
+    if (b) hipFree(b);
+    if (a) hipFree(a);
+    if (e != hipSuccess) PV_FAIL("gpu: copy bench: %s", hipGetErrorString(e));
+    *gbs = ms > 0.f ? 2.0 * (double)bytes * reps / ((double)ms * 1e-3) / 1e9 : 0.0;
+    return 0;
+}
