@@ -47,21 +47,7 @@ typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 namespace pv {
 
-/* per-query auxiliary array layout inside d_qaux (floats) */
-struct QAux {
-    float *qnorm, *eps, *tau;
-    uint32_t *status;
-};
-static inline QAux qaux_of(FvGpuBatch *b) {
-    const int nqp = b->nqt * kQT;
-    float *base = b->d_qaux.as<float>();
-    QAux a;
-    a.qnorm = base;
-    a.eps = base + nqp;
-    a.tau = base + 2 * nqp;
-    a.status = (uint32_t *)(base + 3 * nqp);
-    return a;
-}
+
 
 /* ---------------- 1. query preparation ---------------- */
 
@@ -970,6 +956,8 @@ int stage_batch(FvGpuIndex *g, FvGpuBatch *b, int nq, const float *queries,
     b->nq = nq;
     b->nqt = (nq + kQT - 1) / kQT;
     b->topb = topb;
+    b->max_entry = 0;
+    for (int q = 0; q < nq; q++) b->max_entry = std::max<uint64_t>(b->max_entry, qd[q].entry);
     b->nprog = (int)ps.off.size();
     b->nops = (int)ps.ops.size();
     b->nvals = (int)ps.vals.size();

@@ -64,6 +64,7 @@ struct GraphArgs {
     FvHit *out;
     uint32_t *out_cnt;
     uint8_t *out_exh;
+    uint32_t *status;    /* per-query status words, cleared by the search */
     uint32_t *dbg;       /* [nq][4] iterations, visited, results, overflow */
 };
 
@@ -409,6 +410,7 @@ __global__ __launch_bounds__(64) void k_graph(GraphArgs A) {
     if (l == 0) {
         A.out_cnt[q] = (uint32_t)nR;
         A.out_exh[q] = (uint8_t)exh;
+        A.status[q] = 0;
         if (A.dbg) {
             A.dbg[(uint64_t)q * 4 + 0] = iters;
             A.dbg[(uint64_t)q * 4 + 1] = (uint32_t)min<uint64_t>(visited, 0xFFFFFFFFull);
@@ -427,6 +429,8 @@ int run_graph_opts(FvGpuIndex *g, FvGpuBatch *b, uint32_t ef, uint64_t max_visit
                    int wmax, int fcap, uint32_t *dbg, char *errbuf, size_t errlen) {
     if (g->d_nbr == nullptr) PV_FAIL("gpu: graph not uploaded");
     if (ef < 1 || ef > FV_GPU_MAX_EF) PV_FAIL("gpu: bad graph batch (nq=%d ef=%u)", b->nq, ef);
+    if (b->max_entry >= g->count)
+        PV_FAIL("gpu: entry point %llu out of range", (unsigned long long)b->max_entry);
     if (wmax < 1) wmax = 1;
     while (wmax > 1 && wmax * g->nbr_stride > kMaxNew) wmax--;
     if (g->nbr_stride > kMaxNew) PV_FAIL("gpu: graph degree %d too large", g->nbr_stride);
@@ -438,6 +442,7 @@ int run_graph_opts(FvGpuIndex *g, FvGpuBatch *b, uint32_t ef, uint64_t max_visit
     const uint64_t vwords = (words + 3) & ~3ull;
     PV_TRY(b->d_visit.ensure((size_t)b->nq * vwords * 4));
     b->out_stride = ef;
+    b->last_route = 3;
     PV_TRY(b->d_out.ensure((size_t)b->nq * ef * sizeof(FvHit)));
     GraphArgs A;
     A.vecs = g->d_vecs;
@@ -459,6 +464,7 @@ int run_graph_opts(FvGpuIndex *g, FvGpuBatch *b, uint32_t ef, uint64_t max_visit
     A.out = b->d_out.as<FvHit>();
     A.out_cnt = b->d_cnt.as<uint32_t>();
     A.out_exh = b->d_flag.as<uint8_t>();
+    A.status = qaux_of(b).status;
     A.dbg = dbg;
     if (shm > 64 * 1024)
         PV_TRY(hipFuncSetAttribute((const void *)k_graph,

@@ -138,6 +138,7 @@ struct FvGpuBatch {
     bool any_dedup = false; /* some query's spans may repeat rows */
     uint64_t sum_total = 0; /* sum of span enumeration sizes */
     uint64_t max_total = 0;
+    uint64_t max_entry = 0; /* largest qd[q].entry (graph runs) */
     bool query_exact = true;
     std::vector<uint64_t> total;   /* host copy of qd[q].total */
     std::vector<int32_t> nspans;
@@ -216,6 +217,22 @@ struct FvGpuIndex {
 };
 
 namespace pv {
+
+/* per-query auxiliary array layout inside d_qaux (floats) */
+struct QAux {
+    float *qnorm, *eps, *tau;
+    uint32_t *status;
+};
+inline QAux qaux_of(FvGpuBatch *b) {
+    const int nqp = b->nqt * kQT;
+    float *base = b->d_qaux.as<float>();
+    QAux a;
+    a.qnorm = base;
+    a.eps = base + nqp;
+    a.tau = base + 2 * nqp;
+    a.status = (uint32_t *)(base + 3 * nqp);
+    return a;
+}
 
 /* pv_pred.hip */
 int compile_programs(int nq, const FvGpuQueryDesc *qd,
