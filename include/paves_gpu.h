@@ -252,6 +252,39 @@ void fv_gpu_batch_free(FvGpuBatch *b);
 int fv_gpu_copy_bench(int device, uint64_t bytes, int reps, double *gbs,
                       char *errbuf, size_t errlen);
 
+/*
+ * HNSW graph construction on the host (the reference builds inside
+ * fv_builder_finish, engine.cpp:698-1032; here it is a library call so a
+ * GPU index can be given a graph without writing a file). Semantics are the
+ * reference builder's: M links per new node, M0 = 2M at level 0, splitmix64
+ * levels, efConstruction beam, heuristic pruning, node 0 first then
+ * `threads` workers (threads = 1 gives the reference's single-threaded
+ * graph exactly). The caller's vectors are read during the call only.
+ */
+typedef struct FvHnswGraph FvHnswGraph;
+FvHnswGraph *fv_hnsw_build(const float *vectors, uint64_t count, int dim, int M,
+                           int ef_construction, int threads, char *errbuf,
+                           size_t errlen);
+/* sizes: l0_stride = 2M+1 words per node; upper_slots words of upper slabs */
+int fv_hnsw_info(const FvHnswGraph *h, uint32_t *M, uint32_t *l0_stride,
+                 uint32_t *max_level, uint64_t *entry_node, uint64_t *upper_slots);
+/* copy out (any pointer may be NULL): l0 [count][l0_stride] ([cnt, ids..]),
+ * levels [count], upper_offs [count+1], upper [upper_slots] — the
+ * reference index-file sections (engine.cpp:190-215) */
+int fv_hnsw_export(const FvHnswGraph *h, uint32_t *l0, uint8_t *levels,
+                   uint64_t *upper_offs, uint32_t *upper);
+void fv_hnsw_free(FvHnswGraph *h);
+/* Level-0 entry points of nq queries by greedy descent through the upper
+ * levels (fv_index_graph_entry, engine.cpp:1539-1559), on `threads` host
+ * threads; works on built graphs and on loaded index files alike. */
+int fv_hnsw_entries(const float *vectors, int dim, const uint64_t *upper_offs,
+                    const uint32_t *upper, uint32_t M, uint64_t entry_node,
+                    uint32_t max_level, int nq, const float *queries,
+                    uint32_t *out, int threads);
+/* FP32 squared L2 distance in the reference's summation order
+ * (engine.cpp:43-90), as the graph builder and entry descent use it. */
+float fv_l2sq(const float *a, const float *b, int dim);
+
 #ifdef __cplusplus
 }
 #endif

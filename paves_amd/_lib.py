@@ -43,7 +43,8 @@ REF_SYMBOLS = ["fv_gpu_available", "fv_gpu_index_upload", "fv_gpu_index_free",
 EXT_SYMBOLS = ["fv_gpu_set_timing", "fv_gpu_last_stats", "fv_gpu_brute_batch_ex",
                "fv_gpu_graph_batch_ex", "fv_gpu_batch_stage", "fv_gpu_batch_run_brute",
                "fv_gpu_batch_run_graph", "fv_gpu_sync", "fv_gpu_batch_fetch",
-               "fv_gpu_batch_free", "fv_gpu_copy_bench"]
+               "fv_gpu_batch_free", "fv_gpu_copy_bench", "fv_hnsw_build", "fv_hnsw_info",
+               "fv_hnsw_export", "fv_hnsw_free", "fv_hnsw_entries", "fv_l2sq"]
 
 
 class BruteOpts(C.Structure):
@@ -117,6 +118,18 @@ def load() -> C.CDLL:
     lib.fv_gpu_batch_free.argtypes = [_P]
     lib.fv_gpu_copy_bench.argtypes = [C.c_int, C.c_uint64, C.c_int, C.POINTER(C.c_double),
                                       C.c_char_p, sz]
+    lib.fv_hnsw_build.restype = _P
+    lib.fv_hnsw_build.argtypes = [_P, C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_int,
+                                  C.c_char_p, sz]
+    lib.fv_hnsw_info.argtypes = [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                 C.POINTER(C.c_uint32), C.POINTER(C.c_uint64),
+                                 C.POINTER(C.c_uint64)]
+    lib.fv_hnsw_export.argtypes = [_P, _P, _P, _P, _P]
+    lib.fv_hnsw_free.argtypes = [_P]
+    lib.fv_hnsw_entries.argtypes = [_P, C.c_int, _P, _P, C.c_uint32, C.c_uint64, C.c_uint32,
+                                    C.c_int, _P, _P, C.c_int]
+    lib.fv_l2sq.restype = C.c_float
+    lib.fv_l2sq.argtypes = [_P, _P, C.c_int]
     _lib = lib
     return lib
 

@@ -19,7 +19,7 @@ HDR = os.path.join(ROOT, "include", "paves_gpu.h")
 def declared_functions():
     src = open(HDR).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-    return sorted(set(re.findall(r"\b(fv_gpu_[a-z0-9_]+)\s*\(", src)))
+    return sorted(set(re.findall(r"\b(fv_(?:gpu|hnsw)_[a-z0-9_]+|fv_l2sq)\s*\(", src)))
 
 
 def exported_symbols(path):

@@ -2,9 +2,10 @@
 oracle's restatement of the reference HnswCursor (engine.cpp:1582-1731),
 plus recall against exact brute force and the ABI's output contract.
 
-Graphs: the reference builder's own index files (compiled under oracle/_ref,
-test infrastructure) and this library's builder when present; both hold the
-reference level-0 layout [cnt, neighbours...] with stride 2M+1.
+Graphs come from this library's host builder (paves_amd/hnsw.py, pinned
+bit-for-bit against the reference builder in test_hnsw_build.py), in the
+reference level-0 layout [cnt, neighbours...] with stride 2M+1; entry points
+from its greedy descent (pinned against the oracle).
 """
 import os
 
@@ -13,7 +14,7 @@ import pytest
 
 from paves_amd import _lib as L
 from paves_amd import engine as E
-from paves_amd import synth
+from paves_amd import hnsw, synth
 
 import oracle as O
 
@@ -24,29 +25,15 @@ NQ = 48
 EF = 32
 
 
-def _have_ref():
-    try:
-        O.RefEngine()
-        return True
-    except RuntimeError:
-        return False
-
-
 @pytest.fixture(scope="module")
-def world(tmp_path_factory):
-    if not _have_ref():
-        pytest.skip("reference builder not built (oracle/_ref)")
+def world():
     d = synth.gmm128(N, NQ, seed=31)
     cols = synth.columns(d)
     rows, vals = synth.postings(cols)
-    path = str(tmp_path_factory.mktemp("g") / "g.idx")
-    O.RefEngine().build(path, d["base"], cols, M=8, efc=64, threads=8)
-    f = O.read_index_file(path)
-    entries = np.array([O.graph_entry(f["vectors"], f["upper_offs"], f["upper"], f["M"],
-                                      f["entry_node"], f["max_level"], q) for q in d["queries"]],
-                       dtype=np.uint32)
-    g = E.GpuIndex(d["base"], cols, rows, l0=np.array(f["l0"]), precision=L.FV_GPU_PREC_BF16,
-                   device=0)
+    f = hnsw.build(d["base"], M=8, ef_construction=64, threads=8)
+    f["vectors"] = d["base"]
+    entries = hnsw.entries(f, d["base"], d["queries"])
+    g = E.GpuIndex(d["base"], cols, rows, l0=f["l0"], precision=L.FV_GPU_PREC_BF16, device=0)
     yield {"d": d, "cols": cols, "post": E.Postings(rows, vals), "f": f, "entries": entries,
            "g": g}
     g.close()
