@@ -285,6 +285,60 @@ int fv_hnsw_entries(const float *vectors, int dim, const uint64_t *upper_offs,
  * (engine.cpp:43-90), as the graph builder and entry descent use it. */
 float fv_l2sq(const float *a, const float *b, int dim);
 
+/*
+ * Learned per-query beam width and brute-vs-graph routing (replaces the
+ * fixed policy of exec.c:422-424 and the planner's cost race,
+ * planner.c:185-346; no counterpart in the reference's GPU object).
+ * Model: inputs [query (dim) | log10 selectivity | distances to the 16
+ * nearest centroids] standardised, 64-64 ReLU hidden layers, outputs
+ * (log ef, route logit > 0 = graph). FP32 parameters, trained on the GPU;
+ * inference with BF16 weights. Parameter layout (FP32):
+ *   W1[64][F] b1[64] W2[64][64] b2[64] W3[2][64] b3[2],  F = dim + 17.
+ * Training targets per sample: [log ef, route (0 brute / 1 graph), ef
+ * valid (0/1)]; loss = valid (y0 - log ef)^2 + lambda BCE(y1, route).
+ */
+typedef struct FvGpuModel FvGpuModel;
+int fv_gpu_model_nparams(int dim);
+FvGpuModel *fv_gpu_model_create(int device, int dim, const float *centroids, int ncent,
+                                char *errbuf, size_t errlen);
+void fv_gpu_model_free(FvGpuModel *m);
+int fv_gpu_model_set_params(FvGpuModel *m, const float *params, const float *mean,
+                            const float *std_, char *errbuf, size_t errlen);
+int fv_gpu_model_get_params(FvGpuModel *m, float *params, float *mean, float *std_,
+                            char *errbuf, size_t errlen);
+int fv_gpu_model_grad(FvGpuModel *m, int n, const float *feats, const float *targets,
+                      float lambda, float *grad, double *loss, char *errbuf, size_t errlen);
+int fv_gpu_model_train(FvGpuModel *m, int n, const float *feats, const float *targets,
+                       int epochs, int batch, float lr, float lambda, uint64_t seed,
+                       float *loss_hist, char *errbuf, size_t errlen);
+int fv_gpu_model_predict(FvGpuModel *m, int n, const float *feats, float *y,
+                         char *errbuf, size_t errlen);
+/* feature vectors [nq][dim+17] of a staged batch's queries */
+int fv_gpu_batch_features(FvGpuIndex *g, FvGpuBatch *b, FvGpuModel *m, float *out,
+                          char *errbuf, size_t errlen);
+
+/* A routed run of a staged batch (staged with topb = k): features, BF16
+ * MLP, then brute (BF16 scan + exact re-rank) for queries routed to it and
+ * graph search with the per-query ef for the rest; k hits per query land in
+ * the batch outputs (fv_gpu_batch_fetch). Modes: 0 learned route + learned
+ * ef, 1 graph for all with learned ef, 2 graph for all with ef_fixed (the
+ * fixed-ef fallback), 3 brute for all. */
+typedef struct FvGpuLearnedOpts {
+    uint32_t k;            /* hits per query (= staged topb) */
+    int32_t mode;
+    uint32_t ef_fixed;     /* mode 2 */
+    uint32_t ef_min, ef_max; /* clamp of the predicted ef (ef_min >= k) */
+    float ef_scale;        /* safety factor on the predicted ef (0 = 1) */
+    float route_bias;      /* graph when the route logit exceeds it */
+    uint64_t max_visited;  /* 0 = max(200000, 400 ef) */
+    int32_t precision;     /* brute scan precision, -1 = index default */
+} FvGpuLearnedOpts;
+int fv_gpu_batch_run_learned(FvGpuIndex *g, FvGpuBatch *b, FvGpuModel *m,
+                             const FvGpuLearnedOpts *o, char *errbuf, size_t errlen);
+/* per-query decisions of the batch's last routed run (synchronous) */
+int fv_gpu_batch_routes(FvGpuIndex *g, FvGpuBatch *b, uint8_t *route, uint32_t *ef,
+                        char *errbuf, size_t errlen);
+
 #ifdef __cplusplus
 }
 #endif

@@ -44,7 +44,11 @@ EXT_SYMBOLS = ["fv_gpu_set_timing", "fv_gpu_last_stats", "fv_gpu_brute_batch_ex"
                "fv_gpu_graph_batch_ex", "fv_gpu_batch_stage", "fv_gpu_batch_run_brute",
                "fv_gpu_batch_run_graph", "fv_gpu_sync", "fv_gpu_batch_fetch",
                "fv_gpu_batch_free", "fv_gpu_copy_bench", "fv_hnsw_build", "fv_hnsw_info",
-               "fv_hnsw_export", "fv_hnsw_free", "fv_hnsw_entries", "fv_l2sq"]
+               "fv_hnsw_export", "fv_hnsw_free", "fv_hnsw_entries", "fv_l2sq",
+               "fv_gpu_model_nparams", "fv_gpu_model_create", "fv_gpu_model_free",
+               "fv_gpu_model_set_params", "fv_gpu_model_get_params", "fv_gpu_model_grad",
+               "fv_gpu_model_train", "fv_gpu_model_predict", "fv_gpu_batch_features",
+               "fv_gpu_batch_run_learned", "fv_gpu_batch_routes"]
 
 
 class BruteOpts(C.Structure):
@@ -54,6 +58,20 @@ class BruteOpts(C.Structure):
 
     def __init__(self, precision=-1, route=0, overfetch=0, sample_stride=0):
         super().__init__(precision, route, overfetch, sample_stride)
+
+
+class LearnedOpts(C.Structure):
+    """FvGpuLearnedOpts; modes: 0 learned route + ef, 1 graph with learned
+    ef, 2 graph with fixed ef, 3 brute."""
+    _fields_ = [("k", C.c_uint32), ("mode", C.c_int32), ("ef_fixed", C.c_uint32),
+                ("ef_min", C.c_uint32), ("ef_max", C.c_uint32), ("ef_scale", C.c_float),
+                ("route_bias", C.c_float), ("max_visited", C.c_uint64),
+                ("precision", C.c_int32)]
+
+    def __init__(self, k=10, mode=0, ef_fixed=400, ef_min=10, ef_max=512, ef_scale=1.0,
+                 route_bias=0.0, max_visited=0, precision=-1):
+        super().__init__(k, mode, ef_fixed, ef_min, ef_max, ef_scale, route_bias, max_visited,
+                         precision)
 
 
 class Stats(C.Structure):
@@ -130,6 +148,20 @@ def load() -> C.CDLL:
                                     C.c_int, _P, _P, C.c_int]
     lib.fv_l2sq.restype = C.c_float
     lib.fv_l2sq.argtypes = [_P, _P, C.c_int]
+    lib.fv_gpu_model_nparams.argtypes = [C.c_int]
+    lib.fv_gpu_model_create.restype = _P
+    lib.fv_gpu_model_create.argtypes = [C.c_int, C.c_int, _P, C.c_int, C.c_char_p, sz]
+    lib.fv_gpu_model_free.argtypes = [_P]
+    lib.fv_gpu_model_set_params.argtypes = [_P, _P, _P, _P, C.c_char_p, sz]
+    lib.fv_gpu_model_get_params.argtypes = [_P, _P, _P, _P, C.c_char_p, sz]
+    lib.fv_gpu_model_grad.argtypes = [_P, C.c_int, _P, _P, C.c_float, _P,
+                                      C.POINTER(C.c_double), C.c_char_p, sz]
+    lib.fv_gpu_model_train.argtypes = [_P, C.c_int, _P, _P, C.c_int, C.c_int, C.c_float,
+                                       C.c_float, C.c_uint64, _P, C.c_char_p, sz]
+    lib.fv_gpu_model_predict.argtypes = [_P, C.c_int, _P, _P, C.c_char_p, sz]
+    lib.fv_gpu_batch_features.argtypes = [_P, _P, _P, _P, C.c_char_p, sz]
+    lib.fv_gpu_batch_run_learned.argtypes = [_P, _P, _P, C.POINTER(LearnedOpts), C.c_char_p, sz]
+    lib.fv_gpu_batch_routes.argtypes = [_P, _P, _P, _P, C.c_char_p, sz]
     _lib = lib
     return lib
 

@@ -174,7 +174,18 @@ struct ScanArgs {
     uint32_t ngroups;
     const uint32_t *ones;     /* all-ones bitmap row (queries without predicate) */
     uint32_t *status;         /* per-query status bits */
+    const uint8_t *skip;      /* optional [nq]: 1 = query routed elsewhere */
 };
+
+/* true when every query of the WG's 64-query tile is skipped (uniform) */
+__device__ __forceinline__ bool tile_skipped(const ScanArgs &A, int qbase) {
+    if (A.skip == nullptr) return false;
+    for (int i = 0; i < kQT; i++) {
+        const int q = qbase + i;
+        if (q < A.nq && A.skip[q] == 0) return false;
+    }
+    return true;
+}
 
 constexpr uint32_t kSpill = 512; /* per-WG LDS candidate overflow slots */
 /* pool counters sit one 128-byte line apart: every scan WG reserves pool
@@ -366,6 +377,7 @@ __global__ __launch_bounds__(NW * 64) void k_scan_bf16(ScanArgs A) {
 
     const int tid = threadIdx.x, l = tid & 63, wave = tid >> 6;
     const int qbase = blockIdx.y * kQT;
+    if (tile_skipped(A, qbase)) return;
     const int half = A.dim / 2;
     for (int e = tid; e < 2 * S * 64; e += blockDim.x) {
         const int lane = e & 63, s = (e >> 6) % S, qb = (e >> 6) / S;
@@ -459,6 +471,7 @@ __global__ __launch_bounds__(256, 2) void k_scan_f32(ScanArgs A) {
 
     const int tid = threadIdx.x, l = tid & 63, wave = tid >> 6;
     const int qbase = blockIdx.y * kQT;
+    if (tile_skipped(A, qbase)) return;
     if (!PREPASS) cand_init(C);
 
     float tq[2], qn[2];
@@ -676,11 +689,20 @@ __global__ __launch_bounds__(256) void k_tau(const uint64_t *__restrict__ gmin,
                                              const float *__restrict__ eps,
                                              float *__restrict__ tau,
                                              uint32_t *__restrict__ pcnt,
-                                             uint32_t *__restrict__ status) {
+                                             uint32_t *__restrict__ status,
+                                             const uint8_t *__restrict__ skip) {
     __shared__ uint32_t hist[256], tmp[8], bc[8];
     __shared__ uint64_t sel[1024], wb[4];
     __shared__ uint32_t nsel;
     const int q = blockIdx.x;
+    if (skip && skip[q]) { /* routed elsewhere: no candidates, no outputs */
+        if (threadIdx.x == 0) {
+            tau[q] = -__uint_as_float(0x7f800000u);
+            pcnt[(uint64_t)q * kPcntStride] = 0;
+            status[q] = 0;
+        }
+        return;
+    }
     const uint64_t *g = gmin + (uint64_t)q * ngroups;
     float tv = __uint_as_float(0x7f800000u);
     if (ngroups >= B) {
@@ -785,12 +807,14 @@ __global__ __launch_bounds__(256) void k_select(
     uint64_t cap_pool, uint32_t topb, const float *__restrict__ eps,
     const float *__restrict__ vecs, const float *__restrict__ qf, int dim,
     FvHit *__restrict__ out, uint32_t out_stride, uint32_t *__restrict__ out_cnt,
-    uint8_t *__restrict__ out_flag, uint32_t *__restrict__ status) {
+    uint8_t *__restrict__ out_flag, uint32_t *__restrict__ status,
+    const uint8_t *__restrict__ skip) {
     __shared__ uint64_t sel[kSelMax], wb[4];
     __shared__ uint32_t hist[256], tmp[8], bc[8];
     __shared__ uint32_t nsel;
     extern __shared__ float qsh[];
     const int q = blockIdx.x, t = threadIdx.x;
+    if (skip && skip[q]) return; /* the graph search writes this query */
     const uint64_t *pk = pool + (uint64_t)q * cap_pool;
     const uint32_t ntot = pcnt[(uint64_t)q * kPcntStride];
     const uint32_t n = (uint32_t)min((uint64_t)ntot, cap_pool);
@@ -898,6 +922,11 @@ void free_batch_buffers(FvGpuBatch *b) {
     b->d_flag.release();
     b->h_out.release();
     b->d_visit.release();
+    b->d_progcnt.release();
+    b->d_feat.release();
+    b->d_y.release();
+    b->d_route.release();
+    b->d_efq.release();
 }
 
 int stage_batch(FvGpuIndex *g, FvGpuBatch *b, int nq, const float *queries,
@@ -1017,7 +1046,7 @@ int stage_batch(FvGpuIndex *g, FvGpuBatch *b, int nq, const float *queries,
 }
 
 /* scan precision actually used for a request */
-static int scan_precision(const FvGpuIndex *g, const FvGpuBruteOpts *o) {
+int scan_precision(const FvGpuIndex *g, const FvGpuBruteOpts *o) {
     int p = (o && o->precision >= 0) ? o->precision : g->precision;
     if (p == FV_GPU_PREC_SQ8) p = FV_GPU_PREC_BF16;
     if (p == FV_GPU_PREC_BF16 && (g->d_vbf == nullptr || g->dim % 128 != 0 || g->dim > 1024))
@@ -1025,12 +1054,53 @@ static int scan_precision(const FvGpuIndex *g, const FvGpuBruteOpts *o) {
     return p;
 }
 
+/* Phase 1 of a run: predicate bitmaps (when `bitmap`) and the query
+ * preparation for scan precision `prec`, one launch on stream s. */
+hipError_t launch_phase1(FvGpuIndex *g, FvGpuBatch *b, int prec, bool bitmap, hipStream_t s) {
+    QAux qa = qaux_of(b);
+    {
+        /* predicate programs -> row bitmaps, every run (the gather route
+         * evaluates the predicate on the enumerated rows instead), fused
+         * with the query preparation */
+        Phase1Args P = {};
+        P.ops = b->d_ops;
+        P.poff = b->d_poff;
+        P.plen = b->d_plen;
+        P.vals = b->d_vals;
+        P.cols = g->d_cols;
+        P.count = g->count;
+        P.words = g->count_pad / 32;
+        P.bitmap = b->d_bitmap.as<uint32_t>();
+        P.nprog = b->nprog;
+        P.nops = b->nops;
+        P.nvals = b->nvals;
+        P.ncols = g->ncols;
+        P.nbm = (bitmap && b->nprog > 0) ? (uint32_t)((g->count_pad + kBmRows - 1) / kBmRows) : 0;
+        P.q = b->d_q;
+        P.nq = b->nq;
+        P.nqp = b->nqt * kQT;
+        P.dim = g->dim;
+        P.qbf = b->d_qbf.as<uint16_t>();
+        P.qnorm = qa.qnorm;
+        P.eps = qa.eps;
+        P.xmax_norm = g->xmax_norm;
+        P.vmin = g->vmin;
+        P.vmax = g->vmax;
+        P.all_int = g->all_int ? 1 : 0;
+        P.mode = prec == FV_GPU_PREC_BF16 ? 1 : 0;
+        hipLaunchKernelGGL(k_phase1, dim3(P.nbm + (uint32_t)(P.nqp + 3) / 4), dim3(256), 0, s, P);
+        return hipGetLastError();
+    }
+}
+
 /* Enqueue the brute pipeline. stride_override > 0 forces the pre-pass
  * sample stride (1 = every row: the tightest threshold). */
-static int run_brute(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
-                     uint32_t stride_override, char *errbuf, size_t errlen) {
+int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
+                 uint32_t stride_override, const uint8_t *skip, bool have_phase1,
+                 char *errbuf, size_t errlen) {
     hipStream_t s = b->stream;
     g->ev_stream = s;
+    b->out_stride = b->topb; /* a graph run may have widened it; d_out holds nq x topb */
     const int prec = scan_precision(g, o);
     const int nq = b->nq, nqp = b->nqt * kQT, dim = g->dim;
     const uint32_t B = b->topb;
@@ -1054,39 +1124,7 @@ static int run_brute(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
     st.route = route;
 
     ev_record(g, 0);
-    {
-        /* predicate programs -> row bitmaps, every run (the gather route
-         * evaluates the predicate on the enumerated rows instead), fused
-         * with the query preparation */
-        Phase1Args P = {};
-        P.ops = b->d_ops;
-        P.poff = b->d_poff;
-        P.plen = b->d_plen;
-        P.vals = b->d_vals;
-        P.cols = g->d_cols;
-        P.count = g->count;
-        P.words = g->count_pad / 32;
-        P.bitmap = b->d_bitmap.as<uint32_t>();
-        P.nprog = b->nprog;
-        P.nops = b->nops;
-        P.nvals = b->nvals;
-        P.ncols = g->ncols;
-        P.nbm = (route == 1 && b->nprog > 0) ? (uint32_t)((g->count_pad + kBmRows - 1) / kBmRows) : 0;
-        P.q = b->d_q;
-        P.nq = nq;
-        P.nqp = nqp;
-        P.dim = dim;
-        P.qbf = b->d_qbf.as<uint16_t>();
-        P.qnorm = qa.qnorm;
-        P.eps = qa.eps;
-        P.xmax_norm = g->xmax_norm;
-        P.vmin = g->vmin;
-        P.vmax = g->vmax;
-        P.all_int = g->all_int ? 1 : 0;
-        P.mode = prec == FV_GPU_PREC_BF16 ? 1 : 0;
-        hipLaunchKernelGGL(k_phase1, dim3(P.nbm + (uint32_t)(nqp + 3) / 4), dim3(256), 0, s, P);
-        PV_TRY(hipGetLastError());
-    }
+    if (!have_phase1) PV_TRY(launch_phase1(g, b, prec, route == 1, s));
     ev_record(g, 6);
     ev_record(g, 1);
 
@@ -1117,7 +1155,7 @@ static int run_brute(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
                            b->cap_pool, B, (const float *)nullptr,
                            g->d_vecs, b->d_q, dim, b->d_out.as<FvHit>(),
                            b->out_stride, b->d_cnt.as<uint32_t>(),
-                           b->d_flag.as<uint8_t>(), qa.status);
+                           b->d_flag.as<uint8_t>(), qa.status, skip);
         PV_TRY(hipGetLastError());
         ev_record(g, 5);
         st.scan_rows = b->sum_total;
@@ -1166,6 +1204,7 @@ static int run_brute(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
     A.ngroups = b->ngroups;
     A.ones = g->d_ones;
     A.status = qa.status;
+    A.skip = skip;
 
     const bool bf = prec == FV_GPU_PREC_BF16;
     /* 4 waves per WG up to 256 dims (2 WGs per CU), 8 waves above (the
@@ -1199,7 +1238,7 @@ static int run_brute(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
     ev_record(g, 2);
     hipLaunchKernelGGL(k_tau, dim3(nq), dim3(256), 0, s, b->d_gmin.as<uint64_t>(),
                        b->ngroups, B, qa.eps, qa.tau, b->d_pcnt.as<uint32_t>(),
-                       qa.status);
+                       qa.status, skip);
     PV_TRY(hipGetLastError());
     ev_record(g, 3);
     {
@@ -1222,12 +1261,17 @@ static int run_brute(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
                        b->d_pool.as<uint64_t>(), b->d_pcnt.as<uint32_t>(),
                        b->cap_pool, B, qa.eps, g->d_vecs, b->d_q, dim,
                        b->d_out.as<FvHit>(), b->out_stride,
-                       b->d_cnt.as<uint32_t>(), b->d_flag.as<uint8_t>(), qa.status);
+                       b->d_cnt.as<uint32_t>(), b->d_flag.as<uint8_t>(), qa.status, skip);
     PV_TRY(hipGetLastError());
     ev_record(g, 5);
     st.scan_rows = g->count;
     st.scan_bytes = g->count * (uint64_t)dim * (bf ? 2 : 4) * (uint64_t)b->nqt;
     return 0;
+}
+
+static int run_brute(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
+                     uint32_t stride_override, char *errbuf, size_t errlen) {
+    return run_brute_ex(g, b, o, stride_override, nullptr, false, errbuf, errlen);
 }
 
 /* copy outputs and per-query status to pinned host memory (synchronous) */

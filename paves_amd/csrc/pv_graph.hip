@@ -65,6 +65,9 @@ struct GraphArgs {
     uint32_t *out_cnt;
     uint8_t *out_exh;
     uint32_t *status;    /* per-query status words, cleared by the search */
+    const uint8_t *route; /* optional: queries with route 0 are left alone */
+    const uint32_t *efq;  /* optional per-query ef (<= ef, the LDS sizing) */
+    uint32_t out_k;       /* hits written per query (stride); 0 = ef */
     uint32_t *dbg;       /* [nq][4] iterations, visited, results, overflow */
 };
 
@@ -234,6 +237,8 @@ __device__ __forceinline__ void dist16(const GraphArgs &A, const float *qsh,
 __global__ __launch_bounds__(64) void k_graph(GraphArgs A) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     const int q = blockIdx.x, l = threadIdx.x;
+    if (A.route && A.route[q] == 0) return; /* routed to the brute path */
+    const uint32_t efq = A.efq ? max(1u, min(A.efq[q], A.ef)) : A.ef;
     const int D = A.dim;
     float *qsh = (float *)smem;                                   /* [D] */
     uint64_t *F = (uint64_t *)(smem + (((size_t)D * 4 + 15) & ~(size_t)15)); /* [fcap] */
@@ -285,7 +290,7 @@ __global__ __launch_bounds__(64) void k_graph(GraphArgs A) {
         }
         const uint64_t c0 = F[0];
         const uint64_t worst = nR > 0 ? R[nR - 1] : ~0ull;
-        const bool full = nR >= (int)A.ef;
+        const bool full = nR >= (int)efq;
         if (full && key_dist(c0) > key_dist(worst)) break;
         if (visited >= A.max_visited) {
             exh = 1;
@@ -398,17 +403,19 @@ __global__ __launch_bounds__(64) void k_graph(GraphArgs A) {
             __syncthreads();
         }
         nF = wave_merge(F, nF, w, nwF, mF, A.fcap);
-        if (mR > 0) nR = wave_merge(R, nR, 0, nwR, mR, (int)A.ef);
+        if (mR > 0) nR = wave_merge(R, nR, 0, nwR, mR, (int)efq);
     }
 
-    for (int i = l; i < nR; i += 64) {
+    const uint32_t ostride = A.out_k ? A.out_k : A.ef;
+    const int nout = min(nR, (int)ostride);
+    for (int i = l; i < nout; i += 64) {
         FvHit h;
         h.node = key_row(R[i]);
         h.dist = key_dist(R[i]);
-        A.out[(uint64_t)q * A.ef + i] = h;
+        A.out[(uint64_t)q * ostride + i] = h;
     }
     if (l == 0) {
-        A.out_cnt[q] = (uint32_t)nR;
+        A.out_cnt[q] = (uint32_t)nout;
         A.out_exh[q] = (uint8_t)exh;
         A.status[q] = 0;
         if (A.dbg) {
@@ -425,8 +432,8 @@ size_t graph_shmem(int dim, int fcap, uint32_t ef) {
            2 * kMaxNew * 8 + kHashSize * 4 + kMaxNew * 4 + kMaxNew * 4 + 12 * 4;
 }
 
-int run_graph_opts(FvGpuIndex *g, FvGpuBatch *b, uint32_t ef, uint64_t max_visited,
-                   int wmax, int fcap, uint32_t *dbg, char *errbuf, size_t errlen) {
+int run_graph_ex(FvGpuIndex *g, FvGpuBatch *b, uint32_t ef, uint64_t max_visited, int wmax,
+                 int fcap, uint32_t *dbg, const GraphExtra &x, char *errbuf, size_t errlen) {
     if (g->d_nbr == nullptr) PV_FAIL("gpu: graph not uploaded");
     if (ef < 1 || ef > FV_GPU_MAX_EF) PV_FAIL("gpu: bad graph batch (nq=%d ef=%u)", b->nq, ef);
     if (b->max_entry >= g->count)
@@ -441,9 +448,10 @@ int run_graph_opts(FvGpuIndex *g, FvGpuBatch *b, uint32_t ef, uint64_t max_visit
     const uint64_t words = g->count_pad / 32;
     const uint64_t vwords = (words + 3) & ~3ull;
     PV_TRY(b->d_visit.ensure((size_t)b->nq * vwords * 4));
-    b->out_stride = ef;
+    const uint32_t ostride = x.out_k ? x.out_k : ef;
+    b->out_stride = ostride;
     b->last_route = 3;
-    PV_TRY(b->d_out.ensure((size_t)b->nq * ef * sizeof(FvHit)));
+    PV_TRY(b->d_out.ensure((size_t)b->nq * ostride * sizeof(FvHit)));
     GraphArgs A;
     A.vecs = g->d_vecs;
     A.nbr = g->d_nbr;
@@ -465,19 +473,27 @@ int run_graph_opts(FvGpuIndex *g, FvGpuBatch *b, uint32_t ef, uint64_t max_visit
     A.out_cnt = b->d_cnt.as<uint32_t>();
     A.out_exh = b->d_flag.as<uint8_t>();
     A.status = qaux_of(b).status;
+    A.route = x.route;
+    A.efq = x.efq;
+    A.out_k = x.out_k;
     A.dbg = dbg;
     if (shm > 64 * 1024)
         PV_TRY(hipFuncSetAttribute((const void *)k_graph,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
     g->ev_stream = b->stream;
     ev_record(g, 0);
-    PV_TRY(launch_bitmap(g, b, b->stream));
+    if (!x.bitmap_ready) PV_TRY(launch_bitmap(g, b, b->stream));
     hipLaunchKernelGGL(k_graph, dim3(b->nq), dim3(64), shm, b->stream, A);
     PV_TRY(hipGetLastError());
     ev_record(g, 5);
     memset(&g->last, 0, sizeof(g->last));
     g->last.route = 3;
     return 0;
+}
+
+int run_graph_opts(FvGpuIndex *g, FvGpuBatch *b, uint32_t ef, uint64_t max_visited,
+                   int wmax, int fcap, uint32_t *dbg, char *errbuf, size_t errlen) {
+    return run_graph_ex(g, b, ef, max_visited, wmax, fcap, dbg, GraphExtra(), errbuf, errlen);
 }
 
 int run_graph(FvGpuIndex *g, FvGpuBatch *b, uint32_t ef, uint64_t max_visited,

@@ -174,6 +174,9 @@ struct FvGpuBatch {
     int last_route = 0;
     hipStream_t stream = nullptr; /* one of the index's batch streams */
     pv::DevBuf d_visit;   /* graph: overflow visited bitmaps */
+    pv::DevBuf d_progcnt; /* [nprog] passing rows per program (learned features) */
+    pv::DevBuf d_feat, d_y; /* learned run: features [nq][F], model outputs [nq][2] */
+    pv::DevBuf d_route, d_efq; /* learned run: route u8 [nq] (1 = graph), ef u32 [nq] */
 };
 
 struct FvGpuIndex {
@@ -255,6 +258,23 @@ void free_batch_buffers(FvGpuBatch *b);
 /* pv_graph.hip */
 int run_graph(FvGpuIndex *g, FvGpuBatch *b, uint32_t ef, uint64_t max_visited,
               char *errbuf, size_t errlen);
+/* options of a graph run inside a routed (learned) run */
+struct GraphExtra {
+    const uint8_t *route = nullptr;  /* device [nq]: 0 = brute, leave the query alone */
+    const uint32_t *efq = nullptr;   /* device [nq]: per-query ef (<= ef) */
+    uint32_t out_k = 0;              /* hits per query written (0 = ef) */
+    bool bitmap_ready = false;       /* predicate bitmaps already built this run */
+};
+int run_graph_ex(FvGpuIndex *g, FvGpuBatch *b, uint32_t ef, uint64_t max_visited, int wmax,
+                 int fcap, uint32_t *dbg, const GraphExtra &x, char *errbuf, size_t errlen);
+/* pv_brute.hip: the brute pipeline; skip (device [nq], may be NULL) marks
+ * queries it must leave alone; have_phase1 = bitmaps and query prep done */
+int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
+                 uint32_t stride_override, const uint8_t *skip, bool have_phase1,
+                 char *errbuf, size_t errlen);
+hipError_t launch_phase1(FvGpuIndex *g, FvGpuBatch *b, int prec, bool bitmap, hipStream_t s);
+/* precision the brute scan runs in for these options (FP32 or BF16) */
+int scan_precision(const FvGpuIndex *g, const FvGpuBruteOpts *o);
 
 /* timing helpers (pv_index.hip) */
 void ev_record(FvGpuIndex *g, int i);

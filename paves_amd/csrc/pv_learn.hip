@@ -1,0 +1,731 @@
+/*
+ * pv_learn.hip — the learned per-query beam width and brute-vs-graph router.
+ *
+ * The reference picks ef from a fixed policy (ef = max(k, ef_search x scale)
+ * with scale in {1, 0.8, 0.4} by selectivity, exec.c:422-424) and routes by
+ * a cost race (planner.c:185-346). Here a small MLP predicts, per query,
+ * log(ef) and a brute-vs-graph logit from
+ *   x = [ q (dim) | log10 selectivity | distances to the 16 nearest centroids ]
+ * (dim + 17 inputs; 145 at dim 128), two hidden layers of 64 (ReLU), two
+ * outputs. Inputs are standardised with the training set's mean and std.
+ *
+ * Kernels:
+ *   k_prog_count   passing rows of each predicate program (bitmap popcount)
+ *   k_features     one wave per query: the feature vector above
+ *   k_mlp_infer    one wave per query, lane j = hidden unit j, BF16 weights
+ *                  with FP32 accumulation
+ *   k_train_fwd    one wave per training sample: forward pass, loss and the
+ *                  back-propagated deltas of each layer (FP32)
+ *   k_train_wgrad  one thread per parameter: sum over the minibatch of
+ *                  delta x activation (fixed order, deterministic)
+ *   k_adam         Adam update
+ * Parameter layout (FP32, nparams = 64(F+1) + 64*65 + 2*65):
+ *   W1[64][F] b1[64] W2[64][64] b2[64] W3[2][64] b3[2].
+ * Loss per sample: (y0 - log ef)^2 * valid + lambda * BCE(y1, route), mean
+ * over the minibatch.
+ */
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <random>
+#include <vector>
+
+#include "pv_device.h"
+
+namespace pv {
+
+constexpr int kHid = 64;
+constexpr int kNear = 16;
+constexpr int kMaxFeat = 1024 + 17;
+
+__host__ __device__ inline int model_nfeat(int dim) { return dim + 1 + kNear; }
+__host__ __device__ inline int model_nparams(int F) { return kHid * F + kHid + kHid * kHid + kHid + 2 * kHid + 2; }
+
+struct MlpLayout {
+    int F;
+    int w1, b1, w2, b2, w3, b3, n;
+};
+__host__ __device__ inline MlpLayout mlp_layout(int F) {
+    MlpLayout L;
+    L.F = F;
+    L.w1 = 0;
+    L.b1 = L.w1 + kHid * F;
+    L.w2 = L.b1 + kHid;
+    L.b2 = L.w2 + kHid * kHid;
+    L.w3 = L.b2 + kHid;
+    L.b3 = L.w3 + 2 * kHid;
+    L.n = L.b3 + 2;
+    return L;
+}
+
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+__device__ __forceinline__ float wave_sum(float v) {
+    for (int o = 32; o; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+/* ---------------- features ---------------- */
+
+/* one 256-thread block per program: popcount of its bitmap row */
+__global__ __launch_bounds__(256) void k_prog_count(const uint32_t *__restrict__ bitmap,
+                                                    uint64_t words, uint32_t *__restrict__ cnt) {
+    __shared__ uint32_t part[4];
+    const uint32_t *row = bitmap + (uint64_t)blockIdx.x * words;
+    uint32_t c = 0;
+    for (uint64_t w = threadIdx.x; w < words; w += blockDim.x) c += __popc(row[w]);
+    for (int o = 32; o; o >>= 1) c += __shfl_xor(c, o);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) cnt[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
+}
+
+/*
+ * One wave per query (4 per block). Distances to the centroids are exact
+ * FP32 sums of squared differences; the 16 smallest are taken by repeated
+ * wave-wide minimum (each lane keeps its own candidates sorted).
+ */
+__global__ __launch_bounds__(256) void k_features(const float *__restrict__ q, int nq, int dim,
+                                                  const int32_t *__restrict__ pofq,
+                                                  const uint32_t *__restrict__ pcnt,
+                                                  uint64_t count, const float *__restrict__ cent,
+                                                  int ncent, float *__restrict__ out) {
+    const int qi = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int l = threadIdx.x & 63;
+    if (qi >= nq) return;
+    const int F = model_nfeat(dim);
+    const float *qv = q + (uint64_t)qi * dim;
+    float *o = out + (uint64_t)qi * F;
+    for (int j = l; j < dim; j += 64) o[j] = qv[j];
+    if (l == 0) {
+        const int p = pofq[qi];
+        const double pass = p < 0 ? (double)count : (double)pcnt[p];
+        o[dim] = (float)log10(fmax(pass, 1.0) / (double)count);
+    }
+    /* each lane: distances to centroids l, l+64, ... ; keep the 16 best */
+    float best[kNear];
+#pragma unroll
+    for (int k = 0; k < kNear; k++) best[k] = __uint_as_float(0x7f800000u);
+    for (int c = l; c < ncent; c += 64) {
+        const float *cv = cent + (uint64_t)c * dim;
+        float s = 0.f;
+        for (int j = 0; j < dim; j++) {
+            const float d = qv[j] - cv[j];
+            s = fmaf(d, d, s);
+        }
+        /* insertion into the lane's sorted list */
+        float v = s;
+#pragma unroll
+        for (int k = 0; k < kNear; k++) {
+            const float lo = fminf(best[k], v), hi = fmaxf(best[k], v);
+            best[k] = lo;
+            v = hi;
+        }
+    }
+    /* 16 rounds of wave minimum over the lanes' heads */
+    int head = 0;
+    for (int r = 0; r < kNear; r++) {
+        float h = __uint_as_float(0x7f800000u);
+#pragma unroll
+        for (int k = 0; k < kNear; k++)
+            if (k == head) h = best[k];
+        float m = h;
+        for (int off = 32; off; off >>= 1) m = fminf(m, __shfl_xor(m, off));
+        /* the lowest lane holding the minimum advances its head */
+        const uint64_t who = __ballot(h == m);
+        const int lane = __ffsll((unsigned long long)who) - 1;
+        if (l == lane) head++;
+        if (l == 0) o[dim + 1 + r] = sqrtf(m);
+    }
+}
+
+/* ---------------- inference ---------------- */
+
+/* one wave per query; lane j computes hidden unit j of both layers */
+__global__ __launch_bounds__(256) void k_mlp_infer(const float *__restrict__ x, int n, int F,
+                                                   const uint16_t *__restrict__ wbf,
+                                                   const float *__restrict__ mean,
+                                                   const float *__restrict__ istd,
+                                                   float *__restrict__ y) {
+    __shared__ float xs[4][kMaxFeat];
+    __shared__ float hs[4][kHid];
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int s = blockIdx.x * 4 + w;
+    const bool act = s < n; /* every wave reaches every barrier */
+    const MlpLayout L = mlp_layout(F);
+    for (int j = l; j < F; j += 64) xs[w][j] = act ? (x[(uint64_t)s * F + j] - mean[j]) * istd[j] : 0.f;
+    __syncthreads();
+    float a = bf2f(wbf[L.b1 + l]);
+    const uint16_t *r1 = wbf + L.w1 + (uint64_t)l * F;
+    for (int i = 0; i < F; i++) a = fmaf(bf2f(r1[i]), xs[w][i], a);
+    hs[w][l] = fmaxf(a, 0.f);
+    __syncthreads();
+    float b = bf2f(wbf[L.b2 + l]);
+    const uint16_t *r2 = wbf + L.w2 + (uint64_t)l * kHid;
+    for (int i = 0; i < kHid; i++) b = fmaf(bf2f(r2[i]), hs[w][i], b);
+    b = fmaxf(b, 0.f);
+    const float y0 = wave_sum(bf2f(wbf[L.w3 + l]) * b);
+    const float y1 = wave_sum(bf2f(wbf[L.w3 + kHid + l]) * b);
+    if (act && l == 0) {
+        y[(uint64_t)s * 2 + 0] = y0 + bf2f(wbf[L.b3 + 0]);
+        y[(uint64_t)s * 2 + 1] = y1 + bf2f(wbf[L.b3 + 1]);
+    }
+}
+
+/* ---------------- training ---------------- */
+
+struct TrainArgs {
+    const float *x;       /* [n][F] raw features */
+    const float *t;       /* [n][3]: log ef, route (0/1), ef valid (0/1) */
+    const int32_t *idx;   /* minibatch sample ids [B] */
+    int B, F;
+    const float *w;       /* FP32 params */
+    const float *mean, *istd;
+    float lambda;         /* weight of the routing loss */
+    float *xh;            /* [B][F] standardised inputs */
+    float *a1, *a2;       /* [B][64] post-ReLU activations */
+    float *d1, *d2;       /* [B][64] deltas at the pre-activations */
+    float *d3;            /* [B][2] output deltas */
+    float *loss;          /* [B] per-sample loss */
+};
+
+/* one wave per sample of the minibatch */
+__global__ __launch_bounds__(256) void k_train_fwd(TrainArgs A) {
+    __shared__ float xs[4][kMaxFeat];
+    __shared__ float h1s[4][kHid], d2s[4][kHid];
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int b = blockIdx.x * 4 + w;
+    const bool act = b < A.B; /* every wave reaches every barrier */
+    const int F = A.F;
+    const MlpLayout L = mlp_layout(F);
+    const int s = act ? A.idx[b] : 0;
+    for (int j = l; j < F; j += 64) {
+        const float v = act ? (A.x[(uint64_t)s * F + j] - A.mean[j]) * A.istd[j] : 0.f;
+        xs[w][j] = v;
+        if (act) A.xh[(uint64_t)b * F + j] = v;
+    }
+    __syncthreads();
+    float z1 = A.w[L.b1 + l];
+    for (int i = 0; i < F; i++) z1 = fmaf(A.w[L.w1 + l * F + i], xs[w][i], z1);
+    const float h1 = fmaxf(z1, 0.f);
+    h1s[w][l] = h1;
+    __syncthreads();
+    float z2 = A.w[L.b2 + l];
+    for (int i = 0; i < kHid; i++) z2 = fmaf(A.w[L.w2 + l * kHid + i], h1s[w][i], z2);
+    const float h2 = fmaxf(z2, 0.f);
+    const float y0 = wave_sum(A.w[L.w3 + l] * h2) + A.w[L.b3 + 0];
+    const float y1 = wave_sum(A.w[L.w3 + kHid + l] * h2) + A.w[L.b3 + 1];
+    const float t0 = act ? A.t[(uint64_t)s * 3 + 0] : 0.f;
+    const float t1 = act ? A.t[(uint64_t)s * 3 + 1] : 0.f;
+    const float valid = act ? A.t[(uint64_t)s * 3 + 2] : 0.f;
+    /* d loss / d y: squared error on log ef, logistic loss on the route */
+    const float e0 = y0 - t0;
+    const float sig = 1.f / (1.f + expf(-y1));
+    const float g0 = 2.f * e0 * valid;
+    const float g1 = A.lambda * (sig - t1);
+    const float d2 = (A.w[L.w3 + l] * g0 + A.w[L.w3 + kHid + l] * g1) * (z2 > 0.f ? 1.f : 0.f);
+    d2s[w][l] = d2;
+    __syncthreads();
+    float bd = 0.f;
+    for (int i = 0; i < kHid; i++) bd = fmaf(A.w[L.w2 + i * kHid + l], d2s[w][i], bd);
+    const float d1 = bd * (z1 > 0.f ? 1.f : 0.f);
+    if (!act) return;
+    A.a1[(uint64_t)b * kHid + l] = h1;
+    A.a2[(uint64_t)b * kHid + l] = h2;
+    A.d1[(uint64_t)b * kHid + l] = d1;
+    A.d2[(uint64_t)b * kHid + l] = d2;
+    if (l == 0) {
+        A.d3[(uint64_t)b * 2 + 0] = g0;
+        A.d3[(uint64_t)b * 2 + 1] = g1;
+        /* softplus(y1) - t1 y1, the logistic loss in a stable form */
+        const float sp = fmaxf(y1, 0.f) + log1pf(expf(-fabsf(y1)));
+        A.loss[b] = e0 * e0 * valid + A.lambda * (sp - t1 * y1);
+    }
+}
+
+/* one thread per parameter: mean over the minibatch of delta x input */
+__global__ __launch_bounds__(256) void k_train_wgrad(TrainArgs A, float *__restrict__ grad) {
+    const int F = A.F;
+    const MlpLayout L = mlp_layout(F);
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= L.n) return;
+    const int B = A.B;
+    float s = 0.f;
+    if (p < L.b1) {
+        const int i = p / F, j = p % F;
+        for (int b = 0; b < B; b++) s = fmaf(A.d1[(uint64_t)b * kHid + i], A.xh[(uint64_t)b * F + j], s);
+    } else if (p < L.w2) {
+        const int i = p - L.b1;
+        for (int b = 0; b < B; b++) s += A.d1[(uint64_t)b * kHid + i];
+    } else if (p < L.b2) {
+        const int i = (p - L.w2) / kHid, j = (p - L.w2) % kHid;
+        for (int b = 0; b < B; b++) s = fmaf(A.d2[(uint64_t)b * kHid + i], A.a1[(uint64_t)b * kHid + j], s);
+    } else if (p < L.w3) {
+        const int i = p - L.b2;
+        for (int b = 0; b < B; b++) s += A.d2[(uint64_t)b * kHid + i];
+    } else if (p < L.b3) {
+        const int k = (p - L.w3) / kHid, j = (p - L.w3) % kHid;
+        for (int b = 0; b < B; b++) s = fmaf(A.d3[(uint64_t)b * 2 + k], A.a2[(uint64_t)b * kHid + j], s);
+    } else {
+        const int k = p - L.b3;
+        for (int b = 0; b < B; b++) s += A.d3[(uint64_t)b * 2 + k];
+    }
+    grad[p] = s / (float)B;
+}
+
+__global__ void k_adam(float *__restrict__ w, const float *__restrict__ g, float *__restrict__ m,
+                       float *__restrict__ v, int n, float lr, float b1, float b2, float eps,
+                       float c1, float c2) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const float gg = g[p];
+    const float mm = b1 * m[p] + (1.f - b1) * gg;
+    const float vv = b2 * v[p] + (1.f - b2) * gg * gg;
+    m[p] = mm;
+    v[p] = vv;
+    w[p] -= lr * (mm / c1) / (sqrtf(vv / c2) + eps);
+}
+
+/*
+ * Per-query decision from the model outputs y = (log ef, route logit):
+ * ef = clamp(ceil(exp(y0) * scale), ef_min, ef_max); route 1 (graph) when
+ * y1 > bias. mode 1 forces the graph with the learned ef, mode 2 the graph
+ * with a fixed ef, mode 3 the brute path.
+ */
+__global__ void k_route(const float *__restrict__ y, int nq, int mode, uint32_t ef_fixed,
+                        uint32_t ef_min, uint32_t ef_max, float scale, float bias,
+                        uint8_t *__restrict__ route, uint32_t *__restrict__ efq) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    uint32_t ef = ef_fixed;
+    uint8_t r = 1;
+    if (mode == 3) {
+        r = 0;
+    } else if (mode == 0 || mode == 1) {
+        const float e = expf(fminf(y[2 * q], 20.f)) * scale;
+        ef = (uint32_t)fminf(ceilf(e), (float)ef_max);
+        r = (mode == 1 || y[2 * q + 1] > bias) ? 1 : 0;
+    }
+    efq[q] = min(max(ef, ef_min), ef_max);
+    route[q] = r;
+}
+
+/* FP32 -> BF16 (round to nearest even) copy of the parameters */
+__global__ void k_to_bf16(const float *__restrict__ w, uint16_t *__restrict__ o, int n) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) o[p] = f2bf(w[p]);
+}
+
+} // namespace pv
+
+using namespace pv;
+
+struct FvGpuModel {
+    int device = -1;
+    int dim = 0, F = 0, ncent = 0, np = 0;
+    hipStream_t stream = nullptr;
+    float *d_cent = nullptr;
+    float *d_w = nullptr, *d_m = nullptr, *d_v = nullptr, *d_g = nullptr;
+    uint16_t *d_wbf = nullptr;
+    float *d_mean = nullptr, *d_istd = nullptr;
+    std::vector<float> h_mean, h_istd;
+    int adam_t = 0;
+    /* training scratch */
+    DevBuf d_x, d_t, d_idx, d_xh, d_a1, d_a2, d_d1, d_d2, d_d3, d_loss, d_y;
+};
+
+namespace {
+
+int model_sync_bf16(FvGpuModel *m, char *errbuf, size_t errlen) {
+    hipLaunchKernelGGL(k_to_bf16, dim3((m->np + 255) / 256), dim3(256), 0, m->stream, m->d_w,
+                       m->d_wbf, m->np);
+    PV_TRY(hipGetLastError());
+    return 0;
+}
+
+int ensure_train(FvGpuModel *m, int n, int B, char *errbuf, size_t errlen) {
+    PV_TRY(m->d_x.ensure((size_t)n * m->F * 4));
+    PV_TRY(m->d_t.ensure((size_t)n * 3 * 4));
+    PV_TRY(m->d_idx.ensure((size_t)B * 4));
+    PV_TRY(m->d_xh.ensure((size_t)B * m->F * 4));
+    PV_TRY(m->d_a1.ensure((size_t)B * kHid * 4));
+    PV_TRY(m->d_a2.ensure((size_t)B * kHid * 4));
+    PV_TRY(m->d_d1.ensure((size_t)B * kHid * 4));
+    PV_TRY(m->d_d2.ensure((size_t)B * kHid * 4));
+    PV_TRY(m->d_d3.ensure((size_t)B * 2 * 4));
+    PV_TRY(m->d_loss.ensure((size_t)B * 4));
+    return 0;
+}
+
+TrainArgs train_args(FvGpuModel *m, int B, float lambda) {
+    TrainArgs A;
+    A.x = m->d_x.as<float>();
+    A.t = m->d_t.as<float>();
+    A.idx = m->d_idx.as<int32_t>();
+    A.B = B;
+    A.F = m->F;
+    A.w = m->d_w;
+    A.mean = m->d_mean;
+    A.istd = m->d_istd;
+    A.lambda = lambda;
+    A.xh = m->d_xh.as<float>();
+    A.a1 = m->d_a1.as<float>();
+    A.a2 = m->d_a2.as<float>();
+    A.d1 = m->d_d1.as<float>();
+    A.d2 = m->d_d2.as<float>();
+    A.d3 = m->d_d3.as<float>();
+    A.loss = m->d_loss.as<float>();
+    return A;
+}
+
+/* forward + backward over samples idx[0..B) into m->d_g; mean loss out */
+int grad_step(FvGpuModel *m, int B, float lambda, double *loss, char *errbuf, size_t errlen) {
+    TrainArgs A = train_args(m, B, lambda);
+    hipLaunchKernelGGL(k_train_fwd, dim3((B + 3) / 4), dim3(256), 0, m->stream, A);
+    hipLaunchKernelGGL(k_train_wgrad, dim3((m->np + 255) / 256), dim3(256), 0, m->stream, A,
+                       m->d_g);
+    PV_TRY(hipGetLastError());
+    if (loss) {
+        std::vector<float> l(B);
+        PV_TRY(hipMemcpyAsync(l.data(), A.loss, (size_t)B * 4, hipMemcpyDeviceToHost, m->stream));
+        PV_TRY(hipStreamSynchronize(m->stream));
+        double s = 0.0;
+        for (float v : l) s += v;
+        *loss = s / B;
+    }
+    return 0;
+}
+
+} // namespace
+
+extern "C" int fv_gpu_model_nparams(int dim) { return dim > 0 ? model_nparams(model_nfeat(dim)) : -1; }
+
+extern "C" FvGpuModel *fv_gpu_model_create(int device, int dim, const float *centroids,
+                                           int ncent, char *errbuf, size_t errlen) {
+    if (dim < 1 || dim > 1024 || centroids == nullptr || ncent < kNear || ncent > 65536) {
+        snprintf(errbuf, errlen, "gpu: model: bad arguments (dim=%d ncent=%d)", dim, ncent);
+        return nullptr;
+    }
+    FvGpuModel *m = new FvGpuModel();
+    m->device = device < 0 ? 0 : device;
+    m->dim = dim;
+    m->F = model_nfeat(dim);
+    m->ncent = ncent;
+    m->np = model_nparams(m->F);
+    auto fail = [&](hipError_t e, const char *what) {
+        snprintf(errbuf, errlen, "gpu: model: %s: %s", what, hipGetErrorString(e));
+        fv_gpu_model_free(m);
+        return (FvGpuModel *)nullptr;
+    };
+    hipError_t e;
+    if ((e = hipSetDevice(m->device)) != hipSuccess) return fail(e, "hipSetDevice");
+    if ((e = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking)) != hipSuccess)
+        return fail(e, "stream");
+    const size_t np = (size_t)m->np;
+    if ((e = hipMalloc(&m->d_cent, (size_t)ncent * dim * 4)) != hipSuccess) return fail(e, "alloc");
+    if ((e = hipMalloc(&m->d_w, np * 4)) != hipSuccess) return fail(e, "alloc");
+    if ((e = hipMalloc(&m->d_m, np * 4)) != hipSuccess) return fail(e, "alloc");
+    if ((e = hipMalloc(&m->d_v, np * 4)) != hipSuccess) return fail(e, "alloc");
+    if ((e = hipMalloc(&m->d_g, np * 4)) != hipSuccess) return fail(e, "alloc");
+    if ((e = hipMalloc(&m->d_wbf, np * 2)) != hipSuccess) return fail(e, "alloc");
+    if ((e = hipMalloc(&m->d_mean, (size_t)m->F * 4)) != hipSuccess) return fail(e, "alloc");
+    if ((e = hipMalloc(&m->d_istd, (size_t)m->F * 4)) != hipSuccess) return fail(e, "alloc");
+    if ((e = hipMemcpy(m->d_cent, centroids, (size_t)ncent * dim * 4, hipMemcpyHostToDevice)) != hipSuccess)
+        return fail(e, "copy");
+    /* He-initialised weights from a fixed seed; zero biases; identity scaling */
+    std::vector<float> w(np, 0.f);
+    const MlpLayout L = mlp_layout(m->F);
+    std::mt19937_64 rng(0x5eed5eedULL);
+    std::normal_distribution<double> nd(0.0, 1.0);
+    for (int i = 0; i < kHid * m->F; i++) w[L.w1 + i] = (float)(nd(rng) * sqrt(2.0 / m->F));
+    for (int i = 0; i < kHid * kHid; i++) w[L.w2 + i] = (float)(nd(rng) * sqrt(2.0 / kHid));
+    for (int i = 0; i < 2 * kHid; i++) w[L.w3 + i] = (float)(nd(rng) * sqrt(1.0 / kHid));
+    m->h_mean.assign(m->F, 0.f);
+    m->h_istd.assign(m->F, 1.f);
+    char eb2[256];
+    if (fv_gpu_model_set_params(m, w.data(), m->h_mean.data(), m->h_istd.data(), eb2,
+                                sizeof(eb2)) != 0) {
+        snprintf(errbuf, errlen, "%s", eb2);
+        fv_gpu_model_free(m);
+        return nullptr;
+    }
+    return m;
+}
+
+extern "C" void fv_gpu_model_free(FvGpuModel *m) {
+    if (!m) return;
+    if (m->device >= 0) hipSetDevice(m->device);
+    if (m->stream) hipStreamSynchronize(m->stream);
+    hipFree(m->d_cent);
+    hipFree(m->d_w);
+    hipFree(m->d_m);
+    hipFree(m->d_v);
+    hipFree(m->d_g);
+    hipFree(m->d_wbf);
+    hipFree(m->d_mean);
+    hipFree(m->d_istd);
+    for (DevBuf *b : {&m->d_x, &m->d_t, &m->d_idx, &m->d_xh, &m->d_a1, &m->d_a2, &m->d_d1,
+                      &m->d_d2, &m->d_d3, &m->d_loss, &m->d_y})
+        b->release();
+    if (m->stream) hipStreamDestroy(m->stream);
+    delete m;
+}
+
+/* params: FP32 [nparams]; mean/std: [F] input standardisation (std 0 -> 1) */
+extern "C" int fv_gpu_model_set_params(FvGpuModel *m, const float *params, const float *mean,
+                                       const float *std_, char *errbuf, size_t errlen) {
+    if (m == nullptr || params == nullptr) PV_FAIL("gpu: model: NULL");
+    PV_TRY(hipSetDevice(m->device));
+    std::vector<float> is(m->F, 1.f), mu(m->F, 0.f);
+    for (int j = 0; j < m->F; j++) {
+        if (mean) mu[j] = mean[j];
+        if (std_) is[j] = std_[j] > 0.f ? 1.f / std_[j] : 1.f;
+    }
+    m->h_mean = mu;
+    m->h_istd = is;
+    PV_TRY(hipMemcpyAsync(m->d_w, params, (size_t)m->np * 4, hipMemcpyHostToDevice, m->stream));
+    PV_TRY(hipMemcpyAsync(m->d_mean, mu.data(), (size_t)m->F * 4, hipMemcpyHostToDevice, m->stream));
+    PV_TRY(hipMemcpyAsync(m->d_istd, is.data(), (size_t)m->F * 4, hipMemcpyHostToDevice, m->stream));
+    PV_TRY(hipMemsetAsync(m->d_m, 0, (size_t)m->np * 4, m->stream));
+    PV_TRY(hipMemsetAsync(m->d_v, 0, (size_t)m->np * 4, m->stream));
+    m->adam_t = 0;
+    if (model_sync_bf16(m, errbuf, errlen) != 0) return -1;
+    PV_TRY(hipStreamSynchronize(m->stream));
+    return 0;
+}
+
+extern "C" int fv_gpu_model_get_params(FvGpuModel *m, float *params, float *mean, float *std_,
+                                       char *errbuf, size_t errlen) {
+    if (m == nullptr) PV_FAIL("gpu: model: NULL");
+    PV_TRY(hipSetDevice(m->device));
+    if (params) {
+        PV_TRY(hipMemcpyAsync(params, m->d_w, (size_t)m->np * 4, hipMemcpyDeviceToHost, m->stream));
+        PV_TRY(hipStreamSynchronize(m->stream));
+    }
+    for (int j = 0; j < m->F; j++) {
+        if (mean) mean[j] = m->h_mean[j];
+        if (std_) std_[j] = 1.f / m->h_istd[j];
+    }
+    return 0;
+}
+
+/* one full-batch gradient of the mean loss at the current parameters
+ * (targets [n][3]: log ef, route, ef-valid) */
+extern "C" int fv_gpu_model_grad(FvGpuModel *m, int n, const float *feats, const float *targets,
+                                 float lambda, float *grad, double *loss, char *errbuf,
+                                 size_t errlen) {
+    if (m == nullptr || n < 1 || feats == nullptr || targets == nullptr || grad == nullptr)
+        PV_FAIL("gpu: model grad: bad arguments");
+    PV_TRY(hipSetDevice(m->device));
+    if (ensure_train(m, n, n, errbuf, errlen) != 0) return -1;
+    std::vector<int32_t> idx(n);
+    for (int i = 0; i < n; i++) idx[i] = i;
+    PV_TRY(hipMemcpyAsync(m->d_x.p, feats, (size_t)n * m->F * 4, hipMemcpyHostToDevice, m->stream));
+    PV_TRY(hipMemcpyAsync(m->d_t.p, targets, (size_t)n * 12, hipMemcpyHostToDevice, m->stream));
+    PV_TRY(hipMemcpyAsync(m->d_idx.p, idx.data(), (size_t)n * 4, hipMemcpyHostToDevice, m->stream));
+    if (grad_step(m, n, lambda, loss, errbuf, errlen) != 0) return -1;
+    PV_TRY(hipMemcpyAsync(grad, m->d_g, (size_t)m->np * 4, hipMemcpyDeviceToHost, m->stream));
+    PV_TRY(hipStreamSynchronize(m->stream));
+    return 0;
+}
+
+/*
+ * Minibatch Adam over (feats, targets) for `epochs` passes. The input
+ * standardisation is fitted on `feats` first (mean, std per input). A fixed
+ * seed shuffles the samples each epoch; loss_hist[e] receives the mean
+ * training loss of epoch e.
+ */
+extern "C" int fv_gpu_model_train(FvGpuModel *m, int n, const float *feats,
+                                  const float *targets, int epochs, int batch, float lr,
+                                  float lambda, uint64_t seed, float *loss_hist, char *errbuf,
+                                  size_t errlen) {
+    if (m == nullptr || n < 1 || feats == nullptr || targets == nullptr || epochs < 1 ||
+        batch < 1)
+        PV_FAIL("gpu: model train: bad arguments");
+    PV_TRY(hipSetDevice(m->device));
+    const int B = std::min(batch, n);
+    if (ensure_train(m, n, B, errbuf, errlen) != 0) return -1;
+    /* standardisation from the training set */
+    std::vector<double> mu(m->F, 0.0), var(m->F, 0.0);
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < m->F; j++) mu[j] += feats[(uint64_t)i * m->F + j];
+    for (int j = 0; j < m->F; j++) mu[j] /= n;
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < m->F; j++) {
+            const double d = feats[(uint64_t)i * m->F + j] - mu[j];
+            var[j] += d * d;
+        }
+    std::vector<float> fm(m->F), fs(m->F);
+    for (int j = 0; j < m->F; j++) {
+        fm[j] = (float)mu[j];
+        const double sd = sqrt(var[j] / n);
+        fs[j] = sd > 1e-12 ? (float)sd : 1.f;
+    }
+    std::vector<float> w(m->np);
+    PV_TRY(hipMemcpyAsync(w.data(), m->d_w, (size_t)m->np * 4, hipMemcpyDeviceToHost, m->stream));
+    PV_TRY(hipStreamSynchronize(m->stream));
+    if (fv_gpu_model_set_params(m, w.data(), fm.data(), fs.data(), errbuf, errlen) != 0)
+        return -1;
+    PV_TRY(hipMemcpyAsync(m->d_x.p, feats, (size_t)n * m->F * 4, hipMemcpyHostToDevice, m->stream));
+    PV_TRY(hipMemcpyAsync(m->d_t.p, targets, (size_t)n * 12, hipMemcpyHostToDevice, m->stream));
+    std::mt19937_64 rng(seed);
+    std::vector<int32_t> perm(n);
+    for (int i = 0; i < n; i++) perm[i] = i;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    for (int e = 0; e < epochs; e++) {
+        std::shuffle(perm.begin(), perm.end(), rng);
+        double tot = 0.0;
+        int steps = 0;
+        for (int s = 0; s + B <= n; s += B) {
+            PV_TRY(hipMemcpyAsync(m->d_idx.p, perm.data() + s, (size_t)B * 4,
+                                  hipMemcpyHostToDevice, m->stream));
+            double l = 0.0;
+            if (grad_step(m, B, lambda, loss_hist ? &l : nullptr, errbuf, errlen) != 0) return -1;
+            tot += l;
+            steps++;
+            m->adam_t++;
+            const float c1 = 1.f - powf(b1, (float)m->adam_t);
+            const float c2 = 1.f - powf(b2, (float)m->adam_t);
+            hipLaunchKernelGGL(k_adam, dim3((m->np + 255) / 256), dim3(256), 0, m->stream, m->d_w,
+                               m->d_g, m->d_m, m->d_v, m->np, lr, b1, b2, eps, c1, c2);
+            PV_TRY(hipGetLastError());
+        }
+        if (loss_hist) loss_hist[e] = steps ? (float)(tot / steps) : 0.f;
+    }
+    if (model_sync_bf16(m, errbuf, errlen) != 0) return -1;
+    PV_TRY(hipStreamSynchronize(m->stream));
+    return 0;
+}
+
+/* BF16 inference on host features [n][F] -> y [n][2] (log ef, route logit) */
+extern "C" int fv_gpu_model_predict(FvGpuModel *m, int n, const float *feats, float *y,
+                                    char *errbuf, size_t errlen) {
+    if (m == nullptr || n < 1 || feats == nullptr || y == nullptr)
+        PV_FAIL("gpu: model predict: bad arguments");
+    PV_TRY(hipSetDevice(m->device));
+    PV_TRY(m->d_x.ensure((size_t)n * m->F * 4));
+    PV_TRY(m->d_y.ensure((size_t)n * 8));
+    PV_TRY(hipMemcpyAsync(m->d_x.p, feats, (size_t)n * m->F * 4, hipMemcpyHostToDevice, m->stream));
+    hipLaunchKernelGGL(k_mlp_infer, dim3((n + 3) / 4), dim3(256), 0, m->stream,
+                       m->d_x.as<float>(), n, m->F, m->d_wbf, m->d_mean, m->d_istd,
+                       m->d_y.as<float>());
+    PV_TRY(hipGetLastError());
+    PV_TRY(hipMemcpyAsync(y, m->d_y.p, (size_t)n * 8, hipMemcpyDeviceToHost, m->stream));
+    PV_TRY(hipStreamSynchronize(m->stream));
+    return 0;
+}
+
+/* feature vectors of a staged batch's queries (bitmaps rebuilt on the
+ * batch's stream) -> host [nq][F] */
+extern "C" int fv_gpu_batch_features(FvGpuIndex *g, FvGpuBatch *b, FvGpuModel *m, float *out,
+                                     char *errbuf, size_t errlen) {
+    if (g == nullptr || b == nullptr || b->owner != g || m == nullptr || out == nullptr)
+        PV_FAIL("gpu: features: bad arguments");
+    if (m->dim != g->dim) PV_FAIL("gpu: features: model dim %d != index dim %d", m->dim, g->dim);
+    PV_TRY(hipSetDevice(g->device));
+    hipStream_t s = b->stream;
+    PV_TRY(launch_bitmap(g, b, s));
+    PV_TRY(b->d_progcnt.ensure((size_t)std::max(1, b->nprog) * 4));
+    uint32_t *pc = b->d_progcnt.as<uint32_t>();
+    if (b->nprog > 0)
+        hipLaunchKernelGGL(k_prog_count, dim3(b->nprog), dim3(256), 0, s,
+                           b->d_bitmap.as<uint32_t>(), g->count_pad / 32, pc);
+    DevBuf fb;
+    PV_TRY(fb.ensure((size_t)b->nq * m->F * 4));
+    hipLaunchKernelGGL(k_features, dim3((b->nq + 3) / 4), dim3(256), 0, s, b->d_q, b->nq, g->dim,
+                       b->d_pofq, pc, g->count, m->d_cent, m->ncent, fb.as<float>());
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(out, fb.p, (size_t)b->nq * m->F * 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    fb.release();
+    if (e != hipSuccess) PV_FAIL("gpu: features: %s", hipGetErrorString(e));
+    return 0;
+}
+
+/*
+ * One routed run of a staged batch: bitmaps + query prep, features, BF16
+ * MLP, per-query route and ef, then the brute pipeline over the queries
+ * routed to it (the others skipped on the device) and the graph search over
+ * the rest, both writing k hits per query into the batch outputs.
+ */
+extern "C" int fv_gpu_batch_run_learned(FvGpuIndex *g, FvGpuBatch *b, FvGpuModel *m,
+                                        const FvGpuLearnedOpts *o, char *errbuf,
+                                        size_t errlen) {
+    if (g == nullptr || b == nullptr || b->owner != g || o == nullptr)
+        PV_FAIL("gpu: learned run: bad arguments");
+    const bool need_model = o->mode == 0 || o->mode == 1;
+    if (need_model && m == nullptr) PV_FAIL("gpu: learned run: no model");
+    if (need_model && m->dim != g->dim) PV_FAIL("gpu: learned run: model dim != index dim");
+    if (o->mode < 0 || o->mode > 3) PV_FAIL("gpu: learned run: bad mode %d", o->mode);
+    if (o->mode != 3 && g->d_nbr == nullptr) PV_FAIL("gpu: learned run: graph not uploaded");
+    if (o->k != b->topb) PV_FAIL("gpu: learned run: k %u != staged topb %u", o->k, b->topb);
+    const uint32_t ef_min = std::max<uint32_t>(o->ef_min, o->k);
+    const uint32_t ef_max = std::min<uint32_t>(std::max(o->ef_max, ef_min), FV_GPU_MAX_EF);
+    const uint32_t ef_fixed = std::min<uint32_t>(std::max(o->ef_fixed, ef_min), FV_GPU_MAX_EF);
+    try {
+        PV_TRY(hipSetDevice(g->device));
+        hipStream_t s = b->stream;
+        const int nq = b->nq;
+        FvGpuBruteOpts bo = {};
+        bo.precision = o->precision;
+        PV_TRY(launch_phase1(g, b, scan_precision(g, &bo), true, s));
+        PV_TRY(b->d_route.ensure((size_t)nq));
+        PV_TRY(b->d_efq.ensure((size_t)nq * 4));
+        const float *y = nullptr;
+        if (need_model) {
+            PV_TRY(b->d_progcnt.ensure((size_t)std::max(1, b->nprog) * 4));
+            PV_TRY(b->d_feat.ensure((size_t)nq * m->F * 4));
+            PV_TRY(b->d_y.ensure((size_t)nq * 8));
+            uint32_t *pc = b->d_progcnt.as<uint32_t>();
+            if (b->nprog > 0)
+                hipLaunchKernelGGL(k_prog_count, dim3(b->nprog), dim3(256), 0, s,
+                                   b->d_bitmap.as<uint32_t>(), g->count_pad / 32, pc);
+            hipLaunchKernelGGL(k_features, dim3((nq + 3) / 4), dim3(256), 0, s, b->d_q, nq,
+                               g->dim, b->d_pofq, pc, g->count, m->d_cent, m->ncent,
+                               b->d_feat.as<float>());
+            hipLaunchKernelGGL(k_mlp_infer, dim3((nq + 3) / 4), dim3(256), 0, s,
+                               b->d_feat.as<float>(), nq, m->F, m->d_wbf, m->d_mean, m->d_istd,
+                               b->d_y.as<float>());
+            y = b->d_y.as<float>();
+        }
+        hipLaunchKernelGGL(k_route, dim3((nq + 255) / 256), dim3(256), 0, s, y, nq, o->mode,
+                           ef_fixed, ef_min, ef_max, o->ef_scale > 0.f ? o->ef_scale : 1.f,
+                           o->route_bias, b->d_route.as<uint8_t>(), b->d_efq.as<uint32_t>());
+        PV_TRY(hipGetLastError());
+        const uint8_t *route = b->d_route.as<uint8_t>();
+        if (run_brute_ex(g, b, &bo, 0, route, true, errbuf, errlen) != 0) return -1;
+        if (o->mode != 3) {
+            GraphExtra x;
+            x.route = route;
+            x.efq = b->d_efq.as<uint32_t>();
+            x.out_k = o->k;
+            x.bitmap_ready = true;
+            const uint32_t ef_launch = o->mode == 2 ? ef_fixed : ef_max;
+            const uint64_t mv = o->max_visited ? o->max_visited
+                                               : std::max<uint64_t>(200000, (uint64_t)ef_launch * 400);
+            if (run_graph_ex(g, b, ef_launch, mv, 4, 512, nullptr, x, errbuf, errlen) != 0)
+                return -1;
+        }
+        b->out_stride = o->k;
+        return 0;
+    } catch (...) {
+        snprintf(errbuf, errlen, "gpu: internal error");
+        return -1;
+    }
+}
+
+/* the last routed run's per-query decisions (synchronous) */
+extern "C" int fv_gpu_batch_routes(FvGpuIndex *g, FvGpuBatch *b, uint8_t *route, uint32_t *ef,
+                                   char *errbuf, size_t errlen) {
+    if (g == nullptr || b == nullptr || b->owner != g || b->d_route.p == nullptr)
+        PV_FAIL("gpu: routes: no routed run on this batch");
+    PV_TRY(hipSetDevice(g->device));
+    if (route)
+        PV_TRY(hipMemcpyAsync(route, b->d_route.p, (size_t)b->nq, hipMemcpyDeviceToHost, b->stream));
+    if (ef)
+        PV_TRY(hipMemcpyAsync(ef, b->d_efq.p, (size_t)b->nq * 4, hipMemcpyDeviceToHost, b->stream));
+    PV_TRY(hipStreamSynchronize(b->stream));
+    return 0;
+}

@@ -429,6 +429,21 @@ class GpuIndex:
         if rc != 0:
             raise GpuError(eb.value.decode())
 
+    def run_learned(self, bh, model, opts: L.LearnedOpts):
+        eb = L.errbuf()
+        rc = self.lib.fv_gpu_batch_run_learned(self.h, bh, model.h if model else None,
+                                               C.byref(opts), eb, len(eb))
+        if rc != 0:
+            raise GpuError(eb.value.decode())
+
+    def routes(self, bh, nq: int):
+        route = np.zeros(nq, np.uint8)
+        ef = np.zeros(nq, np.uint32)
+        eb = L.errbuf()
+        if self.lib.fv_gpu_batch_routes(self.h, bh, ptr(route), ptr(ef), eb, len(eb)) != 0:
+            raise GpuError(eb.value.decode())
+        return route, ef
+
     def run_graph(self, bh, ef: int, max_visited: int):
         eb = L.errbuf()
         rc = self.lib.fv_gpu_batch_run_graph(self.h, bh, ef, max_visited, eb, len(eb))
