@@ -53,7 +53,7 @@ sys.path.insert(0, ROOT)
 
 from paves_amd import _lib as L  # noqa: E402
 from paves_amd import engine as E  # noqa: E402
-from paves_amd import indexfile, synth  # noqa: E402
+from paves_amd import hnsw, indexfile, learned, synth  # noqa: E402
 
 METRIC = "filtered top-10 QPS at recall@10≥0.9: learned-ef+BF16 vs plain FP32, 1 GPU"
 HBM_PEAK_GBS = 8000.0
@@ -65,13 +65,14 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def build_batches(d, post, n, nq, bsz, k):
+def build_batches(d, post, n, nq, bsz, k, entries=None):
     classes = synth.FILTER_CLASSES
     batches = []
     for s in range(0, nq, bsz):
         e = min(nq, s + bsz)
         preds = [synth.filter_pred(classes[i % len(classes)], d["qcent"][i]) for i in range(s, e)]
-        b = E.concat([E.serialize(p, n, post) for p in preds], n)
+        b = E.concat([E.serialize(p, n, post) for p in preds], n,
+                     entries=None if entries is None else entries[s:e])
         batches.append((s, e, b))
     return batches
 
@@ -207,6 +208,10 @@ def main():
     ap.add_argument("--nq", type=int, default=10_000)
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--k", type=int, default=10)
+    ap.add_argument("--train-queries", type=int, default=200_000,
+                    help="training queries for the learned router (0 = skip the learned path)")
+    ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 1),
+                    help="host threads (graph build, entry points, CPU baseline)")
     ap.add_argument("--fp32-steps", type=int, default=0, help="0 = max(2, steps // 4)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -225,6 +230,7 @@ def main():
     if args.roofline_only:
         roofline_only(args)
         return
+    learned_on = args.train_queries > 0
 
     traffic, traffic_info = (None, None)
     if not args.no_traffic:
@@ -238,32 +244,42 @@ def main():
     rows, vals = synth.postings(cols)
     post = E.Postings(rows, vals)
     log(f"data: {args.n}x128 + {args.nq} queries in {time.perf_counter() - t:.1f}s")
-    batches = build_batches(d, post, args.n, args.nq, args.batch, args.k)
 
-    g = E.GpuIndex(d["base"], cols, rows, precision=L.FV_GPU_PREC_BF16, device=0)
+    graph, ent, setup = None, None, {}
+    if learned_on:
+        t = time.perf_counter()
+        graph = hnsw.build(d["base"], M=16, ef_construction=200, threads=args.threads)
+        setup["graph_build_s"] = round(time.perf_counter() - t, 2)
+        t = time.perf_counter()
+        ent = hnsw.entries(graph, d["base"], d["queries"], threads=args.threads)
+        setup["entry_points_us_per_query"] = round((time.perf_counter() - t) / args.nq * 1e6, 3)
+        log(f"graph: M=16 efC=200 built in {setup['graph_build_s']}s ({args.threads} threads), "
+            f"entry points {setup['entry_points_us_per_query']} us/query")
+    batches = build_batches(d, post, args.n, args.nq, args.batch, args.k, ent)
+
+    g = E.GpuIndex(d["base"], cols, rows, l0=None if graph is None else graph["l0"],
+                   precision=L.FV_GPU_PREC_BF16, device=0)
     staged = [g.stage(d["queries"][s:e], b, args.k) for s, e, b in batches]
     log(f"staged {len(staged)} micro-batches")
     bf = L.BruteOpts(precision=L.FV_GPU_PREC_BF16, route=0)
     fp = L.BruteOpts(precision=L.FV_GPU_PREC_FP32, route=0)
 
-    # ---- BF16 path: warmup then K timed steps
+    # ---- BF16 brute path: warmup then K timed steps
     run_steps(g, staged, bf, args.warmup)
     g.sync()
     dt = run_steps(g, staged, bf, args.steps)
-    qps = args.steps * args.nq / dt
-    ms_per_step = dt / args.steps * 1e3
-    log(f"bf16: {qps:,.0f} QPS ({ms_per_step:.2f} ms/step)")
-
-    # results of the BF16 path (last step)
+    qps_bf = args.steps * args.nq / dt
+    log(f"bf16 brute: {qps_bf:,.0f} QPS ({dt / args.steps * 1e3:.2f} ms/step)")
     res_bf = [g.fetch(bh, e - s, args.k) for bh, (s, e, _) in zip(staged, batches)]
     overflow = sum(r[3] != 0 for r in res_bf)
+    truth = [o[q, : c[q]]["node"].copy() for (o, c, _, _) in res_bf for q in range(len(c))]
 
     # ---- plain FP32 path on the same staged batches
     fsteps = args.fp32_steps or max(2, args.steps // 4)
     run_steps(g, staged, fp, 1)
     dt32 = run_steps(g, staged, fp, fsteps)
     qps32 = fsteps * args.nq / dt32
-    log(f"fp32: {qps32:,.0f} QPS")
+    log(f"plain fp32 brute: {qps32:,.0f} QPS")
     res_fp = [g.fetch(bh, e - s, args.k) for bh, (s, e, _) in zip(staged, batches)]
 
     # ---- the reference's synchronous entry point (host buffers in and out:
@@ -296,6 +312,11 @@ def main():
             np.array_equal(o1[q, : c1[q]], oh[q, : c1[q]]) for q in range(e - s)))
         log(f"oracle check (first micro-batch, {e - s} queries): {sample_ok}")
 
+    # ---- learned route + ef over BF16 brute / graph (the headline path)
+    lrn = None
+    if learned_on:
+        lrn = learned_path(g, graph, d, post, staged, batches, truth, args, setup)
+
     # ---- roofline of the dominant kernel (BF16 full scan), HIP events
     m = measure_scan(g, batches, staged, bf, args.n)
     avg_scan, alg_bytes = m["avg_scan_us"], m["alg_bytes"]
@@ -319,15 +340,24 @@ def main():
         g.free_batch(bh)
     g.close()
 
+    if lrn is not None:
+        value, steps_ms, rec = lrn["qps"], lrn["ms_per_step"], lrn["recall_at_10"]
+    else:
+        value, steps_ms, rec = qps_bf, dt / args.steps * 1e3, (1.0 if exact else None)
     out = {
-        "metric": METRIC, "value": round(qps, 1), "unit": "queries/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
+        "metric": METRIC, "value": round(value, 1), "unit": "queries/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(steps_ms, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16",
         "data": "synthetic (seeded GMM, see paves_amd/synth.py)",
-        "config": {"workload": "filtered brute-force top-10: 1M x 128 GMM, 10k queries in "
+        "config": {"workload": "filtered top-10, learned route/ef over BF16 brute and HNSW: "
+                               "1M x 128 GMM, 10k queries in micro-batches of 64, six filter "
+                               "classes" if lrn else
+                               "filtered brute-force top-10: 1M x 128 GMM, 10k queries in "
                                "micro-batches of 64, six filter classes",
                    "n": args.n, "dim": 128, "queries": args.nq, "micro_batch": args.batch,
-                   "k": args.k, "topb": args.k},
+                   "k": args.k, "topb": args.k,
+                   "graph": "HNSW M=16 (M0=32) efC=200" if lrn else None,
+                   "train_queries": args.train_queries if lrn else 0},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": int(traffic) if traffic else None, "kernel": SCAN_KERNEL,
@@ -337,16 +367,86 @@ def main():
                      "copy_peak_GBs": round(copy_gbs, 1) if copy_gbs else None,
                      "frac_of_copy_peak": round(achieved / copy_gbs, 4) if copy_gbs else None,
                      "traffic_detail": traffic_info},
+        "recall_at_10": round(rec, 4) if rec is not None else None,
+        "bf16_brute_qps": round(qps_bf, 1),
         "plain_fp32_qps": round(qps32, 1),
+        "speedup_vs_plain_fp32": round(value / qps32, 3),
         "sync_abi_qps_host_buffers": round(qps_sync, 1),
-        "speedup_vs_plain_fp32": round(qps / qps32, 3),
-        "recall_at_10": 1.0 if (exact and sample_ok is not False) else None,
         "ids_bit_exact_bf16_vs_fp32": exact,
         "oracle_sample_ok": sample_ok,
         "overflow_batches": int(overflow),
+        "learned": lrn,
+        "setup": setup,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
+
+
+def learned_path(g, graph, d, post, staged, batches, truth, args, setup):
+    """Train the router on queries drawn apart from the evaluated ones,
+    then time the routed path (mode 0) over the staged micro-batches; also
+    the fixed-ef fallback (graph, ef = 400, FP32 distances) on a sample."""
+    classes = synth.FILTER_CLASSES
+    t = time.perf_counter()
+    tq, tc = synth.more_queries(d, args.train_queries, seed=synth.BASE_SEED + 1)
+    tsers = [E.serialize(synth.filter_pred(classes[i % 6], tc[i]), args.n, post)
+             for i in range(args.train_queries)]
+    setup["train_queries_gen_s"] = round(time.perf_counter() - t, 2)
+    model, info = learned.train_router(g, graph, d["base"], tq, tsers, k=args.k, log=log)
+    setup["router"] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in info.items()}
+    # safety factor on the predicted ef, from training queries held apart
+    nval = min(2048, args.train_queries)
+    vq, vs = tq[-nval:], tsers[-nval:]
+    vent = hnsw.entries(graph, d["base"], vq, threads=args.threads)
+    vtruth = learned.exact_truth(g, vq, vs, args.k)
+    scale = learned.calibrate_scale(g, model, vq, vs, vent, vtruth, k=args.k, mode=0)
+    opts = L.LearnedOpts(k=args.k, mode=0, ef_scale=scale, max_visited=50_000)
+    log(f"router: ef scale {scale}")
+
+    def step():
+        for bh in staged:
+            g.run_learned(bh, model, opts)
+
+    for _ in range(args.warmup):
+        step()
+    g.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    g.sync()
+    dt = time.perf_counter() - t0
+    qps = args.steps * args.nq / dt
+    rec, ng, efs, per = [], 0, [], [[] for _ in classes]
+    for (s, e, _), bh in zip(batches, staged):
+        o, c, f, rc = g.fetch(bh, e - s, args.k)
+        r, ef = g.routes(bh, e - s)
+        for j in range(e - s):
+            tr = set(int(x) for x in truth[s + j])
+            got = set(int(x) for x in o[j, : c[j]]["node"])
+            rr = len(tr & got) / len(tr) if tr else 1.0
+            rec.append(rr)
+            per[(s + j) % 6].append(rr)
+            if r[j]:
+                ng += 1
+                efs.append(int(ef[j]))
+    log(f"learned route+ef: {qps:,.0f} QPS, recall@10 {np.mean(rec):.4f}, "
+        f"graph-routed {ng / args.nq:.3f}")
+    # fixed-ef fallback on the graph (plain FP32 distances), bounded sample
+    nfx = min(args.nq, 1024)
+    fq = d["queries"][:nfx]
+    fs = [E.serialize(synth.filter_pred(classes[i % 6], d["qcent"][i]), args.n, post)
+          for i in range(nfx)]
+    fent = np.concatenate([b.qd["entry"] for _, _, b in batches])[:nfx].astype(np.uint32)
+    fx = learned.run_mode(g, model, fq, fs, fent, truth[:nfx],
+                          L.LearnedOpts(k=args.k, mode=2, ef_fixed=400), micro=args.batch)
+    log(f"fixed ef=400 graph (FP32): {fx[0]:,.0f} QPS, recall {fx[1]:.4f} ({nfx} queries)")
+    model.close()
+    return {"qps": qps, "ms_per_step": dt / args.steps * 1e3, "recall_at_10": float(np.mean(rec)),
+            "recall_per_class": {classes[i]: round(float(np.mean(per[i])), 4) for i in range(6)},
+            "graph_routed_frac": ng / args.nq, "mean_ef_graph_routed": float(np.mean(efs)) if efs else 0.0,
+            "ef_scale": scale,
+            "fixed_ef400_graph_qps": round(fx[0], 1), "fixed_ef400_graph_recall": round(fx[1], 4),
+            "fixed_ef400_sample": nfx}
 
 
 def cpu_baseline(d, cols, rows, vals, args):

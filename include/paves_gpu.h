@@ -335,9 +335,10 @@ typedef struct FvGpuLearnedOpts {
 } FvGpuLearnedOpts;
 int fv_gpu_batch_run_learned(FvGpuIndex *g, FvGpuBatch *b, FvGpuModel *m,
                              const FvGpuLearnedOpts *o, char *errbuf, size_t errlen);
-/* per-query decisions of the batch's last routed run (synchronous) */
+/* per-query decisions of the batch's last routed run (synchronous); any
+ * output may be NULL; y [nq][2] = the model outputs (model modes only) */
 int fv_gpu_batch_routes(FvGpuIndex *g, FvGpuBatch *b, uint8_t *route, uint32_t *ef,
-                        char *errbuf, size_t errlen);
+                        float *y, char *errbuf, size_t errlen);
 
 #ifdef __cplusplus
 }

@@ -161,7 +161,7 @@ def load() -> C.CDLL:
     lib.fv_gpu_model_predict.argtypes = [_P, C.c_int, _P, _P, C.c_char_p, sz]
     lib.fv_gpu_batch_features.argtypes = [_P, _P, _P, _P, C.c_char_p, sz]
     lib.fv_gpu_batch_run_learned.argtypes = [_P, _P, _P, C.POINTER(LearnedOpts), C.c_char_p, sz]
-    lib.fv_gpu_batch_routes.argtypes = [_P, _P, _P, _P, C.c_char_p, sz]
+    lib.fv_gpu_batch_routes.argtypes = [_P, _P, _P, _P, _P, C.c_char_p, sz]
     _lib = lib
     return lib
 

@@ -119,6 +119,7 @@ struct Phase1Args {
     const double *vals, *cols;
     uint64_t count, words;
     uint32_t *bitmap;
+    uint32_t *progcnt;    /* optional: passing rows per program (+=) */
     int nprog, nops, nvals, ncols;
     uint32_t nbm;
     const float *q;
@@ -132,7 +133,7 @@ struct Phase1Args {
 __global__ __launch_bounds__(256) void k_phase1(Phase1Args A) {
     if (blockIdx.x < A.nbm) {
         bitmap_rows(blockIdx.x, A.nprog, A.nops, A.ops, A.poff, A.plen, A.vals, A.nvals, A.cols, A.count,
-                    A.ncols, A.words, A.bitmap);
+                    A.ncols, A.words, A.bitmap, A.progcnt);
         return;
     }
     const int qi = (int)(blockIdx.x - A.nbm) * 4 + (threadIdx.x >> 6);
@@ -808,12 +809,16 @@ __global__ __launch_bounds__(256) void k_select(
     const float *__restrict__ vecs, const float *__restrict__ qf, int dim,
     FvHit *__restrict__ out, uint32_t out_stride, uint32_t *__restrict__ out_cnt,
     uint8_t *__restrict__ out_flag, uint32_t *__restrict__ status,
-    const uint8_t *__restrict__ skip) {
+    const uint8_t *__restrict__ skip, uint32_t *__restrict__ progcnt_reset, int nprog) {
     __shared__ uint64_t sel[kSelMax], wb[4];
     __shared__ uint32_t hist[256], tmp[8], bc[8];
     __shared__ uint32_t nsel;
     extern __shared__ float qsh[];
     const int q = blockIdx.x, t = threadIdx.x;
+    /* passing-row counts of this run were consumed by the routing kernel:
+     * clear them for the batch's next run */
+    if (progcnt_reset && q == 0)
+        for (int i = t; i < nprog; i += blockDim.x) progcnt_reset[i] = 0;
     if (skip && skip[q]) return; /* the graph search writes this query */
     const uint64_t *pk = pool + (uint64_t)q * cap_pool;
     const uint32_t ntot = pcnt[(uint64_t)q * kPcntStride];
@@ -1042,6 +1047,10 @@ int stage_batch(FvGpuIndex *g, FvGpuBatch *b, int nq, const float *queries,
     PV_TRY(b->d_cnt.ensure((size_t)nq * 4));
     PV_TRY(b->d_flag.ensure((size_t)nq));
     PV_TRY(b->d_pcnt.ensure((size_t)nqp * kPcntStride * 4));
+    /* per-program passing-row counts of routed runs: zero here, then
+     * accumulated by each run's bitmap pass and cleared by its select */
+    PV_TRY(b->d_progcnt.ensure((size_t)std::max(1, b->nprog) * 4));
+    PV_TRY(hipMemsetAsync(b->d_progcnt.p, 0, (size_t)std::max(1, b->nprog) * 4, b->stream));
     return 0;
 }
 
@@ -1056,7 +1065,8 @@ int scan_precision(const FvGpuIndex *g, const FvGpuBruteOpts *o) {
 
 /* Phase 1 of a run: predicate bitmaps (when `bitmap`) and the query
  * preparation for scan precision `prec`, one launch on stream s. */
-hipError_t launch_phase1(FvGpuIndex *g, FvGpuBatch *b, int prec, bool bitmap, hipStream_t s) {
+hipError_t launch_phase1(FvGpuIndex *g, FvGpuBatch *b, int prec, bool bitmap, hipStream_t s,
+                         uint32_t *progcnt) {
     QAux qa = qaux_of(b);
     {
         /* predicate programs -> row bitmaps, every run (the gather route
@@ -1071,6 +1081,7 @@ hipError_t launch_phase1(FvGpuIndex *g, FvGpuBatch *b, int prec, bool bitmap, hi
         P.count = g->count;
         P.words = g->count_pad / 32;
         P.bitmap = b->d_bitmap.as<uint32_t>();
+        P.progcnt = progcnt;
         P.nprog = b->nprog;
         P.nops = b->nops;
         P.nvals = b->nvals;
@@ -1097,7 +1108,7 @@ hipError_t launch_phase1(FvGpuIndex *g, FvGpuBatch *b, int prec, bool bitmap, hi
  * sample stride (1 = every row: the tightest threshold). */
 int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
                  uint32_t stride_override, const uint8_t *skip, bool have_phase1,
-                 char *errbuf, size_t errlen) {
+                 uint32_t *progcnt_reset, char *errbuf, size_t errlen) {
     hipStream_t s = b->stream;
     g->ev_stream = s;
     b->out_stride = b->topb; /* a graph run may have widened it; d_out holds nq x topb */
@@ -1124,7 +1135,7 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
     st.route = route;
 
     ev_record(g, 0);
-    if (!have_phase1) PV_TRY(launch_phase1(g, b, prec, route == 1, s));
+    if (!have_phase1) PV_TRY(launch_phase1(g, b, prec, route == 1, s, nullptr));
     ev_record(g, 6);
     ev_record(g, 1);
 
@@ -1155,7 +1166,8 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
                            b->cap_pool, B, (const float *)nullptr,
                            g->d_vecs, b->d_q, dim, b->d_out.as<FvHit>(),
                            b->out_stride, b->d_cnt.as<uint32_t>(),
-                           b->d_flag.as<uint8_t>(), qa.status, skip);
+                           b->d_flag.as<uint8_t>(), qa.status, skip,
+                           progcnt_reset, b->nprog);
         PV_TRY(hipGetLastError());
         ev_record(g, 5);
         st.scan_rows = b->sum_total;
@@ -1261,7 +1273,8 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
                        b->d_pool.as<uint64_t>(), b->d_pcnt.as<uint32_t>(),
                        b->cap_pool, B, qa.eps, g->d_vecs, b->d_q, dim,
                        b->d_out.as<FvHit>(), b->out_stride,
-                       b->d_cnt.as<uint32_t>(), b->d_flag.as<uint8_t>(), qa.status, skip);
+                       b->d_cnt.as<uint32_t>(), b->d_flag.as<uint8_t>(), qa.status, skip,
+                           progcnt_reset, b->nprog);
     PV_TRY(hipGetLastError());
     ev_record(g, 5);
     st.scan_rows = g->count;
@@ -1271,7 +1284,7 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
 
 static int run_brute(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
                      uint32_t stride_override, char *errbuf, size_t errlen) {
-    return run_brute_ex(g, b, o, stride_override, nullptr, false, errbuf, errlen);
+    return run_brute_ex(g, b, o, stride_override, nullptr, false, nullptr, errbuf, errlen);
 }
 
 /* copy outputs and per-query status to pinned host memory (synchronous) */

@@ -148,7 +148,9 @@ __device__ __forceinline__ bool eval_prog(const PvOp *__restrict__ ops, int n,
  * kBmOps ops and (when they fit) the IN values are staged in LDS so that op
  * decoding costs LDS reads instead of chains of dependent scalar-cache
  * misses. Result words of kBmGroup programs are collected in LDS and
- * written as whole 128-byte runs (one run per program per block).
+ * written as whole 128-byte runs (one run per program per block). When
+ * progcnt is given, each program's passing-row count is added to it (one
+ * atomic per program per block; the caller zeroes it beforehand).
  * Called by every thread of the block (it synchronises).
  */
 constexpr int kBmRowsPerThread = 8;
@@ -165,7 +167,8 @@ __device__ __forceinline__ void bitmap_rows(uint64_t block, int nprog, int nops,
                                             const double *__restrict__ vals, int nvals,
                                             const double *__restrict__ cols, uint64_t count,
                                             int ncols, uint64_t words,
-                                            uint32_t *__restrict__ bitmap) {
+                                            uint32_t *__restrict__ bitmap,
+                                            uint32_t *__restrict__ progcnt) {
     constexpr int R = kBmRowsPerThread;
     __shared__ PvOp s_ops[kBmOps];
     __shared__ int32_t s_off[kBmProgs], s_len[kBmProgs];
@@ -329,6 +332,12 @@ __device__ __forceinline__ void bitmap_rows(uint64_t block, int nprog, int nops,
                 if (w < words)
                     bitmap[(uint64_t)(pb + pp) * words + w] =
                         (uint32_t)(s_bm[pp][wd >> 1] >> (32 * (wd & 1)));
+            }
+            if (progcnt != nullptr && t <= slot) {
+                /* passing rows of program pb + t in this block */
+                uint32_t c = 0;
+                for (int k = 0; k < 4 * R; k++) c += (uint32_t)__popcll(s_bm[t][k]);
+                if (c) atomicAdd(&progcnt[pb + t], c);
             }
             __syncthreads();
         }

@@ -271,8 +271,9 @@ int run_graph_ex(FvGpuIndex *g, FvGpuBatch *b, uint32_t ef, uint64_t max_visited
  * queries it must leave alone; have_phase1 = bitmaps and query prep done */
 int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
                  uint32_t stride_override, const uint8_t *skip, bool have_phase1,
-                 char *errbuf, size_t errlen);
-hipError_t launch_phase1(FvGpuIndex *g, FvGpuBatch *b, int prec, bool bitmap, hipStream_t s);
+                 uint32_t *progcnt_reset, char *errbuf, size_t errlen);
+hipError_t launch_phase1(FvGpuIndex *g, FvGpuBatch *b, int prec, bool bitmap, hipStream_t s,
+                         uint32_t *progcnt);
 /* precision the brute scan runs in for these options (FP32 or BF16) */
 int scan_precision(const FvGpuIndex *g, const FvGpuBruteOpts *o);
 

@@ -61,6 +61,14 @@ __host__ __device__ inline MlpLayout mlp_layout(int F) {
 
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
+/* makes a wave's LDS writes visible to the wave's other lanes (no block
+ * barrier: waves of a block work on different queries) */
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __device__ __forceinline__ float wave_sum(float v) {
     for (int o = 32; o; o >>= 1) v += __shfl_xor(v, o);
     return v;
@@ -82,39 +90,44 @@ __global__ __launch_bounds__(256) void k_prog_count(const uint32_t *__restrict__
 }
 
 /*
- * One wave per query (4 per block). Distances to the centroids are exact
- * FP32 sums of squared differences; the 16 smallest are taken by repeated
- * wave-wide minimum (each lane keeps its own candidates sorted).
+ * Feature vector of one query, computed by one wave into xs[0..F):
+ *   [q | log10(max(pass, 1) / count) | sqrt of the 16 smallest squared
+ *    distances to the centroids, ascending].
+ * Lane l owns centroids l, l+64, ...: each row is read with independent
+ * 16-byte loads (many in flight per lane), the query from LDS by
+ * broadcast; each lane keeps its 16 best, then 16 rounds of wave minimum
+ * merge the lanes' lists. Wave-local (no block barrier).
  */
-__global__ __launch_bounds__(256) void k_features(const float *__restrict__ q, int nq, int dim,
-                                                  const int32_t *__restrict__ pofq,
-                                                  const uint32_t *__restrict__ pcnt,
-                                                  uint64_t count, const float *__restrict__ cent,
-                                                  int ncent, float *__restrict__ out) {
-    const int qi = blockIdx.x * 4 + (threadIdx.x >> 6);
+__device__ void wave_features(const float *__restrict__ qv, int dim, double pass, uint64_t count,
+                              const float *__restrict__ cent, int ncent, float *xs) {
     const int l = threadIdx.x & 63;
-    if (qi >= nq) return;
-    const int F = model_nfeat(dim);
-    const float *qv = q + (uint64_t)qi * dim;
-    float *o = out + (uint64_t)qi * F;
-    for (int j = l; j < dim; j += 64) o[j] = qv[j];
-    if (l == 0) {
-        const int p = pofq[qi];
-        const double pass = p < 0 ? (double)count : (double)pcnt[p];
-        o[dim] = (float)log10(fmax(pass, 1.0) / (double)count);
-    }
-    /* each lane: distances to centroids l, l+64, ... ; keep the 16 best */
+    for (int j = l; j < dim; j += 64) xs[j] = qv[j];
+    if (l == 0) xs[dim] = (float)log10(fmax(pass, 1.0) / (double)count);
+    wave_lds_sync();
     float best[kNear];
 #pragma unroll
     for (int k = 0; k < kNear; k++) best[k] = __uint_as_float(0x7f800000u);
     for (int c = l; c < ncent; c += 64) {
-        const float *cv = cent + (uint64_t)c * dim;
+        const float *cr = cent + (uint64_t)c * dim;
         float s = 0.f;
-        for (int j = 0; j < dim; j++) {
-            const float d = qv[j] - cv[j];
-            s = fmaf(d, d, s);
+        if ((dim & 3) == 0) {
+            const float4 *c4 = (const float4 *)cr;
+            const float4 *x4 = (const float4 *)xs;
+#pragma unroll 16
+            for (int k = 0; k < dim / 4; k++) {
+                const float4 v = c4[k], x = x4[k];
+                const float d0 = x.x - v.x, d1 = x.y - v.y, d2 = x.z - v.z, d3 = x.w - v.w;
+                s = fmaf(d0, d0, s);
+                s = fmaf(d1, d1, s);
+                s = fmaf(d2, d2, s);
+                s = fmaf(d3, d3, s);
+            }
+        } else {
+            for (int k = 0; k < dim; k++) {
+                const float d = xs[k] - cr[k];
+                s = fmaf(d, d, s);
+            }
         }
-        /* insertion into the lane's sorted list */
         float v = s;
 #pragma unroll
         for (int k = 0; k < kNear; k++) {
@@ -123,7 +136,6 @@ __global__ __launch_bounds__(256) void k_features(const float *__restrict__ q, i
             v = hi;
         }
     }
-    /* 16 rounds of wave minimum over the lanes' heads */
     int head = 0;
     for (int r = 0; r < kNear; r++) {
         float h = __uint_as_float(0x7f800000u);
@@ -132,44 +144,94 @@ __global__ __launch_bounds__(256) void k_features(const float *__restrict__ q, i
             if (k == head) h = best[k];
         float m = h;
         for (int off = 32; off; off >>= 1) m = fminf(m, __shfl_xor(m, off));
-        /* the lowest lane holding the minimum advances its head */
         const uint64_t who = __ballot(h == m);
         const int lane = __ffsll((unsigned long long)who) - 1;
         if (l == lane) head++;
-        if (l == 0) o[dim + 1 + r] = sqrtf(m);
+        if (l == 0) xs[dim + 1 + r] = sqrtf(m);
     }
+    wave_lds_sync();
+}
+
+/* one wave per query (4 per block): feature vectors to out [nq][F] */
+__global__ __launch_bounds__(256) void k_features(const float *__restrict__ q, int nq, int dim,
+                                                  const int32_t *__restrict__ pofq,
+                                                  const uint32_t *__restrict__ pcnt,
+                                                  uint64_t count, const float *__restrict__ cent,
+                                                  int ncent, float *__restrict__ out) {
+    __shared__ __attribute__((aligned(16))) float xs[4][kMaxFeat];
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int qi = blockIdx.x * 4 + w;
+    if (qi >= nq) return; /* no block barrier below */
+    const int F = model_nfeat(dim);
+    const int p = pofq[qi];
+    wave_features(q + (uint64_t)qi * dim, dim, p < 0 ? (double)count : (double)pcnt[p], count,
+                  cent, ncent, xs[w]);
+    for (int j = l; j < F; j += 64) out[(uint64_t)qi * F + j] = xs[w][j];
+}
+
+/* the BF16 inference parameters staged in LDS when they fit (dim <= 156) */
+constexpr int kLdsParams = 16384;
+
+/* all threads of the block copy n BF16 parameters (n padded to 8) to LDS */
+__device__ __forceinline__ void stage_params(const uint16_t *__restrict__ wt, int n,
+                                             uint16_t *ws) {
+    const uint4 *src = (const uint4 *)wt;
+    uint4 *dst = (uint4 *)ws;
+    for (int i = threadIdx.x; i < (n + 7) / 8; i += blockDim.x) dst[i] = src[i];
+}
+
+/*
+ * MLP forward for one query by one wave (lane j = hidden unit j) with the
+ * inference copy of the parameters (BF16; W1 and W2 transposed, W1t[F][64]
+ * and W2t[64][64], so step i reads 64 consecutive values). xs holds the raw
+ * features (standardised here into xh), hs is 64 floats of LDS.
+ */
+__device__ void wave_mlp(const float *xs, int F, const uint16_t *wt,
+                         const float *__restrict__ mean, const float *__restrict__ istd,
+                         float *xh, float *hs, float &y0, float &y1) {
+    const int l = threadIdx.x & 63;
+    const MlpLayout L = mlp_layout(F);
+    for (int j = l; j < F; j += 64) xh[j] = (xs[j] - mean[j]) * istd[j];
+    wave_lds_sync();
+    float a = bf2f(wt[L.b1 + l]);
+#pragma unroll 8
+    for (int i = 0; i < F; i++) a = fmaf(bf2f(wt[L.w1 + i * kHid + l]), xh[i], a);
+    hs[l] = fmaxf(a, 0.f);
+    wave_lds_sync();
+    float b = bf2f(wt[L.b2 + l]);
+#pragma unroll 8
+    for (int i = 0; i < kHid; i++) b = fmaf(bf2f(wt[L.w2 + i * kHid + l]), hs[i], b);
+    b = fmaxf(b, 0.f);
+    y0 = wave_sum(bf2f(wt[L.w3 + l]) * b) + bf2f(wt[L.b3 + 0]);
+    y1 = wave_sum(bf2f(wt[L.w3 + kHid + l]) * b) + bf2f(wt[L.b3 + 1]);
+    wave_lds_sync();
 }
 
 /* ---------------- inference ---------------- */
 
-/* one wave per query; lane j computes hidden unit j of both layers */
+/* one wave per query (4 per block): model outputs y [n][2] */
 __global__ __launch_bounds__(256) void k_mlp_infer(const float *__restrict__ x, int n, int F,
-                                                   const uint16_t *__restrict__ wbf,
+                                                   const uint16_t *__restrict__ wt,
                                                    const float *__restrict__ mean,
                                                    const float *__restrict__ istd,
                                                    float *__restrict__ y) {
-    __shared__ float xs[4][kMaxFeat];
+    __shared__ __attribute__((aligned(16))) uint16_t ws[kLdsParams];
+    __shared__ float xs[4][kMaxFeat], xh[4][kMaxFeat];
     __shared__ float hs[4][kHid];
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
     const int s = blockIdx.x * 4 + w;
-    const bool act = s < n; /* every wave reaches every barrier */
-    const MlpLayout L = mlp_layout(F);
-    for (int j = l; j < F; j += 64) xs[w][j] = act ? (x[(uint64_t)s * F + j] - mean[j]) * istd[j] : 0.f;
-    __syncthreads();
-    float a = bf2f(wbf[L.b1 + l]);
-    const uint16_t *r1 = wbf + L.w1 + (uint64_t)l * F;
-    for (int i = 0; i < F; i++) a = fmaf(bf2f(r1[i]), xs[w][i], a);
-    hs[w][l] = fmaxf(a, 0.f);
-    __syncthreads();
-    float b = bf2f(wbf[L.b2 + l]);
-    const uint16_t *r2 = wbf + L.w2 + (uint64_t)l * kHid;
-    for (int i = 0; i < kHid; i++) b = fmaf(bf2f(r2[i]), hs[w][i], b);
-    b = fmaxf(b, 0.f);
-    const float y0 = wave_sum(bf2f(wbf[L.w3 + l]) * b);
-    const float y1 = wave_sum(bf2f(wbf[L.w3 + kHid + l]) * b);
-    if (act && l == 0) {
-        y[(uint64_t)s * 2 + 0] = y0 + bf2f(wbf[L.b3 + 0]);
-        y[(uint64_t)s * 2 + 1] = y1 + bf2f(wbf[L.b3 + 1]);
+    const int np = mlp_layout(F).n;
+    const bool lds = np <= kLdsParams;
+    if (lds) stage_params(wt, np, ws);
+    __syncthreads(); /* every wave reaches it (no early return above) */
+    if (s >= n) return;
+    for (int j = l; j < F; j += 64) xs[w][j] = x[(uint64_t)s * F + j];
+    wave_lds_sync();
+    float y0, y1;
+    wave_mlp(xs[w], F, lds ? ws : wt, mean, istd, xh[w], hs[w], y0, y1);
+    if (l == 0) {
+        y[(uint64_t)s * 2 + 0] = y0;
+        y[(uint64_t)s * 2 + 1] = y1;
     }
 }
 
@@ -311,10 +373,83 @@ __global__ void k_route(const float *__restrict__ y, int nq, int mode, uint32_t 
     route[q] = r;
 }
 
-/* FP32 -> BF16 (round to nearest even) copy of the parameters */
-__global__ void k_to_bf16(const float *__restrict__ w, uint16_t *__restrict__ o, int n) {
+/* FP32 -> BF16 (round to nearest even) inference copy of the parameters,
+ * W1 and W2 transposed (W1t[F][64], W2t[64][64]); other blocks in place */
+__global__ void k_to_bf16(const float *__restrict__ w, uint16_t *__restrict__ o, int F) {
+    const MlpLayout L = mlp_layout(F);
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p < n) o[p] = f2bf(w[p]);
+    if (p >= L.n) return;
+    int d = p;
+    if (p < L.b1) {
+        const int i = p / F, j = p % F;
+        d = L.w1 + j * kHid + i;
+    } else if (p >= L.w2 && p < L.b2) {
+        const int i = (p - L.w2) / kHid, j = (p - L.w2) % kHid;
+        d = L.w2 + j * kHid + i;
+    }
+    o[d] = f2bf(w[p]);
+}
+
+/*
+ * The routed run's decision kernel, one wave per query: features (with the
+ * passing-row counts the bitmap pass accumulated), MLP, and route/ef as in
+ * k_route. Optionally stores the features and outputs.
+ */
+struct LearnArgs {
+    const float *q;
+    int nq, dim, F;
+    const int32_t *pofq;
+    const uint32_t *progcnt;
+    uint64_t count;
+    const float *cent;
+    int ncent;
+    const uint16_t *wt;
+    const float *mean, *istd;
+    int mode;
+    uint32_t ef_fixed, ef_min, ef_max;
+    float scale, bias;
+    uint8_t *route;
+    uint32_t *efq;
+    float *feat_out, *y_out;
+};
+
+__global__ __launch_bounds__(256) void k_learn_route(LearnArgs A) {
+    __shared__ __attribute__((aligned(16))) uint16_t ws[kLdsParams];
+    __shared__ __attribute__((aligned(16))) float xs[4][kMaxFeat];
+    __shared__ float xh[4][kMaxFeat];
+    __shared__ float hs[4][kHid];
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int qi = blockIdx.x * 4 + w;
+    const int np = mlp_layout(A.F).n;
+    const bool lds = np <= kLdsParams;
+    if (lds) stage_params(A.wt, np, ws);
+    __syncthreads(); /* every wave reaches it (no early return above) */
+    if (qi >= A.nq) return;
+    const int p = A.pofq[qi];
+    wave_features(A.q + (uint64_t)qi * A.dim, A.dim,
+                  p < 0 ? (double)A.count : (double)A.progcnt[p], A.count, A.cent, A.ncent,
+                  xs[w]);
+    float y0, y1;
+    wave_mlp(xs[w], A.F, lds ? ws : A.wt, A.mean, A.istd, xh[w], hs[w], y0, y1);
+    if (A.feat_out)
+        for (int j = l; j < A.F; j += 64) A.feat_out[(uint64_t)qi * A.F + j] = xs[w][j];
+    if (l == 0) {
+        if (A.y_out) {
+            A.y_out[(uint64_t)qi * 2 + 0] = y0;
+            A.y_out[(uint64_t)qi * 2 + 1] = y1;
+        }
+        uint32_t ef = A.ef_fixed;
+        uint8_t r = 1;
+        if (A.mode == 3) {
+            r = 0;
+        } else if (A.mode == 0 || A.mode == 1) {
+            const float e = expf(fminf(y0, 20.f)) * A.scale;
+            ef = (uint32_t)fminf(ceilf(e), (float)A.ef_max);
+            r = (A.mode == 1 || y1 > A.bias) ? 1 : 0;
+        }
+        A.efq[qi] = min(max(ef, A.ef_min), A.ef_max);
+        A.route[qi] = r;
+    }
 }
 
 } // namespace pv
@@ -339,7 +474,7 @@ namespace {
 
 int model_sync_bf16(FvGpuModel *m, char *errbuf, size_t errlen) {
     hipLaunchKernelGGL(k_to_bf16, dim3((m->np + 255) / 256), dim3(256), 0, m->stream, m->d_w,
-                       m->d_wbf, m->np);
+                       m->d_wbf, m->F);
     PV_TRY(hipGetLastError());
     return 0;
 }
@@ -428,7 +563,8 @@ extern "C" FvGpuModel *fv_gpu_model_create(int device, int dim, const float *cen
     if ((e = hipMalloc(&m->d_m, np * 4)) != hipSuccess) return fail(e, "alloc");
     if ((e = hipMalloc(&m->d_v, np * 4)) != hipSuccess) return fail(e, "alloc");
     if ((e = hipMalloc(&m->d_g, np * 4)) != hipSuccess) return fail(e, "alloc");
-    if ((e = hipMalloc(&m->d_wbf, np * 2)) != hipSuccess) return fail(e, "alloc");
+    if ((e = hipMalloc(&m->d_wbf, (np + 7) / 8 * 16)) != hipSuccess) return fail(e, "alloc");
+    if ((e = hipMemset(m->d_wbf, 0, (np + 7) / 8 * 16)) != hipSuccess) return fail(e, "alloc");
     if ((e = hipMalloc(&m->d_mean, (size_t)m->F * 4)) != hipSuccess) return fail(e, "alloc");
     if ((e = hipMalloc(&m->d_istd, (size_t)m->F * 4)) != hipSuccess) return fail(e, "alloc");
     if ((e = hipMemcpy(m->d_cent, centroids, (size_t)ncent * dim * 4, hipMemcpyHostToDevice)) != hipSuccess)
@@ -626,12 +762,12 @@ extern "C" int fv_gpu_batch_features(FvGpuIndex *g, FvGpuBatch *b, FvGpuModel *m
     PV_TRY(hipSetDevice(g->device));
     hipStream_t s = b->stream;
     PV_TRY(launch_bitmap(g, b, s));
-    PV_TRY(b->d_progcnt.ensure((size_t)std::max(1, b->nprog) * 4));
-    uint32_t *pc = b->d_progcnt.as<uint32_t>();
+    DevBuf cb, fb;
+    PV_TRY(cb.ensure((size_t)std::max(1, b->nprog) * 4));
+    uint32_t *pc = cb.as<uint32_t>();
     if (b->nprog > 0)
         hipLaunchKernelGGL(k_prog_count, dim3(b->nprog), dim3(256), 0, s,
                            b->d_bitmap.as<uint32_t>(), g->count_pad / 32, pc);
-    DevBuf fb;
     PV_TRY(fb.ensure((size_t)b->nq * m->F * 4));
     hipLaunchKernelGGL(k_features, dim3((b->nq + 3) / 4), dim3(256), 0, s, b->d_q, b->nq, g->dim,
                        b->d_pofq, pc, g->count, m->d_cent, m->ncent, fb.as<float>());
@@ -640,6 +776,7 @@ extern "C" int fv_gpu_batch_features(FvGpuIndex *g, FvGpuBatch *b, FvGpuModel *m
         e = hipMemcpyAsync(out, fb.p, (size_t)b->nq * m->F * 4, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     fb.release();
+    cb.release();
     if (e != hipSuccess) PV_FAIL("gpu: features: %s", hipGetErrorString(e));
     return 0;
 }
@@ -670,32 +807,47 @@ extern "C" int fv_gpu_batch_run_learned(FvGpuIndex *g, FvGpuBatch *b, FvGpuModel
         const int nq = b->nq;
         FvGpuBruteOpts bo = {};
         bo.precision = o->precision;
-        PV_TRY(launch_phase1(g, b, scan_precision(g, &bo), true, s));
         PV_TRY(b->d_route.ensure((size_t)nq));
         PV_TRY(b->d_efq.ensure((size_t)nq * 4));
-        const float *y = nullptr;
+        uint32_t *pc = need_model ? b->d_progcnt.as<uint32_t>() : nullptr;
+        if (need_model && pc == nullptr) PV_FAIL("gpu: learned run: batch has no count buffer");
+        /* bitmaps + query prep; the bitmap pass also counts passing rows */
+        PV_TRY(launch_phase1(g, b, scan_precision(g, &bo), true, s, pc));
+        LearnArgs A = {};
+        A.q = b->d_q;
+        A.nq = nq;
+        A.dim = g->dim;
+        A.pofq = b->d_pofq;
+        A.progcnt = pc;
+        A.count = g->count;
+        A.mode = o->mode;
+        A.ef_fixed = ef_fixed;
+        A.ef_min = ef_min;
+        A.ef_max = ef_max;
+        A.scale = o->ef_scale > 0.f ? o->ef_scale : 1.f;
+        A.bias = o->route_bias;
+        A.route = b->d_route.as<uint8_t>();
+        A.efq = b->d_efq.as<uint32_t>();
         if (need_model) {
-            PV_TRY(b->d_progcnt.ensure((size_t)std::max(1, b->nprog) * 4));
             PV_TRY(b->d_feat.ensure((size_t)nq * m->F * 4));
             PV_TRY(b->d_y.ensure((size_t)nq * 8));
-            uint32_t *pc = b->d_progcnt.as<uint32_t>();
-            if (b->nprog > 0)
-                hipLaunchKernelGGL(k_prog_count, dim3(b->nprog), dim3(256), 0, s,
-                                   b->d_bitmap.as<uint32_t>(), g->count_pad / 32, pc);
-            hipLaunchKernelGGL(k_features, dim3((nq + 3) / 4), dim3(256), 0, s, b->d_q, nq,
-                               g->dim, b->d_pofq, pc, g->count, m->d_cent, m->ncent,
-                               b->d_feat.as<float>());
-            hipLaunchKernelGGL(k_mlp_infer, dim3((nq + 3) / 4), dim3(256), 0, s,
-                               b->d_feat.as<float>(), nq, m->F, m->d_wbf, m->d_mean, m->d_istd,
-                               b->d_y.as<float>());
-            y = b->d_y.as<float>();
+            A.F = m->F;
+            A.cent = m->d_cent;
+            A.ncent = m->ncent;
+            A.wt = m->d_wbf;
+            A.mean = m->d_mean;
+            A.istd = m->d_istd;
+            A.feat_out = b->d_feat.as<float>();
+            A.y_out = b->d_y.as<float>();
+            hipLaunchKernelGGL(k_learn_route, dim3((nq + 3) / 4), dim3(256), 0, s, A);
+        } else {
+            hipLaunchKernelGGL(k_route, dim3((nq + 255) / 256), dim3(256), 0, s,
+                               (const float *)nullptr, nq, o->mode, ef_fixed, ef_min, ef_max,
+                               1.f, 0.f, A.route, A.efq);
         }
-        hipLaunchKernelGGL(k_route, dim3((nq + 255) / 256), dim3(256), 0, s, y, nq, o->mode,
-                           ef_fixed, ef_min, ef_max, o->ef_scale > 0.f ? o->ef_scale : 1.f,
-                           o->route_bias, b->d_route.as<uint8_t>(), b->d_efq.as<uint32_t>());
         PV_TRY(hipGetLastError());
         const uint8_t *route = b->d_route.as<uint8_t>();
-        if (run_brute_ex(g, b, &bo, 0, route, true, errbuf, errlen) != 0) return -1;
+        if (run_brute_ex(g, b, &bo, 0, route, true, pc, errbuf, errlen) != 0) return -1;
         if (o->mode != 3) {
             GraphExtra x;
             x.route = route;
@@ -718,7 +870,7 @@ extern "C" int fv_gpu_batch_run_learned(FvGpuIndex *g, FvGpuBatch *b, FvGpuModel
 
 /* the last routed run's per-query decisions (synchronous) */
 extern "C" int fv_gpu_batch_routes(FvGpuIndex *g, FvGpuBatch *b, uint8_t *route, uint32_t *ef,
-                                   char *errbuf, size_t errlen) {
+                                   float *y, char *errbuf, size_t errlen) {
     if (g == nullptr || b == nullptr || b->owner != g || b->d_route.p == nullptr)
         PV_FAIL("gpu: routes: no routed run on this batch");
     PV_TRY(hipSetDevice(g->device));
@@ -726,6 +878,10 @@ extern "C" int fv_gpu_batch_routes(FvGpuIndex *g, FvGpuBatch *b, uint8_t *route,
         PV_TRY(hipMemcpyAsync(route, b->d_route.p, (size_t)b->nq, hipMemcpyDeviceToHost, b->stream));
     if (ef)
         PV_TRY(hipMemcpyAsync(ef, b->d_efq.p, (size_t)b->nq * 4, hipMemcpyDeviceToHost, b->stream));
+    if (y) {
+        if (b->d_y.p == nullptr) PV_FAIL("gpu: routes: the last routed run used no model");
+        PV_TRY(hipMemcpyAsync(y, b->d_y.p, (size_t)b->nq * 8, hipMemcpyDeviceToHost, b->stream));
+    }
     PV_TRY(hipStreamSynchronize(b->stream));
     return 0;
 }

@@ -189,7 +189,7 @@ __global__ __launch_bounds__(256) void k_bitmap(
     int nvals, const double *__restrict__ cols, uint64_t count, int ncols, uint64_t words,
     uint32_t *__restrict__ bitmap) {
     bitmap_rows(blockIdx.x, nprog, nops, ops, poff, plen, vals, nvals, cols, count, ncols,
-                words, bitmap);
+                words, bitmap, nullptr);
 }
 
 hipError_t launch_bitmap(FvGpuIndex *g, FvGpuBatch *b, hipStream_t s) {

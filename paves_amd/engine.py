@@ -436,13 +436,15 @@ class GpuIndex:
         if rc != 0:
             raise GpuError(eb.value.decode())
 
-    def routes(self, bh, nq: int):
+    def routes(self, bh, nq: int, with_y: bool = False):
         route = np.zeros(nq, np.uint8)
         ef = np.zeros(nq, np.uint32)
+        y = np.zeros((nq, 2), np.float32) if with_y else None
         eb = L.errbuf()
-        if self.lib.fv_gpu_batch_routes(self.h, bh, ptr(route), ptr(ef), eb, len(eb)) != 0:
+        if self.lib.fv_gpu_batch_routes(self.h, bh, ptr(route), ptr(ef), ptr(y), eb,
+                                        len(eb)) != 0:
             raise GpuError(eb.value.decode())
-        return route, ef
+        return (route, ef, y) if with_y else (route, ef)
 
     def run_graph(self, bh, ef: int, max_visited: int):
         eb = L.errbuf()

@@ -59,6 +59,24 @@ def gmm128(n: int, nq: int, seed: int = BASE_SEED, ncent: int = 256,
             "qcent": qcent, "bcent": bcent, "centers": centers}
 
 
+def more_queries(data, m: int, seed: int):
+    """m further queries from the mixture of `data` (same centres, sigma 20,
+    clip and rounding as gmm128), drawn from an independent seed: training
+    sets for the learned router that never overlap the evaluated queries."""
+    r_a, r_n = _rngs(seed, 2)
+    centers = data["centers"]
+    ncent, dim = centers.shape
+    cid = r_a.integers(0, ncent, size=m, dtype=np.int32)
+    out = np.empty((m, dim), dtype=np.float32)
+    for s in range(0, m, 65536):
+        e = min(m, s + 65536)
+        x = centers[cid[s:e]] + r_n.standard_normal((e - s, dim), dtype=np.float32) * 20.0
+        np.clip(x, 0.0, 218.0, out=x)
+        np.rint(x, out=x)
+        out[s:e] = x
+    return out, cid
+
+
 def gmm768(n: int, nq: int, seed: int = BASE_SEED + 768, ncent: int = 1024,
            dim: int = 768, sigma: float = 0.15, chunk: int = 32768):
     r_c, r_b, r_bn, r_q, r_qn, r_p, r_cat = _rngs(seed, 7)

@@ -124,3 +124,54 @@ def test_training_fits_known_target(model):
     assert mse < 0.5 * const, (mse, const)
     acc = float(np.mean((pred[:, 1] > 0) == (t[k:, 1] > 0.5)))
     assert acc > 0.8, acc
+
+
+def test_routed_run_decisions_match_model():
+    """The routed run's fused decision kernel (bitmap-pass counts, features,
+    BF16 MLP) gives the outputs the separate feature and predict kernels
+    give, and route/ef follow them; brute-routed queries get the exact
+    brute-force answer, graph-routed ones valid graph hits."""
+    n, nq = 30_000, 40
+    d = synth.gmm128(n, nq, seed=17)
+    cols = synth.columns(d)
+    rows, vals = synth.postings(cols)
+    post = E.Postings(rows, vals)
+    preds = [synth.filter_pred(synth.FILTER_CLASSES[i % 6], d["qcent"][i]) for i in range(nq)]
+    from paves_amd import hnsw
+    gr = hnsw.build(d["base"], M=8, ef_construction=48, threads=8)
+    ent = hnsw.entries(gr, d["base"], d["queries"])
+    b = E.concat([E.serialize(p, n, post) for p in preds], n, entries=ent)
+    g = E.GpuIndex(d["base"], cols, rows, l0=gr["l0"], precision=L.FV_GPU_PREC_BF16)
+    rng = np.random.default_rng(2)
+    m = learned.Model(128, learned.kmeans(d["base"], k=64, iters=2, sample=4000))
+    F = R.nfeat(128)
+    p = (rng.standard_normal(R.nparams(F)) * 0.1).astype(np.float32)
+    bh = g.stage(d["queries"], b, 10)
+    try:
+        f = m.batch_features(g, bh, nq)
+        m.set_params(p, f.mean(0), f.std(0) + 1e-3)
+        want = m.predict(f)
+        # bias 0 on a random model: a mix of routes
+        for rep in range(2):    # the second run checks the count reset
+            g.run_learned(bh, m, L.LearnedOpts(k=10, mode=0, ef_min=10, ef_max=64))
+            g.sync()
+            route, ef, y = g.routes(bh, nq, with_y=True)
+            assert np.array_equal(y, want), rep
+            exp_ef = np.clip(np.ceil(np.exp(np.minimum(want[:, 0], 20.0))), 10, 64)
+            assert np.array_equal(ef, exp_ef.astype(np.uint32))
+            assert np.array_equal(route, (want[:, 1] > 0).astype(np.uint8))
+        out, cnt, flag, rc = g.fetch(bh, nq, 10)
+        assert rc == 0
+        bo, bc, bt = g.brute(d["queries"], E.concat([E.serialize(p, n, post) for p in preds], n), 10)
+        for q in range(nq):
+            ids = out[q, : cnt[q]]["node"]
+            if route[q] == 0:
+                assert cnt[q] == bc[q] and np.array_equal(ids, bo[q, : bc[q]]["node"])
+            else:
+                assert len(set(ids.tolist())) == len(ids)
+                if preds[q] is not None:
+                    assert E.eval_pred(preds[q], cols)[ids.astype(np.int64)].all()
+    finally:
+        g.free_batch(bh)
+        m.close()
+        g.close()
