@@ -947,7 +947,7 @@ int stage_batch(FvGpuIndex *g, FvGpuBatch *b, int nq, const float *queries,
     if (queries == nullptr || qd == nullptr) PV_FAIL("gpu: NULL queries or descriptors");
     if (b->stream == nullptr) {
         b->stream = g->bstreams[g->next_stream];
-        g->next_stream = (g->next_stream + 1) % FvGpuIndex::kBatchStreams;
+        g->next_stream = (g->next_stream + 1) % g->nstreams;
     }
     ProgSet ps;
     if (compile_programs(nq, qd, nodes, nnodes, children, nchildren, invals,
@@ -1496,7 +1496,7 @@ extern "C" int fv_gpu_batch_run_brute(FvGpuIndex *g, FvGpuBatch *b,
 extern "C" int fv_gpu_sync(FvGpuIndex *g, char *errbuf, size_t errlen) {
     if (g == nullptr) PV_FAIL("gpu: NULL index");
     PV_TRY(hipStreamSynchronize(g->stream));
-    for (hipStream_t bs : g->bstreams) PV_TRY(hipStreamSynchronize(bs));
+    for (int i = 0; i < g->nstreams; i++) PV_TRY(hipStreamSynchronize(g->bstreams[i]));
     fill_stats_times(g);
     return 0;
 }

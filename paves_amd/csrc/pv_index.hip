@@ -232,7 +232,13 @@ static FvGpuIndex *upload_impl(const float *vectors, const double *cols,
             g->ncu = prop.multiProcessorCount;
     }
     UP_TRY(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
-    for (hipStream_t &bs : g->bstreams) UP_TRY(hipStreamCreateWithFlags(&bs, hipStreamNonBlocking));
+    {
+        const char *ev = getenv("PAVES_BATCH_STREAMS");
+        const int ns = ev ? atoi(ev) : 3;
+        g->nstreams = std::min(FvGpuIndex::kMaxBatchStreams, std::max(1, ns));
+    }
+    for (int i = 0; i < g->nstreams; i++)
+        UP_TRY(hipStreamCreateWithFlags(&g->bstreams[i], hipStreamNonBlocking));
     for (int i = 0; i < 16; i++) UP_TRY(hipEventCreate(&g->ev[i]));
     hipStream_t s = g->stream;
 

@@ -207,8 +207,9 @@ struct FvGpuIndex {
     /* Batches are spread over a few streams (fixed per batch, so runs of
      * one batch stay ordered) so that the latency-bound small kernels of
      * one batch overlap the HBM-bound scan of another. */
-    static constexpr int kBatchStreams = 3;
-    hipStream_t bstreams[kBatchStreams] = {};
+    static constexpr int kMaxBatchStreams = 8;
+    int nstreams = 3;                /* PAVES_BATCH_STREAMS (1..8), default 3 */
+    hipStream_t bstreams[kMaxBatchStreams] = {};
     int next_stream = 0;
     hipStream_t ev_stream = nullptr; /* stream of the run being timed */
     bool timing = false;
