@@ -169,6 +169,10 @@ typedef struct FvGpuBruteOpts {
     int32_t route;       /* 0 auto, 1 force full scan, 2 force span gather */
     uint32_t overfetch;  /* re-rank candidates per query (0 = auto) */
     uint32_t sample_stride; /* threshold pre-pass stride (0 = auto) */
+    int32_t inline_pred; /* 1: evaluate one-comparison predicates inside the
+                            BF16 scan instead of building bitmaps (batches
+                            whose predicates all compare one column) */
+    int32_t wg_per_cu;   /* main scan workgroups per CU and query tile (0 = auto) */
 } FvGpuBruteOpts;
 
 /* Per-call statistics of the last batch run on this index. Kernel times are
@@ -228,6 +232,17 @@ FvGpuBatch *fv_gpu_batch_stage(FvGpuIndex *g, int nq, const float *queries,
                                const FvGpuSpan *spans, int nspans_total,
                                const uint64_t *spanoff, int nspanoff,
                                uint32_t topb, char *errbuf, size_t errlen);
+/* Uneven batches: up to 4096 queries, each with its own k (topb_q[q] in
+ * 1..topb; outputs have stride topb). Otherwise as fv_gpu_batch_stage. */
+FvGpuBatch *fv_gpu_batch_stage_ex(FvGpuIndex *g, int nq, const float *queries,
+                                  const FvGpuQueryDesc *qd, const uint8_t *dedup,
+                                  const FvGpuPredNode *nodes, int nnodes,
+                                  const int32_t *children, int nchildren,
+                                  const double *invals, int ninvals,
+                                  const FvGpuSpan *spans, int nspans_total,
+                                  const uint64_t *spanoff, int nspanoff,
+                                  const uint32_t *topb_q, uint32_t topb, char *errbuf,
+                                  size_t errlen);
 /* Enqueue the brute pipeline for a staged batch on the batch's stream
  * (asynchronous; 0 / -1 + errbuf). Each batch is given one of the index's
  * batch streams when it is staged, so different batches may overlap. */

@@ -48,16 +48,18 @@ EXT_SYMBOLS = ["fv_gpu_set_timing", "fv_gpu_last_stats", "fv_gpu_brute_batch_ex"
                "fv_gpu_model_nparams", "fv_gpu_model_create", "fv_gpu_model_free",
                "fv_gpu_model_set_params", "fv_gpu_model_get_params", "fv_gpu_model_grad",
                "fv_gpu_model_train", "fv_gpu_model_predict", "fv_gpu_batch_features",
-               "fv_gpu_batch_run_learned", "fv_gpu_batch_routes"]
+               "fv_gpu_batch_run_learned", "fv_gpu_batch_routes", "fv_gpu_batch_stage_ex"]
 
 
 class BruteOpts(C.Structure):
     """FvGpuBruteOpts; precision -1 = the index's precision."""
     _fields_ = [("precision", C.c_int32), ("route", C.c_int32),
-                ("overfetch", C.c_uint32), ("sample_stride", C.c_uint32)]
+                ("overfetch", C.c_uint32), ("sample_stride", C.c_uint32),
+                ("inline_pred", C.c_int32), ("wg_per_cu", C.c_int32)]
 
-    def __init__(self, precision=-1, route=0, overfetch=0, sample_stride=0):
-        super().__init__(precision, route, overfetch, sample_stride)
+    def __init__(self, precision=-1, route=0, overfetch=0, sample_stride=0, inline_pred=0,
+                 wg_per_cu=0):
+        super().__init__(precision, route, overfetch, sample_stride, inline_pred, wg_per_cu)
 
 
 class LearnedOpts(C.Structure):
@@ -129,6 +131,10 @@ def load() -> C.CDLL:
     lib.fv_gpu_batch_stage.argtypes = [_P, C.c_int, _P, _P, _P, _P, C.c_int, _P, C.c_int,
                                        _P, C.c_int, _P, C.c_int, _P, C.c_int, C.c_uint32,
                                        C.c_char_p, sz]
+    lib.fv_gpu_batch_stage_ex.restype = _P
+    lib.fv_gpu_batch_stage_ex.argtypes = [_P, C.c_int, _P, _P, _P, _P, C.c_int, _P, C.c_int,
+                                          _P, C.c_int, _P, C.c_int, _P, C.c_int, _P, C.c_uint32,
+                                          C.c_char_p, sz]
     lib.fv_gpu_batch_run_brute.argtypes = [_P, _P, C.POINTER(BruteOpts), C.c_char_p, sz]
     lib.fv_gpu_batch_run_graph.argtypes = [_P, _P, C.c_uint32, C.c_uint64, C.c_char_p, sz]
     lib.fv_gpu_sync.argtypes = [_P, C.c_char_p, sz]

@@ -176,6 +176,8 @@ struct ScanArgs {
     const uint32_t *ones;     /* all-ones bitmap row (queries without predicate) */
     uint32_t *status;         /* per-query status bits */
     const uint8_t *skip;      /* optional [nq]: 1 = query routed elsewhere */
+    const InlPred *inl;       /* inline predicates [nqp] (INL kernels) */
+    const double *icol;       /* their column [count] */
 };
 
 /* true when every query of the WG's 64-query tile is skipped (uniform) */
@@ -249,8 +251,10 @@ struct TileIn {
     float xn[16];      /* ||x||^2 of the lane's 16 rows */
     uint32_t w[2];     /* predicate bitmap word of the tile, per query block */
     uint32_t valid;    /* rows < count */
+    double cv[16];     /* inline predicates: the column value of the lane's rows */
 };
 
+template <bool INL = false>
 __device__ __forceinline__ void load_tile_in(const ScanArgs &A, uint32_t pt,
                                              const uint32_t *const (&wr)[2], TileIn &in) {
     const int l = threadIdx.x & 63;
@@ -262,8 +266,15 @@ __device__ __forceinline__ void load_tile_in(const ScanArgs &A, uint32_t pt,
         in.xn[g4 * 4 + 2] = v.z;
         in.xn[g4 * 4 + 3] = v.w;
     }
-    in.w[0] = wr[0][pt];
-    in.w[1] = wr[1][pt];
+    if (INL) {
+        const uint64_t last = A.count - 1;
+#pragma unroll
+        for (int i = 0; i < 16; i++)
+            in.cv[i] = A.icol[min((uint64_t)pt * 32 + acc_row(i, l), last)];
+    } else {
+        in.w[0] = wr[0][pt];
+        in.w[1] = wr[1][pt];
+    }
     const uint64_t row0 = (uint64_t)pt * 32;
     in.valid = row0 + 32 <= A.count ? 0xFFFFFFFFu
                : (A.count > row0 ? (0xFFFFFFFFu >> (32 - (A.count - row0))) : 0u);
@@ -277,19 +288,34 @@ __device__ __forceinline__ void load_tile_in(const ScanArgs &A, uint32_t pt,
  * passing distance of the lane's 16 rows per query (one group); MAIN
  * appends passing rows with dist <= tau to the WG's candidates.
  */
-template <bool PREPASS, bool BF>
+template <bool PREPASS, bool BF, bool INL = false>
 __device__ __forceinline__ void tile_epilogue(const ScanArgs &A, int qbase, uint32_t t,
                                               uint32_t pt, const float (&acc)[2][16],
                                               const TileIn &in, const float (&qn)[2],
                                               const float (&tq)[2], bool active,
-                                              CandLds &C) {
+                                              CandLds &C, const int (&imask)[2] = {-1, -1},
+                                              const double (&ival)[2] = {0.0, 0.0}) {
     const int l = threadIdx.x & 63;
 #pragma unroll
     for (int qb = 0; qb < 2; qb++) {
         const int qi = (l & 31) + 32 * qb;
         const int q = qbase + qi;
         if (q >= A.nq || !active) continue;
-        const uint32_t word = in.w[qb] & in.valid;
+        uint32_t word;
+        if (INL) {
+            /* the query's one comparison, on the lane's 16 rows */
+            if (imask[qb] < 0) {
+                word = 0xFFFFFFFFu;
+            } else {
+                word = 0;
+#pragma unroll
+                for (int i = 0; i < 16; i++)
+                    word |= cmp_pass(in.cv[i], ival[qb], imask[qb]) << acc_row(i, l);
+            }
+            word &= in.valid;
+        } else {
+            word = in.w[qb] & in.valid;
+        }
         if (PREPASS) {
             float m = __uint_as_float(0x7f800000u);
 #pragma unroll
@@ -368,7 +394,7 @@ __device__ __forceinline__ void load_query_state(const ScanArgs &A, int qbase,
  * for the wait-count analysis. B operands (the 64 queries) sit in LDS.
  * Requires dim % 128 == 0.
  */
-template <bool PREPASS, int NW>
+template <bool PREPASS, int NW, bool INL = false>
 __global__ __launch_bounds__(NW * 64) void k_scan_bf16(ScanArgs A) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     const int S = A.dim / 16;            /* MFMA k-steps per tile */
@@ -391,6 +417,19 @@ __global__ __launch_bounds__(NW * 64) void k_scan_bf16(ScanArgs A) {
     float tq[2], qn[2];
     const uint32_t *wr[2];
     load_query_state(A, qbase, tq, qn, wr);
+    int imask[2] = {-1, -1};
+    double ival[2] = {0.0, 0.0};
+    if (INL) {
+#pragma unroll
+        for (int qb = 0; qb < 2; qb++) {
+            const int q = qbase + (l & 31) + 32 * qb;
+            if (q < A.nq) {
+                const InlPred ip = A.inl[q];
+                imask[qb] = ip.col < 0 ? -1 : ip.mask;
+                ival[qb] = ip.value;
+            }
+        }
+    }
 
     const uint32_t nwg = gridDim.x;
     const uint32_t t0 = (uint32_t)(((uint64_t)A.ntiles * blockIdx.x) / nwg);
@@ -407,7 +446,7 @@ __global__ __launch_bounds__(NW * 64) void k_scan_bf16(ScanArgs A) {
         const u32x4 *p = vb + (uint64_t)pt * (4 * A.dim) + (uint64_t)(uc % NCH) * 512 + l;
 #pragma unroll
         for (int j = 0; j < 8; j++) a[j] = __builtin_nontemporal_load(p + 64 * j);
-        load_tile_in(A, pt, wr, in);
+        load_tile_in<INL>(A, pt, wr, in);
     };
     auto do_unit = [&](uint32_t u, const u32x4 (&a)[8], const TileIn &in, f32x16 &acc0,
                        f32x16 &acc1, bool active) {
@@ -433,7 +472,8 @@ __global__ __launch_bounds__(NW * 64) void k_scan_bf16(ScanArgs A) {
                 acc[1][i] = acc1[i];
             }
             const uint32_t t = tile_of_unit(u);
-            tile_epilogue<PREPASS, true>(A, qbase, t, t * A.stride, acc, in, qn, tq, active, C);
+            tile_epilogue<PREPASS, true, INL>(A, qbase, t, t * A.stride, acc, in, qn, tq, active,
+                                              C, imask, ival);
         }
     };
 
@@ -691,11 +731,13 @@ __global__ __launch_bounds__(256) void k_tau(const uint64_t *__restrict__ gmin,
                                              float *__restrict__ tau,
                                              uint32_t *__restrict__ pcnt,
                                              uint32_t *__restrict__ status,
-                                             const uint8_t *__restrict__ skip) {
+                                             const uint8_t *__restrict__ skip,
+                                             const uint32_t *__restrict__ topbq) {
     __shared__ uint32_t hist[256], tmp[8], bc[8];
     __shared__ uint64_t sel[1024], wb[4];
     __shared__ uint32_t nsel;
     const int q = blockIdx.x;
+    if (topbq) B = topbq[q];
     if (skip && skip[q]) { /* routed elsewhere: no candidates, no outputs */
         if (threadIdx.x == 0) {
             tau[q] = -__uint_as_float(0x7f800000u);
@@ -809,7 +851,8 @@ __global__ __launch_bounds__(256) void k_select(
     const float *__restrict__ vecs, const float *__restrict__ qf, int dim,
     FvHit *__restrict__ out, uint32_t out_stride, uint32_t *__restrict__ out_cnt,
     uint8_t *__restrict__ out_flag, uint32_t *__restrict__ status,
-    const uint8_t *__restrict__ skip, uint32_t *__restrict__ progcnt_reset, int nprog) {
+    const uint8_t *__restrict__ skip, uint32_t *__restrict__ progcnt_reset, int nprog,
+    const uint32_t *__restrict__ topbq) {
     __shared__ uint64_t sel[kSelMax], wb[4];
     __shared__ uint32_t hist[256], tmp[8], bc[8];
     __shared__ uint32_t nsel;
@@ -819,6 +862,7 @@ __global__ __launch_bounds__(256) void k_select(
      * clear them for the batch's next run */
     if (progcnt_reset && q == 0)
         for (int i = t; i < nprog; i += blockDim.x) progcnt_reset[i] = 0;
+    if (topbq) topb = topbq[q]; /* per-query k of an uneven batch */
     if (skip && skip[q]) return; /* the graph search writes this query */
     const uint64_t *pk = pool + (uint64_t)q * cap_pool;
     const uint32_t ntot = pcnt[(uint64_t)q * kPcntStride];
@@ -940,10 +984,14 @@ int stage_batch(FvGpuIndex *g, FvGpuBatch *b, int nq, const float *queries,
                 const int32_t *children, int nchildren, const double *invals,
                 int ninvals, const FvGpuSpan *spans, int nspans_total,
                 const uint64_t *spanoff, int nspanoff, uint32_t topb,
-                char *errbuf, size_t errlen) {
-    if (nq < 1 || nq > FV_GPU_MAX_BATCH || topb == 0 || topb > FV_GPU_MAX_TOPB ||
+                char *errbuf, size_t errlen, const uint32_t *topb_q, int max_nq) {
+    if (nq < 1 || nq > max_nq || topb == 0 || topb > FV_GPU_MAX_TOPB ||
         nspans_total > FV_GPU_MAX_SPANS * nq)
         PV_FAIL("gpu: bad brute batch (nq=%d topb=%u)", nq, topb);
+    if (topb_q)
+        for (int q = 0; q < nq; q++)
+            if (topb_q[q] == 0 || topb_q[q] > topb)
+                PV_FAIL("gpu: query %d: k %u outside 1..%u", q, topb_q[q], topb);
     if (queries == nullptr || qd == nullptr) PV_FAIL("gpu: NULL queries or descriptors");
     if (b->stream == nullptr) {
         b->stream = g->bstreams[g->next_stream];
@@ -995,8 +1043,21 @@ int stage_batch(FvGpuIndex *g, FvGpuBatch *b, int nq, const float *queries,
     b->nprog = (int)ps.off.size();
     b->nops = (int)ps.ops.size();
     b->nvals = (int)ps.vals.size();
-    if (b->nprog > kBmProgs) PV_FAIL("gpu: %d distinct predicates exceed %d", b->nprog, kBmProgs);
     const int nqp = b->nqt * kQT;
+    /* inline predicates: possible when every predicate is one comparison
+     * on a single column */
+    b->inl_ok = true;
+    b->inl_col = -1;
+    for (int q = 0; q < nq && b->inl_ok; q++) {
+        const int p = ps.of_query[q];
+        if (p < 0) continue;
+        const PvOp &op = ps.ops[ps.off[p]];
+        if (ps.len[p] != 1 || op.kind != OP_CMP || (b->inl_col >= 0 && op.col != b->inl_col))
+            b->inl_ok = false;
+        else
+            b->inl_col = op.col;
+    }
+    b->has_topbq = topb_q != nullptr;
     const int dim = g->dim;
 
     /* packed staging layout */
@@ -1010,6 +1071,8 @@ int stage_batch(FvGpuIndex *g, FvGpuBatch *b, int nq, const float *queries,
     const size_t o_pofq = off; off = al256(off + (size_t)nqp * 4);
     const size_t o_sp = off; off = al256(off + (size_t)std::max(nspans_total, 0) * sizeof(FvGpuSpan));
     const size_t o_so = off; off = al256(off + (size_t)std::max(nspanoff, 0) * 8);
+    const size_t o_inl = off; off = al256(off + (b->inl_ok ? (size_t)nqp * sizeof(InlPred) : 0));
+    const size_t o_tq = off; off = al256(off + (topb_q ? (size_t)nq * 4 : 0));
     const size_t total_bytes = off;
     PV_TRY(b->h_stage.ensure(total_bytes));
     PV_TRY(b->d_stage.ensure(total_bytes));
@@ -1027,6 +1090,22 @@ int stage_batch(FvGpuIndex *g, FvGpuBatch *b, int nq, const float *queries,
     }
     if (nspans_total > 0) memcpy(h + o_sp, spans, (size_t)nspans_total * sizeof(FvGpuSpan));
     if (nspanoff > 0) memcpy(h + o_so, spanoff, (size_t)nspanoff * 8);
+    if (b->inl_ok) {
+        InlPred *ip = (InlPred *)(h + o_inl);
+        for (int q = 0; q < nqp; q++) {
+            const int p = q < nq ? ps.of_query[q] : -1;
+            ip[q].col = -1;
+            ip[q].mask = 0;
+            ip[q].value = 0.0;
+            if (p >= 0) {
+                const PvOp &op = ps.ops[ps.off[p]];
+                ip[q].col = op.col;
+                ip[q].mask = op.b;
+                ip[q].value = op.value;
+            }
+        }
+    }
+    if (topb_q) memcpy(h + o_tq, topb_q, (size_t)nq * 4);
     PV_TRY(hipMemcpyAsync(b->d_stage.p, h, total_bytes, hipMemcpyHostToDevice, b->stream));
     unsigned char *d = b->d_stage.as<unsigned char>();
     b->d_q = (float *)(d + o_q);
@@ -1038,10 +1117,13 @@ int stage_batch(FvGpuIndex *g, FvGpuBatch *b, int nq, const float *queries,
     b->d_pofq = (int32_t *)(d + o_pofq);
     b->d_spans = (FvGpuSpan *)(d + o_sp);
     b->d_spanoff = (uint64_t *)(d + o_so);
+    b->d_inl_p = b->inl_ok ? (const InlPred *)(d + o_inl) : nullptr;
+    b->d_topbq_p = topb_q ? (const uint32_t *)(d + o_tq) : nullptr;
 
     PV_TRY(b->d_qbf.ensure((size_t)nqp * dim * 2));
     PV_TRY(b->d_qaux.ensure((size_t)nqp * 16));
-    if (b->nprog > 0) PV_TRY(b->d_bitmap.ensure((size_t)b->nprog * (g->count_pad / 32) * 4));
+    if (b->nprog > 0 && b->nprog <= kBmProgs)
+        PV_TRY(b->d_bitmap.ensure((size_t)b->nprog * (g->count_pad / 32) * 4));
     b->out_stride = topb;
     PV_TRY(b->d_out.ensure((size_t)nq * topb * sizeof(FvHit)));
     PV_TRY(b->d_cnt.ensure((size_t)nq * 4));
@@ -1061,6 +1143,18 @@ int scan_precision(const FvGpuIndex *g, const FvGpuBruteOpts *o) {
     if (p == FV_GPU_PREC_BF16 && (g->d_vbf == nullptr || g->dim % 128 != 0 || g->dim > 1024))
         p = FV_GPU_PREC_FP32;
     return p;
+}
+
+/* BF16 scan launch: waves per WG (4 / 8) and inline predicates pick the
+ * kernel instance */
+template <bool PREPASS>
+static void launch_scan_bf16(int nw, bool inl, uint32_t gx, int nqt, size_t shm,
+                             hipStream_t s, const ScanArgs &A) {
+    const dim3 grid(gx, nqt);
+    if (nw == 4 && !inl) hipLaunchKernelGGL((k_scan_bf16<PREPASS, 4, false>), grid, dim3(256), shm, s, A);
+    else if (nw == 4) hipLaunchKernelGGL((k_scan_bf16<PREPASS, 4, true>), grid, dim3(256), shm, s, A);
+    else if (!inl) hipLaunchKernelGGL((k_scan_bf16<PREPASS, 8, false>), grid, dim3(512), shm, s, A);
+    else hipLaunchKernelGGL((k_scan_bf16<PREPASS, 8, true>), grid, dim3(512), shm, s, A);
 }
 
 /* Phase 1 of a run: predicate bitmaps (when `bitmap`) and the query
@@ -1134,8 +1228,16 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
     b->last_route = route;
     st.route = route;
 
+    /* inline predicates: one comparison per query evaluated in the scan
+     * epilogue, no bitmap pass (BF16 scan, all predicates on one column) */
+    const bool inl = prec == FV_GPU_PREC_BF16 && o && o->inline_pred && b->inl_ok &&
+                     b->inl_col >= 0 && !have_phase1;
+    if (route == 1 && !inl && b->nprog > kBmProgs)
+        PV_FAIL("gpu: %d distinct predicates: at most %d without inline evaluation",
+                b->nprog, kBmProgs);
+
     ev_record(g, 0);
-    if (!have_phase1) PV_TRY(launch_phase1(g, b, prec, route == 1, s, nullptr));
+    if (!have_phase1) PV_TRY(launch_phase1(g, b, prec, route == 1 && !inl, s, nullptr));
     ev_record(g, 6);
     ev_record(g, 1);
 
@@ -1167,7 +1269,8 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
                            g->d_vecs, b->d_q, dim, b->d_out.as<FvHit>(),
                            b->out_stride, b->d_cnt.as<uint32_t>(),
                            b->d_flag.as<uint8_t>(), qa.status, skip,
-                           progcnt_reset, b->nprog);
+                           progcnt_reset, b->nprog,
+                           b->d_topbq_p);
         PV_TRY(hipGetLastError());
         ev_record(g, 5);
         st.scan_rows = b->sum_total;
@@ -1180,8 +1283,18 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
     const uint32_t ntiles = (uint32_t)(g->count_pad / 32);
     uint32_t stride = stride_override ? stride_override
                                       : (o && o->sample_stride ? o->sample_stride : 0);
+    /* main-scan workgroups per query tile (also bounds the candidates each
+     * WG collects per query: about stride x B / gx of them) */
+    const int nw = dim <= 256 ? 4 : 8;
+    const bool bf = prec == FV_GPU_PREC_BF16;
+    const int wpc = (o && o->wg_per_cu > 0) ? o->wg_per_cu : (bf && nw == 8 ? 1 : 2);
+    uint32_t gx = (uint32_t)std::max(64, (wpc * g->ncu) / std::max(1, b->nqt));
+    gx = std::min<uint32_t>(gx, std::max<uint32_t>(1, ntiles / nw));
     if (stride == 0) {
-        stride = B <= 64 ? 64 : std::max<uint32_t>(1, 4096 / B);
+        /* the B-th smallest minimum over groups of 16 sampled rows: about
+         * stride x B rows of a query fall under it; keep that near half the
+         * WG's per-query LDS slots per WG */
+        stride = std::min<uint32_t>(64, std::max<uint32_t>(1, (uint32_t)kLdsCand / 2 * gx / B));
         const uint32_t min_stride = (ntiles + 131071) / 131072;   /* <= 2^18 groups */
         stride = std::max(stride, min_stride);
     }
@@ -1217,17 +1330,18 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
     A.ones = g->d_ones;
     A.status = qa.status;
     A.skip = skip;
+    A.inl = inl ? b->d_inl_p : nullptr;
+    A.icol = inl ? g->d_cols + (uint64_t)b->inl_col * g->count : nullptr;
 
-    const bool bf = prec == FV_GPU_PREC_BF16;
     /* 4 waves per WG up to 256 dims (2 WGs per CU), 8 waves above (the
      * query operands then take most of the LDS) */
-    const int nw = dim <= 256 ? 4 : 8;
     const size_t shm_bf = (size_t)2 * (dim / 16) * 64 * 16 + sizeof(CandLds);
     if (bf && shm_bf > 64 * 1024) {
-        PV_TRY(hipFuncSetAttribute((const void *)k_scan_bf16<true, 8>,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm_bf));
-        PV_TRY(hipFuncSetAttribute((const void *)k_scan_bf16<false, 8>,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm_bf));
+        for (const void *f : {(const void *)k_scan_bf16<true, 8, false>,
+                              (const void *)k_scan_bf16<false, 8, false>,
+                              (const void *)k_scan_bf16<true, 8, true>,
+                              (const void *)k_scan_bf16<false, 8, true>})
+            PV_TRY(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm_bf));
     }
     if (bf && shm_bf > 160 * 1024) PV_FAIL("gpu: dim %d too large for the BF16 scan", dim);
     /* pre-pass over sampled tiles */
@@ -1237,10 +1351,7 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
         P.stride = stride;
         if (bf) {
             const uint32_t gx = std::max<uint32_t>(1, (stiles + nw - 1) / nw);
-            if (nw == 4)
-                hipLaunchKernelGGL((k_scan_bf16<true, 4>), dim3(gx, b->nqt), dim3(256), shm_bf, s, P);
-            else
-                hipLaunchKernelGGL((k_scan_bf16<true, 8>), dim3(gx, b->nqt), dim3(512), shm_bf, s, P);
+            launch_scan_bf16<true>(nw, inl, gx, b->nqt, shm_bf, s, P);
         } else {
             const uint32_t gx = std::max<uint32_t>(1, (stiles + 3) / 4);
             hipLaunchKernelGGL(k_scan_f32<true>, dim3(gx, b->nqt), dim3(256), 0, s, P);
@@ -1250,20 +1361,15 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
     ev_record(g, 2);
     hipLaunchKernelGGL(k_tau, dim3(nq), dim3(256), 0, s, b->d_gmin.as<uint64_t>(),
                        b->ngroups, B, qa.eps, qa.tau, b->d_pcnt.as<uint32_t>(),
-                       qa.status, skip);
+                       qa.status, skip, b->d_topbq_p);
     PV_TRY(hipGetLastError());
     ev_record(g, 3);
     {
         ScanArgs M = A;
         M.ntiles = ntiles;
         M.stride = 1;
-        uint32_t gx = (uint32_t)std::max(64, (2 * g->ncu) / std::max(1, b->nqt));
-        if (bf && nw == 8) gx = std::max<uint32_t>(64, g->ncu / std::max(1, b->nqt));
-        gx = std::min<uint32_t>(gx, std::max<uint32_t>(1, M.ntiles / nw));
-        if (bf && nw == 4)
-            hipLaunchKernelGGL((k_scan_bf16<false, 4>), dim3(gx, b->nqt), dim3(256), shm_bf, s, M);
-        else if (bf)
-            hipLaunchKernelGGL((k_scan_bf16<false, 8>), dim3(gx, b->nqt), dim3(512), shm_bf, s, M);
+        if (bf)
+            launch_scan_bf16<false>(nw, inl, gx, b->nqt, shm_bf, s, M);
         else
             hipLaunchKernelGGL(k_scan_f32<false>, dim3(gx, b->nqt), dim3(256), 0, s, M);
         PV_TRY(hipGetLastError());
@@ -1274,7 +1380,8 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
                        b->cap_pool, B, qa.eps, g->d_vecs, b->d_q, dim,
                        b->d_out.as<FvHit>(), b->out_stride,
                        b->d_cnt.as<uint32_t>(), b->d_flag.as<uint8_t>(), qa.status, skip,
-                           progcnt_reset, b->nprog);
+                           progcnt_reset, b->nprog,
+                           b->d_topbq_p);
     PV_TRY(hipGetLastError());
     ev_record(g, 5);
     st.scan_rows = g->count;
@@ -1469,6 +1576,37 @@ extern "C" FvGpuBatch *fv_gpu_batch_stage(
         if (stage_batch(g, b, nq, queries, qd, dedup, nodes, nnodes, children,
                         nchildren, invals, ninvals, spans, nspans_total, spanoff,
                         nspanoff, topb, errbuf, errlen) != 0 ||
+            hipStreamSynchronize(b->stream) != hipSuccess) {
+            free_batch_buffers(b);
+            delete b;
+            return nullptr;
+        }
+        return b;
+    } catch (...) {
+        snprintf(errbuf, errlen, "gpu: internal error");
+        return nullptr;
+    }
+}
+
+extern "C" FvGpuBatch *fv_gpu_batch_stage_ex(
+    FvGpuIndex *g, int nq, const float *queries, const FvGpuQueryDesc *qd,
+    const uint8_t *dedup, const FvGpuPredNode *nodes, int nnodes,
+    const int32_t *children, int nchildren, const double *invals, int ninvals,
+    const FvGpuSpan *spans, int nspans_total, const uint64_t *spanoff,
+    int nspanoff, const uint32_t *topb_q, uint32_t topb, char *errbuf, size_t errlen) {
+    if (g == nullptr) {
+        snprintf(errbuf, errlen, "gpu: NULL index");
+        return nullptr;
+    }
+    try {
+        if (hipSetDevice(g->device) != hipSuccess) {
+            snprintf(errbuf, errlen, "gpu: hipSetDevice failed");
+            return nullptr;
+        }
+        FvGpuBatch *b = new FvGpuBatch();
+        if (stage_batch(g, b, nq, queries, qd, dedup, nodes, nnodes, children,
+                        nchildren, invals, ninvals, spans, nspans_total, spanoff,
+                        nspanoff, topb, errbuf, errlen, topb_q, kMaxBatchEx) != 0 ||
             hipStreamSynchronize(b->stream) != hipSuccess) {
             free_batch_buffers(b);
             delete b;

@@ -47,6 +47,16 @@ constexpr int kLdsCand = 32;     /* per-query LDS candidate slots in a scan WG *
 constexpr int kSelMax = 4096;    /* max keys sorted in one select WG */
 constexpr int kMaxProgDepth = 64;
 
+constexpr int kMaxBatchEx = 4096; /* queries per batch through the extension API */
+
+/* one comparison evaluated inside the scan (col -1: no predicate) */
+struct InlPred {
+    int32_t col;
+    int32_t mask;   /* kCmpPassMask of the comparison */
+    double value;
+};
+static_assert(sizeof(InlPred) == 16, "InlPred layout");
+
 /* status bits per query */
 constexpr uint32_t kStOverflow = 1u;   /* candidate pool overflowed */
 constexpr uint32_t kStCertFail = 2u;   /* re-rank set exceeded capacity */
@@ -139,6 +149,13 @@ struct FvGpuBatch {
     uint64_t sum_total = 0; /* sum of span enumeration sizes */
     uint64_t max_total = 0;
     uint64_t max_entry = 0; /* largest qd[q].entry (graph runs) */
+    bool has_topbq = false; /* per-query k (uneven batches); topb = the largest */
+    const uint32_t *d_topbq_p = nullptr;   /* u32 [nq] in the staging area */
+    /* inline predicates: every predicate is one comparison on column
+     * inl_col, evaluated inside the scan instead of through a bitmap */
+    bool inl_ok = false;
+    int inl_col = -1;
+    const pv::InlPred *d_inl_p = nullptr;  /* [nqp] in the staging area */
     bool query_exact = true;
     std::vector<uint64_t> total;   /* host copy of qd[q].total */
     std::vector<int32_t> nspans;
@@ -253,7 +270,8 @@ int stage_batch(FvGpuIndex *g, FvGpuBatch *b, int nq, const float *queries,
                 const int32_t *children, int nchildren, const double *invals,
                 int ninvals, const FvGpuSpan *spans, int nspans_total,
                 const uint64_t *spanoff, int nspanoff, uint32_t topb,
-                char *errbuf, size_t errlen);
+                char *errbuf, size_t errlen, const uint32_t *topb_q = nullptr,
+                int max_nq = FV_GPU_MAX_BATCH);
 void free_batch_buffers(FvGpuBatch *b);
 
 /* pv_graph.hip */

@@ -423,6 +423,20 @@ class GpuIndex:
             raise GpuError(eb.value.decode())
         return C.c_void_p(h)
 
+    def stage_ex(self, queries: np.ndarray, b: Batch, k_per_query: np.ndarray):
+        """Uneven batch: per-query k (outputs have stride max k)."""
+        q = np.ascontiguousarray(queries, dtype=np.float32)
+        kq = np.ascontiguousarray(k_per_query, dtype=np.uint32)
+        eb = L.errbuf()
+        h = self.lib.fv_gpu_batch_stage_ex(self.h, b.nq, ptr(q), ptr(b.qd), ptr(b.dedup),
+                                           ptr(b.nodes), len(b.nodes), ptr(b.children),
+                                           len(b.children), ptr(b.invals), len(b.invals),
+                                           ptr(b.spans), len(b.spans), ptr(b.spanoff),
+                                           len(b.spanoff), ptr(kq), int(kq.max()), eb, len(eb))
+        if not h:
+            raise GpuError(eb.value.decode())
+        return C.c_void_p(h)
+
     def run_brute(self, bh, opts: L.BruteOpts | None = None):
         eb = L.errbuf()
         rc = self.lib.fv_gpu_batch_run_brute(self.h, bh, C.byref(opts) if opts else None, eb, len(eb))

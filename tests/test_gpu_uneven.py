@@ -1,0 +1,83 @@
+"""Uneven batches (fv_gpu_batch_stage_ex): per-query k in {1, 10, 100} and
+per-query selectivity log-uniform in [1e-4, 1] (price < 1000 s), with the
+predicates evaluated inside the scan (inline) or through bitmaps; every
+query must equal the oracle's exact top-k for its own k."""
+import numpy as np
+import pytest
+
+from paves_amd import _lib as L
+from paves_amd import engine as E
+from paves_amd import synth
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _uneven(n, nq, seed):
+    d = synth.gmm128(n, nq, seed=seed)
+    cols = synth.columns(d)
+    rows, vals = synth.postings(cols)
+    rng = np.random.default_rng(seed)
+    kq = rng.choice(np.array([1, 10, 100], np.uint32), nq)
+    sel = 10.0 ** rng.uniform(-4, 0, nq)
+    preds = [E.cmp(0, E.CMP_LT, float(1000.0 * s)) for s in sel]
+    post = E.Postings(rows, vals)
+    b = E.concat([E.serialize(p, n, post) for p in preds], n)
+    return d, cols, rows, kq, b
+
+
+def _check(out, cnt, trunc, d, cols, b, kq, sel=None):
+    idx = np.arange(len(kq)) if sel is None else sel
+    for k in np.unique(kq[idx]):
+        qs = idx[kq[idx] == k]
+        oh, oc, ot, _ = O.brute_batch(d["base"], cols, d["queries"][qs], b.qd[qs], b.nodes,
+                                      b.children, b.invals, int(k))
+        for j, q in enumerate(qs):
+            c = int(cnt[q])
+            assert c == int(oc[j]) and int(trunc[q]) == int(ot[j]), (q, k, c, int(oc[j]))
+            assert np.array_equal(out[q, :c]["node"], oh[j, :c]["node"]), (q, k)
+            assert np.array_equal(out[q, :c]["dist"], oh[j, :c]["dist"]), (q, k)
+
+
+@pytest.mark.parametrize("inline", [1, 0])
+def test_uneven_k_and_selectivity(inline):
+    n, nq = 200_000, 300
+    d, cols, rows, kq, b = _uneven(n, nq, 41)
+    g = E.GpuIndex(d["base"], cols, rows, precision=L.FV_GPU_PREC_BF16)
+    try:
+        bh = g.stage_ex(d["queries"], b, kq)
+        try:
+            for route in (1, 0):
+                g.run_brute(bh, L.BruteOpts(route=route, inline_pred=inline))
+                g.sync()
+                out, cnt, flag, rc = g.fetch(bh, nq, int(kq.max()))
+                assert rc == 0
+                _check(out, cnt, flag, d, cols, b, kq)
+        finally:
+            g.free_batch(bh)
+    finally:
+        g.close()
+
+
+def test_4096_queries_inline():
+    """4096 distinct predicates (beyond the bitmap path's 1024) in one
+    batch: the inline path serves it; a sample of queries is checked."""
+    n, nq = 100_000, 4096
+    d, cols, rows, kq, b = _uneven(n, nq, 43)
+    g = E.GpuIndex(d["base"], cols, rows, precision=L.FV_GPU_PREC_BF16)
+    try:
+        bh = g.stage_ex(d["queries"], b, kq)
+        try:
+            with pytest.raises(E.GpuError):
+                g.run_brute(bh, L.BruteOpts(route=1, inline_pred=0))
+            g.run_brute(bh, L.BruteOpts(route=1, inline_pred=1))
+            g.sync()
+            out, cnt, flag, rc = g.fetch(bh, nq, int(kq.max()))
+            assert rc == 0
+            sel = np.random.default_rng(1).choice(nq, 256, replace=False)
+            _check(out, cnt, flag, d, cols, b, kq, sel)
+        finally:
+            g.free_batch(bh)
+    finally:
+        g.close()
