@@ -55,11 +55,13 @@ class BruteOpts(C.Structure):
     """FvGpuBruteOpts; precision -1 = the index's precision."""
     _fields_ = [("precision", C.c_int32), ("route", C.c_int32),
                 ("overfetch", C.c_uint32), ("sample_stride", C.c_uint32),
-                ("inline_pred", C.c_int32), ("wg_per_cu", C.c_int32)]
+                ("inline_pred", C.c_int32), ("wg_per_cu", C.c_int32), ("qtw", C.c_int32),
+                ("graph", C.c_int32)]
 
     def __init__(self, precision=-1, route=0, overfetch=0, sample_stride=0, inline_pred=0,
-                 wg_per_cu=0):
-        super().__init__(precision, route, overfetch, sample_stride, inline_pred, wg_per_cu)
+                 wg_per_cu=0, qtw=0, graph=0):
+        super().__init__(precision, route, overfetch, sample_stride, inline_pred, wg_per_cu,
+                         qtw, graph)
 
 
 class LearnedOpts(C.Structure):

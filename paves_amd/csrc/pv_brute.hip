@@ -251,8 +251,49 @@ struct TileIn {
     float xn[16];      /* ||x||^2 of the lane's 16 rows */
     uint32_t w[2];     /* predicate bitmap word of the tile, per query block */
     uint32_t valid;    /* rows < count */
-    double cv[16];     /* inline predicates: the column value of the lane's rows */
+    double rv;         /* inline predicates: column value of row (lane & 31) */
 };
+
+/* inline predicates: the column values of the lane's 16 accumulator rows,
+ * gathered from the lanes that loaded them (one value per row per lane) */
+__device__ __forceinline__ void expand_rows(double rv, double (&cv)[16]) {
+    const int l = threadIdx.x & 63;
+#pragma unroll
+    for (int i = 0; i < 16; i++) cv[i] = __shfl(rv, acc_row(i, l));
+}
+
+/* inline predicates staged per WG: comparison mask (-1: no predicate) and
+ * constant of query qi of query tile k at [k * 64 + qi] */
+__device__ __forceinline__ void stage_inline(const ScanArgs &A, int k, int qbase, int *s_im,
+                                             double *s_iv) {
+    for (int i = threadIdx.x; i < kQT; i += blockDim.x) {
+        const int q = qbase + i;
+        int m = -1;
+        double v = 0.0;
+        if (q < A.nq) {
+            const InlPred ip = A.inl[q];
+            m = ip.col < 0 ? -1 : ip.mask;
+            v = ip.value;
+        }
+        s_im[k * kQT + i] = m;
+        s_iv[k * kQT + i] = v;
+    }
+}
+
+/* the predicate word (bit = accumulator row) of the lane's query of
+ * block qb in query tile k, from the expanded column values */
+__device__ __forceinline__ uint32_t inline_word(const double (&cv)[16], const int *s_im,
+                                                const double *s_iv, int k, int qb) {
+    const int l = threadIdx.x & 63;
+    const int qi = k * kQT + (l & 31) + 32 * qb;
+    const int m = s_im[qi];
+    if (m < 0) return 0xFFFFFFFFu;
+    const double c = s_iv[qi];
+    uint32_t w = 0;
+#pragma unroll
+    for (int i = 0; i < 16; i++) w |= cmp_pass(cv[i], c, m) << acc_row(i, l);
+    return w;
+}
 
 template <bool INL = false>
 __device__ __forceinline__ void load_tile_in(const ScanArgs &A, uint32_t pt,
@@ -267,10 +308,7 @@ __device__ __forceinline__ void load_tile_in(const ScanArgs &A, uint32_t pt,
         in.xn[g4 * 4 + 3] = v.w;
     }
     if (INL) {
-        const uint64_t last = A.count - 1;
-#pragma unroll
-        for (int i = 0; i < 16; i++)
-            in.cv[i] = A.icol[min((uint64_t)pt * 32 + acc_row(i, l), last)];
+        in.rv = A.icol[min((uint64_t)pt * 32 + (l & 31), A.count - 1)];
     } else {
         in.w[0] = wr[0][pt];
         in.w[1] = wr[1][pt];
@@ -294,7 +332,8 @@ __device__ __forceinline__ void tile_epilogue(const ScanArgs &A, int qbase, uint
                                               const TileIn &in, const float (&qn)[2],
                                               const float (&tq)[2], bool active,
                                               CandLds &C, const int (&imask)[2] = {-1, -1},
-                                              const double (&ival)[2] = {0.0, 0.0}) {
+                                              const double (&ival)[2] = {0.0, 0.0},
+                                              const double *cv = nullptr) {
     const int l = threadIdx.x & 63;
 #pragma unroll
     for (int qb = 0; qb < 2; qb++) {
@@ -310,7 +349,7 @@ __device__ __forceinline__ void tile_epilogue(const ScanArgs &A, int qbase, uint
                 word = 0;
 #pragma unroll
                 for (int i = 0; i < 16; i++)
-                    word |= cmp_pass(in.cv[i], ival[qb], imask[qb]) << acc_row(i, l);
+                    word |= cmp_pass(cv[i], ival[qb], imask[qb]) << acc_row(i, l);
             }
             word &= in.valid;
         } else {
@@ -411,25 +450,15 @@ __global__ __launch_bounds__(NW * 64) void k_scan_bf16(ScanArgs A) {
         const int q = qbase + (lane & 31) + 32 * qb;
         qsh[e] = *(const u32x4 *)(A.qbf + (uint64_t)q * A.dim + (lane >> 5) * half + 8 * s);
     }
+    __shared__ int s_im[INL ? kQT : 1];
+    __shared__ double s_iv[INL ? kQT : 1];
+    if (INL) stage_inline(A, 0, qbase, s_im, s_iv);
     if (!PREPASS) cand_init(C);
     __syncthreads();
 
     float tq[2], qn[2];
     const uint32_t *wr[2];
     load_query_state(A, qbase, tq, qn, wr);
-    int imask[2] = {-1, -1};
-    double ival[2] = {0.0, 0.0};
-    if (INL) {
-#pragma unroll
-        for (int qb = 0; qb < 2; qb++) {
-            const int q = qbase + (l & 31) + 32 * qb;
-            if (q < A.nq) {
-                const InlPred ip = A.inl[q];
-                imask[qb] = ip.col < 0 ? -1 : ip.mask;
-                ival[qb] = ip.value;
-            }
-        }
-    }
 
     const uint32_t nwg = gridDim.x;
     const uint32_t t0 = (uint32_t)(((uint64_t)A.ntiles * blockIdx.x) / nwg);
@@ -472,8 +501,16 @@ __global__ __launch_bounds__(NW * 64) void k_scan_bf16(ScanArgs A) {
                 acc[1][i] = acc1[i];
             }
             const uint32_t t = tile_of_unit(u);
-            tile_epilogue<PREPASS, true, INL>(A, qbase, t, t * A.stride, acc, in, qn, tq, active,
-                                              C, imask, ival);
+            if (INL) {
+                double cv[16];
+                expand_rows(in.rv, cv);
+                TileIn v = in;
+                v.w[0] = inline_word(cv, s_im, s_iv, 0, 0);
+                v.w[1] = inline_word(cv, s_im, s_iv, 0, 1);
+                tile_epilogue<PREPASS, true>(A, qbase, t, t * A.stride, acc, v, qn, tq, active, C);
+            } else {
+                tile_epilogue<PREPASS, true>(A, qbase, t, t * A.stride, acc, in, qn, tq, active, C);
+            }
         }
     };
 
@@ -490,6 +527,133 @@ __global__ __launch_bounds__(NW * 64) void k_scan_bf16(ScanArgs A) {
         }
     }
     if (!PREPASS) flush_candidates(A, qbase, C);
+}
+
+/*
+ * Multi-query-tile variant of the BF16 scan for dim 128 (one 8 KB unit per
+ * tile): a WG serves QTW query tiles, so every data tile loaded from HBM
+ * feeds QTW x 16 MFMAs and QTW epilogues instead of being re-read by QTW
+ * WGs. Query operands of all QTW tiles and one candidate area per query
+ * tile sit in LDS. grid = (row chunks, ceil(query tiles / QTW)).
+ */
+template <bool PREPASS, int QTW, int NW, bool INL>
+__global__ __launch_bounds__(NW * 64) void k_scan_bf16_mq(ScanArgs A) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    constexpr int S = 8;                     /* dim 128: 8 k-steps, 1 unit per tile */
+    u32x4 *qsh = (u32x4 *)smem;              /* [QTW][2][S][64] x 16 B */
+    CandLds *Cs = (CandLds *)(smem + (size_t)QTW * 2 * S * 64 * 16);
+
+    const int tid = threadIdx.x, l = tid & 63, wave = tid >> 6;
+    const int nqt = (A.nq + kQT - 1) / kQT;
+    int qb0[QTW];
+    bool qon[QTW];
+    bool any = false;
+#pragma unroll
+    for (int k = 0; k < QTW; k++) {
+        const int qt = blockIdx.y * QTW + k;
+        qb0[k] = qt * kQT;
+        qon[k] = qt < nqt && !tile_skipped(A, qb0[k]);
+        any |= qon[k];
+    }
+    if (!any) return;
+    constexpr int half = 64;
+    for (int e = tid; e < QTW * 2 * S * 64; e += blockDim.x) {
+        const int k = e / (2 * S * 64), r = e % (2 * S * 64);
+        const int lane = r & 63, st = (r >> 6) % S, qb = (r >> 6) / S;
+        if (!qon[k]) continue;
+        const int q = qb0[k] + (lane & 31) + 32 * qb;
+        qsh[e] = *(const u32x4 *)(A.qbf + (uint64_t)q * 128 + (lane >> 5) * half + 8 * st);
+    }
+    __shared__ int s_im[INL ? QTW * kQT : 1];
+    __shared__ double s_iv[INL ? QTW * kQT : 1];
+    if (INL)
+#pragma unroll
+        for (int k = 0; k < QTW; k++) stage_inline(A, k, qb0[k], s_im, s_iv);
+    if (!PREPASS)
+#pragma unroll
+        for (int k = 0; k < QTW; k++) cand_init(Cs[k]);
+    __syncthreads();
+
+    float tq[QTW][2], qn[QTW][2];
+    const uint32_t *wr[QTW][2];
+#pragma unroll
+    for (int k = 0; k < QTW; k++) load_query_state(A, qb0[k], tq[k], qn[k], wr[k]);
+
+    const uint32_t nwg = gridDim.x;
+    const uint32_t t0 = (uint32_t)(((uint64_t)A.ntiles * blockIdx.x) / nwg);
+    const uint32_t t1 = (uint32_t)(((uint64_t)A.ntiles * (blockIdx.x + 1)) / nwg);
+    const uint32_t first = t0 + wave;
+    const uint32_t nu = first < t1 ? (t1 - first + NW - 1) / NW : 0;
+    const u32x4 *vb = (const u32x4 *)A.vbf;
+
+    auto tile_of = [&](uint32_t u) { return min(first + NW * u, t1 - 1); };
+    auto load_unit = [&](uint32_t u, u32x4 (&a)[8], TileIn (&in)[QTW]) {
+        const uint32_t pt = tile_of(min(u, nu - 1)) * A.stride;
+        const u32x4 *p = vb + (uint64_t)pt * 512 + l;
+#pragma unroll
+        for (int j = 0; j < 8; j++) a[j] = __builtin_nontemporal_load(p + 64 * j);
+        load_tile_in<INL>(A, pt, wr[0], in[0]);
+#pragma unroll
+        for (int k = 1; k < QTW; k++) {
+            in[k].w[0] = INL ? 0u : wr[k][0][pt];
+            in[k].w[1] = INL ? 0u : wr[k][1][pt];
+        }
+    };
+    auto do_unit = [&](uint32_t u, const u32x4 (&a)[8], const TileIn (&in)[QTW], bool active) {
+        const uint32_t t = tile_of(u);
+        uint32_t iw[QTW][2];
+        if (INL) {
+            double cv[16];
+            expand_rows(in[0].rv, cv);
+#pragma unroll
+            for (int k = 0; k < QTW; k++) {
+                iw[k][0] = inline_word(cv, s_im, s_iv, k, 0);
+                iw[k][1] = inline_word(cv, s_im, s_iv, k, 1);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < QTW; k++) {
+            if (!qon[k]) continue;
+            f32x16 acc0 = {}, acc1 = {};
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                const bf16x8 av = __builtin_bit_cast(bf16x8, a[j]);
+                const bf16x8 b0 = __builtin_bit_cast(bf16x8, qsh[((k * 2 + 0) * S + j) * 64 + l]);
+                const bf16x8 b1 = __builtin_bit_cast(bf16x8, qsh[((k * 2 + 1) * S + j) * 64 + l]);
+                acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, b0, acc0, 0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, b1, acc1, 0, 0, 0);
+            }
+            float acc[2][16];
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                acc[0][i] = acc0[i];
+                acc[1][i] = acc1[i];
+            }
+            /* the tile's norms, validity and inline column values are shared
+             * by the query tiles; the bitmap words are per query tile */
+            TileIn v = in[0];
+            v.w[0] = INL ? iw[k][0] : in[k].w[0];
+            v.w[1] = INL ? iw[k][1] : in[k].w[1];
+            tile_epilogue<PREPASS, true>(A, qb0[k], t, t * A.stride, acc, v, qn[k], tq[k], active,
+                                         Cs[k]);
+        }
+    };
+
+    if (nu > 0) {
+        u32x4 a0[8], a1[8];
+        TileIn in0[QTW], in1[QTW];
+        load_unit(0, a0, in0);
+        for (uint32_t u = 0; u < nu; u += 2) {
+            load_unit(u + 1, a1, in1);
+            do_unit(u, a0, in0, true);
+            load_unit(u + 2, a0, in0);
+            do_unit(u + 1, a1, in1, u + 1 < nu);
+        }
+    }
+    if (!PREPASS)
+#pragma unroll
+        for (int k = 0; k < QTW; k++)
+            if (qon[k]) flush_candidates(A, qb0[k], Cs[k]);
 }
 
 /* ---------------- 3/5. plain FP32 VALU scan ---------------- */
@@ -976,6 +1140,11 @@ void free_batch_buffers(FvGpuBatch *b) {
     b->d_y.release();
     b->d_route.release();
     b->d_efq.release();
+    if (b->gexec) hipGraphExecDestroy(b->gexec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    b->gexec = nullptr;
+    b->graph = nullptr;
+    b->warmed = false;
 }
 
 int stage_batch(FvGpuIndex *g, FvGpuBatch *b, int nq, const float *queries,
@@ -1157,6 +1326,27 @@ static void launch_scan_bf16(int nw, bool inl, uint32_t gx, int nqt, size_t shm,
     else hipLaunchKernelGGL((k_scan_bf16<PREPASS, 8, true>), grid, dim3(512), shm, s, A);
 }
 
+/* multi-query-tile BF16 scan launch (dim 128): QTW 2 -> 4 waves per WG,
+ * QTW 4 -> 8 waves per WG (one WG fills a CU's LDS) */
+template <bool PREPASS>
+static hipError_t launch_scan_mq(int qtw, bool inl, uint32_t gx, int gy, hipStream_t s,
+                                 const ScanArgs &A) {
+    const size_t shm = (size_t)qtw * 2 * 8 * 64 * 16 + (size_t)qtw * sizeof(CandLds);
+    const void *f;
+    if (qtw == 2) f = inl ? (const void *)k_scan_bf16_mq<PREPASS, 2, 4, true>
+                          : (const void *)k_scan_bf16_mq<PREPASS, 2, 4, false>;
+    else f = inl ? (const void *)k_scan_bf16_mq<PREPASS, 4, 8, true>
+                 : (const void *)k_scan_bf16_mq<PREPASS, 4, 8, false>;
+    hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+    if (e != hipSuccess) return e;
+    const dim3 grid(gx, gy);
+    if (qtw == 2 && inl) hipLaunchKernelGGL((k_scan_bf16_mq<PREPASS, 2, 4, true>), grid, dim3(256), shm, s, A);
+    else if (qtw == 2) hipLaunchKernelGGL((k_scan_bf16_mq<PREPASS, 2, 4, false>), grid, dim3(256), shm, s, A);
+    else if (inl) hipLaunchKernelGGL((k_scan_bf16_mq<PREPASS, 4, 8, true>), grid, dim3(512), shm, s, A);
+    else hipLaunchKernelGGL((k_scan_bf16_mq<PREPASS, 4, 8, false>), grid, dim3(512), shm, s, A);
+    return hipGetLastError();
+}
+
 /* Phase 1 of a run: predicate bitmaps (when `bitmap`) and the query
  * preparation for scan precision `prec`, one launch on stream s. */
 hipError_t launch_phase1(FvGpuIndex *g, FvGpuBatch *b, int prec, bool bitmap, hipStream_t s,
@@ -1285,10 +1475,16 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
                                       : (o && o->sample_stride ? o->sample_stride : 0);
     /* main-scan workgroups per query tile (also bounds the candidates each
      * WG collects per query: about stride x B / gx of them) */
-    const int nw = dim <= 256 ? 4 : 8;
     const bool bf = prec == FV_GPU_PREC_BF16;
+    /* query tiles per WG: several when many query tiles re-read the same
+     * data (dim 128, BF16) */
+    int qtw = (o && o->qtw > 0) ? o->qtw : (b->nqt >= 8 ? 4 : 1);
+    if (!bf || dim != 128 || b->nqt < 2) qtw = 1;
+    qtw = qtw >= 4 ? 4 : (qtw >= 2 ? 2 : 1);
+    const int gy = (b->nqt + qtw - 1) / qtw;
+    const int nw = qtw == 4 ? 8 : (dim <= 256 ? 4 : 8);
     const int wpc = (o && o->wg_per_cu > 0) ? o->wg_per_cu : (bf && nw == 8 ? 1 : 2);
-    uint32_t gx = (uint32_t)std::max(64, (wpc * g->ncu) / std::max(1, b->nqt));
+    uint32_t gx = (uint32_t)std::max(64, (wpc * g->ncu) / std::max(1, gy));
     gx = std::min<uint32_t>(gx, std::max<uint32_t>(1, ntiles / nw));
     if (stride == 0) {
         /* the B-th smallest minimum over groups of 16 sampled rows: about
@@ -1350,8 +1546,9 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
         P.ntiles = stiles;
         P.stride = stride;
         if (bf) {
-            const uint32_t gx = std::max<uint32_t>(1, (stiles + nw - 1) / nw);
-            launch_scan_bf16<true>(nw, inl, gx, b->nqt, shm_bf, s, P);
+            const uint32_t gxp = std::max<uint32_t>(1, (stiles + nw - 1) / nw);
+            if (qtw > 1) PV_TRY(launch_scan_mq<true>(qtw, inl, gxp, gy, s, P));
+            else launch_scan_bf16<true>(nw, inl, gxp, b->nqt, shm_bf, s, P);
         } else {
             const uint32_t gx = std::max<uint32_t>(1, (stiles + 3) / 4);
             hipLaunchKernelGGL(k_scan_f32<true>, dim3(gx, b->nqt), dim3(256), 0, s, P);
@@ -1368,7 +1565,9 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
         ScanArgs M = A;
         M.ntiles = ntiles;
         M.stride = 1;
-        if (bf)
+        if (bf && qtw > 1)
+            PV_TRY(launch_scan_mq<false>(qtw, inl, gx, gy, s, M));
+        else if (bf)
             launch_scan_bf16<false>(nw, inl, gx, b->nqt, shm_bf, s, M);
         else
             hipLaunchKernelGGL(k_scan_f32<false>, dim3(gx, b->nqt), dim3(256), 0, s, M);
@@ -1619,11 +1818,47 @@ extern "C" FvGpuBatch *fv_gpu_batch_stage_ex(
     }
 }
 
+/* run through a hipGraph captured for this batch and these options: the
+ * first call runs eagerly (allocating every buffer), the second captures,
+ * later calls replay. Timing events are not recorded in graph mode. */
+static int run_brute_graph(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
+                           char *errbuf, size_t errlen) {
+    if (b->gexec && memcmp(&b->gopts, o, sizeof(*o)) != 0) {
+        hipGraphExecDestroy(b->gexec);
+        hipGraphDestroy(b->graph);
+        b->gexec = nullptr;
+        b->graph = nullptr;
+        b->warmed = false;
+    }
+    if (b->gexec == nullptr) {
+        const bool timing = g->timing;
+        if (!b->warmed || timing) {
+            b->warmed = true;
+            return run_brute(g, b, o, 0, errbuf, errlen);
+        }
+        PV_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        const int rc = run_brute(g, b, o, 0, errbuf, errlen);
+        hipGraph_t gr = nullptr;
+        const hipError_t e = hipStreamEndCapture(b->stream, &gr);
+        if (rc != 0) {
+            if (gr) hipGraphDestroy(gr);
+            return -1;
+        }
+        PV_TRY(e);
+        b->graph = gr;
+        PV_TRY(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+        b->gopts = *o;
+    }
+    PV_TRY(hipGraphLaunch(b->gexec, b->stream));
+    return 0;
+}
+
 extern "C" int fv_gpu_batch_run_brute(FvGpuIndex *g, FvGpuBatch *b,
                                       const FvGpuBruteOpts *opts, char *errbuf,
                                       size_t errlen) {
     if (g == nullptr || b == nullptr || b->owner != g) PV_FAIL("gpu: bad batch");
     try {
+        if (opts && opts->graph && !g->timing) return run_brute_graph(g, b, opts, errbuf, errlen);
         return run_brute(g, b, opts, 0, errbuf, errlen);
     } catch (...) {
         snprintf(errbuf, errlen, "gpu: internal error");

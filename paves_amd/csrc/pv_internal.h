@@ -156,6 +156,12 @@ struct FvGpuBatch {
     bool inl_ok = false;
     int inl_col = -1;
     const pv::InlPred *d_inl_p = nullptr;  /* [nqp] in the staging area */
+    /* a captured run (FvGpuBruteOpts.graph): replayed while the options
+     * it was captured with are unchanged */
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    FvGpuBruteOpts gopts = {};
+    bool warmed = false;    /* one eager run done: every buffer allocated */
     bool query_exact = true;
     std::vector<uint64_t> total;   /* host copy of qd[q].total */
     std::vector<int32_t> nspans;

@@ -40,8 +40,9 @@ def _check(out, cnt, trunc, d, cols, b, kq, sel=None):
             assert np.array_equal(out[q, :c]["dist"], oh[j, :c]["dist"]), (q, k)
 
 
-@pytest.mark.parametrize("inline", [1, 0])
-def test_uneven_k_and_selectivity(inline):
+@pytest.mark.parametrize("inline,qtw", [(1, 0), (0, 0), (1, 1), (0, 2), (1, 4)])
+def test_uneven_k_and_selectivity(inline, qtw):
+    """query tiles per WG: 0 = auto (1 for this 5-tile batch), 1, 2, 4"""
     n, nq = 200_000, 300
     d, cols, rows, kq, b = _uneven(n, nq, 41)
     g = E.GpuIndex(d["base"], cols, rows, precision=L.FV_GPU_PREC_BF16)
@@ -49,7 +50,7 @@ def test_uneven_k_and_selectivity(inline):
         bh = g.stage_ex(d["queries"], b, kq)
         try:
             for route in (1, 0):
-                g.run_brute(bh, L.BruteOpts(route=route, inline_pred=inline))
+                g.run_brute(bh, L.BruteOpts(route=route, inline_pred=inline, qtw=qtw))
                 g.sync()
                 out, cnt, flag, rc = g.fetch(bh, nq, int(kq.max()))
                 assert rc == 0
@@ -78,6 +79,42 @@ def test_4096_queries_inline():
             sel = np.random.default_rng(1).choice(nq, 256, replace=False)
             _check(out, cnt, flag, d, cols, b, kq, sel)
         finally:
+            g.free_batch(bh)
+    finally:
+        g.close()
+
+
+@pytest.mark.parametrize("inline,qtw", [(1, 4), (0, 1)])
+def test_graph_replay_equals_eager(inline, qtw):
+    """FvGpuBruteOpts.graph: the first run is eager, the second captures a
+    hipGraph, later runs replay it. Every replay, with other eager work and
+    synchronisations between replays, equals the eager result."""
+    n, nq = 120_000, 600
+    d, cols, rows, kq, b = _uneven(n, nq, 47)
+    g = E.GpuIndex(d["base"], cols, rows, precision=L.FV_GPU_PREC_BF16)
+    try:
+        bh = g.stage_ex(d["queries"], b, kq)
+        other = g.stage_ex(d["queries"][:64], E.concat([E.serialize(None, n)] * 64, n),
+                           np.full(64, 10, np.uint32))
+        try:
+            g.run_brute(bh, L.BruteOpts(route=1, inline_pred=inline, qtw=qtw))
+            g.sync()
+            ref = g.fetch(bh, nq, int(kq.max()))
+            assert ref[3] == 0
+            _check(ref[0], ref[1], ref[2], d, cols, b, kq,
+                   np.random.default_rng(2).choice(nq, 96, replace=False))
+            opts = L.BruteOpts(route=1, inline_pred=inline, qtw=qtw, graph=1)
+            for it in range(5):
+                g.run_brute(bh, opts)
+                g.sync()
+                out, cnt, flag, rc = g.fetch(bh, nq, int(kq.max()))
+                assert rc == 0, it
+                assert np.array_equal(cnt, ref[1]) and np.array_equal(flag, ref[2]), it
+                assert np.array_equal(out, ref[0]), it
+                g.run_brute(other, L.BruteOpts(route=1))
+                g.sync()
+        finally:
+            g.free_batch(other)
             g.free_batch(bh)
     finally:
         g.close()
