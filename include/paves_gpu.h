@@ -167,8 +167,10 @@ int fv_gpu_mem_probe(uint64_t *free_bytes, uint64_t *total_bytes);
 typedef struct FvGpuBruteOpts {
     int32_t precision;   /* FV_GPU_PREC_* to scan with; -1 = index default */
     int32_t route;       /* 0 auto, 1 force full scan, 2 force span gather */
-    uint32_t overfetch;  /* re-rank candidates per query (0 = auto) */
-    uint32_t sample_stride; /* threshold pre-pass stride (0 = auto) */
+    uint32_t overfetch;  /* reserved, 0 (the re-rank set is certified: every
+                            candidate within 2 eps of the B-th key) */
+    uint32_t sample_stride; /* threshold pre-pass stride = over-fetch factor:
+                               about stride x B candidates per query (0 = auto) */
     int32_t inline_pred; /* 1: evaluate one-comparison predicates inside the
                             BF16 scan instead of building bitmaps (batches
                             whose predicates all compare one column) */
@@ -177,6 +179,11 @@ typedef struct FvGpuBruteOpts {
                             0 = auto; > 1 needs dim 128 and the BF16 scan) */
     int32_t graph;       /* 1: replay the run as a hipGraph captured for this
                             batch and these options (captured on first use) */
+    int32_t order;       /* BF16 scan tile order: 0 = auto (chunked), 1 = each
+                            workgroup streams one contiguous chunk, 2 = all
+                            waves sweep the rows together */
+    int32_t waves;       /* waves per BF16 scan workgroup with one query tile
+                            (4 or 8, 4 only up to 256 dims; 0 = auto: 8) */
 } FvGpuBruteOpts;
 
 /* Per-call statistics of the last batch run on this index. Kernel times are
@@ -253,6 +260,24 @@ FvGpuBatch *fv_gpu_batch_stage_ex(FvGpuIndex *g, int nq, const float *queries,
 int fv_gpu_batch_run_brute(FvGpuIndex *g, FvGpuBatch *b,
                            const FvGpuBruteOpts *opts, char *errbuf,
                            size_t errlen);
+
+/* Autotuner (no reference counterpart): times the candidate launch shapes
+ * of the full-scan route on this staged batch (query tiles per workgroup,
+ * workgroups per CU, pre-pass sampling stride = over-fetch factor) and
+ * writes the fastest whose runs certify every query to *best (precision,
+ * route and inline_pred taken from base). *best_us = its time per run,
+ * *ntried = shapes timed. Leaves the batch's outputs from the last trial. */
+int fv_gpu_batch_tune(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *base,
+                      int reps, FvGpuBruteOpts *best, double *best_us,
+                      int *ntried, char *errbuf, size_t errlen);
+
+/* Main-scan kernel timing (no reference counterpart): runs the batch once,
+ * launches the same full-scan kernel reps times back to back between two
+ * events on the batch's stream (*us = time per launch), then runs the
+ * batch once more so its outputs are those of an ordinary run. Fails when
+ * the batch takes the span-gather route. */
+int fv_gpu_batch_time_scan(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *opts,
+                           int reps, double *us, char *errbuf, size_t errlen);
 /* Enqueue the graph pipeline for a staged batch (asynchronous). */
 int fv_gpu_batch_run_graph(FvGpuIndex *g, FvGpuBatch *b, uint32_t ef,
                            uint64_t max_visited, char *errbuf, size_t errlen);
@@ -269,6 +294,13 @@ void fv_gpu_batch_free(FvGpuBatch *b);
  * The measured copy peak of roofline figures, and the known-size kernel
  * that scales rocprofv3's FETCH_SIZE / WRITE_SIZE readings. */
 int fv_gpu_copy_bench(int device, uint64_t bytes, int reps, double *gbs,
+                      char *errbuf, size_t errlen);
+
+/* Streaming read bandwidth: reads `bytes` (multiple of 16) `reps` times
+ * with k_read_calib (16-byte loads, four in flight per lane,
+ * nothing written) and reports GB/s read: the read-only peak a scan can
+ * approach. Use a buffer well above the 256 MB last-level cache. */
+int fv_gpu_read_bench(int device, uint64_t bytes, int reps, double *gbs,
                       char *errbuf, size_t errlen);
 
 /*

@@ -43,12 +43,13 @@ REF_SYMBOLS = ["fv_gpu_available", "fv_gpu_index_upload", "fv_gpu_index_free",
 EXT_SYMBOLS = ["fv_gpu_set_timing", "fv_gpu_last_stats", "fv_gpu_brute_batch_ex",
                "fv_gpu_graph_batch_ex", "fv_gpu_batch_stage", "fv_gpu_batch_run_brute",
                "fv_gpu_batch_run_graph", "fv_gpu_sync", "fv_gpu_batch_fetch",
-               "fv_gpu_batch_free", "fv_gpu_copy_bench", "fv_hnsw_build", "fv_hnsw_info",
+               "fv_gpu_batch_free", "fv_gpu_copy_bench", "fv_gpu_read_bench", "fv_hnsw_build", "fv_hnsw_info",
                "fv_hnsw_export", "fv_hnsw_free", "fv_hnsw_entries", "fv_l2sq",
                "fv_gpu_model_nparams", "fv_gpu_model_create", "fv_gpu_model_free",
                "fv_gpu_model_set_params", "fv_gpu_model_get_params", "fv_gpu_model_grad",
                "fv_gpu_model_train", "fv_gpu_model_predict", "fv_gpu_batch_features",
-               "fv_gpu_batch_run_learned", "fv_gpu_batch_routes", "fv_gpu_batch_stage_ex"]
+               "fv_gpu_batch_run_learned", "fv_gpu_batch_routes", "fv_gpu_batch_stage_ex",
+               "fv_gpu_batch_tune", "fv_gpu_batch_time_scan"]
 
 
 class BruteOpts(C.Structure):
@@ -56,12 +57,12 @@ class BruteOpts(C.Structure):
     _fields_ = [("precision", C.c_int32), ("route", C.c_int32),
                 ("overfetch", C.c_uint32), ("sample_stride", C.c_uint32),
                 ("inline_pred", C.c_int32), ("wg_per_cu", C.c_int32), ("qtw", C.c_int32),
-                ("graph", C.c_int32)]
+                ("graph", C.c_int32), ("order", C.c_int32), ("waves", C.c_int32)]
 
     def __init__(self, precision=-1, route=0, overfetch=0, sample_stride=0, inline_pred=0,
-                 wg_per_cu=0, qtw=0, graph=0):
+                 wg_per_cu=0, qtw=0, graph=0, order=0, waves=0):
         super().__init__(precision, route, overfetch, sample_stride, inline_pred, wg_per_cu,
-                         qtw, graph)
+                         qtw, graph, order, waves)
 
 
 class LearnedOpts(C.Structure):
@@ -138,11 +139,18 @@ def load() -> C.CDLL:
                                           _P, C.c_int, _P, C.c_int, _P, C.c_int, _P, C.c_uint32,
                                           C.c_char_p, sz]
     lib.fv_gpu_batch_run_brute.argtypes = [_P, _P, C.POINTER(BruteOpts), C.c_char_p, sz]
+    lib.fv_gpu_batch_time_scan.argtypes = [_P, _P, C.POINTER(BruteOpts), C.c_int,
+                                           C.POINTER(C.c_double), C.c_char_p, sz]
+    lib.fv_gpu_batch_tune.argtypes = [_P, _P, C.POINTER(BruteOpts), C.c_int,
+                                      C.POINTER(BruteOpts), C.POINTER(C.c_double),
+                                      C.POINTER(C.c_int), C.c_char_p, sz]
     lib.fv_gpu_batch_run_graph.argtypes = [_P, _P, C.c_uint32, C.c_uint64, C.c_char_p, sz]
     lib.fv_gpu_sync.argtypes = [_P, C.c_char_p, sz]
     lib.fv_gpu_batch_fetch.argtypes = [_P, _P, _P, _P, _P, C.c_char_p, sz]
     lib.fv_gpu_batch_free.argtypes = [_P]
     lib.fv_gpu_copy_bench.argtypes = [C.c_int, C.c_uint64, C.c_int, C.POINTER(C.c_double),
+                                      C.c_char_p, sz]
+    lib.fv_gpu_read_bench.argtypes = [C.c_int, C.c_uint64, C.c_int, C.POINTER(C.c_double),
                                       C.c_char_p, sz]
     lib.fv_hnsw_build.restype = _P
     lib.fv_hnsw_build.argtypes = [_P, C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_int,

@@ -38,6 +38,8 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
+#include <vector>
 
 #include "pv_device.h"
 
@@ -178,7 +180,82 @@ struct ScanArgs {
     const uint8_t *skip;      /* optional [nq]: 1 = query routed elsewhere */
     const InlPred *inl;       /* inline predicates [nqp] (INL kernels) */
     const double *icol;       /* their column [count] */
+    float xn2max;             /* max ||x||^2 over the rows (candidate-test margin) */
+    uint32_t sweep;           /* tile order: 0 = each WG streams its own contiguous
+                                 chunk; 1 = all waves sweep the rows together
+                                 (wave g takes tiles g, g + G, g + 2G, ...) */
 };
+
+/* the main-scan launch of a batch's last full-scan run, kept so the scan
+ * kernel alone can be re-timed (fv_gpu_batch_time_scan) */
+struct ScanLaunch {
+    ScanArgs M;
+    bool bf, inl;
+    int qtw, nw, nqt, gy;
+    uint32_t gx;
+    size_t shm;
+};
+
+/*
+ * The stream of units (tile, unit within the tile) one wave of a BF16 scan
+ * WG loads. Chunked order: the WG owns tiles [t0, t1); each wave starts
+ * with tiles t0 + wave and t0 + NW + wave and then takes the WG's next
+ * untaken tile from an LDS counter, one tile ahead of need (the counter's
+ * answer is read a whole tile after the request). Waves of one WG that see
+ * slower memory take fewer tiles instead of holding the WG's final flush
+ * back (static strides left waves of a WG finishing up to 5 us apart).
+ * Sweep order: wave g of the grid takes tiles g, g + G, ... (static).
+ */
+struct UnitCursor {
+    uint32_t tile, c;   /* next unit to load */
+    uint32_t nxt;       /* the wave's following tile */
+    uint32_t pend;      /* lane 0: counter value requested for the tile after */
+    uint32_t end;       /* first tile past the wave's range */
+    uint32_t step;      /* sweep order: tile stride; 0 = counter */
+};
+
+template <int NW>
+__device__ __forceinline__ UnitCursor cursor_init(const ScanArgs &A, int wave,
+                                                  uint32_t *s_next) {
+    UnitCursor u;
+    const uint32_t nwg = gridDim.x;
+    u.c = 0;
+    u.pend = 0;
+    if (A.sweep) {
+        u.tile = blockIdx.x * (uint32_t)NW + (uint32_t)wave;
+        u.step = nwg * (uint32_t)NW;
+        u.nxt = u.tile + u.step;
+        u.end = A.ntiles;
+    } else {
+        const uint32_t t0 = (uint32_t)(((uint64_t)A.ntiles * blockIdx.x) / nwg);
+        u.end = (uint32_t)(((uint64_t)A.ntiles * (blockIdx.x + 1)) / nwg);
+        u.tile = t0 + (uint32_t)wave;
+        u.nxt = t0 + (uint32_t)NW + (uint32_t)wave;
+        u.step = 0;
+        if (threadIdx.x == 0) *s_next = t0 + 2u * NW;
+    }
+    return u;
+}
+
+/* first counter request of a wave (after the barrier that publishes the
+ * counter's start) */
+__device__ __forceinline__ void cursor_start(UnitCursor &u, uint32_t *s_next) {
+    if (u.step == 0 && (threadIdx.x & 63) == 0) u.pend = atomicAdd(s_next, 1u);
+}
+
+/* move to the next unit; at a tile boundary the following tile becomes
+ * current and a new one is requested */
+__device__ __forceinline__ void cursor_next(UnitCursor &u, uint32_t nch, uint32_t *s_next) {
+    if (++u.c < nch) return;
+    u.c = 0;
+    u.tile = u.nxt;
+    if (u.step) {
+        u.nxt = u.tile + u.step;
+    } else {
+        u.nxt = __builtin_amdgcn_readfirstlane(u.pend);
+        if ((threadIdx.x & 63) == 0) u.pend = atomicAdd(s_next, 1u);
+    }
+}
 
 /* true when every query of the WG's 64-query tile is skipped (uniform) */
 __device__ __forceinline__ bool tile_skipped(const ScanArgs &A, int qbase) {
@@ -195,6 +272,22 @@ constexpr uint32_t kSpill = 512; /* per-WG LDS candidate overflow slots */
  * space with an atomic per query, and counters sharing a line would
  * serialise all of them in one L2 channel */
 constexpr uint32_t kPcntStride = 32;
+/* Each query's pool is split into kPoolShards equal segments, each with its
+ * own counter; scan WG x reserves in segment x % kPoolShards. All WGs flush
+ * at the end of the scan, and one counter per query took the reservations
+ * of every WG of the grid back to back (measured: 5 us of the flush at 256
+ * WGs); the select stage reads the segments through a prefix of their
+ * counts. */
+constexpr uint32_t kPoolShards = 8;
+__device__ __forceinline__ uint64_t pcnt_at(int q, uint32_t s) {
+    return ((uint64_t)q * kPoolShards + s) * kPcntStride;
+}
+/* the segment of a query's pool (cap entries) that shard s owns */
+__device__ __forceinline__ uint64_t shard_cap(uint64_t cap) { return cap / kPoolShards; }
+__device__ __forceinline__ void clear_pcnt(uint32_t *pcnt, int q) {
+#pragma unroll
+    for (uint32_t s = 0; s < kPoolShards; s++) pcnt[pcnt_at(q, s)] = 0;
+}
 
 /* LDS candidate state of one scan WG */
 struct CandLds {
@@ -218,19 +311,30 @@ __device__ __forceinline__ const uint32_t *bitmap_row(const ScanArgs &A, int32_t
  * past the area set the WG's overflow flag (the host then re-runs with the
  * tightest threshold).
  */
-__device__ __forceinline__ void insert_candidate(int qi, uint64_t key, CandLds &C) {
+/* Append a candidate of query qbase + qi: LDS slots first, then the WG's
+ * LDS spill area, then (rare: a WG holding more than its share of one
+ * query's candidates) straight into the WG's global pool segment. */
+__device__ __forceinline__ void insert_candidate(const ScanArgs &A, int qbase, int qi,
+                                                 uint64_t key, CandLds &C) {
     const uint32_t pos = atomicAdd(&C.cnt[qi], 1u);
     if (pos < (uint32_t)kLdsCand) {
         C.buf[qi * kLdsCand + pos] = key;
-    } else {
-        const uint32_t sp = atomicAdd(&C.nsp, 1u);
-        if (sp < kSpill) {
-            C.spk[sp] = key;
-            C.spq[sp] = (uint32_t)qi;
-        } else {
-            C.ovf = 1u;
-        }
+        return;
     }
+    const uint32_t sp = atomicAdd(&C.nsp, 1u);
+    if (sp < kSpill) {
+        C.spk[sp] = key;
+        C.spq[sp] = (uint32_t)qi;
+        return;
+    }
+    const int q = qbase + qi;
+    const uint32_t sh = blockIdx.x % kPoolShards;
+    const uint64_t scap = shard_cap(A.cap_pool);
+    const uint32_t gp = atomicAdd(&A.pcnt[pcnt_at(q, sh)], 1u);
+    if (gp < scap)
+        A.pool[(uint64_t)q * A.cap_pool + sh * scap + gp] = key;
+    else
+        C.ovf = 1u;
 }
 
 __device__ __forceinline__ void cand_init(CandLds &C) {
@@ -246,13 +350,25 @@ __device__ __forceinline__ int acc_row(int i, int l) {
     return (i >> 2) * 8 + (l >> 5) * 4 + (i & 3);
 }
 
-/* per-tile epilogue inputs, loaded together with the tile's rows */
+/* per-tile epilogue inputs, loaded together with the tile's rows: one
+ * value per lane (row lane & 31), expanded to the lane's 16 accumulator
+ * rows by shuffles when the epilogue runs, so a prefetched tile costs a
+ * few registers beyond its operands */
 struct TileIn {
-    float xn[16];      /* ||x||^2 of the lane's 16 rows */
+    float xr;          /* ||x||^2 of row (lane & 31) */
     uint32_t w[2];     /* predicate bitmap word of the tile, per query block */
     uint32_t valid;    /* rows < count */
     double rv;         /* inline predicates: column value of row (lane & 31) */
+    uint32_t t, c;     /* logical tile of the unit and unit within the tile */
+    bool live;         /* the unit is part of this wave's work */
 };
+
+/* ||x||^2 of the lane's 16 accumulator rows */
+__device__ __forceinline__ void expand_norms(float xr, float (&xn)[16]) {
+    const int l = threadIdx.x & 63;
+#pragma unroll
+    for (int i = 0; i < 16; i++) xn[i] = __shfl(xr, acc_row(i, l));
+}
 
 /* inline predicates: the column values of the lane's 16 accumulator rows,
  * gathered from the lanes that loaded them (one value per row per lane) */
@@ -299,14 +415,7 @@ template <bool INL = false>
 __device__ __forceinline__ void load_tile_in(const ScanArgs &A, uint32_t pt,
                                              const uint32_t *const (&wr)[2], TileIn &in) {
     const int l = threadIdx.x & 63;
-#pragma unroll
-    for (int g4 = 0; g4 < 4; g4++) {
-        const float4 v = *(const float4 *)(A.xnorm + (uint64_t)pt * 32 + g4 * 8 + (l >> 5) * 4);
-        in.xn[g4 * 4 + 0] = v.x;
-        in.xn[g4 * 4 + 1] = v.y;
-        in.xn[g4 * 4 + 2] = v.z;
-        in.xn[g4 * 4 + 3] = v.w;
-    }
+    in.xr = A.xnorm[(uint64_t)pt * 32 + (l & 31)];
     if (INL) {
         in.rv = A.icol[min((uint64_t)pt * 32 + (l & 31), A.count - 1)];
     } else {
@@ -326,52 +435,64 @@ __device__ __forceinline__ void load_tile_in(const ScanArgs &A, uint32_t pt,
  * passing distance of the lane's 16 rows per query (one group); MAIN
  * appends passing rows with dist <= tau to the WG's candidates.
  */
-template <bool PREPASS, bool BF, bool INL = false>
+template <bool PREPASS, bool BF>
 __device__ __forceinline__ void tile_epilogue(const ScanArgs &A, int qbase, uint32_t t,
                                               uint32_t pt, const float (&acc)[2][16],
-                                              const TileIn &in, const float (&qn)[2],
-                                              const float (&tq)[2], bool active,
-                                              CandLds &C, const int (&imask)[2] = {-1, -1},
-                                              const double (&ival)[2] = {0.0, 0.0},
-                                              const double *cv = nullptr) {
+                                              const TileIn &in, const float (&xn)[16],
+                                              const float (&qn)[2], const float (&tq)[2],
+                                              bool active, CandLds &C) {
     const int l = threadIdx.x & 63;
 #pragma unroll
     for (int qb = 0; qb < 2; qb++) {
         const int qi = (l & 31) + 32 * qb;
         const int q = qbase + qi;
         if (q >= A.nq || !active) continue;
-        uint32_t word;
-        if (INL) {
-            /* the query's one comparison, on the lane's 16 rows */
-            if (imask[qb] < 0) {
-                word = 0xFFFFFFFFu;
-            } else {
-                word = 0;
-#pragma unroll
-                for (int i = 0; i < 16; i++)
-                    word |= cmp_pass(cv[i], ival[qb], imask[qb]) << acc_row(i, l);
-            }
-            word &= in.valid;
-        } else {
-            word = in.w[qb] & in.valid;
-        }
+        const uint32_t word = in.w[qb] & in.valid;
         if (PREPASS) {
             float m = __uint_as_float(0x7f800000u);
 #pragma unroll
             for (int i = 0; i < 16; i++) {
-                const float d = BF ? fmaxf(in.xn[i] + qn[qb] - 2.f * acc[qb][i], 0.f) : acc[qb][i];
+                const float d = BF ? fmaxf(xn[i] + qn[qb] - 2.f * acc[qb][i], 0.f) : acc[qb][i];
                 if ((word >> acc_row(i, l)) & 1u) m = fminf(m, d);
             }
             const uint32_t grp = t * 2 + (l >> 5);
             A.gmin[(uint64_t)q * A.ngroups + grp] = make_key(m, grp);
+        } else if (BF) {
+            /* d' = ||x||^2 + ||q||^2 - 2 x.q <= tau  <=>  x.q >= ||x||^2 / 2 + h
+             * with h = (||q||^2 - tau) / 2: one FMA and one compare per
+             * element. The two forms round differently by a few ulps of
+             * ||x||^2 + ||q||^2; h is lowered by 2^-19 (max ||x||^2 + ||q||^2),
+             * far above that, so every row with d' <= tau is admitted (a
+             * few more in the margin band are harmless: the select stage
+             * certifies). The per-lane OR of the 16 tests stays a lane mask,
+             * and the divergent branch on it skips the insertion code for
+             * the whole wave when no lane has a candidate among its rows. */
+            const float h = fmaf(0.5f, qn[qb] - tq[qb], -ldexpf(A.xn2max + qn[qb], -19));
+            bool c[16];
+            bool anyl = false;
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                c[i] = ((word >> acc_row(i, l)) & 1u) & (acc[qb][i] >= fmaf(0.5f, xn[i], h));
+                anyl |= c[i];
+            }
+            if (anyl) {
+#pragma unroll
+                for (int i = 0; i < 16; i++)
+                    if (c[i]) {
+                        const float d = fmaxf(xn[i] + qn[qb] - 2.f * acc[qb][i], 0.f);
+                        insert_candidate(A, qbase, qi, make_key(d, (uint32_t)((uint64_t)pt * 32 +
+                                                                    acc_row(i, l))), C);
+                    }
+            }
         } else {
             const float tau = tq[qb];
 #pragma unroll
             for (int i = 0; i < 16; i++) {
-                const float d = BF ? fmaxf(in.xn[i] + qn[qb] - 2.f * acc[qb][i], 0.f) : acc[qb][i];
+                const float d = acc[qb][i];
                 const int r = acc_row(i, l);
                 if (((word >> r) & 1u) && d <= tau)
-                    insert_candidate(qi, make_key(d, (uint32_t)((uint64_t)pt * 32 + r)), C);
+                    insert_candidate(A, qbase, qi,
+                                     make_key(d, (uint32_t)((uint64_t)pt * 32 + r)), C);
             }
         }
     }
@@ -381,24 +502,27 @@ __device__ __forceinline__ void tile_epilogue(const ScanArgs &A, int qbase, uint
 __device__ __forceinline__ void flush_candidates(const ScanArgs &A, int qbase,
                                                  CandLds &C) {
     __syncthreads();
-    /* one reservation per query, all in flight together */
+    /* one reservation per query in this WG's pool segment, all in flight
+     * together */
+    const uint32_t sh = blockIdx.x % kPoolShards;
+    const uint64_t scap = shard_cap(A.cap_pool);
     if (threadIdx.x < kQT) {
         const int qi = threadIdx.x, q = qbase + qi;
         const uint32_t n = min(C.cnt[qi], (uint32_t)kLdsCand);
-        C.base[qi] = (q < A.nq && n > 0) ? atomicAdd(&A.pcnt[(uint64_t)q * kPcntStride], n) : 0u;
+        C.base[qi] = (q < A.nq && n > 0) ? atomicAdd(&A.pcnt[pcnt_at(q, sh)], n) : 0u;
     }
     __syncthreads();
     for (int idx = threadIdx.x; idx < kQT * kLdsCand; idx += blockDim.x) {
         const int qi = idx / kLdsCand, j = idx % kLdsCand, q = qbase + qi;
         const uint32_t n = min(C.cnt[qi], (uint32_t)kLdsCand);
-        if (q < A.nq && (uint32_t)j < n && C.base[qi] + j < A.cap_pool)
-            A.pool[(uint64_t)q * A.cap_pool + C.base[qi] + j] = C.buf[idx];
+        if (q < A.nq && (uint32_t)j < n && C.base[qi] + j < scap)
+            A.pool[(uint64_t)q * A.cap_pool + sh * scap + C.base[qi] + j] = C.buf[idx];
     }
     const uint32_t ns = min(C.nsp, kSpill);
     for (uint32_t i = threadIdx.x; i < ns; i += blockDim.x) {
         const uint32_t q = (uint32_t)qbase + C.spq[i];
-        const uint32_t gp = atomicAdd(&A.pcnt[(uint64_t)q * kPcntStride], 1u);
-        if (gp < A.cap_pool) A.pool[(uint64_t)q * A.cap_pool + gp] = C.spk[i];
+        const uint32_t gp = atomicAdd(&A.pcnt[pcnt_at(q, sh)], 1u);
+        if (gp < scap) A.pool[(uint64_t)q * A.cap_pool + sh * scap + gp] = C.spk[i];
     }
     if (C.ovf && threadIdx.x < kQT && qbase + (int)threadIdx.x < A.nq)
         atomicOr(&A.status[qbase + threadIdx.x], kStOverflow);
@@ -423,18 +547,23 @@ __device__ __forceinline__ void load_query_state(const ScanArgs &A, int qbase,
 
 /*
  * grid.x: row chunks, grid.y: query tiles of 64; NW waves per WG. A wave
- * walks its tiles (t0 + wave + NW i) as a stream of "units" of 8 k-steps
+ * walks its tiles (UnitCursor: taken from its WG's contiguous chunk through
+ * an LDS counter, or a static sweep over all rows shared by every wave of
+ * the grid) as a stream of "units" of 8 k-steps
  * (8 KB of the MFMA-tiled copy; dim/128 units per tile). Units alternate
  * between two register buffers: the loads of unit u+1 — its rows, and the
  * tile's ||x||^2 and bitmap words — are issued before the 16
  * v_mfma_f32_32x32x16_bf16 of unit u, and after the last unit of a tile its
- * epilogue runs while the next tile streams in. Loads are unconditional
- * (indices clamped to the last unit) so the pipeline is one straight line
- * for the wait-count analysis. B operands (the 64 queries) sit in LDS.
+ * epilogue runs while the next tile streams in (a third buffer, two units
+ * in flight per wave, measured slower: the stream is not short of loads in
+ * flight at two waves per SIMD). Loads are unconditional (indices clamped
+ * to the last unit) so the pipeline is one straight line for the
+ * wait-count analysis. B operands (the 64 queries) sit in LDS.
  * Requires dim % 128 == 0.
  */
 template <bool PREPASS, int NW, bool INL = false>
-__global__ __launch_bounds__(NW * 64) void k_scan_bf16(ScanArgs A) {
+__global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(2)))
+void k_scan_bf16(ScanArgs A) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     const int S = A.dim / 16;            /* MFMA k-steps per tile */
     const uint32_t NCH = (uint32_t)(S / 8); /* units per tile */
@@ -444,6 +573,22 @@ __global__ __launch_bounds__(NW * 64) void k_scan_bf16(ScanArgs A) {
     const int tid = threadIdx.x, l = tid & 63, wave = tid >> 6;
     const int qbase = blockIdx.y * kQT;
     if (tile_skipped(A, qbase)) return;
+
+    __shared__ uint32_t s_next;
+    UnitCursor cur = cursor_init<NW>(A, wave, &s_next);
+    const u32x4 *vb = (const u32x4 *)A.vbf;
+    /* units past the wave's range load a valid tile and are not consumed */
+    auto phys = [&](uint32_t t) { return min(t, A.ntiles - 1) * A.stride; };
+    auto load_rows = [&](u32x4 (&a)[8]) {
+        const u32x4 *p = vb + (uint64_t)phys(cur.tile) * (4 * A.dim) + (uint64_t)cur.c * 512 + l;
+#pragma unroll
+        for (int j = 0; j < 8; j++) a[j] = p[64 * j];
+    };
+    /* the first unit's rows are requested before the query operands are
+     * staged, so their latencies overlap instead of adding up */
+    u32x4 a0[8], a1[8];
+    load_rows(a0);
+
     const int half = A.dim / 2;
     for (int e = tid; e < 2 * S * 64; e += blockDim.x) {
         const int lane = e & 63, s = (e >> 6) % S, qb = (e >> 6) / S;
@@ -454,32 +599,24 @@ __global__ __launch_bounds__(NW * 64) void k_scan_bf16(ScanArgs A) {
     __shared__ double s_iv[INL ? kQT : 1];
     if (INL) stage_inline(A, 0, qbase, s_im, s_iv);
     if (!PREPASS) cand_init(C);
-    __syncthreads();
 
     float tq[2], qn[2];
     const uint32_t *wr[2];
     load_query_state(A, qbase, tq, qn, wr);
-
-    const uint32_t nwg = gridDim.x;
-    const uint32_t t0 = (uint32_t)(((uint64_t)A.ntiles * blockIdx.x) / nwg);
-    const uint32_t t1 = (uint32_t)(((uint64_t)A.ntiles * (blockIdx.x + 1)) / nwg);
-    const uint32_t first = t0 + wave;
-    const uint32_t nt = first < t1 ? (t1 - first + NW - 1) / NW : 0;
-    const uint32_t nu = nt * NCH;
-    const u32x4 *vb = (const u32x4 *)A.vbf;
-
-    auto tile_of_unit = [&](uint32_t u) { return min(first + NW * (u / NCH), t1 - 1); };
-    auto load_unit = [&](uint32_t u, u32x4 (&a)[8], TileIn &in) {
-        const uint32_t uc = min(u, nu - 1);
-        const uint32_t pt = tile_of_unit(uc) * A.stride;
-        const u32x4 *p = vb + (uint64_t)pt * (4 * A.dim) + (uint64_t)(uc % NCH) * 512 + l;
-#pragma unroll
-        for (int j = 0; j < 8; j++) a[j] = __builtin_nontemporal_load(p + 64 * j);
-        load_tile_in<INL>(A, pt, wr, in);
+    /* the unit's epilogue inputs and identity; then step the cursor */
+    auto load_in = [&](TileIn &in) {
+        load_tile_in<INL>(A, phys(cur.tile), wr, in);
+        in.t = cur.tile;
+        in.c = cur.c;
+        in.live = cur.tile < cur.end;
+        cursor_next(cur, NCH, &s_next);
     };
-    auto do_unit = [&](uint32_t u, const u32x4 (&a)[8], const TileIn &in, f32x16 &acc0,
-                       f32x16 &acc1, bool active) {
-        const uint32_t c = u % NCH;
+    TileIn in0, in1;
+    __syncthreads();
+    cursor_start(cur, &s_next);
+    load_in(in0);
+    auto do_unit = [&](const u32x4 (&a)[8], const TileIn &in, f32x16 &acc0, f32x16 &acc1) {
+        const uint32_t c = in.c;
         if (c == 0) {
             acc0 = f32x16{};
             acc1 = f32x16{};
@@ -500,31 +637,33 @@ __global__ __launch_bounds__(NW * 64) void k_scan_bf16(ScanArgs A) {
                 acc[0][i] = acc0[i];
                 acc[1][i] = acc1[i];
             }
-            const uint32_t t = tile_of_unit(u);
+            const uint32_t t = in.t;
+            float xn[16];
+            expand_norms(in.xr, xn);
             if (INL) {
                 double cv[16];
                 expand_rows(in.rv, cv);
                 TileIn v = in;
                 v.w[0] = inline_word(cv, s_im, s_iv, 0, 0);
                 v.w[1] = inline_word(cv, s_im, s_iv, 0, 1);
-                tile_epilogue<PREPASS, true>(A, qbase, t, t * A.stride, acc, v, qn, tq, active, C);
+                tile_epilogue<PREPASS, true>(A, qbase, t, t * A.stride, acc, v, xn, qn, tq,
+                                             true, C);
             } else {
-                tile_epilogue<PREPASS, true>(A, qbase, t, t * A.stride, acc, in, qn, tq, active, C);
+                tile_epilogue<PREPASS, true>(A, qbase, t, t * A.stride, acc, in, xn, qn, tq,
+                                             true, C);
             }
         }
     };
 
-    if (nu > 0) {
-        u32x4 a0[8], a1[8];
-        TileIn in0, in1;
-        f32x16 acc0 = {}, acc1 = {};
-        load_unit(0, a0, in0);
-        for (uint32_t u = 0; u < nu; u += 2) {
-            load_unit(u + 1, a1, in1);
-            do_unit(u, a0, in0, acc0, acc1, true);
-            load_unit(u + 2, a0, in0);
-            do_unit(u + 1, a1, in1, acc0, acc1, u + 1 < nu);
-        }
+    f32x16 acc0 = {}, acc1 = {};
+    while (in0.live) {
+        load_rows(a1);
+        load_in(in1);
+        do_unit(a0, in0, acc0, acc1);
+        if (!in1.live) break;
+        load_rows(a0);
+        load_in(in0);
+        do_unit(a1, in1, acc0, acc1);
     }
     if (!PREPASS) flush_candidates(A, qbase, C);
 }
@@ -537,7 +676,8 @@ __global__ __launch_bounds__(NW * 64) void k_scan_bf16(ScanArgs A) {
  * tile sit in LDS. grid = (row chunks, ceil(query tiles / QTW)).
  */
 template <bool PREPASS, int QTW, int NW, bool INL>
-__global__ __launch_bounds__(NW * 64) void k_scan_bf16_mq(ScanArgs A) {
+__global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(2)))
+void k_scan_bf16_mq(ScanArgs A) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     constexpr int S = 8;                     /* dim 128: 8 k-steps, 1 unit per tile */
     u32x4 *qsh = (u32x4 *)smem;              /* [QTW][2][S][64] x 16 B */
@@ -556,6 +696,20 @@ __global__ __launch_bounds__(NW * 64) void k_scan_bf16_mq(ScanArgs A) {
         any |= qon[k];
     }
     if (!any) return;
+
+    __shared__ uint32_t s_next;
+    UnitCursor cur = cursor_init<NW>(A, wave, &s_next);
+    const u32x4 *vb = (const u32x4 *)A.vbf;
+    auto phys = [&](uint32_t t) { return min(t, A.ntiles - 1) * A.stride; };
+    auto load_rows = [&](u32x4 (&a)[8]) {
+        const u32x4 *p = vb + (uint64_t)phys(cur.tile) * 512 + l;
+#pragma unroll
+        for (int j = 0; j < 8; j++) a[j] = p[64 * j];
+    };
+    /* first rows in flight while the query operands are staged */
+    u32x4 a0[8], a1[8];
+    load_rows(a0);
+
     constexpr int half = 64;
     for (int e = tid; e < QTW * 2 * S * 64; e += blockDim.x) {
         const int k = e / (2 * S * 64), r = e % (2 * S * 64);
@@ -572,26 +726,17 @@ __global__ __launch_bounds__(NW * 64) void k_scan_bf16_mq(ScanArgs A) {
     if (!PREPASS)
 #pragma unroll
         for (int k = 0; k < QTW; k++) cand_init(Cs[k]);
-    __syncthreads();
 
     float tq[QTW][2], qn[QTW][2];
     const uint32_t *wr[QTW][2];
 #pragma unroll
     for (int k = 0; k < QTW; k++) load_query_state(A, qb0[k], tq[k], qn[k], wr[k]);
 
-    const uint32_t nwg = gridDim.x;
-    const uint32_t t0 = (uint32_t)(((uint64_t)A.ntiles * blockIdx.x) / nwg);
-    const uint32_t t1 = (uint32_t)(((uint64_t)A.ntiles * (blockIdx.x + 1)) / nwg);
-    const uint32_t first = t0 + wave;
-    const uint32_t nu = first < t1 ? (t1 - first + NW - 1) / NW : 0;
-    const u32x4 *vb = (const u32x4 *)A.vbf;
-
-    auto tile_of = [&](uint32_t u) { return min(first + NW * u, t1 - 1); };
-    auto load_unit = [&](uint32_t u, u32x4 (&a)[8], TileIn (&in)[QTW]) {
-        const uint32_t pt = tile_of(min(u, nu - 1)) * A.stride;
-        const u32x4 *p = vb + (uint64_t)pt * 512 + l;
-#pragma unroll
-        for (int j = 0; j < 8; j++) a[j] = __builtin_nontemporal_load(p + 64 * j);
+    auto load_in = [&](TileIn (&in)[QTW]) {
+        const uint32_t pt = phys(cur.tile);
+        in[0].t = cur.tile;
+        in[0].live = cur.tile < cur.end;
+        cursor_next(cur, 1, &s_next);
         load_tile_in<INL>(A, pt, wr[0], in[0]);
 #pragma unroll
         for (int k = 1; k < QTW; k++) {
@@ -599,8 +744,10 @@ __global__ __launch_bounds__(NW * 64) void k_scan_bf16_mq(ScanArgs A) {
             in[k].w[1] = INL ? 0u : wr[k][1][pt];
         }
     };
-    auto do_unit = [&](uint32_t u, const u32x4 (&a)[8], const TileIn (&in)[QTW], bool active) {
-        const uint32_t t = tile_of(u);
+    auto do_unit = [&](const u32x4 (&a)[8], const TileIn (&in)[QTW]) {
+        const uint32_t t = in[0].t;
+        float xn[16];
+        expand_norms(in[0].xr, xn);
         uint32_t iw[QTW][2];
         if (INL) {
             double cv[16];
@@ -634,21 +781,23 @@ __global__ __launch_bounds__(NW * 64) void k_scan_bf16_mq(ScanArgs A) {
             TileIn v = in[0];
             v.w[0] = INL ? iw[k][0] : in[k].w[0];
             v.w[1] = INL ? iw[k][1] : in[k].w[1];
-            tile_epilogue<PREPASS, true>(A, qb0[k], t, t * A.stride, acc, v, qn[k], tq[k], active,
-                                         Cs[k]);
+            tile_epilogue<PREPASS, true>(A, qb0[k], t, t * A.stride, acc, v, xn, qn[k], tq[k],
+                                         true, Cs[k]);
         }
     };
 
-    if (nu > 0) {
-        u32x4 a0[8], a1[8];
-        TileIn in0[QTW], in1[QTW];
-        load_unit(0, a0, in0);
-        for (uint32_t u = 0; u < nu; u += 2) {
-            load_unit(u + 1, a1, in1);
-            do_unit(u, a0, in0, true);
-            load_unit(u + 2, a0, in0);
-            do_unit(u + 1, a1, in1, u + 1 < nu);
-        }
+    TileIn in0[QTW], in1[QTW];
+    __syncthreads();
+    cursor_start(cur, &s_next);
+    load_in(in0);
+    while (in0[0].live) {
+        load_rows(a1);
+        load_in(in1);
+        do_unit(a0, in0);
+        if (!in1[0].live) break;
+        load_rows(a0);
+        load_in(in0);
+        do_unit(a1, in1);
     }
     if (!PREPASS)
 #pragma unroll
@@ -733,7 +882,8 @@ __global__ __launch_bounds__(256, 2) void k_scan_f32(ScanArgs A) {
         if (live) {
             TileIn in;
             load_tile_in(A, pt, wr, in);
-            tile_epilogue<PREPASS, false>(A, qbase, t, pt, acc, in, qn, tq, true, C);
+            const float xn[16] = {};
+            tile_epilogue<PREPASS, false>(A, qbase, t, pt, acc, in, xn, qn, tq, true, C);
         }
     }
     if (!PREPASS) flush_candidates(A, qbase, C);
@@ -905,7 +1055,7 @@ __global__ __launch_bounds__(256) void k_tau(const uint64_t *__restrict__ gmin,
     if (skip && skip[q]) { /* routed elsewhere: no candidates, no outputs */
         if (threadIdx.x == 0) {
             tau[q] = -__uint_as_float(0x7f800000u);
-            pcnt[(uint64_t)q * kPcntStride] = 0;
+            clear_pcnt(pcnt, q);
             status[q] = 0;
         }
         return;
@@ -928,7 +1078,7 @@ __global__ __launch_bounds__(256) void k_tau(const uint64_t *__restrict__ gmin,
     }
     if (threadIdx.x == 0) {
         tau[q] = tv;
-        pcnt[(uint64_t)q * kPcntStride] = 0;
+        clear_pcnt(pcnt, q);
         status[q] = 0;
     }
 }
@@ -938,7 +1088,7 @@ __global__ void k_reset(int nq, float *tau, uint32_t *pcnt, uint32_t *status) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q < nq) {
         tau[q] = __uint_as_float(0x7f800000u);
-        pcnt[(uint64_t)q * kPcntStride] = 0;
+        clear_pcnt(pcnt, q);
         status[q] = 0;
     }
 }
@@ -992,8 +1142,10 @@ __global__ __launch_bounds__(256) void k_gather(
         if (pass && row < count)
             d = cpu_order_l2(vecs + (uint64_t)row * dim, qv, dim, l16);
         if (pass && row < count && l16 == 0) {
-            const uint32_t gp = atomicAdd(&pcnt[(uint64_t)q * kPcntStride], 1u);
-            if (gp < cap_pool) pool[(uint64_t)q * cap_pool + gp] = make_key(d, row);
+            const uint32_t sh = blockIdx.y % kPoolShards;
+            const uint64_t scap = shard_cap(cap_pool);
+            const uint32_t gp = atomicAdd(&pcnt[pcnt_at(q, sh)], 1u);
+            if (gp < scap) pool[(uint64_t)q * cap_pool + sh * scap + gp] = make_key(d, row);
         }
     }
 }
@@ -1028,11 +1180,26 @@ __global__ __launch_bounds__(256) void k_select(
         for (int i = t; i < nprog; i += blockDim.x) progcnt_reset[i] = 0;
     if (topbq) topb = topbq[q]; /* per-query k of an uneven batch */
     if (skip && skip[q]) return; /* the graph search writes this query */
-    const uint64_t *pk = pool + (uint64_t)q * cap_pool;
-    const uint32_t ntot = pcnt[(uint64_t)q * kPcntStride];
-    const uint32_t n = (uint32_t)min((uint64_t)ntot, cap_pool);
+    /* the query's pool segments: counts, their prefix, and an index map
+     * i -> segment entry over the concatenation */
+    const uint64_t *pq = pool + (uint64_t)q * cap_pool;
+    const uint64_t scap = shard_cap(cap_pool);
+    uint32_t pre[kPoolShards + 1];
     uint32_t st = 0;
-    if ((uint64_t)ntot > cap_pool) st |= kStOverflow;
+    pre[0] = 0;
+#pragma unroll
+    for (uint32_t s = 0; s < kPoolShards; s++) {
+        const uint32_t c = pcnt[pcnt_at(q, s)];
+        if ((uint64_t)c > scap) st |= kStOverflow;
+        pre[s + 1] = pre[s] + (uint32_t)min((uint64_t)c, scap);
+    }
+    const uint32_t n = pre[kPoolShards];
+    auto pk_at = [&](uint32_t i) {
+        uint32_t s = 0;
+#pragma unroll
+        for (uint32_t j = 1; j < kPoolShards; j++) s += i >= pre[j] ? 1u : 0u;
+        return pq[s * scap + (i - pre[s])];
+    };
     const uint32_t B = min(topb, n);
     if (B == 0) {
         if (t == 0) {
@@ -1045,9 +1212,8 @@ __global__ __launch_bounds__(256) void k_select(
     for (int j = t; j < dim; j += blockDim.x) qsh[j] = qf[(uint64_t)q * dim + j];
     uint32_t m0 = kSelMax + 1;
     const uint64_t kth =
-        B <= 64 ? block_kth_small([&](uint32_t i) { return pk[i]; }, n, B, sel, kSelMax, &nsel,
-                                  wb, hist, tmp, bc, &m0)
-                : block_kth([&](uint32_t i) { return pk[i]; }, n, B, hist, tmp, bc);
+        B <= 64 ? block_kth_small(pk_at, n, B, sel, kSelMax, &nsel, wb, hist, tmp, bc, &m0)
+                : block_kth(pk_at, n, B, hist, tmp, bc);
     const float e = eps ? eps[q] : 0.f;
     float lim = key_dist(kth);
     if (e > 0.f) {
@@ -1064,7 +1230,7 @@ __global__ __launch_bounds__(256) void k_select(
         if (t == 0) nsel = 0;
         __syncthreads();
         for (uint32_t i = t; i < n; i += blockDim.x) {
-            const uint64_t k = pk[i];
+            const uint64_t k = pk_at(i);
             const bool take = (e > 0.f) ? (key_dist(k) <= lim || k <= kth) : (k <= kth);
             if (take) {
                 const uint32_t pos = atomicAdd(&nsel, 1u);
@@ -1145,6 +1311,8 @@ void free_batch_buffers(FvGpuBatch *b) {
     b->gexec = nullptr;
     b->graph = nullptr;
     b->warmed = false;
+    delete (ScanLaunch *)b->last_scan;
+    b->last_scan = nullptr;
 }
 
 int stage_batch(FvGpuIndex *g, FvGpuBatch *b, int nq, const float *queries,
@@ -1297,7 +1465,7 @@ int stage_batch(FvGpuIndex *g, FvGpuBatch *b, int nq, const float *queries,
     PV_TRY(b->d_out.ensure((size_t)nq * topb * sizeof(FvHit)));
     PV_TRY(b->d_cnt.ensure((size_t)nq * 4));
     PV_TRY(b->d_flag.ensure((size_t)nq));
-    PV_TRY(b->d_pcnt.ensure((size_t)nqp * kPcntStride * 4));
+    PV_TRY(b->d_pcnt.ensure((size_t)nqp * kPoolShards * kPcntStride * 4));
     /* per-program passing-row counts of routed runs: zero here, then
      * accumulated by each run's bitmap pass and cleared by its select */
     PV_TRY(b->d_progcnt.ensure((size_t)std::max(1, b->nprog) * 4));
@@ -1388,6 +1556,77 @@ hipError_t launch_phase1(FvGpuIndex *g, FvGpuBatch *b, int prec, bool bitmap, hi
     }
 }
 
+/* Launch shape of the full-scan route: query tiles per WG, waves per WG,
+ * WGs per CU, grid and the pre-pass sampling stride. The stride is the
+ * over-fetch factor: a tau at the B-th smallest minimum of groups sampled
+ * every stride-th tile admits about stride x B candidates per query. */
+struct ScanShape {
+    int qtw, nw, wpc, gy;
+    uint32_t gx, stride;
+};
+
+static ScanShape scan_shape(const FvGpuIndex *g, const FvGpuBatch *b, const FvGpuBruteOpts *o,
+                            int prec, uint32_t stride_override) {
+    ScanShape sh;
+    const int dim = g->dim;
+    const uint32_t B = b->topb;
+    const uint32_t ntiles = (uint32_t)(g->count_pad / 32);
+    const bool bf = prec == FV_GPU_PREC_BF16;
+    /* query tiles per WG: several when many query tiles re-read the same
+     * data (dim 128, BF16) */
+    int qtw = (o && o->qtw > 0) ? o->qtw : (b->nqt >= 8 ? 4 : 1);
+    if (!bf || dim != 128 || b->nqt < 2) qtw = 1;
+    qtw = qtw >= 4 ? 4 : (qtw >= 2 ? 2 : 1);
+    sh.qtw = qtw;
+    sh.gy = (b->nqt + qtw - 1) / qtw;
+    /* 8 waves per WG (one WG per CU) sharing one copy of the query operands
+     * and one tile counter measured best for the single-tile BF16 scan;
+     * 4 waves for the two-tile variant */
+    const bool forced = qtw == 1 && o && (o->waves == 4 || o->waves == 8) &&
+                        !(dim > 256 && o->waves == 4);
+    sh.nw = forced ? o->waves : (qtw == 2 ? 4 : 8);
+    /* main-scan workgroups per query tile (also bounds the candidates each
+     * WG collects per query: at least B / gx of them) */
+    auto grid_x = [&]() {
+        sh.wpc = (o && o->wg_per_cu > 0) ? std::min(o->wg_per_cu, 16)
+                                         : (bf && sh.nw == 8 ? 1 : 2);
+        uint32_t x = (uint32_t)std::max(64, (sh.wpc * g->ncu) / std::max(1, sh.gy));
+        return std::min<uint32_t>(x, std::max<uint32_t>(1, ntiles / sh.nw));
+    };
+    uint32_t gx = grid_x();
+    /* a large B over few rows: 4-wave WGs (twice the grid) keep each WG's
+     * share of a query's candidates within its LDS slots */
+    if (!forced && qtw == 1 && dim <= 256 && sh.nw == 8 && (uint64_t)2 * B > (uint64_t)gx * kLdsCand) {
+        sh.nw = 4;
+        gx = grid_x();
+    }
+    sh.gx = gx;
+    uint32_t stride = stride_override ? stride_override
+                                      : (o && o->sample_stride ? o->sample_stride : 0);
+    if (stride == 0) {
+        /* the B-th smallest minimum over groups of 16 sampled rows: about
+         * stride x B rows of a query fall under it; keep that near half the
+         * WG's per-query LDS slots per WG */
+        stride = std::min<uint32_t>(64, std::max<uint32_t>(1, (uint32_t)kLdsCand / 2 * gx / B));
+        const uint32_t min_stride = (ntiles + 131071) / 131072;   /* <= 2^18 groups */
+        stride = std::max(stride, min_stride);
+    }
+    sh.stride = std::min(stride, ntiles);
+    return sh;
+}
+
+static hipError_t launch_main_scan(const ScanLaunch &L, hipStream_t s) {
+    if (L.bf && L.qtw > 1) {
+        const hipError_t e = launch_scan_mq<false>(L.qtw, L.inl, L.gx, L.gy, s, L.M);
+        if (e != hipSuccess) return e;
+    } else if (L.bf) {
+        launch_scan_bf16<false>(L.nw, L.inl, L.gx, L.nqt, L.shm, s, L.M);
+    } else {
+        hipLaunchKernelGGL(k_scan_f32<false>, dim3(L.gx, L.nqt), dim3(256), 0, s, L.M);
+    }
+    return hipGetLastError();
+}
+
 /* Enqueue the brute pipeline. stride_override > 0 forces the pre-pass
  * sample stride (1 = every row: the tightest threshold). */
 int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
@@ -1439,12 +1678,16 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
         if (b->max_total > (uint64_t)(1u << 22))
             PV_FAIL("gpu: gather route: enumeration of %llu rows too large",
                     (unsigned long long)b->max_total);
-        b->cap_pool = std::max<uint64_t>(1024, b->max_total);
+        const uint32_t chunks = (uint32_t)std::min<uint64_t>(
+            64, std::max<uint64_t>(1, (b->max_total + 4095) / 4096));
+        /* gather chunk y appends to pool segment y % kPoolShards: a segment
+         * holds at most ceil(chunks / shards) chunks of ceil(total / chunks) */
+        const uint64_t per_chunk = (b->max_total + chunks - 1) / chunks;
+        const uint64_t seg = (uint64_t)((chunks + kPoolShards - 1) / kPoolShards) * per_chunk;
+        b->cap_pool = kPoolShards * std::max<uint64_t>(128, seg);
         PV_TRY(b->d_pool.ensure((size_t)nq * b->cap_pool * 8));
         hipLaunchKernelGGL(k_reset, dim3((nq + 255) / 256), dim3(256), 0, s, nq,
                            qa.tau, b->d_pcnt.as<uint32_t>(), qa.status);
-        const uint32_t chunks = (uint32_t)std::min<uint64_t>(
-            64, std::max<uint64_t>(1, (b->max_total + 4095) / 4096));
         hipLaunchKernelGGL(k_gather, dim3(nq, chunks), dim3(256), 0, s, g->d_vecs,
                            g->d_cols, g->d_prows, g->count, dim, b->d_q, b->d_qd,
                            b->d_spans, b->d_spanoff, b->d_ops, b->d_poff, b->d_plen,
@@ -1471,30 +1714,10 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
     /* ---- full-scan route ---- */
     /* pre-pass samples every stride-th 32-row tile */
     const uint32_t ntiles = (uint32_t)(g->count_pad / 32);
-    uint32_t stride = stride_override ? stride_override
-                                      : (o && o->sample_stride ? o->sample_stride : 0);
-    /* main-scan workgroups per query tile (also bounds the candidates each
-     * WG collects per query: about stride x B / gx of them) */
     const bool bf = prec == FV_GPU_PREC_BF16;
-    /* query tiles per WG: several when many query tiles re-read the same
-     * data (dim 128, BF16) */
-    int qtw = (o && o->qtw > 0) ? o->qtw : (b->nqt >= 8 ? 4 : 1);
-    if (!bf || dim != 128 || b->nqt < 2) qtw = 1;
-    qtw = qtw >= 4 ? 4 : (qtw >= 2 ? 2 : 1);
-    const int gy = (b->nqt + qtw - 1) / qtw;
-    const int nw = qtw == 4 ? 8 : (dim <= 256 ? 4 : 8);
-    const int wpc = (o && o->wg_per_cu > 0) ? o->wg_per_cu : (bf && nw == 8 ? 1 : 2);
-    uint32_t gx = (uint32_t)std::max(64, (wpc * g->ncu) / std::max(1, gy));
-    gx = std::min<uint32_t>(gx, std::max<uint32_t>(1, ntiles / nw));
-    if (stride == 0) {
-        /* the B-th smallest minimum over groups of 16 sampled rows: about
-         * stride x B rows of a query fall under it; keep that near half the
-         * WG's per-query LDS slots per WG */
-        stride = std::min<uint32_t>(64, std::max<uint32_t>(1, (uint32_t)kLdsCand / 2 * gx / B));
-        const uint32_t min_stride = (ntiles + 131071) / 131072;   /* <= 2^18 groups */
-        stride = std::max(stride, min_stride);
-    }
-    stride = std::min(stride, ntiles);
+    const ScanShape sh = scan_shape(g, b, o, prec, stride_override);
+    const int qtw = sh.qtw, gy = sh.gy, nw = sh.nw;
+    const uint32_t gx = sh.gx, stride = sh.stride;
     const uint32_t stiles = (ntiles + stride - 1) / stride;
     b->stride = stride;
     b->ngroups = stiles * 2;
@@ -1528,6 +1751,8 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
     A.skip = skip;
     A.inl = inl ? b->d_inl_p : nullptr;
     A.icol = inl ? g->d_cols + (uint64_t)b->inl_col * g->count : nullptr;
+    A.sweep = (o && o->order == 2) ? 1u : 0u;
+    A.xn2max = g->xmax_norm * g->xmax_norm;
 
     /* 4 waves per WG up to 256 dims (2 WGs per CU), 8 waves above (the
      * query operands then take most of the LDS) */
@@ -1546,7 +1771,10 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
         P.ntiles = stiles;
         P.stride = stride;
         if (bf) {
-            const uint32_t gxp = std::max<uint32_t>(1, (stiles + nw - 1) / nw);
+            /* each pre-pass WG walks several sampled tiles (at most the
+             * main scan's grid): staging 64 query operands per WG costs as
+             * much as a few tiles */
+            const uint32_t gxp = std::min<uint32_t>(gx, std::max<uint32_t>(1, (stiles + nw - 1) / nw));
             if (qtw > 1) PV_TRY(launch_scan_mq<true>(qtw, inl, gxp, gy, s, P));
             else launch_scan_bf16<true>(nw, inl, gxp, b->nqt, shm_bf, s, P);
         } else {
@@ -1562,16 +1790,21 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
     PV_TRY(hipGetLastError());
     ev_record(g, 3);
     {
-        ScanArgs M = A;
-        M.ntiles = ntiles;
-        M.stride = 1;
-        if (bf && qtw > 1)
-            PV_TRY(launch_scan_mq<false>(qtw, inl, gx, gy, s, M));
-        else if (bf)
-            launch_scan_bf16<false>(nw, inl, gx, b->nqt, shm_bf, s, M);
-        else
-            hipLaunchKernelGGL(k_scan_f32<false>, dim3(gx, b->nqt), dim3(256), 0, s, M);
-        PV_TRY(hipGetLastError());
+        ScanLaunch L;
+        L.M = A;
+        L.M.ntiles = ntiles;
+        L.M.stride = 1;
+        L.bf = bf;
+        L.inl = inl;
+        L.qtw = qtw;
+        L.nw = nw;
+        L.nqt = b->nqt;
+        L.gy = gy;
+        L.gx = gx;
+        L.shm = shm_bf;
+        PV_TRY(launch_main_scan(L, s));
+        if (b->last_scan == nullptr) b->last_scan = new ScanLaunch();
+        *(ScanLaunch *)b->last_scan = L;
     }
     ev_record(g, 4);
     hipLaunchKernelGGL(k_select, dim3(nq), dim3(256), dim * 4, s,
@@ -1639,11 +1872,13 @@ static void fill_stats_times(FvGpuIndex *g) {
     /* candidate-pool sizes of the last batch (timing mode only: a copy) */
     FvGpuBatch *b = g->last_batch;
     if (b && b->nq > 0 && b->d_pcnt.p) {
-        std::vector<uint32_t> pc((size_t)b->nq * kPcntStride);
+        std::vector<uint32_t> pc((size_t)b->nq * kPoolShards * kPcntStride);
         if (hipMemcpy(pc.data(), b->d_pcnt.p, pc.size() * 4, hipMemcpyDeviceToHost) ==
             hipSuccess) {
             uint64_t s = 0;
-            for (int q = 0; q < b->nq; q++) s += pc[(size_t)q * kPcntStride];
+            for (int q = 0; q < b->nq; q++)
+                for (uint32_t h = 0; h < kPoolShards; h++)
+                    s += pc[((size_t)q * kPoolShards + h) * kPcntStride];
             st.candidates = s;
         }
     }
@@ -1859,6 +2094,159 @@ extern "C" int fv_gpu_batch_run_brute(FvGpuIndex *g, FvGpuBatch *b,
     if (g == nullptr || b == nullptr || b->owner != g) PV_FAIL("gpu: bad batch");
     try {
         if (opts && opts->graph && !g->timing) return run_brute_graph(g, b, opts, errbuf, errlen);
+        return run_brute(g, b, opts, 0, errbuf, errlen);
+    } catch (...) {
+        snprintf(errbuf, errlen, "gpu: internal error");
+        return -1;
+    }
+}
+
+/* Autotuner: times every candidate launch shape of the full-scan route on
+ * this batch (query tiles per WG {1, 2, 4} where the multi-tile scan
+ * applies, waves per single-tile WG {4, 8} up to 256 dims, WGs per CU
+ * {1, 2, 3, 4}, tile order {chunked, sweep}, pre-pass
+ * stride = over-fetch factor {auto/2, auto, 2 auto}), keeps those whose
+ * runs certify every query (no
+ * pool overflow, no oversized re-rank set) and returns the fastest. All
+ * candidates compute the same exact result; only the time differs. */
+static int tune_batch(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *base, int reps,
+                      FvGpuBruteOpts *best, double *best_us, int *ntried, char *errbuf,
+                      size_t errlen) {
+    FvGpuBruteOpts o0 = base ? *base : FvGpuBruteOpts{};
+    o0.graph = 0;
+    o0.qtw = 0;
+    o0.wg_per_cu = 0;
+    o0.sample_stride = 0;
+    o0.order = 0;
+    o0.waves = 0;
+    const int prec = scan_precision(g, &o0);
+    reps = std::max(1, std::min(reps, 64));
+    std::vector<uint32_t> st((size_t)b->nq);
+    QAux qa = qaux_of(b);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    PV_TRY(hipEventCreate(&e0));
+    PV_TRY(hipEventCreate(&e1));
+    struct Ev {
+        hipEvent_t a, b;
+        ~Ev() { hipEventDestroy(a); hipEventDestroy(b); }
+    } guard{e0, e1};
+
+    /* one timed candidate: a warm run, then reps runs between events;
+     * +inf when a query is not certified */
+    auto trial = [&](const FvGpuBruteOpts &o, double *us) -> int {
+        if (run_brute(g, b, &o, 0, errbuf, errlen) != 0) return -1;
+        PV_TRY(hipEventRecord(e0, b->stream));
+        for (int r = 0; r < reps; r++)
+            if (run_brute(g, b, &o, 0, errbuf, errlen) != 0) return -1;
+        PV_TRY(hipEventRecord(e1, b->stream));
+        PV_TRY(hipEventSynchronize(e1));
+        float ms = 0.f;
+        PV_TRY(hipEventElapsedTime(&ms, e0, e1));
+        PV_TRY(hipMemcpy(st.data(), qa.status, st.size() * 4, hipMemcpyDeviceToHost));
+        bool ok = true;
+        for (uint32_t v : st) ok &= v == 0;
+        *us = ok ? 1e3 * ms / reps : HUGE_VAL;
+        return 0;
+    };
+
+    double t0;
+    if (trial(o0, &t0) != 0) return -1;
+    *best = o0;
+    *best_us = t0;
+    *ntried = 1;
+    if (b->last_route != 1) return 0;   /* gather route: no scan shape to tune */
+    const bool mq = prec == FV_GPU_PREC_BF16 && g->dim == 128 && b->nqt >= 2;
+    const int qtws[3] = {1, 2, 4};
+    const bool wv = prec == FV_GPU_PREC_BF16 && g->dim <= 256;
+    for (int qi = 0; qi < (mq ? 3 : 1); qi++)
+        for (int waves = 4; waves <= ((qi == 0 && wv) ? 8 : 4); waves += 4)
+        for (int wpc = 1; wpc <= 4; wpc++)
+        for (int order = 1; order <= 2; order++) {
+            FvGpuBruteOpts o = o0;
+            o.qtw = qtws[qi];
+            o.waves = qi == 0 && wv ? waves : 0;
+            o.wg_per_cu = wpc;
+            o.order = order;
+            const uint32_t sa = scan_shape(g, b, &o, prec, 0).stride;
+            const uint32_t cand[3] = {std::max<uint32_t>(1, sa / 2), sa,
+                                      std::min<uint32_t>(64, sa * 2)};
+            for (int si = 0; si < 3; si++) {
+                if (si > 0 && cand[si] == cand[si - 1]) continue;
+                o.sample_stride = cand[si];
+                double t;
+                if (trial(o, &t) != 0) return -1;
+                ++*ntried;
+                if (t < *best_us) {
+                    *best_us = t;
+                    *best = o;
+                }
+            }
+        }
+    if (!std::isfinite(*best_us))
+        PV_FAIL("gpu: no launch shape certifies every query of this batch");
+    return 0;
+}
+
+extern "C" int fv_gpu_batch_tune(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *base,
+                                 int reps, FvGpuBruteOpts *best, double *best_us,
+                                 int *ntried, char *errbuf, size_t errlen) {
+    if (g == nullptr || b == nullptr || b->owner != g || best == nullptr)
+        PV_FAIL("gpu: bad batch");
+    try {
+        PV_TRY(hipSetDevice(g->device));
+        const bool timing = g->timing;
+        g->timing = false;
+        double us = 0.0;
+        int n = 0;
+        const int rc = tune_batch(g, b, base, reps, best, &us, &n, errbuf, errlen);
+        g->timing = timing;
+        if (best_us) *best_us = us;
+        if (ntried) *ntried = n;
+        return rc;
+    } catch (...) {
+        snprintf(errbuf, errlen, "gpu: internal error");
+        return -1;
+    }
+}
+
+/* Main-scan kernel time of a batch: one full run (so tau and the pool
+ * are set up), then the same scan kernel launched reps times back to back
+ * between two events on the batch's stream, then one more full run so the
+ * batch's outputs and pool state are those of an ordinary run. */
+extern "C" int fv_gpu_batch_time_scan(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *opts,
+                                      int reps, double *us, char *errbuf, size_t errlen) {
+    if (g == nullptr || b == nullptr || b->owner != g || us == nullptr || reps < 1)
+        PV_FAIL("gpu: bad batch");
+    try {
+        PV_TRY(hipSetDevice(g->device));
+        const bool timing = g->timing;
+        g->timing = false;
+        struct Restore {
+            FvGpuIndex *g;
+            bool t;
+            ~Restore() { g->timing = t; }
+        } restore{g, timing};
+        if (run_brute(g, b, opts, 0, errbuf, errlen) != 0) return -1;
+        if (b->last_route != 1 || b->last_scan == nullptr)
+            PV_FAIL("gpu: time_scan: the batch did not take the full-scan route");
+        const ScanLaunch L = *(const ScanLaunch *)b->last_scan;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        PV_TRY(hipEventCreate(&e0));
+        struct Ev {
+            hipEvent_t *a, *b;
+            ~Ev() {
+                if (*a) hipEventDestroy(*a);
+                if (*b) hipEventDestroy(*b);
+            }
+        } guard{&e0, &e1};
+        PV_TRY(hipEventCreate(&e1));
+        PV_TRY(hipEventRecord(e0, b->stream));
+        for (int r = 0; r < reps; r++) PV_TRY(launch_main_scan(L, b->stream));
+        PV_TRY(hipEventRecord(e1, b->stream));
+        PV_TRY(hipEventSynchronize(e1));
+        float ms = 0.f;
+        PV_TRY(hipEventElapsedTime(&ms, e0, e1));
+        *us = 1e3 * (double)ms / reps;
         return run_brute(g, b, opts, 0, errbuf, errlen);
     } catch (...) {
         snprintf(errbuf, errlen, "gpu: internal error");

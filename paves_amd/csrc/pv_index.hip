@@ -366,19 +366,41 @@ __global__ __launch_bounds__(256) void k_copy_calib(const cp_u32x4 *__restrict__
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride)
         __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
 }
+
+/* grid-stride 16-byte streaming read (ordinary cached loads, as the scans
+ * use), four loads in flight per lane; the words are folded by XOR and
+ * stored only if they match a value the fill never produces, so the loads
+ * stay live while nothing is written */
+__global__ __launch_bounds__(256) void k_read_calib(const cp_u32x4 *__restrict__ src,
+                                                    cp_u32x4 *__restrict__ dst, uint64_t n16) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    cp_u32x4 acc = {0u, 0u, 0u, 0u};
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + 3 * stride < n16; i += 4 * stride) {
+        const cp_u32x4 a = src[i];
+        const cp_u32x4 b = src[i + stride];
+        const cp_u32x4 c = src[i + 2 * stride];
+        const cp_u32x4 d = src[i + 3 * stride];
+        acc ^= a ^ b ^ c ^ d;
+    }
+    for (; i < n16; i += stride) acc ^= src[i];
+    if (acc.x == 0xDEADBEEFu && acc.y == 0x0BADF00Du) dst[0] = acc;
+}
 } // namespace
 } // namespace pv
 
-extern "C" int fv_gpu_copy_bench(int device, uint64_t bytes, int reps, double *gbs,
-                                 char *errbuf, size_t errlen) {
+/* device copy (read_only = false: read + write) or streaming read over
+ * `bytes`, timed with events over reps launches; GB/s of bytes moved */
+static int mem_bench(int device, uint64_t bytes, int reps, bool read_only, double *gbs,
+                     char *errbuf, size_t errlen) {
     if (bytes == 0 || bytes % 16 != 0 || reps < 1 || gbs == nullptr)
-        PV_FAIL("gpu: copy bench: bad arguments");
+        PV_FAIL("gpu: memory bench: bad arguments");
     PV_TRY(hipSetDevice(device));
     void *a = nullptr, *b = nullptr;
     hipStream_t s = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     hipError_t e = hipMalloc(&a, bytes);
-    if (e == hipSuccess) e = hipMalloc(&b, bytes);
+    if (e == hipSuccess) e = hipMalloc(&b, read_only ? 4096 : bytes);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&e0);
     if (e == hipSuccess) e = hipEventCreate(&e1);
@@ -387,14 +409,18 @@ extern "C" int fv_gpu_copy_bench(int device, uint64_t bytes, int reps, double *g
     if (e == hipSuccess) {
         const uint64_t n16 = bytes / 16;
         const unsigned grid = 8192;
-        hipLaunchKernelGGL(pv::k_copy_calib, dim3(grid), dim3(256), 0, s, (const pv::cp_u32x4 *)a,
-                           (pv::cp_u32x4 *)b, n16);
-        e = hipEventRecord(e0, s);
-        for (int r = 0; r < reps && e == hipSuccess; r++) {
-            hipLaunchKernelGGL(pv::k_copy_calib, dim3(grid), dim3(256), 0, s,
-                               (const pv::cp_u32x4 *)a, (pv::cp_u32x4 *)b, n16);
-            e = hipGetLastError();
-        }
+        auto launch = [&]() {
+            if (read_only)
+                hipLaunchKernelGGL(pv::k_read_calib, dim3(grid), dim3(256), 0, s,
+                                   (const pv::cp_u32x4 *)a, (pv::cp_u32x4 *)b, n16);
+            else
+                hipLaunchKernelGGL(pv::k_copy_calib, dim3(grid), dim3(256), 0, s,
+                                   (const pv::cp_u32x4 *)a, (pv::cp_u32x4 *)b, n16);
+            return hipGetLastError();
+        };
+        e = launch();
+        if (e == hipSuccess) e = hipEventRecord(e0, s);
+        for (int r = 0; r < reps && e == hipSuccess; r++) e = launch();
         if (e == hipSuccess) e = hipEventRecord(e1, s);
         if (e == hipSuccess) e = hipEventSynchronize(e1);
         if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
@@ -404,7 +430,18 @@ extern "C" int fv_gpu_copy_bench(int device, uint64_t bytes, int reps, double *g
     if (s) hipStreamDestroy(s);
     if (b) hipFree(b);
     if (a) hipFree(a);
-    if (e != hipSuccess) PV_FAIL("gpu: copy bench: %s", hipGetErrorString(e));
-    *gbs = ms > 0.f ? 2.0 * (double)bytes * reps / ((double)ms * 1e-3) / 1e9 : 0.0;
+    if (e != hipSuccess) PV_FAIL("gpu: memory bench: %s", hipGetErrorString(e));
+    const double moved = (read_only ? 1.0 : 2.0) * (double)bytes * reps;
+    *gbs = ms > 0.f ? moved / ((double)ms * 1e-3) / 1e9 : 0.0;
     return 0;
+}
+
+extern "C" int fv_gpu_copy_bench(int device, uint64_t bytes, int reps, double *gbs,
+                                 char *errbuf, size_t errlen) {
+    return mem_bench(device, bytes, reps, false, gbs, errbuf, errlen);
+}
+
+extern "C" int fv_gpu_read_bench(int device, uint64_t bytes, int reps, double *gbs,
+                                 char *errbuf, size_t errlen) {
+    return mem_bench(device, bytes, reps, true, gbs, errbuf, errlen);
 }

@@ -443,6 +443,29 @@ class GpuIndex:
         if rc != 0:
             raise GpuError(eb.value.decode())
 
+    def tune(self, bh, base: L.BruteOpts | None = None, reps: int = 3):
+        """Autotune the launch shape for a staged batch: (best opts, us per
+        run, shapes timed)."""
+        best = L.BruteOpts()
+        us = C.c_double(0.0)
+        n = C.c_int(0)
+        eb = L.errbuf()
+        rc = self.lib.fv_gpu_batch_tune(self.h, bh, C.byref(base) if base else None, reps,
+                                        C.byref(best), C.byref(us), C.byref(n), eb, len(eb))
+        if rc != 0:
+            raise GpuError(eb.value.decode())
+        return best, us.value, n.value
+
+    def time_scan(self, bh, opts: L.BruteOpts | None = None, reps: int = 20) -> float:
+        """Microseconds per main-scan launch, reps launches back to back."""
+        us = C.c_double(0.0)
+        eb = L.errbuf()
+        rc = self.lib.fv_gpu_batch_time_scan(self.h, bh, C.byref(opts) if opts else None, reps,
+                                             C.byref(us), eb, len(eb))
+        if rc != 0:
+            raise GpuError(eb.value.decode())
+        return us.value
+
     def run_learned(self, bh, model, opts: L.LearnedOpts):
         eb = L.errbuf()
         rc = self.lib.fv_gpu_batch_run_learned(self.h, bh, model.h if model else None,

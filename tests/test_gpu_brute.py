@@ -135,6 +135,11 @@ def test_max_batch_and_large_topb():
             oh, oc, ot, _ = O.brute_batch(d["base"], cols, qs, b4.qd, b4.nodes, b4.children,
                                           b4.invals, topb)
             assert_same(out, cnt, trunc, oh, oc, ot, f"topb={topb}")
+            # 8-wave WGs forced: at topb 4096 a WG holds more of a query's
+            # candidates than its LDS slots and spill area; the rest go
+            # straight to its global pool segment
+            out, cnt, trunc = g.brute(qs, b4, topb, L.BruteOpts(route=1, waves=8))
+            assert_same(out, cnt, trunc, oh, oc, ot, f"topb={topb} waves=8")
     finally:
         g.close()
 
