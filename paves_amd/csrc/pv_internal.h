@@ -163,6 +163,7 @@ struct FvGpuBatch {
     FvGpuBruteOpts gopts = {};
     bool warmed = false;    /* one eager run done: every buffer allocated */
     void *last_scan = nullptr; /* launch record of the last main scan (timing) */
+    bool capturing = false; /* a hipGraph capture of this batch's run is open */
     bool query_exact = true;
     std::vector<uint64_t> total;   /* host copy of qd[q].total */
     std::vector<int32_t> nspans;
