@@ -325,6 +325,9 @@ def main():
     log(f"plain fp32 brute: {qps32:,.0f} QPS")
     res_fp = [g.fetch(bh, e - s, args.k) for bh, (s, e, _) in zip(staged, batches)]
 
+    # ---- SQ8 (exact INT8 MFMA scan of x - centre) on a second index
+    sq8 = sq8_path(d, cols, rows, batches, res_bf, args)
+
     # ---- the reference's synchronous entry point (host buffers in and out:
     # predicate compile, H2D staging, pipeline, D2H), one pass, for the
     # PCIe-inclusive rate
@@ -425,6 +428,7 @@ def main():
         "plain_fp32_qps": round(qps32, 1),
         "speedup_vs_plain_fp32": round(value / qps32, 3),
         "sync_abi_qps_host_buffers": round(qps_sync, 1),
+        "sq8": sq8,
         "ids_bit_exact_bf16_vs_fp32": exact,
         "oracle_sample_ok": sample_ok,
         "overflow_batches": int(overflow),
@@ -693,6 +697,48 @@ def uneven_path(args):
         "cpu_baseline": cpu,
     }
     print(json.dumps(res), flush=True)
+
+
+def sq8_path(d, cols, rows, batches, res_bf, args):
+    """The same micro-batches on an index uploaded with FV_GPU_PREC_SQ8:
+    integer data spanning <= 255 values scan as int8 (x - centre) with
+    v_mfma_i32_32x32x32_i8 — exact, half the bytes of BF16. Throughput,
+    results against the BF16 run, and the INT8 scan kernel's rate
+    (back-to-back launches, as for the BF16 roofline)."""
+    g8 = E.GpuIndex(d["base"], cols, rows, precision=L.FV_GPU_PREC_SQ8, device=0)
+    staged8 = [g8.stage(d["queries"][s:e], b, args.k) for s, e, b in batches]
+    sq = L.BruteOpts(precision=L.FV_GPU_PREC_SQ8, route=0)
+    run_steps(g8, staged8, sq, max(1, args.warmup))
+    dt8 = run_steps(g8, staged8, sq, args.steps)
+    qps8 = args.steps * args.nq / dt8
+    res8 = [g8.fetch(bh, e - s, args.k) for bh, (s, e, _) in zip(staged8, batches)]
+    same = all(np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2]) and
+               all(np.array_equal(a[0][q, : a[1][q]], b[0][q, : b[1][q]])
+                   for q in range(len(a[1]))) for a, b in zip(res8, res_bf))
+    scan_us, nprog, used = [], [], 0
+    for bh in staged8[:40]:
+        g8.run_brute(bh, sq)
+        g8.sync()
+        st = g8.stats()
+        used += st.precision == L.FV_GPU_PREC_SQ8
+        if st.route == 1 and st.precision == L.FV_GPU_PREC_SQ8:
+            nprog.append(st.nprog)
+            scan_us.append(g8.time_scan(bh, sq, 10))
+    for bh in staged8:
+        g8.free_batch(bh)
+    g8.close()
+    if not scan_us:
+        log("sq8: the INT8 scan was not taken")
+        return {"qps": round(qps8, 1), "int8_scan_taken": False, "exact_vs_bf16": same}
+    us = float(np.mean(scan_us))
+    alg = args.n * (128 + 4) + float(np.mean(nprog)) * args.n / 8.0
+    gbs = alg / (us * 1e-6) / 1e9
+    log(f"sq8 (INT8 scan): {qps8:,.0f} QPS, results equal BF16: {same}; scan kernel "
+        f"{us:.1f} us -> {gbs:.0f} GB/s")
+    return {"qps": round(qps8, 1), "int8_scan_taken": True, "batches_int8": used,
+            "exact_vs_bf16": same, "kernel": "k_scan_i8<false, 8, 2, false>",
+            "avg_launch_us": round(us, 2), "alg_bytes_per_launch": int(alg),
+            "achieved_GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
 
 
 def learned_path(g, graph, d, post, staged, batches, truth, args, setup):

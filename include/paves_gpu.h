@@ -274,8 +274,8 @@ int fv_gpu_batch_tune(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *base,
 /* Main-scan kernel timing (no reference counterpart): runs the batch once,
  * launches the same full-scan kernel reps times back to back between two
  * events on the batch's stream (*us = time per launch), then runs the
- * batch once more so its outputs are those of an ordinary run. Fails when
- * the batch takes the span-gather route. */
+ * batch once more so its outputs are those of an ordinary run, and waits
+ * for it. Fails when the batch takes the span-gather route. */
 int fv_gpu_batch_time_scan(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *opts,
                            int reps, double *us, char *errbuf, size_t errlen);
 /* Enqueue the graph pipeline for a staged batch (asynchronous). */

@@ -46,6 +46,8 @@
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef int32_t i32x4 __attribute__((ext_vector_type(4)));
+typedef int32_t i32x16 __attribute__((ext_vector_type(16)));
 
 namespace pv {
 
@@ -54,7 +56,13 @@ namespace pv {
 /* ---------------- 1. query preparation ---------------- */
 
 /*
- * One wave per (padded) query. mode: 0 = FP32 scan bound, 1 = BF16 bound.
+ * One wave per (padded) query. mode: 0 = FP32 scan bound, 1 = BF16 bound,
+ * 2 = INT8 scan: also writes q - centre as int8 and ||q - centre||^2; the
+ * INT8 scan is exact (eps = 0) and the host takes it only when every query
+ * value is an integer within [centre - 128, centre + 127] (a value outside
+ * sets kStPrepFail in the query's status, which the threshold stage keeps,
+ * so such a run can never pass as exact). Every mode writes the status
+ * word, clearing the bit a previous run may have left.
  * The scan is exact (eps = 0) when every vector and query value is an
  * integer of magnitude <= 256 (exact in BF16, products exact in FP32) and
  * every partial sum stays below 2^24: sum (x-q)^2 <= D * delta^2 with
@@ -64,11 +72,12 @@ namespace pv {
 __device__ __forceinline__ void prep_query(
     int qi, const float *__restrict__ q, int nq, int dim, uint16_t *__restrict__ qbf,
     float *__restrict__ qnorm, float *__restrict__ eps, float xmax_norm,
-    float vmin, float vmax, int all_int, int mode) {
+    float vmin, float vmax, int all_int, int mode, int8_t *__restrict__ qi8 = nullptr,
+    float *__restrict__ qn8 = nullptr, int centre = 0, uint32_t *__restrict__ status = nullptr) {
     const int l = threadIdx.x & 63;
     const float *src = q + (uint64_t)qi * dim;
-    float ss = 0.f, pmax = 0.f, nmax = 0.f;
-    int nonint = 0;
+    float ss = 0.f, pmax = 0.f, nmax = 0.f, s8 = 0.f;
+    int nonint = 0, out8 = 0;
     for (int j = l; j < dim; j += 64) {
         float v = src[j];
         qbf[(uint64_t)qi * dim + j] = f2bf(v);
@@ -76,13 +85,26 @@ __device__ __forceinline__ void prep_query(
         pmax = fmaxf(pmax, v);
         nmax = fmaxf(nmax, -v);
         nonint |= (v != rintf(v)) ? 1 : 0;
+        if (mode == 2) {
+            const float c = v - (float)centre;
+            out8 |= (c != rintf(c) || c < -128.f || c > 127.f) ? 1 : 0;
+            qi8[(uint64_t)qi * dim + j] = (int8_t)(int)fminf(fmaxf(c, -128.f), 127.f);
+            s8 = fmaf(c, c, s8);
+        }
     }
     for (int o = 32; o; o >>= 1) {
         ss += __shfl_xor(ss, o);
         pmax = fmaxf(pmax, __shfl_xor(pmax, o));
         nmax = fmaxf(nmax, __shfl_xor(nmax, o));
         nonint |= __shfl_xor(nonint, o);
+        if (mode == 2) {
+            s8 += __shfl_xor(s8, o);
+            out8 |= __shfl_xor(out8, o);
+        }
     }
+    if (l == 0 && mode == 2) qn8[qi] = s8;
+    if (l == 0 && status != nullptr && qi < nq)
+        status[qi] = (mode == 2 && out8) ? kStPrepFail : 0u;
     if (l == 0) {
         qnorm[qi] = ss;
         double e = 0.0;
@@ -94,7 +116,7 @@ __device__ __forceinline__ void prep_query(
         const bool exact = all_int && !nonint && V <= 256.0 && Q <= 256.0 &&
                            D * delta * delta < lim && D * (V * V + Q * Q) < lim &&
                            2.0 * D * V * Q < lim;
-        if (!exact && qi < nq) {
+        if (!exact && qi < nq && mode != 2) {
             const double qn = sqrt((double)ss), xn = (double)xmax_norm;
             const double g = (D + 8.0) * ldexp(1.0, -24);
             const double big = (xn + qn) * (xn + qn);
@@ -130,6 +152,10 @@ struct Phase1Args {
     float *qnorm, *eps;
     float xmax_norm, vmin, vmax;
     int all_int, mode;
+    int8_t *qi8;          /* mode 2 */
+    float *qn8;
+    int centre;
+    uint32_t *status;
 };
 
 __global__ __launch_bounds__(256) void k_phase1(Phase1Args A) {
@@ -141,7 +167,7 @@ __global__ __launch_bounds__(256) void k_phase1(Phase1Args A) {
     const int qi = (int)(blockIdx.x - A.nbm) * 4 + (threadIdx.x >> 6);
     if (qi >= A.nqp) return;
     prep_query(qi, A.q, A.nq, A.dim, A.qbf, A.qnorm, A.eps, A.xmax_norm, A.vmin, A.vmax,
-               A.all_int, A.mode);
+               A.all_int, A.mode, A.qi8, A.qn8, A.centre, A.status);
 }
 
 /* ---------------- shared scan pieces ---------------- */
@@ -156,6 +182,8 @@ __global__ __launch_bounds__(256) void k_phase1(Phase1Args A) {
  */
 struct ScanArgs {
     const uint16_t *vbf;
+    const int8_t *vi8;        /* INT8 scan: tiled x - centre */
+    const int8_t *qi8;        /* INT8 scan: [nqp][dim] q - centre */
     const float *vecs;
     const float *xnorm;
     uint64_t count, count_pad;
@@ -190,7 +218,8 @@ struct ScanArgs {
  * kernel alone can be re-timed (fv_gpu_batch_time_scan) */
 struct ScanLaunch {
     ScanArgs M;
-    bool bf, inl;
+    bool bf, inl, i8;
+    int pt;                 /* INT8: 32-row tiles per 8 KB unit */
     int qtw, nw, nqt, gy;
     uint32_t gx;
     size_t shm;
@@ -805,6 +834,119 @@ void k_scan_bf16_mq(ScanArgs A) {
             if (qon[k]) flush_candidates(A, qb0[k], Cs[k]);
 }
 
+/*
+ * Exact INT8 scan (SQ8 indexes over integer data; dim 128 or 256). Rows and
+ * queries are shifted by the index centre into int8; x.q accumulates in
+ * int32 with v_mfma_i32_32x32x32_i8 (exact), and with ||x - c||^2 and
+ * ||q - c||^2 (integers below 2^24) the epilogue sees the exact distance,
+ * so eps = 0 and the select stage certifies as for integer BF16 data. The
+ * copy is half the bytes of BF16 for the same rows. Layout: 32-row tiles,
+ * k-steps of 32 dims as 1 KB chunks (slot l: row l & 31, dims 32 s + 16
+ * (l >> 5) .. +16); a unit is 8 chunks = 8 KB = PT tiles (PT = 256 / dim),
+ * the cursor's logical tile. Same pipeline as k_scan_bf16 otherwise.
+ */
+template <bool PREPASS, int NW, int PT, bool INL>
+__global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(2)))
+void k_scan_i8(ScanArgs A) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    constexpr int KS = 8 / PT;               /* k-steps of 32 dims per tile */
+    u32x4 *qsh = (u32x4 *)smem;              /* [2][KS][64] x 16 B */
+    CandLds &C = *(CandLds *)(smem + (size_t)2 * KS * 64 * 16);
+
+    const int tid = threadIdx.x, l = tid & 63, wave = tid >> 6;
+    const int qbase = blockIdx.y * kQT;
+    if (tile_skipped(A, qbase)) return;
+
+    __shared__ uint32_t s_next;
+    UnitCursor cur = cursor_init<NW>(A, wave, &s_next);
+    const u32x4 *vb = (const u32x4 *)A.vi8;
+    /* physical unit of a logical tile (units past the range load a valid one) */
+    auto phys = [&](uint32_t t) { return min(t, A.ntiles - 1) * A.stride; };
+    auto load_rows = [&](u32x4 (&a)[8]) {
+        const u32x4 *p = vb + (uint64_t)phys(cur.tile) * 512 + l;
+#pragma unroll
+        for (int j = 0; j < 8; j++) a[j] = p[64 * j];
+    };
+    u32x4 a0[8], a1[8];
+    load_rows(a0);
+
+    for (int e = tid; e < 2 * KS * 64; e += blockDim.x) {
+        const int lane = e & 63, s = (e >> 6) % KS, qb = (e >> 6) / KS;
+        const int q = qbase + (lane & 31) + 32 * qb;
+        qsh[e] = *(const u32x4 *)(A.qi8 + (uint64_t)q * A.dim + 32 * s + 16 * (lane >> 5));
+    }
+    __shared__ int s_im[INL ? kQT : 1];
+    __shared__ double s_iv[INL ? kQT : 1];
+    if (INL) stage_inline(A, 0, qbase, s_im, s_iv);
+    if (!PREPASS) cand_init(C);
+
+    float tq[2], qn[2];
+    const uint32_t *wr[2];
+    load_query_state(A, qbase, tq, qn, wr);
+    auto load_in = [&](TileIn (&in)[PT]) {
+        const uint32_t u = phys(cur.tile);
+#pragma unroll
+        for (int p = 0; p < PT; p++) {
+            load_tile_in<INL>(A, u * PT + p, wr, in[p]);
+            in[p].t = cur.tile;
+            in[p].live = cur.tile < cur.end;
+        }
+        cursor_next(cur, 1, &s_next);
+    };
+    TileIn in0[PT], in1[PT];
+    __syncthreads();
+    cursor_start(cur, &s_next);
+    load_in(in0);
+
+    auto do_unit = [&](const u32x4 (&a)[8], const TileIn (&in)[PT]) {
+        const uint32_t u = phys(in[0].t);
+#pragma unroll
+        for (int p = 0; p < PT; p++) {
+            i32x16 c0 = {}, c1 = {};
+#pragma unroll
+            for (int k = 0; k < KS; k++) {
+                const i32x4 av = __builtin_bit_cast(i32x4, a[p * KS + k]);
+                const i32x4 b0 = __builtin_bit_cast(i32x4, qsh[(0 * KS + k) * 64 + l]);
+                const i32x4 b1 = __builtin_bit_cast(i32x4, qsh[(1 * KS + k) * 64 + l]);
+                c0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, b0, c0, 0, 0, 0);
+                c1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, b1, c1, 0, 0, 0);
+            }
+            float acc[2][16];
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                acc[0][i] = (float)c0[i];
+                acc[1][i] = (float)c1[i];
+            }
+            /* group index of the pre-pass: sampled tile x PT + p */
+            const uint32_t tg = in[p].t * PT + p;
+            const uint32_t pt = u * PT + p;
+            float xn[16];
+            expand_norms(in[p].xr, xn);
+            if (INL) {
+                double cv[16];
+                expand_rows(in[p].rv, cv);
+                TileIn v = in[p];
+                v.w[0] = inline_word(cv, s_im, s_iv, 0, 0);
+                v.w[1] = inline_word(cv, s_im, s_iv, 0, 1);
+                tile_epilogue<PREPASS, true>(A, qbase, tg, pt, acc, v, xn, qn, tq, true, C);
+            } else {
+                tile_epilogue<PREPASS, true>(A, qbase, tg, pt, acc, in[p], xn, qn, tq, true, C);
+            }
+        }
+    };
+
+    while (in0[0].live) {
+        load_rows(a1);
+        load_in(in1);
+        do_unit(a0, in0);
+        if (!in1[0].live) break;
+        load_rows(a0);
+        load_in(in0);
+        do_unit(a1, in1);
+    }
+    if (!PREPASS) flush_candidates(A, qbase, C);
+}
+
 /* ---------------- 3/5. plain FP32 VALU scan ---------------- */
 
 /*
@@ -1079,7 +1221,7 @@ __global__ __launch_bounds__(256) void k_tau(const uint64_t *__restrict__ gmin,
     if (threadIdx.x == 0) {
         tau[q] = tv;
         clear_pcnt(pcnt, q);
-        status[q] = 0;
+        status[q] &= kStPrepFail;   /* written by this run's query prep */
     }
 }
 
@@ -1306,6 +1448,8 @@ void free_batch_buffers(FvGpuBatch *b) {
     b->d_y.release();
     b->d_route.release();
     b->d_efq.release();
+    b->d_qi8.release();
+    b->d_qn8.release();
     if (b->gexec) hipGraphExecDestroy(b->gexec);
     if (b->graph) hipGraphDestroy(b->graph);
     b->gexec = nullptr;
@@ -1459,6 +1603,20 @@ int stage_batch(FvGpuIndex *g, FvGpuBatch *b, int nq, const float *queries,
 
     PV_TRY(b->d_qbf.ensure((size_t)nqp * dim * 2));
     PV_TRY(b->d_qaux.ensure((size_t)nqp * 16));
+    /* INT8 scan eligibility: every query value an integer inside the
+     * index's INT8 window [centre - 128, centre + 127] */
+    b->i8_ok = false;
+    if (g->d_vi8 != nullptr) {
+        bool ok = true;
+        const float lo = (float)g->i8_center - 128.f, hi = (float)g->i8_center + 127.f;
+        for (size_t i = 0, n = (size_t)nq * dim; i < n && ok; i++) {
+            const float v = queries[i];
+            ok = v == rintf(v) && v >= lo && v <= hi;
+        }
+        b->i8_ok = ok;
+        PV_TRY(b->d_qi8.ensure((size_t)nqp * dim));
+        PV_TRY(b->d_qn8.ensure((size_t)nqp * 4));
+    }
     if (b->nprog > 0 && b->nprog <= kBmProgs)
         PV_TRY(b->d_bitmap.ensure((size_t)b->nprog * (g->count_pad / 32) * 4));
     b->out_stride = topb;
@@ -1474,9 +1632,13 @@ int stage_batch(FvGpuIndex *g, FvGpuBatch *b, int nq, const float *queries,
 }
 
 /* scan precision actually used for a request */
-int scan_precision(const FvGpuIndex *g, const FvGpuBruteOpts *o) {
+/* the scan a run of batch b takes: SQ8 = the exact INT8 scan (index holds
+ * the INT8 copy and every query of the batch fits its window), else the
+ * BF16 scan, else plain FP32 */
+int scan_precision(const FvGpuIndex *g, const FvGpuBatch *b, const FvGpuBruteOpts *o) {
     int p = (o && o->precision >= 0) ? o->precision : g->precision;
-    if (p == FV_GPU_PREC_SQ8) p = FV_GPU_PREC_BF16;
+    if (p == FV_GPU_PREC_SQ8 && !(g->d_vi8 != nullptr && b != nullptr && b->i8_ok))
+        p = FV_GPU_PREC_BF16;
     if (p == FV_GPU_PREC_BF16 && (g->d_vbf == nullptr || g->dim % 128 != 0 || g->dim > 1024))
         p = FV_GPU_PREC_FP32;
     return p;
@@ -1550,7 +1712,11 @@ hipError_t launch_phase1(FvGpuIndex *g, FvGpuBatch *b, int prec, bool bitmap, hi
         P.vmin = g->vmin;
         P.vmax = g->vmax;
         P.all_int = g->all_int ? 1 : 0;
-        P.mode = prec == FV_GPU_PREC_BF16 ? 1 : 0;
+        P.mode = prec == FV_GPU_PREC_SQ8 ? 2 : (prec == FV_GPU_PREC_BF16 ? 1 : 0);
+        P.qi8 = b->d_qi8.as<int8_t>();
+        P.qn8 = b->d_qn8.as<float>();
+        P.centre = g->i8_center;
+        P.status = qa.status;
         hipLaunchKernelGGL(k_phase1, dim3(P.nbm + (uint32_t)(P.nqp + 3) / 4), dim3(256), 0, s, P);
         return hipGetLastError();
     }
@@ -1564,6 +1730,11 @@ struct ScanShape {
     int qtw, nw, wpc, gy;
     uint32_t gx, stride;
 };
+
+/* 32-row tiles per scan unit: the INT8 scan's 8 KB unit spans 256 / dim */
+static inline uint32_t scan_unit_tiles(int prec, int dim) {
+    return prec == FV_GPU_PREC_SQ8 ? (uint32_t)(256 / dim) : 1u;
+}
 
 /* true when another batch stream of the index still has work queued: the
  * scan then shares the chip with those batches' kernels */
@@ -1580,12 +1751,13 @@ static ScanShape scan_shape(const FvGpuIndex *g, const FvGpuBatch *b, const FvGp
     ScanShape sh;
     const int dim = g->dim;
     const uint32_t B = b->topb;
-    const uint32_t ntiles = (uint32_t)(g->count_pad / 32);
-    const bool bf = prec == FV_GPU_PREC_BF16;
+    /* scan tiles: 32 rows, or for the INT8 scan its 8 KB units */
+    const uint32_t ntiles = (uint32_t)(g->count_pad / 32) / scan_unit_tiles(prec, dim);
+    const bool bf = prec == FV_GPU_PREC_BF16 || prec == FV_GPU_PREC_SQ8;
     /* query tiles per WG: several when many query tiles re-read the same
      * data (dim 128, BF16) */
     int qtw = (o && o->qtw > 0) ? o->qtw : (b->nqt >= 8 ? 4 : 1);
-    if (!bf || dim != 128 || b->nqt < 2) qtw = 1;
+    if (prec != FV_GPU_PREC_BF16 || dim != 128 || b->nqt < 2) qtw = 1;
     qtw = qtw >= 4 ? 4 : (qtw >= 2 ? 2 : 1);
     sh.qtw = qtw;
     sh.gy = (b->nqt + qtw - 1) / qtw;
@@ -1631,8 +1803,28 @@ static ScanShape scan_shape(const FvGpuIndex *g, const FvGpuBatch *b, const FvGp
     return sh;
 }
 
+/* INT8 scan launch: waves per WG (4 / 8), tiles per unit (dim 128: 2,
+ * dim 256: 1) and inline predicates pick the kernel instance */
+template <bool PREPASS>
+static void launch_scan_i8(int nw, int pt, bool inl, uint32_t gx, int nqt, size_t shm,
+                           hipStream_t s, const ScanArgs &A) {
+    const dim3 grid(gx, nqt);
+#define PV_I8(NW, PT, INL) \
+    hipLaunchKernelGGL((k_scan_i8<PREPASS, NW, PT, INL>), grid, dim3(NW * 64), shm, s, A)
+    if (nw == 8) {
+        if (pt == 2) { if (inl) PV_I8(8, 2, true); else PV_I8(8, 2, false); }
+        else { if (inl) PV_I8(8, 1, true); else PV_I8(8, 1, false); }
+    } else {
+        if (pt == 2) { if (inl) PV_I8(4, 2, true); else PV_I8(4, 2, false); }
+        else { if (inl) PV_I8(4, 1, true); else PV_I8(4, 1, false); }
+    }
+#undef PV_I8
+}
+
 static hipError_t launch_main_scan(const ScanLaunch &L, hipStream_t s) {
-    if (L.bf && L.qtw > 1) {
+    if (L.i8) {
+        launch_scan_i8<false>(L.nw, L.pt, L.inl, L.gx, L.nqt, L.shm, s, L.M);
+    } else if (L.bf && L.qtw > 1) {
         const hipError_t e = launch_scan_mq<false>(L.qtw, L.inl, L.gx, L.gy, s, L.M);
         if (e != hipSuccess) return e;
     } else if (L.bf) {
@@ -1651,7 +1843,7 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
     hipStream_t s = b->stream;
     g->ev_stream = s;
     b->out_stride = b->topb; /* a graph run may have widened it; d_out holds nq x topb */
-    const int prec = scan_precision(g, o);
+    const int prec = scan_precision(g, b, o);
     const int nq = b->nq, nqp = b->nqt * kQT, dim = g->dim;
     const uint32_t B = b->topb;
     QAux qa = qaux_of(b);
@@ -1665,7 +1857,8 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
     if (route == 0) {
         /* gather when every query enumerates a short span list whose
          * random-row reads cost less than one shared scan pass */
-        const double row_bytes = (double)dim * (prec == FV_GPU_PREC_BF16 ? 2.0 : 4.0);
+        const double row_bytes =
+            (double)dim * (prec == FV_GPU_PREC_BF16 ? 2.0 : (prec == FV_GPU_PREC_SQ8 ? 1.0 : 4.0));
         const double scan_bytes = (double)g->count * row_bytes;
         const double gather_bytes = (double)b->sum_total * dim * 4.0 * 2.0;
         route = (b->all_spans && b->max_total <= 65536 && gather_bytes < scan_bytes) ? 2 : 1;
@@ -1675,7 +1868,8 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
 
     /* inline predicates: one comparison per query evaluated in the scan
      * epilogue, no bitmap pass (BF16 scan, all predicates on one column) */
-    const bool inl = prec == FV_GPU_PREC_BF16 && o && o->inline_pred && b->inl_ok &&
+    const bool inl = (prec == FV_GPU_PREC_BF16 || prec == FV_GPU_PREC_SQ8) && o &&
+                     o->inline_pred && b->inl_ok &&
                      b->inl_col >= 0 && !have_phase1;
     if (route == 1 && !inl && b->nprog > kBmProgs)
         PV_FAIL("gpu: %d distinct predicates: at most %d without inline evaluation",
@@ -1728,17 +1922,19 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
     }
 
     /* ---- full-scan route ---- */
-    /* pre-pass samples every stride-th 32-row tile */
-    const uint32_t ntiles = (uint32_t)(g->count_pad / 32);
+    /* pre-pass samples every stride-th tile (INT8: 8 KB unit of PT tiles) */
+    const bool i8 = prec == FV_GPU_PREC_SQ8;
+    const int pt = (int)scan_unit_tiles(prec, dim);
+    const uint32_t ntiles = (uint32_t)(g->count_pad / 32) / (uint32_t)pt;
     const bool bf = prec == FV_GPU_PREC_BF16;
     const bool auto_shape = !(o && (o->waves || o->wg_per_cu));
     const ScanShape sh = scan_shape(g, b, o, prec, stride_override,
-                                    bf && auto_shape && other_batches_in_flight(g, b));
+                                    (bf || i8) && auto_shape && other_batches_in_flight(g, b));
     const int qtw = sh.qtw, gy = sh.gy, nw = sh.nw;
     const uint32_t gx = sh.gx, stride = sh.stride;
     const uint32_t stiles = (ntiles + stride - 1) / stride;
     b->stride = stride;
-    b->ngroups = stiles * 2;
+    b->ngroups = stiles * 2 * (uint32_t)pt;
     b->cap_pool = std::max<uint64_t>(16384, (uint64_t)8 * B * stride);
     if (stride_override == 1) b->cap_pool = std::max<uint64_t>(b->cap_pool, (uint64_t)64 * B + 4096);
     PV_TRY(b->d_gmin.ensure((size_t)nqp * b->ngroups * 8));
@@ -1771,6 +1967,14 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
     A.icol = inl ? g->d_cols + (uint64_t)b->inl_col * g->count : nullptr;
     A.sweep = (o && o->order == 2) ? 1u : 0u;
     A.xn2max = g->xmax_norm * g->xmax_norm;
+    if (i8) {
+        A.vi8 = g->d_vi8;
+        A.qi8 = b->d_qi8.as<int8_t>();
+        A.xnorm = g->d_norm8;
+        A.qnorm = b->d_qn8.as<float>();
+        A.xn2max = (float)dim * 16384.f;
+    }
+    const size_t shm_i8 = (size_t)2 * (8 / pt) * 64 * 16 + sizeof(CandLds);
 
     /* 4 waves per WG up to 256 dims (2 WGs per CU), 8 waves above (the
      * query operands then take most of the LDS) */
@@ -1788,12 +1992,13 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
         ScanArgs P = A;
         P.ntiles = stiles;
         P.stride = stride;
-        if (bf) {
+        if (bf || i8) {
             /* each pre-pass WG walks several sampled tiles (at most the
              * main scan's grid): staging 64 query operands per WG costs as
              * much as a few tiles */
             const uint32_t gxp = std::min<uint32_t>(gx, std::max<uint32_t>(1, (stiles + nw - 1) / nw));
-            if (qtw > 1) PV_TRY(launch_scan_mq<true>(qtw, inl, gxp, gy, s, P));
+            if (i8) launch_scan_i8<true>(nw, pt, inl, gxp, b->nqt, shm_i8, s, P);
+            else if (qtw > 1) PV_TRY(launch_scan_mq<true>(qtw, inl, gxp, gy, s, P));
             else launch_scan_bf16<true>(nw, inl, gxp, b->nqt, shm_bf, s, P);
         } else {
             const uint32_t gx = std::max<uint32_t>(1, (stiles + 3) / 4);
@@ -1814,12 +2019,14 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
         L.M.stride = 1;
         L.bf = bf;
         L.inl = inl;
+        L.i8 = i8;
+        L.pt = pt;
         L.qtw = qtw;
         L.nw = nw;
         L.nqt = b->nqt;
         L.gy = gy;
         L.gx = gx;
-        L.shm = shm_bf;
+        L.shm = i8 ? shm_i8 : shm_bf;
         PV_TRY(launch_main_scan(L, s));
         if (b->last_scan == nullptr) b->last_scan = new ScanLaunch();
         *(ScanLaunch *)b->last_scan = L;
@@ -1835,7 +2042,7 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
     PV_TRY(hipGetLastError());
     ev_record(g, 5);
     st.scan_rows = g->count;
-    st.scan_bytes = g->count * (uint64_t)dim * (bf ? 2 : 4) * (uint64_t)b->nqt;
+    st.scan_bytes = g->count * (uint64_t)dim * (i8 ? 1 : (bf ? 2 : 4)) * (uint64_t)b->nqt;
     return 0;
 }
 
@@ -2139,7 +2346,7 @@ static int tune_batch(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *base, 
     o0.sample_stride = 0;
     o0.order = 0;
     o0.waves = 0;
-    const int prec = scan_precision(g, &o0);
+    const int prec = scan_precision(g, b, &o0);
     reps = std::max(1, std::min(reps, 64));
     std::vector<uint32_t> st((size_t)b->nq);
     QAux qa = qaux_of(b);
@@ -2177,7 +2384,7 @@ static int tune_batch(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *base, 
     if (b->last_route != 1) return 0;   /* gather route: no scan shape to tune */
     const bool mq = prec == FV_GPU_PREC_BF16 && g->dim == 128 && b->nqt >= 2;
     const int qtws[3] = {1, 2, 4};
-    const bool wv = prec == FV_GPU_PREC_BF16 && g->dim <= 256;
+    const bool wv = (prec == FV_GPU_PREC_BF16 || prec == FV_GPU_PREC_SQ8) && g->dim <= 256;
     for (int qi = 0; qi < (mq ? 3 : 1); qi++)
         for (int waves = 4; waves <= ((qi == 0 && wv) ? 8 : 4); waves += 4)
         for (int wpc = 1; wpc <= 4; wpc++)
@@ -2232,7 +2439,8 @@ extern "C" int fv_gpu_batch_tune(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteO
 /* Main-scan kernel time of a batch: one full run (so tau and the pool
  * are set up), then the same scan kernel launched reps times back to back
  * between two events on the batch's stream, then one more full run so the
- * batch's outputs and pool state are those of an ordinary run. */
+ * batch's outputs and pool state are those of an ordinary run; returns
+ * when that run has finished. */
 extern "C" int fv_gpu_batch_time_scan(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *opts,
                                       int reps, double *us, char *errbuf, size_t errlen) {
     if (g == nullptr || b == nullptr || b->owner != g || us == nullptr || reps < 1)
@@ -2267,7 +2475,12 @@ extern "C" int fv_gpu_batch_time_scan(FvGpuIndex *g, FvGpuBatch *b, const FvGpuB
         float ms = 0.f;
         PV_TRY(hipEventElapsedTime(&ms, e0, e1));
         *us = 1e3 * (double)ms / reps;
-        return run_brute(g, b, opts, 0, errbuf, errlen);
+        /* restore an ordinary run's outputs and wait for it: a run left in
+         * flight would make the next batch's scan take the shared-chip
+         * shape */
+        if (run_brute(g, b, opts, 0, errbuf, errlen) != 0) return -1;
+        PV_TRY(hipStreamSynchronize(b->stream));
+        return 0;
     } catch (...) {
         snprintf(errbuf, errlen, "gpu: internal error");
         return -1;

@@ -107,6 +107,30 @@ __global__ __launch_bounds__(256) void k_derive_rows(
     }
 }
 
+/* INT8 copy for SQ8 indexes over integer data: x - centre in the
+ * MFMA-tiled int8 layout of v_mfma_i32_32x32x32_i8 — tiles of 32 rows,
+ * k-steps of 32 dims as 1 KB chunks, slot l (16 bytes) holding row
+ * (l & 31), dims 32 s + 16 (l >> 5) .. +16 — and ||x - centre||^2 (exact:
+ * an integer below 2^24). One wave per row. */
+__global__ __launch_bounds__(256) void k_derive_i8(const float *__restrict__ v, uint64_t count,
+                                                   uint64_t count_pad, int dim, int centre,
+                                                   int8_t *__restrict__ vi8,
+                                                   float *__restrict__ norm8) {
+    const int l = threadIdx.x & 63;
+    const uint64_t row = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= count_pad) return;
+    const uint64_t tbase = (row >> 5) * 32ull * dim;
+    float ss = 0.f;
+    for (int j = l; j < dim; j += 64) {
+        const float x = (row < count ? v[row * dim + j] : 0.f) - (float)centre;
+        const int s = j >> 5, g = (j >> 4) & 1;
+        vi8[tbase + (uint64_t)s * 1024 + ((row & 31) + 32 * g) * 16 + (j & 15)] = (int8_t)(int)x;
+        ss = fmaf(x, x, ss);
+    }
+    for (int o = 32; o; o >>= 1) ss += __shfl_xor(ss, o);
+    if (l == 0) norm8[row] = row < count ? ss : 0.f;
+}
+
 /* level-0 lists [cnt, neigh...] (stride l0_stride) -> padded rows of
  * nbr_stride ids, 0xFFFFFFFF after the count; ids >= count are dropped. */
 __global__ void k_pack_l0(const uint32_t *__restrict__ l0, uint32_t l0_stride,
@@ -153,6 +177,8 @@ extern "C" void fv_gpu_index_free(FvGpuIndex *g) {
     hipFree(g->d_vecs);
     hipFree(g->d_vbf);
     hipFree(g->d_norm);
+    hipFree(g->d_vi8);
+    hipFree(g->d_norm8);
     hipFree(g->d_cols);
     hipFree(g->d_prows);
     hipFree(g->d_nbr);
@@ -276,6 +302,19 @@ static FvGpuIndex *upload_impl(const float *vectors, const double *cols,
     g->vmax_abs = fmaxf(pmax, nmax);
     g->xmax_norm = sqrtf(n2max) * (1.0f + 1e-6f);
     g->all_int = hs[2] == 0;
+
+    /* SQ8: an exact INT8 copy when every value is an integer and the values
+     * (0 included) span at most 255, so x - centre fits int8; dot products
+     * of D <= 256 such values and the norms stay below 2^24 (exact in FP32) */
+    if (precision == FV_GPU_PREC_SQ8 && g->all_int && g->vmax - g->vmin <= 255.0f &&
+        (dim == 128 || dim == 256)) {
+        g->i8_center = (int)g->vmin + 128;
+        UP_TRY(hipMalloc(&g->d_vi8, count_pad * (size_t)dim));
+        UP_TRY(hipMalloc(&g->d_norm8, count_pad * 4));
+        hipLaunchKernelGGL(k_derive_i8, dim3((unsigned)((count_pad + 3) / 4)), dim3(256), 0, s,
+                           g->d_vecs, count, count_pad, dim, g->i8_center, g->d_vi8, g->d_norm8);
+        UP_TRY(hipGetLastError());
+    }
 
     if (ncols > 0) {
         UP_TRY(hipMalloc(&g->d_cols, (size_t)ncols * count * 8));

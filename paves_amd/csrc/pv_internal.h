@@ -60,6 +60,7 @@ static_assert(sizeof(InlPred) == 16, "InlPred layout");
 /* status bits per query */
 constexpr uint32_t kStOverflow = 1u;   /* candidate pool overflowed */
 constexpr uint32_t kStCertFail = 2u;   /* re-rank set exceeded capacity */
+constexpr uint32_t kStPrepFail = 4u;   /* query outside the INT8 window of an INT8 scan */
 
 /* predicate program op (postfix; see pv_pred.hip) */
 enum : int32_t { OP_CMP = 0, OP_IN = 1, OP_AND2 = 2, OP_OR2 = 3, OP_NOT = 4,
@@ -164,6 +165,9 @@ struct FvGpuBatch {
     bool warmed = false;    /* one eager run done: every buffer allocated */
     void *last_scan = nullptr; /* launch record of the last main scan (timing) */
     bool capturing = false; /* a hipGraph capture of this batch's run is open */
+    bool i8_ok = false;     /* every query is an integer within the INT8 window */
+    pv::DevBuf d_qi8;       /* [nqp][dim] int8 queries minus the index centre */
+    pv::DevBuf d_qn8;       /* [nqp] ||q - centre||^2 */
     bool query_exact = true;
     std::vector<uint64_t> total;   /* host copy of qd[q].total */
     std::vector<int32_t> nspans;
@@ -222,6 +226,11 @@ struct FvGpuIndex {
     uint32_t *d_prows = nullptr;
     uint32_t *d_nbr = nullptr;
     uint32_t *d_ones = nullptr;  /* all-ones bitmap row [count_pad/32] */
+    /* INT8 copy (SQ8 indexes over integer data spanning <= 255 values):
+     * x - i8_center in the MFMA-tiled int8 layout, and ||x - i8_center||^2 */
+    int8_t *d_vi8 = nullptr;
+    float *d_norm8 = nullptr;
+    int i8_center = 0;
 
     float xmax_norm = 0.f;   /* max ||x|| */
     float vmax_abs = 0.f;    /* max |x_i| */
@@ -302,7 +311,7 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
 hipError_t launch_phase1(FvGpuIndex *g, FvGpuBatch *b, int prec, bool bitmap, hipStream_t s,
                          uint32_t *progcnt);
 /* precision the brute scan runs in for these options (FP32 or BF16) */
-int scan_precision(const FvGpuIndex *g, const FvGpuBruteOpts *o);
+int scan_precision(const FvGpuIndex *g, const FvGpuBatch *b, const FvGpuBruteOpts *o);
 
 /* timing helpers (pv_index.hip) */
 void ev_record(FvGpuIndex *g, int i);

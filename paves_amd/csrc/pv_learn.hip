@@ -812,7 +812,7 @@ extern "C" int fv_gpu_batch_run_learned(FvGpuIndex *g, FvGpuBatch *b, FvGpuModel
         uint32_t *pc = need_model ? b->d_progcnt.as<uint32_t>() : nullptr;
         if (need_model && pc == nullptr) PV_FAIL("gpu: learned run: batch has no count buffer");
         /* bitmaps + query prep; the bitmap pass also counts passing rows */
-        PV_TRY(launch_phase1(g, b, scan_precision(g, &bo), true, s, pc));
+        PV_TRY(launch_phase1(g, b, scan_precision(g, b, &bo), true, s, pc));
         LearnArgs A = {};
         A.q = b->d_q;
         A.nq = nq;
