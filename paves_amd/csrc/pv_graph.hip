@@ -23,7 +23,7 @@
  *   - each iteration pops up to W frontier heads that pass the stop test,
  *     gathers their padded neighbour rows (one 128-byte line per node for
  *     M0 = 32), and computes exact FP32 distances in the CPU cursor's
- *     summation order, 16 lanes per node, 16 nodes per memory round trip;
+ *     summation order, 16 lanes per node, 32 nodes per memory round trip;
  *   - the predicate is a bit test in the batch's predicate bitmap.
  * With W = 1 and a frontier that never overflows, the search visits the
  * nodes the CPU cursor visits and returns the same list.
@@ -42,7 +42,8 @@ namespace pv {
 constexpr int kHashBits = 11;
 constexpr int kHashSize = 1 << kHashBits; /* 2048 slots */
 constexpr int kProbe = 16;
-constexpr int kMaxNew = 128;              /* W * nbr_stride <= 128 */
+constexpr int kMaxNew = 256;              /* W * nbr_stride <= 256 (W <= 8 at degree 32) */
+constexpr int kNS = kMaxNew / 64;         /* new-node slots per lane */
 
 struct GraphArgs {
     const float *vecs;
@@ -88,16 +89,27 @@ __device__ __forceinline__ int hash_insert(uint32_t *h, uint32_t node) {
     return 2;
 }
 
-/* ascending bitonic sort of 128 keys in LDS by one wave */
-__device__ __forceinline__ void wave_sort128(uint64_t *a) {
+/* ascending sort of the first m (<= kMaxNew) keys in LDS by one wave,
+ * padding with ~0 up to the next power of two >= 64: up to 64 keys in
+ * registers (shuffle bitonic, no barrier), else a bitonic pass over 128 or
+ * 256 entries in LDS. Entries m.. hold ~0 on entry. */
+__device__ __forceinline__ void wave_sort_new(uint64_t *a, int m) {
     const int l = threadIdx.x;
-    for (int k = 2; k <= 128; k <<= 1) {
+    if (m <= 64) {
+        uint64_t x = a[l];
+        x = wave_sort64(x);
+        a[l] = x;
+        __syncthreads();
+        return;
+    }
+    const int n = m <= 128 ? 128 : 256;
+    for (int k = 2; k <= n; k <<= 1) {
         for (int j = k >> 1; j > 0; j >>= 1) {
 #pragma unroll
-            for (int h = 0; h < 2; h++) {
+            for (int h = 0; h < kNS; h++) {
                 const int i = l + 64 * h;
                 const int ixj = i ^ j;
-                if (ixj > i) {
+                if (i < n && ixj > i) {
                     const bool up = (i & k) == 0;
                     const uint64_t x = a[i], y = a[ixj];
                     if ((x > y) == up) {
@@ -122,15 +134,15 @@ __device__ __forceinline__ int lower_rank(const uint64_t *a, int n, uint64_t key
 }
 
 /*
- * Merge: sorted A[skip .. n) with the m unsorted keys nw[0..m) (m <= 128,
- * slots m..127 must hold ~0) into A[0 .. min(cap, n-skip+m)), ascending.
+ * Merge: sorted A[skip .. n) with the m unsorted keys nw[0..m) (m <= kMaxNew,
+ * slots m..kMaxNew-1 must hold ~0) into A[0 .. min(cap, n-skip+m)), ascending.
  * Keys are unique. Returns the new count. A may hold up to cap (<= 1024)
  * entries; every lane reads its elements before any write.
  */
 __device__ int wave_merge(uint64_t *A, int n, int skip, uint64_t *nw, int m,
                           int cap) {
     const int l = threadIdx.x;
-    if (m > 0) wave_sort128(nw);
+    if (m > 0) wave_sort_new(nw, m);
     const int nold = n - skip;
     uint64_t ov[16];
     int op[16];
@@ -143,10 +155,10 @@ __device__ int wave_merge(uint64_t *A, int n, int skip, uint64_t *nw, int m,
             op[k] = i + lower_rank(nw, m, ov[k]);
         }
     }
-    uint64_t nv[2];
-    int np[2];
+    uint64_t nv[kNS];
+    int np[kNS];
 #pragma unroll
-    for (int k = 0; k < 2; k++) {
+    for (int k = 0; k < kNS; k++) {
         const int j = l + 64 * k;
         np[k] = -1;
         if (j < m) {
@@ -159,44 +171,47 @@ __device__ int wave_merge(uint64_t *A, int n, int skip, uint64_t *nw, int m,
     for (int k = 0; k < 16; k++)
         if (op[k] >= 0 && op[k] < cap) A[op[k]] = ov[k];
 #pragma unroll
-    for (int k = 0; k < 2; k++)
+    for (int k = 0; k < kNS; k++)
         if (np[k] >= 0 && np[k] < cap) A[np[k]] = nv[k];
     __syncthreads();
     return min(cap, nold + m);
 }
 
 /*
- * Exact CPU-order distances for up to 4 nodes per 16-lane group (16 nodes
- * per call): all row loads are issued before any arithmetic so one call
- * costs one memory round trip. The summation order is the one of
+ * Exact CPU-order distances for up to NPG nodes per 16-lane group (4 NPG
+ * nodes per call): all row loads are issued before any arithmetic so one
+ * call costs one memory round trip. The summation order is the one of
  * cpu_order_l2 (engine.cpp:43-63).
  */
-__device__ __forceinline__ void dist16(const GraphArgs &A, const float *qsh,
-                                       const uint32_t *nodes, int nn, int base,
-                                       float *outd) {
+template <int NPG>
+__device__ __forceinline__ void dist_nodes(const GraphArgs &A, const float *qsh,
+                                           const uint32_t *nodes, int nn, int base,
+                                           float *outd) {
     const int l = threadIdx.x, g = l >> 4, l16 = l & 15;
     const int D = A.dim;
-    float acc0[4] = {0.f, 0.f, 0.f, 0.f}, acc1[4] = {0.f, 0.f, 0.f, 0.f};
-    const float *rp[4];
-    bool ok[4];
+    float acc0[NPG], acc1[NPG];
+    const float *rp[NPG];
+    bool ok[NPG];
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
+    for (int k = 0; k < NPG; k++) {
         const int j = base + g + 4 * k;
         ok[k] = j < nn;
         rp[k] = A.vecs + (uint64_t)(ok[k] ? nodes[j] : 0) * D;
+        acc0[k] = 0.f;
+        acc1[k] = 0.f;
     }
     int i = 0;
     for (; i + 128 <= D; i += 128) {
-        float x[4][8];
+        float x[NPG][8];
 #pragma unroll
-        for (int k = 0; k < 4; k++)
+        for (int k = 0; k < NPG; k++)
 #pragma unroll
             for (int b = 0; b < 4; b++) {
                 x[k][2 * b] = rp[k][i + 32 * b + l16];
                 x[k][2 * b + 1] = rp[k][i + 32 * b + 16 + l16];
             }
 #pragma unroll
-        for (int k = 0; k < 4; k++)
+        for (int k = 0; k < NPG; k++)
 #pragma unroll
             for (int b = 0; b < 4; b++) {
                 const float d0 = __fsub_rn(x[k][2 * b], qsh[i + 32 * b + l16]);
@@ -207,7 +222,7 @@ __device__ __forceinline__ void dist16(const GraphArgs &A, const float *qsh,
     }
     for (; i + 32 <= D; i += 32) {
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
+        for (int k = 0; k < NPG; k++) {
             const float d0 = __fsub_rn(rp[k][i + l16], qsh[i + l16]);
             const float d1 = __fsub_rn(rp[k][i + 16 + l16], qsh[i + 16 + l16]);
             acc0[k] = __fmaf_rn(d0, d0, acc0[k]);
@@ -216,13 +231,13 @@ __device__ __forceinline__ void dist16(const GraphArgs &A, const float *qsh,
     }
     for (; i + 16 <= D; i += 16) {
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
+        for (int k = 0; k < NPG; k++) {
             const float d = __fsub_rn(rp[k][i + l16], qsh[i + l16]);
             acc0[k] = __fmaf_rn(d, d, acc0[k]);
         }
     }
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
+    for (int k = 0; k < NPG; k++) {
         float t = __fadd_rn(acc0[k], acc1[k]);
         t = __fadd_rn(t, __shfl_xor(t, 8, 16));
         t = __fadd_rn(t, __shfl_xor(t, 4, 16));
@@ -243,11 +258,11 @@ __global__ __launch_bounds__(64) void k_graph(GraphArgs A) {
     float *qsh = (float *)smem;                                   /* [D] */
     uint64_t *F = (uint64_t *)(smem + (((size_t)D * 4 + 15) & ~(size_t)15)); /* [fcap] */
     uint64_t *R = F + A.fcap;                                     /* [ef] */
-    uint64_t *nwF = R + A.ef;                                     /* [128] */
-    uint64_t *nwR = nwF + kMaxNew;                                /* [128] */
+    uint64_t *nwF = R + A.ef;                                     /* [kMaxNew] */
+    uint64_t *nwR = nwF + kMaxNew;                                /* [kMaxNew] */
     uint32_t *H = (uint32_t *)(nwR + kMaxNew);                    /* [2048] */
-    uint32_t *nb = H + kHashSize;                                 /* [128] new nodes */
-    float *nd = (float *)(nb + kMaxNew);                          /* [128] */
+    uint32_t *nb = H + kHashSize;                                 /* [kMaxNew] new nodes */
+    float *nd = (float *)(nb + kMaxNew);                          /* [kMaxNew] */
     uint32_t *pops = (uint32_t *)(nd + kMaxNew);                  /* [8] */
 
     const FvGpuQueryDesc qd = A.qd[q];
@@ -264,7 +279,7 @@ __global__ __launch_bounds__(64) void k_graph(GraphArgs A) {
     /* entry point: distance by lanes 0..15 (others compute a masked copy) */
     nb[0] = entry;
     __syncthreads();
-    dist16(A, qsh, nb, 1, 0, nd);
+    dist_nodes<4>(A, qsh, nb, 1, 0, nd);
     if (l == 0) hash_insert(H, entry);
     __syncthreads();
     int nF = 1, nR = 0;
@@ -307,12 +322,12 @@ __global__ __launch_bounds__(64) void k_graph(GraphArgs A) {
         if (l < w) pops[l] = key_row(F[l]);
         __syncthreads();
 
-        /* gather neighbours of the popped nodes: w * nbs <= 128 slots */
+        /* gather neighbours of the popped nodes: w * nbs <= kMaxNew slots */
         const int nslot = w * nbs;
-        int isnew[2];
-        uint32_t cand[2];
+        int isnew[kNS];
+        uint32_t cand[kNS];
 #pragma unroll
-        for (int h = 0; h < 2; h++) {
+        for (int h = 0; h < kNS; h++) {
             const int s = l + 64 * h;
             cand[h] = kNoNode;
             isnew[h] = 0;
@@ -321,7 +336,7 @@ __global__ __launch_bounds__(64) void k_graph(GraphArgs A) {
         /* visited test-and-set */
         bool need_vb = false;
 #pragma unroll
-        for (int h = 0; h < 2; h++) {
+        for (int h = 0; h < kNS; h++) {
             if (cand[h] == kNoNode) continue;
             const int r = hash_insert(H, cand[h]);
             if (r == 1) isnew[h] = 1;
@@ -341,65 +356,81 @@ __global__ __launch_bounds__(64) void k_graph(GraphArgs A) {
                 __syncthreads();
                 vb_ready = true;
             }
+            /* all test-and-sets in flight together */
+            uint32_t old[kNS];
 #pragma unroll
-            for (int h = 0; h < 2; h++) {
-                if (isnew[h] != 2) continue;
-                const uint32_t node = cand[h];
-                const uint32_t m = 1u << (node & 31);
-                const uint32_t old = atomicOr(&vb[node >> 5], m);
-                isnew[h] = (old & m) ? 0 : 1;
-                ovf++;
-            }
+            for (int h = 0; h < kNS; h++)
+                if (isnew[h] == 2)
+                    old[h] = atomicOr(&vb[cand[h] >> 5], 1u << (cand[h] & 31));
+#pragma unroll
+            for (int h = 0; h < kNS; h++)
+                if (isnew[h] == 2) {
+                    isnew[h] = (old[h] & (1u << (cand[h] & 31))) ? 0 : 1;
+                    ovf++;
+                }
         }
         /* compact new nodes */
-        const uint64_t b0 = __ballot(isnew[0] == 1), b1 = __ballot(isnew[1] == 1);
         const uint64_t lt = (l == 0) ? 0ull : (~0ull >> (64 - l));
-        const int n0 = __popcll(b0);
-        if (isnew[0] == 1) nb[__popcll(b0 & lt)] = cand[0];
-        if (isnew[1] == 1) nb[n0 + __popcll(b1 & lt)] = cand[1];
-        const int nn = n0 + __popcll(b1);
+        int nn = 0;
+#pragma unroll
+        for (int h = 0; h < kNS; h++) {
+            const uint64_t bb = __ballot(isnew[h] == 1);
+            if (isnew[h] == 1) nb[nn + __popcll(bb & lt)] = cand[h];
+            nn += __popcll(bb);
+        }
         __syncthreads();
         visited += (uint64_t)nn;
 
-        /* distances: 16 nodes per round trip */
-        for (int base = 0; base < nn; base += 16) dist16(A, qsh, nb, nn, base, nd);
+        /* predicate words of the new nodes, requested with their rows (the
+         * admission below reads them after the distances) */
+        uint32_t pw[kNS];
+#pragma unroll
+        for (int h = 0; h < kNS; h++) {
+            const int j = l + 64 * h;
+            pw[h] = ~0u;
+            if (bm != nullptr && j < nn) pw[h] = bm[nb[j] >> 5];
+        }
+        /* distances: 32 nodes per round trip (16 for the last few) */
+        int base = 0;
+        for (; base + 16 < nn; base += 32) dist_nodes<8>(A, qsh, nb, nn, base, nd);
+        if (base < nn) dist_nodes<4>(A, qsh, nb, nn, base, nd);
         __syncthreads();
 
         /* admission against the worst result at iteration start */
         const float wd = key_dist(worst);
         int mF = 0, mR = 0;
         {
-            bool adm[2], pas[2];
-            uint64_t key[2];
+            bool adm[kNS], pas[kNS];
+            uint64_t key[kNS];
 #pragma unroll
-            for (int h = 0; h < 2; h++) {
+            for (int h = 0; h < kNS; h++) {
                 const int j = l + 64 * h;
                 adm[h] = false;
                 pas[h] = false;
+                key[h] = ~0ull;
                 if (j < nn) {
                     const float d = nd[j];
                     const uint32_t node = nb[j];
                     key[h] = make_key(d, node);
                     adm[h] = !full || d < wd;
-                    if (adm[h])
-                        pas[h] = bm == nullptr || ((bm[node >> 5] >> (node & 31)) & 1u);
+                    if (adm[h]) pas[h] = ((pw[h] >> (node & 31)) & 1u) != 0u;
                 }
             }
-            const uint64_t a0 = __ballot(adm[0]), a1 = __ballot(adm[1]);
-            const uint64_t p0 = __ballot(pas[0]), p1 = __ballot(pas[1]);
-            const int na0 = __popcll(a0), np0 = __popcll(p0);
             /* fill with ~0 first, then scatter */
-            nwF[l] = ~0ull;
-            nwF[l + 64] = ~0ull;
-            nwR[l] = ~0ull;
-            nwR[l + 64] = ~0ull;
+#pragma unroll
+            for (int h = 0; h < kNS; h++) {
+                nwF[l + 64 * h] = ~0ull;
+                nwR[l + 64 * h] = ~0ull;
+            }
             __syncthreads();
-            if (adm[0]) nwF[__popcll(a0 & lt)] = key[0];
-            if (adm[1]) nwF[na0 + __popcll(a1 & lt)] = key[1];
-            if (pas[0]) nwR[__popcll(p0 & lt)] = key[0];
-            if (pas[1]) nwR[np0 + __popcll(p1 & lt)] = key[1];
-            mF = na0 + __popcll(a1);
-            mR = np0 + __popcll(p1);
+#pragma unroll
+            for (int h = 0; h < kNS; h++) {
+                const uint64_t ab = __ballot(adm[h]), pb = __ballot(pas[h]);
+                if (adm[h]) nwF[mF + __popcll(ab & lt)] = key[h];
+                if (pas[h]) nwR[mR + __popcll(pb & lt)] = key[h];
+                mF += __popcll(ab);
+                mR += __popcll(pb);
+            }
             __syncthreads();
         }
         nF = wave_merge(F, nF, w, nwF, mF, A.fcap);
@@ -498,7 +529,7 @@ int run_graph_opts(FvGpuIndex *g, FvGpuBatch *b, uint32_t ef, uint64_t max_visit
 
 int run_graph(FvGpuIndex *g, FvGpuBatch *b, uint32_t ef, uint64_t max_visited,
               char *errbuf, size_t errlen) {
-    return run_graph_opts(g, b, ef, max_visited, 4, 512, nullptr, errbuf, errlen);
+    return run_graph_opts(g, b, ef, max_visited, 8, 512, nullptr, errbuf, errlen);
 }
 
 } // namespace pv
@@ -576,7 +607,7 @@ extern "C" int fv_gpu_graph_batch(FvGpuIndex *g, int nq, const float *queries,
                                   uint8_t *exhausted, char *errbuf, size_t errlen) {
     try {
         return graph_sync(g, nq, queries, qd, nodes, nnodes, children, nchildren, invals,
-                          ninvals, ef, max_visited, 4, 512, out, out_cnt, exhausted,
+                          ninvals, ef, max_visited, 8, 512, out, out_cnt, exhausted,
                           nullptr, errbuf, errlen);
     } catch (...) {
         snprintf(errbuf, errlen, "gpu: internal error");
