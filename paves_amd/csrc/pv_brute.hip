@@ -156,6 +156,12 @@ struct Phase1Args {
     float *qn8;
     int centre;
     uint32_t *status;
+    /* learned routing (dim <= 256): router features of each query except
+     * its selectivity, [nqp][nfeat] — the query, then (slot dim left for the
+     * selectivity) its kNear nearest-centroid distances */
+    const float *cent_t;  /* centroids transposed [dim][ncent] */
+    int ncent, nfeat;
+    float *feat;
 };
 
 __global__ __launch_bounds__(256) void k_phase1(Phase1Args A) {
@@ -164,10 +170,36 @@ __global__ __launch_bounds__(256) void k_phase1(Phase1Args A) {
                     A.ncols, A.words, A.bitmap, A.progcnt);
         return;
     }
-    const int qi = (int)(blockIdx.x - A.nbm) * 4 + (threadIdx.x >> 6);
+    const int w = threadIdx.x >> 6;
+    const int qi = (int)(blockIdx.x - A.nbm) * 4 + w;
     if (qi >= A.nqp) return;
     prep_query(qi, A.q, A.nq, A.dim, A.qbf, A.qnorm, A.eps, A.xmax_norm, A.vmin, A.vmax,
                A.all_int, A.mode, A.qi8, A.qn8, A.centre, A.status);
+    if (A.feat != nullptr && qi < A.nq) {
+        /* router features while the bitmap blocks run (they only add the
+         * selectivity, which needs the finished bitmap counts) */
+        __shared__ float xq[4][256];
+        const int l = threadIdx.x & 63;
+        const float *qv = A.q + (uint64_t)qi * A.dim;
+        float *fo = A.feat + (uint64_t)qi * A.nfeat;
+        for (int j = l; j < A.dim; j += 64) {
+            xq[w][j] = qv[j];
+            fo[j] = qv[j];
+        }
+        wave_lds_sync();
+        float best[kNear];
+#pragma unroll
+        for (int k = 0; k < kNear; k++) best[k] = __uint_as_float(0x7f800000u);
+        for (int c0 = 0; c0 < A.ncent; c0 += 64) {
+            const int c = min(c0 + l, A.ncent - 1);
+            const float v = cent_dist(xq[w], A.cent_t, c, A.dim, A.ncent);
+            near_insert(best, c0 + l < A.ncent ? v : __uint_as_float(0x7f800000u));
+        }
+        __shared__ float wn[4][kNear];
+        wave_nearest(best, wn[w]);
+        wave_lds_sync();
+        if (l < kNear) fo[A.dim + 1 + l] = sqrtf(wn[w][l]);
+    }
 }
 
 /* ---------------- shared scan pieces ---------------- */
@@ -1680,7 +1712,7 @@ static hipError_t launch_scan_mq(int qtw, bool inl, uint32_t gx, int gy, hipStre
 /* Phase 1 of a run: predicate bitmaps (when `bitmap`) and the query
  * preparation for scan precision `prec`, one launch on stream s. */
 hipError_t launch_phase1(FvGpuIndex *g, FvGpuBatch *b, int prec, bool bitmap, hipStream_t s,
-                         uint32_t *progcnt) {
+                         uint32_t *progcnt, const PrepFeatures *pf) {
     QAux qa = qaux_of(b);
     {
         /* predicate programs -> row bitmaps, every run (the gather route
@@ -1717,6 +1749,12 @@ hipError_t launch_phase1(FvGpuIndex *g, FvGpuBatch *b, int prec, bool bitmap, hi
         P.qn8 = b->d_qn8.as<float>();
         P.centre = g->i8_center;
         P.status = qa.status;
+        if (pf != nullptr && g->dim <= 256) {
+            P.cent_t = pf->cent_t;
+            P.ncent = pf->ncent;
+            P.nfeat = pf->nfeat;
+            P.feat = pf->feat;
+        }
         hipLaunchKernelGGL(k_phase1, dim3(P.nbm + (uint32_t)(P.nqp + 3) / 4), dim3(256), 0, s, P);
         return hipGetLastError();
     }

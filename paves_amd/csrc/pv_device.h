@@ -368,6 +368,76 @@ __device__ __forceinline__ uint64_t wave_sort64(uint64_t x) {
     return x;
 }
 
+/* ---- learned-router features (shared by the query prep of a run and by
+ * the feature / decision kernels, so every path computes the same values) */
+
+constexpr int kNear = 16;   /* nearest-centroid distance features */
+
+/* make a wave's LDS writes visible to its own lanes */
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+/* squared distance of the query in xs (LDS) to centroid c of the
+ * transposed centroid array [dim][ncent]: 32 coordinates per step, all
+ * loads issued before the arithmetic (L2-latency bound), four partial sums.
+ * The one summation order of every feature path (training and inference). */
+__device__ __forceinline__ float cent_dist(const float *xs, const float *__restrict__ cent_t,
+                                           int c, int dim, int ncent) {
+    const float *ck = cent_t + c;
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+    int k = 0;
+    for (; k + 32 <= dim; k += 32) {
+        float v[32];
+#pragma unroll
+        for (int j = 0; j < 32; j++) v[j] = ck[(uint64_t)(k + j) * ncent];
+#pragma unroll
+        for (int j = 0; j < 32; j += 4) {
+            const float d0 = xs[k + j] - v[j], d1 = xs[k + j + 1] - v[j + 1];
+            const float d2 = xs[k + j + 2] - v[j + 2], d3 = xs[k + j + 3] - v[j + 3];
+            s0 = fmaf(d0, d0, s0);
+            s1 = fmaf(d1, d1, s1);
+            s2 = fmaf(d2, d2, s2);
+            s3 = fmaf(d3, d3, s3);
+        }
+    }
+    for (; k < dim; k++) {
+        const float d = xs[k] - ck[(uint64_t)k * ncent];
+        s0 = fmaf(d, d, s0);
+    }
+    return (s0 + s1) + (s2 + s3);
+}
+
+/* lane-sorted insertion of v into the lane's kNear smallest */
+__device__ __forceinline__ void near_insert(float (&best)[kNear], float v) {
+#pragma unroll
+    for (int k = 0; k < kNear; k++) {
+        const float lo = fminf(best[k], v), hi = fmaxf(best[k], v);
+        best[k] = lo;
+        v = hi;
+    }
+}
+
+/* the wave's kNear smallest of all lanes' lists, ascending, to out[0..kNear) */
+__device__ __forceinline__ void wave_nearest(const float (&best)[kNear], float *out) {
+    const int l = threadIdx.x & 63;
+    int head = 0;
+    for (int r = 0; r < kNear; r++) {
+        float h = __uint_as_float(0x7f800000u);
+#pragma unroll
+        for (int k = 0; k < kNear; k++)
+            if (k == head) h = best[k];
+        float m = h;
+        for (int off = 32; off; off >>= 1) m = fminf(m, __shfl_xor(m, off));
+        const uint64_t who = __ballot(h == m);
+        const int lane = __ffsll((unsigned long long)who) - 1;
+        if (l == lane) head++;
+        if (l == 0) out[r] = m;
+    }
+}
+
 } // namespace pv
 
 #endif

@@ -308,8 +308,14 @@ int run_graph_ex(FvGpuIndex *g, FvGpuBatch *b, uint32_t ef, uint64_t max_visited
 int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
                  uint32_t stride_override, const uint8_t *skip, bool have_phase1,
                  uint32_t *progcnt_reset, char *errbuf, size_t errlen);
+/* router features computed by the query prep of phase 1 (dim <= 256) */
+struct PrepFeatures {
+    const float *cent_t;
+    int ncent, nfeat;
+    float *feat;
+};
 hipError_t launch_phase1(FvGpuIndex *g, FvGpuBatch *b, int prec, bool bitmap, hipStream_t s,
-                         uint32_t *progcnt);
+                         uint32_t *progcnt, const PrepFeatures *pf = nullptr);
 /* precision the brute scan runs in for these options (FP32 or BF16) */
 int scan_precision(const FvGpuIndex *g, const FvGpuBatch *b, const FvGpuBruteOpts *o);
 

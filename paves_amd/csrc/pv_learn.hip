@@ -36,7 +36,6 @@
 namespace pv {
 
 constexpr int kHid = 64;
-constexpr int kNear = 16;
 constexpr int kMaxFeat = 1024 + 17;
 
 __host__ __device__ inline int model_nfeat(int dim) { return dim + 1 + kNear; }
@@ -63,11 +62,6 @@ __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint
 
 /* makes a wave's LDS writes visible to the wave's other lanes (no block
  * barrier: waves of a block work on different queries) */
-__device__ __forceinline__ void wave_lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 __device__ __forceinline__ float wave_sum(float v) {
     for (int o = 32; o; o >>= 1) v += __shfl_xor(v, o);
@@ -98,8 +92,13 @@ __global__ __launch_bounds__(256) void k_prog_count(const uint32_t *__restrict__
  * broadcast; each lane keeps its 16 best, then 16 rounds of wave minimum
  * merge the lanes' lists. Wave-local (no block barrier).
  */
+/*
+ * Features of one query by one wave into xs[0..F): the query, log10 of its
+ * predicate's selectivity, and the distances to its kNear nearest
+ * centroids, ascending (centroids transposed, [dim][ncent]).
+ */
 __device__ void wave_features(const float *__restrict__ qv, int dim, double pass, uint64_t count,
-                              const float *__restrict__ cent, int ncent, float *xs) {
+                              const float *__restrict__ cent_t, int ncent, float *xs) {
     const int l = threadIdx.x & 63;
     for (int j = l; j < dim; j += 64) xs[j] = qv[j];
     if (l == 0) xs[dim] = (float)log10(fmax(pass, 1.0) / (double)count);
@@ -107,48 +106,15 @@ __device__ void wave_features(const float *__restrict__ qv, int dim, double pass
     float best[kNear];
 #pragma unroll
     for (int k = 0; k < kNear; k++) best[k] = __uint_as_float(0x7f800000u);
-    for (int c = l; c < ncent; c += 64) {
-        const float *cr = cent + (uint64_t)c * dim;
-        float s = 0.f;
-        if ((dim & 3) == 0) {
-            const float4 *c4 = (const float4 *)cr;
-            const float4 *x4 = (const float4 *)xs;
-#pragma unroll 16
-            for (int k = 0; k < dim / 4; k++) {
-                const float4 v = c4[k], x = x4[k];
-                const float d0 = x.x - v.x, d1 = x.y - v.y, d2 = x.z - v.z, d3 = x.w - v.w;
-                s = fmaf(d0, d0, s);
-                s = fmaf(d1, d1, s);
-                s = fmaf(d2, d2, s);
-                s = fmaf(d3, d3, s);
-            }
-        } else {
-            for (int k = 0; k < dim; k++) {
-                const float d = xs[k] - cr[k];
-                s = fmaf(d, d, s);
-            }
-        }
-        float v = s;
-#pragma unroll
-        for (int k = 0; k < kNear; k++) {
-            const float lo = fminf(best[k], v), hi = fmaxf(best[k], v);
-            best[k] = lo;
-            v = hi;
-        }
+    for (int c0 = 0; c0 < ncent; c0 += 64) {
+        const int c = min(c0 + l, ncent - 1);
+        const float v = cent_dist(xs, cent_t, c, dim, ncent);
+        near_insert(best, c0 + l < ncent ? v : __uint_as_float(0x7f800000u));
     }
-    int head = 0;
-    for (int r = 0; r < kNear; r++) {
-        float h = __uint_as_float(0x7f800000u);
-#pragma unroll
-        for (int k = 0; k < kNear; k++)
-            if (k == head) h = best[k];
-        float m = h;
-        for (int off = 32; off; off >>= 1) m = fminf(m, __shfl_xor(m, off));
-        const uint64_t who = __ballot(h == m);
-        const int lane = __ffsll((unsigned long long)who) - 1;
-        if (l == lane) head++;
-        if (l == 0) xs[dim + 1 + r] = sqrtf(m);
-    }
+    __shared__ float wn[4][kNear];
+    wave_nearest(best, wn[(threadIdx.x >> 6) & 3]);
+    wave_lds_sync();
+    if (l < kNear) xs[dim + 1 + l] = sqrtf(wn[(threadIdx.x >> 6) & 3][l]);
     wave_lds_sync();
 }
 
@@ -391,9 +357,11 @@ __global__ void k_to_bf16(const float *__restrict__ w, uint16_t *__restrict__ o,
 }
 
 /*
- * The routed run's decision kernel, one wave per query: features (with the
- * passing-row counts the bitmap pass accumulated), MLP, and route/ef as in
- * k_route. Optionally stores the features and outputs.
+ * The routed run's decision kernel, one 4-wave block per query: the waves
+ * split the centroid distances (each keeps its kNear nearest), wave 0 merges
+ * the 4 x kNear candidates with a register sort, then runs the MLP and sets
+ * route/ef as in k_route. Features equal those of k_features (same
+ * distances, same nearest set). Optionally stores the features and outputs.
  */
 struct LearnArgs {
     const float *q;
@@ -413,7 +381,28 @@ struct LearnArgs {
     float *feat_out, *y_out;
 };
 
-__global__ __launch_bounds__(256) void k_learn_route(LearnArgs A) {
+/* route/ef from the model outputs (mode 0/1) or the fixed policy */
+__device__ __forceinline__ void route_decision(const LearnArgs &A, int qi, float y0, float y1) {
+    uint32_t ef = A.ef_fixed;
+    uint8_t r = 1;
+    if (A.mode == 3) {
+        r = 0;
+    } else if (A.mode == 0 || A.mode == 1) {
+        const float e = expf(fminf(y0, 20.f)) * A.scale;
+        ef = (uint32_t)fminf(ceilf(e), (float)A.ef_max);
+        r = (A.mode == 1 || y1 > A.bias) ? 1 : 0;
+    }
+    A.efq[qi] = min(max(ef, A.ef_min), A.ef_max);
+    A.route[qi] = r;
+}
+
+/*
+ * Decision kernel when the query prep of phase 1 already wrote each query's
+ * features except its selectivity into feat_out (dim <= 256): one wave per
+ * query, four per block — the selectivity from the bitmap pass's counts,
+ * then the MLP and route/ef.
+ */
+__global__ __launch_bounds__(256) void k_learn_route_pre(LearnArgs A) {
     __shared__ __attribute__((aligned(16))) uint16_t ws[kLdsParams];
     __shared__ __attribute__((aligned(16))) float xs[4][kMaxFeat];
     __shared__ float xh[4][kMaxFeat];
@@ -425,30 +414,75 @@ __global__ __launch_bounds__(256) void k_learn_route(LearnArgs A) {
     if (lds) stage_params(A.wt, np, ws);
     __syncthreads(); /* every wave reaches it (no early return above) */
     if (qi >= A.nq) return;
-    const int p = A.pofq[qi];
-    wave_features(A.q + (uint64_t)qi * A.dim, A.dim,
-                  p < 0 ? (double)A.count : (double)A.progcnt[p], A.count, A.cent, A.ncent,
-                  xs[w]);
+    float *fq = A.feat_out + (uint64_t)qi * A.F;
+    for (int j = l; j < A.F; j += 64) xs[w][j] = fq[j];
+    if (l == 0) {
+        const int p = A.pofq[qi];
+        const double pass = p < 0 ? (double)A.count : (double)A.progcnt[p];
+        const float sel = (float)log10(fmax(pass, 1.0) / (double)A.count);
+        xs[w][A.dim] = sel;
+        fq[A.dim] = sel;
+    }
+    wave_lds_sync();
     float y0, y1;
     wave_mlp(xs[w], A.F, lds ? ws : A.wt, A.mean, A.istd, xh[w], hs[w], y0, y1);
-    if (A.feat_out)
-        for (int j = l; j < A.F; j += 64) A.feat_out[(uint64_t)qi * A.F + j] = xs[w][j];
     if (l == 0) {
         if (A.y_out) {
             A.y_out[(uint64_t)qi * 2 + 0] = y0;
             A.y_out[(uint64_t)qi * 2 + 1] = y1;
         }
-        uint32_t ef = A.ef_fixed;
-        uint8_t r = 1;
-        if (A.mode == 3) {
-            r = 0;
-        } else if (A.mode == 0 || A.mode == 1) {
-            const float e = expf(fminf(y0, 20.f)) * A.scale;
-            ef = (uint32_t)fminf(ceilf(e), (float)A.ef_max);
-            r = (A.mode == 1 || y1 > A.bias) ? 1 : 0;
+        route_decision(A, qi, y0, y1);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_learn_route(LearnArgs A) {
+    __shared__ __attribute__((aligned(16))) uint16_t ws[kLdsParams];
+    __shared__ __attribute__((aligned(16))) float xs[kMaxFeat];
+    __shared__ float xh[kMaxFeat];
+    __shared__ float hs[kHid];
+    __shared__ float wn[4][kNear];
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int qi = blockIdx.x;
+    const int np = mlp_layout(A.F).n;
+    const bool lds = np <= kLdsParams;
+    if (lds) stage_params(A.wt, np, ws);
+    const float *qv = A.q + (uint64_t)qi * A.dim;
+    for (int j = threadIdx.x; j < A.dim; j += blockDim.x) xs[j] = qv[j];
+    if (threadIdx.x == 0) {
+        const int p = A.pofq[qi];
+        const double pass = p < 0 ? (double)A.count : (double)A.progcnt[p];
+        xs[A.dim] = (float)log10(fmax(pass, 1.0) / (double)A.count);
+    }
+    __syncthreads();
+    float best[kNear];
+#pragma unroll
+    for (int k = 0; k < kNear; k++) best[k] = __uint_as_float(0x7f800000u);
+    for (int c0 = 64 * w; c0 < A.ncent; c0 += 256) {
+        const int c = min(c0 + l, A.ncent - 1);
+        const float v = cent_dist(xs, A.cent, c, A.dim, A.ncent);
+        near_insert(best, c0 + l < A.ncent ? v : __uint_as_float(0x7f800000u));
+    }
+    wave_nearest(best, wn[w]);
+    __syncthreads();
+    if (w != 0) return; /* no block barrier below */
+    {
+        /* 4 x kNear candidates -> the kNear smallest, ascending (distances
+         * are non-negative, so float bits order as the values) */
+        const float cv = wn[l >> 4][l & 15];
+        const uint64_t sk = wave_sort64((uint64_t)__float_as_uint(cv));
+        if (l < kNear) xs[A.dim + 1 + l] = sqrtf(__uint_as_float((uint32_t)sk));
+        wave_lds_sync();
+    }
+    float y0, y1;
+    wave_mlp(xs, A.F, lds ? ws : A.wt, A.mean, A.istd, xh, hs, y0, y1);
+    if (A.feat_out)
+        for (int j = l; j < A.F; j += 64) A.feat_out[(uint64_t)qi * A.F + j] = xs[j];
+    if (l == 0) {
+        if (A.y_out) {
+            A.y_out[(uint64_t)qi * 2 + 0] = y0;
+            A.y_out[(uint64_t)qi * 2 + 1] = y1;
         }
-        A.efq[qi] = min(max(ef, A.ef_min), A.ef_max);
-        A.route[qi] = r;
+        route_decision(A, qi, y0, y1);
     }
 }
 
@@ -567,8 +601,15 @@ extern "C" FvGpuModel *fv_gpu_model_create(int device, int dim, const float *cen
     if ((e = hipMemset(m->d_wbf, 0, (np + 7) / 8 * 16)) != hipSuccess) return fail(e, "alloc");
     if ((e = hipMalloc(&m->d_mean, (size_t)m->F * 4)) != hipSuccess) return fail(e, "alloc");
     if ((e = hipMalloc(&m->d_istd, (size_t)m->F * 4)) != hipSuccess) return fail(e, "alloc");
-    if ((e = hipMemcpy(m->d_cent, centroids, (size_t)ncent * dim * 4, hipMemcpyHostToDevice)) != hipSuccess)
-        return fail(e, "copy");
+    {
+        /* centroids stored transposed, [dim][ncent]: a wave reads coordinate
+         * k of 64 consecutive centroids with one coalesced load */
+        std::vector<float> ct((size_t)ncent * dim);
+        for (int c = 0; c < ncent; c++)
+            for (int k = 0; k < dim; k++) ct[(size_t)k * ncent + c] = centroids[(size_t)c * dim + k];
+        if ((e = hipMemcpy(m->d_cent, ct.data(), ct.size() * 4, hipMemcpyHostToDevice)) != hipSuccess)
+            return fail(e, "copy");
+    }
     /* He-initialised weights from a fixed seed; zero biases; identity scaling */
     std::vector<float> w(np, 0.f);
     const MlpLayout L = mlp_layout(m->F);
@@ -811,8 +852,22 @@ extern "C" int fv_gpu_batch_run_learned(FvGpuIndex *g, FvGpuBatch *b, FvGpuModel
         PV_TRY(b->d_efq.ensure((size_t)nq * 4));
         uint32_t *pc = need_model ? b->d_progcnt.as<uint32_t>() : nullptr;
         if (need_model && pc == nullptr) PV_FAIL("gpu: learned run: batch has no count buffer");
-        /* bitmaps + query prep; the bitmap pass also counts passing rows */
-        PV_TRY(launch_phase1(g, b, scan_precision(g, b, &bo), true, s, pc));
+        /* bitmaps + query prep; the bitmap pass also counts passing rows,
+         * and up to 256 dims the query prep also computes the router
+         * features (all but the selectivity) */
+        const bool pre = need_model && g->dim <= 256;
+        if (need_model) {
+            PV_TRY(b->d_feat.ensure((size_t)nq * m->F * 4));
+            PV_TRY(b->d_y.ensure((size_t)nq * 8));
+        }
+        PrepFeatures pf = {};
+        if (pre) {
+            pf.cent_t = m->d_cent;
+            pf.ncent = m->ncent;
+            pf.nfeat = m->F;
+            pf.feat = b->d_feat.as<float>();
+        }
+        PV_TRY(launch_phase1(g, b, scan_precision(g, b, &bo), true, s, pc, pre ? &pf : nullptr));
         LearnArgs A = {};
         A.q = b->d_q;
         A.nq = nq;
@@ -829,8 +884,6 @@ extern "C" int fv_gpu_batch_run_learned(FvGpuIndex *g, FvGpuBatch *b, FvGpuModel
         A.route = b->d_route.as<uint8_t>();
         A.efq = b->d_efq.as<uint32_t>();
         if (need_model) {
-            PV_TRY(b->d_feat.ensure((size_t)nq * m->F * 4));
-            PV_TRY(b->d_y.ensure((size_t)nq * 8));
             A.F = m->F;
             A.cent = m->d_cent;
             A.ncent = m->ncent;
@@ -839,7 +892,10 @@ extern "C" int fv_gpu_batch_run_learned(FvGpuIndex *g, FvGpuBatch *b, FvGpuModel
             A.istd = m->d_istd;
             A.feat_out = b->d_feat.as<float>();
             A.y_out = b->d_y.as<float>();
-            hipLaunchKernelGGL(k_learn_route, dim3((nq + 3) / 4), dim3(256), 0, s, A);
+            if (pre)
+                hipLaunchKernelGGL(k_learn_route_pre, dim3((nq + 3) / 4), dim3(256), 0, s, A);
+            else
+                hipLaunchKernelGGL(k_learn_route, dim3(nq), dim3(256), 0, s, A);
         } else {
             hipLaunchKernelGGL(k_route, dim3((nq + 255) / 256), dim3(256), 0, s,
                                (const float *)nullptr, nq, o->mode, ef_fixed, ef_min, ef_max,
