@@ -7,15 +7,17 @@
  * trip inside a run):
  *
  *   1. k_phase1         predicate programs -> row bitmaps (bitmap_rows),
- *                       and in the same launch fp32 queries -> bf16 copy,
- *                       ||q||^2, per-query error bound eps
+ *                       and in the same launch fp32 queries -> bf16 (or
+ *                       int8) copy, ||q||^2, per-query error bound eps, and
+ *                       for routed runs the router features
  *   2. k_scan<PREPASS>  distances of every stride-th 32-row tile; per
  *                       16-row group the minimum passing distance
  *   3. k_tau            tau[q] = B-th smallest group minimum (+2 eps): at
  *                       least B passing rows lie at or below it, so every
  *                       row of the exact top-B does too
  *   4. k_scan<MAIN>     full scan; rows with dist <= tau enter a per-query
- *                       candidate pool (LDS, flushed to HBM per WG)
+ *                       candidate pool (LDS, flushed to HBM per WG into one
+ *                       of the query's pool segments)
  *   5. k_select         B-th smallest pool key (wave-sorted bound + 64-lane
  *                       register sort for B <= 64, radix select otherwise),
  *                       collect the keys within 2 eps of it, re-rank them
@@ -23,12 +25,13 @@
  *                       sort by (dist, row), emit FvHit + count + truncated
  *
  * Distances in the scans: BF16 MFMA (v_mfma_f32_32x32x16_bf16) computing
- * x.q with the ||x||^2 + ||q||^2 - 2 x.q epilogue, or plain FP32 VALU
- * sum (x - q)^2. Both are approximations of the reference l2sq value with a
- * per-query bound eps (0 when every value is an integer small enough that
- * all arithmetic is exact); the re-rank restores the exact FP32 value, so
- * ids and distances equal the CPU cursor's (BruteCursor, engine.cpp:1050-
- * 1130) whatever scan precision is used.
+ * x.q with the ||x||^2 + ||q||^2 - 2 x.q epilogue, INT8 MFMA
+ * (v_mfma_i32_32x32x32_i8) on x - c and q - c for integer data (exact), or
+ * plain FP32 VALU sum (x - q)^2. Each approximates the reference l2sq value
+ * within a per-query bound eps (0 when every value is an integer small
+ * enough that all arithmetic is exact); the re-rank restores the exact FP32
+ * value, so ids and distances equal the CPU cursor's (BruteCursor,
+ * engine.cpp:1050-1130) whatever scan precision is used.
  *
  * Span-gather route: for batches whose driver spans (engine.cpp:464-599)
  * enumerate few rows, k_gather computes exact distances for the enumerated
