@@ -611,11 +611,13 @@ def uneven_path(args):
     t = time.perf_counter()
     best, best_us, ntried = g.tune(bh, base, reps=3)
     tune_s = time.perf_counter() - t
-    shape = {"qtw": best.qtw, "wg_per_cu": best.wg_per_cu, "sample_stride": best.sample_stride}
+    shape = {"qtw": best.qtw, "waves": best.waves, "wg_per_cu": best.wg_per_cu,
+             "order": {1: "chunked", 2: "sweep", 3: "query-major"}.get(best.order, "auto"),
+             "sample_stride": best.sample_stride}
     log(f"autotuner: {ntried} shapes in {tune_s:.1f}s, best {shape} at {best_us:.0f} us")
     opts = L.BruteOpts(precision=best.precision, route=best.route, inline_pred=best.inline_pred,
-                       wg_per_cu=best.wg_per_cu, qtw=best.qtw,
-                       sample_stride=best.sample_stride, graph=1)
+                       wg_per_cu=best.wg_per_cu, qtw=best.qtw, order=best.order,
+                       waves=best.waves, sample_stride=best.sample_stride, graph=1)
     for _ in range(max(2, args.warmup)):      # eager run, then capture
         g.run_brute(bh, opts)
     g.sync()
@@ -630,7 +632,8 @@ def uneven_path(args):
 
     # per-kernel times of one eager run with the same shape (HIP events)
     eager = L.BruteOpts(precision=best.precision, route=best.route, inline_pred=best.inline_pred,
-                        wg_per_cu=best.wg_per_cu, qtw=best.qtw, sample_stride=best.sample_stride)
+                        wg_per_cu=best.wg_per_cu, qtw=best.qtw, order=best.order,
+                        waves=best.waves, sample_stride=best.sample_stride)
     g.set_timing(True)
     scans = []
     for _ in range(3):

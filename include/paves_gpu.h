@@ -179,9 +179,12 @@ typedef struct FvGpuBruteOpts {
                             0 = auto; > 1 needs dim 128 and the BF16 scan) */
     int32_t graph;       /* 1: replay the run as a hipGraph captured for this
                             batch and these options (captured on first use) */
-    int32_t order;       /* BF16 scan tile order: 0 = auto (chunked), 1 = each
-                            workgroup streams one contiguous chunk, 2 = all
-                            waves sweep the rows together */
+    int32_t order;       /* BF16 / INT8 scan tile order: 0 = auto (3 above
+                            256 dims with several query tiles, else 1),
+                            1 = each workgroup streams one contiguous chunk,
+                            2 = all waves sweep the rows together, 3 = chunked
+                            with a query-major grid (the query tiles reading
+                            one chunk run together and share its reads) */
     int32_t waves;       /* waves per BF16 scan workgroup with one query tile
                             (4 or 8, 4 only up to 256 dims; 0 = auto: 8) */
 } FvGpuBruteOpts;

@@ -244,10 +244,26 @@ struct ScanArgs {
     const InlPred *inl;       /* inline predicates [nqp] (INL kernels) */
     const double *icol;       /* their column [count] */
     float xn2max;             /* max ||x||^2 over the rows (candidate-test margin) */
+    uint32_t qmajor;          /* grid (query tiles, row chunks): the query tiles
+                                 reading one chunk are dispatched together */
     uint32_t sweep;           /* tile order: 0 = each WG streams its own contiguous
                                  chunk; 1 = all waves sweep the rows together
                                  (wave g takes tiles g, g + G, g + 2G, ...) */
 };
+
+/* row chunk (WG of the row dimension), chunk count and query-tile group of
+ * a scan WG: grid (chunks, query tiles), or (query tiles, chunks) when
+ * qmajor so that the query tiles sharing a chunk run together and read it
+ * once from HBM (later reads hit in L2 / the last-level cache) */
+__device__ __forceinline__ uint32_t scan_chunk(const ScanArgs &A) {
+    return A.qmajor ? blockIdx.y : blockIdx.x;
+}
+__device__ __forceinline__ uint32_t scan_nchunks(const ScanArgs &A) {
+    return A.qmajor ? gridDim.y : gridDim.x;
+}
+__device__ __forceinline__ uint32_t scan_qgroup(const ScanArgs &A) {
+    return A.qmajor ? blockIdx.x : blockIdx.y;
+}
 
 /* the main-scan launch of a batch's last full-scan run, kept so the scan
  * kernel alone can be re-timed (fv_gpu_batch_time_scan) */
@@ -282,17 +298,17 @@ template <int NW>
 __device__ __forceinline__ UnitCursor cursor_init(const ScanArgs &A, int wave,
                                                   uint32_t *s_next) {
     UnitCursor u;
-    const uint32_t nwg = gridDim.x;
+    const uint32_t nwg = scan_nchunks(A), wg = scan_chunk(A);
     u.c = 0;
     u.pend = 0;
     if (A.sweep) {
-        u.tile = blockIdx.x * (uint32_t)NW + (uint32_t)wave;
+        u.tile = wg * (uint32_t)NW + (uint32_t)wave;
         u.step = nwg * (uint32_t)NW;
         u.nxt = u.tile + u.step;
         u.end = A.ntiles;
     } else {
-        const uint32_t t0 = (uint32_t)(((uint64_t)A.ntiles * blockIdx.x) / nwg);
-        u.end = (uint32_t)(((uint64_t)A.ntiles * (blockIdx.x + 1)) / nwg);
+        const uint32_t t0 = (uint32_t)(((uint64_t)A.ntiles * wg) / nwg);
+        u.end = (uint32_t)(((uint64_t)A.ntiles * (wg + 1)) / nwg);
         u.tile = t0 + (uint32_t)wave;
         u.nxt = t0 + (uint32_t)NW + (uint32_t)wave;
         u.step = 0;
@@ -392,7 +408,7 @@ __device__ __forceinline__ void insert_candidate(const ScanArgs &A, int qbase, i
         return;
     }
     const int q = qbase + qi;
-    const uint32_t sh = blockIdx.x % kPoolShards;
+    const uint32_t sh = scan_chunk(A) % kPoolShards;
     const uint64_t scap = shard_cap(A.cap_pool);
     const uint32_t gp = atomicAdd(&A.pcnt[pcnt_at(q, sh)], 1u);
     if (gp < scap)
@@ -568,7 +584,7 @@ __device__ __forceinline__ void flush_candidates(const ScanArgs &A, int qbase,
     __syncthreads();
     /* one reservation per query in this WG's pool segment, all in flight
      * together */
-    const uint32_t sh = blockIdx.x % kPoolShards;
+    const uint32_t sh = scan_chunk(A) % kPoolShards;
     const uint64_t scap = shard_cap(A.cap_pool);
     if (threadIdx.x < kQT) {
         const int qi = threadIdx.x, q = qbase + qi;
@@ -635,7 +651,7 @@ void k_scan_bf16(ScanArgs A) {
     CandLds &C = *(CandLds *)(smem + (size_t)2 * S * 64 * 16);
 
     const int tid = threadIdx.x, l = tid & 63, wave = tid >> 6;
-    const int qbase = blockIdx.y * kQT;
+    const int qbase = scan_qgroup(A) * kQT;
     if (tile_skipped(A, qbase)) return;
 
     __shared__ uint32_t s_next;
@@ -754,7 +770,7 @@ void k_scan_bf16_mq(ScanArgs A) {
     bool any = false;
 #pragma unroll
     for (int k = 0; k < QTW; k++) {
-        const int qt = blockIdx.y * QTW + k;
+        const int qt = scan_qgroup(A) * QTW + k;
         qb0[k] = qt * kQT;
         qon[k] = qt < nqt && !tile_skipped(A, qb0[k]);
         any |= qon[k];
@@ -889,7 +905,7 @@ void k_scan_i8(ScanArgs A) {
     CandLds &C = *(CandLds *)(smem + (size_t)2 * KS * 64 * 16);
 
     const int tid = threadIdx.x, l = tid & 63, wave = tid >> 6;
-    const int qbase = blockIdx.y * kQT;
+    const int qbase = scan_qgroup(A) * kQT;
     if (tile_skipped(A, qbase)) return;
 
     __shared__ uint32_t s_next;
@@ -1684,7 +1700,7 @@ int scan_precision(const FvGpuIndex *g, const FvGpuBatch *b, const FvGpuBruteOpt
 template <bool PREPASS>
 static void launch_scan_bf16(int nw, bool inl, uint32_t gx, int nqt, size_t shm,
                              hipStream_t s, const ScanArgs &A) {
-    const dim3 grid(gx, nqt);
+    const dim3 grid = A.qmajor ? dim3(nqt, gx) : dim3(gx, nqt);
     if (nw == 4 && !inl) hipLaunchKernelGGL((k_scan_bf16<PREPASS, 4, false>), grid, dim3(256), shm, s, A);
     else if (nw == 4) hipLaunchKernelGGL((k_scan_bf16<PREPASS, 4, true>), grid, dim3(256), shm, s, A);
     else if (!inl) hipLaunchKernelGGL((k_scan_bf16<PREPASS, 8, false>), grid, dim3(512), shm, s, A);
@@ -1704,7 +1720,7 @@ static hipError_t launch_scan_mq(int qtw, bool inl, uint32_t gx, int gy, hipStre
                  : (const void *)k_scan_bf16_mq<PREPASS, 4, 8, false>;
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
     if (e != hipSuccess) return e;
-    const dim3 grid(gx, gy);
+    const dim3 grid = A.qmajor ? dim3(gy, gx) : dim3(gx, gy);
     if (qtw == 2 && inl) hipLaunchKernelGGL((k_scan_bf16_mq<PREPASS, 2, 4, true>), grid, dim3(256), shm, s, A);
     else if (qtw == 2) hipLaunchKernelGGL((k_scan_bf16_mq<PREPASS, 2, 4, false>), grid, dim3(256), shm, s, A);
     else if (inl) hipLaunchKernelGGL((k_scan_bf16_mq<PREPASS, 4, 8, true>), grid, dim3(512), shm, s, A);
@@ -1758,7 +1774,8 @@ hipError_t launch_phase1(FvGpuIndex *g, FvGpuBatch *b, int prec, bool bitmap, hi
             P.nfeat = pf->nfeat;
             P.feat = pf->feat;
         }
-        hipLaunchKernelGGL(k_phase1, dim3(P.nbm + (uint32_t)(P.nqp + 3) / 4), dim3(256), 0, s, P);
+        hipLaunchKernelGGL(k_phase1, dim3(P.nbm + (uint32_t)(P.nqp + 3) / 4), dim3(256),
+                           P.nbm ? bitmap_lds_bytes(P.ncols) : 0, s, P);
         return hipGetLastError();
     }
 }
@@ -1849,7 +1866,7 @@ static ScanShape scan_shape(const FvGpuIndex *g, const FvGpuBatch *b, const FvGp
 template <bool PREPASS>
 static void launch_scan_i8(int nw, int pt, bool inl, uint32_t gx, int nqt, size_t shm,
                            hipStream_t s, const ScanArgs &A) {
-    const dim3 grid(gx, nqt);
+    const dim3 grid = A.qmajor ? dim3(nqt, gx) : dim3(gx, nqt);
 #define PV_I8(NW, PT, INL) \
     hipLaunchKernelGGL((k_scan_i8<PREPASS, NW, PT, INL>), grid, dim3(NW * 64), shm, s, A)
     if (nw == 8) {
@@ -2007,6 +2024,11 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
     A.inl = inl ? b->d_inl_p : nullptr;
     A.icol = inl ? g->d_cols + (uint64_t)b->inl_col * g->count : nullptr;
     A.sweep = (o && o->order == 2) ? 1u : 0u;
+    /* query-major grid: asked for, or by default for rows above 256 dims
+     * (several query tiles re-read each chunk: running them together lets
+     * the re-reads hit in cache) */
+    const int order = o ? o->order : 0;
+    A.qmajor = ((bf || i8) && (order == 3 || (order == 0 && dim > 256 && b->nqt >= 2))) ? 1u : 0u;
     A.xn2max = g->xmax_norm * g->xmax_norm;
     if (i8) {
         A.vi8 = g->d_vi8;
@@ -2372,7 +2394,7 @@ extern "C" int fv_gpu_batch_run_brute(FvGpuIndex *g, FvGpuBatch *b,
 /* Autotuner: times every candidate launch shape of the full-scan route on
  * this batch (query tiles per WG {1, 2, 4} where the multi-tile scan
  * applies, waves per single-tile WG {4, 8} up to 256 dims, WGs per CU
- * {1, 2, 3, 4}, tile order {chunked, sweep}, pre-pass
+ * {1, 2, 3, 4}, tile order {chunked, sweep, query-major grid}, pre-pass
  * stride = over-fetch factor {auto/2, auto, 2 auto}), keeps those whose
  * runs certify every query (no
  * pool overflow, no oversized re-rank set) and returns the fastest. All
@@ -2429,7 +2451,7 @@ static int tune_batch(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *base, 
     for (int qi = 0; qi < (mq ? 3 : 1); qi++)
         for (int waves = 4; waves <= ((qi == 0 && wv) ? 8 : 4); waves += 4)
         for (int wpc = 1; wpc <= 4; wpc++)
-        for (int order = 1; order <= 2; order++) {
+        for (int order = 1; order <= (b->nqt >= 2 ? 3 : 2); order++) {
             FvGpuBruteOpts o = o0;
             o.qtw = qtws[qi];
             o.waves = qi == 0 && wv ? waves : 0;

@@ -139,26 +139,50 @@ __device__ __forceinline__ bool eval_prog(const PvOp *__restrict__ ops, int n,
                         [&](int c) { return cols[(uint64_t)c * count + row]; });
 }
 
+/* make a wave's LDS writes visible to its own lanes */
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 /*
- * Predicate bitmaps of kBmRows consecutive rows per 256-thread block, row
- * r = block * kBmRows + j * 256 + threadIdx.x for j < kBmRowsPerThread.
- * Every distinct program is evaluated for the block's rows with each op
- * decoded once for all of a thread's rows; the rows' first four column
- * values stay in registers across programs. Program offsets, a window of
- * kBmOps ops and (when they fit) the IN values are staged in LDS so that op
- * decoding costs LDS reads instead of chains of dependent scalar-cache
- * misses. Result words of kBmGroup programs are collected in LDS and
- * written as whole 128-byte runs (one run per program per block). When
- * progcnt is given, each program's passing-row count is added to it (one
+ * Predicate bitmaps of kBmRows consecutive rows per 256-thread block.
+ *
+ * The block first stages its rows' values of the first kBmLdsCols columns
+ * in LDS (dynamic shared memory of bitmap_lds_bytes(ncols) bytes), with the
+ * program table, the first kBmOps ops and (when they fit) the IN values.
+ * Then the four waves split the programs — wave w evaluates programs w,
+ * w + 4, ... over all of the block's rows — so the serial chain of a wave
+ * (decode, compare, ballot: latency-bound) is a quarter of the programs
+ * rather than all of them. A wave covers the rows in kBmSlots slots of 64
+ * (lane l of slot j: row j * 64 + l of the block), kBmChunk slots at a time.
+ *
+ * Single-op programs (one comparison, or one IN list: most filters) are
+ * evaluated straight into lane masks: the ballot of one f64 compare per
+ * slot is the result. Longer programs run the postfix interpreter on a
+ * 64-bit per-lane stack and ballot its top. Each program's masks (its
+ * kBmRows / 32 bitmap words) are staged in the wave's LDS slot and written
+ * every kBmGroup programs as 16-byte runs (stores inside the program loop
+ * would make later reuse of their registers wait for them). When progcnt is
+ * given, each program's passing-row count in the block is added to it (one
  * atomic per program per block; the caller zeroes it beforehand).
  * Called by every thread of the block (it synchronises).
  */
-constexpr int kBmRowsPerThread = 8;
-constexpr int kBmRows = 256 * kBmRowsPerThread;
-constexpr int kBmOps = 512;      /* LDS op window */
-constexpr int kBmVals = 256;     /* IN values staged in LDS when they fit */
-constexpr int kBmProgs = 1024;   /* max distinct programs (one per query) */
-constexpr int kBmGroup = 16;     /* programs per coalesced write-out */
+constexpr int kBmRows = 1024;              /* rows per block */
+constexpr int kBmSlots = kBmRows / 64;     /* 64-row slots per wave */
+constexpr int kBmChunk = 8;                /* slots evaluated together */
+constexpr int kBmLdsCols = 4;              /* columns staged in LDS */
+constexpr int kBmOps = 128;                /* ops staged in LDS */
+constexpr int kBmVals = 256;               /* IN values staged in LDS when they fit */
+constexpr int kBmProgs = 1024;             /* max distinct programs (one per query) */
+constexpr int kBmProgLds = 256;            /* program table entries staged in LDS */
+constexpr int kBmGroup = 8;                /* programs per wave between write-outs */
+
+/* dynamic LDS of a bitmap block */
+__host__ __device__ inline size_t bitmap_lds_bytes(int ncols) {
+    return (size_t)(ncols < kBmLdsCols ? ncols : kBmLdsCols) * kBmRows * sizeof(double);
+}
 
 __device__ __forceinline__ void bitmap_rows(uint64_t block, int nprog, int nops,
                                             const PvOp *__restrict__ ops,
@@ -169,178 +193,245 @@ __device__ __forceinline__ void bitmap_rows(uint64_t block, int nprog, int nops,
                                             int ncols, uint64_t words,
                                             uint32_t *__restrict__ bitmap,
                                             uint32_t *__restrict__ progcnt) {
-    constexpr int R = kBmRowsPerThread;
+    constexpr int CH = kBmChunk;
+    extern __shared__ double s_col[];          /* [min(ncols, kBmLdsCols)][kBmRows] */
     __shared__ PvOp s_ops[kBmOps];
-    __shared__ int32_t s_off[kBmProgs], s_len[kBmProgs];
+    __shared__ int2 s_prog[kBmProgLds];
     __shared__ double s_vals[kBmVals];
-    __shared__ uint64_t s_bm[kBmGroup][4 * R];
-    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-    for (int i = t; i < nprog; i += blockDim.x) {
-        s_off[i] = poff[i];
-        s_len[i] = plen[i];
-    }
+    __shared__ uint4 s_stage[4][kBmGroup][kBmSlots / 2];
+    __shared__ uint32_t s_pc[4][kBmGroup];
+    const int t = threadIdx.x, lane = t & 63;
+    /* wave index as a scalar: the program loop below is wave-uniform */
+    const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
+    const uint64_t r0 = block * kBmRows;
+    const int nc = ncols < kBmLdsCols ? ncols : kBmLdsCols;
+    const int nwin = nops < kBmOps ? nops : kBmOps;
     const bool vals_lds = nvals <= kBmVals;
-    if (vals_lds)
-        for (int i = t; i < nvals; i += blockDim.x) s_vals[i] = vals[i];
+    /* every global load of the staging is issued before any LDS write, so
+     * the prologue costs one memory round trip, not one per table */
+    static_assert(kBmOps <= 256 && kBmProgLds <= 256 && kBmVals <= 256, "one entry per thread");
+    double cvals[kBmLdsCols][kBmRows / 256];
+#pragma unroll
+    for (int c = 0; c < kBmLdsCols; c++)
+#pragma unroll
+        for (int k = 0; k < kBmRows / 256; k++) {
+            const uint64_t row = r0 + (uint64_t)k * 256 + t;
+            cvals[c][k] = (c < nc && row < count) ? cols[(uint64_t)c * count + row] : 0.0;
+        }
+    const bool has_prog = t < nprog && t < kBmProgLds;
+    const int2 prog = has_prog ? make_int2(poff[t], plen[t]) : make_int2(0, 0);
+    const bool has_op = t < nwin;
+    const PvOp op = has_op ? ops[t] : PvOp{};
+    const bool has_val = vals_lds && t < nvals;
+    const double val = has_val ? vals[t] : 0.0;
+#pragma unroll
+    for (int c = 0; c < kBmLdsCols; c++)
+        if (c < nc) {
+#pragma unroll
+            for (int k = 0; k < kBmRows / 256; k++) s_col[c * kBmRows + k * 256 + t] = cvals[c][k];
+        }
+    if (has_prog) s_prog[t] = prog;
+    if (has_op) s_ops[t] = op;
+    if (has_val) s_vals[t] = val;
+    __syncthreads();
 
-    uint64_t row[R];
-    bool live[R];
-    double cv[R][4];
+    const bool full = r0 + kBmRows <= count;
+    /* a column's values for slots [j0, j0 + CH) of the lane (c is uniform) */
+    auto colv = [&](int c, int j0, double (&v)[CH]) {
+        if (c < nc) {
 #pragma unroll
-    for (int j = 0; j < R; j++) {
-        row[j] = block * kBmRows + (uint64_t)j * 256 + t;
-        live[j] = row[j] < count;
+            for (int j = 0; j < CH; j++) v[j] = s_col[c * kBmRows + (j0 + j) * 64 + lane];
+        } else {
 #pragma unroll
-        for (int c = 0; c < 4; c++)
-            cv[j][c] = (c < ncols && live[j]) ? cols[(uint64_t)c * count + row[j]] : 0.0;
-    }
-    /* one column's values for all of the thread's rows (c is uniform) */
-    auto colv = [&](int c, double (&v)[R]) {
-        switch (c) {
-            case 0:
-#pragma unroll
-                for (int j = 0; j < R; j++) v[j] = cv[j][0];
-                break;
-            case 1:
-#pragma unroll
-                for (int j = 0; j < R; j++) v[j] = cv[j][1];
-                break;
-            case 2:
-#pragma unroll
-                for (int j = 0; j < R; j++) v[j] = cv[j][2];
-                break;
-            case 3:
-#pragma unroll
-                for (int j = 0; j < R; j++) v[j] = cv[j][3];
-                break;
-            default:
-#pragma unroll
-                for (int j = 0; j < R; j++)
-                    v[j] = live[j] ? cols[(uint64_t)c * count + row[j]] : 0.0;
+            for (int j = 0; j < CH; j++) {
+                const uint64_t row = r0 + (uint64_t)(j0 + j) * 64 + lane;
+                v[j] = row < count ? cols[(uint64_t)c * count + row] : 0.0;
+            }
         }
     };
-    int win_lo = 0, win_hi = 0; /* ops[win_lo, win_hi) are in s_ops */
-    __syncthreads();
-    for (int p = 0; p < nprog; p++) {
-        /* block-uniform: every thread walks the same programs and ops */
-        const int o = __builtin_amdgcn_readfirstlane(s_off[p]);
-        const int n = __builtin_amdgcn_readfirstlane(s_len[p]);
-        uint64_t st[R];
+    /* write out the wave's staged programs: ordinals q0 .. q0 + ng - 1 of
+     * the wave are programs 4 q + wv; lane l stores words 4 (l & 7) .. +3
+     * of staged program l >> 3 */
+    auto flush = [&](int q0, int ng) {
+        wave_lds_sync();
+        const int gi = lane >> 3, part = lane & 7;
+        if (gi < ng) {
+            const uint4 v = s_stage[wv][gi][part];
+            const uint64_t w = block * (kBmRows / 32) + 4 * part;
+            uint2 *dst = (uint2 *)(bitmap + (uint64_t)(4 * (q0 + gi) + wv) * words + w);
+            if (w < words) dst[0] = make_uint2(v.x, v.y);
+            if (w + 2 < words) dst[1] = make_uint2(v.z, v.w);
+        }
+        if (progcnt != nullptr && lane < ng && s_pc[wv][lane])
+            atomicAdd(&progcnt[4 * (q0 + lane) + wv], s_pc[wv][lane]);
+        wave_lds_sync();
+    };
+
+    for (int p = wv; p < nprog; p += 4) {
+        const int2 pr = p < kBmProgLds ? s_prog[p] : make_int2(poff[p], plen[p]);
+        const int o = __builtin_amdgcn_readfirstlane(pr.x);
+        const int n = __builtin_amdgcn_readfirstlane(pr.y);
+        const int q = p >> 2, gs = q % kBmGroup;
+        uint32_t pc = 0;
+        for (int j0 = 0; j0 < kBmSlots; j0 += CH) {
+            uint64_t m[CH];
+            uint64_t st[CH];
 #pragma unroll
-        for (int j = 0; j < R; j++) st[j] = 0;
-        for (int i = 0; i < n; i++) {
-            const int oi = o + i;
-            if (oi >= win_hi) {
-                __syncthreads();
-                win_lo = oi;
-                win_hi = min(oi + kBmOps, nops);
-                for (int k = t; k < win_hi - win_lo; k += blockDim.x) s_ops[k] = ops[win_lo + k];
-                __syncthreads();
-            }
-            /* op fields into scalar registers: every dispatch below is a
-             * scalar branch taken once per op for all of the thread's rows */
-            const int32_t *ow = (const int32_t *)&s_ops[oi - win_lo];
-            const int kind = __builtin_amdgcn_readfirstlane(ow[0]);
-            const int col = __builtin_amdgcn_readfirstlane(ow[1]);
-            const int oa = __builtin_amdgcn_readfirstlane(ow[2]);
-            const int ob = __builtin_amdgcn_readfirstlane(ow[3]);
-            const double c = __hiloint2double(__builtin_amdgcn_readfirstlane(ow[5]),
-                                              __builtin_amdgcn_readfirstlane(ow[4]));
-            uint32_t r[R];
-            if (kind == OP_CMP) {
-                double v[R];
-                colv(col, v);
-                switch (oa) {
-                    case FV_CMP_LT:
+            for (int j = 0; j < CH; j++) st[j] = 0;
+            for (int i = 0; i < n; i++) {
+                /* op fields into scalar registers: every dispatch below is
+                 * a scalar branch taken once per op for all of the slots */
+                const int oi = o + i;
+                int32_t f[6];
+                if (oi < nwin) {
+                    const int32_t *ow = (const int32_t *)&s_ops[oi];
 #pragma unroll
-                        for (int j = 0; j < R; j++) r[j] = v[j] < c;
-                        break;
-                    case FV_CMP_LE:
-#pragma unroll
-                        for (int j = 0; j < R; j++) r[j] = v[j] <= c;
-                        break;
-                    case FV_CMP_EQ:
-#pragma unroll
-                        for (int j = 0; j < R; j++) r[j] = v[j] == c;
-                        break;
-                    case FV_CMP_GE:
-#pragma unroll
-                        for (int j = 0; j < R; j++) r[j] = v[j] >= c;
-                        break;
-                    case FV_CMP_GT:
-#pragma unroll
-                        for (int j = 0; j < R; j++) r[j] = v[j] > c;
-                        break;
-                    default: /* NE: non-NULL and different */
-#pragma unroll
-                        for (int j = 0; j < R; j++) r[j] = (v[j] == v[j]) && (v[j] != c);
-                }
-#pragma unroll
-                for (int j = 0; j < R; j++) st[j] = (st[j] << 1) | r[j];
-            } else if (kind == OP_IN) {
-                double v[R];
-                colv(col, v);
-#pragma unroll
-                for (int j = 0; j < R; j++) r[j] = 0;
-                if (vals_lds) {
-                    for (int k = 0; k < ob; k++) {
-                        const double x = s_vals[oa + k];
-#pragma unroll
-                        for (int j = 0; j < R; j++) r[j] |= (v[j] == x) ? 1u : 0u;
-                    }
+                    for (int k = 0; k < 6; k++) f[k] = ow[k];
                 } else {
-                    for (int k = 0; k < ob; k++) {
-                        const double x = vals[oa + k];
+                    const int32_t *ow = (const int32_t *)&ops[oi];
 #pragma unroll
-                        for (int j = 0; j < R; j++) r[j] |= (v[j] == x) ? 1u : 0u;
-                    }
+                    for (int k = 0; k < 6; k++) f[k] = ow[k];
                 }
+                const int kind = __builtin_amdgcn_readfirstlane(f[0]);
+                const int col = __builtin_amdgcn_readfirstlane(f[1]);
+                const int oa = __builtin_amdgcn_readfirstlane(f[2]);
+                const int ob = __builtin_amdgcn_readfirstlane(f[3]);
+                const double c = __hiloint2double(__builtin_amdgcn_readfirstlane(f[5]),
+                                                  __builtin_amdgcn_readfirstlane(f[4]));
+                if (n == 1 && kind == OP_CMP) {
+                    /* the whole program is one comparison: its ballots are
+                     * the result */
+                    double v[CH];
+                    colv(col, j0, v);
+                    switch (oa) {
+                        case FV_CMP_LT:
 #pragma unroll
-                for (int j = 0; j < R; j++) st[j] = (st[j] << 1) | r[j];
-            } else if (kind == OP_AND2) {
+                            for (int j = 0; j < CH; j++) m[j] = __ballot(v[j] < c);
+                            break;
+                        case FV_CMP_LE:
 #pragma unroll
-                for (int j = 0; j < R; j++)
-                    st[j] = ((st[j] >> 2) << 1) | (st[j] & (st[j] >> 1) & 1u);
-            } else if (kind == OP_OR2) {
+                            for (int j = 0; j < CH; j++) m[j] = __ballot(v[j] <= c);
+                            break;
+                        case FV_CMP_EQ:
 #pragma unroll
-                for (int j = 0; j < R; j++)
-                    st[j] = ((st[j] >> 2) << 1) | ((st[j] | (st[j] >> 1)) & 1u);
-            } else if (kind == OP_NOT) {
+                            for (int j = 0; j < CH; j++) m[j] = __ballot(v[j] == c);
+                            break;
+                        case FV_CMP_GE:
 #pragma unroll
-                for (int j = 0; j < R; j++) st[j] ^= 1u;
-            } else if (kind == OP_TRUE) {
+                            for (int j = 0; j < CH; j++) m[j] = __ballot(v[j] >= c);
+                            break;
+                        case FV_CMP_GT:
 #pragma unroll
-                for (int j = 0; j < R; j++) st[j] = (st[j] << 1) | 1u;
-            } else { /* OP_FALSE */
+                            for (int j = 0; j < CH; j++) m[j] = __ballot(v[j] > c);
+                            break;
+                        default: /* NE: non-NULL and different */
 #pragma unroll
-                for (int j = 0; j < R; j++) st[j] <<= 1;
+                            for (int j = 0; j < CH; j++)
+                                m[j] = __ballot((v[j] == v[j]) && (v[j] != c));
+                    }
+                    break;
+                }
+                if (n == 1 && kind == OP_IN) {
+                    double v[CH];
+                    colv(col, j0, v);
+#pragma unroll
+                    for (int j = 0; j < CH; j++) m[j] = 0;
+                    for (int k = 0; k < ob; k++) {
+                        const double x = vals_lds ? s_vals[oa + k] : vals[oa + k];
+#pragma unroll
+                        for (int j = 0; j < CH; j++) m[j] |= __ballot(v[j] == x);
+                    }
+                    break;
+                }
+                uint32_t r[CH];
+                if (kind == OP_CMP) {
+                    double v[CH];
+                    colv(col, j0, v);
+                    switch (oa) {
+                        case FV_CMP_LT:
+#pragma unroll
+                            for (int j = 0; j < CH; j++) r[j] = v[j] < c;
+                            break;
+                        case FV_CMP_LE:
+#pragma unroll
+                            for (int j = 0; j < CH; j++) r[j] = v[j] <= c;
+                            break;
+                        case FV_CMP_EQ:
+#pragma unroll
+                            for (int j = 0; j < CH; j++) r[j] = v[j] == c;
+                            break;
+                        case FV_CMP_GE:
+#pragma unroll
+                            for (int j = 0; j < CH; j++) r[j] = v[j] >= c;
+                            break;
+                        case FV_CMP_GT:
+#pragma unroll
+                            for (int j = 0; j < CH; j++) r[j] = v[j] > c;
+                            break;
+                        default: /* NE: non-NULL and different */
+#pragma unroll
+                            for (int j = 0; j < CH; j++) r[j] = (v[j] == v[j]) && (v[j] != c);
+                    }
+#pragma unroll
+                    for (int j = 0; j < CH; j++) st[j] = (st[j] << 1) | r[j];
+                } else if (kind == OP_IN) {
+                    double v[CH];
+                    colv(col, j0, v);
+#pragma unroll
+                    for (int j = 0; j < CH; j++) r[j] = 0;
+                    for (int k = 0; k < ob; k++) {
+                        const double x = vals_lds ? s_vals[oa + k] : vals[oa + k];
+#pragma unroll
+                        for (int j = 0; j < CH; j++) r[j] |= (v[j] == x) ? 1u : 0u;
+                    }
+#pragma unroll
+                    for (int j = 0; j < CH; j++) st[j] = (st[j] << 1) | r[j];
+                } else if (kind == OP_AND2) {
+#pragma unroll
+                    for (int j = 0; j < CH; j++)
+                        st[j] = ((st[j] >> 2) << 1) | (st[j] & (st[j] >> 1) & 1u);
+                } else if (kind == OP_OR2) {
+#pragma unroll
+                    for (int j = 0; j < CH; j++)
+                        st[j] = ((st[j] >> 2) << 1) | ((st[j] | (st[j] >> 1)) & 1u);
+                } else if (kind == OP_NOT) {
+#pragma unroll
+                    for (int j = 0; j < CH; j++) st[j] ^= 1u;
+                } else if (kind == OP_TRUE) {
+#pragma unroll
+                    for (int j = 0; j < CH; j++) st[j] = (st[j] << 1) | 1u;
+                } else { /* OP_FALSE */
+#pragma unroll
+                    for (int j = 0; j < CH; j++) st[j] <<= 1;
+                }
+                if (i == n - 1) {
+#pragma unroll
+                    for (int j = 0; j < CH; j++) m[j] = __ballot((st[j] & 1u) != 0);
+                }
+            }
+            if (n == 0) {
+#pragma unroll
+                for (int j = 0; j < CH; j++) m[j] = 0;
+            }
+            if (!full) {
+#pragma unroll
+                for (int j = 0; j < CH; j++)
+                    m[j] &= __ballot(r0 + (uint64_t)(j0 + j) * 64 + lane < count);
+            }
+            if (progcnt != nullptr) {
+#pragma unroll
+                for (int j = 0; j < CH; j++) pc += (uint32_t)__popcll(m[j]);
+            }
+            if (lane == 0) {
+#pragma unroll
+                for (int k = 0; k < CH / 2; k++)
+                    s_stage[wv][gs][j0 / 2 + k] =
+                        make_uint4((uint32_t)m[2 * k], (uint32_t)(m[2 * k] >> 32),
+                                   (uint32_t)m[2 * k + 1], (uint32_t)(m[2 * k + 1] >> 32));
             }
         }
-        const int slot = p % kBmGroup;
-#pragma unroll
-        for (int j = 0; j < R; j++) {
-            /* rows j*256 + wv*64 .. +63 of the block: 64-bit word 4j + wv */
-            const uint64_t m = __ballot(live[j] && (st[j] & 1u));
-            if (lane == 0) s_bm[slot][4 * j + wv] = m;
-        }
-        if (slot == kBmGroup - 1 || p == nprog - 1) {
-            __syncthreads();
-            const int pb = p - slot;
-            for (int i = t; i < (slot + 1) * 2 * 4 * R; i += blockDim.x) {
-                const int pp = i / (8 * R), wd = i % (8 * R);
-                const uint64_t w = block * (kBmRows / 32) + wd;
-                if (w < words)
-                    bitmap[(uint64_t)(pb + pp) * words + w] =
-                        (uint32_t)(s_bm[pp][wd >> 1] >> (32 * (wd & 1)));
-            }
-            if (progcnt != nullptr && t <= slot) {
-                /* passing rows of program pb + t in this block */
-                uint32_t c = 0;
-                for (int k = 0; k < 4 * R; k++) c += (uint32_t)__popcll(s_bm[t][k]);
-                if (c) atomicAdd(&progcnt[pb + t], c);
-            }
-            __syncthreads();
-        }
+        if (lane == 0) s_pc[wv][gs] = pc;
+        if (gs == kBmGroup - 1 || p + 4 >= nprog) flush(q - gs, gs + 1);
     }
 }
 
@@ -372,13 +463,6 @@ __device__ __forceinline__ uint64_t wave_sort64(uint64_t x) {
  * the feature / decision kernels, so every path computes the same values) */
 
 constexpr int kNear = 16;   /* nearest-centroid distance features */
-
-/* make a wave's LDS writes visible to its own lanes */
-__device__ __forceinline__ void wave_lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 /* squared distance of the query in xs (LDS) to centroid c of the
  * transposed centroid array [dim][ncent]: 32 coordinates per step, all

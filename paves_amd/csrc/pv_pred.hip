@@ -8,8 +8,8 @@
  * so the evaluation stack depth is bounded by the tree depth), validates
  * every index, and merges byte-identical programs: queries with the same
  * predicate share one bitmap. The bitmap kernel evaluates each distinct
- * program for every row — one wave per 64 rows, one ballot per wave — and
- * stores bit (row & 31) of word [prog][row >> 5].
+ * program for every row — the waves of a block split the programs, one
+ * ballot per 64 rows — and stores bit (row & 31) of word [prog][row >> 5].
  *
  * Semantics follow FvPred::eval_node (engine.cpp:397-430) and the CUDA
  * interpreter (gpu_engine.cu:238-279): comparisons in double on the f64
@@ -196,7 +196,8 @@ hipError_t launch_bitmap(FvGpuIndex *g, FvGpuBatch *b, hipStream_t s) {
     if (b->nprog == 0) return hipSuccess;
     const uint64_t words = g->count_pad / 32;
     const unsigned blocks = (unsigned)((g->count_pad + kBmRows - 1) / kBmRows);
-    hipLaunchKernelGGL(k_bitmap, dim3(blocks), dim3(256), 0, s, b->d_ops, b->d_poff,
+    hipLaunchKernelGGL(k_bitmap, dim3(blocks), dim3(256), bitmap_lds_bytes(g->ncols), s,
+                       b->d_ops, b->d_poff,
                        b->d_plen, b->nprog, b->nops, b->d_vals, b->nvals, g->d_cols, g->count,
                        g->ncols, words, b->d_bitmap.as<uint32_t>());
     return hipGetLastError();

@@ -40,9 +40,11 @@ def _check(out, cnt, trunc, d, cols, b, kq, sel=None):
             assert np.array_equal(out[q, :c]["dist"], oh[j, :c]["dist"]), (q, k)
 
 
-@pytest.mark.parametrize("inline,qtw", [(1, 0), (0, 0), (1, 1), (0, 2), (1, 4)])
-def test_uneven_k_and_selectivity(inline, qtw):
-    """query tiles per WG: 0 = auto (1 for this 5-tile batch), 1, 2, 4"""
+@pytest.mark.parametrize("inline,qtw,order", [(1, 0, 0), (0, 0, 0), (1, 1, 0), (0, 2, 0),
+                                              (1, 4, 0), (1, 1, 3), (0, 4, 3), (1, 2, 2)])
+def test_uneven_k_and_selectivity(inline, qtw, order):
+    """query tiles per WG: 0 = auto (1 for this 5-tile batch), 1, 2, 4; tile
+    order 0 = auto, 2 = sweep, 3 = query-major grid"""
     n, nq = 200_000, 300
     d, cols, rows, kq, b = _uneven(n, nq, 41)
     g = E.GpuIndex(d["base"], cols, rows, precision=L.FV_GPU_PREC_BF16)
@@ -50,7 +52,8 @@ def test_uneven_k_and_selectivity(inline, qtw):
         bh = g.stage_ex(d["queries"], b, kq)
         try:
             for route in (1, 0):
-                g.run_brute(bh, L.BruteOpts(route=route, inline_pred=inline, qtw=qtw))
+                g.run_brute(bh, L.BruteOpts(route=route, inline_pred=inline, qtw=qtw,
+                                            order=order))
                 g.sync()
                 out, cnt, flag, rc = g.fetch(bh, nq, int(kq.max()))
                 assert rc == 0
