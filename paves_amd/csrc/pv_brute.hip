@@ -146,7 +146,8 @@ struct Phase1Args {
     const double *vals, *cols;
     uint64_t count, words;
     uint32_t *bitmap;
-    uint32_t *progcnt;    /* optional: passing rows per program (+=) */
+    uint32_t *progpart;   /* optional: passing rows per program and bitmap
+                             block, [nprog][nbm] */
     int nprog, nops, nvals, ncols;
     uint32_t nbm;
     const float *q;
@@ -170,7 +171,7 @@ struct Phase1Args {
 __global__ __launch_bounds__(256) void k_phase1(Phase1Args A) {
     if (blockIdx.x < A.nbm) {
         bitmap_rows(blockIdx.x, A.nprog, A.nops, A.ops, A.poff, A.plen, A.vals, A.nvals, A.cols, A.count,
-                    A.ncols, A.words, A.bitmap, A.progcnt);
+                    A.ncols, A.words, A.bitmap, A.progpart, A.nbm);
         return;
     }
     const int w = threadIdx.x >> 6;
@@ -1360,17 +1361,12 @@ __global__ __launch_bounds__(256) void k_select(
     const float *__restrict__ vecs, const float *__restrict__ qf, int dim,
     FvHit *__restrict__ out, uint32_t out_stride, uint32_t *__restrict__ out_cnt,
     uint8_t *__restrict__ out_flag, uint32_t *__restrict__ status,
-    const uint8_t *__restrict__ skip, uint32_t *__restrict__ progcnt_reset, int nprog,
-    const uint32_t *__restrict__ topbq) {
+    const uint8_t *__restrict__ skip, const uint32_t *__restrict__ topbq) {
     __shared__ uint64_t sel[kSelMax], wb[4];
     __shared__ uint32_t hist[256], tmp[8], bc[8];
     __shared__ uint32_t nsel;
     extern __shared__ float qsh[];
     const int q = blockIdx.x, t = threadIdx.x;
-    /* passing-row counts of this run were consumed by the routing kernel:
-     * clear them for the batch's next run */
-    if (progcnt_reset && q == 0)
-        for (int i = t; i < nprog; i += blockDim.x) progcnt_reset[i] = 0;
     if (topbq) topb = topbq[q]; /* per-query k of an uneven batch */
     if (skip && skip[q]) return; /* the graph search writes this query */
     /* the query's pool segments: counts, their prefix, and an index map
@@ -1675,10 +1671,9 @@ int stage_batch(FvGpuIndex *g, FvGpuBatch *b, int nq, const float *queries,
     PV_TRY(b->d_cnt.ensure((size_t)nq * 4));
     PV_TRY(b->d_flag.ensure((size_t)nq));
     PV_TRY(b->d_pcnt.ensure((size_t)nqp * kPoolShards * kPcntStride * 4));
-    /* per-program passing-row counts of routed runs: zero here, then
-     * accumulated by each run's bitmap pass and cleared by its select */
-    PV_TRY(b->d_progcnt.ensure((size_t)std::max(1, b->nprog) * 4));
-    PV_TRY(hipMemsetAsync(b->d_progcnt.p, 0, (size_t)std::max(1, b->nprog) * 4, b->stream));
+    /* per-program, per-bitmap-block passing-row counts of routed runs
+     * (every run's bitmap pass writes all of them) */
+    PV_TRY(b->d_progcnt.ensure((size_t)std::max(1, b->nprog) * bitmap_blocks(g) * 4));
     return 0;
 }
 
@@ -1731,7 +1726,7 @@ static hipError_t launch_scan_mq(int qtw, bool inl, uint32_t gx, int gy, hipStre
 /* Phase 1 of a run: predicate bitmaps (when `bitmap`) and the query
  * preparation for scan precision `prec`, one launch on stream s. */
 hipError_t launch_phase1(FvGpuIndex *g, FvGpuBatch *b, int prec, bool bitmap, hipStream_t s,
-                         uint32_t *progcnt, const PrepFeatures *pf) {
+                         uint32_t *progpart, const PrepFeatures *pf) {
     QAux qa = qaux_of(b);
     {
         /* predicate programs -> row bitmaps, every run (the gather route
@@ -1746,12 +1741,12 @@ hipError_t launch_phase1(FvGpuIndex *g, FvGpuBatch *b, int prec, bool bitmap, hi
         P.count = g->count;
         P.words = g->count_pad / 32;
         P.bitmap = b->d_bitmap.as<uint32_t>();
-        P.progcnt = progcnt;
+        P.progpart = progpart;
         P.nprog = b->nprog;
         P.nops = b->nops;
         P.nvals = b->nvals;
         P.ncols = g->ncols;
-        P.nbm = (bitmap && b->nprog > 0) ? (uint32_t)((g->count_pad + kBmRows - 1) / kBmRows) : 0;
+        P.nbm = (bitmap && b->nprog > 0) ? bitmap_blocks(g) : 0;
         P.q = b->d_q;
         P.nq = b->nq;
         P.nqp = b->nqt * kQT;
@@ -1897,7 +1892,7 @@ static hipError_t launch_main_scan(const ScanLaunch &L, hipStream_t s) {
  * sample stride (1 = every row: the tightest threshold). */
 int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
                  uint32_t stride_override, const uint8_t *skip, bool have_phase1,
-                 uint32_t *progcnt_reset, char *errbuf, size_t errlen) {
+                 char *errbuf, size_t errlen) {
     hipStream_t s = b->stream;
     g->ev_stream = s;
     b->out_stride = b->topb; /* a graph run may have widened it; d_out holds nq x topb */
@@ -1970,7 +1965,6 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
                            g->d_vecs, b->d_q, dim, b->d_out.as<FvHit>(),
                            b->out_stride, b->d_cnt.as<uint32_t>(),
                            b->d_flag.as<uint8_t>(), qa.status, skip,
-                           progcnt_reset, b->nprog,
                            b->d_topbq_p);
         PV_TRY(hipGetLastError());
         ev_record(g, 5);
@@ -2100,7 +2094,6 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
                        b->cap_pool, B, qa.eps, g->d_vecs, b->d_q, dim,
                        b->d_out.as<FvHit>(), b->out_stride,
                        b->d_cnt.as<uint32_t>(), b->d_flag.as<uint8_t>(), qa.status, skip,
-                           progcnt_reset, b->nprog,
                            b->d_topbq_p);
     PV_TRY(hipGetLastError());
     ev_record(g, 5);
@@ -2111,7 +2104,7 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
 
 static int run_brute(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
                      uint32_t stride_override, char *errbuf, size_t errlen) {
-    return run_brute_ex(g, b, o, stride_override, nullptr, false, nullptr, errbuf, errlen);
+    return run_brute_ex(g, b, o, stride_override, nullptr, false, errbuf, errlen);
 }
 
 /* copy outputs and per-query status to pinned host memory (synchronous) */

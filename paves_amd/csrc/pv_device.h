@@ -164,9 +164,11 @@ __device__ __forceinline__ void wave_lds_sync() {
  * 64-bit per-lane stack and ballot its top. Each program's masks (its
  * kBmRows / 32 bitmap words) are staged in the wave's LDS slot and written
  * every kBmGroup programs as 16-byte runs (stores inside the program loop
- * would make later reuse of their registers wait for them). When progcnt is
- * given, each program's passing-row count in the block is added to it (one
- * atomic per program per block; the caller zeroes it beforehand).
+ * would make later reuse of their registers wait for them). When progpart is
+ * given, the block stores each program's passing-row count in it, layout
+ * [nprog][bitmap blocks] (plain stores, zeros included: no clearing, no
+ * atomics — per-block atomics on a few adjacent counters serialise in one
+ * L2 channel).
  * Called by every thread of the block (it synchronises).
  */
 constexpr int kBmRows = 1024;              /* rows per block */
@@ -179,9 +181,82 @@ constexpr int kBmProgs = 1024;             /* max distinct programs (one per que
 constexpr int kBmProgLds = 256;            /* program table entries staged in LDS */
 constexpr int kBmGroup = 8;                /* programs per wave between write-outs */
 
+/* blocks of a bitmap pass over an index */
+inline uint32_t bitmap_blocks(const FvGpuIndex *g) {
+    return (uint32_t)((g->count_pad + kBmRows - 1) / kBmRows);
+}
+
 /* dynamic LDS of a bitmap block */
 __host__ __device__ inline size_t bitmap_lds_bytes(int ncols) {
     return (size_t)(ncols < kBmLdsCols ? ncols : kBmLdsCols) * kBmRows * sizeof(double);
+}
+
+/* result masks of kBmSlots slots into a wave's staging slot (lane 0 writes;
+ * the masks are wave-uniform); rows at or beyond `count` are cleared;
+ * returns the number of passing rows */
+__device__ __forceinline__ uint32_t bm_stage(uint64_t (&m)[kBmSlots], uint4 *stg, int lane,
+                                             bool full, uint64_t r0, uint64_t count,
+                                             bool counting) {
+    if (!full) {
+#pragma unroll
+        for (int j = 0; j < kBmSlots; j++)
+            m[j] &= __ballot(r0 + (uint64_t)j * 64 + lane < count);
+    }
+    uint32_t pc = 0;
+    if (counting) {
+#pragma unroll
+        for (int j = 0; j < kBmSlots; j++) pc += (uint32_t)__popcll(m[j]);
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < kBmSlots / 2; k++)
+            stg[k] = make_uint4((uint32_t)m[2 * k], (uint32_t)(m[2 * k] >> 32),
+                                (uint32_t)m[2 * k + 1], (uint32_t)(m[2 * k + 1] >> 32));
+    }
+    return pc;
+}
+
+/* a single-comparison program over the block's rows staged in LDS (colp:
+ * the column's kBmRows values): one compare per slot, its ballot is the
+ * result; straight-line code per comparison kind */
+template <int OP>
+__device__ __forceinline__ uint32_t bm_cmp(const double *colp, double c, uint4 *stg, int lane,
+                                           bool full, uint64_t r0, uint64_t count,
+                                           bool counting) {
+    double v[kBmSlots];
+#pragma unroll
+    for (int j = 0; j < kBmSlots; j++) v[j] = colp[j * 64 + lane];
+    uint64_t m[kBmSlots];
+#pragma unroll
+    for (int j = 0; j < kBmSlots; j++) {
+        bool r;
+        if (OP == FV_CMP_LT) r = v[j] < c;
+        else if (OP == FV_CMP_LE) r = v[j] <= c;
+        else if (OP == FV_CMP_EQ) r = v[j] == c;
+        else if (OP == FV_CMP_GE) r = v[j] >= c;
+        else if (OP == FV_CMP_GT) r = v[j] > c;
+        else r = (v[j] == v[j]) && (v[j] != c); /* NE: non-NULL and different */
+        m[j] = __ballot(r);
+    }
+    return bm_stage(m, stg, lane, full, r0, count, counting);
+}
+
+/* a single-IN program over the block's rows staged in LDS */
+__device__ __forceinline__ uint32_t bm_in(const double *colp, const double *inv, int nv,
+                                          uint4 *stg, int lane, bool full, uint64_t r0,
+                                          uint64_t count, bool counting) {
+    double v[kBmSlots];
+#pragma unroll
+    for (int j = 0; j < kBmSlots; j++) v[j] = colp[j * 64 + lane];
+    uint64_t m[kBmSlots];
+#pragma unroll
+    for (int j = 0; j < kBmSlots; j++) m[j] = 0;
+    for (int k = 0; k < nv; k++) {
+        const double x = inv[k];
+#pragma unroll
+        for (int j = 0; j < kBmSlots; j++) m[j] |= __ballot(v[j] == x);
+    }
+    return bm_stage(m, stg, lane, full, r0, count, counting);
 }
 
 __device__ __forceinline__ void bitmap_rows(uint64_t block, int nprog, int nops,
@@ -192,7 +267,7 @@ __device__ __forceinline__ void bitmap_rows(uint64_t block, int nprog, int nops,
                                             const double *__restrict__ cols, uint64_t count,
                                             int ncols, uint64_t words,
                                             uint32_t *__restrict__ bitmap,
-                                            uint32_t *__restrict__ progcnt) {
+                                            uint32_t *__restrict__ progpart, uint32_t nblocks) {
     constexpr int CH = kBmChunk;
     extern __shared__ double s_col[];          /* [min(ncols, kBmLdsCols)][kBmRows] */
     __shared__ PvOp s_ops[kBmOps];
@@ -262,8 +337,8 @@ __device__ __forceinline__ void bitmap_rows(uint64_t block, int nprog, int nops,
             if (w < words) dst[0] = make_uint2(v.x, v.y);
             if (w + 2 < words) dst[1] = make_uint2(v.z, v.w);
         }
-        if (progcnt != nullptr && lane < ng && s_pc[wv][lane])
-            atomicAdd(&progcnt[4 * (q0 + lane) + wv], s_pc[wv][lane]);
+        if (progpart != nullptr && lane < ng)
+            progpart[(uint64_t)(4 * (q0 + lane) + wv) * nblocks + block] = s_pc[wv][lane];
         wave_lds_sync();
     };
 
@@ -273,7 +348,31 @@ __device__ __forceinline__ void bitmap_rows(uint64_t block, int nprog, int nops,
         const int n = __builtin_amdgcn_readfirstlane(pr.y);
         const int q = p >> 2, gs = q % kBmGroup;
         uint32_t pc = 0;
-        for (int j0 = 0; j0 < kBmSlots; j0 += CH) {
+        /* single-op programs on a staged column: straight-line code */
+        const bool cnt = progpart != nullptr;
+        bool done = false;
+        if (n == 1 && o < nwin) {
+            const int32_t *ow = (const int32_t *)&s_ops[o];
+            const int kind = __builtin_amdgcn_readfirstlane(ow[0]);
+            const int col = __builtin_amdgcn_readfirstlane(ow[1]);
+            const int oa = __builtin_amdgcn_readfirstlane(ow[2]);
+            const int ob = __builtin_amdgcn_readfirstlane(ow[3]);
+            const double c = __hiloint2double(__builtin_amdgcn_readfirstlane(ow[5]),
+                                              __builtin_amdgcn_readfirstlane(ow[4]));
+            if (col < nc && (kind == OP_CMP || (kind == OP_IN && vals_lds))) {
+                const double *colp = s_col + col * kBmRows;
+                uint4 *stg = s_stage[wv][gs];
+                if (kind == OP_IN) pc = bm_in(colp, s_vals + oa, ob, stg, lane, full, r0, count, cnt);
+                else if (oa == FV_CMP_LT) pc = bm_cmp<FV_CMP_LT>(colp, c, stg, lane, full, r0, count, cnt);
+                else if (oa == FV_CMP_LE) pc = bm_cmp<FV_CMP_LE>(colp, c, stg, lane, full, r0, count, cnt);
+                else if (oa == FV_CMP_EQ) pc = bm_cmp<FV_CMP_EQ>(colp, c, stg, lane, full, r0, count, cnt);
+                else if (oa == FV_CMP_GE) pc = bm_cmp<FV_CMP_GE>(colp, c, stg, lane, full, r0, count, cnt);
+                else if (oa == FV_CMP_GT) pc = bm_cmp<FV_CMP_GT>(colp, c, stg, lane, full, r0, count, cnt);
+                else pc = bm_cmp<FV_CMP_NE>(colp, c, stg, lane, full, r0, count, cnt);
+                done = true;
+            }
+        }
+        for (int j0 = 0; j0 < (done ? 0 : kBmSlots); j0 += CH) {
             uint64_t m[CH];
             uint64_t st[CH];
 #pragma unroll
@@ -418,7 +517,7 @@ __device__ __forceinline__ void bitmap_rows(uint64_t block, int nprog, int nops,
                 for (int j = 0; j < CH; j++)
                     m[j] &= __ballot(r0 + (uint64_t)(j0 + j) * 64 + lane < count);
             }
-            if (progcnt != nullptr) {
+            if (cnt) {
 #pragma unroll
                 for (int j = 0; j < CH; j++) pc += (uint32_t)__popcll(m[j]);
             }

@@ -203,7 +203,8 @@ struct FvGpuBatch {
     int last_route = 0;
     hipStream_t stream = nullptr; /* one of the index's batch streams */
     pv::DevBuf d_visit;   /* graph: overflow visited bitmaps */
-    pv::DevBuf d_progcnt; /* [nprog] passing rows per program (learned features) */
+    pv::DevBuf d_progcnt; /* [nprog][bitmap blocks] passing rows per program and
+                             bitmap block (learned features) */
     pv::DevBuf d_feat, d_y; /* learned run: features [nq][F], model outputs [nq][2] */
     pv::DevBuf d_route, d_efq; /* learned run: route u8 [nq] (1 = graph), ef u32 [nq] */
 };
@@ -307,7 +308,7 @@ int run_graph_ex(FvGpuIndex *g, FvGpuBatch *b, uint32_t ef, uint64_t max_visited
  * queries it must leave alone; have_phase1 = bitmaps and query prep done */
 int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
                  uint32_t stride_override, const uint8_t *skip, bool have_phase1,
-                 uint32_t *progcnt_reset, char *errbuf, size_t errlen);
+                 char *errbuf, size_t errlen);
 /* router features computed by the query prep of phase 1 (dim <= 256) */
 struct PrepFeatures {
     const float *cent_t;
@@ -315,7 +316,7 @@ struct PrepFeatures {
     float *feat;
 };
 hipError_t launch_phase1(FvGpuIndex *g, FvGpuBatch *b, int prec, bool bitmap, hipStream_t s,
-                         uint32_t *progcnt, const PrepFeatures *pf = nullptr);
+                         uint32_t *progpart, const PrepFeatures *pf = nullptr);
 /* precision the brute scan runs in for these options (FP32 or BF16) */
 int scan_precision(const FvGpuIndex *g, const FvGpuBatch *b, const FvGpuBruteOpts *o);
 

@@ -367,7 +367,8 @@ struct LearnArgs {
     const float *q;
     int nq, dim, F;
     const int32_t *pofq;
-    const uint32_t *progcnt;
+    const uint32_t *progpart;  /* [nprog][nbm] passing rows per bitmap block */
+    uint32_t nbm;
     uint64_t count;
     const float *cent;
     int ncent;
@@ -380,6 +381,18 @@ struct LearnArgs {
     uint32_t *efq;
     float *feat_out, *y_out;
 };
+
+/* passing rows of query qi's predicate (all rows without one): the wave
+ * sums its program's per-block counts of the bitmap pass */
+__device__ __forceinline__ double wave_pass_count(const LearnArgs &A, int qi) {
+    const int p = A.pofq[qi];
+    if (p < 0) return (double)A.count;
+    const uint32_t *part = A.progpart + (uint64_t)p * A.nbm;
+    uint32_t c = 0;
+    for (uint32_t i = threadIdx.x & 63; i < A.nbm; i += 64) c += part[i];
+    for (int o = 32; o; o >>= 1) c += __shfl_xor(c, o);
+    return (double)c;
+}
 
 /* route/ef from the model outputs (mode 0/1) or the fixed policy */
 __device__ __forceinline__ void route_decision(const LearnArgs &A, int qi, float y0, float y1) {
@@ -412,13 +425,14 @@ __global__ __launch_bounds__(256) void k_learn_route_pre(LearnArgs A) {
     const int np = mlp_layout(A.F).n;
     const bool lds = np <= kLdsParams;
     if (lds) stage_params(A.wt, np, ws);
+    const bool live = qi < A.nq;
+    const double pass = live ? wave_pass_count(A, qi) : 0.0;
+    float *fq = A.feat_out + (uint64_t)(live ? qi : 0) * A.F;
+    if (live)
+        for (int j = l; j < A.F; j += 64) xs[w][j] = fq[j];
     __syncthreads(); /* every wave reaches it (no early return above) */
-    if (qi >= A.nq) return;
-    float *fq = A.feat_out + (uint64_t)qi * A.F;
-    for (int j = l; j < A.F; j += 64) xs[w][j] = fq[j];
+    if (!live) return;
     if (l == 0) {
-        const int p = A.pofq[qi];
-        const double pass = p < 0 ? (double)A.count : (double)A.progcnt[p];
         const float sel = (float)log10(fmax(pass, 1.0) / (double)A.count);
         xs[w][A.dim] = sel;
         fq[A.dim] = sel;
@@ -448,10 +462,9 @@ __global__ __launch_bounds__(256) void k_learn_route(LearnArgs A) {
     if (lds) stage_params(A.wt, np, ws);
     const float *qv = A.q + (uint64_t)qi * A.dim;
     for (int j = threadIdx.x; j < A.dim; j += blockDim.x) xs[j] = qv[j];
-    if (threadIdx.x == 0) {
-        const int p = A.pofq[qi];
-        const double pass = p < 0 ? (double)A.count : (double)A.progcnt[p];
-        xs[A.dim] = (float)log10(fmax(pass, 1.0) / (double)A.count);
+    if (w == 0) {
+        const double pass = wave_pass_count(A, qi);
+        if (l == 0) xs[A.dim] = (float)log10(fmax(pass, 1.0) / (double)A.count);
     }
     __syncthreads();
     float best[kNear];
@@ -873,7 +886,8 @@ extern "C" int fv_gpu_batch_run_learned(FvGpuIndex *g, FvGpuBatch *b, FvGpuModel
         A.nq = nq;
         A.dim = g->dim;
         A.pofq = b->d_pofq;
-        A.progcnt = pc;
+        A.progpart = pc;
+        A.nbm = bitmap_blocks(g);
         A.count = g->count;
         A.mode = o->mode;
         A.ef_fixed = ef_fixed;
@@ -903,7 +917,7 @@ extern "C" int fv_gpu_batch_run_learned(FvGpuIndex *g, FvGpuBatch *b, FvGpuModel
         }
         PV_TRY(hipGetLastError());
         const uint8_t *route = b->d_route.as<uint8_t>();
-        if (run_brute_ex(g, b, &bo, 0, route, true, pc, errbuf, errlen) != 0) return -1;
+        if (run_brute_ex(g, b, &bo, 0, route, true, errbuf, errlen) != 0) return -1;
         if (o->mode != 3) {
             GraphExtra x;
             x.route = route;

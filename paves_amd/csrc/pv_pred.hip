@@ -189,13 +189,13 @@ __global__ __launch_bounds__(256) void k_bitmap(
     int nvals, const double *__restrict__ cols, uint64_t count, int ncols, uint64_t words,
     uint32_t *__restrict__ bitmap) {
     bitmap_rows(blockIdx.x, nprog, nops, ops, poff, plen, vals, nvals, cols, count, ncols,
-                words, bitmap, nullptr);
+                words, bitmap, nullptr, 0);
 }
 
 hipError_t launch_bitmap(FvGpuIndex *g, FvGpuBatch *b, hipStream_t s) {
     if (b->nprog == 0) return hipSuccess;
     const uint64_t words = g->count_pad / 32;
-    const unsigned blocks = (unsigned)((g->count_pad + kBmRows - 1) / kBmRows);
+    const unsigned blocks = bitmap_blocks(g);
     hipLaunchKernelGGL(k_bitmap, dim3(blocks), dim3(256), bitmap_lds_bytes(g->ncols), s,
                        b->d_ops, b->d_poff,
                        b->d_plen, b->nprog, b->nops, b->d_vals, b->nvals, g->d_cols, g->count,
