@@ -44,7 +44,7 @@
 #include <cmath>
 #include <vector>
 
-#include "pv_device.h"
+#include "pv_mlp.h"
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
@@ -136,9 +136,11 @@ __device__ __forceinline__ void prep_query(
 }
 
 /*
- * Phase 1 of a run, one launch: blocks [0, nbm) build the predicate
- * bitmaps (one row per thread, every distinct program per row), the rest
- * prepare the queries (one wave per padded query, four per block).
+ * Phase 1 of a run, one launch: blocks [0, nprep) prepare the queries (one
+ * wave per padded query, four per block; one block per query when they also
+ * compute the router features), the other nbm blocks build the predicate
+ * bitmaps (bitmap_rows: kBmRows rows per block, the waves split the
+ * distinct programs).
  */
 struct Phase1Args {
     const PvOp *ops;
@@ -149,7 +151,8 @@ struct Phase1Args {
     uint32_t *progpart;   /* optional: passing rows per program and bitmap
                              block, [nprog][nbm] */
     int nprog, nops, nvals, ncols;
-    uint32_t nbm;
+    uint32_t nbm;         /* bitmap blocks */
+    uint32_t nprep;       /* query prep blocks (first in the grid) */
     const float *q;
     int nq, nqp, dim;
     uint16_t *qbf;
@@ -169,40 +172,59 @@ struct Phase1Args {
 };
 
 __global__ __launch_bounds__(256) void k_phase1(Phase1Args A) {
-    if (blockIdx.x < A.nbm) {
-        bitmap_rows(blockIdx.x, A.nprog, A.nops, A.ops, A.poff, A.plen, A.vals, A.nvals, A.cols, A.count,
-                    A.ncols, A.words, A.bitmap, A.progpart, A.nbm);
+    /* the query prep blocks come first: dispatched before the bitmap
+     * blocks, they never wait for a free slot behind them */
+    if (blockIdx.x >= A.nprep) {
+        bitmap_rows(blockIdx.x - A.nprep, A.nprog, A.nops, A.ops, A.poff, A.plen, A.vals,
+                    A.nvals, A.cols, A.count, A.ncols, A.words, A.bitmap, A.progpart, A.nbm);
         return;
     }
-    const int w = threadIdx.x >> 6;
-    const int qi = (int)(blockIdx.x - A.nbm) * 4 + w;
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    if (A.feat == nullptr) {
+        /* query prep only: one wave per padded query */
+        const int qi = (int)blockIdx.x * 4 + w;
+        if (qi >= A.nqp) return;
+        prep_query(qi, A.q, A.nq, A.dim, A.qbf, A.qnorm, A.eps, A.xmax_norm, A.vmin, A.vmax,
+                   A.all_int, A.mode, A.qi8, A.qn8, A.centre, A.status);
+        return;
+    }
+    /* with router features: one block per padded query; wave 0 preps it,
+     * the four waves split the centroid distances (each keeps its kNear
+     * nearest) and wave 0 merges the 4 x kNear — the same kNear smallest
+     * distances as one wave over all centroids. The selectivity feature
+     * needs the finished bitmap counts and is added by the decision. */
+    const int qi = (int)blockIdx.x;
     if (qi >= A.nqp) return;
-    prep_query(qi, A.q, A.nq, A.dim, A.qbf, A.qnorm, A.eps, A.xmax_norm, A.vmin, A.vmax,
-               A.all_int, A.mode, A.qi8, A.qn8, A.centre, A.status);
-    if (A.feat != nullptr && qi < A.nq) {
-        /* router features while the bitmap blocks run (they only add the
-         * selectivity, which needs the finished bitmap counts) */
-        __shared__ float xq[4][256];
-        const int l = threadIdx.x & 63;
-        const float *qv = A.q + (uint64_t)qi * A.dim;
-        float *fo = A.feat + (uint64_t)qi * A.nfeat;
-        for (int j = l; j < A.dim; j += 64) {
-            xq[w][j] = qv[j];
-            fo[j] = qv[j];
-        }
-        wave_lds_sync();
-        float best[kNear];
+    if (w == 0)
+        prep_query(qi, A.q, A.nq, A.dim, A.qbf, A.qnorm, A.eps, A.xmax_norm, A.vmin, A.vmax,
+                   A.all_int, A.mode, A.qi8, A.qn8, A.centre, A.status);
+    if (qi >= A.nq) return; /* block-uniform */
+    __shared__ float xq[256];
+    __shared__ float wn[4][kNear], wm[kNear];
+    const float *qv = A.q + (uint64_t)qi * A.dim;
+    float *fo = A.feat + (uint64_t)qi * A.nfeat;
+    for (int j = threadIdx.x; j < A.dim; j += blockDim.x) {
+        xq[j] = qv[j];
+        fo[j] = qv[j];
+    }
+    __syncthreads();
+    float best[kNear];
+#pragma unroll
+    for (int k = 0; k < kNear; k++) best[k] = __uint_as_float(0x7f800000u);
+    for (int c0 = 64 * w; c0 < A.ncent; c0 += 256) {
+        const int c = min(c0 + l, A.ncent - 1);
+        const float v = cent_dist(xq, A.cent_t, c, A.dim, A.ncent);
+        near_insert(best, c0 + l < A.ncent ? v : __uint_as_float(0x7f800000u));
+    }
+    wave_nearest(best, wn[w]);
+    __syncthreads();
+    if (w == 0) {
 #pragma unroll
         for (int k = 0; k < kNear; k++) best[k] = __uint_as_float(0x7f800000u);
-        for (int c0 = 0; c0 < A.ncent; c0 += 64) {
-            const int c = min(c0 + l, A.ncent - 1);
-            const float v = cent_dist(xq[w], A.cent_t, c, A.dim, A.ncent);
-            near_insert(best, c0 + l < A.ncent ? v : __uint_as_float(0x7f800000u));
-        }
-        __shared__ float wn[4][kNear];
-        wave_nearest(best, wn[w]);
+        near_insert(best, wn[l / kNear][l % kNear]);
+        wave_nearest(best, wm);
         wave_lds_sync();
-        if (l < kNear) fo[A.dim + 1 + l] = sqrtf(wn[w][l]);
+        if (l < kNear) fo[A.dim + 1 + l] = sqrtf(wm[l]);
     }
 }
 
@@ -1233,18 +1255,30 @@ __device__ void block_bitonic(uint64_t *a, uint32_t n) {
 /* One block per query: tau = B-th smallest group-minimum key's distance,
  * widened by 2 eps and a relative 2^-20 slack; also clears the pool
  * counter and status (a fill, so the pipeline can be graph-captured). */
-__global__ __launch_bounds__(256) void k_tau(const uint64_t *__restrict__ gmin,
-                                             uint32_t ngroups, uint32_t B,
-                                             const float *__restrict__ eps,
-                                             float *__restrict__ tau,
-                                             uint32_t *__restrict__ pcnt,
-                                             uint32_t *__restrict__ status,
-                                             const uint8_t *__restrict__ skip,
-                                             const uint32_t *__restrict__ topbq) {
+struct TauArgs {
+    const uint64_t *gmin;
+    uint32_t ngroups, B;
+    const float *eps;
+    float *tau;
+    uint32_t *pcnt;
+    uint32_t *status;
+    const uint8_t *skip;
+    const uint32_t *topbq;
+};
+
+__device__ __forceinline__ void tau_block(const TauArgs &A, int q) {
     __shared__ uint32_t hist[256], tmp[8], bc[8];
     __shared__ uint64_t sel[1024], wb[4];
     __shared__ uint32_t nsel;
-    const int q = blockIdx.x;
+    const uint64_t *__restrict__ gmin = A.gmin;
+    const uint32_t ngroups = A.ngroups;
+    uint32_t B = A.B;
+    const float *__restrict__ eps = A.eps;
+    float *__restrict__ tau = A.tau;
+    uint32_t *__restrict__ pcnt = A.pcnt;
+    uint32_t *__restrict__ status = A.status;
+    const uint8_t *__restrict__ skip = A.skip;
+    const uint32_t *__restrict__ topbq = A.topbq;
     if (topbq) B = topbq[q];
     if (skip && skip[q]) { /* routed elsewhere: no candidates, no outputs */
         if (threadIdx.x == 0) {
@@ -1275,6 +1309,18 @@ __global__ __launch_bounds__(256) void k_tau(const uint64_t *__restrict__ gmin,
         clear_pcnt(pcnt, q);
         status[q] &= kStPrepFail;   /* written by this run's query prep */
     }
+}
+
+__global__ __launch_bounds__(256) void k_tau(TauArgs A) { tau_block(A, blockIdx.x); }
+
+/* a routed run's threshold launch: blocks [0, nq) compute every query's
+ * threshold (graph-routed ones included: the decision is made beside them,
+ * and the scan and select read it), the rest are the routing decision's
+ * blocks (four queries each) — one launch instead of two in the run's
+ * chain */
+__global__ __launch_bounds__(256) void k_tau_route(TauArgs A, LearnArgs R, int nq) {
+    if ((int)blockIdx.x < nq) tau_block(A, blockIdx.x);
+    else route_block(R, blockIdx.x - nq);
 }
 
 /* clears pool counters and status, tau = +inf (gather route / no pre-pass) */
@@ -1769,7 +1815,10 @@ hipError_t launch_phase1(FvGpuIndex *g, FvGpuBatch *b, int prec, bool bitmap, hi
             P.nfeat = pf->nfeat;
             P.feat = pf->feat;
         }
-        hipLaunchKernelGGL(k_phase1, dim3(P.nbm + (uint32_t)(P.nqp + 3) / 4), dim3(256),
+        /* query prep: four queries per block, or one per block with the
+         * router features */
+        P.nprep = P.feat ? (uint32_t)P.nqp : (uint32_t)(P.nqp + 3) / 4;
+        hipLaunchKernelGGL(k_phase1, dim3(P.nbm + P.nprep), dim3(256),
                            P.nbm ? bitmap_lds_bytes(P.ncols) : 0, s, P);
         return hipGetLastError();
     }
@@ -1892,7 +1941,7 @@ static hipError_t launch_main_scan(const ScanLaunch &L, hipStream_t s) {
  * sample stride (1 = every row: the tightest threshold). */
 int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
                  uint32_t stride_override, const uint8_t *skip, bool have_phase1,
-                 char *errbuf, size_t errlen) {
+                 char *errbuf, size_t errlen, const LearnArgs *decide) {
     hipStream_t s = b->stream;
     g->ev_stream = s;
     b->out_stride = b->topb; /* a graph run may have widened it; d_out holds nq x topb */
@@ -1957,6 +2006,12 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
                            b->d_pofq, b->d_vals, b->d_pool.as<uint64_t>(),
                            b->d_pcnt.as<uint32_t>(), b->cap_pool);
         PV_TRY(hipGetLastError());
+        if (decide) {
+            /* no threshold launch on this route: the decision blocks alone */
+            hipLaunchKernelGGL(k_tau_route, dim3((nq + 3) / 4), dim3(256), 0, s, TauArgs{},
+                               *decide, 0);
+            PV_TRY(hipGetLastError());
+        }
         ev_record(g, 4);
         /* gathered distances are already exact: eps = 0 */
         hipLaunchKernelGGL(k_select, dim3(nq), dim3(256), dim * 4, s,
@@ -2044,9 +2099,11 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
             PV_TRY(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm_bf));
     }
     if (bf && shm_bf > 160 * 1024) PV_FAIL("gpu: dim %d too large for the BF16 scan", dim);
-    /* pre-pass over sampled tiles */
+    /* pre-pass over sampled tiles (with a decision made in this run's
+     * threshold launch, the route is not known yet: no skipping) */
     {
         ScanArgs P = A;
+        if (decide) P.skip = nullptr;
         P.ntiles = stiles;
         P.stride = stride;
         if (bf || i8) {
@@ -2064,9 +2121,23 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
         PV_TRY(hipGetLastError());
     }
     ev_record(g, 2);
-    hipLaunchKernelGGL(k_tau, dim3(nq), dim3(256), 0, s, b->d_gmin.as<uint64_t>(),
-                       b->ngroups, B, qa.eps, qa.tau, b->d_pcnt.as<uint32_t>(),
-                       qa.status, skip, b->d_topbq_p);
+    {
+        TauArgs T;
+        T.gmin = b->d_gmin.as<uint64_t>();
+        T.ngroups = b->ngroups;
+        T.B = B;
+        T.eps = qa.eps;
+        T.tau = qa.tau;
+        T.pcnt = b->d_pcnt.as<uint32_t>();
+        T.status = qa.status;
+        T.skip = decide ? nullptr : skip;
+        T.topbq = b->d_topbq_p;
+        if (decide)
+            hipLaunchKernelGGL(k_tau_route, dim3(nq + (nq + 3) / 4), dim3(256), 0, s, T, *decide,
+                               nq);
+        else
+            hipLaunchKernelGGL(k_tau, dim3(nq), dim3(256), 0, s, T);
+    }
     PV_TRY(hipGetLastError());
     ev_record(g, 3);
     {

@@ -305,10 +305,13 @@ struct GraphExtra {
 int run_graph_ex(FvGpuIndex *g, FvGpuBatch *b, uint32_t ef, uint64_t max_visited, int wmax,
                  int fcap, uint32_t *dbg, const GraphExtra &x, char *errbuf, size_t errlen);
 /* pv_brute.hip: the brute pipeline; skip (device [nq], may be NULL) marks
- * queries it must leave alone; have_phase1 = bitmaps and query prep done */
+ * queries it must leave alone; have_phase1 = bitmaps and query prep done;
+ * decide (may be NULL): the routing decision, run in the threshold launch,
+ * that writes skip */
+struct LearnArgs;
 int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
                  uint32_t stride_override, const uint8_t *skip, bool have_phase1,
-                 char *errbuf, size_t errlen);
+                 char *errbuf, size_t errlen, const LearnArgs *decide = nullptr);
 /* router features computed by the query prep of phase 1 (dim <= 256) */
 struct PrepFeatures {
     const float *cent_t;
