@@ -531,62 +531,31 @@ __device__ __forceinline__ void load_tile_in(const ScanArgs &A, uint32_t pt,
 }
 
 /*
- * Consume one 32-row x 64-query tile held in the MFMA C layout: lane l owns
- * queries (l & 31) + 32 qb and the 16 rows acc_row(i, l). BF: distances are
- * ||x||^2 + ||q||^2 - 2 x.q (clamped at 0) from dot products; otherwise the
- * accumulators already hold sum (x - q)^2. PREPASS writes the minimum
- * passing distance of the lane's 16 rows per query (one group); MAIN
- * appends passing rows with dist <= tau to the WG's candidates.
+ * Consume one 32-row x 64-query tile of FP32 distances sum (x - q)^2 held
+ * in the MFMA C layout (the plain FP32 scan): lane l owns queries (l & 31)
+ * + 32 qb and the 16 rows acc_row(i, l). PREPASS writes the minimum passing
+ * distance of the lane's 16 rows per query (one group); MAIN appends
+ * passing rows with dist <= tau to the WG's candidates.
  */
-template <bool PREPASS, bool BF>
+template <bool PREPASS>
 __device__ __forceinline__ void tile_epilogue(const ScanArgs &A, int qbase, uint32_t t,
                                               uint32_t pt, const float (&acc)[2][16],
-                                              const TileIn &in, const float (&xn)[16],
-                                              const float (&qn)[2], const float (&tq)[2],
-                                              bool active, CandLds &C) {
+                                              const TileIn &in, const float (&tq)[2],
+                                              CandLds &C) {
     const int l = threadIdx.x & 63;
 #pragma unroll
     for (int qb = 0; qb < 2; qb++) {
         const int qi = (l & 31) + 32 * qb;
         const int q = qbase + qi;
-        if (q >= A.nq || !active) continue;
+        if (q >= A.nq) continue;
         const uint32_t word = in.w[qb] & in.valid;
         if (PREPASS) {
             float m = __uint_as_float(0x7f800000u);
 #pragma unroll
-            for (int i = 0; i < 16; i++) {
-                const float d = BF ? fmaxf(xn[i] + qn[qb] - 2.f * acc[qb][i], 0.f) : acc[qb][i];
-                if ((word >> acc_row(i, l)) & 1u) m = fminf(m, d);
-            }
+            for (int i = 0; i < 16; i++)
+                if ((word >> acc_row(i, l)) & 1u) m = fminf(m, acc[qb][i]);
             const uint32_t grp = t * 2 + (l >> 5);
             A.gmin[(uint64_t)q * A.ngroups + grp] = make_key(m, grp);
-        } else if (BF) {
-            /* d' = ||x||^2 + ||q||^2 - 2 x.q <= tau  <=>  x.q >= ||x||^2 / 2 + h
-             * with h = (||q||^2 - tau) / 2: one FMA and one compare per
-             * element. The two forms round differently by a few ulps of
-             * ||x||^2 + ||q||^2; h is lowered by 2^-19 (max ||x||^2 + ||q||^2),
-             * far above that, so every row with d' <= tau is admitted (a
-             * few more in the margin band are harmless: the select stage
-             * certifies). The per-lane OR of the 16 tests stays a lane mask,
-             * and the divergent branch on it skips the insertion code for
-             * the whole wave when no lane has a candidate among its rows. */
-            const float h = fmaf(0.5f, qn[qb] - tq[qb], -ldexpf(A.xn2max + qn[qb], -19));
-            bool c[16];
-            bool anyl = false;
-#pragma unroll
-            for (int i = 0; i < 16; i++) {
-                c[i] = ((word >> acc_row(i, l)) & 1u) & (acc[qb][i] >= fmaf(0.5f, xn[i], h));
-                anyl |= c[i];
-            }
-            if (anyl) {
-#pragma unroll
-                for (int i = 0; i < 16; i++)
-                    if (c[i]) {
-                        const float d = fmaxf(xn[i] + qn[qb] - 2.f * acc[qb][i], 0.f);
-                        insert_candidate(A, qbase, qi, make_key(d, (uint32_t)((uint64_t)pt * 32 +
-                                                                    acc_row(i, l))), C);
-                    }
-            }
         } else {
             const float tau = tq[qb];
 #pragma unroll
@@ -597,6 +566,157 @@ __device__ __forceinline__ void tile_epilogue(const ScanArgs &A, int qbase, uint
                     insert_candidate(A, qbase, qi,
                                      make_key(d, (uint32_t)((uint64_t)pt * 32 + r)), C);
             }
+        }
+    }
+}
+
+/*
+ * Seeded epilogue of the MFMA scans. The accumulators of a tile start from
+ * a per-element seed instead of zero, which folds in the row norm and the
+ * query's filter:
+ *   BF16: -||x_r||^2 / 2 where row r passes query q's filter, -inf
+ *         otherwise; then acc = x.q - ||x||^2 / 2 and d = ||q||^2 - 2 acc;
+ *   INT8: -floor(||x_r||^2 / 2), or kI8Off (far below any real value);
+ *         then acc = x.q - floor(||x||^2 / 2) (exact, int32) and
+ *         d = ||q||^2 + (||x||^2 mod 2) - 2 acc.
+ * d <= tau becomes acc >= h(q), one threshold per query, so the lane's 16
+ * elements of a query reduce to their maximum before the one comparison;
+ * the filter test and the norm term are paid once per element, in the
+ * seed (an extract of the filter bit and a bit-select).
+ */
+constexpr int32_t kI8Off = -(1 << 30);
+
+/* the lane's filter word, shifted so that element i's row is at bit
+ * (i >> 2) * 8 + (i & 3) */
+__device__ __forceinline__ uint32_t lane_word(uint32_t w) {
+    return w >> (((threadIdx.x & 63) >> 5) * 4);
+}
+
+/* all ones if element i's row passes (bit of a lane_word), else zero */
+__device__ __forceinline__ uint32_t elem_mask(uint32_t wa, int i) {
+    const int p = (i >> 2) * 8 + (i & 3);
+    return (uint32_t)((int32_t)(wa << (31 - p)) >> 31);
+}
+
+/* BF16 seeds of a tile: xn from expand_norms, w0 / w1 the query blocks'
+ * filter words with the validity mask applied */
+__device__ __forceinline__ void seed_bf(const float (&xn)[16], uint32_t w0, uint32_t w1,
+                                        f32x16 &a0, f32x16 &a1) {
+    const uint32_t v0 = lane_word(w0), v1 = lane_word(w1);
+    constexpr uint32_t kNegInf = 0xff800000u;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        const uint32_t s = __float_as_uint(-0.5f * xn[i]);
+        const uint32_t m0 = elem_mask(v0, i), m1 = elem_mask(v1, i);
+        a0[i] = __uint_as_float((s & m0) | (kNegInf & ~m0));
+        a1[i] = __uint_as_float((s & m1) | (kNegInf & ~m1));
+    }
+}
+
+/* INT8 seeds; par gets bit i = ||x||^2 of element i's row is odd */
+__device__ __forceinline__ void seed_i8(const float (&xn)[16], uint32_t w0, uint32_t w1,
+                                        i32x16 &c0, i32x16 &c1, uint32_t &par) {
+    const uint32_t v0 = lane_word(w0), v1 = lane_word(w1);
+    par = 0;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        const int32_t xi = (int32_t)xn[i];
+        const uint32_t s = (uint32_t)(-(xi >> 1));
+        par |= (uint32_t)(xi & 1) << i;
+        const uint32_t m0 = elem_mask(v0, i), m1 = elem_mask(v1, i);
+        c0[i] = (int32_t)((s & m0) | ((uint32_t)kI8Off & ~m0));
+        c1[i] = (int32_t)((s & m1) | ((uint32_t)kI8Off & ~m1));
+    }
+}
+
+/* BF16: the admission threshold of a query: acc >= h admits every row whose
+ * d = ||q||^2 - 2 acc (rounded) is <= tau. The two forms round differently
+ * by a few ulps of ||x||^2 + ||q||^2; h is lowered by 2^-19 (max ||x||^2 +
+ * ||q||^2), far above that (a few more rows in the margin band are
+ * harmless: the select stage certifies). Kept finite so that -inf seeds
+ * (filtered rows) never pass, also when tau = +inf admits every row. */
+__device__ __forceinline__ float bf_threshold(const ScanArgs &A, float qn, float tq) {
+    const float h = fmaf(0.5f, qn - tq, -ldexpf(A.xn2max + qn, -19));
+    return fmaxf(h, -__FLT_MAX__);
+}
+
+/* INT8: d <= tau  =>  2 acc >= ||q||^2 - tau + parity  =>  acc >= ceil(ceil(
+ * ||q||^2 - tau) / 2): a superset (rows up to 2 units beyond tau), exact in
+ * integers; clamped between the filtered seed and any real value */
+__device__ __forceinline__ int32_t i8_threshold(float qn, float tq) {
+    const double k = ceil((double)qn - (double)tq);
+    const double tt = ceil(k * 0.5);
+    return (int32_t)fmin(fmax(tt, (double)(kI8Off / 2)), (double)(1 << 30));
+}
+
+template <bool PREPASS>
+__device__ __forceinline__ void epilogue_bf(const ScanArgs &A, int qbase, uint32_t t,
+                                            uint32_t pt, const f32x16 &a0, const f32x16 &a1,
+                                            const float (&qn)[2], const float (&tq)[2],
+                                            CandLds &C) {
+    const int l = threadIdx.x & 63;
+#pragma unroll
+    for (int qb = 0; qb < 2; qb++) {
+        const f32x16 &a = qb ? a1 : a0;
+        const int qi = (l & 31) + 32 * qb;
+        const int q = qbase + qi;
+        if (q >= A.nq) continue;
+        float mx = a[0];
+#pragma unroll
+        for (int i = 1; i < 16; i++) mx = fmaxf(mx, a[i]);
+        if (PREPASS) {
+            /* all rows filtered: mx = -inf, d = +inf */
+            const float d = fmaxf(fmaf(-2.f, mx, qn[qb]), 0.f);
+            const uint32_t grp = t * 2 + (l >> 5);
+            A.gmin[(uint64_t)q * A.ngroups + grp] = make_key(d, grp);
+        } else {
+            const float h = bf_threshold(A, qn[qb], tq[qb]);
+            /* lane-divergent: the wave skips the insertion code when no
+             * lane has a candidate among its rows */
+            if (mx >= h) {
+#pragma unroll
+                for (int i = 0; i < 16; i++)
+                    if (a[i] >= h) {
+                        const float d = fmaxf(fmaf(-2.f, a[i], qn[qb]), 0.f);
+                        insert_candidate(A, qbase, qi, make_key(d, (uint32_t)((uint64_t)pt * 32 +
+                                                                    acc_row(i, l))), C);
+                    }
+            }
+        }
+    }
+}
+
+template <bool PREPASS>
+__device__ __forceinline__ void epilogue_i8(const ScanArgs &A, int qbase, uint32_t t,
+                                            uint32_t pt, const i32x16 &c0, const i32x16 &c1,
+                                            uint32_t par, const float (&qn)[2],
+                                            const int32_t (&th)[2], CandLds &C) {
+    const int l = threadIdx.x & 63;
+#pragma unroll
+    for (int qb = 0; qb < 2; qb++) {
+        const i32x16 &a = qb ? c1 : c0;
+        const int qi = (l & 31) + 32 * qb;
+        const int q = qbase + qi;
+        if (q >= A.nq) continue;
+        int32_t mx = a[0];
+#pragma unroll
+        for (int i = 1; i < 16; i++) mx = max(mx, a[i]);
+        const int32_t qni = (int32_t)qn[qb];
+        if (PREPASS) {
+            /* an upper bound of the group's minimum (parity taken as 1):
+             * at least as large as a real row's distance, at most 1 more */
+            const float d = mx > kI8Off / 2 ? (float)(qni + 1 - 2 * mx)
+                                             : __uint_as_float(0x7f800000u);
+            const uint32_t grp = t * 2 + (l >> 5);
+            A.gmin[(uint64_t)q * A.ngroups + grp] = make_key(d, grp);
+        } else if (mx >= th[qb]) {
+#pragma unroll
+            for (int i = 0; i < 16; i++)
+                if (a[i] >= th[qb]) {
+                    const float d = (float)(qni + (int32_t)((par >> i) & 1u) - 2 * a[i]);
+                    insert_candidate(A, qbase, qi, make_key(d, (uint32_t)((uint64_t)pt * 32 +
+                                                                acc_row(i, l))), C);
+                }
         }
     }
 }
@@ -721,8 +841,17 @@ void k_scan_bf16(ScanArgs A) {
     auto do_unit = [&](const u32x4 (&a)[8], const TileIn &in, f32x16 &acc0, f32x16 &acc1) {
         const uint32_t c = in.c;
         if (c == 0) {
-            acc0 = f32x16{};
-            acc1 = f32x16{};
+            /* seeds: row norms and the queries' filters (epilogue_bf) */
+            float xn[16];
+            expand_norms(in.xr, xn);
+            uint32_t w0 = in.w[0], w1 = in.w[1];
+            if (INL) {
+                double cv[16];
+                expand_rows(in.rv, cv);
+                w0 = inline_word(cv, s_im, s_iv, 0, 0);
+                w1 = inline_word(cv, s_im, s_iv, 0, 1);
+            }
+            seed_bf(xn, w0 & in.valid, w1 & in.valid, acc0, acc1);
         }
 #pragma unroll
         for (int j = 0; j < 8; j++) {
@@ -733,29 +862,8 @@ void k_scan_bf16(ScanArgs A) {
             acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, b0, acc0, 0, 0, 0);
             acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, b1, acc1, 0, 0, 0);
         }
-        if (c == NCH - 1) {
-            float acc[2][16];
-#pragma unroll
-            for (int i = 0; i < 16; i++) {
-                acc[0][i] = acc0[i];
-                acc[1][i] = acc1[i];
-            }
-            const uint32_t t = in.t;
-            float xn[16];
-            expand_norms(in.xr, xn);
-            if (INL) {
-                double cv[16];
-                expand_rows(in.rv, cv);
-                TileIn v = in;
-                v.w[0] = inline_word(cv, s_im, s_iv, 0, 0);
-                v.w[1] = inline_word(cv, s_im, s_iv, 0, 1);
-                tile_epilogue<PREPASS, true>(A, qbase, t, t * A.stride, acc, v, xn, qn, tq,
-                                             true, C);
-            } else {
-                tile_epilogue<PREPASS, true>(A, qbase, t, t * A.stride, acc, in, xn, qn, tq,
-                                             true, C);
-            }
-        }
+        if (c == NCH - 1)
+            epilogue_bf<PREPASS>(A, qbase, in.t, in.t * A.stride, acc0, acc1, qn, tq, C);
     };
 
     f32x16 acc0 = {}, acc1 = {};
@@ -851,6 +959,7 @@ void k_scan_bf16_mq(ScanArgs A) {
         const uint32_t t = in[0].t;
         float xn[16];
         expand_norms(in[0].xr, xn);
+        const uint32_t valid = in[0].valid;
         uint32_t iw[QTW][2];
         if (INL) {
             double cv[16];
@@ -864,7 +973,11 @@ void k_scan_bf16_mq(ScanArgs A) {
 #pragma unroll
         for (int k = 0; k < QTW; k++) {
             if (!qon[k]) continue;
-            f32x16 acc0 = {}, acc1 = {};
+            /* the tile's norms, validity and inline column values are shared
+             * by the query tiles; the bitmap words are per query tile */
+            f32x16 acc0, acc1;
+            seed_bf(xn, (INL ? iw[k][0] : in[k].w[0]) & valid, (INL ? iw[k][1] : in[k].w[1]) & valid,
+                    acc0, acc1);
 #pragma unroll
             for (int j = 0; j < 8; j++) {
                 const bf16x8 av = __builtin_bit_cast(bf16x8, a[j]);
@@ -873,19 +986,7 @@ void k_scan_bf16_mq(ScanArgs A) {
                 acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, b0, acc0, 0, 0, 0);
                 acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, b1, acc1, 0, 0, 0);
             }
-            float acc[2][16];
-#pragma unroll
-            for (int i = 0; i < 16; i++) {
-                acc[0][i] = acc0[i];
-                acc[1][i] = acc1[i];
-            }
-            /* the tile's norms, validity and inline column values are shared
-             * by the query tiles; the bitmap words are per query tile */
-            TileIn v = in[0];
-            v.w[0] = INL ? iw[k][0] : in[k].w[0];
-            v.w[1] = INL ? iw[k][1] : in[k].w[1];
-            tile_epilogue<PREPASS, true>(A, qb0[k], t, t * A.stride, acc, v, xn, qn[k], tq[k],
-                                         true, Cs[k]);
+            epilogue_bf<PREPASS>(A, qb0[k], t, t * A.stride, acc0, acc1, qn[k], tq[k], Cs[k]);
         }
     };
 
@@ -957,6 +1058,7 @@ void k_scan_i8(ScanArgs A) {
     float tq[2], qn[2];
     const uint32_t *wr[2];
     load_query_state(A, qbase, tq, qn, wr);
+    const int32_t th[2] = {i8_threshold(qn[0], tq[0]), i8_threshold(qn[1], tq[1])};
     auto load_in = [&](TileIn (&in)[PT]) {
         const uint32_t u = phys(cur.tile);
 #pragma unroll
@@ -976,7 +1078,19 @@ void k_scan_i8(ScanArgs A) {
         const uint32_t u = phys(in[0].t);
 #pragma unroll
         for (int p = 0; p < PT; p++) {
-            i32x16 c0 = {}, c1 = {};
+            /* seeds: row norms and the queries' filters (epilogue_i8) */
+            float xn[16];
+            expand_norms(in[p].xr, xn);
+            uint32_t w0 = in[p].w[0], w1 = in[p].w[1];
+            if (INL) {
+                double cv[16];
+                expand_rows(in[p].rv, cv);
+                w0 = inline_word(cv, s_im, s_iv, 0, 0);
+                w1 = inline_word(cv, s_im, s_iv, 0, 1);
+            }
+            i32x16 c0, c1;
+            uint32_t par;
+            seed_i8(xn, w0 & in[p].valid, w1 & in[p].valid, c0, c1, par);
 #pragma unroll
             for (int k = 0; k < KS; k++) {
                 const i32x4 av = __builtin_bit_cast(i32x4, a[p * KS + k]);
@@ -985,27 +1099,10 @@ void k_scan_i8(ScanArgs A) {
                 c0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, b0, c0, 0, 0, 0);
                 c1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, b1, c1, 0, 0, 0);
             }
-            float acc[2][16];
-#pragma unroll
-            for (int i = 0; i < 16; i++) {
-                acc[0][i] = (float)c0[i];
-                acc[1][i] = (float)c1[i];
-            }
             /* group index of the pre-pass: sampled tile x PT + p */
             const uint32_t tg = in[p].t * PT + p;
             const uint32_t pt = u * PT + p;
-            float xn[16];
-            expand_norms(in[p].xr, xn);
-            if (INL) {
-                double cv[16];
-                expand_rows(in[p].rv, cv);
-                TileIn v = in[p];
-                v.w[0] = inline_word(cv, s_im, s_iv, 0, 0);
-                v.w[1] = inline_word(cv, s_im, s_iv, 0, 1);
-                tile_epilogue<PREPASS, true>(A, qbase, tg, pt, acc, v, xn, qn, tq, true, C);
-            } else {
-                tile_epilogue<PREPASS, true>(A, qbase, tg, pt, acc, in[p], xn, qn, tq, true, C);
-            }
+            epilogue_i8<PREPASS>(A, qbase, tg, pt, c0, c1, par, qn, th, C);
         }
     };
 
@@ -1098,8 +1195,7 @@ __global__ __launch_bounds__(256, 2) void k_scan_f32(ScanArgs A) {
         if (live) {
             TileIn in;
             load_tile_in(A, pt, wr, in);
-            const float xn[16] = {};
-            tile_epilogue<PREPASS, false>(A, qbase, t, pt, acc, in, xn, qn, tq, true, C);
+            tile_epilogue<PREPASS>(A, qbase, t, pt, acc, in, tq, C);
         }
     }
     if (!PREPASS) flush_candidates(A, qbase, C);
