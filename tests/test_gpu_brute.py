@@ -167,6 +167,15 @@ def test_edge_cases():
         assert cnt[1] == 0 and trunc[1] == 0
     finally:
         g.close()
+    # the INT8 scan (an SQ8 index over these integer rows): seeds of filtered
+    # rows must stay out also where a query's threshold admits every row
+    g8 = E.GpuIndex(d["base"], cols, rows, precision=L.FV_GPU_PREC_SQ8)
+    try:
+        out, cnt, trunc = g8.brute(d["queries"], b, 20, L.BruteOpts(route=1))
+        assert g8.stats().precision == L.FV_GPU_PREC_SQ8
+        assert_same(out, cnt, trunc, oh, oc, ot, "edge/sq8")
+    finally:
+        g8.close()
 
 
 def test_exact_ties_broken_by_id():
