@@ -47,12 +47,28 @@ __device__ __forceinline__ float wave_sum(float v) {
 /* the BF16 inference parameters staged in LDS when they fit (dim <= 156) */
 constexpr int kLdsParams = 16384;
 
-/* all threads of the block copy n BF16 parameters (n padded to 8) to LDS */
+/* all threads of the block copy n BF16 parameters (n padded to 8) to LDS;
+ * a thread's loads are all issued before its LDS writes (one memory round
+ * trip for a 256-thread block) */
 __device__ __forceinline__ void stage_params(const uint16_t *__restrict__ wt, int n,
                                              uint16_t *ws) {
+    constexpr int kPer = 8;
     const uint4 *src = (const uint4 *)wt;
     uint4 *dst = (uint4 *)ws;
-    for (int i = threadIdx.x; i < (n + 7) / 8; i += blockDim.x) dst[i] = src[i];
+    const int n16 = (n + 7) / 8;
+    for (int base = 0; base < n16; base += kPer * blockDim.x) {
+        uint4 v[kPer];
+#pragma unroll
+        for (int k = 0; k < kPer; k++) {
+            const int i = base + k * blockDim.x + threadIdx.x;
+            if (i < n16) v[k] = src[i];
+        }
+#pragma unroll
+        for (int k = 0; k < kPer; k++) {
+            const int i = base + k * blockDim.x + threadIdx.x;
+            if (i < n16) dst[i] = v[k];
+        }
+    }
 }
 
 /*
@@ -61,25 +77,48 @@ __device__ __forceinline__ void stage_params(const uint16_t *__restrict__ wt, in
  * and W2t[64][64], so step i reads 64 consecutive values). xs holds the raw
  * features (standardised here into xh), hs is 64 floats of LDS.
  */
-__device__ inline void wave_mlp(const float *xs, int F, const uint16_t *wt,
-                         const float *__restrict__ mean, const float *__restrict__ istd,
-                         float *xh, float *hs, float &y0, float &y1) {
+/* the MLP on standardised inputs xh (LDS) */
+__device__ inline void wave_mlp_std(const float *xh, int F, const uint16_t *wt, float *hs,
+                                    float &y0, float &y1) {
     const int l = threadIdx.x & 63;
     const MlpLayout L = mlp_layout(F);
-    for (int j = l; j < F; j += 64) xh[j] = (xs[j] - mean[j]) * istd[j];
-    wave_lds_sync();
-    float a = bf2f(wt[L.b1 + l]);
-#pragma unroll 8
-    for (int i = 0; i < F; i++) a = fmaf(bf2f(wt[L.w1 + i * kHid + l]), xh[i], a);
+    /* four partial sums over i mod 4 (independent FMA chains), summed as
+     * ((s0 + s1) + (s2 + s3)) + bias: the one order of every inference path */
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+    int i = 0;
+#pragma unroll 4
+    for (; i + 4 <= F; i += 4) {
+        s0 = fmaf(bf2f(wt[L.w1 + (i + 0) * kHid + l]), xh[i + 0], s0);
+        s1 = fmaf(bf2f(wt[L.w1 + (i + 1) * kHid + l]), xh[i + 1], s1);
+        s2 = fmaf(bf2f(wt[L.w1 + (i + 2) * kHid + l]), xh[i + 2], s2);
+        s3 = fmaf(bf2f(wt[L.w1 + (i + 3) * kHid + l]), xh[i + 3], s3);
+    }
+    for (; i < F; i++) s0 = fmaf(bf2f(wt[L.w1 + i * kHid + l]), xh[i], s0);
+    const float a = ((s0 + s1) + (s2 + s3)) + bf2f(wt[L.b1 + l]);
     hs[l] = fmaxf(a, 0.f);
     wave_lds_sync();
-    float b = bf2f(wt[L.b2 + l]);
-#pragma unroll 8
-    for (int i = 0; i < kHid; i++) b = fmaf(bf2f(wt[L.w2 + i * kHid + l]), hs[i], b);
+    float t0 = 0.f, t1 = 0.f, t2 = 0.f, t3 = 0.f;
+#pragma unroll 4
+    for (int k = 0; k < kHid; k += 4) {
+        t0 = fmaf(bf2f(wt[L.w2 + (k + 0) * kHid + l]), hs[k + 0], t0);
+        t1 = fmaf(bf2f(wt[L.w2 + (k + 1) * kHid + l]), hs[k + 1], t1);
+        t2 = fmaf(bf2f(wt[L.w2 + (k + 2) * kHid + l]), hs[k + 2], t2);
+        t3 = fmaf(bf2f(wt[L.w2 + (k + 3) * kHid + l]), hs[k + 3], t3);
+    }
+    float b = ((t0 + t1) + (t2 + t3)) + bf2f(wt[L.b2 + l]);
     b = fmaxf(b, 0.f);
     y0 = wave_sum(bf2f(wt[L.w3 + l]) * b) + bf2f(wt[L.b3 + 0]);
     y1 = wave_sum(bf2f(wt[L.w3 + kHid + l]) * b) + bf2f(wt[L.b3 + 1]);
     wave_lds_sync();
+}
+
+__device__ inline void wave_mlp(const float *xs, int F, const uint16_t *wt,
+                                const float *__restrict__ mean, const float *__restrict__ istd,
+                                float *xh, float *hs, float &y0, float &y1) {
+    const int l = threadIdx.x & 63;
+    for (int j = l; j < F; j += 64) xh[j] = (xs[j] - mean[j]) * istd[j];
+    wave_lds_sync();
+    wave_mlp_std(xh, F, wt, hs, y0, y1);
 }
 
 /* arguments of a routed run's decision: features, model, policy, and the
@@ -109,8 +148,19 @@ __device__ __forceinline__ double wave_pass_count(const LearnArgs &A, int qi) {
     const int p = A.pofq[qi];
     if (p < 0) return (double)A.count;
     const uint32_t *part = A.progpart + (uint64_t)p * A.nbm;
+    const uint32_t l = threadIdx.x & 63;
     uint32_t c = 0;
-    for (uint32_t i = threadIdx.x & 63; i < A.nbm; i += 64) c += part[i];
+    /* 16 loads in flight per lane per step (one round trip per 1M rows) */
+    for (uint32_t base = 0; base < A.nbm; base += 64 * 16) {
+        uint32_t v[16];
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            const uint32_t i = base + 64 * k + l;
+            v[k] = i < A.nbm ? part[i] : 0u;
+        }
+#pragma unroll
+        for (int k = 0; k < 16; k++) c += v[k];
+    }
     for (int o = 32; o; o >>= 1) c += __shfl_xor(c, o);
     return (double)c;
 }
@@ -137,30 +187,43 @@ __device__ __forceinline__ void route_decision(const LearnArgs &A, int qi, float
  * then the MLP and route/ef.
  */
 __device__ __forceinline__ void route_block(const LearnArgs &A, int blk) {
+    constexpr int kFeat = 256 + 1 + kNear; /* F <= dim + 17 with dim <= 256 */
+    constexpr int kPerLane = (kFeat + 63) / 64;
     __shared__ __attribute__((aligned(16))) uint16_t ws[kLdsParams];
-    __shared__ __attribute__((aligned(16))) float xs[4][kMaxFeat];
-    __shared__ float xh[4][kMaxFeat];
+    __shared__ float xh[4][kFeat];
     __shared__ float hs[4][kHid];
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
     const int qi = blk * 4 + w;
     const int np = mlp_layout(A.F).n;
     const bool lds = np <= kLdsParams;
-    if (lds) stage_params(A.wt, np, ws);
+    /* every global read of the block is issued before the first wait:
+     * features and standardisation into registers, the parameters into
+     * LDS, the per-block counts (after the query's program index) */
     const bool live = qi < A.nq;
-    const double pass = live ? wave_pass_count(A, qi) : 0.0;
     float *fq = A.feat_out + (uint64_t)(live ? qi : 0) * A.F;
-    if (live)
-        for (int j = l; j < A.F; j += 64) xs[w][j] = fq[j];
+    float fv[kPerLane], mu[kPerLane], is[kPerLane];
+#pragma unroll
+    for (int k = 0; k < kPerLane; k++) {
+        const int j = l + 64 * k;
+        const bool in = live && j < A.F;
+        fv[k] = in ? fq[j] : 0.f;
+        mu[k] = in ? A.mean[j] : 0.f;
+        is[k] = in ? A.istd[j] : 0.f;
+    }
+    if (lds) stage_params(A.wt, np, ws);
+    const double pass = live ? wave_pass_count(A, qi) : 0.0;
     __syncthreads(); /* every wave reaches it (no early return above) */
     if (!live) return;
-    if (l == 0) {
-        const float sel = (float)log10(fmax(pass, 1.0) / (double)A.count);
-        xs[w][A.dim] = sel;
-        fq[A.dim] = sel;
+    const float sel = (float)log10(fmax(pass, 1.0) / (double)A.count);
+#pragma unroll
+    for (int k = 0; k < kPerLane; k++) {
+        const int j = l + 64 * k;
+        if (j < A.F) xh[w][j] = ((j == A.dim ? sel : fv[k]) - mu[k]) * is[k];
     }
+    if (l == 0) fq[A.dim] = sel;
     wave_lds_sync();
     float y0, y1;
-    wave_mlp(xs[w], A.F, lds ? ws : A.wt, A.mean, A.istd, xh[w], hs[w], y0, y1);
+    wave_mlp_std(xh[w], A.F, lds ? ws : A.wt, hs[w], y0, y1);
     if (l == 0) {
         if (A.y_out) {
             A.y_out[(uint64_t)qi * 2 + 0] = y0;
