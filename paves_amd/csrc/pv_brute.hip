@@ -267,6 +267,7 @@ struct ScanArgs {
     const InlPred *inl;       /* inline predicates [nqp] (INL kernels) */
     const double *icol;       /* their column [count] */
     float xn2max;             /* max ||x||^2 over the rows (candidate-test margin) */
+    uint32_t nchunk, ngroup;  /* query-major grid: row chunks, query groups */
     uint32_t qmajor;          /* grid (query tiles, row chunks): the query tiles
                                  reading one chunk are dispatched together */
     uint32_t sweep;           /* tile order: 0 = each WG streams its own contiguous
@@ -278,14 +279,24 @@ struct ScanArgs {
  * a scan WG: grid (chunks, query tiles), or (query tiles, chunks) when
  * qmajor so that the query tiles sharing a chunk run together and read it
  * once from HBM (later reads hit in L2 / the last-level cache) */
+/* Query-major grid (1D): workgroups are dispatched to the XCDs round-robin
+ * (block b on XCD b % kXcds), so the ngroup query groups of chunk c are the
+ * blocks kXcds (k ngroup + g) + c % kXcds with k = c / kXcds: all on chunk
+ * c's XCD and consecutive there, so the chunk is read into that XCD's L2
+ * once and reused by every query group. Blocks past the last chunk exit. */
+constexpr uint32_t kXcds = 8;
 __device__ __forceinline__ uint32_t scan_chunk(const ScanArgs &A) {
-    return A.qmajor ? blockIdx.y : blockIdx.x;
+    return A.qmajor ? (blockIdx.x / kXcds / A.ngroup) * kXcds + blockIdx.x % kXcds : blockIdx.x;
 }
 __device__ __forceinline__ uint32_t scan_nchunks(const ScanArgs &A) {
-    return A.qmajor ? gridDim.y : gridDim.x;
+    return A.qmajor ? A.nchunk : gridDim.x;
 }
 __device__ __forceinline__ uint32_t scan_qgroup(const ScanArgs &A) {
-    return A.qmajor ? blockIdx.x : blockIdx.y;
+    return A.qmajor ? (blockIdx.x / kXcds) % A.ngroup : blockIdx.y;
+}
+/* a padding block of the query-major grid (uniform) */
+__device__ __forceinline__ bool scan_pad(const ScanArgs &A) {
+    return A.qmajor && scan_chunk(A) >= A.nchunk;
 }
 
 /* the main-scan launch of a batch's last full-scan run, kept so the scan
@@ -794,6 +805,7 @@ void k_scan_bf16(ScanArgs A) {
     CandLds &C = *(CandLds *)(smem + (size_t)2 * S * 64 * 16);
 
     const int tid = threadIdx.x, l = tid & 63, wave = tid >> 6;
+    if (scan_pad(A)) return;
     const int qbase = scan_qgroup(A) * kQT;
     if (tile_skipped(A, qbase)) return;
 
@@ -895,6 +907,7 @@ void k_scan_bf16_mq(ScanArgs A) {
     CandLds *Cs = (CandLds *)(smem + (size_t)QTW * 2 * S * 64 * 16);
 
     const int tid = threadIdx.x, l = tid & 63, wave = tid >> 6;
+    if (scan_pad(A)) return;
     const int nqt = (A.nq + kQT - 1) / kQT;
     int qb0[QTW];
     bool qon[QTW];
@@ -1029,6 +1042,7 @@ void k_scan_i8(ScanArgs A) {
     CandLds &C = *(CandLds *)(smem + (size_t)2 * KS * 64 * 16);
 
     const int tid = threadIdx.x, l = tid & 63, wave = tid >> 6;
+    if (scan_pad(A)) return;
     const int qbase = scan_qgroup(A) * kQT;
     if (tile_skipped(A, qbase)) return;
 
@@ -1834,10 +1848,20 @@ int scan_precision(const FvGpuIndex *g, const FvGpuBatch *b, const FvGpuBruteOpt
 
 /* BF16 scan launch: waves per WG (4 / 8) and inline predicates pick the
  * kernel instance */
+/* grid of a scan launch: (chunks, query groups), or the 1D query-major
+ * grid (scan_chunk) with its chunk and group counts set in Q */
+static dim3 scan_grid(ScanArgs &Q, uint32_t gx, uint32_t groups) {
+    if (!Q.qmajor) return dim3(gx, groups);
+    Q.nchunk = gx;
+    Q.ngroup = groups;
+    return dim3(kXcds * groups * ((gx + kXcds - 1) / kXcds));
+}
+
 template <bool PREPASS>
 static void launch_scan_bf16(int nw, bool inl, uint32_t gx, int nqt, size_t shm,
-                             hipStream_t s, const ScanArgs &A) {
-    const dim3 grid = A.qmajor ? dim3(nqt, gx) : dim3(gx, nqt);
+                             hipStream_t s, const ScanArgs &A0) {
+    ScanArgs A = A0;
+    const dim3 grid = scan_grid(A, gx, (uint32_t)nqt);
     if (nw == 4 && !inl) hipLaunchKernelGGL((k_scan_bf16<PREPASS, 4, false>), grid, dim3(256), shm, s, A);
     else if (nw == 4) hipLaunchKernelGGL((k_scan_bf16<PREPASS, 4, true>), grid, dim3(256), shm, s, A);
     else if (!inl) hipLaunchKernelGGL((k_scan_bf16<PREPASS, 8, false>), grid, dim3(512), shm, s, A);
@@ -1848,7 +1872,8 @@ static void launch_scan_bf16(int nw, bool inl, uint32_t gx, int nqt, size_t shm,
  * QTW 4 -> 8 waves per WG (one WG fills a CU's LDS) */
 template <bool PREPASS>
 static hipError_t launch_scan_mq(int qtw, bool inl, uint32_t gx, int gy, hipStream_t s,
-                                 const ScanArgs &A) {
+                                 const ScanArgs &A0) {
+    ScanArgs A = A0;
     const size_t shm = (size_t)qtw * 2 * 8 * 64 * 16 + (size_t)qtw * sizeof(CandLds);
     const void *f;
     if (qtw == 2) f = inl ? (const void *)k_scan_bf16_mq<PREPASS, 2, 4, true>
@@ -1857,7 +1882,7 @@ static hipError_t launch_scan_mq(int qtw, bool inl, uint32_t gx, int gy, hipStre
                  : (const void *)k_scan_bf16_mq<PREPASS, 4, 8, false>;
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
     if (e != hipSuccess) return e;
-    const dim3 grid = A.qmajor ? dim3(gy, gx) : dim3(gx, gy);
+    const dim3 grid = scan_grid(A, gx, (uint32_t)gy);
     if (qtw == 2 && inl) hipLaunchKernelGGL((k_scan_bf16_mq<PREPASS, 2, 4, true>), grid, dim3(256), shm, s, A);
     else if (qtw == 2) hipLaunchKernelGGL((k_scan_bf16_mq<PREPASS, 2, 4, false>), grid, dim3(256), shm, s, A);
     else if (inl) hipLaunchKernelGGL((k_scan_bf16_mq<PREPASS, 4, 8, true>), grid, dim3(512), shm, s, A);
@@ -2005,8 +2030,9 @@ static ScanShape scan_shape(const FvGpuIndex *g, const FvGpuBatch *b, const FvGp
  * dim 256: 1) and inline predicates pick the kernel instance */
 template <bool PREPASS>
 static void launch_scan_i8(int nw, int pt, bool inl, uint32_t gx, int nqt, size_t shm,
-                           hipStream_t s, const ScanArgs &A) {
-    const dim3 grid = A.qmajor ? dim3(nqt, gx) : dim3(gx, nqt);
+                           hipStream_t s, const ScanArgs &A0) {
+    ScanArgs A = A0;
+    const dim3 grid = scan_grid(A, gx, (uint32_t)nqt);
 #define PV_I8(NW, PT, INL) \
     hipLaunchKernelGGL((k_scan_i8<PREPASS, NW, PT, INL>), grid, dim3(NW * 64), shm, s, A)
     if (nw == 8) {
