@@ -95,14 +95,15 @@ __device__ __forceinline__ void prep_query(
             s8 = fmaf(c, c, s8);
         }
     }
+#pragma unroll
     for (int o = 32; o; o >>= 1) {
-        ss += __shfl_xor(ss, o);
-        pmax = fmaxf(pmax, __shfl_xor(pmax, o));
-        nmax = fmaxf(nmax, __shfl_xor(nmax, o));
-        nonint |= __shfl_xor(nonint, o);
+        ss += lane_xor(ss, o);
+        pmax = fmaxf(pmax, lane_xor(pmax, o));
+        nmax = fmaxf(nmax, lane_xor(nmax, o));
+        nonint |= lane_xor(nonint, o);
         if (mode == 2) {
-            s8 += __shfl_xor(s8, o);
-            out8 |= __shfl_xor(out8, o);
+            s8 += lane_xor(s8, o);
+            out8 |= lane_xor(out8, o);
         }
     }
     if (l == 0 && mode == 2) qn8[qi] = s8;
@@ -1220,11 +1221,7 @@ __global__ __launch_bounds__(256, 2) void k_scan_f32(ScanArgs A) {
 /* inclusive prefix sum over a 256-thread block */
 __device__ __forceinline__ uint32_t block_scan_incl(uint32_t v, uint32_t *tmp) {
     const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        uint32_t n = __shfl_up(v, o);
-        if (l >= o) v += n;
-    }
+    v = wave_scan_incl(v);
     if (l == 63) tmp[w] = v;
     __syncthreads();
     uint32_t add = 0;

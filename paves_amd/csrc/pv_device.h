@@ -27,6 +27,64 @@ __device__ __forceinline__ uint16_t f2bf(float x) {
 }
 
 /*
+ * Lane exchange for the butterflies of the sorts and reductions:
+ * lane_xor(x, j) is lane (l ^ j)'s x, for j in {1, 2, 4, 8, 16, 32}.
+ *   j = 1, 2:      DPP quad_perm on a VALU move (no LDS round trip)
+ *   j = 4, 8, 16:  ds_swizzle in bitmask mode (and 0x1f, xor j) inside each
+ *                  32-lane half (no address register, unlike ds_bpermute)
+ *   j = 32:        v_permlane32_swap (VALU): swapping lanes 32..63 of one
+ *                  copy with lanes 0..31 of another leaves the upper half's
+ *                  partners in the first result and the lower half's in
+ *                  the second
+ * j must fold to a constant once the caller's loop is unrolled (any other
+ * value takes ds_bpermute), and every lane of the aligned group of 2j
+ * lanes around the caller must be active: the partner's register is read
+ * directly. Partners are exactly l ^ j, so a butterfly built on it
+ * associates its sums exactly as one built on __shfl_xor.
+ */
+__device__ __forceinline__ uint32_t lane_xor(uint32_t x, int j) {
+    switch (j) {
+    case 1: return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0xB1, 0xF, 0xF, false);
+    case 2: return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x4E, 0xF, 0xF, false);
+    case 4: return (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, 0x1F | (4 << 10));
+    case 8: return (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, 0x1F | (8 << 10));
+    case 16: return (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, 0x1F | (16 << 10));
+    case 32: {
+        const auto p = __builtin_amdgcn_permlane32_swap(x, x, false, false);
+        return (__lane_id() & 32) ? p[0] : p[1];
+    }
+    default: return (uint32_t)__shfl_xor((int)x, j);
+    }
+}
+__device__ __forceinline__ float lane_xor(float x, int j) {
+    return __uint_as_float(lane_xor(__float_as_uint(x), j));
+}
+__device__ __forceinline__ int lane_xor(int x, int j) {
+    return (int)lane_xor((uint32_t)x, j);
+}
+__device__ __forceinline__ uint64_t lane_xor(uint64_t x, int j) {
+    const uint32_t lo = lane_xor((uint32_t)x, j), hi = lane_xor((uint32_t)(x >> 32), j);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+/*
+ * Inclusive prefix sum across a 64-lane wave on DPP alone: Hillis-Steele
+ * inside each 16-lane row (row_shr 1, 2, 4, 8; lanes shifted in from
+ * outside the row read 0), then row 15's total into rows 1 and 3
+ * (row_bcast:15, rows 0 and 2 masked off) and lane 31's into rows 2 and 3
+ * (row_bcast:31). Every lane must be active.
+ */
+__device__ __forceinline__ uint32_t wave_scan_incl(uint32_t v) {
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, true);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xF, 0xF, true);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xF, true);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, true);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false);
+    return v;
+}
+
+/*
  * Scalar tail of the reference l2sq (dims past the last 16-dim block) as
  * the reference compiler emits `sum += d * d` (-O3): with >= 8 elements
  * left, one 8-wide in-order block with separately rounded products, then a
@@ -74,10 +132,10 @@ __device__ __forceinline__ float cpu_order_l2(const float *__restrict__ x,
         acc0 = __fmaf_rn(d, d, acc0);
     }
     float t = __fadd_rn(acc0, acc1);
-    t = __fadd_rn(t, __shfl_xor(t, 8, 16));
-    t = __fadd_rn(t, __shfl_xor(t, 4, 16));
-    t = __fadd_rn(t, __shfl_xor(t, 2, 16));
-    t = __fadd_rn(t, __shfl_xor(t, 1, 16));
+    t = __fadd_rn(t, lane_xor(t, 8));
+    t = __fadd_rn(t, lane_xor(t, 4));
+    t = __fadd_rn(t, lane_xor(t, 2));
+    t = __fadd_rn(t, lane_xor(t, 1));
     t = tail_l2(x, q, i, dim, t);
     return t;
 }
@@ -534,22 +592,15 @@ __device__ __forceinline__ void bitmap_rows(uint64_t block, int nprog, int nops,
     }
 }
 
-/* 64-bit lane exchange (two 32-bit bpermutes) */
-__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t x, int m) {
-    const int lo = __shfl_xor((int)(uint32_t)x, m);
-    const int hi = __shfl_xor((int)(uint32_t)(x >> 32), m);
-    return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
-}
-
 /* ascending bitonic sort of one key per lane across a 64-lane wave
- * (register-resident, no LDS) */
+ * (register-resident: DPP, swizzle and permlane exchanges, no LDS data) */
 __device__ __forceinline__ uint64_t wave_sort64(uint64_t x) {
     const int l = threadIdx.x & 63;
 #pragma unroll
     for (int k = 2; k <= 64; k <<= 1) {
 #pragma unroll
         for (int j = k >> 1; j > 0; j >>= 1) {
-            const uint64_t y = shfl_xor_u64(x, j);
+            const uint64_t y = lane_xor(x, j);
             const bool keep_min = ((l & j) == 0) == ((l & k) == 0);
             const uint64_t lo = x < y ? x : y, hi = x < y ? y : x;
             x = keep_min ? lo : hi;
@@ -613,7 +664,8 @@ __device__ __forceinline__ void wave_nearest(const float (&best)[kNear], float *
         for (int k = 0; k < kNear; k++)
             if (k == head) h = best[k];
         float m = h;
-        for (int off = 32; off; off >>= 1) m = fminf(m, __shfl_xor(m, off));
+#pragma unroll
+        for (int off = 32; off; off >>= 1) m = fminf(m, lane_xor(m, off));
         const uint64_t who = __ballot(h == m);
         const int lane = __ffsll((unsigned long long)who) - 1;
         if (l == lane) head++;

@@ -239,10 +239,10 @@ __device__ __forceinline__ void dist_nodes(const GraphArgs &A, const float *qsh,
 #pragma unroll
     for (int k = 0; k < NPG; k++) {
         float t = __fadd_rn(acc0[k], acc1[k]);
-        t = __fadd_rn(t, __shfl_xor(t, 8, 16));
-        t = __fadd_rn(t, __shfl_xor(t, 4, 16));
-        t = __fadd_rn(t, __shfl_xor(t, 2, 16));
-        t = __fadd_rn(t, __shfl_xor(t, 1, 16));
+        t = __fadd_rn(t, lane_xor(t, 8));
+        t = __fadd_rn(t, lane_xor(t, 4));
+        t = __fadd_rn(t, lane_xor(t, 2));
+        t = __fadd_rn(t, lane_xor(t, 1));
         t = tail_l2(rp[k], qsh, i, D, t);
         const int j = base + g + 4 * k;
         if (ok[k] && l16 == 0) outd[j] = t;

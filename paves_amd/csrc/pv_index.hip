@@ -90,11 +90,12 @@ __global__ __launch_bounds__(256) void k_derive_rows(
         nmax = fmaxf(nmax, -x);
         nonint |= (x != rintf(x)) ? 1 : 0;
     }
+#pragma unroll
     for (int o = 32; o; o >>= 1) {
-        ss += __shfl_xor(ss, o);
-        pmax = fmaxf(pmax, __shfl_xor(pmax, o));
-        nmax = fmaxf(nmax, __shfl_xor(nmax, o));
-        nonint |= __shfl_xor(nonint, o);
+        ss += lane_xor(ss, o);
+        pmax = fmaxf(pmax, lane_xor(pmax, o));
+        nmax = fmaxf(nmax, lane_xor(nmax, o));
+        nonint |= lane_xor(nonint, o);
     }
     if (l == 0 && row < count) {
         norm[row] = ss;
@@ -127,7 +128,8 @@ __global__ __launch_bounds__(256) void k_derive_i8(const float *__restrict__ v, 
         vi8[tbase + (uint64_t)s * 1024 + ((row & 31) + 32 * g) * 16 + (j & 15)] = (int8_t)(int)x;
         ss = fmaf(x, x, ss);
     }
-    for (int o = 32; o; o >>= 1) ss += __shfl_xor(ss, o);
+#pragma unroll
+    for (int o = 32; o; o >>= 1) ss += lane_xor(ss, o);
     if (l == 0) norm8[row] = row < count ? ss : 0.f;
 }
 

@@ -45,7 +45,8 @@ __global__ __launch_bounds__(256) void k_prog_count(const uint32_t *__restrict__
     const uint32_t *row = bitmap + (uint64_t)blockIdx.x * words;
     uint32_t c = 0;
     for (uint64_t w = threadIdx.x; w < words; w += blockDim.x) c += __popc(row[w]);
-    for (int o = 32; o; o >>= 1) c += __shfl_xor(c, o);
+#pragma unroll
+    for (int o = 32; o; o >>= 1) c += lane_xor(c, o);
     if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = c;
     __syncthreads();
     if (threadIdx.x == 0) cnt[blockIdx.x] = part[0] + part[1] + part[2] + part[3];

@@ -40,7 +40,8 @@ __host__ __device__ inline MlpLayout mlp_layout(int F) {
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
 __device__ __forceinline__ float wave_sum(float v) {
-    for (int o = 32; o; o >>= 1) v += __shfl_xor(v, o);
+#pragma unroll
+    for (int o = 32; o; o >>= 1) v += lane_xor(v, o);
     return v;
 }
 
@@ -161,7 +162,8 @@ __device__ __forceinline__ double wave_pass_count(const LearnArgs &A, int qi) {
 #pragma unroll
         for (int k = 0; k < 16; k++) c += v[k];
     }
-    for (int o = 32; o; o >>= 1) c += __shfl_xor(c, o);
+#pragma unroll
+    for (int o = 32; o; o >>= 1) c += lane_xor(c, o);
     return (double)c;
 }
 
