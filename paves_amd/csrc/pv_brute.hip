@@ -372,6 +372,15 @@ __device__ __forceinline__ void cursor_next(UnitCursor &u, uint32_t nch, uint32_
     }
 }
 
+/* the last tile a wave's loads may touch. The pipeline loads one unit
+ * ahead, so after its last consumed unit a wave still loads one; clamped
+ * here it re-reads the end of the wave's own range, which its WG has just
+ * read (an L2 hit), instead of the next chunk's first tile (an HBM read
+ * that chunk's WG, on another XCD, makes again). Never above ntiles - 1. */
+__device__ __forceinline__ uint32_t cursor_last(const UnitCursor &u, uint32_t ntiles) {
+    return max(min(u.end, ntiles), 1u) - 1u;
+}
+
 /* true when every query of the WG's 64-query tile is skipped (uniform) */
 __device__ __forceinline__ bool tile_skipped(const ScanArgs &A, int qbase) {
     if (A.skip == nullptr) return false;
@@ -814,7 +823,8 @@ void k_scan_bf16(ScanArgs A) {
     UnitCursor cur = cursor_init<NW>(A, wave, &s_next);
     const u32x4 *vb = (const u32x4 *)A.vbf;
     /* units past the wave's range load a valid tile and are not consumed */
-    auto phys = [&](uint32_t t) { return min(t, A.ntiles - 1) * A.stride; };
+    const uint32_t tlast = cursor_last(cur, A.ntiles);
+    auto phys = [&](uint32_t t) { return min(t, tlast) * A.stride; };
     auto load_rows = [&](u32x4 (&a)[8]) {
         const u32x4 *p = vb + (uint64_t)phys(cur.tile) * (4 * A.dim) + (uint64_t)cur.c * 512 + l;
 #pragma unroll
@@ -925,7 +935,8 @@ void k_scan_bf16_mq(ScanArgs A) {
     __shared__ uint32_t s_next;
     UnitCursor cur = cursor_init<NW>(A, wave, &s_next);
     const u32x4 *vb = (const u32x4 *)A.vbf;
-    auto phys = [&](uint32_t t) { return min(t, A.ntiles - 1) * A.stride; };
+    const uint32_t tlast = cursor_last(cur, A.ntiles);
+    auto phys = [&](uint32_t t) { return min(t, tlast) * A.stride; };
     auto load_rows = [&](u32x4 (&a)[8]) {
         const u32x4 *p = vb + (uint64_t)phys(cur.tile) * 512 + l;
 #pragma unroll
@@ -1051,7 +1062,8 @@ void k_scan_i8(ScanArgs A) {
     UnitCursor cur = cursor_init<NW>(A, wave, &s_next);
     const u32x4 *vb = (const u32x4 *)A.vi8;
     /* physical unit of a logical tile (units past the range load a valid one) */
-    auto phys = [&](uint32_t t) { return min(t, A.ntiles - 1) * A.stride; };
+    const uint32_t tlast = cursor_last(cur, A.ntiles);
+    auto phys = [&](uint32_t t) { return min(t, tlast) * A.stride; };
     auto load_rows = [&](u32x4 (&a)[8]) {
         const u32x4 *p = vb + (uint64_t)phys(cur.tile) * 512 + l;
 #pragma unroll
