@@ -487,7 +487,7 @@ struct TileIn {
     bool live;         /* the unit is part of this wave's work */
 };
 
-/* ||x||^2 of the lane's 16 accumulator rows */
+/* a per-row value (held by lane row & 31) for the lane's 16 accumulator rows */
 __device__ __forceinline__ void expand_norms(float xr, float (&xn)[16]) {
     const int l = threadIdx.x & 63;
 #pragma unroll
@@ -619,31 +619,37 @@ __device__ __forceinline__ uint32_t elem_mask(uint32_t wa, int i) {
     return (uint32_t)((int32_t)(wa << (31 - p)) >> 31);
 }
 
-/* BF16 seeds of a tile: xn from expand_norms, w0 / w1 the query blocks'
- * filter words with the validity mask applied */
-__device__ __forceinline__ void seed_bf(const float (&xn)[16], uint32_t w0, uint32_t w1,
+/* BF16 seeds of a tile: sn = -||x||^2 / 2 of the lane's 16 accumulator
+ * rows (expand_norms of the lane's own row's value, halved and negated
+ * once per lane, both exact), w0 / w1 the query blocks' filter words with
+ * the validity mask applied */
+__device__ __forceinline__ void seed_bf(const float (&sn)[16], uint32_t w0, uint32_t w1,
                                         f32x16 &a0, f32x16 &a1) {
     const uint32_t v0 = lane_word(w0), v1 = lane_word(w1);
     constexpr uint32_t kNegInf = 0xff800000u;
 #pragma unroll
     for (int i = 0; i < 16; i++) {
-        const uint32_t s = __float_as_uint(-0.5f * xn[i]);
+        const uint32_t s = __float_as_uint(sn[i]);
         const uint32_t m0 = elem_mask(v0, i), m1 = elem_mask(v1, i);
         a0[i] = __uint_as_float((s & m0) | (kNegInf & ~m0));
         a1[i] = __uint_as_float((s & m1) | (kNegInf & ~m1));
     }
 }
 
-/* INT8 seeds; par gets bit i = ||x||^2 of element i's row is odd */
-__device__ __forceinline__ void seed_i8(const float (&xn)[16], uint32_t w0, uint32_t w1,
+/* INT8 seeds from xr = ||x||^2 of the lane's own row (lane & 31, an
+ * integer): -floor(xr / 2) is formed once per lane and spread to the 16
+ * accumulator rows; par gets bit r = ||x_r||^2 is odd for rows 0..31 (one
+ * ballot), read only when a candidate is emitted */
+__device__ __forceinline__ void seed_i8(float xr, uint32_t w0, uint32_t w1,
                                         i32x16 &c0, i32x16 &c1, uint32_t &par) {
+    const int l = threadIdx.x & 63;
+    const int32_t xi = (int32_t)xr;
+    const int32_t half = -(xi >> 1);
+    par = (uint32_t)__ballot(xi & 1);
     const uint32_t v0 = lane_word(w0), v1 = lane_word(w1);
-    par = 0;
 #pragma unroll
     for (int i = 0; i < 16; i++) {
-        const int32_t xi = (int32_t)xn[i];
-        const uint32_t s = (uint32_t)(-(xi >> 1));
-        par |= (uint32_t)(xi & 1) << i;
+        const uint32_t s = (uint32_t)__shfl(half, acc_row(i, l));
         const uint32_t m0 = elem_mask(v0, i), m1 = elem_mask(v1, i);
         c0[i] = (int32_t)((s & m0) | ((uint32_t)kI8Off & ~m0));
         c1[i] = (int32_t)((s & m1) | ((uint32_t)kI8Off & ~m1));
@@ -734,7 +740,8 @@ __device__ __forceinline__ void epilogue_i8(const ScanArgs &A, int qbase, uint32
 #pragma unroll
             for (int i = 0; i < 16; i++)
                 if (a[i] >= th[qb]) {
-                    const float d = (float)(qni + (int32_t)((par >> i) & 1u) - 2 * a[i]);
+                    const float d =
+                        (float)(qni + (int32_t)((par >> acc_row(i, l)) & 1u) - 2 * a[i]);
                     insert_candidate(A, qbase, qi, make_key(d, (uint32_t)((uint64_t)pt * 32 +
                                                                 acc_row(i, l))), C);
                 }
@@ -865,8 +872,8 @@ void k_scan_bf16(ScanArgs A) {
         const uint32_t c = in.c;
         if (c == 0) {
             /* seeds: row norms and the queries' filters (epilogue_bf) */
-            float xn[16];
-            expand_norms(in.xr, xn);
+            float sn[16];
+            expand_norms(-0.5f * in.xr, sn);
             uint32_t w0 = in.w[0], w1 = in.w[1];
             if (INL) {
                 double cv[16];
@@ -874,7 +881,7 @@ void k_scan_bf16(ScanArgs A) {
                 w0 = inline_word(cv, s_im, s_iv, 0, 0);
                 w1 = inline_word(cv, s_im, s_iv, 0, 1);
             }
-            seed_bf(xn, w0 & in.valid, w1 & in.valid, acc0, acc1);
+            seed_bf(sn, w0 & in.valid, w1 & in.valid, acc0, acc1);
         }
 #pragma unroll
         for (int j = 0; j < 8; j++) {
@@ -982,8 +989,8 @@ void k_scan_bf16_mq(ScanArgs A) {
     };
     auto do_unit = [&](const u32x4 (&a)[8], const TileIn (&in)[QTW]) {
         const uint32_t t = in[0].t;
-        float xn[16];
-        expand_norms(in[0].xr, xn);
+        float sn[16];
+        expand_norms(-0.5f * in[0].xr, sn);
         const uint32_t valid = in[0].valid;
         uint32_t iw[QTW][2];
         if (INL) {
@@ -1001,7 +1008,7 @@ void k_scan_bf16_mq(ScanArgs A) {
             /* the tile's norms, validity and inline column values are shared
              * by the query tiles; the bitmap words are per query tile */
             f32x16 acc0, acc1;
-            seed_bf(xn, (INL ? iw[k][0] : in[k].w[0]) & valid, (INL ? iw[k][1] : in[k].w[1]) & valid,
+            seed_bf(sn, (INL ? iw[k][0] : in[k].w[0]) & valid, (INL ? iw[k][1] : in[k].w[1]) & valid,
                     acc0, acc1);
 #pragma unroll
             for (int j = 0; j < 8; j++) {
@@ -1106,8 +1113,6 @@ void k_scan_i8(ScanArgs A) {
 #pragma unroll
         for (int p = 0; p < PT; p++) {
             /* seeds: row norms and the queries' filters (epilogue_i8) */
-            float xn[16];
-            expand_norms(in[p].xr, xn);
             uint32_t w0 = in[p].w[0], w1 = in[p].w[1];
             if (INL) {
                 double cv[16];
@@ -1117,7 +1122,7 @@ void k_scan_i8(ScanArgs A) {
             }
             i32x16 c0, c1;
             uint32_t par;
-            seed_i8(xn, w0 & in[p].valid, w1 & in[p].valid, c0, c1, par);
+            seed_i8(in[p].xr, w0 & in[p].valid, w1 & in[p].valid, c0, c1, par);
 #pragma unroll
             for (int k = 0; k < KS; k++) {
                 const i32x4 av = __builtin_bit_cast(i32x4, a[p * KS + k]);
