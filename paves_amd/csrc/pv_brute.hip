@@ -1568,12 +1568,18 @@ __global__ __launch_bounds__(256) void k_select(
         }
         return;
     }
-    for (int j = t; j < dim; j += blockDim.x) qsh[j] = qf[(uint64_t)q * dim + j];
+    /* eps = 0 (integer data inside the exactness bounds checked at query
+     * prep): every scan form and the reference's FP32 sum are exact there,
+     * so the pool keys already hold the reference's distances and the
+     * re-rank (and its copy of the query) is skipped */
+    const float e = eps ? eps[q] : 0.f;
+    const bool rerank = !(e == 0.f);
+    if (rerank)
+        for (int j = t; j < dim; j += blockDim.x) qsh[j] = qf[(uint64_t)q * dim + j];
     uint32_t m0 = kSelMax + 1;
     const uint64_t kth =
         B <= 64 ? block_kth_small(pk_at, n, B, sel, kSelMax, &nsel, wb, hist, tmp, bc, &m0)
                 : block_kth(pk_at, n, B, hist, tmp, bc);
-    const float e = eps ? eps[q] : 0.f;
     float lim = key_dist(kth);
     if (e > 0.f) {
         double w = (double)lim + 2.0 * (double)e;
@@ -1608,11 +1614,13 @@ __global__ __launch_bounds__(256) void k_select(
         return;
     }
     /* exact re-rank: 16 lanes per key */
-    const int grp = t >> 4, l16 = t & 15;
-    for (uint32_t i = grp; i < m; i += blockDim.x / 16) {
-        const uint32_t row = key_row(sel[i]);
-        const float d = cpu_order_l2(vecs + (uint64_t)row * dim, qsh, dim, l16);
-        if (l16 == 0) sel[i] = make_key(d, row);
+    if (rerank) {
+        const int grp = t >> 4, l16 = t & 15;
+        for (uint32_t i = grp; i < m; i += blockDim.x / 16) {
+            const uint32_t row = key_row(sel[i]);
+            const float d = cpu_order_l2(vecs + (uint64_t)row * dim, qsh, dim, l16);
+            if (l16 == 0) sel[i] = make_key(d, row);
+        }
     }
     __syncthreads();
     if (m <= 64) {
