@@ -1311,8 +1311,18 @@ template <typename Get>
 __device__ uint32_t block_collect_small(Get get, uint32_t n, uint32_t B, uint64_t *sel,
                                         uint32_t cap, uint32_t *nsel, uint64_t *wb) {
     const int t = threadIdx.x, l = t & 63, w = t >> 6;
+    /* the first kHeld keys of each thread stay in registers for the second
+     * pass (one global round trip instead of two) */
+    constexpr int kHeld = 4;
+    uint64_t held[kHeld];
     uint64_t mn = ~0ull;
-    for (uint32_t i = t; i < n; i += blockDim.x) {
+#pragma unroll
+    for (int j = 0; j < kHeld; j++) {
+        const uint32_t i = t + (uint32_t)j * blockDim.x;
+        held[j] = i < n ? get(i) : ~0ull;
+        mn = held[j] < mn ? held[j] : mn;
+    }
+    for (uint32_t i = t + kHeld * blockDim.x; i < n; i += blockDim.x) {
         const uint64_t k = get(i);
         mn = k < mn ? k : mn;
     }
@@ -1322,13 +1332,16 @@ __device__ uint32_t block_collect_small(Get get, uint32_t n, uint32_t B, uint64_
     __syncthreads();
     uint64_t bound = wb[0];
     for (int i = 1; i < (int)(blockDim.x >> 6); i++) bound = wb[i] < bound ? wb[i] : bound;
-    for (uint32_t i = t; i < n; i += blockDim.x) {
-        const uint64_t k = get(i);
+    auto take = [&](uint64_t k) {
         if (k <= bound) {
             const uint32_t p = atomicAdd(nsel, 1u);
             if (p < cap) sel[p] = k;
         }
-    }
+    };
+#pragma unroll
+    for (int j = 0; j < kHeld; j++)
+        if (t + (uint32_t)j * blockDim.x < n) take(held[j]);
+    for (uint32_t i = t + kHeld * blockDim.x; i < n; i += blockDim.x) take(get(i));
     __syncthreads();
     const uint32_t m = *nsel;
     if (m <= 64 && w == 0) {
@@ -1537,18 +1550,26 @@ __global__ __launch_bounds__(256) void k_select(
     __shared__ uint32_t nsel;
     extern __shared__ float qsh[];
     const int q = blockIdx.x, t = threadIdx.x;
-    if (topbq) topb = topbq[q]; /* per-query k of an uneven batch */
-    if (skip && skip[q]) return; /* the graph search writes this query */
+    /* the query's per-query words and pool counts are requested together
+     * (one round trip), before the skip test that needs one of them */
+    const uint32_t tbq = topbq ? topbq[q] : topb; /* per-query k of an uneven batch */
+    const bool skipped = skip && skip[q];         /* the graph search writes this query */
+    const float e = eps ? eps[q] : 0.f;
+    const uint64_t scap = shard_cap(cap_pool);
+    uint32_t cnt[kPoolShards];
+#pragma unroll
+    for (uint32_t s = 0; s < kPoolShards; s++) cnt[s] = pcnt[pcnt_at(q, s)];
+    if (skipped) return;
+    topb = tbq;
     /* the query's pool segments: counts, their prefix, and an index map
      * i -> segment entry over the concatenation */
     const uint64_t *pq = pool + (uint64_t)q * cap_pool;
-    const uint64_t scap = shard_cap(cap_pool);
     uint32_t pre[kPoolShards + 1];
     uint32_t st = 0;
     pre[0] = 0;
 #pragma unroll
     for (uint32_t s = 0; s < kPoolShards; s++) {
-        const uint32_t c = pcnt[pcnt_at(q, s)];
+        const uint32_t c = cnt[s];
         if ((uint64_t)c > scap) st |= kStOverflow;
         pre[s + 1] = pre[s] + (uint32_t)min((uint64_t)c, scap);
     }
@@ -1572,7 +1593,6 @@ __global__ __launch_bounds__(256) void k_select(
      * prep): every scan form and the reference's FP32 sum are exact there,
      * so the pool keys already hold the reference's distances and the
      * re-rank (and its copy of the query) is skipped */
-    const float e = eps ? eps[q] : 0.f;
     const bool rerank = !(e == 0.f);
     if (rerank)
         for (int j = t; j < dim; j += blockDim.x) qsh[j] = qf[(uint64_t)q * dim + j];
