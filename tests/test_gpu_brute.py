@@ -5,7 +5,8 @@ oracle on the same seeded inputs: ids, distances, out_cnt and truncated must
 be identical. Covers both scan precisions, the six filter classes, the
 span-gather route, uneven batch sizes, edge cases (no passing rows, fewer
 passing rows than topb, exact ties, row counts that are not multiples of the
-tile), non-integer 768-d data (certified re-rank), the device-resident batch
+tile), mixed-sign integer data at the exactness bounds (no re-rank at eps =
+0), non-integer 768-d data (certified re-rank), the device-resident batch
 entry points, ABI error returns, and the reference engine's own host glue
 (engine.cpp built with -DFV_GPU) driving this library as a drop-in.
 """
@@ -195,6 +196,34 @@ def test_exact_ties_broken_by_id():
             assert list(out[0, :100]["node"]) == want
     finally:
         g.close()
+
+
+@pytest.mark.parametrize("dim,lim", [(128, 127), (256, 90)])
+def test_mixed_sign_integers_exact_without_rerank(dim, lim):
+    # integer rows and queries of both signs at the edge of the exactness
+    # bounds (|v| <= lim; D delta^2 and D (V^2 + Q^2) below 2^24, so eps = 0):
+    # the seeded MFMA sums pass through large negative values (a third of
+    # the queries are negated rows) and must still equal the reference's
+    # FP32 distances, since the select stage skips its re-rank at eps = 0
+    rng = np.random.default_rng(7 + dim)
+    n, nq = 4000 + 13, 12
+    base = rng.integers(-lim, lim + 1, (n, dim)).astype(np.float32)
+    qs = rng.integers(-lim, lim + 1, (nq, dim)).astype(np.float32)
+    qs[:4] = -base[rng.integers(0, n, 4)]
+    cols = np.stack([rng.uniform(0, 1000, n).astype(np.float32).astype(np.float64),
+                     rng.integers(0, 32, n).astype(np.float64)])
+    preds = [None, E.cmp(0, E.CMP_LT, 500.0), E.isin(1, [3.0, 7.0])] * 4
+    b = E.concat([E.serialize(p, n) for p in preds], n)
+    oh, oc, ot, _ = O.brute_batch(base, cols, qs, b.qd, b.nodes, b.children, b.invals, 20)
+    for prec in (L.FV_GPU_PREC_BF16, L.FV_GPU_PREC_FP32, L.FV_GPU_PREC_SQ8):
+        g = E.GpuIndex(base, cols, None, precision=prec)
+        try:
+            out, cnt, trunc = g.brute(qs, b, 20, L.BruteOpts(route=1))
+            if prec == L.FV_GPU_PREC_SQ8:
+                assert g.stats().precision == L.FV_GPU_PREC_SQ8
+            assert_same(out, cnt, trunc, oh, oc, ot, f"mixed-sign {dim}/{prec}")
+        finally:
+            g.close()
 
 
 def test_non_integer_768d_certified_rerank():
