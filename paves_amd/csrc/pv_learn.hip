@@ -308,7 +308,7 @@ __global__ __launch_bounds__(256) void k_learn_route(LearnArgs A) {
     const float *qv = A.q + (uint64_t)qi * A.dim;
     for (int j = threadIdx.x; j < A.dim; j += blockDim.x) xs[j] = qv[j];
     if (w == 0) {
-        const double pass = wave_pass_count(A, qi);
+        const double pass = wave_pass_count(A, A.pofq[qi]);
         if (l == 0) xs[A.dim] = (float)log10(fmax(pass, 1.0) / (double)A.count);
     }
     __syncthreads();

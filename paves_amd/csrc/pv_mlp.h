@@ -48,6 +48,31 @@ __device__ __forceinline__ float wave_sum(float v) {
 /* the BF16 inference parameters staged in LDS when they fit (dim <= 156) */
 constexpr int kLdsParams = 16384;
 
+/* the same staging split in two for a 256-thread block and n <= kLdsParams
+ * (at most kStagePer 16-byte pieces per thread): loads into registers,
+ * then the LDS stores, so other loads can be issued in between */
+constexpr int kStagePer = 8;
+static_assert(kLdsParams <= kStagePer * 8 * 256, "one pass of a 256-thread block");
+__device__ __forceinline__ void stage_load(const uint16_t *__restrict__ wt, int n,
+                                           uint4 (&v)[kStagePer]) {
+    const uint4 *src = (const uint4 *)wt;
+    const int n16 = (n + 7) / 8;
+#pragma unroll
+    for (int k = 0; k < kStagePer; k++) {
+        const int i = k * 256 + threadIdx.x;
+        if (i < n16) v[k] = src[i];
+    }
+}
+__device__ __forceinline__ void stage_store(int n, const uint4 (&v)[kStagePer], uint16_t *ws) {
+    uint4 *dst = (uint4 *)ws;
+    const int n16 = (n + 7) / 8;
+#pragma unroll
+    for (int k = 0; k < kStagePer; k++) {
+        const int i = k * 256 + threadIdx.x;
+        if (i < n16) dst[i] = v[k];
+    }
+}
+
 /* all threads of the block copy n BF16 parameters (n padded to 8) to LDS;
  * a thread's loads are all issued before its LDS writes (one memory round
  * trip for a 256-thread block) */
@@ -143,10 +168,11 @@ struct LearnArgs {
     float *feat_out, *y_out;
 };
 
-/* passing rows of query qi's predicate (all rows without one): the wave
- * sums its program's per-block counts of the bitmap pass */
-__device__ __forceinline__ double wave_pass_count(const LearnArgs &A, int qi) {
-    const int p = A.pofq[qi];
+/* passing rows of a query whose program index is p (all rows without a
+ * predicate, p < 0): the wave sums the program's per-block counts of the
+ * bitmap pass. The caller reads p (A.pofq) early, so that its round trip
+ * overlaps its other loads. */
+__device__ __forceinline__ double wave_pass_count(const LearnArgs &A, int p) {
     if (p < 0) return (double)A.count;
     const uint32_t *part = A.progpart + (uint64_t)p * A.nbm;
     const uint32_t l = threadIdx.x & 63;
@@ -202,6 +228,7 @@ __device__ __forceinline__ void route_block(const LearnArgs &A, int blk) {
      * features and standardisation into registers, the parameters into
      * LDS, the per-block counts (after the query's program index) */
     const bool live = qi < A.nq;
+    const int prog = live ? A.pofq[qi] : -1; /* the pass count's loads depend on it */
     float *fq = A.feat_out + (uint64_t)(live ? qi : 0) * A.F;
     float fv[kPerLane], mu[kPerLane], is[kPerLane];
 #pragma unroll
@@ -212,8 +239,12 @@ __device__ __forceinline__ void route_block(const LearnArgs &A, int blk) {
         mu[k] = in ? A.mean[j] : 0.f;
         is[k] = in ? A.istd[j] : 0.f;
     }
-    if (lds) stage_params(A.wt, np, ws);
-    const double pass = live ? wave_pass_count(A, qi) : 0.0;
+    /* the parameters go to LDS after the pass count's loads are issued:
+     * one round trip for the program index, one for everything else */
+    uint4 sv[kStagePer];
+    if (lds) stage_load(A.wt, np, sv);
+    const double pass = live ? wave_pass_count(A, prog) : 0.0;
+    if (lds) stage_store(np, sv, ws);
     __syncthreads(); /* every wave reaches it (no early return above) */
     if (!live) return;
     const float sel = (float)log10(fmax(pass, 1.0) / (double)A.count);
