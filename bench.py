@@ -257,8 +257,8 @@ def main():
     ap.add_argument("--copy-calib", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(the index lives on a single MI355X; see the sharding note in DESIGN.md)",
+        print(f"bench.py: measures one GPU; --gpus {args.gpus} is not supported "
+              "(the index lives on a single MI355X; sharding it across GPUs is not built)",
               file=sys.stderr)
         sys.exit(2)
     args.n_set = args.n is not None
