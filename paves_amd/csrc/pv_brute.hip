@@ -1699,6 +1699,7 @@ void free_batch_buffers(FvGpuBatch *b) {
     if (b->graph) hipGraphDestroy(b->graph);
     b->gexec = nullptr;
     b->graph = nullptr;
+    b->gsig.clear();
     b->warmed = false;
     delete (ScanLaunch *)b->last_scan;
     b->last_scan = nullptr;
@@ -1936,6 +1937,8 @@ static hipError_t launch_scan_mq(int qtw, bool inl, uint32_t gx, int gy, hipStre
  * preparation for scan precision `prec`, one launch on stream s. */
 hipError_t launch_phase1(FvGpuIndex *g, FvGpuBatch *b, int prec, bool bitmap, hipStream_t s,
                          uint32_t *progpart, const PrepFeatures *pf) {
+    if (bitmap && b->nprog > 0 && (b->nprog > kBmProgs || b->d_bitmap.p == nullptr))
+        return hipErrorInvalidValue;
     QAux qa = qaux_of(b);
     {
         /* predicate programs -> row bitmaps, every run (the gather route
@@ -2137,9 +2140,7 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
     const bool inl = (prec == FV_GPU_PREC_BF16 || prec == FV_GPU_PREC_SQ8) && o &&
                      o->inline_pred && b->inl_ok &&
                      b->inl_col >= 0 && !have_phase1;
-    if (route == 1 && !inl && b->nprog > kBmProgs)
-        PV_FAIL("gpu: %d distinct predicates: at most %d without inline evaluation",
-                b->nprog, kBmProgs);
+    if (route == 1 && !inl && require_bitmaps(b, errbuf, errlen) != 0) return -1;
 
     ev_record(g, 0);
     if (!have_phase1) PV_TRY(launch_phase1(g, b, prec, route == 1 && !inl, s, nullptr));
@@ -2569,16 +2570,45 @@ extern "C" FvGpuBatch *fv_gpu_batch_stage_ex(
     }
 }
 
+/* What a captured run bakes into its kernel arguments: every device buffer
+ * of the batch and the sizes that shape the launches. A graph search or a
+ * routed run on the same batch may grow (reallocate) the output buffer or
+ * another one; a replay after that would write through a freed pointer,
+ * so a changed signature discards the capture. */
+static std::vector<uint64_t> capture_signature(const FvGpuBatch *b) {
+    return {(uint64_t)(uintptr_t)b->d_stage.p, (uint64_t)(uintptr_t)b->d_qbf.p,
+            (uint64_t)(uintptr_t)b->d_qaux.p, (uint64_t)(uintptr_t)b->d_bitmap.p,
+            (uint64_t)(uintptr_t)b->d_gmin.p, (uint64_t)(uintptr_t)b->d_pool.p,
+            (uint64_t)(uintptr_t)b->d_pcnt.p, (uint64_t)(uintptr_t)b->d_out.p,
+            (uint64_t)(uintptr_t)b->d_cnt.p, (uint64_t)(uintptr_t)b->d_flag.p,
+            (uint64_t)(uintptr_t)b->d_qi8.p, (uint64_t)(uintptr_t)b->d_qn8.p,
+            (uint64_t)b->nq, (uint64_t)b->topb, (uint64_t)b->nprog, (uint64_t)b->nops,
+            (uint64_t)b->nvals, b->cap_pool, (uint64_t)b->ngroups, (uint64_t)b->stride,
+            (uint64_t)b->inl_ok, (uint64_t)(int64_t)b->inl_col, (uint64_t)b->i8_ok,
+            (uint64_t)(uintptr_t)b->d_inl_p, (uint64_t)(uintptr_t)b->d_topbq_p};
+}
+
+static void drop_capture(FvGpuBatch *b) {
+    if (b->gexec) hipGraphExecDestroy(b->gexec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    b->gexec = nullptr;
+    b->graph = nullptr;
+    b->gsig.clear();
+}
+
 /* run through a hipGraph captured for this batch and these options: the
  * first call runs eagerly (allocating every buffer), the second captures,
  * later calls replay. Timing events are not recorded in graph mode. */
 static int run_brute_graph(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
                            char *errbuf, size_t errlen) {
+    if (b->gexec && b->gsig != capture_signature(b)) {
+        /* a buffer the capture points at was reallocated (or the batch
+         * changed shape): run eagerly once more, then capture again */
+        drop_capture(b);
+        b->warmed = false;
+    }
     if (b->gexec && memcmp(&b->gopts, o, sizeof(*o)) != 0) {
-        hipGraphExecDestroy(b->gexec);
-        hipGraphDestroy(b->graph);
-        b->gexec = nullptr;
-        b->graph = nullptr;
+        drop_capture(b);
         b->warmed = false;
     }
     if (b->gexec == nullptr) {
@@ -2601,7 +2631,11 @@ static int run_brute_graph(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o
         b->graph = gr;
         PV_TRY(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
         b->gopts = *o;
+        b->gsig = capture_signature(b);
     }
+    /* the replay's select writes topb hits per query (a graph search in
+     * between may have left the graph's stride) */
+    b->out_stride = b->topb;
     PV_TRY(hipGraphLaunch(b->gexec, b->stream));
     return 0;
 }

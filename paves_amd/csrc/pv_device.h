@@ -239,6 +239,17 @@ constexpr int kBmProgs = 1024;             /* max distinct programs (one per que
 constexpr int kBmProgLds = 256;            /* program table entries staged in LDS */
 constexpr int kBmGroup = 8;                /* programs per wave between write-outs */
 
+/* Predicate bitmaps exist for batches of at most kBmProgs distinct
+ * programs (one bitmap row each is allocated at staging). Every entry point
+ * that builds or reads them checks this first; larger batches (up to
+ * kMaxBatchEx queries) run only the brute path with inline predicates. */
+inline int require_bitmaps(const FvGpuBatch *b, char *errbuf, size_t errlen) {
+    if (b->nprog > kBmProgs)
+        PV_FAIL("gpu: %d distinct predicates: predicate bitmaps hold at most %d", b->nprog,
+                kBmProgs);
+    return 0;
+}
+
 /* blocks of a bitmap pass over an index */
 inline uint32_t bitmap_blocks(const FvGpuIndex *g) {
     return (uint32_t)((g->count_pad + kBmRows - 1) / kBmRows);

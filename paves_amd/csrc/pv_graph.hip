@@ -467,6 +467,7 @@ int run_graph_ex(FvGpuIndex *g, FvGpuBatch *b, uint32_t ef, uint64_t max_visited
                  int fcap, uint32_t *dbg, const GraphExtra &x, char *errbuf, size_t errlen) {
     if (g->d_nbr == nullptr) PV_FAIL("gpu: graph not uploaded");
     if (ef < 1 || ef > FV_GPU_MAX_EF) PV_FAIL("gpu: bad graph batch (nq=%d ef=%u)", b->nq, ef);
+    if (require_bitmaps(b, errbuf, errlen) != 0) return -1;
     if (b->max_entry >= g->count)
         PV_FAIL("gpu: entry point %llu out of range", (unsigned long long)b->max_entry);
     if (wmax < 1) wmax = 1;

@@ -158,10 +158,12 @@ struct FvGpuBatch {
     int inl_col = -1;
     const pv::InlPred *d_inl_p = nullptr;  /* [nqp] in the staging area */
     /* a captured run (FvGpuBruteOpts.graph): replayed while the options
-     * it was captured with are unchanged */
+     * it was captured with and every device buffer and size baked into its
+     * kernel arguments (gsig) are unchanged */
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     FvGpuBruteOpts gopts = {};
+    std::vector<uint64_t> gsig;
     bool warmed = false;    /* one eager run done: every buffer allocated */
     void *last_scan = nullptr; /* launch record of the last main scan (timing) */
     bool capturing = false; /* a hipGraph capture of this batch's run is open */

@@ -658,6 +658,7 @@ extern "C" int fv_gpu_batch_features(FvGpuIndex *g, FvGpuBatch *b, FvGpuModel *m
     if (g == nullptr || b == nullptr || b->owner != g || m == nullptr || out == nullptr)
         PV_FAIL("gpu: features: bad arguments");
     if (m->dim != g->dim) PV_FAIL("gpu: features: model dim %d != index dim %d", m->dim, g->dim);
+    if (require_bitmaps(b, errbuf, errlen) != 0) return -1;
     PV_TRY(hipSetDevice(g->device));
     hipStream_t s = b->stream;
     PV_TRY(launch_bitmap(g, b, s));
@@ -691,12 +692,23 @@ extern "C" int fv_gpu_batch_run_learned(FvGpuIndex *g, FvGpuBatch *b, FvGpuModel
                                         size_t errlen) {
     if (g == nullptr || b == nullptr || b->owner != g || o == nullptr)
         PV_FAIL("gpu: learned run: bad arguments");
+    if (o->mode < 0 || o->mode > 3) PV_FAIL("gpu: learned run: bad mode %d", o->mode);
+    if (o->k != b->topb) PV_FAIL("gpu: learned run: k %u != staged topb %u", o->k, b->topb);
+    /* the graph search returns at most FV_GPU_MAX_EF hits: with a larger k
+     * the graph cannot answer (the reference falls back to the CPU cursor,
+     * gpu_engine.h:40-42), so graph-only modes fail and the learned route
+     * sends every query to brute force */
+    if (o->k > FV_GPU_MAX_EF && (o->mode == 1 || o->mode == 2))
+        PV_FAIL("gpu: learned run: k %u exceeds the graph search's %d results", o->k,
+                FV_GPU_MAX_EF);
+    FvGpuLearnedOpts oc = *o;
+    if (o->k > FV_GPU_MAX_EF && o->mode == 0) oc.mode = 3;
+    o = &oc;
     const bool need_model = o->mode == 0 || o->mode == 1;
     if (need_model && m == nullptr) PV_FAIL("gpu: learned run: no model");
     if (need_model && m->dim != g->dim) PV_FAIL("gpu: learned run: model dim != index dim");
-    if (o->mode < 0 || o->mode > 3) PV_FAIL("gpu: learned run: bad mode %d", o->mode);
     if (o->mode != 3 && g->d_nbr == nullptr) PV_FAIL("gpu: learned run: graph not uploaded");
-    if (o->k != b->topb) PV_FAIL("gpu: learned run: k %u != staged topb %u", o->k, b->topb);
+    if (require_bitmaps(b, errbuf, errlen) != 0) return -1;
     const uint32_t ef_min = std::max<uint32_t>(o->ef_min, o->k);
     const uint32_t ef_max = std::min<uint32_t>(std::max(o->ef_max, ef_min), FV_GPU_MAX_EF);
     const uint32_t ef_fixed = std::min<uint32_t>(std::max(o->ef_fixed, ef_min), FV_GPU_MAX_EF);

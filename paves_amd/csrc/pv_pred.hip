@@ -194,6 +194,7 @@ __global__ __launch_bounds__(256) void k_bitmap(
 
 hipError_t launch_bitmap(FvGpuIndex *g, FvGpuBatch *b, hipStream_t s) {
     if (b->nprog == 0) return hipSuccess;
+    if (b->nprog > kBmProgs || b->d_bitmap.p == nullptr) return hipErrorInvalidValue;
     const uint64_t words = g->count_pad / 32;
     const unsigned blocks = bitmap_blocks(g);
     hipLaunchKernelGGL(k_bitmap, dim3(blocks), dim3(256), bitmap_lds_bytes(g->ncols), s,
