@@ -1233,69 +1233,6 @@ __global__ __launch_bounds__(256, 2) void k_scan_f32(ScanArgs A) {
     if (!PREPASS) flush_candidates(A, qbase, C);
 }
 
-/* ---------------- block-level helpers ---------------- */
-
-/* inclusive prefix sum over a 256-thread block */
-__device__ __forceinline__ uint32_t block_scan_incl(uint32_t v, uint32_t *tmp) {
-    const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
-    v = wave_scan_incl(v);
-    if (l == 63) tmp[w] = v;
-    __syncthreads();
-    uint32_t add = 0;
-    for (int i = 0; i < w; i++) add += tmp[i];
-    __syncthreads();
-    return v + add;
-}
-
-/*
- * k-th smallest (1-based) of n unique u64 keys, 8-bit MSD radix select
- * over a 256-thread block. Keys are read through `get`.
- */
-template <typename Get>
-__device__ uint64_t block_kth(Get get, uint32_t n, uint32_t k, uint32_t *hist,
-                              uint32_t *tmp, uint32_t *bc) {
-    uint64_t prefix = 0;
-    const int t = threadIdx.x;
-    for (int shift = 56; shift >= 0; shift -= 8) {
-        hist[t] = 0;
-        __syncthreads();
-        for (uint32_t i = t; i < n; i += blockDim.x) {
-            const uint64_t key = get(i);
-            if (shift == 56 || (key >> (shift + 8)) == prefix)
-                atomicAdd(&hist[(key >> shift) & 255u], 1u);
-        }
-        __syncthreads();
-        const uint32_t c = hist[t];
-        const uint32_t incl = block_scan_incl(c, tmp);
-        const uint32_t excl = incl - c;
-        if (excl < k && k <= incl) {
-            bc[0] = t;
-            bc[1] = excl;
-            bc[2] = c;
-        }
-        __syncthreads();
-        const uint32_t dg = bc[0], ex = bc[1], cnt = bc[2];
-        prefix = (prefix << 8) | dg;
-        k -= ex;
-        __syncthreads();
-        if (cnt == 1 && shift > 0) {
-            /* unique key under this prefix: fetch it */
-            for (uint32_t i = t; i < n; i += blockDim.x) {
-                const uint64_t key = get(i);
-                if ((key >> shift) == prefix) {
-                    bc[4] = (uint32_t)key;
-                    bc[5] = (uint32_t)(key >> 32);
-                }
-            }
-            __syncthreads();
-            const uint64_t r = ((uint64_t)bc[5] << 32) | bc[4];
-            __syncthreads();
-            return r;
-        }
-    }
-    return prefix;
-}
-
 /*
  * Gathers into sel[] every key that is <= an upper bound of the B-th
  * smallest (B <= 64) of n keys, over a 256-thread block. Bound: each wave
@@ -1365,26 +1302,6 @@ __device__ uint64_t block_kth_small(Get get, uint32_t n, uint32_t B, uint64_t *s
     if (m <= 64) return sel[B - 1];
     if (m <= cap) return block_kth([&](uint32_t i) { return sel[i]; }, m, B, hist, tmp, bc);
     return block_kth(get, n, B, hist, tmp, bc);
-}
-
-/* ascending bitonic sort of n (power of two) keys in LDS */
-__device__ void block_bitonic(uint64_t *a, uint32_t n) {
-    for (uint32_t k = 2; k <= n; k <<= 1) {
-        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-            for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
-                const uint32_t ixj = i ^ j;
-                if (ixj > i) {
-                    const bool up = (i & k) == 0;
-                    const uint64_t x = a[i], y = a[ixj];
-                    if ((x > y) == up) {
-                        a[i] = y;
-                        a[ixj] = x;
-                    }
-                }
-            }
-            __syncthreads();
-        }
-    }
 }
 
 /* ---------------- 4. threshold from the pre-pass ---------------- */
@@ -1688,6 +1605,7 @@ void free_batch_buffers(FvGpuBatch *b) {
     b->d_flag.release();
     b->h_out.release();
     b->d_visit.release();
+    b->d_fspill.release();
     b->d_progcnt.release();
     b->d_feat.release();
     b->d_y.release();

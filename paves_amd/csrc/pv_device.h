@@ -88,15 +88,21 @@ __device__ __forceinline__ uint32_t wave_scan_incl(uint32_t v) {
  * Scalar tail of the reference l2sq (dims past the last 16-dim block) as
  * the reference compiler emits `sum += d * d` (-O3): with >= 8 elements
  * left, one 8-wide in-order block with separately rounded products, then a
- * fused multiply-add per remaining element.
+ * fused multiply-add per remaining element. Contraction is off here: the
+ * library builds with -ffp-contract=fast, which would otherwise fuse the
+ * 8-wide block's separately rounded products into the sums.
  */
 __device__ __forceinline__ float tail_l2(const float *__restrict__ x,
                                          const float *__restrict__ q, int i,
                                          int dim, float t) {
+#pragma clang fp contract(off)
     if (dim - i >= 8) {
         for (int j = 0; j < 8; j++, i++) {
-            const float d = __fsub_rn(x[i], q[i]);
-            t = __fadd_rn(t, __fmul_rn(d, d));
+            /* plain operators: this block's contract(off) applies to them
+             * (the __f*_rn helpers are compiled with the file's setting) */
+            const float d = x[i] - q[i];
+            const float p = d * d;
+            t = t + p;
         }
     }
     for (; i < dim; i++) {
@@ -119,6 +125,7 @@ __device__ __forceinline__ float tail_l2(const float *__restrict__ x,
 __device__ __forceinline__ float cpu_order_l2(const float *__restrict__ x,
                                               const float *__restrict__ q,
                                               int dim, int lane16) {
+#pragma clang fp contract(off)
     float acc0 = 0.f, acc1 = 0.f;
     int i = 0;
     for (; i + 32 <= dim; i += 32) {
@@ -600,6 +607,89 @@ __device__ __forceinline__ void bitmap_rows(uint64_t block, int nprog, int nops,
         }
         if (lane == 0) s_pc[wv][gs] = pc;
         if (gs == kBmGroup - 1 || p + 4 >= nprog) flush(q - gs, gs + 1);
+    }
+}
+
+/* ---- block-level select and sort (256-thread blocks) ---- */
+
+/* inclusive prefix sum over a 256-thread block */
+__device__ __forceinline__ uint32_t block_scan_incl(uint32_t v, uint32_t *tmp) {
+    const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+    v = wave_scan_incl(v);
+    if (l == 63) tmp[w] = v;
+    __syncthreads();
+    uint32_t add = 0;
+    for (int i = 0; i < w; i++) add += tmp[i];
+    __syncthreads();
+    return v + add;
+}
+
+/*
+ * k-th smallest (1-based) of n unique u64 keys, 8-bit MSD radix select
+ * over a 256-thread block. Keys are read through `get`.
+ */
+template <typename Get>
+__device__ uint64_t block_kth(Get get, uint32_t n, uint32_t k, uint32_t *hist,
+                              uint32_t *tmp, uint32_t *bc) {
+    uint64_t prefix = 0;
+    const int t = threadIdx.x;
+    for (int shift = 56; shift >= 0; shift -= 8) {
+        hist[t] = 0;
+        __syncthreads();
+        for (uint32_t i = t; i < n; i += blockDim.x) {
+            const uint64_t key = get(i);
+            if (shift == 56 || (key >> (shift + 8)) == prefix)
+                atomicAdd(&hist[(key >> shift) & 255u], 1u);
+        }
+        __syncthreads();
+        const uint32_t c = hist[t];
+        const uint32_t incl = block_scan_incl(c, tmp);
+        const uint32_t excl = incl - c;
+        if (excl < k && k <= incl) {
+            bc[0] = t;
+            bc[1] = excl;
+            bc[2] = c;
+        }
+        __syncthreads();
+        const uint32_t dg = bc[0], ex = bc[1], cnt = bc[2];
+        prefix = (prefix << 8) | dg;
+        k -= ex;
+        __syncthreads();
+        if (cnt == 1 && shift > 0) {
+            /* unique key under this prefix: fetch it */
+            for (uint32_t i = t; i < n; i += blockDim.x) {
+                const uint64_t key = get(i);
+                if ((key >> shift) == prefix) {
+                    bc[4] = (uint32_t)key;
+                    bc[5] = (uint32_t)(key >> 32);
+                }
+            }
+            __syncthreads();
+            const uint64_t r = ((uint64_t)bc[5] << 32) | bc[4];
+            __syncthreads();
+            return r;
+        }
+    }
+    return prefix;
+}
+
+/* ascending bitonic sort of n (power of two) keys in LDS */
+__device__ inline void block_bitonic(uint64_t *a, uint32_t n) {
+    for (uint32_t k = 2; k <= n; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+                const uint32_t ixj = i ^ j;
+                if (ixj > i) {
+                    const bool up = (i & k) == 0;
+                    const uint64_t x = a[i], y = a[ixj];
+                    if ((x > y) == up) {
+                        a[i] = y;
+                        a[ixj] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
     }
 }
 

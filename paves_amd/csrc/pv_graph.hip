@@ -9,24 +9,34 @@
  * entry is worse than the worst result with the list full, when visited
  * reaches max_visited (exhausted), or when the frontier drains (exhausted).
  *
- * CDNA4 design: one 64-lane wave per query (a block is one wave, so every
- * barrier is wave-local and cheap; many queries per CU hide HBM latency).
+ * CDNA4 design: one 256-thread workgroup (four waves) per query, so a
+ * query's per-iteration work is spread over 256 lanes and a 64-query batch
+ * still puts 256 waves on the chip.
  *   - frontier and results are sorted key arrays in LDS (key = dist bits
- *     << 32 | node, so key order = (dist, node) order, the CPU heaps'
- *     pair order); new entries are merged wave-parallel (bitonic sort of
- *     the new keys, then merge-path ranks by binary search), truncated to
- *     the frontier capacity / ef;
- *   - visited: an LDS open-addressing hash (linear probing, bounded probe
- *     length); a node whose probe window is full goes to a per-query
- *     global bitmap that the wave zero-fills the first time it needs it —
- *     exact membership, no pre-launch memset;
- *   - each iteration pops up to W frontier heads that pass the stop test,
- *     gathers their padded neighbour rows (one 128-byte line per node for
- *     M0 = 32), and computes exact FP32 distances in the CPU cursor's
- *     summation order, 16 lanes per node, 32 nodes per memory round trip;
+ *     << 32 | node, so key order = (dist, node) order, the CPU heaps' pair
+ *     order). Each iteration's admitted keys are sorted per wave in
+ *     registers (64-key bitonic on DPP / swizzle / permlane exchanges: the
+ *     wave's lane holds its own node's key, so no compaction is needed) and
+ *     merged by rank: an entry's new position is its own index plus the
+ *     number of keys below it in every other sorted run (binary searches in
+ *     LDS, all independent), so a merge costs two barriers;
+ *   - the frontier keeps at most fcap entries (the reference GPU kernel's
+ *     FV_FRONTIER_CAP, gpu_engine.cu:1069); entries pushed past it are
+ *     dropped and counted (debug word 3), the only way the search can
+ *     depart from the unbounded CPU cursor;
+ *   - visited: a 4096-slot LDS open-addressing hash (linear probing,
+ *     bounded probe length); a node whose probe window is full goes to a
+ *     per-query global bitmap that the workgroup zero-fills the first time
+ *     it needs it — exact membership, no pre-launch memset;
+ *   - each iteration pops up to W frontier heads that pass the stop test;
+ *     thread t reads neighbour slot t of the popped heads' padded lists (one
+ *     128-byte line per head for M0 = 32), test-and-sets it, and the new
+ *     nodes' rows are read with exact FP32 distances in the CPU cursor's
+ *     summation order, 16 lanes per node, all row loads of a pass issued
+ *     before the arithmetic;
  *   - the predicate is a bit test in the batch's predicate bitmap.
- * With W = 1 and a frontier that never overflows, the search visits the
- * nodes the CPU cursor visits and returns the same list.
+ * With W = 1 and a frontier that never drops an entry, the search visits
+ * the nodes the CPU cursor visits and returns the same list.
  */
 #include <math.h>
 #include <string.h>
@@ -39,11 +49,12 @@ typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
 
 namespace pv {
 
-constexpr int kHashBits = 11;
-constexpr int kHashSize = 1 << kHashBits; /* 2048 slots */
+constexpr int kGW = 4;                    /* waves per query workgroup */
+constexpr int kGT = kGW * 64;             /* threads per query workgroup */
+constexpr int kHashBits = 12;
+constexpr int kHashSize = 1 << kHashBits; /* 4096 slots */
 constexpr int kProbe = 16;
-constexpr int kMaxNew = 256;              /* W * nbr_stride <= 256 (W <= 8 at degree 32) */
-constexpr int kNS = kMaxNew / 64;         /* new-node slots per lane */
+constexpr int kMaxNew = kGT;              /* W * nbr_stride <= 256 (W <= 8 at degree 32) */
 
 struct GraphArgs {
     const float *vecs;
@@ -69,7 +80,9 @@ struct GraphArgs {
     const uint8_t *route; /* optional: queries with route 0 are left alone */
     const uint32_t *efq;  /* optional per-query ef (<= ef, the LDS sizing) */
     uint32_t out_k;       /* hits written per query (stride); 0 = ef */
-    uint32_t *dbg;       /* [nq][4] iterations, visited, results, overflow */
+    uint64_t *spill;     /* [nq][scap + fcap] frontier spill stores */
+    uint32_t scap;       /* spilled entries a store accepts */
+    uint32_t *dbg;       /* [nq][4] iterations, visited, results, frontier entries lost */
 };
 
 __device__ __forceinline__ uint32_t hash_slot(uint32_t node) {
@@ -77,7 +90,9 @@ __device__ __forceinline__ uint32_t hash_slot(uint32_t node) {
 }
 
 /* Test-and-set in the LDS hash; returns 1 if newly inserted, 0 if already
- * present, 2 if the probe window is full (caller uses the global bitmap). */
+ * present, 2 if the probe window is full (caller uses the global bitmap).
+ * Slots are never cleared, so a node whose window is full stays out of the
+ * hash for the whole search and is always found in the bitmap. */
 __device__ __forceinline__ int hash_insert(uint32_t *h, uint32_t node) {
     uint32_t s = hash_slot(node);
     for (int p = 0; p < kProbe; p++) {
@@ -87,40 +102,6 @@ __device__ __forceinline__ int hash_insert(uint32_t *h, uint32_t node) {
         if (old == node) return 0;
     }
     return 2;
-}
-
-/* ascending sort of the first m (<= kMaxNew) keys in LDS by one wave,
- * padding with ~0 up to the next power of two >= 64: up to 64 keys in
- * registers (shuffle bitonic, no barrier), else a bitonic pass over 128 or
- * 256 entries in LDS. Entries m.. hold ~0 on entry. */
-__device__ __forceinline__ void wave_sort_new(uint64_t *a, int m) {
-    const int l = threadIdx.x;
-    if (m <= 64) {
-        uint64_t x = a[l];
-        x = wave_sort64(x);
-        a[l] = x;
-        __syncthreads();
-        return;
-    }
-    const int n = m <= 128 ? 128 : 256;
-    for (int k = 2; k <= n; k <<= 1) {
-        for (int j = k >> 1; j > 0; j >>= 1) {
-#pragma unroll
-            for (int h = 0; h < kNS; h++) {
-                const int i = l + 64 * h;
-                const int ixj = i ^ j;
-                if (i < n && ixj > i) {
-                    const bool up = (i & k) == 0;
-                    const uint64_t x = a[i], y = a[ixj];
-                    if ((x > y) == up) {
-                        a[i] = y;
-                        a[ixj] = x;
-                    }
-                }
-            }
-            __syncthreads();
-        }
-    }
 }
 
 /* number of entries of sorted a[0..n) strictly less than key */
@@ -133,68 +114,209 @@ __device__ __forceinline__ int lower_rank(const uint64_t *a, int n, uint64_t key
     return lo;
 }
 
+/* number of entries of sorted a[0..n) at or below key (key < ~0) */
+__device__ __forceinline__ int upper_rank(const uint64_t *a, int n, uint64_t key) {
+    return lower_rank(a, n, key + 1);
+}
+
+/* per-workgroup scalars shared through LDS */
+struct GraphShared {
+    int wcnt[kGW];       /* new nodes per wave (compaction) */
+    int rcF[kGW];        /* admitted keys per wave */
+    int rcR[kGW];        /* passing admitted keys per wave */
+    uint64_t smin;       /* smallest key a merge spilled */
+    uint64_t wmin[kGW];  /* per-wave minima (refill) */
+    uint32_t wsum[kGW];  /* per-wave sums (refill) */
+    uint32_t nsel, nrest;
+    uint32_t hist[256], tmp[8], bc[8];  /* radix select (refill) */
+};
+
 /*
- * Merge: sorted A[skip .. n) with the m unsorted keys nw[0..m) (m <= kMaxNew,
- * slots m..kMaxNew-1 must hold ~0) into A[0 .. min(cap, n-skip+m)), ascending.
- * Keys are unique. Returns the new count. A may hold up to cap (<= 1024)
- * entries; every lane reads its elements before any write.
+ * Merge the sorted entries A[skip .. n) with kGW sorted runs of new keys
+ * (run r: runs[64 r .. 64 r + rc[r]), wave r's keys, ~0-padded). Keys are
+ * unique across A and the runs. Keys above lim are dead (they can never be
+ * popped) and are discarded. Of the `live` remaining entries, in ascending
+ * order, the first cap go to A[0 ..) and the rest to spill[nspill ..)
+ * (those beyond scap are lost); S.smin receives the first spilled key. An
+ * entry's position is its index in its own sequence plus its rank in each
+ * of the others. Called by all kGT threads with identical arguments; every
+ * thread reads its entries before any write. Returns live.
  */
-__device__ int wave_merge(uint64_t *A, int n, int skip, uint64_t *nw, int m,
-                          int cap) {
-    const int l = threadIdx.x;
-    if (m > 0) wave_sort_new(nw, m);
+__device__ int block_merge(uint64_t *A, int n, int skip, const uint64_t *runs, const int (&rc)[kGW],
+                           int cap, uint64_t lim, uint64_t *spill, uint32_t nspill, uint32_t scap,
+                           GraphShared &S) {
+    const int t = threadIdx.x, w = t >> 6, l = t & 63;
     const int nold = n - skip;
-    uint64_t ov[16];
-    int op[16];
+    /* live entries of each sequence (uniform) */
+    const bool all_live = lim == ~0ull;
+    const int lold = all_live ? nold : upper_rank(A + skip, nold, lim);
+    int live = lold;
+    int lrc[kGW];
 #pragma unroll
-    for (int k = 0; k < 16; k++) {
-        const int i = l + 64 * k;
+    for (int r = 0; r < kGW; r++) {
+        lrc[r] = (all_live || rc[r] == 0) ? rc[r] : upper_rank(runs + 64 * r, rc[r], lim);
+        live += lrc[r];
+    }
+    constexpr int kOld = 1024 / kGT;   /* A holds at most 1024 entries */
+    uint64_t ov[kOld];
+    int op[kOld];
+#pragma unroll
+    for (int k = 0; k < kOld; k++) {
+        const int i = t + kGT * k;
         op[k] = -1;
-        if (i < nold) {
+        if (i < lold) {
             ov[k] = A[skip + i];
-            op[k] = i + lower_rank(nw, m, ov[k]);
+            int p = i;
+#pragma unroll
+            for (int r = 0; r < kGW; r++)
+                if (lrc[r] > 0) p += lower_rank(runs + 64 * r, lrc[r], ov[k]);
+            op[k] = p;
         }
     }
-    uint64_t nv[kNS];
-    int np[kNS];
+    uint64_t nv = ~0ull;
+    int np = -1;
+    if (l < lrc[w]) {
+        nv = runs[64 * w + l];
+        int p = l + lower_rank(A + skip, lold, nv);
 #pragma unroll
-    for (int k = 0; k < kNS; k++) {
-        const int j = l + 64 * k;
-        np[k] = -1;
-        if (j < m) {
-            nv[k] = nw[j];
-            np[k] = j + lower_rank(A + skip, nold, nv[k]);
-        }
+        for (int r = 0; r < kGW; r++)
+            if (r != w && lrc[r] > 0) p += lower_rank(runs + 64 * r, lrc[r], nv);
+        np = p;
     }
     __syncthreads();
+    auto put = [&](int p, uint64_t key) {
+        if (p < cap) {
+            A[p] = key;
+        } else if (spill != nullptr) {
+            const uint32_t s = nspill + (uint32_t)(p - cap);
+            if (s < scap) spill[s] = key;
+            if (p == cap) S.smin = key;
+        }
+    };
 #pragma unroll
-    for (int k = 0; k < 16; k++)
-        if (op[k] >= 0 && op[k] < cap) A[op[k]] = ov[k];
-#pragma unroll
-    for (int k = 0; k < kNS; k++)
-        if (np[k] >= 0 && np[k] < cap) A[np[k]] = nv[k];
+    for (int k = 0; k < kOld; k++)
+        if (op[k] >= 0) put(op[k], ov[k]);
+    if (np >= 0) put(np, nv);
+    __threadfence_block();
     __syncthreads();
-    return min(cap, nold + m);
+    return live;
+}
+
+/* sum / minimum over the workgroup (uniform result) */
+__device__ __forceinline__ uint32_t block_sum(uint32_t v, GraphShared &S) {
+#pragma unroll
+    for (int o = 32; o; o >>= 1) v += lane_xor(v, o);
+    if ((threadIdx.x & 63) == 0) S.wsum[threadIdx.x >> 6] = v;
+    __syncthreads();
+    uint32_t s = 0;
+#pragma unroll
+    for (int r = 0; r < kGW; r++) s += S.wsum[r];
+    __syncthreads();
+    return s;
+}
+__device__ __forceinline__ uint64_t block_min(uint64_t v, GraphShared &S) {
+#pragma unroll
+    for (int o = 32; o; o >>= 1) {
+        const uint64_t u = lane_xor(v, o);
+        v = u < v ? u : v;
+    }
+    if ((threadIdx.x & 63) == 0) S.wmin[threadIdx.x >> 6] = v;
+    __syncthreads();
+    uint64_t m = ~0ull;
+#pragma unroll
+    for (int r = 0; r < kGW; r++) m = S.wmin[r] < m ? S.wmin[r] : m;
+    __syncthreads();
+    return m;
 }
 
 /*
- * Exact CPU-order distances for up to NPG nodes per 16-lane group (4 NPG
- * nodes per call): all row loads are issued before any arithmetic so one
- * call costs one memory round trip. The summation order is the one of
- * cpu_order_l2 (engine.cpp:43-63).
+ * The frontier's spill store O holds live entries that did not fit in the
+ * LDS frontier; omin is its smallest key. When omin beats the LDS head (or
+ * the LDS frontier is empty), the LDS entries join the store and the
+ * smallest min(live, cap / 2) entries of the store come back to LDS,
+ * sorted: the k-th smallest by radix select over the store, a partition
+ * pass (in chunks read before they are overwritten: the store is compacted
+ * in place), dead entries (above lim) dropped, then a bitonic sort in LDS.
+ * cap is a power of two; the store has room for cap entries beyond the
+ * spills it accepts. Rare: only a frontier that outgrows LDS (a result
+ * list that fills slowly under a selective filter) spills.
+ */
+__device__ void frontier_refill(uint64_t *F, int &nF, uint64_t *O, uint32_t &nO, uint64_t &omin,
+                                uint64_t lim, int cap, bool &pruned, GraphShared &S) {
+    const int t = threadIdx.x;
+    for (int i = t; i < nF; i += kGT) O[nO + i] = F[i];
+    const uint32_t n = nO + (uint32_t)nF;
+    __threadfence_block();
+    __syncthreads();
+    uint32_t c = 0;
+    for (uint32_t i = t; i < n; i += kGT) c += O[i] <= lim ? 1u : 0u;
+    const uint32_t live = block_sum(c, S);
+    if (live < n) pruned = true;
+    if (live == 0) {
+        nF = 0;
+        nO = 0;
+        omin = ~0ull;
+        return;
+    }
+    const uint32_t k = min(live, (uint32_t)cap / 2);
+    const uint64_t kth = block_kth([&](uint32_t i) { return O[i]; }, n, k, S.hist, S.tmp, S.bc);
+    if (t == 0) {
+        S.nsel = 0;
+        S.nrest = 0;
+    }
+    __syncthreads();
+    uint64_t rmin = ~0ull;
+    for (uint32_t b = 0; b < n; b += 4 * kGT) {
+        uint64_t v[4];
+        bool ok[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint32_t i = b + t + kGT * j;
+            ok[j] = i < n;
+            v[j] = ok[j] ? O[i] : ~0ull;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            if (!ok[j]) continue;
+            if (v[j] <= kth) {
+                F[atomicAdd(&S.nsel, 1u)] = v[j];
+            } else if (v[j] <= lim) {
+                O[atomicAdd(&S.nrest, 1u)] = v[j];
+                rmin = v[j] < rmin ? v[j] : rmin;
+            }
+        }
+        __threadfence_block();
+        __syncthreads();
+    }
+    nF = (int)S.nsel;
+    nO = S.nrest;
+    omin = block_min(rmin, S);
+    uint32_t np2 = 2;
+    while (np2 < (uint32_t)nF) np2 <<= 1;
+    for (uint32_t i = (uint32_t)nF + t; i < np2; i += kGT) F[i] = ~0ull;
+    __syncthreads();
+    block_bitonic(F, np2);
+}
+
+/*
+ * Exact CPU-order distances of nodes[base + g + 16 k] (k < NPG) by the
+ * sixteen 16-lane groups g of the workgroup: all row loads are issued
+ * before any arithmetic, so one call costs one memory round trip. The
+ * summation order is the one of cpu_order_l2 (engine.cpp:43-63).
  */
 template <int NPG>
 __device__ __forceinline__ void dist_nodes(const GraphArgs &A, const float *qsh,
                                            const uint32_t *nodes, int nn, int base,
                                            float *outd) {
-    const int l = threadIdx.x, g = l >> 4, l16 = l & 15;
+    const int t = threadIdx.x, g = t >> 4, l16 = t & 15;
     const int D = A.dim;
     float acc0[NPG], acc1[NPG];
     const float *rp[NPG];
     bool ok[NPG];
 #pragma unroll
     for (int k = 0; k < NPG; k++) {
-        const int j = base + g + 4 * k;
+        const int j = base + g + 16 * k;
         ok[k] = j < nn;
         rp[k] = A.vecs + (uint64_t)(ok[k] ? nodes[j] : 0) * D;
         acc0[k] = 0.f;
@@ -238,54 +360,125 @@ __device__ __forceinline__ void dist_nodes(const GraphArgs &A, const float *qsh,
     }
 #pragma unroll
     for (int k = 0; k < NPG; k++) {
-        float t = __fadd_rn(acc0[k], acc1[k]);
-        t = __fadd_rn(t, lane_xor(t, 8));
-        t = __fadd_rn(t, lane_xor(t, 4));
-        t = __fadd_rn(t, lane_xor(t, 2));
-        t = __fadd_rn(t, lane_xor(t, 1));
-        t = tail_l2(rp[k], qsh, i, D, t);
-        const int j = base + g + 4 * k;
-        if (ok[k] && l16 == 0) outd[j] = t;
+        float s = __fadd_rn(acc0[k], acc1[k]);
+        s = __fadd_rn(s, lane_xor(s, 8));
+        s = __fadd_rn(s, lane_xor(s, 4));
+        s = __fadd_rn(s, lane_xor(s, 2));
+        s = __fadd_rn(s, lane_xor(s, 1));
+        s = tail_l2(rp[k], qsh, i, D, s);
+        const int j = base + g + 16 * k;
+        if (ok[k] && l16 == 0) outd[j] = s;
     }
 }
 
-__global__ __launch_bounds__(64) void k_graph(GraphArgs A) {
+/*
+ * 128-dim rows (the common case): the same arithmetic as dist_nodes<4>, with
+ * the query slice in registers and two row buffers, so the loads of the
+ * next 64 nodes are in flight while the current 64 are reduced.
+ */
+__device__ __forceinline__ void dist_all_128(const GraphArgs &A, const float *qsh,
+                                             const uint32_t *nodes, int nn, float *outd) {
+    const int t = threadIdx.x, g = t >> 4, l16 = t & 15;
+    float qv[8];
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+        qv[2 * b] = qsh[32 * b + l16];
+        qv[2 * b + 1] = qsh[32 * b + 16 + l16];
+    }
+    float xa[4][8], xb[4][8];
+    auto load = [&](float (&x)[4][8], int base) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int j = base + g + 16 * k;
+            const float *rp = A.vecs + (uint64_t)(j < nn ? nodes[j] : 0) * 128;
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                x[k][2 * b] = rp[32 * b + l16];
+                x[k][2 * b + 1] = rp[32 * b + 16 + l16];
+            }
+        }
+    };
+    auto reduce = [&](const float (&x)[4][8], int base) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            float a0 = 0.f, a1 = 0.f;
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                const float d0 = __fsub_rn(x[k][2 * b], qv[2 * b]);
+                const float d1 = __fsub_rn(x[k][2 * b + 1], qv[2 * b + 1]);
+                a0 = __fmaf_rn(d0, d0, a0);
+                a1 = __fmaf_rn(d1, d1, a1);
+            }
+            float s = __fadd_rn(a0, a1);
+            s = __fadd_rn(s, lane_xor(s, 8));
+            s = __fadd_rn(s, lane_xor(s, 4));
+            s = __fadd_rn(s, lane_xor(s, 2));
+            s = __fadd_rn(s, lane_xor(s, 1));
+            const int j = base + g + 16 * k;
+            if (j < nn && l16 == 0) outd[j] = s;
+        }
+    };
+    int base = 0;
+    load(xa, 0);
+    for (;;) {
+        if (base + 64 < nn) load(xb, base + 64);
+        reduce(xa, base);
+        base += 64;
+        if (base >= nn) break;
+        if (base + 64 < nn) load(xa, base + 64);
+        reduce(xb, base);
+        base += 64;
+        if (base >= nn) break;
+    }
+}
+
+/* distances of nodes[0 .. nn): 64 per round trip */
+__device__ __forceinline__ void dist_all(const GraphArgs &A, const float *qsh,
+                                         const uint32_t *nodes, int nn, float *outd) {
+    if (A.dim == 128) {
+        if (nn > 0) dist_all_128(A, qsh, nodes, nn, outd);
+        return;
+    }
+    for (int base = 0; base < nn; base += 64) dist_nodes<4>(A, qsh, nodes, nn, base, outd);
+}
+
+
+__global__ __launch_bounds__(kGT) __attribute__((amdgpu_waves_per_eu(3)))
+void k_graph(GraphArgs A) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
-    const int q = blockIdx.x, l = threadIdx.x;
+    const int q = blockIdx.x, t = threadIdx.x, w = t >> 6, l = t & 63;
     if (A.route && A.route[q] == 0) return; /* routed to the brute path */
     const uint32_t efq = A.efq ? max(1u, min(A.efq[q], A.ef)) : A.ef;
     const int D = A.dim;
     float *qsh = (float *)smem;                                   /* [D] */
     uint64_t *F = (uint64_t *)(smem + (((size_t)D * 4 + 15) & ~(size_t)15)); /* [fcap] */
     uint64_t *R = F + A.fcap;                                     /* [ef] */
-    uint64_t *nwF = R + A.ef;                                     /* [kMaxNew] */
-    uint64_t *nwR = nwF + kMaxNew;                                /* [kMaxNew] */
-    uint32_t *H = (uint32_t *)(nwR + kMaxNew);                    /* [2048] */
+    uint64_t *runF = R + A.ef;                                    /* [kGW][64] */
+    uint64_t *runR = runF + kGT;                                  /* [kGW][64] */
+    uint32_t *H = (uint32_t *)(runR + kGT);                       /* [kHashSize] */
     uint32_t *nb = H + kHashSize;                                 /* [kMaxNew] new nodes */
     float *nd = (float *)(nb + kMaxNew);                          /* [kMaxNew] */
-    uint32_t *pops = (uint32_t *)(nd + kMaxNew);                  /* [8] */
+    GraphShared &S = *(GraphShared *)(nd + kMaxNew);
 
     const FvGpuQueryDesc qd = A.qd[q];
     const int prog = A.pofq[q];
     const uint32_t *bm = prog >= 0 ? A.bitmap + (uint64_t)prog * A.words : nullptr;
     uint32_t *vb = A.visit + (uint64_t)q * A.vwords;
+    uint64_t *O = A.spill + (uint64_t)q * (A.scap + (uint32_t)A.fcap); /* frontier spill store */
 
-    for (int j = l; j < D; j += 64) qsh[j] = A.qf[(uint64_t)q * D + j];
-    for (int j = l; j < kHashSize; j += 64) H[j] = kNoNode;
-    __syncthreads();
-
+    for (int j = t; j < D; j += kGT) qsh[j] = A.qf[(uint64_t)q * D + j];
+    for (int j = t; j < kHashSize; j += kGT) H[j] = kNoNode;
     uint32_t entry = qd.entry;
     if (entry >= A.count) entry = 0;
-    /* entry point: distance by lanes 0..15 (others compute a masked copy) */
-    nb[0] = entry;
+    if (t == 0) nb[0] = entry;
     __syncthreads();
-    dist_nodes<4>(A, qsh, nb, 1, 0, nd);
-    if (l == 0) hash_insert(H, entry);
+    dist_nodes<1>(A, qsh, nb, 1, 0, nd);
+    if (t == 0) hash_insert(H, entry);
     __syncthreads();
     int nF = 1, nR = 0;
     const float ed = nd[0];
     const bool epass = bm == nullptr || ((bm[entry >> 5] >> (entry & 31)) & 1u);
-    if (l == 0) {
+    if (t == 0) {
         F[0] = make_key(ed, entry);
         if (epass) R[0] = make_key(ed, entry);
     }
@@ -295,17 +488,27 @@ __global__ __launch_bounds__(64) void k_graph(GraphArgs A) {
     uint64_t visited = 1;
     int exh = 0;
     bool vb_ready = false;
-    uint32_t iters = 0, ovf = 0;
+    bool pruned = false;        /* dead frontier entries were discarded */
+    uint32_t nO = 0, lost = 0;  /* spilled entries; live entries beyond the store */
+    uint64_t omin = ~0ull;      /* smallest spilled key */
+    uint32_t iters = 0;
     const int nbs = A.nbr_stride;
 
     for (;;) {
+        const bool full = nR >= (int)efq;
+        const uint64_t worst = nR > 0 ? R[nR - 1] : ~0ull;
+        /* frontier entries above lim can never be popped: when the list is
+         * full the CPU cursor stops at them (engine.cpp:1666) */
+        const uint64_t lim = full ? make_key(key_dist(worst), kNoNode) : ~0ull;
+        if (nO > 0 && (nF == 0 || omin < F[0]))
+            frontier_refill(F, nF, O, nO, omin, lim, A.fcap, pruned, S);
         if (nF == 0) {
-            exh = 1;
+            /* frontier drained; with discarded dead entries the CPU cursor's
+             * heap still holds one, and it stops on the full-list test */
+            exh = pruned ? 0 : 1;
             break;
         }
         const uint64_t c0 = F[0];
-        const uint64_t worst = nR > 0 ? R[nR - 1] : ~0ull;
-        const bool full = nR >= (int)efq;
         if (full && key_dist(c0) > key_dist(worst)) break;
         if (visited >= A.max_visited) {
             exh = 1;
@@ -316,136 +519,122 @@ __global__ __launch_bounds__(64) void k_graph(GraphArgs A) {
             break;
         }
         iters++;
-        /* pop up to W heads that pass the stop test */
-        int w = 1;
-        while (w < A.wmax && w < nF && (!full || key_dist(F[w]) <= key_dist(worst))) w++;
-        if (l < w) pops[l] = key_row(F[l]);
-        __syncthreads();
+        /* pop up to W heads that pass the stop test and precede every
+         * spilled entry (uniform) */
+        int nh = 1;
+        while (nh < A.wmax && nh < nF && F[nh] < omin &&
+               (!full || key_dist(F[nh]) <= key_dist(worst)))
+            nh++;
 
-        /* gather neighbours of the popped nodes: w * nbs <= kMaxNew slots */
-        const int nslot = w * nbs;
-        int isnew[kNS];
-        uint32_t cand[kNS];
-#pragma unroll
-        for (int h = 0; h < kNS; h++) {
-            const int s = l + 64 * h;
-            cand[h] = kNoNode;
-            isnew[h] = 0;
-            if (s < nslot) cand[h] = A.nbr[(uint64_t)pops[s / nbs] * nbs + (s % nbs)];
-        }
-        /* visited test-and-set */
-        bool need_vb = false;
-#pragma unroll
-        for (int h = 0; h < kNS; h++) {
-            if (cand[h] == kNoNode) continue;
-            const int r = hash_insert(H, cand[h]);
-            if (r == 1) isnew[h] = 1;
-            else if (r == 2) {
-                isnew[h] = 2;
-                need_vb = true;
-            }
-        }
-        if (__any(need_vb)) {
+        /* neighbour slot t of the popped heads, then its visited test-and-set */
+        uint32_t cand = kNoNode;
+        if (t < nh * nbs) cand = A.nbr[(uint64_t)key_row(F[t / nbs]) * nbs + (t % nbs)];
+        int isnew = 0;
+        if (cand != kNoNode) isnew = hash_insert(H, cand);
+        if (__syncthreads_or(isnew == 2)) {
             if (!vb_ready) {
-                /* first overflow: zero this query's bitmap */
+                /* first overflow of this search: zero this query's bitmap */
                 u32x4_t z = {0u, 0u, 0u, 0u};
                 const uint64_t n4 = A.words / 4;
-                for (uint64_t i = l; i < n4; i += 64) ((u32x4_t *)vb)[i] = z;
-                for (uint64_t i = n4 * 4 + l; i < A.words; i += 64) vb[i] = 0;
-                __threadfence_block();
+                for (uint64_t i = t; i < n4; i += kGT) ((u32x4_t *)vb)[i] = z;
+                for (uint64_t i = n4 * 4 + t; i < A.words; i += kGT) vb[i] = 0;
+                __threadfence();
                 __syncthreads();
                 vb_ready = true;
             }
-            /* all test-and-sets in flight together */
-            uint32_t old[kNS];
-#pragma unroll
-            for (int h = 0; h < kNS; h++)
-                if (isnew[h] == 2)
-                    old[h] = atomicOr(&vb[cand[h] >> 5], 1u << (cand[h] & 31));
-#pragma unroll
-            for (int h = 0; h < kNS; h++)
-                if (isnew[h] == 2) {
-                    isnew[h] = (old[h] & (1u << (cand[h] & 31))) ? 0 : 1;
-                    ovf++;
-                }
+            if (isnew == 2) {
+                const uint32_t old = atomicOr(&vb[cand >> 5], 1u << (cand & 31));
+                isnew = (old & (1u << (cand & 31))) ? 0 : 1;
+            }
         }
-        /* compact new nodes */
+        /* compact the new nodes into nb[] (wave ballots + wave offsets) */
         const uint64_t lt = (l == 0) ? 0ull : (~0ull >> (64 - l));
-        int nn = 0;
+        const uint64_t bb = __ballot(isnew == 1);
+        if (l == 0) S.wcnt[w] = __popcll(bb);
+        __syncthreads();
+        int woff = 0, nn = 0;
 #pragma unroll
-        for (int h = 0; h < kNS; h++) {
-            const uint64_t bb = __ballot(isnew[h] == 1);
-            if (isnew[h] == 1) nb[nn + __popcll(bb & lt)] = cand[h];
-            nn += __popcll(bb);
+        for (int r = 0; r < kGW; r++) {
+            woff += r < w ? S.wcnt[r] : 0;
+            nn += S.wcnt[r];
         }
+        if (isnew == 1) nb[woff + __popcll(bb & lt)] = cand;
         __syncthreads();
         visited += (uint64_t)nn;
 
-        /* predicate words of the new nodes, requested with their rows (the
-         * admission below reads them after the distances) */
-        uint32_t pw[kNS];
-#pragma unroll
-        for (int h = 0; h < kNS; h++) {
-            const int j = l + 64 * h;
-            pw[h] = ~0u;
-            if (bm != nullptr && j < nn) pw[h] = bm[nb[j] >> 5];
-        }
-        /* distances: 32 nodes per round trip (16 for the last few) */
-        int base = 0;
-        for (; base + 16 < nn; base += 32) dist_nodes<8>(A, qsh, nb, nn, base, nd);
-        if (base < nn) dist_nodes<4>(A, qsh, nb, nn, base, nd);
+        /* predicate word of thread t's new node, requested with the rows */
+        const bool mine = t < nn;
+        const uint32_t node = mine ? nb[t] : 0u;
+        const uint32_t pw = (mine && bm != nullptr) ? bm[node >> 5] : ~0u;
+        dist_all(A, qsh, nb, nn, nd);
         __syncthreads();
 
-        /* admission against the worst result at iteration start */
+        /* admission against the worst result at iteration start; wave w's
+         * admitted (and passing) keys sorted in registers form run w */
         const float wd = key_dist(worst);
-        int mF = 0, mR = 0;
-        {
-            bool adm[kNS], pas[kNS];
-            uint64_t key[kNS];
-#pragma unroll
-            for (int h = 0; h < kNS; h++) {
-                const int j = l + 64 * h;
-                adm[h] = false;
-                pas[h] = false;
-                key[h] = ~0ull;
-                if (j < nn) {
-                    const float d = nd[j];
-                    const uint32_t node = nb[j];
-                    key[h] = make_key(d, node);
-                    adm[h] = !full || d < wd;
-                    if (adm[h]) pas[h] = ((pw[h] >> (node & 31)) & 1u) != 0u;
-                }
-            }
-            /* fill with ~0 first, then scatter */
-#pragma unroll
-            for (int h = 0; h < kNS; h++) {
-                nwF[l + 64 * h] = ~0ull;
-                nwR[l + 64 * h] = ~0ull;
-            }
-            __syncthreads();
-#pragma unroll
-            for (int h = 0; h < kNS; h++) {
-                const uint64_t ab = __ballot(adm[h]), pb = __ballot(pas[h]);
-                if (adm[h]) nwF[mF + __popcll(ab & lt)] = key[h];
-                if (pas[h]) nwR[mR + __popcll(pb & lt)] = key[h];
-                mF += __popcll(ab);
-                mR += __popcll(pb);
-            }
-            __syncthreads();
+        bool adm = false, pas = false;
+        uint64_t key = ~0ull;
+        if (mine) {
+            const float d = nd[t];
+            key = make_key(d, node);
+            adm = !full || d < wd;
+            pas = adm && (((pw >> (node & 31)) & 1u) != 0u);
         }
-        nF = wave_merge(F, nF, w, nwF, mF, A.fcap);
-        if (mR > 0) nR = wave_merge(R, nR, 0, nwR, mR, (int)efq);
+        const uint64_t sF = wave_sort64(adm ? key : ~0ull);
+        const uint64_t sR = wave_sort64(pas ? key : ~0ull);
+        runF[t] = sF;
+        runR[t] = sR;
+        const int cF = __popcll(__ballot(adm)), cR = __popcll(__ballot(pas));
+        if (l == 0) {
+            S.rcF[w] = cF;
+            S.rcR[w] = cR;
+        }
+        if (t == 0) S.smin = ~0ull;
+        __syncthreads();
+        int rcF[kGW], rcR[kGW], mF = 0, mR = 0;
+#pragma unroll
+        for (int r = 0; r < kGW; r++) {
+            rcF[r] = S.rcF[r];
+            rcR[r] = S.rcR[r];
+            mF += rcF[r];
+            mR += rcR[r];
+        }
+        /* results first: the frontier merge discards what the new worst
+         * result makes dead */
+        if (mR > 0)
+            nR = min((int)efq, block_merge(R, nR, 0, runR, rcR, (int)efq, ~0ull, nullptr, 0, 0, S));
+        const bool full2 = nR >= (int)efq;
+        const uint64_t lim2 = full2 ? make_key(key_dist(R[nR - 1]), kNoNode) : ~0ull;
+        const int nold = nF - nh;
+        const int live = block_merge(F, nF, nh, runF, rcF, A.fcap, lim2, O, nO, A.scap, S);
+        if (live < nold + mF) pruned = true;
+        nF = min(A.fcap, live);
+        if (live > A.fcap) {
+            /* the merge stored the first min(add, room) of them, the
+             * smallest one first */
+            const uint32_t add = (uint32_t)(live - A.fcap);
+            const uint32_t room = A.scap - min(nO, A.scap);
+            lost += add > room ? add - room : 0u;
+            nO += min(add, room);
+            if (room > 0) omin = S.smin < omin ? S.smin : omin;
+        }
+        if (full2 && nO > 0 && omin > lim2) {
+            /* every spilled entry is dead */
+            nO = 0;
+            omin = ~0ull;
+            pruned = true;
+        }
     }
 
     const uint32_t ostride = A.out_k ? A.out_k : A.ef;
     const int nout = min(nR, (int)ostride);
-    for (int i = l; i < nout; i += 64) {
+    for (int i = t; i < nout; i += kGT) {
         FvHit h;
         h.node = key_row(R[i]);
         h.dist = key_dist(R[i]);
         A.out[(uint64_t)q * ostride + i] = h;
     }
-    if (l == 0) {
+    if (t == 0) {
         A.out_cnt[q] = (uint32_t)nout;
         A.out_exh[q] = (uint8_t)exh;
         A.status[q] = 0;
@@ -453,14 +642,15 @@ __global__ __launch_bounds__(64) void k_graph(GraphArgs A) {
             A.dbg[(uint64_t)q * 4 + 0] = iters;
             A.dbg[(uint64_t)q * 4 + 1] = (uint32_t)min<uint64_t>(visited, 0xFFFFFFFFull);
             A.dbg[(uint64_t)q * 4 + 2] = (uint32_t)nR;
-            A.dbg[(uint64_t)q * 4 + 3] = ovf;
+            A.dbg[(uint64_t)q * 4 + 3] = lost;
         }
     }
 }
 
 size_t graph_shmem(int dim, int fcap, uint32_t ef) {
     return (((size_t)dim * 4 + 15) & ~(size_t)15) + (size_t)fcap * 8 + (size_t)ef * 8 +
-           2 * kMaxNew * 8 + kHashSize * 4 + kMaxNew * 4 + kMaxNew * 4 + 12 * 4;
+           2 * (size_t)kGT * 8 + kHashSize * 4 + kMaxNew * 4 + kMaxNew * 4 +
+           sizeof(GraphShared);
 }
 
 int run_graph_ex(FvGpuIndex *g, FvGpuBatch *b, uint32_t ef, uint64_t max_visited, int wmax,
@@ -470,16 +660,27 @@ int run_graph_ex(FvGpuIndex *g, FvGpuBatch *b, uint32_t ef, uint64_t max_visited
     if (require_bitmaps(b, errbuf, errlen) != 0) return -1;
     if (b->max_entry >= g->count)
         PV_FAIL("gpu: entry point %llu out of range", (unsigned long long)b->max_entry);
+    if (g->nbr_stride > kMaxNew) PV_FAIL("gpu: graph degree %d too large", g->nbr_stride);
     if (wmax < 1) wmax = 1;
     while (wmax > 1 && wmax * g->nbr_stride > kMaxNew) wmax--;
-    if (g->nbr_stride > kMaxNew) PV_FAIL("gpu: graph degree %d too large", g->nbr_stride);
-    if (fcap < 64) fcap = 64;
-    if (fcap > 1024) fcap = 1024;
+    /* the LDS frontier: a power of two in [64, 1024] (its refill sorts up
+     * to half of it with a bitonic network) */
+    fcap = std::min(1024, std::max(64, fcap));
+    while (fcap & (fcap - 1)) fcap &= fcap - 1;
     const size_t shm = graph_shmem(g->dim, fcap, ef);
     if (shm > 160 * 1024) PV_FAIL("gpu: graph search LDS %zu bytes exceeds the CU", shm);
     const uint64_t words = g->count_pad / 32;
     const uint64_t vwords = (words + 3) & ~3ull;
     PV_TRY(b->d_visit.ensure((size_t)b->nq * vwords * 4));
+    /* frontier spill stores: every node enters a frontier at most once, so
+     * min(max_visited, count) entries (plus one iteration's overshoot of
+     * max_visited) never overflow; the total is capped at 4 GiB, beyond
+     * which entries are lost and counted (debug word 3) */
+    uint64_t scap = std::min<uint64_t>(max_visited, g->count) + kMaxNew;
+    const uint64_t kStoreBytes = (uint64_t)4 << 30;
+    if ((uint64_t)b->nq * (scap + fcap) * 8 > kStoreBytes)
+        scap = std::max<uint64_t>(kStoreBytes / ((uint64_t)b->nq * 8), (uint64_t)fcap + 64) - fcap;
+    PV_TRY(b->d_fspill.ensure((size_t)b->nq * (scap + fcap) * 8));
     const uint32_t ostride = x.out_k ? x.out_k : ef;
     b->out_stride = ostride;
     b->last_route = 3;
@@ -508,6 +709,8 @@ int run_graph_ex(FvGpuIndex *g, FvGpuBatch *b, uint32_t ef, uint64_t max_visited
     A.route = x.route;
     A.efq = x.efq;
     A.out_k = x.out_k;
+    A.spill = b->d_fspill.as<uint64_t>();
+    A.scap = (uint32_t)scap;
     A.dbg = dbg;
     if (shm > 64 * 1024)
         PV_TRY(hipFuncSetAttribute((const void *)k_graph,
@@ -515,7 +718,7 @@ int run_graph_ex(FvGpuIndex *g, FvGpuBatch *b, uint32_t ef, uint64_t max_visited
     g->ev_stream = b->stream;
     ev_record(g, 0);
     if (!x.bitmap_ready) PV_TRY(launch_bitmap(g, b, b->stream));
-    hipLaunchKernelGGL(k_graph, dim3(b->nq), dim3(64), shm, b->stream, A);
+    hipLaunchKernelGGL(k_graph, dim3(b->nq), dim3(kGT), shm, b->stream, A);
     PV_TRY(hipGetLastError());
     ev_record(g, 5);
     memset(&g->last, 0, sizeof(g->last));
@@ -530,7 +733,7 @@ int run_graph_opts(FvGpuIndex *g, FvGpuBatch *b, uint32_t ef, uint64_t max_visit
 
 int run_graph(FvGpuIndex *g, FvGpuBatch *b, uint32_t ef, uint64_t max_visited,
               char *errbuf, size_t errlen) {
-    return run_graph_opts(g, b, ef, max_visited, 8, 512, nullptr, errbuf, errlen);
+    return run_graph_opts(g, b, ef, max_visited, 8, 1024, nullptr, errbuf, errlen);
 }
 
 } // namespace pv
@@ -608,7 +811,7 @@ extern "C" int fv_gpu_graph_batch(FvGpuIndex *g, int nq, const float *queries,
                                   uint8_t *exhausted, char *errbuf, size_t errlen) {
     try {
         return graph_sync(g, nq, queries, qd, nodes, nnodes, children, nchildren, invals,
-                          ninvals, ef, max_visited, 8, 512, out, out_cnt, exhausted,
+                          ninvals, ef, max_visited, 8, 1024, out, out_cnt, exhausted,
                           nullptr, errbuf, errlen);
     } catch (...) {
         snprintf(errbuf, errlen, "gpu: internal error");

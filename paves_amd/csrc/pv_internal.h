@@ -205,6 +205,7 @@ struct FvGpuBatch {
     int last_route = 0;
     hipStream_t stream = nullptr; /* one of the index's batch streams */
     pv::DevBuf d_visit;   /* graph: overflow visited bitmaps */
+    pv::DevBuf d_fspill;  /* graph: frontier spill stores */
     pv::DevBuf d_progcnt; /* [nprog][bitmap blocks] passing rows per program and
                              bitmap block (learned features) */
     pv::DevBuf d_feat, d_y; /* learned run: features [nq][F], model outputs [nq][2] */

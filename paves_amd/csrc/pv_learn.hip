@@ -785,7 +785,7 @@ extern "C" int fv_gpu_batch_run_learned(FvGpuIndex *g, FvGpuBatch *b, FvGpuModel
             const uint32_t ef_launch = o->mode == 2 ? ef_fixed : ef_max;
             const uint64_t mv = o->max_visited ? o->max_visited
                                                : std::max<uint64_t>(200000, (uint64_t)ef_launch * 400);
-            if (run_graph_ex(g, b, ef_launch, mv, 8, 512, nullptr, x, errbuf, errlen) != 0)
+            if (run_graph_ex(g, b, ef_launch, mv, 8, 1024, nullptr, x, errbuf, errlen) != 0)
                 return -1;
         }
         b->out_stride = o->k;

@@ -392,7 +392,7 @@ class GpuIndex:
         return out, cnt, trunc
 
     def graph(self, queries: np.ndarray, b: Batch, ef: int, max_visited: int,
-              wmax: int | None = None, fcap: int = 512, debug: bool = False):
+              wmax: int | None = None, fcap: int = 1024, debug: bool = False):
         q = np.ascontiguousarray(queries, dtype=np.float32)
         out = np.zeros((b.nq, ef), dtype=HIT_DT)
         cnt = np.zeros(b.nq, dtype=np.uint32)

@@ -1,0 +1,111 @@
+"""Filtered HNSW beam search at the layout of the 1M x 128 graph workload:
+M = 16 (layer-0 degree 32), efConstruction = 200, on 100k x 128 rows of the
+same mixture, six filter classes through the predicate bitmap, ef in
+{40, 80, 160, 400}, batches of 64 and 1024.
+
+* One frontier head per iteration against the oracle's restatement of the
+  reference HnswCursor (engine.cpp:1582-1731): ids, FP32 distances and the
+  exhausted flag are identical for every query. The LDS frontier holds
+  1024 entries (the reference GPU kernel's FV_FRONTIER_CAP,
+  gpu_engine.cu:1069); entries that can never be popped are discarded and
+  live ones beyond it go to a per-query spill store, so no entry of the
+  unbounded CPU heap that could be popped is lost (debug word 3 = 0).
+* The default search (up to 8 frontier heads per iteration): recall@10
+  against the oracle's exact brute force >= 0.9 for every filter class at
+  every ef, plus the output contract (results pass the predicate, unique,
+  ascending by (dist, node), exact FP32 distances).
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from paves_amd import _lib as L
+from paves_amd import engine as E
+from paves_amd import hnsw, synth
+
+pytestmark = pytest.mark.gpu
+
+N = 100_000
+NQ = 1024
+EFS = (40, 80, 160, 400)
+
+
+@pytest.fixture(scope="module")
+def world():
+    d = synth.gmm128(N, NQ, seed=303)
+    cols = synth.columns(d)
+    rows, vals = synth.postings(cols)
+    post = E.Postings(rows, vals)
+    f = hnsw.build(d["base"], M=16, ef_construction=200, threads=16)
+    assert f["l0"].shape[1] == 33
+    ent = hnsw.entries(f, d["base"], d["queries"])
+    cls = [synth.FILTER_CLASSES[i % 6] for i in range(NQ)]
+    preds = [synth.filter_pred(c, d["qcent"][i]) for i, c in enumerate(cls)]
+    b = E.concat([E.serialize(p, N, post) for p in preds], N, entries=ent)
+    th, tc, _, _ = O.brute_batch(d["base"], cols, d["queries"], b.qd, b.nodes, b.children,
+                                 b.invals, 10)
+    g = E.GpuIndex(d["base"], cols, rows, l0=f["l0"], precision=L.FV_GPU_PREC_BF16, device=0)
+    yield {"d": d, "cols": cols, "post": post, "f": f, "ent": ent, "cls": cls,
+           "preds": preds, "b": b, "th": th, "tc": tc, "g": g}
+    g.close()
+
+
+def _sub(w, nq):
+    return E.concat([E.serialize(p, N, w["post"]) for p in w["preds"][:nq]], N,
+                    entries=w["ent"][:nq])
+
+
+def _contract(w, out, cnt, q, ef):
+    c = int(cnt[q])
+    assert c <= ef
+    ids = out[q, :c]["node"].astype(np.int64)
+    assert len(set(ids.tolist())) == c, q
+    p = w["preds"][q]
+    if p is not None:
+        assert E.eval_pred(p, w["cols"])[ids].all(), q
+    dist = out[q, :c]["dist"]
+    keys = (dist.view(np.uint32).astype(np.uint64) << np.uint64(32)) | ids.astype(np.uint64)
+    assert np.all(keys[1:] > keys[:-1]), q
+    for k in range(0, c, max(1, c // 8)):
+        assert dist[k] == O.l2sq(w["d"]["queries"][q], w["d"]["base"][ids[k]]), (q, k)
+
+
+@pytest.mark.parametrize("ef", EFS)
+@pytest.mark.parametrize("nq", [64, 1024])
+def test_one_head_equals_cursor(world, ef, nq):
+    w = world
+    d = w["d"]
+    b = _sub(w, nq)
+    mv = max(200000, 400 * ef)
+    print(f"case config3 w1 ef={ef} nq={nq}", flush=True)
+    out, cnt, exh, dbg = w["g"].graph(d["queries"][:nq], b, ef, mv, wmax=1, fcap=1024,
+                                      debug=True)
+    oh, oc, oe = O.hnsw_batch(d["base"], w["f"]["l0"], w["cols"], d["queries"][:nq], b.qd,
+                              b.nodes, b.children, b.invals, w["ent"][:nq], ef, mv)
+    assert not dbg[:, 3].any(), "frontier entries lost"
+    for q in range(nq):
+        c = int(cnt[q])
+        assert c == int(oc[q]), (q, w["cls"][q], c, int(oc[q]))
+        assert np.array_equal(out[q, :c]["node"], oh[q, :c]["node"]), (q, w["cls"][q])
+        assert np.array_equal(out[q, :c]["dist"].view(np.uint32),
+                              oh[q, :c]["dist"].view(np.uint32)), (q, w["cls"][q])
+        assert int(exh[q]) == int(oe[q]), (q, w["cls"][q])
+
+
+@pytest.mark.parametrize("ef", EFS)
+@pytest.mark.parametrize("nq", [64, 1024])
+def test_default_recall_per_class(world, ef, nq):
+    w = world
+    d = w["d"]
+    b = _sub(w, nq)
+    print(f"case config3 default ef={ef} nq={nq}", flush=True)
+    out, cnt, exh = w["g"].graph(d["queries"][:nq], b, ef, max(200000, 400 * ef))
+    rec = {c: [] for c in synth.FILTER_CLASSES}
+    for q in range(nq):
+        _contract(w, out, cnt, q, ef) if q % 16 == 0 else None
+        got = out[q, : min(10, int(cnt[q]))]["node"]
+        rec[w["cls"][q]].append(O.recall(got, w["th"][q, : w["tc"][q]]["node"]))
+    means = {c: float(np.mean(v)) for c, v in rec.items()}
+    print("recall", ef, nq, means, flush=True)
+    for c, m in means.items():
+        assert m >= 0.9, (c, ef, nq, means)

@@ -121,3 +121,61 @@ def test_graph_replay_equals_eager(inline, qtw):
             g.free_batch(bh)
     finally:
         g.close()
+
+
+def _uneven768(n, nq, seed):
+    d = synth.gmm768(n, nq, seed=seed)
+    cols = synth.columns(d)
+    rng = np.random.default_rng(seed)
+    kq = rng.choice(np.array([1, 10, 100], np.uint32), nq)
+    sel = 10.0 ** rng.uniform(-4, 0, nq)
+    preds = [E.cmp(0, E.CMP_LT, float(1000.0 * s)) for s in sel]
+    b = E.concat([E.serialize(p, n) for p in preds], n)
+    return d, cols, kq, b
+
+
+@pytest.mark.parametrize("dim", [128, 768])
+def test_autotuned_uneven_batch_and_replay(dim):
+    """The uneven workload at both vector shapes (4096 queries at 128 dims,
+    1024 at 768; k in {1, 10, 100}, selectivity 10^U(-4, 0) inline): the
+    autotuner's chosen launch shape certifies every query, the eager run
+    with that shape equals the oracle on a sample of queries (distances
+    within 1e-5 relative of FP64 at 768 dims, where sums are not exact),
+    and hipGraph replays of the tuned shape equal the eager run."""
+    if dim == 128:
+        n, nq = 100_000, 4096
+        d, cols, rows, kq, b = _uneven(n, nq, 53)
+    else:
+        n, nq = 50_000, 1024
+        d, cols, kq, b = _uneven768(n, nq, 57)
+    g = E.GpuIndex(d["base"], cols, None, precision=L.FV_GPU_PREC_BF16)
+    try:
+        bh = g.stage_ex(d["queries"], b, kq)
+        try:
+            print(f"case tune dim={dim}", flush=True)
+            best, us, ntried = g.tune(bh, L.BruteOpts(route=1, inline_pred=1), reps=2)
+            assert ntried > 1 and np.isfinite(us) and us > 0
+            assert best.inline_pred == 1
+            g.run_brute(bh, best)
+            g.sync()
+            ref = g.fetch(bh, nq, int(kq.max()))
+            assert ref[3] == 0
+            sel = np.random.default_rng(3).choice(nq, 128, replace=False)
+            _check(ref[0], ref[1], ref[2], d, cols, b, kq, sel)
+            if dim == 768:
+                for q in sel[:16]:
+                    for k in range(int(ref[1][q])):
+                        f64 = O.l2sq_f64(d["queries"][q], d["base"][int(ref[0][q, k]["node"])])
+                        assert abs(float(ref[0][q, k]["dist"]) - f64) <= 1e-5 * max(f64, 1e-30)
+            opts = L.BruteOpts(**{f: getattr(best, f) for f, _ in L.BruteOpts._fields_})
+            opts.graph = 1
+            for it in range(4):
+                g.run_brute(bh, opts)
+                g.sync()
+                out, cnt, flag, rc = g.fetch(bh, nq, int(kq.max()))
+                assert rc == 0, it
+                assert np.array_equal(cnt, ref[1]) and np.array_equal(out, ref[0]), it
+        finally:
+            g.free_batch(bh)
+    finally:
+        g.close()
