@@ -2022,6 +2022,14 @@ static hipError_t launch_main_scan(const ScanLaunch &L, hipStream_t s) {
     return hipGetLastError();
 }
 
+/* the options a NULL pointer stands for: index precision, automatic route
+ * and shape (a zero-filled struct would ask for the FP32 scan) */
+static inline FvGpuBruteOpts default_opts() {
+    FvGpuBruteOpts o = {};
+    o.precision = -1;
+    return o;
+}
+
 /* Enqueue the brute pipeline. stride_override > 0 forces the pre-pass
  * sample stride (1 = every row: the tightest threshold). */
 int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
@@ -2358,7 +2366,7 @@ static int brute_sync(FvGpuIndex *g, int nq, const float *queries,
     if (bad) {
         /* pool overflow or oversized re-rank set: retry with the tightest
          * threshold (pre-pass over every row) */
-        FvGpuBruteOpts o2 = opts ? *opts : FvGpuBruteOpts{};
+        FvGpuBruteOpts o2 = opts ? *opts : default_opts();
         if (o2.route == 0) o2.route = 1;
         g->last.certified_fail = 0;
         for (int q = 0; q < nq; q++) g->last.certified_fail += hs[q] != 0;
@@ -2582,7 +2590,7 @@ extern "C" int fv_gpu_batch_run_brute(FvGpuIndex *g, FvGpuBatch *b,
 static int tune_batch(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *base, int reps,
                       FvGpuBruteOpts *best, double *best_us, int *ntried, char *errbuf,
                       size_t errlen) {
-    FvGpuBruteOpts o0 = base ? *base : FvGpuBruteOpts{};
+    FvGpuBruteOpts o0 = base ? *base : default_opts();
     o0.graph = 0;
     o0.qtw = 0;
     o0.wg_per_cu = 0;

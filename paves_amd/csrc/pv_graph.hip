@@ -104,19 +104,30 @@ __device__ __forceinline__ int hash_insert(uint32_t *h, uint32_t node) {
     return 2;
 }
 
-/* number of entries of sorted a[0..n) strictly less than key */
-__device__ __forceinline__ int lower_rank(const uint64_t *a, int n, uint64_t key) {
-    int lo = 0, hi = n;
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (a[mid] < key) lo = mid + 1; else hi = mid;
-    }
-    return lo;
+/* number of keys below `key` in a sorted run of 64 padded with ~0 (no
+ * key equals ~0), by binary lifting: six fixed steps, no data-dependent
+ * loop, so the many independent searches of a merge interleave their LDS
+ * reads */
+__device__ __forceinline__ int run_rank(const uint64_t *run, uint64_t key) {
+    int pos = 0;
+#pragma unroll
+    for (int step = 32; step > 0; step >>= 1)
+        if (run[pos + step - 1] < key) pos += step;
+    /* the steps reach at most 63: a 64th key below `key` is counted here
+     * (for pos < 63, run[pos] >= key already) */
+    return pos + (run[pos] < key ? 1 : 0);
 }
 
-/* number of entries of sorted a[0..n) at or below key (key < ~0) */
-__device__ __forceinline__ int upper_rank(const uint64_t *a, int n, uint64_t key) {
-    return lower_rank(a, n, key + 1);
+/* number of entries of sorted a[0..n) below key, n <= 1024: binary
+ * lifting in ten fixed steps */
+__device__ __forceinline__ int lift_rank(const uint64_t *a, int n, uint64_t key) {
+    int pos = 0;
+#pragma unroll
+    for (int step = 512; step > 0; step >>= 1)
+        if (pos + step <= n && a[pos + step - 1] < key) pos += step;
+    /* the steps reach at most 1023: with n = 1024 the last entry is
+     * counted here (otherwise a[pos] >= key or pos == n) */
+    return pos + ((pos < n && a[pos] < key) ? 1 : 0);
 }
 
 /* per-workgroup scalars shared through LDS */
@@ -149,12 +160,12 @@ __device__ int block_merge(uint64_t *A, int n, int skip, const uint64_t *runs, c
     const int nold = n - skip;
     /* live entries of each sequence (uniform) */
     const bool all_live = lim == ~0ull;
-    const int lold = all_live ? nold : upper_rank(A + skip, nold, lim);
+    const int lold = all_live ? nold : lift_rank(A + skip, nold, lim + 1);
     int live = lold;
     int lrc[kGW];
 #pragma unroll
     for (int r = 0; r < kGW; r++) {
-        lrc[r] = (all_live || rc[r] == 0) ? rc[r] : upper_rank(runs + 64 * r, rc[r], lim);
+        lrc[r] = (all_live || rc[r] == 0) ? rc[r] : run_rank(runs + 64 * r, lim + 1);
         live += lrc[r];
     }
     constexpr int kOld = 1024 / kGT;   /* A holds at most 1024 entries */
@@ -169,7 +180,7 @@ __device__ int block_merge(uint64_t *A, int n, int skip, const uint64_t *runs, c
             int p = i;
 #pragma unroll
             for (int r = 0; r < kGW; r++)
-                if (lrc[r] > 0) p += lower_rank(runs + 64 * r, lrc[r], ov[k]);
+                if (lrc[r] > 0) p += run_rank(runs + 64 * r, ov[k]);
             op[k] = p;
         }
     }
@@ -177,10 +188,10 @@ __device__ int block_merge(uint64_t *A, int n, int skip, const uint64_t *runs, c
     int np = -1;
     if (l < lrc[w]) {
         nv = runs[64 * w + l];
-        int p = l + lower_rank(A + skip, lold, nv);
+        int p = l + lift_rank(A + skip, lold, nv);
 #pragma unroll
         for (int r = 0; r < kGW; r++)
-            if (r != w && lrc[r] > 0) p += lower_rank(runs + 64 * r, lrc[r], nv);
+            if (r != w && lrc[r] > 0) p += run_rank(runs + 64 * r, nv);
         np = p;
     }
     __syncthreads();
@@ -521,14 +532,24 @@ void k_graph(GraphArgs A) {
         iters++;
         /* pop up to W heads that pass the stop test and precede every
          * spilled entry (uniform) */
-        int nh = 1;
-        while (nh < A.wmax && nh < nF && F[nh] < omin &&
-               (!full || key_dist(F[nh]) <= key_dist(worst)))
-            nh++;
+        int nh;
+        {
+            /* lane j tests head j; the heads are the leading run of passes */
+            bool ok = l == 0;
+            if (l > 0 && l < A.wmax && l < nF) {
+                const uint64_t fj = F[l];
+                ok = fj < omin && (!full || key_dist(fj) <= key_dist(worst));
+            }
+            const uint64_t okb = __ballot(ok);
+            nh = __ffsll((unsigned long long)~okb) - 1;   /* okb bit 0 is set */
+        }
 
         /* neighbour slot t of the popped heads, then its visited test-and-set */
         uint32_t cand = kNoNode;
-        if (t < nh * nbs) cand = A.nbr[(uint64_t)key_row(F[t / nbs]) * nbs + (t % nbs)];
+        if (t < nh * nbs) {
+            const uint32_t row = key_row(F[t / nbs]);
+            if (row < A.count) cand = A.nbr[(uint64_t)row * nbs + (t % nbs)];
+        }
         int isnew = 0;
         if (cand != kNoNode) isnew = hash_insert(H, cand);
         if (__syncthreads_or(isnew == 2)) {
