@@ -206,6 +206,27 @@ typedef struct FvGpuStats {
 void fv_gpu_set_timing(FvGpuIndex *g, int on);
 void fv_gpu_last_stats(const FvGpuIndex *g, FvGpuStats *st);
 
+/* fv_gpu_brute_batch without any memory allocation. No counterpart in the
+ * reference: for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). Same
+ * arguments, results and errors as fv_gpu_brute_batch (entries past
+ * out_cnt[q] in out are unspecified). Every device buffer of the call is
+ * carved from one static 64 MiB scratch array per GPU (held until the
+ * process ends) and host staging from a static 4 MiB array; a batch whose
+ * buffers exceed them (e.g. 1024 queries at topb 4096, or more than about
+ * 64 queries over 1M rows) fails with -1 and an errbuf naming the limit.
+ * Calls are serialised by a process-wide lock (concurrent calls would
+ * share the scratch). */
+int fv_gpu_brute_batch_noalloc(FvGpuIndex *g, int nq, const float *queries,
+                               const FvGpuQueryDesc *qd, const uint8_t *dedup,
+                               const FvGpuPredNode *nodes, int nnodes,
+                               const int32_t *children, int nchildren,
+                               const double *invals, int ninvals,
+                               const FvGpuSpan *spans, int nspans_total,
+                               const uint64_t *spanoff, int nspanoff, uint32_t topb,
+                               FvHit *out, uint32_t *out_cnt, uint8_t *truncated,
+                               char *errbuf, size_t errlen);
+
 /* fv_gpu_brute_batch with options (same arguments otherwise). */
 int fv_gpu_brute_batch_ex(FvGpuIndex *g, int nq, const float *queries,
                           const FvGpuQueryDesc *qd, const uint8_t *dedup,

@@ -49,7 +49,7 @@ EXT_SYMBOLS = ["fv_gpu_set_timing", "fv_gpu_last_stats", "fv_gpu_brute_batch_ex"
                "fv_gpu_model_set_params", "fv_gpu_model_get_params", "fv_gpu_model_grad",
                "fv_gpu_model_train", "fv_gpu_model_predict", "fv_gpu_batch_features",
                "fv_gpu_batch_run_learned", "fv_gpu_batch_routes", "fv_gpu_batch_stage_ex",
-               "fv_gpu_batch_tune", "fv_gpu_batch_time_scan"]
+               "fv_gpu_batch_tune", "fv_gpu_batch_time_scan", "fv_gpu_brute_batch_noalloc"]
 
 
 class BruteOpts(C.Structure):
@@ -118,6 +118,8 @@ def load() -> C.CDLL:
     lib.fv_gpu_brute_batch_ex.argtypes = [_P, C.c_int, _P, _P, _P, _P, C.c_int, _P, C.c_int,
                                           _P, C.c_int, _P, C.c_int, _P, C.c_int, C.c_uint32,
                                           C.POINTER(BruteOpts), _P, _P, _P, C.c_char_p, sz]
+    lib.fv_gpu_brute_batch_noalloc.restype = C.c_int
+    lib.fv_gpu_brute_batch_noalloc.argtypes = lib.fv_gpu_brute_batch.argtypes
     lib.fv_gpu_graph_batch.restype = C.c_int
     lib.fv_gpu_graph_batch.argtypes = [_P, C.c_int, _P, _P, _P, C.c_int, _P, C.c_int, _P,
                                        C.c_int, C.c_uint32, C.c_uint64, _P, _P, _P,

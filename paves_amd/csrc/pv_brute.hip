@@ -2247,8 +2247,10 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
         L.gx = gx;
         L.shm = i8 ? shm_i8 : shm_bf;
         PV_TRY(launch_main_scan(L, s));
-        if (b->last_scan == nullptr) b->last_scan = new ScanLaunch();
-        *(ScanLaunch *)b->last_scan = L;
+        if (!b->arena_bound) {
+            if (b->last_scan == nullptr) b->last_scan = new ScanLaunch();
+            *(ScanLaunch *)b->last_scan = L;
+        }
     }
     ev_record(g, 4);
     hipLaunchKernelGGL(k_select, dim3(nq), dim3(256), dim * 4, s,

@@ -82,12 +82,32 @@ static_assert(sizeof(PvOp) == 24, "PvOp layout");
  */
 constexpr int32_t kCmpPassMask[6] = {0x01, 0x03, 0x02, 0x06, 0x04, 0x15};
 
-/* grow-only device buffer */
+/* A bump arena over caller-provided memory: buffers bound to one take
+ * their space from it instead of allocating (the allocation-free entry
+ * point binds a batch's buffers to the library's static device scratch). */
+struct Arena {
+    unsigned char *base = nullptr;
+    size_t size = 0, used = 0;
+    void *take(size_t n) {
+        const size_t at = (used + 255) & ~(size_t)255;
+        if (base == nullptr || at + n > size) return nullptr;
+        used = at + n;
+        return base + at;
+    }
+};
+
+/* grow-only device buffer (from the arena when bound to one) */
 struct DevBuf {
     void *p = nullptr;
     size_t cap = 0;
+    Arena *arena = nullptr;
     hipError_t ensure(size_t need) {
         if (cap >= need && p) return hipSuccess;
+        if (arena) {
+            p = arena->take(need);
+            cap = p ? need : 0;
+            return p ? hipSuccess : hipErrorOutOfMemory;
+        }
         if (p) hipFree(p);
         p = nullptr;
         cap = 0;
@@ -97,19 +117,25 @@ struct DevBuf {
         return e;
     }
     void release() {
-        if (p) hipFree(p);
+        if (p && !arena) hipFree(p);
         p = nullptr;
         cap = 0;
     }
     template <typename T> T *as() const { return (T *)p; }
 };
 
-/* grow-only pinned host buffer */
+/* grow-only pinned host buffer (from a host arena when bound to one) */
 struct PinBuf {
     void *p = nullptr;
     size_t cap = 0;
+    Arena *arena = nullptr;
     hipError_t ensure(size_t need) {
         if (cap >= need && p) return hipSuccess;
+        if (arena) {
+            p = arena->take(need);
+            cap = p ? need : 0;
+            return p ? hipSuccess : hipErrorOutOfMemory;
+        }
         if (p) hipHostFree(p);
         p = nullptr;
         cap = 0;
@@ -119,7 +145,7 @@ struct PinBuf {
         return e;
     }
     void release() {
-        if (p) hipHostFree(p);
+        if (p && !arena) hipHostFree(p);
         p = nullptr;
         cap = 0;
     }
@@ -165,6 +191,7 @@ struct FvGpuBatch {
     FvGpuBruteOpts gopts = {};
     std::vector<uint64_t> gsig;
     bool warmed = false;    /* one eager run done: every buffer allocated */
+    bool arena_bound = false; /* buffers carved from an arena: keep no host records */
     void *last_scan = nullptr; /* launch record of the last main scan (timing) */
     bool capturing = false; /* a hipGraph capture of this batch's run is open */
     bool i8_ok = false;     /* every query is an integer within the INT8 window */
