@@ -70,9 +70,12 @@ typedef struct FvHit {
 #ifndef PAVES_GPU_ENGINE_H
 typedef struct FvGpuIndex FvGpuIndex;
 
-/* gpu_engine.h:33-34. FP32: exact, bit-identical ids and distances to the
- * CPU cursors. SQ8 (the reference's reduced-precision request) is served
- * here by the BF16 path below, which keeps the exact contract. */
+/* gpu_engine.h:33-34. Every precision returns exact results: ids and
+ * FP32 distances bit-identical to the CPU cursors. An FP32 index is
+ * scanned in BF16 (FV_GPU_PREC_BF16) unless a call asks for FP32; SQ8 (the
+ * reference's reduced-precision request) scans an exact INT8 copy where
+ * the data are integers spanning at most 255 values at dim 128 or 256,
+ * else BF16; a certified FP32 re-rank makes every scan exact. */
 #define FV_GPU_PREC_FP32 0
 #define FV_GPU_PREC_SQ8 1
 
@@ -96,6 +99,12 @@ typedef struct FvGpuQueryDesc {
 /* BF16 MFMA distance scan + exact FP32 re-rank of an over-fetched,
  * error-bounded candidate set; ids and distances equal the FP32 path. */
 #define FV_GPU_PREC_BF16 2
+/* Scan precision of FvGpuBruteOpts only (not an index precision): the FP32
+ * scan on the vector ALUs, sum (x - q)^2 (the reference tiled kernel's
+ * arithmetic, gpu_engine.cu:534-566). FV_GPU_PREC_FP32 asked for in the
+ * options scans x.q on FP32 MFMA (dim % 8 == 0, dim <= 512; other dims take
+ * the VALU scan). Every scan returns the same exact results. */
+#define FV_GPU_PREC_FP32_VALU 3
 
 #ifdef __cplusplus
 #define PV_STATIC_ASSERT static_assert

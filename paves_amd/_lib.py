@@ -28,6 +28,7 @@ assert QD_DT.itemsize == 32 and HIT_DT.itemsize == 16
 FV_GPU_PREC_FP32 = 0
 FV_GPU_PREC_SQ8 = 1
 FV_GPU_PREC_BF16 = 2
+FV_GPU_PREC_FP32_VALU = 3
 FV_GPU_MAX_TOPB = 4096
 FV_GPU_MAX_SPANS = 4096
 FV_GPU_MAX_EF = 512

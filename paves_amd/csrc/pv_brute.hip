@@ -59,7 +59,8 @@ namespace pv {
 /* ---------------- 1. query preparation ---------------- */
 
 /*
- * One wave per (padded) query. mode: 0 = FP32 scan bound, 1 = BF16 bound,
+ * One wave per (padded) query. mode: 0 = FP32 VALU scan bound, 1 = BF16
+ * bound, 3 = FP32 MFMA bound,
  * 2 = INT8 scan: also writes q - centre as int8 and ||q - centre||^2; the
  * INT8 scan is exact (eps = 0) and the host takes it only when every query
  * value is an integer within [centre - 128, centre + 127] (a value outside
@@ -127,6 +128,10 @@ __device__ __forceinline__ void prep_query(
             if (mode == 1) {
                 const double u = ldexp(1.0, -8);
                 e = (4.0 * u + 2.0 * u * u) * xn * qn + 4.0 * g * big;
+            } else if (mode == 3) {
+                /* FP32 MFMA x.q with the norm seed: unrounded inputs, the
+                 * BF16 form's accumulation allowance */
+                e = 4.0 * g * big;
             } else {
                 e = 2.0 * g * big;
             }
@@ -304,7 +309,7 @@ __device__ __forceinline__ bool scan_pad(const ScanArgs &A) {
  * kernel alone can be re-timed (fv_gpu_batch_time_scan) */
 struct ScanLaunch {
     ScanArgs M;
-    bool bf, inl, i8;
+    bool bf, inl, i8, f32m;
     int pt;                 /* INT8: 32-row tiles per 8 KB unit */
     int qtw, nw, nqt, gy;
     uint32_t gx;
@@ -1150,6 +1155,115 @@ void k_scan_i8(ScanArgs A) {
     if (!PREPASS) flush_candidates(A, qbase, C);
 }
 
+/* ---------------- 3/5. FP32 MFMA scan ---------------- */
+
+/*
+ * The plain FP32 path: x.q in FP32 on v_mfma_f32_32x32x2_f32 (FP32 inputs
+ * unrounded, FP32 accumulation) with the seeded epilogue of the BF16 scan
+ * (acc = x.q - ||x||^2 / 2, filtered rows seeded -inf), and the same
+ * certified select: the scan value differs from the reference's FP32 sum
+ * only by accumulation rounding (eps of query-prep mode 3), and the exact
+ * re-rank restores the reference's distances.
+ *
+ * Reads the row-major FP32 copy directly. A unit is 32 rows x 64 dims
+ * (8 KB): lane l loads eight 16-byte pieces, piece j = dims 8 j + 4 (l >> 5)
+ * .. +4 of row (l & 31); element e of piece j is the A operand of the
+ * e-th of four MFMAs over that 8-dim chunk (lanes 0-31 give k = 0 = dim
+ * 8 j + e, lanes 32-63 give k = 1 = dim 8 j + 4 + e), so four MFMAs sum
+ * the chunk's eight products per (row, query). The queries sit in LDS in
+ * the same lane layout. dim % 8 == 0; the last unit of a tile may cover
+ * fewer than 8 chunks (its missing pieces are not loaded).
+ * grid.x: row chunks, grid.y: query tiles; NW waves per WG.
+ */
+template <bool PREPASS, int NW>
+__global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(2)))
+void k_scan_f32m(ScanArgs A) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    const int D = A.dim;
+    const int NC = D / 8;                       /* 8-dim chunks per row */
+    const uint32_t NCH = (uint32_t)((NC + 7) / 8); /* units per tile */
+    u32x4 *qsh = (u32x4 *)smem;                 /* [2][NC][64] x 16 B */
+    CandLds &C = *(CandLds *)(smem + (size_t)2 * NC * 64 * 16);
+
+    const int tid = threadIdx.x, l = tid & 63, wave = tid >> 6;
+    const int qbase = blockIdx.y * kQT;
+    if (tile_skipped(A, qbase)) return;
+
+    __shared__ uint32_t s_next;
+    UnitCursor cur = cursor_init<NW>(A, wave, &s_next);
+    const uint32_t tlast = cursor_last(cur, A.ntiles);
+    auto phys = [&](uint32_t t) { return min(t, tlast) * A.stride; };
+    const int half = l >> 5;
+    auto load_rows = [&](u32x4 (&a)[8]) {
+        const uint64_t row = (uint64_t)phys(cur.tile) * 32 + (l & 31);
+        const float *rp = A.vecs + row * D + 64 * cur.c + 4 * half;
+        const int nc = min(8, NC - 8 * (int)cur.c);
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+            if (j < nc) a[j] = *(const u32x4 *)(rp + 8 * j);
+    };
+    u32x4 a0[8], a1[8];
+    load_rows(a0);
+
+    for (int e = tid; e < 2 * NC * 64; e += blockDim.x) {
+        const int lane = e & 63, c = (e >> 6) % NC, qb = (e >> 6) / NC;
+        const int q = qbase + (lane & 31) + 32 * qb;
+        qsh[e] = *(const u32x4 *)(A.qf + (uint64_t)q * D + 8 * c + 4 * (lane >> 5));
+    }
+    if (!PREPASS) cand_init(C);
+
+    float tq[2], qn[2];
+    const uint32_t *wr[2];
+    load_query_state(A, qbase, tq, qn, wr);
+    auto load_in = [&](TileIn &in) {
+        load_tile_in<false>(A, phys(cur.tile), wr, in);
+        in.t = cur.tile;
+        in.c = cur.c;
+        in.live = cur.tile < cur.end;
+        cursor_next(cur, NCH, &s_next);
+    };
+    TileIn in0, in1;
+    __syncthreads();
+    cursor_start(cur, &s_next);
+    load_in(in0);
+    auto do_unit = [&](const u32x4 (&a)[8], const TileIn &in, f32x16 &acc0, f32x16 &acc1) {
+        const uint32_t c = in.c;
+        if (c == 0) {
+            float sn[16];
+            expand_norms(-0.5f * in.xr, sn);
+            seed_bf(sn, in.w[0] & in.valid, in.w[1] & in.valid, acc0, acc1);
+        }
+        const int nc = min(8, NC - 8 * (int)c);
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            if (j >= nc) break;
+            const int ch = (int)c * 8 + j;
+            const u32x4 b0 = qsh[(0 * NC + ch) * 64 + l];
+            const u32x4 b1 = qsh[(1 * NC + ch) * 64 + l];
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const float av = __uint_as_float(a[j][e]);
+                acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(av, __uint_as_float(b0[e]), acc0, 0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(av, __uint_as_float(b1[e]), acc1, 0, 0, 0);
+            }
+        }
+        if (c == NCH - 1)
+            epilogue_bf<PREPASS>(A, qbase, in.t, in.t * A.stride, acc0, acc1, qn, tq, C);
+    };
+
+    f32x16 acc0 = {}, acc1 = {};
+    while (in0.live) {
+        load_rows(a1);
+        load_in(in1);
+        do_unit(a0, in0, acc0, acc1);
+        if (!in1.live) break;
+        load_rows(a0);
+        load_in(in0);
+        do_unit(a1, in1, acc0, acc1);
+    }
+    if (!PREPASS) flush_candidates(A, qbase, C);
+}
+
 /* ---------------- 3/5. plain FP32 VALU scan ---------------- */
 
 /*
@@ -1799,11 +1913,18 @@ int stage_batch(FvGpuIndex *g, FvGpuBatch *b, int nq, const float *queries,
  * the INT8 copy and every query of the batch fits its window), else the
  * BF16 scan, else plain FP32 */
 int scan_precision(const FvGpuIndex *g, const FvGpuBatch *b, const FvGpuBruteOpts *o) {
-    int p = (o && o->precision >= 0) ? o->precision : g->precision;
+    const bool asked = o && o->precision >= 0;
+    int p = asked ? o->precision : g->precision;
+    /* an FP32 index is served by the BF16 scan unless FP32 is asked for:
+     * the certified re-rank returns the same ids and distances */
+    if (!asked && p == FV_GPU_PREC_FP32) p = FV_GPU_PREC_BF16;
     if (p == FV_GPU_PREC_SQ8 && !(g->d_vi8 != nullptr && b != nullptr && b->i8_ok))
         p = FV_GPU_PREC_BF16;
     if (p == FV_GPU_PREC_BF16 && (g->d_vbf == nullptr || g->dim % 128 != 0 || g->dim > 1024))
         p = FV_GPU_PREC_FP32;
+    if (p == FV_GPU_PREC_FP32 && (g->dim % 8 != 0 || g->dim > 512)) p = FV_GPU_PREC_FP32_VALU;
+    if (p != FV_GPU_PREC_FP32 && p != FV_GPU_PREC_SQ8 && p != FV_GPU_PREC_BF16)
+        p = FV_GPU_PREC_FP32_VALU;
     return p;
 }
 
@@ -1888,7 +2009,9 @@ hipError_t launch_phase1(FvGpuIndex *g, FvGpuBatch *b, int prec, bool bitmap, hi
         P.vmin = g->vmin;
         P.vmax = g->vmax;
         P.all_int = g->all_int ? 1 : 0;
-        P.mode = prec == FV_GPU_PREC_SQ8 ? 2 : (prec == FV_GPU_PREC_BF16 ? 1 : 0);
+        P.mode = prec == FV_GPU_PREC_SQ8 ? 2
+                 : prec == FV_GPU_PREC_BF16 ? 1
+                 : prec == FV_GPU_PREC_FP32 ? 3 : 0;
         P.qi8 = b->d_qi8.as<int8_t>();
         P.qn8 = b->d_qn8.as<float>();
         P.centre = g->i8_center;
@@ -1939,7 +2062,8 @@ static ScanShape scan_shape(const FvGpuIndex *g, const FvGpuBatch *b, const FvGp
     const uint32_t B = b->topb;
     /* scan tiles: 32 rows, or for the INT8 scan its 8 KB units */
     const uint32_t ntiles = (uint32_t)(g->count_pad / 32) / scan_unit_tiles(prec, dim);
-    const bool bf = prec == FV_GPU_PREC_BF16 || prec == FV_GPU_PREC_SQ8;
+    const bool bf = prec == FV_GPU_PREC_BF16 || prec == FV_GPU_PREC_SQ8 ||
+                    prec == FV_GPU_PREC_FP32;
     /* query tiles per WG: several when many query tiles re-read the same
      * data (dim 128, BF16) */
     int qtw = (o && o->qtw > 0) ? o->qtw : (b->nqt >= 8 ? 4 : 1);
@@ -2009,7 +2133,9 @@ static void launch_scan_i8(int nw, int pt, bool inl, uint32_t gx, int nqt, size_
 }
 
 static hipError_t launch_main_scan(const ScanLaunch &L, hipStream_t s) {
-    if (L.i8) {
+    if (L.f32m) {
+        hipLaunchKernelGGL((k_scan_f32m<false, 8>), dim3(L.gx, L.nqt), dim3(512), L.shm, s, L.M);
+    } else if (L.i8) {
         launch_scan_i8<false>(L.nw, L.pt, L.inl, L.gx, L.nqt, L.shm, s, L.M);
     } else if (L.bf && L.qtw > 1) {
         const hipError_t e = launch_scan_mq<false>(L.qtw, L.inl, L.gx, L.gy, s, L.M);
@@ -2125,10 +2251,11 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
     const int pt = (int)scan_unit_tiles(prec, dim);
     const uint32_t ntiles = (uint32_t)(g->count_pad / 32) / (uint32_t)pt;
     const bool bf = prec == FV_GPU_PREC_BF16;
+    const bool f32m = prec == FV_GPU_PREC_FP32;   /* FP32 MFMA scan (8-wave WGs) */
     const bool auto_shape = !(o && (o->waves || o->wg_per_cu));
     const ScanShape sh = scan_shape(g, b, o, prec, stride_override,
                                     (bf || i8) && auto_shape && other_batches_in_flight(g, b));
-    const int qtw = sh.qtw, gy = sh.gy, nw = sh.nw;
+    const int qtw = sh.qtw, gy = sh.gy, nw = f32m ? 8 : sh.nw;
     const uint32_t gx = sh.gx, stride = sh.stride;
     const uint32_t stiles = (ntiles + stride - 1) / stride;
     b->stride = stride;
@@ -2190,6 +2317,12 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
             PV_TRY(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm_bf));
     }
     if (bf && shm_bf > 160 * 1024) PV_FAIL("gpu: dim %d too large for the BF16 scan", dim);
+    /* FP32 MFMA scan: the 64 queries' FP32 operands (dim x 256 bytes) in LDS */
+    const size_t shm_f32m = (size_t)2 * (dim / 8) * 64 * 16 + sizeof(CandLds);
+    if (f32m && shm_f32m > 160 * 1024) PV_FAIL("gpu: dim %d too large for the FP32 MFMA scan", dim);
+    if (f32m && shm_f32m > 64 * 1024)
+        for (const void *f : {(const void *)k_scan_f32m<true, 8>, (const void *)k_scan_f32m<false, 8>})
+            PV_TRY(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm_f32m));
     /* pre-pass over sampled tiles (with a decision made in this run's
      * threshold launch, the route is not known yet: no skipping) */
     {
@@ -2197,7 +2330,10 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
         if (decide) P.skip = nullptr;
         P.ntiles = stiles;
         P.stride = stride;
-        if (bf || i8) {
+        if (f32m) {
+            const uint32_t gxp = std::min<uint32_t>(gx, std::max<uint32_t>(1, (stiles + nw - 1) / nw));
+            hipLaunchKernelGGL((k_scan_f32m<true, 8>), dim3(gxp, b->nqt), dim3(512), shm_f32m, s, P);
+        } else if (bf || i8) {
             /* each pre-pass WG walks several sampled tiles (at most the
              * main scan's grid): staging 64 query operands per WG costs as
              * much as a few tiles */
@@ -2237,6 +2373,7 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
         L.M.ntiles = ntiles;
         L.M.stride = 1;
         L.bf = bf;
+        L.f32m = f32m;
         L.inl = inl;
         L.i8 = i8;
         L.pt = pt;
@@ -2245,7 +2382,7 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
         L.nqt = b->nqt;
         L.gy = gy;
         L.gx = gx;
-        L.shm = i8 ? shm_i8 : shm_bf;
+        L.shm = i8 ? shm_i8 : (f32m ? shm_f32m : shm_bf);
         PV_TRY(launch_main_scan(L, s));
         if (!b->arena_bound) {
             if (b->last_scan == nullptr) b->last_scan = new ScanLaunch();

@@ -62,7 +62,8 @@ def class_preds(cls, qcent):
     return [synth.filter_pred(cls, c) for c in qcent]
 
 
-@pytest.mark.parametrize("prec", [L.FV_GPU_PREC_FP32, L.FV_GPU_PREC_BF16])
+@pytest.mark.parametrize("prec", [L.FV_GPU_PREC_FP32, L.FV_GPU_PREC_BF16,
+                                  L.FV_GPU_PREC_FP32_VALU])
 @pytest.mark.parametrize("cls", synth.FILTER_CLASSES)
 @pytest.mark.parametrize("topb", [10, 64])
 def test_filter_classes(index100k, data100k, prec, cls, topb):
@@ -159,7 +160,7 @@ def test_edge_cases():
                  E.or_(E.cmp(0, E.CMP_LT, 2.0), E.cmp(0, E.CMP_GT, 998.0)),
                  E.isin(1, []), E.and_()]
         b = E.concat([E.serialize(p, n, post) for p in preds], n)
-        for prec in (L.FV_GPU_PREC_FP32, L.FV_GPU_PREC_BF16):
+        for prec in (L.FV_GPU_PREC_FP32, L.FV_GPU_PREC_BF16, L.FV_GPU_PREC_FP32_VALU):
             for route in (1, 0):
                 out, cnt, trunc = g.brute(d["queries"], b, 20, L.BruteOpts(precision=prec, route=route))
                 oh, oc, ot, _ = O.brute_batch(d["base"], cols, d["queries"], b.qd, b.nodes,
@@ -188,7 +189,7 @@ def test_exact_ties_broken_by_id():
     g = E.GpuIndex(base, None, None, precision=L.FV_GPU_PREC_BF16)
     try:
         b = E.concat([E.serialize(None, n)] * 3, n)
-        for prec in (L.FV_GPU_PREC_FP32, L.FV_GPU_PREC_BF16):
+        for prec in (L.FV_GPU_PREC_FP32, L.FV_GPU_PREC_BF16, L.FV_GPU_PREC_FP32_VALU):
             out, cnt, trunc = g.brute(q, b, 100, L.BruteOpts(precision=prec, route=1))
             oh, oc, ot, _ = O.brute_batch(base, None, q, b.qd, b.nodes, b.children, b.invals, 100)
             assert_same(out, cnt, trunc, oh, oc, ot, f"ties/{prec}")
@@ -222,6 +223,13 @@ def test_mixed_sign_integers_exact_without_rerank(dim, lim):
             if prec == L.FV_GPU_PREC_SQ8:
                 assert g.stats().precision == L.FV_GPU_PREC_SQ8
             assert_same(out, cnt, trunc, oh, oc, ot, f"mixed-sign {dim}/{prec}")
+            if prec == L.FV_GPU_PREC_FP32:
+                # an FP32 index is scanned in BF16 unless FP32 is asked for
+                assert g.stats().precision == L.FV_GPU_PREC_BF16
+                for p2 in (L.FV_GPU_PREC_FP32, L.FV_GPU_PREC_FP32_VALU):
+                    out, cnt, trunc = g.brute(qs, b, 20, L.BruteOpts(route=1, precision=p2))
+                    assert g.stats().precision == p2
+                    assert_same(out, cnt, trunc, oh, oc, ot, f"mixed-sign {dim}/{p2}")
         finally:
             g.close()
 
