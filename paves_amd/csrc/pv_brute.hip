@@ -397,17 +397,6 @@ __device__ __forceinline__ bool tile_skipped(const ScanArgs &A, int qbase) {
 }
 
 constexpr uint32_t kSpill = 512; /* per-WG LDS candidate overflow slots */
-/* pool counters sit one 128-byte line apart: every scan WG reserves pool
- * space with an atomic per query, and counters sharing a line would
- * serialise all of them in one L2 channel */
-constexpr uint32_t kPcntStride = 32;
-/* Each query's pool is split into kPoolShards equal segments, each with its
- * own counter; scan WG x reserves in segment x % kPoolShards. All WGs flush
- * at the end of the scan, and one counter per query took the reservations
- * of every WG of the grid back to back (measured: 5 us of the flush at 256
- * WGs); the select stage reads the segments through a prefix of their
- * counts. */
-constexpr uint32_t kPoolShards = 8;
 __device__ __forceinline__ uint64_t pcnt_at(int q, uint32_t s) {
     return ((uint64_t)q * kPoolShards + s) * kPcntStride;
 }
@@ -1491,6 +1480,11 @@ __global__ __launch_bounds__(256) void k_tau_route(TauArgs A, LearnArgs R, int n
     else route_block(R, blockIdx.x - nq);
 }
 
+hipError_t launch_route_blocks(const LearnArgs &A, int nq, hipStream_t s) {
+    hipLaunchKernelGGL(k_tau_route, dim3((nq + 3) / 4), dim3(256), 0, s, TauArgs{}, A, 0);
+    return hipGetLastError();
+}
+
 /* clears pool counters and status, tau = +inf (gather route / no pre-pass) */
 __global__ void k_reset(int nq, float *tau, uint32_t *pcnt, uint32_t *status) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1727,6 +1721,13 @@ void free_batch_buffers(FvGpuBatch *b) {
     b->d_efq.release();
     b->d_qi8.release();
     b->d_qn8.release();
+    b->d_vmap.release();
+    if (b->view) {
+        if (b->owner && b->owner->last_batch == b->view) b->owner->last_batch = nullptr;
+        free_batch_buffers(b->view);   /* its bitmap is borrowed: not freed */
+        delete b->view;
+        b->view = nullptr;
+    }
     if (b->gexec) hipGraphExecDestroy(b->gexec);
     if (b->graph) hipGraphDestroy(b->graph);
     b->gexec = nullptr;

@@ -49,6 +49,19 @@ constexpr int kMaxProgDepth = 64;
 
 constexpr int kMaxBatchEx = 4096; /* queries per batch through the extension API */
 
+/* pool counters sit one 128-byte line apart: every scan WG reserves pool
+ * space with an atomic per query, and counters sharing a line would
+ * serialise all of them in one L2 channel */
+constexpr uint32_t kPcntStride = 32;
+/* Each query's pool is split into kPoolShards equal segments, each with its
+ * own counter; scan WG x reserves in segment x % kPoolShards. All WGs flush
+ * at the end of the scan, and one counter per query took the reservations
+ * of every WG of the grid back to back (measured: 5 us of the flush at 256
+ * WGs); the select stage reads the segments through a prefix of their
+ * counts. */
+constexpr uint32_t kPoolShards = 8;
+
+
 /* one comparison evaluated inside the scan (col -1: no predicate) */
 struct InlPred {
     int32_t col;
@@ -237,6 +250,12 @@ struct FvGpuBatch {
                              bitmap block (learned features) */
     pv::DevBuf d_feat, d_y; /* learned run: features [nq][F], model outputs [nq][2] */
     pv::DevBuf d_route, d_efq; /* learned run: route u8 [nq] (1 = graph), ef u32 [nq] */
+    /* learned run over several query tiles: the brute-routed queries packed
+     * to the front of this companion batch (sharing the parent's predicate
+     * programs and bitmaps), so the scan runs only the tiles they fill */
+    FvGpuBatch *view = nullptr;
+    pv::DevBuf d_vmap;    /* view slot -> parent query [nqp] (i32) */
+    uint8_t *d_vskip = nullptr; /* a view's skip flags [nqp] (its staging area) */
 };
 
 struct FvGpuIndex {
@@ -342,6 +361,8 @@ struct LearnArgs;
 int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
                  uint32_t stride_override, const uint8_t *skip, bool have_phase1,
                  char *errbuf, size_t errlen, const LearnArgs *decide = nullptr);
+/* the routing decision's blocks alone (features already in A.feat_out) */
+hipError_t launch_route_blocks(const LearnArgs &A, int nq, hipStream_t s);
 /* router features computed by the query prep of phase 1 (dim <= 256) */
 struct PrepFeatures {
     const float *cent_t;

@@ -681,6 +681,162 @@ extern "C" int fv_gpu_batch_features(FvGpuIndex *g, FvGpuBatch *b, FvGpuModel *m
     return 0;
 }
 
+namespace pv {
+
+/*
+ * Compaction of a routed run over several query tiles: the brute-routed
+ * queries (route 0) of the parent batch, in order, become slots 0 .. nb-1
+ * of the view batch; later slots are marked skipped, so every query tile
+ * past the packed queries is skipped whole by the brute pipeline. One
+ * 1024-thread block (nq <= 4096: four consecutive queries per thread, a
+ * block-wide prefix over the per-thread counts), then the packed rows are
+ * copied by all threads.
+ */
+struct CompactArgs {
+    const uint8_t *route;
+    int nq, nqp, dim;
+    const float *q;
+    const int32_t *pofq;
+    const uint32_t *topbq;   /* per-query k (uneven batches) or null */
+    float *vq;
+    int32_t *vpofq;
+    uint32_t *vtopbq;
+    uint8_t *vskip;
+    int32_t *vmap;
+};
+
+__global__ __launch_bounds__(1024) void k_compact(CompactArgs A) {
+    __shared__ uint32_t tmp[16];
+    __shared__ int32_t smap[kMaxBatchEx];
+    const int t = threadIdx.x, l = t & 63, w = t >> 6;
+    const int q0 = t * 4;
+    uint32_t c = 0;
+    bool br[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int q = q0 + j;
+        br[j] = q < A.nq && A.route[q] == 0;
+        c += br[j] ? 1u : 0u;
+    }
+    /* inclusive prefix over the block's 16 waves */
+    uint32_t v = wave_scan_incl(c);
+    if (l == 63) tmp[w] = v;
+    __syncthreads();
+    uint32_t add = 0, nb = 0;
+    for (int i = 0; i < 16; i++) {
+        add += i < w ? tmp[i] : 0u;
+        nb += tmp[i];
+    }
+    uint32_t pos = v - c + add;
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+        if (br[j]) smap[pos++] = q0 + j;
+    __syncthreads();
+    for (int i = t; i < A.nqp; i += blockDim.x) {
+        const bool live = i < (int)nb;
+        const int src = live ? smap[i] : -1;
+        A.vmap[i] = src;
+        A.vskip[i] = live ? 0 : 1;
+        A.vpofq[i] = live ? A.pofq[src] : -1;
+        if (A.vtopbq && i < A.nq) A.vtopbq[i] = live ? A.topbq[src] : 1u;
+    }
+    /* the packed rows (zeros past them), one wave per row */
+    for (int i = w; i < A.nqp; i += 16) {
+        const int src = i < (int)nb ? smap[i] : -1;
+        const float *sr = A.q + (uint64_t)(src < 0 ? 0 : src) * A.dim;
+        float *dr = A.vq + (uint64_t)i * A.dim;
+        for (int j = l; j < A.dim; j += 64) dr[j] = src < 0 ? 0.f : sr[j];
+    }
+}
+
+/* the view's results back to the parent's query positions: block i copies
+ * slot i's hits, count, flag and status */
+__global__ __launch_bounds__(64) void k_scatter(const int32_t *vmap, int nq, uint32_t stride,
+                                                const FvHit *vout, const uint32_t *vcnt,
+                                                const uint8_t *vflag, const uint32_t *vstatus,
+                                                FvHit *out, uint32_t *cnt, uint8_t *flag,
+                                                uint32_t *status) {
+    const int i = blockIdx.x;
+    const int dst = vmap[i];
+    if (dst < 0 || i >= nq) return;
+    const uint32_t c = vcnt[i];
+    for (uint32_t j = threadIdx.x; j < c && j < stride; j += 64)
+        out[(uint64_t)dst * stride + j] = vout[(uint64_t)i * stride + j];
+    if (threadIdx.x == 0) {
+        cnt[dst] = c;
+        flag[dst] = vflag[i];
+        status[dst] = vstatus[i];
+    }
+}
+
+/* an arena with no memory: a buffer bound to it is borrowed (never freed,
+ * never grown) */
+static Arena borrow_arena;
+
+/* the view batch of b for a compacted brute run (created on first use) */
+static int setup_view(FvGpuIndex *g, FvGpuBatch *b, char *errbuf, size_t errlen) {
+    if (b->view == nullptr) b->view = new FvGpuBatch();
+    FvGpuBatch *v = b->view;
+    const int nqp = b->nqt * kQT, dim = g->dim;
+    v->owner = g;
+    v->nq = b->nq;
+    v->nqt = b->nqt;
+    v->topb = b->topb;
+    v->nprog = b->nprog;
+    v->nops = b->nops;
+    v->nvals = b->nvals;
+    v->all_spans = false;
+    v->any_dedup = false;
+    v->sum_total = (uint64_t)b->nq * g->count;
+    v->max_total = g->count;
+    v->max_entry = 0;
+    v->has_topbq = b->has_topbq;
+    v->inl_ok = false;
+    v->inl_col = -1;
+    v->d_inl_p = nullptr;
+    v->i8_ok = b->i8_ok;
+    v->stream = b->stream;
+    /* the programs and their bitmaps are the parent's */
+    v->d_ops = b->d_ops;
+    v->d_vals = b->d_vals;
+    v->d_poff = b->d_poff;
+    v->d_plen = b->d_plen;
+    v->d_bitmap.p = b->d_bitmap.p;
+    v->d_bitmap.cap = b->d_bitmap.cap;
+    v->d_bitmap.arena = &borrow_arena;
+    v->d_qd = b->d_qd;
+    v->d_spans = b->d_spans;
+    v->d_spanoff = b->d_spanoff;
+    /* the view's own inputs: packed queries, programs of queries, per-query
+     * k, skip flags */
+    const size_t o_q = 0;
+    const size_t o_p = (o_q + (size_t)nqp * dim * 4 + 255) & ~(size_t)255;
+    const size_t o_k = (o_p + (size_t)nqp * 4 + 255) & ~(size_t)255;
+    const size_t o_s = (o_k + (size_t)nqp * 4 + 255) & ~(size_t)255;
+    const size_t tot = o_s + (size_t)nqp;
+    PV_TRY(v->d_stage.ensure(tot));
+    unsigned char *d = v->d_stage.as<unsigned char>();
+    v->d_q = (float *)(d + o_q);
+    v->d_pofq = (int32_t *)(d + o_p);
+    v->d_topbq_p = b->has_topbq ? (const uint32_t *)(d + o_k) : nullptr;
+    v->d_vskip = d + o_s;
+    PV_TRY(b->d_vmap.ensure((size_t)nqp * 4));
+    PV_TRY(v->d_qbf.ensure((size_t)nqp * dim * 2));
+    PV_TRY(v->d_qaux.ensure((size_t)nqp * 16));
+    if (g->d_vi8 != nullptr) {
+        PV_TRY(v->d_qi8.ensure((size_t)nqp * dim));
+        PV_TRY(v->d_qn8.ensure((size_t)nqp * 4));
+    }
+    v->out_stride = b->topb;
+    PV_TRY(v->d_out.ensure((size_t)b->nq * b->topb * sizeof(FvHit)));
+    PV_TRY(v->d_cnt.ensure((size_t)b->nq * 4));
+    PV_TRY(v->d_flag.ensure((size_t)b->nq));
+    PV_TRY(v->d_pcnt.ensure((size_t)nqp * kPoolShards * kPcntStride * 4));
+    return 0;
+}
+
+} // namespace pv
+
 /*
  * One routed run of a staged batch: bitmaps + query prep, features, BF16
  * MLP, per-query route and ef, then the brute pipeline over the queries
@@ -773,9 +929,46 @@ extern "C" int fv_gpu_batch_run_learned(FvGpuIndex *g, FvGpuBatch *b, FvGpuModel
         }
         PV_TRY(hipGetLastError());
         const uint8_t *route = b->d_route.as<uint8_t>();
-        if (run_brute_ex(g, b, &bo, 0, route, true, errbuf, errlen,
-                         need_model && pre ? &A : nullptr) != 0)
+        /* several query tiles under the learned route: decide first, pack
+         * the brute-routed queries into the leading tiles of the view batch
+         * and scan only those (query tiles cost the same whatever share of
+         * their queries is routed away) */
+        const bool compact = o->mode == 0 && b->nqt > 1;
+        if (compact) {
+            if (pre) {
+                PV_TRY(launch_route_blocks(A, nq, s));
+            }
+            if (setup_view(g, b, errbuf, errlen) != 0) return -1;
+            FvGpuBatch *v = b->view;
+            CompactArgs C;
+            C.route = route;
+            C.nq = nq;
+            C.nqp = b->nqt * kQT;
+            C.dim = g->dim;
+            C.q = b->d_q;
+            C.pofq = b->d_pofq;
+            C.topbq = b->d_topbq_p;
+            C.vq = v->d_q;
+            C.vpofq = v->d_pofq;
+            C.vtopbq = const_cast<uint32_t *>(v->d_topbq_p);
+            C.vskip = v->d_vskip;
+            C.vmap = b->d_vmap.as<int32_t>();
+            hipLaunchKernelGGL(k_compact, dim3(1), dim3(1024), 0, s, C);
+            PV_TRY(hipGetLastError());
+            const int prec = scan_precision(g, v, &bo);
+            PV_TRY(launch_phase1(g, v, prec, false, s, nullptr, nullptr));
+            if (run_brute_ex(g, v, &bo, 0, v->d_vskip, true, errbuf, errlen, nullptr) != 0)
+                return -1;
+            hipLaunchKernelGGL(k_scatter, dim3(nq), dim3(64), 0, s, b->d_vmap.as<int32_t>(), nq,
+                               b->topb, v->d_out.as<FvHit>(), v->d_cnt.as<uint32_t>(),
+                               v->d_flag.as<uint8_t>(), qaux_of(v).status, b->d_out.as<FvHit>(),
+                               b->d_cnt.as<uint32_t>(), b->d_flag.as<uint8_t>(),
+                               qaux_of(b).status);
+            PV_TRY(hipGetLastError());
+        } else if (run_brute_ex(g, b, &bo, 0, route, true, errbuf, errlen,
+                                need_model && pre ? &A : nullptr) != 0) {
             return -1;
+        }
         if (o->mode != 3) {
             GraphExtra x;
             x.route = route;

@@ -80,3 +80,50 @@ def test_learned_ef_beats_fixed_ef_on_graph(setup):
     assert lrn[1] >= 0.9 and fixed[1] >= 0.9
     assert lrn[3] < 400
     assert lrn[0] > 1.2 * fixed[0], (lrn, fixed)
+
+
+def test_compacted_routed_run_equals_per_tile_runs(setup):
+    """A routed run over ten query tiles packs the brute-routed queries into
+    the leading tiles of a companion batch (the scan then runs only those)
+    and scatters their hits back; its outputs equal, query for query, the
+    same queries run as separate one-tile batches (no packing). The route
+    bias is set at the median logit so both routes are taken."""
+    s = setup
+    g, m = s["g"], s["model"]
+    nq = 640
+    q, sers, ent = s["q"][:nq], s["sers"][:nq], s["ent"][:nq]
+    b = E.concat(sers, g.count, entries=ent)
+    bh = g.stage(q, b, 10)
+    try:
+        g.run_learned(bh, m, L.LearnedOpts(k=10, mode=0, max_visited=50_000))
+        g.sync()
+        _, _, y = g.routes(bh, nq, with_y=True)
+        bias = float(np.median(y[:, 1]))
+        opts = L.LearnedOpts(k=10, mode=0, route_bias=bias, max_visited=50_000)
+        g.run_learned(bh, m, opts)
+        g.sync()
+        out, cnt, flag, rc = g.fetch(bh, nq, 10)
+        route, ef = g.routes(bh, nq)
+        assert rc == 0
+        assert 0.2 < route.mean() < 0.8, route.mean()
+    finally:
+        g.free_batch(bh)
+    for s0 in range(0, nq, 64):
+        bt = E.concat(sers[s0:s0 + 64], g.count, entries=ent[s0:s0 + 64])
+        bh = g.stage(q[s0:s0 + 64], bt, 10)
+        try:
+            g.run_learned(bh, m, opts)
+            g.sync()
+            o2, c2, f2, rc2 = g.fetch(bh, 64, 10)
+            r2, e2 = g.routes(bh, 64)
+            assert rc2 == 0
+            assert np.array_equal(r2, route[s0:s0 + 64]) and np.array_equal(e2, ef[s0:s0 + 64])
+            assert np.array_equal(c2, cnt[s0:s0 + 64]) and np.array_equal(f2, flag[s0:s0 + 64])
+            for j in range(64):
+                c = int(c2[j])
+                assert np.array_equal(o2[j, :c], out[s0 + j, :c]), (s0 + j, int(r2[j]))
+        finally:
+            g.free_batch(bh)
+    # the brute-routed queries equal the exact truth
+    for j in np.flatnonzero(route == 0)[:64]:
+        assert np.array_equal(out[j, : cnt[j]]["node"], s["truth"][j]), j
