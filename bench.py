@@ -76,6 +76,7 @@ from paves_amd import hnsw, indexfile, learned, synth  # noqa: E402
 
 METRIC = "filtered top-10 QPS at recall@10≥0.9: learned-ef+BF16 vs plain FP32, 1 GPU"
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFS = 157.3    # dense FP32 matrix (MFMA) peak of MI355X
 SCAN_KERNEL = "k_scan_bf16<false, 8, false>"   # main scan: 8 waves per WG, bitmap predicates
 COPY_BYTES = 1 << 28   # 256 MiB device copy used to calibrate the PMC byte counters
 
@@ -234,20 +235,22 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--workload", choices=["filtered", "uneven", "graph"], default="filtered",
-                    help="filtered: the 1M x 128 micro-batch workload (default); uneven: "
-                         "one 4096-query batch with mixed k and selectivity; graph: filtered "
-                         "HNSW beam search, ef sweep with recall@10")
+                    help="filtered: the 1M x 128 workload (default); uneven: one 4096-query "
+                         "batch with mixed k and selectivity; graph: filtered HNSW beam "
+                         "search, ef sweep with recall@10")
     ap.add_argument("--dim", type=int, choices=[128, 768], default=128,
                     help="uneven workload: 1M x 128 or 2M x 768")
     ap.add_argument("--nq", type=int, default=10_000)
-    ap.add_argument("--batch", type=int, default=64)
+    ap.add_argument("--batch", type=int, default=64, help="micro-batch of the brute baselines")
+    ap.add_argument("--learned-batch", type=int, default=1024,
+                    help="queries per routed run of the learned path (packed brute tiles)")
     ap.add_argument("--k", type=int, default=10)
     ap.add_argument("--train-queries", type=int, default=200_000,
                     help="training queries for the learned router (0 = skip the learned path)")
     ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 1),
-                    help="host threads (graph build, entry points, CPU baseline)")
+                    help="host threads (graph build, entry points, CPU baselines)")
     ap.add_argument("--fp32-steps", type=int, default=0, help="0 = max(2, steps // 4)")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--no-traffic", action="store_true",
@@ -299,53 +302,58 @@ def main():
         log(f"graph: M=16 efC=200 built in {setup['graph_build_s']}s ({args.threads} threads), "
             f"entry points {setup['entry_points_us_per_query']} us/query")
     batches = build_batches(d, post, args.n, args.nq, args.batch, args.k, ent)
+    batchesL = build_batches(d, post, args.n, args.nq, args.learned_batch, args.k, ent)
 
     g = E.GpuIndex(d["base"], cols, rows, l0=None if graph is None else graph["l0"],
                    precision=L.FV_GPU_PREC_BF16, device=0)
     staged = [g.stage(d["queries"][s:e], b, args.k) for s, e, b in batches]
-    log(f"staged {len(staged)} micro-batches")
+    stagedL = [g.stage(d["queries"][s:e], b, args.k) for s, e, b in batchesL]
+    log(f"staged {len(staged)} micro-batches of {args.batch} and {len(stagedL)} batches of "
+        f"{args.learned_batch}")
     bf = L.BruteOpts(precision=L.FV_GPU_PREC_BF16, route=0)
     fp = L.BruteOpts(precision=L.FV_GPU_PREC_FP32, route=0)
 
-    # ---- BF16 brute path: warmup then K timed steps
+    # ---- BF16 brute path: micro-batches of 64, then batches of 1024
     run_steps(g, staged, bf, args.warmup)
     g.sync()
     dt = run_steps(g, staged, bf, args.steps)
     qps_bf = args.steps * args.nq / dt
-    log(f"bf16 brute: {qps_bf:,.0f} QPS ({dt / args.steps * 1e3:.2f} ms/step)")
+    log(f"bf16 brute, micro-batches of {args.batch}: {qps_bf:,.0f} QPS "
+        f"({dt / args.steps * 1e3:.2f} ms/step)")
     res_bf = [g.fetch(bh, e - s, args.k) for bh, (s, e, _) in zip(staged, batches)]
     overflow = sum(r[3] != 0 for r in res_bf)
     truth = [o[q, : c[q]]["node"].copy() for (o, c, _, _) in res_bf for q in range(len(c))]
+    run_steps(g, stagedL, bf, args.warmup)
+    dtL = run_steps(g, stagedL, bf, args.steps)
+    qps_bfL = args.steps * args.nq / dtL
+    log(f"bf16 brute, batches of {args.learned_batch}: {qps_bfL:,.0f} QPS")
+    res_bfL = [g.fetch(bh, e - s, args.k) for bh, (s, e, _) in zip(stagedL, batchesL)]
 
-    # ---- plain FP32 path on the same staged batches
+    # ---- plain FP32 path (FP32 MFMA x.q + certified exact re-rank) on the
+    # same staged batches, at both batch sizes
     fsteps = args.fp32_steps or max(2, args.steps // 4)
     run_steps(g, staged, fp, 1)
     dt32 = run_steps(g, staged, fp, fsteps)
     qps32 = fsteps * args.nq / dt32
-    log(f"plain fp32 brute: {qps32:,.0f} QPS")
     res_fp = [g.fetch(bh, e - s, args.k) for bh, (s, e, _) in zip(staged, batches)]
+    run_steps(g, stagedL, fp, 1)
+    dt32L = run_steps(g, stagedL, fp, fsteps)
+    qps32L = fsteps * args.nq / dt32L
+    log(f"plain fp32 (FP32 MFMA) brute: {qps32:,.0f} QPS (batches of {args.batch}), "
+        f"{qps32L:,.0f} QPS (batches of {args.learned_batch})")
+    fp32_scan = measure_fp32_scan(g, staged, fp, args.n)
 
     # ---- SQ8 (exact INT8 MFMA scan of x - centre) on a second index
     sq8 = sq8_path(d, cols, rows, batches, res_bf, args)
 
     # ---- the reference's synchronous entry point (host buffers in and out:
     # predicate compile, H2D staging, pipeline, D2H), one pass, for the
-    # PCIe-inclusive rate
-    t0 = time.perf_counter()
-    for s, e, b in batches:
-        g.brute(d["queries"][s:e], b, args.k)
-    qps_sync = args.nq / (time.perf_counter() - t0)
-    log(f"fv_gpu_brute_batch (host buffers, synchronous): {qps_sync:,.0f} QPS")
+    # PCIe-inclusive rate, with its per-stage breakdown
+    qps_sync, sync_detail = sync_abi(g, d, batches, args)
 
-    # ---- correctness: BF16 == FP32 (both exact), sample == oracle
-    exact = True
-    for (o1, c1, f1, _), (o2, c2, f2, _) in zip(res_bf, res_fp):
-        if not (np.array_equal(c1, c2) and np.array_equal(f1, f2)):
-            exact = False
-        for q in range(len(c1)):
-            c = int(c1[q])
-            if not np.array_equal(o1[q, :c], o2[q, :c]):
-                exact = False
+    # ---- correctness: BF16 == FP32 (both exact), batch sizes agree, sample == oracle
+    exact = same_results(res_bf, res_fp) and same_results(
+        [r for r in res_bf], split_results(res_bfL, batchesL, batches))
     sample_ok = None
     if not args.no_check:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -361,9 +369,10 @@ def main():
     # ---- learned route + ef over BF16 brute / graph (the headline path)
     lrn = None
     if learned_on:
-        lrn = learned_path(g, graph, d, post, staged, batches, truth, args, setup)
+        lrn = learned_path(g, graph, d, post, stagedL, batchesL, truth, args, setup)
 
-    # ---- roofline of the dominant kernel (BF16 full scan), HIP events
+    # ---- roofline of the brute-force kernel (BF16 full scan, one query
+    # tile), HIP events; and the batched scan's MFMA rate
     m = measure_scan(g, batches, staged, bf, args.n)
     avg_scan, alg_bytes = m["avg_scan_us"], m["alg_bytes"]
     achieved = alg_bytes / (avg_scan * 1e-6) / 1e9
@@ -371,6 +380,7 @@ def main():
         f"{m['isolated_scan_us']:.1f} us); prepass+tau "
         f"{m['prepass_tau_us']:.1f} us, select {m['select_us']:.1f} us, "
         f"total {m['total_us']:.1f} us (isolated)")
+    batched_scan = measure_batched_scan(g, stagedL, batchesL, bf, args)
     copy_gbs = None
     try:
         copy_gbs = copy_calibration()
@@ -378,12 +388,15 @@ def main():
     except RuntimeError as ex:
         log(f"device copy not measured: {ex}")
 
-    # ---- CPU baseline: reference BruteCursor on a bounded sample
-    cpu = None
+    # ---- CPU baselines: reference BruteCursor and HnswCursor, bounded samples
+    cpu, cpu_hnsw = None, None
     if not args.no_cpu:
         cpu = cpu_baseline(d, cols, rows, vals, args)
+        if graph is not None:
+            cpu_hnsw = graph_cpu_baseline(d, cols, rows, vals, graph, ent, truth, args,
+                                          ef=lrn["cpu_hnsw_ef"] if lrn else 400)
 
-    for bh in staged:
+    for bh in staged + stagedL:
         g.free_batch(bh)
     g.close()
 
@@ -391,17 +404,19 @@ def main():
         value, steps_ms, rec = lrn["qps"], lrn["ms_per_step"], lrn["recall_at_10"]
     else:
         value, steps_ms, rec = qps_bf, dt / args.steps * 1e3, (1.0 if exact else None)
+    plain = max(qps32, qps32L)
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "queries/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(steps_ms, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16",
         "data": "synthetic (seeded GMM, see paves_amd/synth.py)",
-        "config": {"workload": "filtered top-10, learned route/ef over BF16 brute and HNSW: "
-                               "1M x 128 GMM, 10k queries in micro-batches of 64, six filter "
-                               "classes" if lrn else
-                               "filtered brute-force top-10: 1M x 128 GMM, 10k queries in "
-                               "micro-batches of 64, six filter classes",
+        "config": {"workload": (f"filtered top-10, learned route/ef over BF16 brute and HNSW, "
+                                f"routed runs of {args.learned_batch} queries: 1M x 128 GMM, "
+                                f"10k queries, six filter classes") if lrn else
+                               (f"filtered brute-force top-10: 1M x 128 GMM, 10k queries in "
+                                f"micro-batches of {args.batch}, six filter classes"),
                    "n": args.n, "dim": 128, "queries": args.nq, "micro_batch": args.batch,
+                   "learned_batch": args.learned_batch if lrn else None,
                    "k": args.k, "topb": args.k,
                    "graph": "HNSW M=16 (M0=32) efC=200" if lrn else None,
                    "train_queries": args.train_queries if lrn else 0},
@@ -415,6 +430,10 @@ def main():
                      "copy_peak_GBs": round(copy_gbs, 1) if copy_gbs else None,
                      "frac_of_copy_peak": round(achieved / copy_gbs, 4) if copy_gbs else None,
                      "traffic_detail": traffic_info,
+                     "note": "the brute-force kernel (one 64-query tile per workgroup, the "
+                             "micro-batch shape); the learned path's batches of "
+                             f"{args.learned_batch} run the multi-tile scan reported in "
+                             "batched_scan",
                      "overlapped": {
                          "kernel": "k_scan_bf16<false, 4, false>",
                          "note": "in the timed region batches overlap on the library's streams "
@@ -423,11 +442,17 @@ def main():
                                  "all scan-route micro-batches per step / step time",
                          "aggregate_GBs": round(m["scan_batches"] * alg_bytes * args.steps / dt
                                                 / 1e9, 1)}},
+        "batched_scan": batched_scan,
         "recall_at_10": round(rec, 4) if rec is not None else None,
         "bf16_brute_qps": round(qps_bf, 1),
-        "plain_fp32_qps": round(qps32, 1),
-        "speedup_vs_plain_fp32": round(value / qps32, 3),
+        "bf16_brute_qps_batched": round(qps_bfL, 1),
+        "plain_fp32_qps": round(plain, 1),
+        "plain_fp32": {"kind": "FP32 MFMA x.q (v_mfma_f32_32x32x2_f32) + certified exact "
+                               "FP32 re-rank", "qps_micro_batches": round(qps32, 1),
+                       "qps_batched": round(qps32L, 1), "scan": fp32_scan},
+        "speedup_vs_plain_fp32": round(value / plain, 3),
         "sync_abi_qps_host_buffers": round(qps_sync, 1),
+        "sync_abi_breakdown": sync_detail,
         "sq8": sq8,
         "ids_bit_exact_bf16_vs_fp32": exact,
         "oracle_sample_ok": sample_ok,
@@ -435,8 +460,84 @@ def main():
         "learned": lrn,
         "setup": setup,
         "cpu_baseline": cpu,
+        "cpu_hnsw_baseline": cpu_hnsw,
     }
     print(json.dumps(out))
+
+
+def same_results(ra, rb):
+    """Two lists of (out, cnt, flag, rc) agree in counts, flags, ids and distances."""
+    for (o1, c1, f1, _), (o2, c2, f2, _) in zip(ra, rb):
+        if not (np.array_equal(c1, c2) and np.array_equal(f1, f2)):
+            return False
+        for q in range(len(c1)):
+            c = int(c1[q])
+            if not np.array_equal(o1[q, :c], o2[q, :c]):
+                return False
+    return True
+
+
+def split_results(res, batches_from, batches_to):
+    """Results fetched per batch of one size, re-cut to another batch size."""
+    o = np.concatenate([r[0] for r in res])
+    c = np.concatenate([r[1] for r in res])
+    f = np.concatenate([r[2] for r in res])
+    return [(o[s:e], c[s:e], f[s:e], 0) for s, e, _ in batches_to]
+
+
+def measure_fp32_scan(g, staged, fp, n, reps=5, nb=8):
+    """The FP32 MFMA scan kernel alone (back-to-back launches on a few
+    micro-batches): duration and FLOP rate against the FP32 matrix peak."""
+    us = float(np.mean([g.time_scan(bh, fp, reps) for bh in staged[:nb]]))
+    flops = 2.0 * n * 128 * 64
+    tf = flops / (us * 1e-6) / 1e12
+    log(f"fp32 MFMA scan: {us:.1f} us per 64-query launch -> {tf:.1f} TFLOP/s")
+    return {"kernel": "k_scan_f32m<false, 8>", "avg_launch_us": round(us, 2),
+            "flops_per_launch": flops, "achieved_TFLOPs": round(tf, 2),
+            "peak_TFLOPs": FP32_PEAK_TFS, "frac": round(tf / FP32_PEAK_TFS, 4)}
+
+
+def measure_batched_scan(g, stagedL, batchesL, bf, args, reps=5):
+    """The main scan of a full batch of args.learned_batch queries (the
+    multi-query-tile kernel) alone: MFMA rate and HBM rate."""
+    full = [bh for bh, (s, e, _) in zip(stagedL, batchesL) if e - s == args.learned_batch]
+    if not full:
+        return None
+    g.run_brute(full[0], bf)
+    g.sync()
+    us = g.time_scan(full[0], bf, reps)
+    nq = args.learned_batch
+    flops = 2.0 * args.n * 128 * nq
+    tf = flops / (us * 1e-6) / 1e12
+    log(f"batched scan ({nq} queries): {us:.1f} us -> {tf:.0f} TFLOP/s")
+    return {"kernel": "k_scan_bf16_mq (query tiles per workgroup: library default)",
+            "queries": nq, "avg_launch_us": round(us, 2), "flops_per_launch": flops,
+            "achieved_TFLOPs": round(tf, 1), "peak_TFLOPs": 2500.0,
+            "frac": round(tf / 2500.0, 4)}
+
+
+def sync_abi(g, d, batches, args):
+    """fv_gpu_brute_batch (host buffers in and out) over every micro-batch,
+    once; with event timing, the device time of its pipeline per call."""
+    t0 = time.perf_counter()
+    for s, e, b in batches:
+        g.brute(d["queries"][s:e], b, args.k)
+    el = time.perf_counter() - t0
+    qps = args.nq / el
+    g.set_timing(True)
+    dev = []
+    for s, e, b in batches[:32]:
+        g.brute(d["queries"][s:e], b, args.k)
+        dev.append(g.stats().t_total_us)
+    g.set_timing(False)
+    per_call = el / len(batches) * 1e6
+    dev_us = float(np.mean(dev))
+    log(f"fv_gpu_brute_batch (host buffers, synchronous): {qps:,.0f} QPS; {per_call:.1f} us "
+        f"per call, of which {dev_us:.1f} us device pipeline")
+    return qps, {"us_per_call": round(per_call, 1), "device_pipeline_us": round(dev_us, 1),
+                 "host_and_transfer_us": round(per_call - dev_us, 1),
+                 "note": "per call: predicate compile and packing on the host, one H2D copy, "
+                         "the device pipeline, D2H of results, synchronisation"}
 
 
 GRAPH_EFS = (40, 80, 160, 400)
@@ -520,8 +621,8 @@ def O_recall(got, truth):
     return len(tr & set(int(x) for x in got)) / len(tr) if tr else 1.0
 
 
-def graph_cpu_baseline(d, cols, rows, vals, graph, ent, truth, args):
-    """Reference HnswCursor (engine.cpp, oracle/_ref) at ef = 400 on a
+def graph_cpu_baseline(d, cols, rows, vals, graph, ent, truth, args, ef=400):
+    """Reference HnswCursor (engine.cpp, oracle/_ref) at the given ef on a
     bounded sample, one query per host thread, reading a reference-format
     index file carrying the same graph. Each thread opens the index itself:
     the reference cursor borrows a per-index visited array (not shared
@@ -555,7 +656,7 @@ def graph_cpu_baseline(d, cols, rows, vals, graph, ent, truth, args):
                 p = synth.filter_pred(classes[qi % 6], d["qcent"][qi])
                 ph, keep = ref.compile(idx, p)
                 try:
-                    hits = ref.hnsw(idx, d["queries"][qi], ph, 400, args.k)
+                    hits = ref.hnsw(idx, d["queries"][qi], ph, ef, args.k)
                 finally:
                     if ph:
                         ref.L.fv_pred_free(ph)
@@ -571,11 +672,13 @@ def graph_cpu_baseline(d, cols, rows, vals, graph, ent, truth, args):
             ref.close(idx)
         recs = [x for r in recs for x in r]
         done = len(recs)
-        log(f"cpu reference HNSW ef=400: {done} queries in {el:.1f}s, recall {np.mean(recs):.4f}")
+        log(f"cpu reference HNSW ef={ef}: {done} queries in {el:.1f}s, "
+            f"recall {np.mean(recs):.4f}")
         return {"value": round(done / el, 2), "unit": "queries/s", "cores": threads,
                 "kind": "reference", "recall_at_10": round(float(np.mean(recs)), 4),
+                "ef": ef, **host_info(),
                 "sample": f"first {done} queries (classes interleaved), reference HnswCursor "
-                          f"(engine.cpp) at ef=400, one query per thread on {threads} threads, "
+                          f"(engine.cpp) at ef={ef}, one query per thread on {threads} threads, "
                           "same graph read from a reference-format index file"}
     finally:
         try:
@@ -754,16 +857,21 @@ def learned_path(g, graph, d, post, staged, batches, truth, args, setup):
     tsers = [E.serialize(synth.filter_pred(classes[i % 6], tc[i]), args.n, post)
              for i in range(args.train_queries)]
     setup["train_queries_gen_s"] = round(time.perf_counter() - t, 2)
-    model, info = learned.train_router(g, graph, d["base"], tq, tsers, k=args.k, log=log)
+    model, info = learned.train_router(g, graph, d["base"], tq, tsers, k=args.k, log=log,
+                                       micro=args.learned_batch)
     setup["router"] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in info.items()}
     # safety factor on the predicted ef, from training queries held apart
-    nval = min(2048, args.train_queries)
+    nval = min(4096, args.train_queries)
     vq, vs = tq[-nval:], tsers[-nval:]
     vent = hnsw.entries(graph, d["base"], vq, threads=args.threads)
     vtruth = learned.exact_truth(g, vq, vs, args.k)
-    scale = learned.calibrate_scale(g, model, vq, vs, vent, vtruth, k=args.k, mode=0)
-    opts = L.LearnedOpts(k=args.k, mode=0, ef_scale=scale, max_visited=50_000)
-    log(f"router: ef scale {scale}")
+    scale = learned.calibrate_scale(g, model, vq, vs, vent, vtruth, k=args.k, mode=0,
+                                    micro=args.learned_batch)
+    bias, btable = learned.calibrate_bias(g, model, vq, vs, vent, vtruth, k=args.k, scale=scale,
+                                          micro=args.learned_batch, reps=5)
+    opts = L.LearnedOpts(k=args.k, mode=0, ef_scale=scale, route_bias=bias, max_visited=50_000)
+    log(f"router: ef scale {scale}, route bias {bias:.3g}; held-out calibration (graph share, "
+        f"QPS, recall): " + ", ".join(f"({t[0]:.2f}, {t[2]:,.0f}, {t[3]:.3f})" for t in btable))
 
     def step():
         for bh in staged:
@@ -806,9 +914,32 @@ def learned_path(g, graph, d, post, staged, batches, truth, args, setup):
     return {"qps": qps, "ms_per_step": dt / args.steps * 1e3, "recall_at_10": float(np.mean(rec)),
             "recall_per_class": {classes[i]: round(float(np.mean(per[i])), 4) for i in range(6)},
             "graph_routed_frac": ng / args.nq, "mean_ef_graph_routed": float(np.mean(efs)) if efs else 0.0,
-            "ef_scale": scale,
+            "ef_scale": scale, "route_bias": bias,
+            "bias_calibration": [{"graph_share_target": t[0], "qps": round(t[2], 1),
+                                  "recall_at_10": round(t[3], 4),
+                                  "graph_routed_frac": round(t[4], 4)} for t in btable],
             "fixed_ef400_graph_qps": round(fx[0], 1), "fixed_ef400_graph_recall": round(fx[1], 4),
-            "fixed_ef400_sample": nfx}
+            "fixed_ef400_sample": nfx, "cpu_hnsw_ef": 80,
+            "batch": args.learned_batch,
+            "note": "each routed run decides every query's route and ef on the GPU, packs the "
+                    "brute-routed queries into the leading query tiles (the scan runs only "
+                    "those), then runs the graph search for the rest"}
+
+
+def host_info():
+    """nproc and the CPU model of the host the baseline ran on."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "cpu_model": model,
+            "sched_affinity": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity")
+            else None}
 
 
 def cpu_baseline(d, cols, rows, vals, args):
@@ -843,7 +974,7 @@ def cpu_baseline(d, cols, rows, vals, args):
         ref.close(idx)
         log(f"cpu reference: {done} queries in {el:.1f}s ({threads} threads per query)")
         return {"value": round(done / el, 2), "unit": "queries/s", "cores": threads,
-                "kind": "reference",
+                "kind": "reference", **host_info(),
                 "sample": f"first {done} of the 10k queries (classes interleaved), "
                           f"reference BruteCursor (engine.cpp) with {threads} threads per query, "
                           f"same 1M x 128 data read from a reference-format index file"}

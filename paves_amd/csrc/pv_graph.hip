@@ -736,10 +736,11 @@ int run_graph_ex(FvGpuIndex *g, FvGpuBatch *b, uint32_t ef, uint64_t max_visited
     if (shm > 64 * 1024)
         PV_TRY(hipFuncSetAttribute((const void *)k_graph,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-    g->ev_stream = b->stream;
+    hipStream_t gs = x.stream ? x.stream : b->stream;
+    g->ev_stream = gs;
     ev_record(g, 0);
-    if (!x.bitmap_ready) PV_TRY(launch_bitmap(g, b, b->stream));
-    hipLaunchKernelGGL(k_graph, dim3(b->nq), dim3(kGT), shm, b->stream, A);
+    if (!x.bitmap_ready) PV_TRY(launch_bitmap(g, b, gs));
+    hipLaunchKernelGGL(k_graph, dim3(b->nq), dim3(kGT), shm, gs, A);
     PV_TRY(hipGetLastError());
     ev_record(g, 5);
     memset(&g->last, 0, sizeof(g->last));

@@ -256,6 +256,10 @@ struct FvGpuBatch {
     FvGpuBatch *view = nullptr;
     pv::DevBuf d_vmap;    /* view slot -> parent query [nqp] (i32) */
     uint8_t *d_vskip = nullptr; /* a view's skip flags [nqp] (its staging area) */
+    /* a routed run's graph search runs beside the packed brute pipeline on
+     * this second stream (default priority), joined by events */
+    hipStream_t aux_stream = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
 
 struct FvGpuIndex {
@@ -350,6 +354,7 @@ struct GraphExtra {
     const uint32_t *efq = nullptr;   /* device [nq]: per-query ef (<= ef) */
     uint32_t out_k = 0;              /* hits per query written (0 = ef) */
     bool bitmap_ready = false;       /* predicate bitmaps already built this run */
+    hipStream_t stream = nullptr;    /* launch here instead of the batch's stream */
 };
 int run_graph_ex(FvGpuIndex *g, FvGpuBatch *b, uint32_t ef, uint64_t max_visited, int wmax,
                  int fcap, uint32_t *dbg, const GraphExtra &x, char *errbuf, size_t errlen);

@@ -689,8 +689,8 @@ namespace pv {
  * of the view batch; later slots are marked skipped, so every query tile
  * past the packed queries is skipped whole by the brute pipeline. One
  * 1024-thread block (nq <= 4096: four consecutive queries per thread, a
- * block-wide prefix over the per-thread counts), then the packed rows are
- * copied by all threads.
+ * block-wide prefix over the per-thread counts); k_gather_rows then copies
+ * the packed rows.
  */
 struct CompactArgs {
     const uint8_t *route;
@@ -740,13 +740,19 @@ __global__ __launch_bounds__(1024) void k_compact(CompactArgs A) {
         A.vpofq[i] = live ? A.pofq[src] : -1;
         if (A.vtopbq && i < A.nq) A.vtopbq[i] = live ? A.topbq[src] : 1u;
     }
-    /* the packed rows (zeros past them), one wave per row */
-    for (int i = w; i < A.nqp; i += 16) {
-        const int src = i < (int)nb ? smap[i] : -1;
-        const float *sr = A.q + (uint64_t)(src < 0 ? 0 : src) * A.dim;
-        float *dr = A.vq + (uint64_t)i * A.dim;
-        for (int j = l; j < A.dim; j += 64) dr[j] = src < 0 ? 0.f : sr[j];
-    }
+}
+
+/* the packed query rows (zeros past the packed queries): one wave per row,
+ * four rows per block */
+__global__ __launch_bounds__(256) void k_gather_rows(const int32_t *__restrict__ vmap, int nqp,
+                                                     int dim, const float *__restrict__ q,
+                                                     float *__restrict__ vq) {
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6), l = threadIdx.x & 63;
+    if (i >= nqp) return;
+    const int src = vmap[i];
+    const float *sr = q + (uint64_t)(src < 0 ? 0 : src) * dim;
+    float *dr = vq + (uint64_t)i * dim;
+    for (int j = l; j < dim; j += 64) dr[j] = src < 0 ? 0.f : sr[j];
 }
 
 /* the view's results back to the parent's query positions: block i copies
@@ -934,10 +940,34 @@ extern "C" int fv_gpu_batch_run_learned(FvGpuIndex *g, FvGpuBatch *b, FvGpuModel
          * and scan only those (query tiles cost the same whatever share of
          * their queries is routed away) */
         const bool compact = o->mode == 0 && b->nqt > 1;
+        const uint32_t ef_launch = o->mode == 2 ? ef_fixed : ef_max;
+        const uint64_t mv = o->max_visited ? o->max_visited
+                                           : std::max<uint64_t>(200000, (uint64_t)ef_launch * 400);
+        GraphExtra x;
+        x.route = route;
+        x.efq = b->d_efq.as<uint32_t>();
+        x.out_k = o->k;
+        x.bitmap_ready = true;
         if (compact) {
             if (pre) {
                 PV_TRY(launch_route_blocks(A, nq, s));
             }
+            /* the graph search of the graph-routed queries on a second
+             * stream, beside the packed brute pipeline (disjoint outputs);
+             * it is enqueued first, so any buffer it grows is grown before
+             * the brute side's kernels are queued */
+            if (b->aux_stream == nullptr)
+                PV_TRY(hipStreamCreateWithFlags(&b->aux_stream, hipStreamNonBlocking));
+            if (b->ev_fork == nullptr)
+                PV_TRY(hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming));
+            if (b->ev_join == nullptr)
+                PV_TRY(hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming));
+            PV_TRY(hipEventRecord(b->ev_fork, s));
+            PV_TRY(hipStreamWaitEvent(b->aux_stream, b->ev_fork, 0));
+            x.stream = b->aux_stream;
+            if (run_graph_ex(g, b, ef_launch, mv, 8, 1024, nullptr, x, errbuf, errlen) != 0)
+                return -1;
+            PV_TRY(hipEventRecord(b->ev_join, b->aux_stream));
             if (setup_view(g, b, errbuf, errlen) != 0) return -1;
             FvGpuBatch *v = b->view;
             CompactArgs C;
@@ -955,6 +985,9 @@ extern "C" int fv_gpu_batch_run_learned(FvGpuIndex *g, FvGpuBatch *b, FvGpuModel
             C.vmap = b->d_vmap.as<int32_t>();
             hipLaunchKernelGGL(k_compact, dim3(1), dim3(1024), 0, s, C);
             PV_TRY(hipGetLastError());
+            hipLaunchKernelGGL(k_gather_rows, dim3((C.nqp + 3) / 4), dim3(256), 0, s, C.vmap,
+                               C.nqp, C.dim, C.q, C.vq);
+            PV_TRY(hipGetLastError());
             const int prec = scan_precision(g, v, &bo);
             PV_TRY(launch_phase1(g, v, prec, false, s, nullptr, nullptr));
             if (run_brute_ex(g, v, &bo, 0, v->d_vskip, true, errbuf, errlen, nullptr) != 0)
@@ -965,20 +998,13 @@ extern "C" int fv_gpu_batch_run_learned(FvGpuIndex *g, FvGpuBatch *b, FvGpuModel
                                b->d_cnt.as<uint32_t>(), b->d_flag.as<uint8_t>(),
                                qaux_of(b).status);
             PV_TRY(hipGetLastError());
-        } else if (run_brute_ex(g, b, &bo, 0, route, true, errbuf, errlen,
-                                need_model && pre ? &A : nullptr) != 0) {
-            return -1;
-        }
-        if (o->mode != 3) {
-            GraphExtra x;
-            x.route = route;
-            x.efq = b->d_efq.as<uint32_t>();
-            x.out_k = o->k;
-            x.bitmap_ready = true;
-            const uint32_t ef_launch = o->mode == 2 ? ef_fixed : ef_max;
-            const uint64_t mv = o->max_visited ? o->max_visited
-                                               : std::max<uint64_t>(200000, (uint64_t)ef_launch * 400);
-            if (run_graph_ex(g, b, ef_launch, mv, 8, 1024, nullptr, x, errbuf, errlen) != 0)
+            PV_TRY(hipStreamWaitEvent(s, b->ev_join, 0));
+        } else {
+            if (run_brute_ex(g, b, &bo, 0, route, true, errbuf, errlen,
+                             need_model && pre ? &A : nullptr) != 0)
+                return -1;
+            if (o->mode != 3 &&
+                run_graph_ex(g, b, ef_launch, mv, 8, 1024, nullptr, x, errbuf, errlen) != 0)
                 return -1;
         }
         b->out_stride = o->k;

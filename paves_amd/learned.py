@@ -156,10 +156,11 @@ def ef_sweep(index, queries, sers, entries, truth, k=10, grid=EF_GRID, target=0.
 
 
 def service_costs(index, queries, sers, entries, k=10, ef=64, micro=64):
-    """Measured per-query service time of the brute path (micro-batches of
-    `micro`, device-resident, overlapped as in production) and per visited
-    node of the graph search (a batch of up to 1024 queries): the cost
-    model that labels the route."""
+    """Measured per-query service time of the brute path (batches of
+    `micro` queries, device-resident, overlapped as in production) and per
+    visited node of the graph search (a batch of up to 1024 queries): the
+    cost model that labels the route, for routed runs over batches of
+    `micro` queries."""
     import time
     from . import engine as E
     from . import _lib as LL
@@ -180,12 +181,17 @@ def service_costs(index, queries, sers, entries, k=10, ef=64, micro=64):
     t_brute = (time.perf_counter() - t0) / (3 * nb * micro)
     for bh in staged:
         index.free_batch(bh)
+    # graph throughput per visited node: the search kernel alone (HIP
+    # events), a full batch of unfiltered queries, so the figure is the
+    # kernel's rate with the chip occupied rather than the latency tail of a
+    # few long filtered searches
     m = min(len(queries), 1024)
-    b = E.concat(sers[:m], index.count, entries=entries[:m])
+    b = E.concat([E.serialize(None, index.count)] * m, index.count, entries=entries[:m])
     index.graph(queries[:m], b, ef, 200_000)
-    t0 = time.perf_counter()
+    index.set_timing(True)
     _, _, _, dbg = index.graph(queries[:m], b, ef, 200_000, debug=True)
-    t_graph = time.perf_counter() - t0
+    t_graph = index.stats().t_graph_us * 1e-6
+    index.set_timing(False)
     t_node = t_graph / max(1.0, float(dbg[:, 1].sum()))
     return t_brute, t_node
 
@@ -252,20 +258,21 @@ def run_mode(index, model, queries, sers, entries, truth, opts, micro=64, reps=1
 
 
 def calibrate_scale(index, model, queries, sers, entries, truth, k=10, target=0.9,
-                    scales=(1.0, 1.25, 1.5, 2.0, 3.0, 4.0), mode=1):
+                    scales=(1.0, 1.25, 1.5, 2.0, 3.0, 4.0), mode=1, micro=64):
     """Smallest safety factor on the predicted ef whose mean recall over
     (queries) reaches the target in the given mode."""
     from . import _lib as LL
     for s in scales:
         _, rec, _, _ = run_mode(index, model, queries, sers, entries, truth,
-                                LL.LearnedOpts(k=k, mode=mode, ef_scale=s, max_visited=50_000))
+                                LL.LearnedOpts(k=k, mode=mode, ef_scale=s, max_visited=50_000),
+                                micro=micro)
         if rec >= target:
             return s
     return scales[-1]
 
 
 def train_router(index, graph, base, queries, sers, k=10, target=0.9, epochs=30, batch=1024,
-                 lr=2e-3, centroids=None, log=None):
+                 lr=2e-3, centroids=None, log=None, micro=64):
     """End to end: exact truth, ef labels, cost-model routes, features,
     GPU training. Returns (model, info)."""
     import time
@@ -276,7 +283,7 @@ def train_router(index, graph, base, queries, sers, k=10, target=0.9, epochs=30,
     t1 = time.perf_counter()
     min_ef, visited = ef_sweep(index, queries, sers, ent, truth, k=k, target=target)
     t2 = time.perf_counter()
-    t_b, t_v = service_costs(index, queries, sers, ent, k=k)
+    t_b, t_v = service_costs(index, queries, sers, ent, k=k, micro=micro)
     tg = targets_from_labels(min_ef, visited, t_b, t_v)
     cent = centroids if centroids is not None else kmeans(base, 256, iters=6, sample=30_000)
     model = Model(base.shape[1], cent)
@@ -293,3 +300,25 @@ def train_router(index, graph, base, queries, sers, k=10, target=0.9, epochs=30,
     if log:
         log(f"router trained: {info}")
     return model, info
+
+
+def calibrate_bias(index, model, queries, sers, entries, truth, k=10, target=0.9, scale=1.0,
+                   micro=1024, fracs=(0.0, 0.05, 0.1, 0.2, 0.3, 0.5), reps=3):
+    """The route threshold (graph when the route logit exceeds it) that
+    gives the highest measured throughput of routed runs over (queries) at
+    mean recall@k >= target. Candidates send the given fractions of these
+    queries to the graph (0: all brute, the fallback). Returns (bias,
+    table of (fraction, bias, QPS, recall, graph-routed fraction))."""
+    from . import _lib as LL
+    y = model.predict(features_of(model, index, queries, sers))
+    table, best = [], None
+    for f in fracs:
+        bias = 1e30 if f == 0.0 else float(np.quantile(y[:, 1], 1.0 - f))
+        qps, rec, gfrac, _ = run_mode(index, model, queries, sers, entries, truth,
+                                      LL.LearnedOpts(k=k, mode=0, ef_scale=scale,
+                                                     route_bias=bias, max_visited=50_000),
+                                      micro=micro, reps=reps)
+        table.append((f, bias, qps, rec, gfrac))
+        if rec >= target and (best is None or qps > best[2]):
+            best = table[-1]
+    return (best[1] if best else 1e30), table
