@@ -863,15 +863,18 @@ def learned_path(g, graph, d, post, staged, batches, truth, args, setup):
     # safety factor on the predicted ef, from training queries held apart
     nval = min(4096, args.train_queries)
     vq, vs = tq[-nval:], tsers[-nval:]
+    vcls = np.arange(args.train_queries - nval, args.train_queries) % 6
     vent = hnsw.entries(graph, d["base"], vq, threads=args.threads)
     vtruth = learned.exact_truth(g, vq, vs, args.k)
-    scale = learned.calibrate_scale(g, model, vq, vs, vent, vtruth, k=args.k, mode=0,
-                                    micro=args.learned_batch)
-    bias, btable = learned.calibrate_bias(g, model, vq, vs, vent, vtruth, k=args.k, scale=scale,
-                                          micro=args.learned_batch, reps=5)
+    # route threshold and ef factor: fastest held-out setting whose recall
+    # reaches the target in every filter class
+    bias, scale, btable = learned.calibrate_bias(g, model, vq, vs, vent, vtruth, k=args.k,
+                                                 micro=args.learned_batch, reps=5, groups=vcls,
+                                                 scales=(1.0, 1.5, 2.0, 3.0))
     opts = L.LearnedOpts(k=args.k, mode=0, ef_scale=scale, route_bias=bias, max_visited=50_000)
     log(f"router: ef scale {scale}, route bias {bias:.3g}; held-out calibration (graph share, "
-        f"QPS, recall): " + ", ".join(f"({t[0]:.2f}, {t[2]:,.0f}, {t[3]:.3f})" for t in btable))
+        f"ef scale, QPS, worst-class recall): " +
+        ", ".join(f"({t[0]:.2f}, {t[1]}, {t[3]:,.0f}, {t[4]:.3f})" for t in btable))
 
     def step():
         for bh in staged:
@@ -915,9 +918,9 @@ def learned_path(g, graph, d, post, staged, batches, truth, args, setup):
             "recall_per_class": {classes[i]: round(float(np.mean(per[i])), 4) for i in range(6)},
             "graph_routed_frac": ng / args.nq, "mean_ef_graph_routed": float(np.mean(efs)) if efs else 0.0,
             "ef_scale": scale, "route_bias": bias,
-            "bias_calibration": [{"graph_share_target": t[0], "qps": round(t[2], 1),
-                                  "recall_at_10": round(t[3], 4),
-                                  "graph_routed_frac": round(t[4], 4)} for t in btable],
+            "bias_calibration": [{"graph_share_target": t[0], "ef_scale": t[1],
+                                  "qps": round(t[3], 1), "worst_class_recall_at_10": round(t[4], 4),
+                                  "graph_routed_frac": round(t[5], 4)} for t in btable],
             "fixed_ef400_graph_qps": round(fx[0], 1), "fixed_ef400_graph_recall": round(fx[1], 4),
             "fixed_ef400_sample": nfx, "cpu_hnsw_ef": 80,
             "batch": args.learned_batch,

@@ -287,6 +287,23 @@ FvGpuBatch *fv_gpu_batch_stage_ex(FvGpuIndex *g, int nq, const float *queries,
                                   const uint64_t *spanoff, int nspanoff,
                                   const uint32_t *topb_q, uint32_t topb, char *errbuf,
                                   size_t errlen);
+/* Restage (no reference counterpart; the reference re-uploads every batch,
+ * worker.c:453-598): write a new set of queries, predicates and spans into
+ * an existing batch's own buffers, after its pending work. topb_q may be
+ * NULL (same k for every query). The staging layout is kept when the new
+ * contents fit it (same query count and per-query-k mode, predicate and
+ * span tables within the capacities reserved at staging: powers of two),
+ * so the batch's captured run (FvGpuBruteOpts.graph) replays for the new
+ * batch with no capture: a stream of batches pays one capture. Returns 1
+ * when the captured run stays valid, 0 when there is none or the next graph
+ * run captures again, -1 + errbuf on error. */
+int fv_gpu_batch_restage(FvGpuIndex *g, FvGpuBatch *b, int nq, const float *queries,
+                         const FvGpuQueryDesc *qd, const uint8_t *dedup,
+                         const FvGpuPredNode *nodes, int nnodes, const int32_t *children,
+                         int nchildren, const double *invals, int ninvals,
+                         const FvGpuSpan *spans, int nspans_total, const uint64_t *spanoff,
+                         int nspanoff, const uint32_t *topb_q, uint32_t topb, char *errbuf,
+                         size_t errlen);
 /* Enqueue the brute pipeline for a staged batch on the batch's stream
  * (asynchronous; 0 / -1 + errbuf). Each batch is given one of the index's
  * batch streams when it is staged, so different batches may overlap. */

@@ -50,7 +50,8 @@ EXT_SYMBOLS = ["fv_gpu_set_timing", "fv_gpu_last_stats", "fv_gpu_brute_batch_ex"
                "fv_gpu_model_set_params", "fv_gpu_model_get_params", "fv_gpu_model_grad",
                "fv_gpu_model_train", "fv_gpu_model_predict", "fv_gpu_batch_features",
                "fv_gpu_batch_run_learned", "fv_gpu_batch_routes", "fv_gpu_batch_stage_ex",
-               "fv_gpu_batch_tune", "fv_gpu_batch_time_scan", "fv_gpu_brute_batch_noalloc"]
+               "fv_gpu_batch_tune", "fv_gpu_batch_time_scan", "fv_gpu_brute_batch_noalloc",
+               "fv_gpu_batch_restage"]
 
 
 class BruteOpts(C.Structure):
@@ -141,6 +142,9 @@ def load() -> C.CDLL:
     lib.fv_gpu_batch_stage_ex.argtypes = [_P, C.c_int, _P, _P, _P, _P, C.c_int, _P, C.c_int,
                                           _P, C.c_int, _P, C.c_int, _P, C.c_int, _P, C.c_uint32,
                                           C.c_char_p, sz]
+    lib.fv_gpu_batch_restage.argtypes = [_P, _P, C.c_int, _P, _P, _P, _P, C.c_int, _P,
+                                         C.c_int, _P, C.c_int, _P, C.c_int, _P, C.c_int, _P,
+                                         C.c_uint32, C.c_char_p, sz]
     lib.fv_gpu_batch_run_brute.argtypes = [_P, _P, C.POINTER(BruteOpts), C.c_char_p, sz]
     lib.fv_gpu_batch_time_scan.argtypes = [_P, _P, C.POINTER(BruteOpts), C.c_int,
                                            C.POINTER(C.c_double), C.c_char_p, sz]

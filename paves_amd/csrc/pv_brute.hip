@@ -74,7 +74,7 @@ namespace pv {
  * |2 x.q| <= 2 D V Q (V, Q = largest magnitudes).
  */
 __device__ __forceinline__ void prep_query(
-    int qi, const float *__restrict__ q, int nq, int dim, uint16_t *__restrict__ qbf,
+    int qi, const float *__restrict__ q, int nq, int dim, int dimb, uint16_t *__restrict__ qbf,
     float *__restrict__ qnorm, float *__restrict__ eps, float xmax_norm,
     float vmin, float vmax, int all_int, int mode, int8_t *__restrict__ qi8 = nullptr,
     float *__restrict__ qn8 = nullptr, int centre = 0, uint32_t *__restrict__ status = nullptr) {
@@ -84,7 +84,7 @@ __device__ __forceinline__ void prep_query(
     int nonint = 0, out8 = 0;
     for (int j = l; j < dim; j += 64) {
         float v = src[j];
-        qbf[(uint64_t)qi * dim + j] = f2bf(v);
+        qbf[(uint64_t)qi * dimb + j] = f2bf(v);
         ss = fmaf(v, v, ss);
         pmax = fmaxf(pmax, v);
         nmax = fmaxf(nmax, -v);
@@ -96,6 +96,8 @@ __device__ __forceinline__ void prep_query(
             s8 = fmaf(c, c, s8);
         }
     }
+    /* the BF16 operand's zero padding (the tiled copy's dims past dim) */
+    for (int j = dim + l; j < dimb; j += 64) qbf[(uint64_t)qi * dimb + j] = 0;
 #pragma unroll
     for (int o = 32; o; o >>= 1) {
         ss += lane_xor(ss, o);
@@ -126,8 +128,10 @@ __device__ __forceinline__ void prep_query(
             const double g = (D + 8.0) * ldexp(1.0, -24);
             const double big = (xn + qn) * (xn + qn);
             if (mode == 1) {
+                /* the MFMA k-steps run over the padded dims */
                 const double u = ldexp(1.0, -8);
-                e = (4.0 * u + 2.0 * u * u) * xn * qn + 4.0 * g * big;
+                const double gb = ((double)dimb + 8.0) * ldexp(1.0, -24);
+                e = (4.0 * u + 2.0 * u * u) * xn * qn + 4.0 * gb * big;
             } else if (mode == 3) {
                 /* FP32 MFMA x.q with the norm seed: unrounded inputs, the
                  * BF16 form's accumulation allowance */
@@ -156,11 +160,13 @@ struct Phase1Args {
     uint32_t *bitmap;
     uint32_t *progpart;   /* optional: passing rows per program and bitmap
                              block, [nprog][nbm] */
-    int nprog, nops, nvals, ncols;
+    const int32_t *hdr;   /* device [nprog, nops, nvals] */
+    int ncols;
     uint32_t nbm;         /* bitmap blocks */
     uint32_t nprep;       /* query prep blocks (first in the grid) */
     const float *q;
     int nq, nqp, dim;
+    int dimb;             /* row length of qbf (the BF16 copy's padded dims) */
     uint16_t *qbf;
     float *qnorm, *eps;
     float xmax_norm, vmin, vmax;
@@ -181,8 +187,8 @@ __global__ __launch_bounds__(256) void k_phase1(Phase1Args A) {
     /* the query prep blocks come first: dispatched before the bitmap
      * blocks, they never wait for a free slot behind them */
     if (blockIdx.x >= A.nprep) {
-        bitmap_rows(blockIdx.x - A.nprep, A.nprog, A.nops, A.ops, A.poff, A.plen, A.vals,
-                    A.nvals, A.cols, A.count, A.ncols, A.words, A.bitmap, A.progpart, A.nbm);
+        bitmap_rows(blockIdx.x - A.nprep, A.hdr, A.ops, A.poff, A.plen, A.vals, A.cols,
+                    A.count, A.ncols, A.words, A.bitmap, A.progpart, A.nbm);
         return;
     }
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
@@ -190,7 +196,7 @@ __global__ __launch_bounds__(256) void k_phase1(Phase1Args A) {
         /* query prep only: one wave per padded query */
         const int qi = (int)blockIdx.x * 4 + w;
         if (qi >= A.nqp) return;
-        prep_query(qi, A.q, A.nq, A.dim, A.qbf, A.qnorm, A.eps, A.xmax_norm, A.vmin, A.vmax,
+        prep_query(qi, A.q, A.nq, A.dim, A.dimb, A.qbf, A.qnorm, A.eps, A.xmax_norm, A.vmin, A.vmax,
                    A.all_int, A.mode, A.qi8, A.qn8, A.centre, A.status);
         return;
     }
@@ -202,7 +208,7 @@ __global__ __launch_bounds__(256) void k_phase1(Phase1Args A) {
     const int qi = (int)blockIdx.x;
     if (qi >= A.nqp) return;
     if (w == 0)
-        prep_query(qi, A.q, A.nq, A.dim, A.qbf, A.qnorm, A.eps, A.xmax_norm, A.vmin, A.vmax,
+        prep_query(qi, A.q, A.nq, A.dim, A.dimb, A.qbf, A.qnorm, A.eps, A.xmax_norm, A.vmin, A.vmax,
                    A.all_int, A.mode, A.qi8, A.qn8, A.centre, A.status);
     if (qi >= A.nq) return; /* block-uniform */
     __shared__ float xq[256];
@@ -252,6 +258,8 @@ struct ScanArgs {
     const float *xnorm;
     uint64_t count, count_pad;
     int dim;
+    int dimb;                 /* BF16 scan: dims of the tiled copy and of qbf
+                                 rows (dim rounded up to a multiple of 128) */
     const uint16_t *qbf;
     const float *qf;
     const float *qnorm;
@@ -804,13 +812,14 @@ __device__ __forceinline__ void load_query_state(const ScanArgs &A, int qbase,
  * flight at two waves per SIMD). Loads are unconditional (indices clamped
  * to the last unit) so the pipeline is one straight line for the
  * wait-count analysis. B operands (the 64 queries) sit in LDS.
- * Requires dim % 128 == 0.
+ * Runs over the tiled copy's dimb dims (dim rounded up to a multiple of
+ * 128, zero-filled in rows and queries).
  */
 template <bool PREPASS, int NW, bool INL = false>
 __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(2)))
 void k_scan_bf16(ScanArgs A) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
-    const int S = A.dim / 16;            /* MFMA k-steps per tile */
+    const int S = A.dimb / 16;           /* MFMA k-steps per tile */
     const uint32_t NCH = (uint32_t)(S / 8); /* units per tile */
     u32x4 *qsh = (u32x4 *)smem;          /* [2][S][64] x 16 B */
     CandLds &C = *(CandLds *)(smem + (size_t)2 * S * 64 * 16);
@@ -827,7 +836,7 @@ void k_scan_bf16(ScanArgs A) {
     const uint32_t tlast = cursor_last(cur, A.ntiles);
     auto phys = [&](uint32_t t) { return min(t, tlast) * A.stride; };
     auto load_rows = [&](u32x4 (&a)[8]) {
-        const u32x4 *p = vb + (uint64_t)phys(cur.tile) * (4 * A.dim) + (uint64_t)cur.c * 512 + l;
+        const u32x4 *p = vb + (uint64_t)phys(cur.tile) * (4 * A.dimb) + (uint64_t)cur.c * 512 + l;
 #pragma unroll
         for (int j = 0; j < 8; j++) a[j] = p[64 * j];
     };
@@ -836,11 +845,11 @@ void k_scan_bf16(ScanArgs A) {
     u32x4 a0[8], a1[8];
     load_rows(a0);
 
-    const int half = A.dim / 2;
+    const int half = A.dimb / 2;
     for (int e = tid; e < 2 * S * 64; e += blockDim.x) {
         const int lane = e & 63, s = (e >> 6) % S, qb = (e >> 6) / S;
         const int q = qbase + (lane & 31) + 32 * qb;
-        qsh[e] = *(const u32x4 *)(A.qbf + (uint64_t)q * A.dim + (lane >> 5) * half + 8 * s);
+        qsh[e] = *(const u32x4 *)(A.qbf + (uint64_t)q * A.dimb + (lane >> 5) * half + 8 * s);
     }
     __shared__ int s_im[INL ? kQT : 1];
     __shared__ double s_iv[INL ? kQT : 1];
@@ -904,7 +913,7 @@ void k_scan_bf16(ScanArgs A) {
 }
 
 /*
- * Multi-query-tile variant of the BF16 scan for dim 128 (one 8 KB unit per
+ * Multi-query-tile variant of the BF16 scan for dimb 128 (one 8 KB unit per
  * tile): a WG serves QTW query tiles, so every data tile loaded from HBM
  * feeds QTW x 16 MFMAs and QTW epilogues instead of being re-read by QTW
  * WGs. Query operands of all QTW tiles and one candidate area per query
@@ -1752,7 +1761,8 @@ int stage_batch(FvGpuIndex *g, FvGpuBatch *b, int nq, const float *queries,
                 const int32_t *children, int nchildren, const double *invals,
                 int ninvals, const FvGpuSpan *spans, int nspans_total,
                 const uint64_t *spanoff, int nspanoff, uint32_t topb,
-                char *errbuf, size_t errlen, const uint32_t *topb_q, int max_nq) {
+                char *errbuf, size_t errlen, const uint32_t *topb_q, int max_nq,
+                bool keep_layout) {
     if (nq < 1 || nq > max_nq || topb == 0 || topb > FV_GPU_MAX_TOPB ||
         nspans_total > FV_GPU_MAX_SPANS * nq)
         PV_FAIL("gpu: bad brute batch (nq=%d topb=%u)", nq, topb);
@@ -1825,41 +1835,74 @@ int stage_batch(FvGpuIndex *g, FvGpuBatch *b, int nq, const float *queries,
         else
             b->inl_col = op.col;
     }
+    if (!b->inl_ok) b->inl_col = -1;
     b->has_topbq = topb_q != nullptr;
     const int dim = g->dim;
 
-    /* packed staging layout */
-    size_t off = 0;
-    const size_t o_q = off; off = al256(off + (size_t)nqp * dim * 4);
-    const size_t o_qd = off; off = al256(off + (size_t)nq * sizeof(FvGpuQueryDesc));
-    const size_t o_ops = off; off = al256(off + ps.ops.size() * sizeof(PvOp));
-    const size_t o_vals = off; off = al256(off + ps.vals.size() * 8);
-    const size_t o_poff = off; off = al256(off + ps.off.size() * 4);
-    const size_t o_plen = off; off = al256(off + ps.len.size() * 4);
-    const size_t o_pofq = off; off = al256(off + (size_t)nqp * 4);
-    const size_t o_sp = off; off = al256(off + (size_t)std::max(nspans_total, 0) * sizeof(FvGpuSpan));
-    const size_t o_so = off; off = al256(off + (size_t)std::max(nspanoff, 0) * 8);
-    const size_t o_inl = off; off = al256(off + (b->inl_ok ? (size_t)nqp * sizeof(InlPred) : 0));
-    const size_t o_tq = off; off = al256(off + (topb_q ? (size_t)nq * 4 : 0));
-    const size_t total_bytes = off;
+    /* packed staging layout: the regions sized by the query count first,
+     * then the predicate and span regions at capacities of at least twice
+     * their size (powers of two); a restage keeps the batch's layout when
+     * its contents fit, so the device pointers stay the same */
+    const size_t n_ops = ps.ops.size(), n_vals = ps.vals.size(), n_prog = ps.off.size();
+    const size_t n_sp = (size_t)std::max(nspans_total, 0), n_so = (size_t)std::max(nspanoff, 0);
+    StageLayout &L = b->lay;
+    const bool fits = keep_layout && L.nq == nq && L.tq == (topb_q != nullptr) &&
+                      n_ops <= L.c_ops && n_vals <= L.c_vals && n_prog <= L.c_prog &&
+                      n_sp <= L.c_sp && n_so <= L.c_so;
+    if (!fits) {
+        auto cap = [](size_t n) {
+            size_t c = 16;
+            while (c < 2 * n) c *= 2;
+            return c;
+        };
+        L = StageLayout();
+        L.nq = nq;
+        L.tq = topb_q != nullptr;
+        L.c_ops = cap(n_ops);
+        L.c_vals = cap(n_vals);
+        L.c_prog = cap(n_prog);
+        L.c_sp = cap(n_sp);
+        L.c_so = cap(n_so);
+        size_t off = 0;
+        L.o_q = off; off = al256(off + (size_t)nqp * dim * 4);
+        L.o_qd = off; off = al256(off + (size_t)nq * sizeof(FvGpuQueryDesc));
+        L.o_hdr = off; off = al256(off + 3 * sizeof(int32_t));
+        L.o_pofq = off; off = al256(off + (size_t)nqp * 4);
+        L.o_inl = off; off = al256(off + (size_t)nqp * sizeof(InlPred));
+        L.o_tq = off; off = al256(off + (topb_q ? (size_t)nq * 4 : 0));
+        L.o_ops = off; off = al256(off + L.c_ops * sizeof(PvOp));
+        L.o_vals = off; off = al256(off + L.c_vals * 8);
+        L.o_poff = off; off = al256(off + L.c_prog * 4);
+        L.o_plen = off; off = al256(off + L.c_prog * 4);
+        L.o_sp = off; off = al256(off + L.c_sp * sizeof(FvGpuSpan));
+        L.o_so = off; off = al256(off + L.c_so * 8);
+        L.total = off;
+    }
+    const size_t total_bytes = L.total;
     PV_TRY(b->h_stage.ensure(total_bytes));
     PV_TRY(b->d_stage.ensure(total_bytes));
     unsigned char *h = b->h_stage.as<unsigned char>();
-    memset(h + o_q, 0, (size_t)nqp * dim * 4);
-    memcpy(h + o_q, queries, (size_t)nq * dim * 4);
-    memcpy(h + o_qd, qd, (size_t)nq * sizeof(FvGpuQueryDesc));
-    if (!ps.ops.empty()) memcpy(h + o_ops, ps.ops.data(), ps.ops.size() * sizeof(PvOp));
-    if (!ps.vals.empty()) memcpy(h + o_vals, ps.vals.data(), ps.vals.size() * 8);
-    if (!ps.off.empty()) memcpy(h + o_poff, ps.off.data(), ps.off.size() * 4);
-    if (!ps.len.empty()) memcpy(h + o_plen, ps.len.data(), ps.len.size() * 4);
+    memcpy(h + L.o_q, queries, (size_t)nq * dim * 4);
+    memset(h + L.o_q + (size_t)nq * dim * 4, 0, (size_t)(nqp - nq) * dim * 4);
+    memcpy(h + L.o_qd, qd, (size_t)nq * sizeof(FvGpuQueryDesc));
     {
-        int32_t *pofq = (int32_t *)(h + o_pofq);
+        int32_t *hdr = (int32_t *)(h + L.o_hdr);
+        hdr[0] = b->nprog;
+        hdr[1] = b->nops;
+        hdr[2] = b->nvals;
+    }
+    if (n_ops) memcpy(h + L.o_ops, ps.ops.data(), n_ops * sizeof(PvOp));
+    if (n_vals) memcpy(h + L.o_vals, ps.vals.data(), n_vals * 8);
+    if (n_prog) memcpy(h + L.o_poff, ps.off.data(), n_prog * 4);
+    if (n_prog) memcpy(h + L.o_plen, ps.len.data(), n_prog * 4);
+    {
+        int32_t *pofq = (int32_t *)(h + L.o_pofq);
         for (int q = 0; q < nqp; q++) pofq[q] = q < nq ? ps.of_query[q] : -1;
     }
-    if (nspans_total > 0) memcpy(h + o_sp, spans, (size_t)nspans_total * sizeof(FvGpuSpan));
-    if (nspanoff > 0) memcpy(h + o_so, spanoff, (size_t)nspanoff * 8);
+    if (n_sp) memcpy(h + L.o_sp, spans, n_sp * sizeof(FvGpuSpan));
+    if (n_so) memcpy(h + L.o_so, spanoff, n_so * 8);
     if (b->inl_ok) {
-        InlPred *ip = (InlPred *)(h + o_inl);
+        InlPred *ip = (InlPred *)(h + L.o_inl);
         for (int q = 0; q < nqp; q++) {
             const int p = q < nq ? ps.of_query[q] : -1;
             ip[q].col = -1;
@@ -1873,22 +1916,23 @@ int stage_batch(FvGpuIndex *g, FvGpuBatch *b, int nq, const float *queries,
             }
         }
     }
-    if (topb_q) memcpy(h + o_tq, topb_q, (size_t)nq * 4);
+    if (topb_q) memcpy(h + L.o_tq, topb_q, (size_t)nq * 4);
     PV_TRY(hipMemcpyAsync(b->d_stage.p, h, total_bytes, hipMemcpyHostToDevice, b->stream));
     unsigned char *d = b->d_stage.as<unsigned char>();
-    b->d_q = (float *)(d + o_q);
-    b->d_qd = (FvGpuQueryDesc *)(d + o_qd);
-    b->d_ops = (PvOp *)(d + o_ops);
-    b->d_vals = (double *)(d + o_vals);
-    b->d_poff = (int32_t *)(d + o_poff);
-    b->d_plen = (int32_t *)(d + o_plen);
-    b->d_pofq = (int32_t *)(d + o_pofq);
-    b->d_spans = (FvGpuSpan *)(d + o_sp);
-    b->d_spanoff = (uint64_t *)(d + o_so);
-    b->d_inl_p = b->inl_ok ? (const InlPred *)(d + o_inl) : nullptr;
-    b->d_topbq_p = topb_q ? (const uint32_t *)(d + o_tq) : nullptr;
+    b->d_q = (float *)(d + L.o_q);
+    b->d_qd = (FvGpuQueryDesc *)(d + L.o_qd);
+    b->d_hdr = (const int32_t *)(d + L.o_hdr);
+    b->d_ops = (PvOp *)(d + L.o_ops);
+    b->d_vals = (double *)(d + L.o_vals);
+    b->d_poff = (int32_t *)(d + L.o_poff);
+    b->d_plen = (int32_t *)(d + L.o_plen);
+    b->d_pofq = (int32_t *)(d + L.o_pofq);
+    b->d_spans = (FvGpuSpan *)(d + L.o_sp);
+    b->d_spanoff = (uint64_t *)(d + L.o_so);
+    b->d_inl_p = b->inl_ok ? (const InlPred *)(d + L.o_inl) : nullptr;
+    b->d_topbq_p = topb_q ? (const uint32_t *)(d + L.o_tq) : nullptr;
 
-    PV_TRY(b->d_qbf.ensure((size_t)nqp * dim * 2));
+    PV_TRY(b->d_qbf.ensure((size_t)nqp * qbf_dim(g) * 2));
     PV_TRY(b->d_qaux.ensure((size_t)nqp * 16));
     /* INT8 scan eligibility: every query value an integer inside the
      * index's INT8 window [centre - 128, centre + 127] */
@@ -1904,8 +1948,11 @@ int stage_batch(FvGpuIndex *g, FvGpuBatch *b, int nq, const float *queries,
         PV_TRY(b->d_qi8.ensure((size_t)nqp * dim));
         PV_TRY(b->d_qn8.ensure((size_t)nqp * 4));
     }
+    /* bitmaps and per-program counts for the layout's program capacity
+     * (bitmaps: at most kBmProgs), so a restaged batch keeps the buffers */
+    const size_t pcap = b->arena_bound ? (size_t)b->nprog : std::min<size_t>(L.c_prog, kBmProgs);
     if (b->nprog > 0 && b->nprog <= kBmProgs)
-        PV_TRY(b->d_bitmap.ensure((size_t)b->nprog * (g->count_pad / 32) * 4));
+        PV_TRY(b->d_bitmap.ensure(std::max<size_t>(pcap, b->nprog) * (g->count_pad / 32) * 4));
     b->out_stride = topb;
     PV_TRY(b->d_out.ensure((size_t)nq * topb * sizeof(FvHit)));
     PV_TRY(b->d_cnt.ensure((size_t)nq * 4));
@@ -1913,7 +1960,8 @@ int stage_batch(FvGpuIndex *g, FvGpuBatch *b, int nq, const float *queries,
     PV_TRY(b->d_pcnt.ensure((size_t)nqp * kPoolShards * kPcntStride * 4));
     /* per-program, per-bitmap-block passing-row counts of routed runs
      * (every run's bitmap pass writes all of them) */
-    PV_TRY(b->d_progcnt.ensure((size_t)std::max(1, b->nprog) * bitmap_blocks(g) * 4));
+    PV_TRY(b->d_progcnt.ensure(std::max<size_t>({(size_t)1, pcap, (size_t)b->nprog}) *
+                               bitmap_blocks(g) * 4));
     return 0;
 }
 
@@ -1929,7 +1977,7 @@ int scan_precision(const FvGpuIndex *g, const FvGpuBatch *b, const FvGpuBruteOpt
     if (!asked && p == FV_GPU_PREC_FP32) p = FV_GPU_PREC_BF16;
     if (p == FV_GPU_PREC_SQ8 && !(g->d_vi8 != nullptr && b != nullptr && b->i8_ok))
         p = FV_GPU_PREC_BF16;
-    if (p == FV_GPU_PREC_BF16 && (g->d_vbf == nullptr || g->dim % 128 != 0 || g->dim > 1024))
+    if (p == FV_GPU_PREC_BF16 && (g->d_vbf == nullptr || g->dimb % 128 != 0 || g->dimb > 1024))
         p = FV_GPU_PREC_FP32;
     if (p == FV_GPU_PREC_FP32 && (g->dim % 8 != 0 || g->dim > 512)) p = FV_GPU_PREC_FP32_VALU;
     if (p != FV_GPU_PREC_FP32 && p != FV_GPU_PREC_SQ8 && p != FV_GPU_PREC_BF16)
@@ -2002,15 +2050,14 @@ hipError_t launch_phase1(FvGpuIndex *g, FvGpuBatch *b, int prec, bool bitmap, hi
         P.words = g->count_pad / 32;
         P.bitmap = b->d_bitmap.as<uint32_t>();
         P.progpart = progpart;
-        P.nprog = b->nprog;
-        P.nops = b->nops;
-        P.nvals = b->nvals;
+        P.hdr = b->d_hdr;
         P.ncols = g->ncols;
         P.nbm = (bitmap && b->nprog > 0) ? bitmap_blocks(g) : 0;
         P.q = b->d_q;
         P.nq = b->nq;
         P.nqp = b->nqt * kQT;
         P.dim = g->dim;
+        P.dimb = qbf_dim(g);
         P.qbf = b->d_qbf.as<uint16_t>();
         P.qnorm = qa.qnorm;
         P.eps = qa.eps;
@@ -2079,7 +2126,7 @@ static ScanShape scan_shape(const FvGpuIndex *g, const FvGpuBatch *b, const FvGp
      * batches at every WG count and tile order (1024 queries at 1M x 128:
      * 0.35-0.39 ms against 0.43-0.45 ms with four tiles per 8-wave WG) */
     int qtw = (o && o->qtw > 0) ? o->qtw : (b->nqt >= 2 ? 2 : 1);
-    if (prec != FV_GPU_PREC_BF16 || dim != 128 || b->nqt < 2) qtw = 1;
+    if (prec != FV_GPU_PREC_BF16 || g->dimb != 128 || b->nqt < 2) qtw = 1;
     qtw = qtw >= 4 ? 4 : (qtw >= 2 ? 2 : 1);
     sh.qtw = qtw;
     sh.gy = (b->nqt + qtw - 1) / qtw;
@@ -2205,6 +2252,7 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
                      o->inline_pred && b->inl_ok &&
                      b->inl_col >= 0 && !have_phase1;
     if (route == 1 && !inl && require_bitmaps(b, errbuf, errlen) != 0) return -1;
+    b->ran_bitmap = route == 1 && !inl;
 
     ev_record(g, 0);
     if (!have_phase1) PV_TRY(launch_phase1(g, b, prec, route == 1 && !inl, s, nullptr));
@@ -2284,6 +2332,7 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
     A.count = g->count;
     A.count_pad = g->count_pad;
     A.dim = dim;
+    A.dimb = qbf_dim(g);
     A.qbf = b->d_qbf.as<uint16_t>();
     A.qf = b->d_q;
     A.qnorm = qa.qnorm;
@@ -2320,7 +2369,7 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
 
     /* 4 waves per WG up to 256 dims (2 WGs per CU), 8 waves above (the
      * query operands then take most of the LDS) */
-    const size_t shm_bf = (size_t)2 * (dim / 16) * 64 * 16 + sizeof(CandLds);
+    const size_t shm_bf = (size_t)2 * (A.dimb / 16) * 64 * 16 + sizeof(CandLds);
     if (bf && shm_bf > 64 * 1024) {
         for (const void *f : {(const void *)k_scan_bf16<true, 8, false>,
                               (const void *)k_scan_bf16<false, 8, false>,
@@ -2411,7 +2460,8 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
     PV_TRY(hipGetLastError());
     ev_record(g, 5);
     st.scan_rows = g->count;
-    st.scan_bytes = g->count * (uint64_t)dim * (i8 ? 1 : (bf ? 2 : 4)) * (uint64_t)b->nqt;
+    st.scan_bytes = g->count * (bf ? (uint64_t)A.dimb * 2 : (uint64_t)dim * (i8 ? 1 : 4)) *
+                    (uint64_t)b->nqt;
     return 0;
 }
 
@@ -2651,18 +2701,30 @@ extern "C" FvGpuBatch *fv_gpu_batch_stage_ex(
  * of the batch and the sizes that shape the launches. A graph search or a
  * routed run on the same batch may grow (reallocate) the output buffer or
  * another one; a replay after that would write through a freed pointer,
- * so a changed signature discards the capture. */
+ * so a changed signature discards the capture. The program counts are not
+ * part of it (the kernels read them from the staging area), nor are the
+ * spans unless the captured run took the gather route, whose grid and pool
+ * follow the longest enumeration. */
 static std::vector<uint64_t> capture_signature(const FvGpuBatch *b) {
+    const bool gather = b->last_route == 2;
     return {(uint64_t)(uintptr_t)b->d_stage.p, (uint64_t)(uintptr_t)b->d_qbf.p,
             (uint64_t)(uintptr_t)b->d_qaux.p, (uint64_t)(uintptr_t)b->d_bitmap.p,
             (uint64_t)(uintptr_t)b->d_gmin.p, (uint64_t)(uintptr_t)b->d_pool.p,
             (uint64_t)(uintptr_t)b->d_pcnt.p, (uint64_t)(uintptr_t)b->d_out.p,
             (uint64_t)(uintptr_t)b->d_cnt.p, (uint64_t)(uintptr_t)b->d_flag.p,
             (uint64_t)(uintptr_t)b->d_qi8.p, (uint64_t)(uintptr_t)b->d_qn8.p,
-            (uint64_t)b->nq, (uint64_t)b->topb, (uint64_t)b->nprog, (uint64_t)b->nops,
-            (uint64_t)b->nvals, b->cap_pool, (uint64_t)b->ngroups, (uint64_t)b->stride,
+            (uint64_t)(uintptr_t)b->d_progcnt.p, (uint64_t)(uintptr_t)b->d_hdr,
+            (uint64_t)(uintptr_t)b->d_ops, (uint64_t)(uintptr_t)b->d_vals,
+            (uint64_t)(uintptr_t)b->d_poff, (uint64_t)(uintptr_t)b->d_plen,
+            (uint64_t)(uintptr_t)b->d_pofq, (uint64_t)(uintptr_t)b->d_spans,
+            (uint64_t)(uintptr_t)b->d_spanoff, (uint64_t)(uintptr_t)b->d_q,
+            (uint64_t)(uintptr_t)b->d_qd,
+            (uint64_t)b->nq, (uint64_t)b->topb, (uint64_t)(b->nprog > 0),
+            b->cap_pool, (uint64_t)b->ngroups, (uint64_t)b->stride,
             (uint64_t)b->inl_ok, (uint64_t)(int64_t)b->inl_col, (uint64_t)b->i8_ok,
-            (uint64_t)(uintptr_t)b->d_inl_p, (uint64_t)(uintptr_t)b->d_topbq_p};
+            (uint64_t)(uintptr_t)b->d_inl_p, (uint64_t)(uintptr_t)b->d_topbq_p,
+            (uint64_t)b->last_route, gather ? b->max_total : 0,
+            (uint64_t)(gather && b->all_spans), (uint64_t)(gather && b->any_dedup)};
 }
 
 static void drop_capture(FvGpuBatch *b) {
@@ -2673,12 +2735,21 @@ static void drop_capture(FvGpuBatch *b) {
     b->gsig.clear();
 }
 
+/* the capture still fits the batch as staged now (same buffers and launch
+ * sizes; bitmaps, if the captured run builds them, hold every program) */
+static bool capture_valid(const FvGpuBatch *b) {
+    if (b->gexec == nullptr || b->gsig != capture_signature(b)) return false;
+    if (b->graph_bitmap && b->nprog > kBmProgs) return false;
+    return true;
+}
+
 /* run through a hipGraph captured for this batch and these options: the
  * first call runs eagerly (allocating every buffer), the second captures,
- * later calls replay. Timing events are not recorded in graph mode. */
+ * later calls replay — also after fv_gpu_batch_restage wrote a new batch
+ * into the same buffers. Timing events are not recorded in graph mode. */
 static int run_brute_graph(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
                            char *errbuf, size_t errlen) {
-    if (b->gexec && b->gsig != capture_signature(b)) {
+    if (b->gexec && !capture_valid(b)) {
         /* a buffer the capture points at was reallocated (or the batch
          * changed shape): run eagerly once more, then capture again */
         drop_capture(b);
@@ -2709,12 +2780,46 @@ static int run_brute_graph(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o
         PV_TRY(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
         b->gopts = *o;
         b->gsig = capture_signature(b);
+        b->graph_bitmap = b->ran_bitmap;
     }
     /* the replay's select writes topb hits per query (a graph search in
      * between may have left the graph's stride) */
     b->out_stride = b->topb;
     PV_TRY(hipGraphLaunch(b->gexec, b->stream));
     return 0;
+}
+
+/* Restage: a new set of queries, predicates and spans written into an
+ * existing batch. Waits for the batch's pending work, then stages through
+ * its own host and device buffers, keeping the staging layout when the new
+ * contents fit it. Returns 1 when the batch's captured run stays valid (a
+ * graph run replays it as is), 0 when none is captured or the next graph
+ * run captures again, -1 on error (the batch is then unusable until staged
+ * again successfully). */
+extern "C" int fv_gpu_batch_restage(
+    FvGpuIndex *g, FvGpuBatch *b, int nq, const float *queries, const FvGpuQueryDesc *qd,
+    const uint8_t *dedup, const FvGpuPredNode *nodes, int nnodes, const int32_t *children,
+    int nchildren, const double *invals, int ninvals, const FvGpuSpan *spans,
+    int nspans_total, const uint64_t *spanoff, int nspanoff, const uint32_t *topb_q,
+    uint32_t topb, char *errbuf, size_t errlen) {
+    if (g == nullptr || b == nullptr || b->owner != g) PV_FAIL("gpu: bad batch");
+    try {
+        PV_TRY(hipSetDevice(g->device));
+        PV_TRY(hipStreamSynchronize(b->stream));
+        if (b->aux_stream) PV_TRY(hipStreamSynchronize(b->aux_stream));
+        if (stage_batch(g, b, nq, queries, qd, dedup, nodes, nnodes, children, nchildren, invals,
+                        ninvals, spans, nspans_total, spanoff, nspanoff, topb, errbuf, errlen,
+                        topb_q, kMaxBatchEx, true) != 0) {
+            drop_capture(b);
+            b->warmed = false;
+            return -1;
+        }
+        PV_TRY(hipStreamSynchronize(b->stream));
+        return capture_valid(b) ? 1 : 0;
+    } catch (...) {
+        snprintf(errbuf, errlen, "gpu: internal error");
+        return -1;
+    }
 }
 
 extern "C" int fv_gpu_batch_run_brute(FvGpuIndex *g, FvGpuBatch *b,
@@ -2784,7 +2889,7 @@ static int tune_batch(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *base, 
     *best_us = t0;
     *ntried = 1;
     if (b->last_route != 1) return 0;   /* gather route: no scan shape to tune */
-    const bool mq = prec == FV_GPU_PREC_BF16 && g->dim == 128 && b->nqt >= 2;
+    const bool mq = prec == FV_GPU_PREC_BF16 && g->dimb == 128 && b->nqt >= 2;
     const int qtws[3] = {1, 2, 4};
     const bool wv = (prec == FV_GPU_PREC_BF16 || prec == FV_GPU_PREC_SQ8) && g->dim <= 256;
     for (int qi = 0; qi < (mq ? 3 : 1); qi++)

@@ -335,11 +335,14 @@ __device__ __forceinline__ uint32_t bm_in(const double *colp, const double *inv,
     return bm_stage(m, stg, lane, full, r0, count, counting);
 }
 
-__device__ __forceinline__ void bitmap_rows(uint64_t block, int nprog, int nops,
+/* The program, op and IN-value counts come from device memory (hdr =
+ * [nprog, nops, nvals] in the batch's staging area), so a captured launch
+ * serves any batch restaged into the same buffers. */
+__device__ __forceinline__ void bitmap_rows(uint64_t block, const int32_t *__restrict__ hdr,
                                             const PvOp *__restrict__ ops,
                                             const int32_t *__restrict__ poff,
                                             const int32_t *__restrict__ plen,
-                                            const double *__restrict__ vals, int nvals,
+                                            const double *__restrict__ vals,
                                             const double *__restrict__ cols, uint64_t count,
                                             int ncols, uint64_t words,
                                             uint32_t *__restrict__ bitmap,
@@ -352,6 +355,9 @@ __device__ __forceinline__ void bitmap_rows(uint64_t block, int nprog, int nops,
     __shared__ uint4 s_stage[4][kBmGroup][kBmSlots / 2];
     __shared__ uint32_t s_pc[4][kBmGroup];
     const int t = threadIdx.x, lane = t & 63;
+    const int nprog = __builtin_amdgcn_readfirstlane(hdr[0]);
+    const int nops = __builtin_amdgcn_readfirstlane(hdr[1]);
+    const int nvals = __builtin_amdgcn_readfirstlane(hdr[2]);
     /* wave index as a scalar: the program loop below is wave-uniform */
     const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
     const uint64_t r0 = block * kBmRows;

@@ -65,9 +65,10 @@ int pick_device(size_t need) {
 
 /* fp32 -> MFMA-tiled bf16 copy and ||x||^2, plus the largest positive and
  * negative coordinate, max ||x||^2 and an "all integers" flag. One wave per
- * row. */
+ * row. The copy has dimb >= dim dims (a multiple of 128); dims past dim
+ * hold zeros, which add exactly nothing to a dot product. */
 __global__ __launch_bounds__(256) void k_derive_rows(
-    const float *__restrict__ v, uint64_t count, uint64_t count_pad, int dim,
+    const float *__restrict__ v, uint64_t count, uint64_t count_pad, int dim, int dimb,
     uint16_t *__restrict__ vbf, float *__restrict__ norm,
     unsigned int *__restrict__ stats /* [0]=max|x| bits, [1]=max||x||^2 bits, [2]=nonint */) {
     const int l = threadIdx.x & 63;
@@ -76,11 +77,12 @@ __global__ __launch_bounds__(256) void k_derive_rows(
     float ss = 0.f, pmax = 0.f, nmax = 0.f;
     int nonint = 0;
     /* MFMA-tiled BF16 position of (row, j): tile row/32, k-step
-     * (j mod dim/2)/8, lane (row mod 32) + 32 (j div dim/2), element j mod 8 */
-    const uint64_t tbase = (row >> 5) * 32ull * dim;
-    const int half = dim / 2;
-    for (int j = l; j < dim; j += 64) {
-        float x = row < count ? v[row * dim + j] : 0.f;
+     * (j mod dimb/2)/8, lane (row mod 32) + 32 (j div dimb/2), element j mod 8 */
+    const uint64_t tbase = (row >> 5) * 32ull * dimb;
+    const int half = dimb / 2;
+    const int jend = vbf ? dimb : dim;
+    for (int j = l; j < jend; j += 64) {
+        float x = (row < count && j < dim) ? v[row * dim + j] : 0.f;
         if (vbf) {
             const int g = j / half, w = j % half;
             vbf[tbase + (uint64_t)(w >> 3) * 512 + ((row & 31) + 32 * g) * 8 + (w & 7)] = f2bf(x);
@@ -232,8 +234,11 @@ static FvGpuIndex *upload_impl(const float *vectors, const double *cols,
         return nullptr;
     }
     const uint64_t count_pad = (count + 63) & ~63ull;
-    const bool want_bf = dim % 16 == 0;
-    const size_t need = count_pad * (size_t)dim * (want_bf ? 6 : 4) + count_pad * 4 +
+    /* the BF16 copy, zero-padded to a multiple of 128 dims, serves every
+     * dim up to 1024 */
+    const bool want_bf = dim <= 1024;
+    const int dimb = want_bf ? (dim + 127) / 128 * 128 : 0;
+    const size_t need = count_pad * ((size_t)dim * 4 + (size_t)dimb * 2) + count_pad * 4 +
                         count * (size_t)ncols * 12 + (l0 ? count * (size_t)l0_stride * 8 : 0);
     if (device < 0) device = pick_device(need);
     if (device < 0) {
@@ -247,6 +252,7 @@ static FvGpuIndex *upload_impl(const float *vectors, const double *cols,
     g->count = count;
     g->count_pad = count_pad;
     g->dim = dim;
+    g->dimb = dimb;
     g->ncols = ncols;
     g->l0_stride = l0 ? l0_stride : 0;
     UP_TRY(hipSetDevice(device));
@@ -274,13 +280,13 @@ static FvGpuIndex *upload_impl(const float *vectors, const double *cols,
     UP_TRY(hipMemsetAsync(g->d_vecs, 0, count_pad * (size_t)dim * 4, s));
     UP_TRY(hipMemcpyAsync(g->d_vecs, vectors, count * (size_t)dim * 4,
                           hipMemcpyHostToDevice, s));
-    if (want_bf) UP_TRY(hipMalloc(&g->d_vbf, count_pad * (size_t)dim * 2));
+    if (want_bf) UP_TRY(hipMalloc(&g->d_vbf, count_pad * (size_t)dimb * 2));
     UP_TRY(hipMalloc(&g->d_norm, count_pad * 4));
     unsigned int *d_stats = nullptr;
     UP_TRY(hipMalloc(&d_stats, 16));
     UP_TRY(hipMemsetAsync(d_stats, 0, 16, s));
     hipLaunchKernelGGL(k_derive_rows, dim3((unsigned)((count_pad + 3) / 4)), dim3(256), 0, s,
-                       g->d_vecs, count, count_pad, dim, g->d_vbf, g->d_norm, d_stats);
+                       g->d_vecs, count, count_pad, dim, dimb, g->d_vbf, g->d_norm, d_stats);
     {
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) {

@@ -173,6 +173,20 @@ struct ProgSet {
     std::vector<int32_t> of_query;   /* per query: program index, -1 = none */
 };
 
+/* Packed staging area of a batch: byte offsets of its regions and the
+ * element capacities of the regions whose size depends on the predicates
+ * and spans. A batch restaged with the same query count, per-query-k mode
+ * and contents that fit these capacities keeps every offset, so every
+ * device pointer a captured run baked in stays valid. */
+struct StageLayout {
+    int nq = 0;              /* 0 = no layout yet */
+    bool tq = false;         /* per-query k region present */
+    size_t o_q = 0, o_qd = 0, o_hdr = 0, o_pofq = 0, o_inl = 0, o_tq = 0;
+    size_t o_ops = 0, o_vals = 0, o_poff = 0, o_plen = 0, o_sp = 0, o_so = 0;
+    size_t c_ops = 0, c_vals = 0, c_prog = 0, c_sp = 0, c_so = 0;
+    size_t total = 0;
+};
+
 } // namespace pv
 
 /* A staged batch: inputs resident in HBM plus device outputs. */
@@ -204,6 +218,8 @@ struct FvGpuBatch {
     FvGpuBruteOpts gopts = {};
     std::vector<uint64_t> gsig;
     bool warmed = false;    /* one eager run done: every buffer allocated */
+    bool ran_bitmap = false;  /* the last run built predicate bitmaps */
+    bool graph_bitmap = false; /* the captured run builds predicate bitmaps */
     bool arena_bound = false; /* buffers carved from an arena: keep no host records */
     void *last_scan = nullptr; /* launch record of the last main scan (timing) */
     bool capturing = false; /* a hipGraph capture of this batch's run is open */
@@ -217,6 +233,10 @@ struct FvGpuBatch {
     /* packed staging area: one H2D copy per stage */
     pv::PinBuf h_stage;
     pv::DevBuf d_stage;
+    pv::StageLayout lay;
+    const int32_t *d_hdr = nullptr; /* device [nprog, nops, nvals] (staging area):
+                                       kernels read the program counts here, so
+                                       a captured run serves restaged batches */
     float *d_q = nullptr;          /* [nqt*64][dim] fp32 (zero padded) */
     FvGpuQueryDesc *d_qd = nullptr;
     pv::PvOp *d_ops = nullptr;
@@ -268,6 +288,8 @@ struct FvGpuIndex {
     uint64_t count = 0;
     uint64_t count_pad = 0;
     int dim = 0;
+    int dimb = 0;            /* dims of the BF16 tiled copy: dim rounded up to
+                                a multiple of 128, zero-filled (0 = no copy) */
     int ncols = 0;
     uint32_t l0_stride = 0;
     int nbr_stride = 0;
@@ -298,6 +320,7 @@ struct FvGpuIndex {
     static constexpr int kMaxBatchStreams = 8;
     int nstreams = 3;                /* PAVES_BATCH_STREAMS (1..8), default 3 */
     hipStream_t bstreams[kMaxBatchStreams] = {};
+
     int next_stream = 0;
     hipStream_t ev_stream = nullptr; /* stream of the run being timed */
     bool timing = false;
@@ -307,6 +330,12 @@ struct FvGpuIndex {
     FvGpuBatch *sync_batch = nullptr; /* reused by the synchronous ABI */
     FvGpuBatch *last_batch = nullptr; /* batch of the last brute run (stats) */
 };
+
+namespace pv {
+/* row length of a batch's BF16 query operands: the tiled copy's padded
+ * dims, or dim for an index without the copy */
+inline int qbf_dim(const FvGpuIndex *g) { return g->dimb > 0 ? g->dimb : g->dim; }
+} // namespace pv
 
 namespace pv {
 
@@ -342,7 +371,7 @@ int stage_batch(FvGpuIndex *g, FvGpuBatch *b, int nq, const float *queries,
                 int ninvals, const FvGpuSpan *spans, int nspans_total,
                 const uint64_t *spanoff, int nspanoff, uint32_t topb,
                 char *errbuf, size_t errlen, const uint32_t *topb_q = nullptr,
-                int max_nq = FV_GPU_MAX_BATCH);
+                int max_nq = FV_GPU_MAX_BATCH, bool keep_layout = false);
 void free_batch_buffers(FvGpuBatch *b);
 
 /* pv_graph.hip */

@@ -803,6 +803,7 @@ static int setup_view(FvGpuIndex *g, FvGpuBatch *b, char *errbuf, size_t errlen)
     v->i8_ok = b->i8_ok;
     v->stream = b->stream;
     /* the programs and their bitmaps are the parent's */
+    v->d_hdr = b->d_hdr;
     v->d_ops = b->d_ops;
     v->d_vals = b->d_vals;
     v->d_poff = b->d_poff;
@@ -827,7 +828,7 @@ static int setup_view(FvGpuIndex *g, FvGpuBatch *b, char *errbuf, size_t errlen)
     v->d_topbq_p = b->has_topbq ? (const uint32_t *)(d + o_k) : nullptr;
     v->d_vskip = d + o_s;
     PV_TRY(b->d_vmap.ensure((size_t)nqp * 4));
-    PV_TRY(v->d_qbf.ensure((size_t)nqp * dim * 2));
+    PV_TRY(v->d_qbf.ensure((size_t)nqp * qbf_dim(g) * 2));
     PV_TRY(v->d_qaux.ensure((size_t)nqp * 16));
     if (g->d_vi8 != nullptr) {
         PV_TRY(v->d_qi8.ensure((size_t)nqp * dim));

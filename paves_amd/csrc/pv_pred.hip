@@ -184,12 +184,12 @@ int compile_programs(int nq, const FvGpuQueryDesc *qd,
 
 /* All distinct programs over kBmRows rows per block (bitmap_rows). */
 __global__ __launch_bounds__(256) void k_bitmap(
-    const PvOp *__restrict__ ops, const int32_t *__restrict__ poff,
-    const int32_t *__restrict__ plen, int nprog, int nops, const double *__restrict__ vals,
-    int nvals, const double *__restrict__ cols, uint64_t count, int ncols, uint64_t words,
-    uint32_t *__restrict__ bitmap) {
-    bitmap_rows(blockIdx.x, nprog, nops, ops, poff, plen, vals, nvals, cols, count, ncols,
-                words, bitmap, nullptr, 0);
+    const int32_t *__restrict__ hdr, const PvOp *__restrict__ ops,
+    const int32_t *__restrict__ poff, const int32_t *__restrict__ plen,
+    const double *__restrict__ vals, const double *__restrict__ cols, uint64_t count,
+    int ncols, uint64_t words, uint32_t *__restrict__ bitmap) {
+    bitmap_rows(blockIdx.x, hdr, ops, poff, plen, vals, cols, count, ncols, words, bitmap,
+                nullptr, 0);
 }
 
 hipError_t launch_bitmap(FvGpuIndex *g, FvGpuBatch *b, hipStream_t s) {
@@ -198,9 +198,8 @@ hipError_t launch_bitmap(FvGpuIndex *g, FvGpuBatch *b, hipStream_t s) {
     const uint64_t words = g->count_pad / 32;
     const unsigned blocks = bitmap_blocks(g);
     hipLaunchKernelGGL(k_bitmap, dim3(blocks), dim3(256), bitmap_lds_bytes(g->ncols), s,
-                       b->d_ops, b->d_poff,
-                       b->d_plen, b->nprog, b->nops, b->d_vals, b->nvals, g->d_cols, g->count,
-                       g->ncols, words, b->d_bitmap.as<uint32_t>());
+                       b->d_hdr, b->d_ops, b->d_poff, b->d_plen, b->d_vals, g->d_cols,
+                       g->count, g->ncols, words, b->d_bitmap.as<uint32_t>());
     return hipGetLastError();
 }
 

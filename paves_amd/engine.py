@@ -437,6 +437,24 @@ class GpuIndex:
             raise GpuError(eb.value.decode())
         return C.c_void_p(h)
 
+    def restage(self, bh, queries: np.ndarray, b: Batch, topb: int,
+                k_per_query: np.ndarray | None = None) -> bool:
+        """Write a new batch into staged batch `bh` (its buffers, after its
+        pending work). True when its captured run replays for the new
+        batch as is."""
+        q = np.ascontiguousarray(queries, dtype=np.float32)
+        kq = None if k_per_query is None else np.ascontiguousarray(k_per_query, dtype=np.uint32)
+        eb = L.errbuf()
+        rc = self.lib.fv_gpu_batch_restage(self.h, bh, b.nq, ptr(q), ptr(b.qd), ptr(b.dedup),
+                                           ptr(b.nodes), len(b.nodes), ptr(b.children),
+                                           len(b.children), ptr(b.invals), len(b.invals),
+                                           ptr(b.spans), len(b.spans), ptr(b.spanoff),
+                                           len(b.spanoff), None if kq is None else ptr(kq),
+                                           topb, eb, len(eb))
+        if rc < 0:
+            raise GpuError(eb.value.decode())
+        return rc == 1
+
     def run_brute(self, bh, opts: L.BruteOpts | None = None):
         eb = L.errbuf()
         rc = self.lib.fv_gpu_batch_run_brute(self.h, bh, C.byref(opts) if opts else None, eb, len(eb))

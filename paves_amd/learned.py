@@ -219,10 +219,12 @@ def features_of(model, index, queries, sers, batch=1024):
     return np.concatenate(out, 0)
 
 
-def run_mode(index, model, queries, sers, entries, truth, opts, micro=64, reps=1):
+def run_mode(index, model, queries, sers, entries, truth, opts, micro=64, reps=1,
+             groups=None):
     """Routed runs over `queries` in micro-batches (device-resident,
     overlapped on the index's streams); returns (QPS, mean recall@k,
-    graph-routed fraction, mean ef of graph-routed queries)."""
+    graph-routed fraction, mean ef of graph-routed queries). With `groups`
+    (a group id per query) the recall is the lowest of the groups' means."""
     import time
     from . import engine as E
     k = opts.k
@@ -250,8 +252,13 @@ def run_mode(index, model, queries, sers, entries, truth, opts, micro=64, reps=1
                 if r[j]:
                     ng += 1
                     efs.append(int(ef[j]))
-        return (len(queries) / dt, float(np.mean(rec)), ng / len(queries),
-                float(np.mean(efs)) if efs else 0.0)
+        rec = np.asarray(rec)
+        if groups is None:
+            r = float(np.mean(rec))
+        else:
+            gid = np.asarray(groups)
+            r = min(float(np.mean(rec[gid == u])) for u in np.unique(gid))
+        return (len(queries) / dt, r, ng / len(queries), float(np.mean(efs)) if efs else 0.0)
     finally:
         for s, e, bh in staged:
             index.free_batch(bh)
@@ -303,22 +310,29 @@ def train_router(index, graph, base, queries, sers, k=10, target=0.9, epochs=30,
 
 
 def calibrate_bias(index, model, queries, sers, entries, truth, k=10, target=0.9, scale=1.0,
-                   micro=1024, fracs=(0.0, 0.05, 0.1, 0.2, 0.3, 0.5), reps=3):
-    """The route threshold (graph when the route logit exceeds it) that
-    gives the highest measured throughput of routed runs over (queries) at
-    mean recall@k >= target. Candidates send the given fractions of these
-    queries to the graph (0: all brute, the fallback). Returns (bias,
-    table of (fraction, bias, QPS, recall, graph-routed fraction))."""
+                   micro=1024, fracs=(0.0, 0.05, 0.1, 0.2, 0.3, 0.5), reps=3, groups=None,
+                   scales=None):
+    """The route threshold (graph when the route logit exceeds it) and ef
+    safety factor that give the highest measured throughput of routed runs
+    over (queries) at recall@k >= target — the mean over all queries, or
+    with `groups` (a filter-class id per query) every group's mean.
+    Candidates send the given fractions of these queries to the graph (0:
+    all brute, the fallback, which needs no factor) at each factor in
+    `scales` (default: `scale` alone). Returns (bias, scale, table of
+    (fraction, scale, bias, QPS, recall, graph-routed fraction))."""
     from . import _lib as LL
     y = model.predict(features_of(model, index, queries, sers))
     table, best = [], None
     for f in fracs:
         bias = 1e30 if f == 0.0 else float(np.quantile(y[:, 1], 1.0 - f))
-        qps, rec, gfrac, _ = run_mode(index, model, queries, sers, entries, truth,
-                                      LL.LearnedOpts(k=k, mode=0, ef_scale=scale,
-                                                     route_bias=bias, max_visited=50_000),
-                                      micro=micro, reps=reps)
-        table.append((f, bias, qps, rec, gfrac))
-        if rec >= target and (best is None or qps > best[2]):
-            best = table[-1]
-    return (best[1] if best else 1e30), table
+        for sc in ((scale,) if f == 0.0 or not scales else scales):
+            qps, rec, gfrac, _ = run_mode(index, model, queries, sers, entries, truth,
+                                          LL.LearnedOpts(k=k, mode=0, ef_scale=sc,
+                                                         route_bias=bias, max_visited=50_000),
+                                          micro=micro, reps=reps, groups=groups)
+            table.append((f, sc, bias, qps, rec, gfrac))
+            if rec >= target and (best is None or qps > best[3]):
+                best = table[-1]
+    if best is None:
+        return 1e30, scale, table
+    return best[2], best[1], table
