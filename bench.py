@@ -79,6 +79,8 @@ HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFS = 157.3    # dense FP32 matrix (MFMA) peak of MI355X
 SCAN_KERNEL = "k_scan_bf16<false, 8, false>"   # main scan: 8 waves per WG, bitmap predicates
 COPY_BYTES = 1 << 28   # 256 MiB device copy used to calibrate the PMC byte counters
+GRAPH_KERNEL = "k_graph"   # filtered level-0 beam search, one 4-wave WG per query
+GRAPH_NBR = 32             # neighbour-list slots read per expanded head (M0 = 32)
 
 
 def log(*a):
@@ -149,17 +151,20 @@ def copy_calibration(reps=20):
     return gbs.value
 
 
-def pmc_pass(counter, args):
-    """One rocprofv3 --pmc pass over `bench.py --roofline-only --copy-calib`
-    (a child process); returns {kernel-kind: mean KiB per dispatch}."""
+def pmc_pass(counter, args, child=None, kernel=None):
+    """One rocprofv3 --pmc pass over a child `bench.py` run (default:
+    `--roofline-only --copy-calib`, the scan kernel); returns the median KiB
+    per dispatch of the measured kernel and of the copy kernel."""
     prof = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
     if not os.path.exists(prof):
         return None
     out = tempfile.mkdtemp(prefix="paves_pmc_", dir="/tmp")
+    if child is None:
+        child = ["--roofline-only", "--copy-calib", "--n", str(args.n),
+                 "--nq", str(min(args.nq, 1024)), "--batch", str(args.batch), "--k", str(args.k)]
+    kernel = kernel or SCAN_KERNEL
     cmd = [prof, "--pmc", counter, "-d", out, "-o", "pmc", "--",
-           sys.executable, os.path.abspath(__file__), "--roofline-only", "--copy-calib",
-           "--n", str(args.n), "--nq", str(min(args.nq, 1024)), "--batch", str(args.batch),
-           "--k", str(args.k)]
+           sys.executable, os.path.abspath(__file__)] + child
     env = dict(os.environ, TMPDIR="/tmp")
     try:
         p = subprocess.Popen(cmd, cwd="/tmp", env=env, stdout=subprocess.DEVNULL,
@@ -181,7 +186,7 @@ def pmc_pass(counter, args):
         c.close()
     finally:
         shutil.rmtree(out, ignore_errors=True)
-    scan = [v for _, k, v in rows if SCAN_KERNEL.replace(" ", "") in k.replace(" ", "")]
+    scan = [v for _, k, v in rows if kernel.replace(" ", "") in k.replace(" ", "")]
     copy = [v for _, k, v in rows if "k_copy_calib" in k]
     if not scan or not copy:
         log(f"pmc {counter}: scan {len(scan)} / copy {len(copy)} dispatches found")
@@ -204,6 +209,76 @@ def measure_traffic(args):
             "fetch_scale": round(fs, 4), "write_scale": round(ws, 4),
             "fetch_bytes": int(fetch), "write_bytes": int(write)}
     return fetch + write, info
+
+
+def graph_launch(g, queries, b, ef, mv, dim):
+    """One beam-search launch over a batch: its kernel duration (HIP events
+    around the search kernel) and its algorithmic bytes from the kernel's
+    own counters — one neighbour list (GRAPH_NBR ids) per expanded head,
+    one FP32 vector row per visited node and, for a filtered query, one
+    predicate-bitmap word per visited node."""
+    g.graph(queries, b, ef, mv)          # warm-up (buffers, code)
+    g.set_timing(True)
+    _, _, _, dbg = g.graph(queries, b, ef, mv, debug=True)
+    us = g.stats().t_graph_us
+    g.set_timing(False)
+    heads = dbg[:, 0].astype(np.float64)
+    vis = dbg[:, 1].astype(np.float64)
+    filt = (b.qd["proot"] >= 0).astype(np.float64)
+    alg = float((heads * GRAPH_NBR * 4).sum() + (vis * (dim * 4 + 4 * filt)).sum())
+    gbs = alg / (us * 1e-6) / 1e9
+    return {"kernel": GRAPH_KERNEL, "queries": len(queries), "ef": ef,
+            "heads_expanded": int(heads.sum()), "nodes_visited": int(vis.sum()),
+            "alg_bytes_per_launch": int(alg), "avg_launch_us": round(us, 2),
+            "achieved_GBs": round(gbs, 1), "peak_GBs": HBM_PEAK_GBS,
+            "frac": round(gbs / HBM_PEAK_GBS, 4),
+            "ns_per_visited_node": round(us * 1e3 / max(1.0, float(vis.sum())), 4)}
+
+
+def graph_roofline_only(args):
+    """The beam-search launch of graph_launch over the first 1024 queries of
+    the workload (six classes), the graph and entry points read from
+    --graph-file (written by the parent run) or built here; prints one JSON
+    line."""
+    d = synth.gmm128(args.n, args.nq)
+    cols = synth.columns(d)
+    rows, vals = synth.postings(cols)
+    post = E.Postings(rows, vals)
+    if args.graph_file:
+        z = np.load(args.graph_file)
+    else:
+        f = hnsw.build(d["base"], M=16, ef_construction=200, threads=args.threads)
+        z = {"l0": f["l0"], "ent": hnsw.entries(f, d["base"], d["queries"][:1024],
+                                                threads=args.threads)}
+    nq = len(z["ent"])
+    preds = [synth.filter_pred(synth.FILTER_CLASSES[i % 6], d["qcent"][i]) for i in range(nq)]
+    b = E.concat([E.serialize(p, args.n, post) for p in preds], args.n, entries=z["ent"])
+    g = E.GpuIndex(d["base"], cols, rows, l0=z["l0"], precision=L.FV_GPU_PREC_BF16, device=0)
+    m = graph_launch(g, d["queries"][:nq], b, args.graph_ef, max(200_000, 400 * args.graph_ef),
+                     128)
+    copy_calibration(reps=3)
+    g.close()
+    print(json.dumps(m), flush=True)
+
+
+def graph_traffic(args, graph, ent):
+    """HBM bytes of the beam-search launch: one FETCH_SIZE pass over the
+    child run of graph_roofline_only, scaled by the copy kernel's reading
+    (the kernel writes only its results: ef hits per query)."""
+    path = os.path.join(tempfile.mkdtemp(prefix="paves_graph_", dir="/tmp"), "g.npz")
+    np.savez(path, l0=graph["l0"], ent=ent)
+    try:
+        f = pmc_pass("FETCH_SIZE", args, child=["--graph-roofline-only", "--graph-file", path,
+                                                "--graph-ef", str(args.graph_ef),
+                                                "--n", str(args.n), "--nq", str(args.nq)],
+                     kernel=GRAPH_KERNEL)
+    finally:
+        shutil.rmtree(os.path.dirname(path), ignore_errors=True)
+    if not f:
+        return None
+    fs = COPY_BYTES / 1024.0 / f["copy_kib"]
+    return {"fetch_kib_raw": round(f["scan_kib"], 1), "fetch_scale": round(fs, 4),
+            "fetch_bytes": int(f["scan_kib"] * fs * 1024.0)}
 
 
 def roofline_only(args):
@@ -241,6 +316,8 @@ def main():
     ap.add_argument("--dim", type=int, choices=[128, 768], default=128,
                     help="uneven workload: 1M x 128 or 2M x 768")
     ap.add_argument("--nq", type=int, default=10_000)
+    ap.add_argument("--stream-batches", type=int, default=4,
+                    help="uneven workload: distinct batches restaged through one capture")
     ap.add_argument("--batch", type=int, default=64, help="micro-batch of the brute baselines")
     ap.add_argument("--learned-batch", type=int, default=1024,
                     help="queries per routed run of the learned path (packed brute tiles)")
@@ -258,6 +335,10 @@ def main():
     ap.add_argument("--roofline-only", action="store_true",
                     help="only the isolated scan runs (the rocprofv3 target)")
     ap.add_argument("--copy-calib", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--graph-roofline-only", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--graph-file", default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--graph-ef", type=int, default=80,
+                    help="ef of the beam-search roofline launch (1024 queries, six classes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: measures one GPU; --gpus {args.gpus} is not supported "
@@ -275,6 +356,9 @@ def main():
         return
     if args.roofline_only:
         roofline_only(args)
+        return
+    if args.graph_roofline_only:
+        graph_roofline_only(args)
         return
     learned_on = args.train_queries > 0
 
@@ -597,6 +681,10 @@ def graph_path(args):
                 best = row
         for bh in staged:
             g.free_batch(bh)
+    b0 = build_batches(d, post, args.n, min(nq, 1024), 1024, args.k, ent)[0][2]
+    roof = graph_launch(g, d["queries"][:min(nq, 1024)], b0, args.graph_ef,
+                        max(200_000, 400 * args.graph_ef), 128)
+    log(f"graph kernel roofline: {roof}")
     cpu = None if args.no_cpu else graph_cpu_baseline(d, cols, rows, vals, graph, ent, truth, args)
     g.close()
     res = {
@@ -611,6 +699,8 @@ def graph_path(args):
                    "n": args.n, "dim": 128, "queries": nq, "graph_build_s": round(build_s, 1),
                    "best": best},
         "sweep": sweep,
+        "roofline": {"bound": "hbm", "achieved": roof["achieved_GBs"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": roof["frac"], "traffic": None, "detail": roof},
         "cpu_baseline": cpu,
     }
     print(json.dumps(res), flush=True)
@@ -701,14 +791,24 @@ def uneven_path(args):
     dim = args.dim
     n = args.n if args.n_set else (1_000_000 if dim == 128 else 2_000_000)
     nq = 4096
+    ns = args.stream_batches
     t = time.perf_counter()
-    d = synth.gmm128(n, nq) if dim == 128 else synth.gmm768(n, nq)
+    d = synth.gmm128(n, nq * (1 + ns)) if dim == 128 else synth.gmm768(n, nq * (1 + ns))
+    allq = d["queries"]
+    d["queries"] = allq[:nq]
     cols = synth.columns(d)
     rows, vals = synth.postings(cols)
     post = E.Postings(rows, vals)
     kq, sel, b = uneven_batch(d, n, nq, 20240617 + dim, post)
-    log(f"data: {n}x{dim} + {nq} uneven queries in {time.perf_counter() - t:.1f}s")
+    stream = []
+    for i in range(ns):
+        kqi, _, bi = uneven_batch(d, n, nq, 20240618 + dim + i, post)
+        kqi[0] = 100                  # every batch's largest k: the staged batch's
+        stream.append((allq[(i + 1) * nq:(i + 2) * nq], kqi, bi))
+    log(f"data: {n}x{dim} + {nq} uneven queries (+ {ns} more batches) in "
+        f"{time.perf_counter() - t:.1f}s")
     g = E.GpuIndex(d["base"], cols, rows, precision=L.FV_GPU_PREC_BF16, device=0)
+    kq[0] = 100
     bh = g.stage_ex(d["queries"], b, kq)
     base = L.BruteOpts(precision=L.FV_GPU_PREC_BF16, route=1, inline_pred=1)
     t = time.perf_counter()
@@ -732,6 +832,29 @@ def uneven_path(args):
     qps = nq / dt
     log(f"hipGraph replay: {dt * 1e3:.3f} ms per 4096-query launch -> {qps:,.0f} QPS")
     out, cnt, flag, rc = g.fetch(bh, nq, 100)
+
+    # a stream of distinct batches through the one capture: each is restaged
+    # into the batch (host compile and packing, one H2D copy), replayed and
+    # fetched; the same opts keep the capture
+    restream = None
+    if ns > 0:
+        sb = g.stage_ex(d["queries"], b, kq)
+        for _ in range(2):
+            g.run_brute(sb, opts)
+        g.sync()
+        kept, t0 = [], time.perf_counter()
+        for qs_i, kq_i, b_i in stream:
+            kept.append(g.restage(sb, qs_i, b_i, 100, kq_i))
+            g.run_brute(sb, opts)
+            g.fetch(sb, nq, 100)
+        dts = time.perf_counter() - t0
+        g.free_batch(sb)
+        restream = {"batches": ns, "queries_per_batch": nq, "qps": round(ns * nq / dts, 1),
+                    "ms_per_batch": round(dts / ns * 1e3, 3), "capture_kept": bool(all(kept)),
+                    "note": "per batch: fv_gpu_batch_restage (predicate compile, packing, one "
+                            "H2D copy), one replay of the captured run, fetch of the results"}
+        log(f"restaged stream: {ns} distinct batches, {dts / ns * 1e3:.2f} ms per batch "
+            f"({ns * nq / dts:,.0f} QPS), capture kept: {all(kept)}")
 
     # per-kernel times of one eager run with the same shape (HIP events)
     eager = L.BruteOpts(precision=best.precision, route=best.route, inline_pred=best.inline_pred,
@@ -799,6 +922,7 @@ def uneven_path(args):
                      "avg_launch_us": round(scan_us, 1), "flops_per_launch": flops,
                      "data_passes": gy, "hbm_GBs": round(hbm_gbs, 1)},
         "prepass_tau_us": round(pre_us, 1), "select_us": round(sel_us, 1),
+        "restaged_stream": restream,
         "oracle_sample_ok": sample_ok, "fetch_rc": rc,
         "cpu_baseline": cpu,
     }
@@ -913,16 +1037,34 @@ def learned_path(g, graph, d, post, staged, batches, truth, args, setup):
     fx = learned.run_mode(g, model, fq, fs, fent, truth[:nfx],
                           L.LearnedOpts(k=args.k, mode=2, ef_fixed=400), micro=args.batch)
     log(f"fixed ef=400 graph (FP32): {fx[0]:,.0f} QPS, recall {fx[1]:.4f} ({nfx} queries)")
+    # the beam-search kernel on its roofline: one launch over the first
+    # batch of the workload (six classes) at ef = graph_ef
+    s0, e0, b0 = batches[0]
+    mvg = max(200_000, 400 * args.graph_ef)
+    gk = graph_launch(g, d["queries"][s0:e0], b0, args.graph_ef, mvg, 128)
+    gk["traffic"] = None if args.no_traffic else graph_traffic(args, graph,
+                                                              np.asarray(b0.qd["entry"]))
+    gk["note"] = ("the six filter classes: the launch lasts as long as its slowest search "
+                  "(selective classes visit up to max_visited nodes)")
+    bu = E.concat([E.serialize(None, args.n)] * (e0 - s0), args.n, entries=b0.qd["entry"])
+    gk["unfiltered"] = graph_launch(g, d["queries"][s0:e0], bu, args.graph_ef, mvg, 128)
+    for tag, m in (("six classes", gk), ("unfiltered", gk["unfiltered"])):
+        log(f"graph kernel, {tag} (ef {args.graph_ef}, {e0 - s0} queries): "
+            f"{m['avg_launch_us']:.0f} us, {m['nodes_visited']:,} nodes visited -> "
+            f"{m['achieved_GBs']:.0f} GB/s ({m['ns_per_visited_node']} ns/node)")
+    log(f"graph kernel traffic (six classes): {gk['traffic']}")
     model.close()
     return {"qps": qps, "ms_per_step": dt / args.steps * 1e3, "recall_at_10": float(np.mean(rec)),
             "recall_per_class": {classes[i]: round(float(np.mean(per[i])), 4) for i in range(6)},
             "graph_routed_frac": ng / args.nq, "mean_ef_graph_routed": float(np.mean(efs)) if efs else 0.0,
-            "ef_scale": scale, "route_bias": bias,
+            "ef_scale": scale, "route_bias": bias if np.isfinite(bias) else None,
+            "route_disabled": not np.isfinite(bias),
             "bias_calibration": [{"graph_share_target": t[0], "ef_scale": t[1],
                                   "qps": round(t[3], 1), "worst_class_recall_at_10": round(t[4], 4),
                                   "graph_routed_frac": round(t[5], 4)} for t in btable],
             "fixed_ef400_graph_qps": round(fx[0], 1), "fixed_ef400_graph_recall": round(fx[1], 4),
             "fixed_ef400_sample": nfx, "cpu_hnsw_ef": 80,
+            "graph_kernel": gk,
             "batch": args.learned_batch,
             "note": "each routed run decides every query's route and ef on the GPU, packs the "
                     "brute-routed queries into the leading query tiles (the scan runs only "

@@ -70,12 +70,17 @@ typedef struct FvHit {
 #ifndef PAVES_GPU_ENGINE_H
 typedef struct FvGpuIndex FvGpuIndex;
 
-/* gpu_engine.h:33-34. Every precision returns exact results: ids and
- * FP32 distances bit-identical to the CPU cursors. An FP32 index is
- * scanned in BF16 (FV_GPU_PREC_BF16) unless a call asks for FP32; SQ8 (the
- * reference's reduced-precision request) scans an exact INT8 copy where
- * the data are integers spanning at most 255 values at dim 128 or 256,
- * else BF16; a certified FP32 re-rank makes every scan exact. */
+/* gpu_engine.h:33-34. FP32 and BF16 return exact results: ids and FP32
+ * distances bit-identical to the CPU cursors (a certified FP32 re-rank
+ * makes every scan exact). An FP32 index is scanned in BF16
+ * (FV_GPU_PREC_BF16) unless a call asks for FP32. SQ8 (the reference's
+ * reduced-precision index, quantized scan + re-rank of the candidates):
+ * over integer data spanning at most 255 values in at most 256 dims, an
+ * exact INT8 copy (exact results, as FP32); over any other data up to 1024
+ * dims, int8 codes of the midrange-centred rows under one scale — the scan
+ * ranks by code distance and the top FvGpuBruteOpts.overfetch x B rows are
+ * re-ranked with the exact FP32 distance (approximate, like the
+ * reference's SQ8; returned distances are exact). */
 #define FV_GPU_PREC_FP32 0
 #define FV_GPU_PREC_SQ8 1
 
@@ -176,8 +181,13 @@ int fv_gpu_mem_probe(uint64_t *free_bytes, uint64_t *total_bytes);
 typedef struct FvGpuBruteOpts {
     int32_t precision;   /* FV_GPU_PREC_* to scan with; -1 = index default */
     int32_t route;       /* 0 auto, 1 force full scan, 2 force span gather */
-    uint32_t overfetch;  /* reserved, 0 (the re-rank set is certified: every
-                            candidate within 2 eps of the B-th key) */
+    uint32_t overfetch;  /* quantized SQ8 scans (an SQ8 index over data
+                            outside the exact INT8 window): the scan keeps
+                            the overfetch x B rows nearest in code distance
+                            and all are re-ranked exactly (0 = 4, at most 64,
+                            at most 4096 rows). Exact scans ignore it: their
+                            re-rank set is certified (every candidate within
+                            2 eps of the B-th key) */
     uint32_t sample_stride; /* threshold pre-pass stride = over-fetch factor:
                                about stride x B candidates per query (0 = auto) */
     int32_t inline_pred; /* 1: evaluate one-comparison predicates inside the
@@ -250,8 +260,10 @@ int fv_gpu_brute_batch_ex(FvGpuIndex *g, int nq, const float *queries,
 
 /* fv_gpu_graph_batch with search knobs: wmax = frontier heads expanded per
  * iteration (1 = the CPU cursor's one-at-a-time order), fcap = frontier
- * capacity (64..1024), dbg = optional host array [nq][4] receiving
- * iterations, visited, results and hash-overflow counts per query. */
+ * capacity (64..1024), dbg = optional host array [nq][4] receiving per
+ * query the frontier heads expanded (each reads one neighbour list), the
+ * nodes visited (each reads one vector row), the results, and the live
+ * frontier entries lost (0 while the spill store holds them). */
 int fv_gpu_graph_batch_ex(FvGpuIndex *g, int nq, const float *queries,
                           const FvGpuQueryDesc *qd, const FvGpuPredNode *nodes,
                           int nnodes, const int32_t *children, int nchildren,
@@ -430,7 +442,8 @@ typedef struct FvGpuLearnedOpts {
     uint32_t ef_fixed;     /* mode 2 */
     uint32_t ef_min, ef_max; /* clamp of the predicted ef (ef_min >= k) */
     float ef_scale;        /* safety factor on the predicted ef (0 = 1) */
-    float route_bias;      /* graph when the route logit exceeds it */
+    float route_bias;      /* graph when the route logit exceeds it (+inf: never;
+                              the run is then the brute pipeline alone) */
     uint64_t max_visited;  /* 0 = max(200000, 400 ef) */
     int32_t precision;     /* brute scan precision, -1 = index default */
 } FvGpuLearnedOpts;

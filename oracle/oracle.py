@@ -56,9 +56,13 @@ def lib():
         L.or_brute_batch.restype = C.c_int
         L.or_brute_batch.argtypes = [_P, C.c_uint64, C.c_int, _P, C.c_int, _P, _P, _P, _P, _P,
                                      C.c_uint32, _P, _P, _P, _P]
-        L.or_brute_rows.restype = C.c_uint32
-        L.or_brute_rows.argtypes = [_P, C.c_int, _P, C.c_uint64, _P, _P, C.c_uint64, _P, _P, _P,
-                                    C.c_int32, C.c_int, C.c_int, C.c_uint32, _P, _P]
+        L.or_sq8_params.restype = None
+        L.or_sq8_params.argtypes = [_P, C.c_uint64, C.c_int, _P, _P]
+        L.or_sq8_codes.restype = None
+        L.or_sq8_codes.argtypes = [_P, C.c_uint64, C.c_int, _P, C.c_float, _P]
+        L.or_sq8_brute_batch.restype = C.c_int
+        L.or_sq8_brute_batch.argtypes = [_P, C.c_uint64, C.c_int, _P, C.c_int, _P, _P, _P, _P, _P,
+                                         C.c_uint32, C.c_uint32, _P, _P, _P]
         L.or_hnsw_search.restype = C.c_uint32
         L.or_hnsw_search.argtypes = [_P, C.c_uint64, C.c_int, _P, C.c_uint32, _P, _P, _P, _P,
                                      C.c_int32, _P, C.c_uint32, C.c_uint32, C.c_uint64, _P, _P, _P]
@@ -109,6 +113,45 @@ def brute_batch(vectors, cols, queries, qd, nodes, children, invals, topb):
                          _ptr(nodes), _ptr(children), _ptr(invals), topb, _ptr(out), _ptr(cnt),
                          _ptr(trunc), _ptr(passing))
     return out, cnt, trunc, passing
+
+
+def sq8_params(vectors):
+    """SQ8 quantizer of a row set: (per-dim midrange centre, scale)."""
+    vectors = np.ascontiguousarray(vectors, np.float32)
+    n, dim = vectors.shape
+    mid = np.zeros(dim, np.float32)
+    inv = np.zeros(1, np.float32)
+    lib().or_sq8_params(_ptr(vectors), n, dim, _ptr(mid), _ptr(inv))
+    return mid, np.float32(inv[0])
+
+
+def sq8_codes(x, mid, inv):
+    """int8 codes clamp(rint((x - mid) * inv), -127, 127) of rows x."""
+    x = np.ascontiguousarray(np.atleast_2d(x), np.float32)
+    mid = np.ascontiguousarray(mid, np.float32)
+    out = np.zeros(x.shape, np.int8)
+    lib().or_sq8_codes(_ptr(x), x.shape[0], x.shape[1], _ptr(mid), float(inv), _ptr(out))
+    return out
+
+
+def sq8_brute_batch(vectors, cols, queries, qd, nodes, children, invals, topb, over):
+    """The quantized SQ8 path: top over x topb by code distance, re-ranked
+    exactly; returns (hits, cnt, trunc)."""
+    vectors = np.ascontiguousarray(vectors, np.float32)
+    queries = np.ascontiguousarray(queries, np.float32)
+    n, dim = vectors.shape
+    nq = queries.shape[0]
+    cols = None if cols is None else np.ascontiguousarray(cols, np.float64)
+    proots = np.ascontiguousarray(qd["proot"], np.int32)
+    out = np.zeros((nq, topb), dtype=HIT_DT)
+    cnt = np.zeros(nq, np.uint32)
+    trunc = np.zeros(nq, np.uint8)
+    rc = lib().or_sq8_brute_batch(_ptr(vectors), n, dim, _ptr(cols), nq, _ptr(queries),
+                                  _ptr(proots), _ptr(nodes), _ptr(children), _ptr(invals), topb,
+                                  over, _ptr(out), _ptr(cnt), _ptr(trunc))
+    if rc != 0:
+        raise MemoryError("oracle sq8: allocation failed")
+    return out, cnt, trunc
 
 
 def hnsw_batch(vectors, l0, cols, queries, qd, nodes, children, invals, entries, ef, max_visited):

@@ -239,34 +239,107 @@ int or_brute_batch(const float *vectors, uint64_t count, int dim,
     return 0;
 }
 
-/* Exact top-B over an explicit row list (the span-enumeration semantics of
- * BruteCursor::compute, engine.cpp:1082-1130): rows may repeat when spans
- * overlap; duplicates are removed after the sort when dedup is set. */
-uint32_t or_brute_rows(const float *vectors, int dim, const double *cols,
-                       uint64_t count, const float *query, const uint32_t *rows,
-                       uint64_t nrows, const OrPredNode *nodes,
-                       const int32_t *children, const double *invals,
-                       int32_t proot, int need_eval, int dedup, uint32_t topb,
-                       OrHit *scratch, OrHit *out) {
-    uint64_t n = 0;
-    for (uint64_t i = 0; i < nrows; i++) {
-        uint32_t r = rows[i];
-        if (need_eval && proot >= 0 &&
-            !or_pred_eval(nodes, children, invals, cols, count, proot, r))
+/* ---------------------------------------------------------------------
+ * SQ8 over arbitrary float data: the scan ranks rows by the distance of
+ * int8 codes and the top overfetch x B are re-ranked exactly — the role of
+ * the reference's SQ8 tiled path (quantize, int8 scan to 2 topb candidates,
+ * re-rank: gpu_engine.cu:643-760, 1911-2080). This library's quantizer:
+ * per-dim midrange centre mid[d] = (lo[d] + hi[d]) * 0.5f and one scale
+ * inv = 127 / max_d (hi[d] - lo[d]) * 0.5f (1 when every dim is constant);
+ * code = clamp(rint((x - mid) * inv), -127, 127), the same map for rows and
+ * queries. Code distance |xc|^2 + |qc|^2 - 2 xc.qc in integers, as FP32;
+ * the R = min(over x B, 4096, passing) passing rows smallest by (code
+ * distance, row) are re-ranked with or_l2sq and sorted by (dist, row); the
+ * first min(B, passing) are the result, truncated = (count == B).
+ * ------------------------------------------------------------------- */
+void or_sq8_params(const float *vectors, uint64_t count, int dim, float *mid, float *inv) {
+    float hmax = 0.f;
+    for (int d = 0; d < dim; d++) {
+        float lo = INFINITY, hi = -INFINITY;
+        for (uint64_t r = 0; r < count; r++) {
+            const float x = vectors[r * (uint64_t)dim + d];
+            lo = x < lo ? x : lo;
+            hi = x > hi ? x : hi;
+        }
+        mid[d] = (lo + hi) * 0.5f;
+        const float h = (hi - lo) * 0.5f;
+        hmax = h > hmax ? h : hmax;
+    }
+    *inv = hmax > 0.f ? 127.0f / hmax : 1.0f;
+}
+
+void or_sq8_codes(const float *x, uint64_t n, int dim, const float *mid, float inv,
+                  int8_t *codes) {
+    for (uint64_t r = 0; r < n; r++)
+        for (int d = 0; d < dim; d++) {
+            float c = rintf((x[r * (uint64_t)dim + d] - mid[d]) * inv);
+            c = c < -127.f ? -127.f : (c > 127.f ? 127.f : c);
+            codes[r * (uint64_t)dim + d] = (int8_t)(int)c;
+        }
+}
+
+int or_sq8_brute_batch(const float *vectors, uint64_t count, int dim, const double *cols,
+                       int nq, const float *queries, const int32_t *proots,
+                       const OrPredNode *nodes, const int32_t *children,
+                       const double *invals, uint32_t topb, uint32_t over, OrHit *out,
+                       uint32_t *out_cnt, uint8_t *truncated) {
+    float *mid = (float *)malloc(sizeof(float) * (size_t)dim);
+    int8_t *xc = (int8_t *)malloc((size_t)count * dim);
+    int32_t *xn = (int32_t *)malloc(sizeof(int32_t) * (size_t)count);
+    if (!mid || !xc || !xn) {
+        free(mid); free(xc); free(xn);
+        return -1;
+    }
+    float inv;
+    or_sq8_params(vectors, count, dim, mid, &inv);
+    or_sq8_codes(vectors, count, dim, mid, inv, xc);
+    for (uint64_t r = 0; r < count; r++) {
+        int32_t s = 0;
+        for (int d = 0; d < dim; d++) s += (int32_t)xc[r * (uint64_t)dim + d] * xc[r * (uint64_t)dim + d];
+        xn[r] = s;
+    }
+    int rc = 0;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int q = 0; q < nq; q++) {
+        int8_t *qc = (int8_t *)malloc((size_t)dim);
+        OrHit *cand = (OrHit *)malloc(sizeof(OrHit) * ((size_t)count + 1));
+        if (!qc || !cand) {
+            free(qc); free(cand);
+            rc = -1;
             continue;
-        scratch[n].node = r;
-        scratch[n].dist = or_l2sq(query, vectors + (uint64_t)r * dim, dim);
-        n++;
+        }
+        const float *qv = queries + (uint64_t)q * dim;
+        or_sq8_codes(qv, 1, dim, mid, inv, qc);
+        int32_t qn = 0;
+        for (int d = 0; d < dim; d++) qn += (int32_t)qc[d] * qc[d];
+        uint64_t n = 0;
+        for (uint64_t r = 0; r < count; r++) {
+            if (proots[q] >= 0 &&
+                !or_pred_eval(nodes, children, invals, cols, count, proots[q], (uint32_t)r))
+                continue;
+            const int8_t *x = xc + r * (uint64_t)dim;
+            int32_t dot = 0;
+            for (int d = 0; d < dim; d++) dot += (int32_t)x[d] * qc[d];
+            cand[n].node = r;
+            cand[n].dist = (float)(xn[r] + qn - 2 * dot);
+            n++;
+        }
+        qsort(cand, n, sizeof(OrHit), hit_cmp);
+        uint64_t R = (uint64_t)topb * over;
+        if (R > 4096) R = 4096;
+        if (R > n) R = n;
+        for (uint64_t i = 0; i < R; i++)
+            cand[i].dist = or_l2sq(qv, vectors + cand[i].node * (uint64_t)dim, dim);
+        qsort(cand, R, sizeof(OrHit), hit_cmp);
+        const uint32_t B = (uint32_t)(n < topb ? n : topb);
+        memcpy(out + (uint64_t)q * topb, cand, B * sizeof(OrHit));
+        out_cnt[q] = B;
+        truncated[q] = (B == topb) ? 1 : 0;
+        free(qc);
+        free(cand);
     }
-    qsort(scratch, n, sizeof(OrHit), hit_cmp);
-    uint64_t m = 0;
-    for (uint64_t i = 0; i < n; i++) {
-        if (dedup && m > 0 && scratch[m - 1].node == scratch[i].node) continue;
-        scratch[m++] = scratch[i];
-    }
-    uint32_t k = (uint32_t)(m < topb ? m : topb);
-    memcpy(out, scratch, k * sizeof(OrHit));
-    return k;
+    free(mid); free(xc); free(xn);
+    return rc;
 }
 
 /* ---------------------------------------------------------------------

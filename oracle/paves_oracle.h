@@ -39,12 +39,14 @@ int or_brute_batch(const float *vectors, uint64_t count, int dim,
                    const int32_t *children, const double *invals,
                    uint32_t topb, OrHit *out, uint32_t *out_cnt,
                    uint8_t *truncated, uint64_t *passing);
-uint32_t or_brute_rows(const float *vectors, int dim, const double *cols,
-                       uint64_t count, const float *query, const uint32_t *rows,
-                       uint64_t nrows, const OrPredNode *nodes,
-                       const int32_t *children, const double *invals,
-                       int32_t proot, int need_eval, int dedup, uint32_t topb,
-                       OrHit *scratch, OrHit *out);
+void or_sq8_params(const float *vectors, uint64_t count, int dim, float *mid, float *inv);
+void or_sq8_codes(const float *x, uint64_t n, int dim, const float *mid, float inv,
+                  int8_t *codes);
+int or_sq8_brute_batch(const float *vectors, uint64_t count, int dim, const double *cols,
+                       int nq, const float *queries, const int32_t *proots,
+                       const OrPredNode *nodes, const int32_t *children,
+                       const double *invals, uint32_t topb, uint32_t over, OrHit *out,
+                       uint32_t *out_cnt, uint8_t *truncated);
 uint32_t or_hnsw_search(const float *vectors, uint64_t count, int dim,
                         const uint32_t *l0, uint32_t l0_stride,
                         const double *cols, const OrPredNode *nodes,

@@ -61,12 +61,15 @@ namespace pv {
 /*
  * One wave per (padded) query. mode: 0 = FP32 VALU scan bound, 1 = BF16
  * bound, 3 = FP32 MFMA bound,
- * 2 = INT8 scan: also writes q - centre as int8 and ||q - centre||^2; the
- * INT8 scan is exact (eps = 0) and the host takes it only when every query
- * value is an integer within [centre - 128, centre + 127] (a value outside
- * sets kStPrepFail in the query's status, which the threshold stage keeps,
- * so such a run can never pass as exact). Every mode writes the status
- * word, clearing the bit a previous run may have left.
+ * 2 = INT8 scan: also writes the query's int8 codes (dimp per query, zero
+ * past dim) and their squared norm. Exact mode (i8mid == nullptr): codes
+ * q - centre; the INT8 scan is exact (eps = 0) and the host takes it only
+ * when every query value is an integer within [centre - 128, centre + 127]
+ * (a value outside sets kStPrepFail in the query's status, which the
+ * threshold stage keeps, so such a run can never pass as exact).
+ * Quantized mode: codes clamp(rint((q - i8mid[d]) * i8inv), -127, 127), the
+ * rows' transform. Every mode writes the status word, clearing the bit a
+ * previous run may have left.
  * The scan is exact (eps = 0) when every vector and query value is an
  * integer of magnitude <= 256 (exact in BF16, products exact in FP32) and
  * every partial sum stays below 2^24: sum (x-q)^2 <= D * delta^2 with
@@ -77,7 +80,9 @@ __device__ __forceinline__ void prep_query(
     int qi, const float *__restrict__ q, int nq, int dim, int dimb, uint16_t *__restrict__ qbf,
     float *__restrict__ qnorm, float *__restrict__ eps, float xmax_norm,
     float vmin, float vmax, int all_int, int mode, int8_t *__restrict__ qi8 = nullptr,
-    float *__restrict__ qn8 = nullptr, int centre = 0, uint32_t *__restrict__ status = nullptr) {
+    float *__restrict__ qn8 = nullptr, int centre = 0, uint32_t *__restrict__ status = nullptr,
+    const float *__restrict__ i8mid = nullptr, float i8inv = 1.f, int dimp = 0) {
+#pragma clang fp contract(off)
     const int l = threadIdx.x & 63;
     const float *src = q + (uint64_t)qi * dim;
     float ss = 0.f, pmax = 0.f, nmax = 0.f, s8 = 0.f;
@@ -90,12 +95,20 @@ __device__ __forceinline__ void prep_query(
         nmax = fmaxf(nmax, -v);
         nonint |= (v != rintf(v)) ? 1 : 0;
         if (mode == 2) {
-            const float c = v - (float)centre;
-            out8 |= (c != rintf(c) || c < -128.f || c > 127.f) ? 1 : 0;
-            qi8[(uint64_t)qi * dim + j] = (int8_t)(int)fminf(fmaxf(c, -128.f), 127.f);
+            float c;
+            if (i8mid == nullptr) {
+                c = v - (float)centre;
+                out8 |= (c != rintf(c) || c < -128.f || c > 127.f) ? 1 : 0;
+                qi8[(uint64_t)qi * dimp + j] = (int8_t)(int)fminf(fmaxf(c, -128.f), 127.f);
+            } else {
+                c = fminf(fmaxf(rintf((v - i8mid[j]) * i8inv), -127.f), 127.f);
+                qi8[(uint64_t)qi * dimp + j] = (int8_t)(int)c;
+            }
             s8 = fmaf(c, c, s8);
         }
     }
+    if (mode == 2)
+        for (int j = dim + l; j < dimp; j += 64) qi8[(uint64_t)qi * dimp + j] = 0;
     /* the BF16 operand's zero padding (the tiled copy's dims past dim) */
     for (int j = dim + l; j < dimb; j += 64) qbf[(uint64_t)qi * dimb + j] = 0;
 #pragma unroll
@@ -174,6 +187,9 @@ struct Phase1Args {
     int8_t *qi8;          /* mode 2 */
     float *qn8;
     int centre;
+    const float *i8mid;   /* mode 2, quantized codes: per-dim centre, scale */
+    float i8inv;
+    int dimp;             /* mode 2: code row length */
     uint32_t *status;
     /* learned routing (dim <= 256): router features of each query except
      * its selectivity, [nqp][nfeat] — the query, then (slot dim left for the
@@ -197,7 +213,7 @@ __global__ __launch_bounds__(256) void k_phase1(Phase1Args A) {
         const int qi = (int)blockIdx.x * 4 + w;
         if (qi >= A.nqp) return;
         prep_query(qi, A.q, A.nq, A.dim, A.dimb, A.qbf, A.qnorm, A.eps, A.xmax_norm, A.vmin, A.vmax,
-                   A.all_int, A.mode, A.qi8, A.qn8, A.centre, A.status);
+                   A.all_int, A.mode, A.qi8, A.qn8, A.centre, A.status, A.i8mid, A.i8inv, A.dimp);
         return;
     }
     /* with router features: one block per padded query; wave 0 preps it,
@@ -209,7 +225,7 @@ __global__ __launch_bounds__(256) void k_phase1(Phase1Args A) {
     if (qi >= A.nqp) return;
     if (w == 0)
         prep_query(qi, A.q, A.nq, A.dim, A.dimb, A.qbf, A.qnorm, A.eps, A.xmax_norm, A.vmin, A.vmax,
-                   A.all_int, A.mode, A.qi8, A.qn8, A.centre, A.status);
+                   A.all_int, A.mode, A.qi8, A.qn8, A.centre, A.status, A.i8mid, A.i8inv, A.dimp);
     if (qi >= A.nq) return; /* block-uniform */
     __shared__ float xq[256];
     __shared__ float wn[4][kNear], wm[kNear];
@@ -260,6 +276,7 @@ struct ScanArgs {
     int dim;
     int dimb;                 /* BF16 scan: dims of the tiled copy and of qbf
                                  rows (dim rounded up to a multiple of 128) */
+    int dimp;                 /* INT8 scan: dims of the code copy and of qi8 rows */
     const uint16_t *qbf;
     const float *qf;
     const float *qnorm;
@@ -1045,15 +1062,17 @@ void k_scan_bf16_mq(ScanArgs A) {
 }
 
 /*
- * Exact INT8 scan (SQ8 indexes over integer data; dim 128 or 256). Rows and
- * queries are shifted by the index centre into int8; x.q accumulates in
- * int32 with v_mfma_i32_32x32x32_i8 (exact), and with ||x - c||^2 and
- * ||q - c||^2 (integers below 2^24) the epilogue sees the exact distance,
- * so eps = 0 and the select stage certifies as for integer BF16 data. The
- * copy is half the bytes of BF16 for the same rows. Layout: 32-row tiles,
- * k-steps of 32 dims as 1 KB chunks (slot l: row l & 31, dims 32 s + 16
- * (l >> 5) .. +16); a unit is 8 chunks = 8 KB = PT tiles (PT = 256 / dim),
- * the cursor's logical tile. Same pipeline as k_scan_bf16 otherwise.
+ * INT8 scan (SQ8 indexes; code dims dimp = 64, 128 or 256). x.q of the
+ * int8 codes accumulates in int32 with v_mfma_i32_32x32x32_i8 (exact), and
+ * with the squared code norms (integers below 2^24) the epilogue sees the
+ * exact code distance. Exact mode (integer data, codes x - centre): that is
+ * the distance, eps = 0 and the select stage certifies as for integer BF16
+ * data. Quantized mode: the code distance ranks the rows and the select
+ * stage re-ranks the top overfetch x k exactly. The copy is half the bytes
+ * of BF16 for the same rows. Layout: 32-row tiles, k-steps of 32 dims as
+ * 1 KB chunks (slot l: row l & 31, dims 32 s + 16 (l >> 5) .. +16); a unit
+ * is 8 chunks = 8 KB = PT tiles (PT = 256 / dimp), the cursor's logical
+ * tile. Same pipeline as k_scan_bf16 otherwise.
  */
 template <bool PREPASS, int NW, int PT, bool INL>
 __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(2)))
@@ -1085,7 +1104,7 @@ void k_scan_i8(ScanArgs A) {
     for (int e = tid; e < 2 * KS * 64; e += blockDim.x) {
         const int lane = e & 63, s = (e >> 6) % KS, qb = (e >> 6) / KS;
         const int q = qbase + (lane & 31) + 32 * qb;
-        qsh[e] = *(const u32x4 *)(A.qi8 + (uint64_t)q * A.dim + 32 * s + 16 * (lane >> 5));
+        qsh[e] = *(const u32x4 *)(A.qi8 + (uint64_t)q * A.dimp + 32 * s + 16 * (lane >> 5));
     }
     __shared__ int s_im[INL ? kQT : 1];
     __shared__ double s_iv[INL ? kQT : 1];
@@ -1146,6 +1165,103 @@ void k_scan_i8(ScanArgs A) {
         load_in(in1);
         do_unit(a0, in0);
         if (!in1[0].live) break;
+        load_rows(a0);
+        load_in(in0);
+        do_unit(a1, in1);
+    }
+    if (!PREPASS) flush_candidates(A, qbase, C);
+}
+
+/*
+ * INT8 scan over code rows wider than one unit (dimp = 512, 768 or 1024):
+ * a 32-row tile is dimp / 256 units of 8 KB, streamed as k_scan_bf16 streams
+ * its units (loads of unit u + 1 issued before the 16 MFMAs of unit u; the
+ * seeds at a tile's first unit, its epilogue after the last), with the
+ * int32 accumulators, seeds and epilogue of k_scan_i8.
+ */
+template <bool PREPASS, int NW, bool INL>
+__global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(2)))
+void k_scan_i8w(ScanArgs A) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    const int S = A.dimp / 32;              /* k-steps of 32 dims per tile */
+    const uint32_t NCH = (uint32_t)(S / 8);  /* units per tile */
+    u32x4 *qsh = (u32x4 *)smem;             /* [2][S][64] x 16 B */
+    CandLds &C = *(CandLds *)(smem + (size_t)2 * S * 64 * 16);
+
+    const int tid = threadIdx.x, l = tid & 63, wave = tid >> 6;
+    if (scan_pad(A)) return;
+    const int qbase = scan_qgroup(A) * kQT;
+    if (tile_skipped(A, qbase)) return;
+
+    __shared__ uint32_t s_next;
+    UnitCursor cur = cursor_init<NW>(A, wave, &s_next);
+    const u32x4 *vb = (const u32x4 *)A.vi8;
+    const uint32_t tlast = cursor_last(cur, A.ntiles);
+    auto phys = [&](uint32_t t) { return min(t, tlast) * A.stride; };
+    auto load_rows = [&](u32x4 (&a)[8]) {
+        const u32x4 *p = vb + (uint64_t)phys(cur.tile) * (2 * A.dimp) + (uint64_t)cur.c * 512 + l;
+#pragma unroll
+        for (int j = 0; j < 8; j++) a[j] = p[64 * j];
+    };
+    u32x4 a0[8], a1[8];
+    load_rows(a0);
+
+    for (int e = tid; e < 2 * S * 64; e += blockDim.x) {
+        const int lane = e & 63, s = (e >> 6) % S, qb = (e >> 6) / S;
+        const int q = qbase + (lane & 31) + 32 * qb;
+        qsh[e] = *(const u32x4 *)(A.qi8 + (uint64_t)q * A.dimp + 32 * s + 16 * (lane >> 5));
+    }
+    __shared__ int s_im[INL ? kQT : 1];
+    __shared__ double s_iv[INL ? kQT : 1];
+    if (INL) stage_inline(A, 0, qbase, s_im, s_iv);
+    if (!PREPASS) cand_init(C);
+
+    float tq[2], qn[2];
+    const uint32_t *wr[2];
+    load_query_state(A, qbase, tq, qn, wr);
+    const int32_t th[2] = {i8_threshold(qn[0], tq[0]), i8_threshold(qn[1], tq[1])};
+    auto load_in = [&](TileIn &in) {
+        load_tile_in<INL>(A, phys(cur.tile), wr, in);
+        in.t = cur.tile;
+        in.c = cur.c;
+        in.live = cur.tile < cur.end;
+        cursor_next(cur, NCH, &s_next);
+    };
+    TileIn in0, in1;
+    __syncthreads();
+    cursor_start(cur, &s_next);
+    load_in(in0);
+    i32x16 c0 = {}, c1 = {};
+    uint32_t par = 0;
+    auto do_unit = [&](const u32x4 (&a)[8], const TileIn &in) {
+        const uint32_t c = in.c;
+        if (c == 0) {
+            uint32_t w0 = in.w[0], w1 = in.w[1];
+            if (INL) {
+                double cv[16];
+                expand_rows(in.rv, cv);
+                w0 = inline_word(cv, s_im, s_iv, 0, 0);
+                w1 = inline_word(cv, s_im, s_iv, 0, 1);
+            }
+            seed_i8(in.xr, w0 & in.valid, w1 & in.valid, c0, c1, par);
+        }
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int s = (int)c * 8 + j;
+            const i32x4 av = __builtin_bit_cast(i32x4, a[j]);
+            const i32x4 b0 = __builtin_bit_cast(i32x4, qsh[(0 * S + s) * 64 + l]);
+            const i32x4 b1 = __builtin_bit_cast(i32x4, qsh[(1 * S + s) * 64 + l]);
+            c0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, b0, c0, 0, 0, 0);
+            c1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, b1, c1, 0, 0, 0);
+        }
+        if (c == NCH - 1)
+            epilogue_i8<PREPASS>(A, qbase, in.t, in.t * A.stride, c0, c1, par, qn, th, C);
+    };
+    while (in0.live) {
+        load_rows(a1);
+        load_in(in1);
+        do_unit(a0, in0);
+        if (!in1.live) break;
         load_rows(a0);
         load_in(in0);
         do_unit(a1, in1);
@@ -1424,6 +1540,7 @@ __device__ uint64_t block_kth_small(Get get, uint32_t n, uint32_t B, uint64_t *s
 struct TauArgs {
     const uint64_t *gmin;
     uint32_t ngroups, B;
+    uint32_t over;          /* candidates kept per query: over x k (at most kSelMax) */
     const float *eps;
     float *tau;
     uint32_t *pcnt;
@@ -1446,6 +1563,7 @@ __device__ __forceinline__ void tau_block(const TauArgs &A, int q) {
     const uint8_t *__restrict__ skip = A.skip;
     const uint32_t *__restrict__ topbq = A.topbq;
     if (topbq) B = topbq[q];
+    B = min(B * max(A.over, 1u), (uint32_t)kSelMax);
     if (skip && skip[q]) { /* routed elsewhere: no candidates, no outputs */
         if (threadIdx.x == 0) {
             tau[q] = -__uint_as_float(0x7f800000u);
@@ -1571,10 +1689,13 @@ __global__ __launch_bounds__(256) void k_gather(
  * exact CPU-order FP32 distance, sorts by (dist, row) and emits the first
  * B. truncated = (B == topb) (gpu_engine.cu:1059-1064). Status bits flag a
  * pool overflow or a re-rank set beyond kSelMax; the host then re-runs.
+ * over > 1 (quantized INT8 scan, keys hold code distances): the set is the
+ * min(over x topb, kSelMax, pool size) smallest keys, all re-ranked exactly
+ * (the reference SQ8 path's candidate re-rank, gpu_engine.cu:1911-2080).
  */
 __global__ __launch_bounds__(256) void k_select(
     const uint64_t *__restrict__ pool, const uint32_t *__restrict__ pcnt,
-    uint64_t cap_pool, uint32_t topb, const float *__restrict__ eps,
+    uint64_t cap_pool, uint32_t topb, uint32_t over, const float *__restrict__ eps,
     const float *__restrict__ vecs, const float *__restrict__ qf, int dim,
     FvHit *__restrict__ out, uint32_t out_stride, uint32_t *__restrict__ out_cnt,
     uint8_t *__restrict__ out_flag, uint32_t *__restrict__ status,
@@ -1615,6 +1736,8 @@ __global__ __launch_bounds__(256) void k_select(
         return pq[s * scap + (i - pre[s])];
     };
     const uint32_t B = min(topb, n);
+    /* keys selected for the re-rank (quantized scan: the overfetch set) */
+    const uint32_t R = over > 1 ? min(min(topb * over, (uint32_t)kSelMax), n) : B;
     if (B == 0) {
         if (t == 0) {
             out_cnt[q] = 0;
@@ -1627,13 +1750,13 @@ __global__ __launch_bounds__(256) void k_select(
      * prep): every scan form and the reference's FP32 sum are exact there,
      * so the pool keys already hold the reference's distances and the
      * re-rank (and its copy of the query) is skipped */
-    const bool rerank = !(e == 0.f);
+    const bool rerank = !(e == 0.f) || over > 1;
     if (rerank)
         for (int j = t; j < dim; j += blockDim.x) qsh[j] = qf[(uint64_t)q * dim + j];
     uint32_t m0 = kSelMax + 1;
     const uint64_t kth =
-        B <= 64 ? block_kth_small(pk_at, n, B, sel, kSelMax, &nsel, wb, hist, tmp, bc, &m0)
-                : block_kth(pk_at, n, B, hist, tmp, bc);
+        R <= 64 ? block_kth_small(pk_at, n, R, sel, kSelMax, &nsel, wb, hist, tmp, bc, &m0)
+                : block_kth(pk_at, n, R, hist, tmp, bc);
     float lim = key_dist(kth);
     if (e > 0.f) {
         double w = (double)lim + 2.0 * (double)e;
@@ -1642,8 +1765,8 @@ __global__ __launch_bounds__(256) void k_select(
     }
     uint32_t m;
     if (e == 0.f && m0 <= 64) {
-        /* the B smallest keys are already sorted at the front of sel */
-        m = B;
+        /* the R smallest keys are already sorted at the front of sel */
+        m = R;
     } else {
         __syncthreads();
         if (t == 0) nsel = 0;
@@ -1939,13 +2062,15 @@ int stage_batch(FvGpuIndex *g, FvGpuBatch *b, int nq, const float *queries,
     b->i8_ok = false;
     if (g->d_vi8 != nullptr) {
         bool ok = true;
-        const float lo = (float)g->i8_center - 128.f, hi = (float)g->i8_center + 127.f;
-        for (size_t i = 0, n = (size_t)nq * dim; i < n && ok; i++) {
-            const float v = queries[i];
-            ok = v == rintf(v) && v >= lo && v <= hi;
+        if (g->i8_mode == 1) {
+            const float lo = (float)g->i8_center - 128.f, hi = (float)g->i8_center + 127.f;
+            for (size_t i = 0, n = (size_t)nq * dim; i < n && ok; i++) {
+                const float v = queries[i];
+                ok = v == rintf(v) && v >= lo && v <= hi;
+            }
         }
         b->i8_ok = ok;
-        PV_TRY(b->d_qi8.ensure((size_t)nqp * dim));
+        PV_TRY(b->d_qi8.ensure((size_t)nqp * g->dimp));
         PV_TRY(b->d_qn8.ensure((size_t)nqp * 4));
     }
     /* bitmaps and per-program counts for the layout's program capacity
@@ -2071,6 +2196,9 @@ hipError_t launch_phase1(FvGpuIndex *g, FvGpuBatch *b, int prec, bool bitmap, hi
         P.qi8 = b->d_qi8.as<int8_t>();
         P.qn8 = b->d_qn8.as<float>();
         P.centre = g->i8_center;
+        P.i8mid = g->i8_mode == 2 ? g->d_i8mid : nullptr;
+        P.i8inv = g->i8_inv;
+        P.dimp = g->dimp;
         P.status = qa.status;
         if (pf != nullptr && g->dim <= 256) {
             P.cent_t = pf->cent_t;
@@ -2096,9 +2224,10 @@ struct ScanShape {
     uint32_t gx, stride;
 };
 
-/* 32-row tiles per scan unit: the INT8 scan's 8 KB unit spans 256 / dim */
-static inline uint32_t scan_unit_tiles(int prec, int dim) {
-    return prec == FV_GPU_PREC_SQ8 ? (uint32_t)(256 / dim) : 1u;
+/* 32-row tiles per scan unit: the INT8 scan's 8 KB unit spans 256 / dimp
+ * tiles (code rows of up to 256 dims) */
+static inline uint32_t scan_unit_tiles(const FvGpuIndex *g, int prec) {
+    return (prec == FV_GPU_PREC_SQ8 && g->dimp <= 256) ? (uint32_t)(256 / g->dimp) : 1u;
 }
 
 /* true when another batch stream of the index still has work queued: the
@@ -2111,13 +2240,26 @@ static bool other_batches_in_flight(const FvGpuIndex *g, const FvGpuBatch *b) {
     return false;
 }
 
+/* candidates per query the scan must keep: topb, or for the quantized INT8
+ * scan the overfetch x topb rows that the select stage re-ranks exactly
+ * (FvGpuBruteOpts.overfetch, default 4), at most kSelMax */
+static uint32_t scan_overfetch(const FvGpuIndex *g, int prec, const FvGpuBruteOpts *o) {
+    if (!(prec == FV_GPU_PREC_SQ8 && g->i8_mode == 2)) return 1;
+    const uint32_t f = (o && o->overfetch) ? o->overfetch : 4u;
+    return std::min<uint32_t>(f, 64);
+}
+static uint32_t scan_candidates(const FvGpuIndex *g, const FvGpuBatch *b, int prec,
+                                const FvGpuBruteOpts *o) {
+    return std::min<uint32_t>(b->topb * scan_overfetch(g, prec, o), (uint32_t)kSelMax);
+}
+
 static ScanShape scan_shape(const FvGpuIndex *g, const FvGpuBatch *b, const FvGpuBruteOpts *o,
                             int prec, uint32_t stride_override, bool shared = false) {
     ScanShape sh;
     const int dim = g->dim;
-    const uint32_t B = b->topb;
+    const uint32_t B = scan_candidates(g, b, prec, o);
     /* scan tiles: 32 rows, or for the INT8 scan its 8 KB units */
-    const uint32_t ntiles = (uint32_t)(g->count_pad / 32) / scan_unit_tiles(prec, dim);
+    const uint32_t ntiles = (uint32_t)(g->count_pad / 32) / scan_unit_tiles(g, prec);
     const bool bf = prec == FV_GPU_PREC_BF16 || prec == FV_GPU_PREC_SQ8 ||
                     prec == FV_GPU_PREC_FP32;
     /* query tiles per WG: several when many query tiles re-read the same
@@ -2172,8 +2314,9 @@ static ScanShape scan_shape(const FvGpuIndex *g, const FvGpuBatch *b, const FvGp
     return sh;
 }
 
-/* INT8 scan launch: waves per WG (4 / 8), tiles per unit (dim 128: 2,
- * dim 256: 1) and inline predicates pick the kernel instance */
+/* INT8 scan launch: waves per WG (4 / 8), tiles per unit (code dims 64:
+ * 4, 128: 2, 256: 1; wider rows: the multi-unit kernel) and inline
+ * predicates pick the kernel instance */
 template <bool PREPASS>
 static void launch_scan_i8(int nw, int pt, bool inl, uint32_t gx, int nqt, size_t shm,
                            hipStream_t s, const ScanArgs &A0) {
@@ -2181,14 +2324,22 @@ static void launch_scan_i8(int nw, int pt, bool inl, uint32_t gx, int nqt, size_
     const dim3 grid = scan_grid(A, gx, (uint32_t)nqt);
 #define PV_I8(NW, PT, INL) \
     hipLaunchKernelGGL((k_scan_i8<PREPASS, NW, PT, INL>), grid, dim3(NW * 64), shm, s, A)
-    if (nw == 8) {
-        if (pt == 2) { if (inl) PV_I8(8, 2, true); else PV_I8(8, 2, false); }
+#define PV_I8W(NW, INL) \
+    hipLaunchKernelGGL((k_scan_i8w<PREPASS, NW, INL>), grid, dim3(NW * 64), shm, s, A)
+    if (A.dimp > 256) {
+        if (nw == 8) { if (inl) PV_I8W(8, true); else PV_I8W(8, false); }
+        else { if (inl) PV_I8W(4, true); else PV_I8W(4, false); }
+    } else if (nw == 8) {
+        if (pt == 4) { if (inl) PV_I8(8, 4, true); else PV_I8(8, 4, false); }
+        else if (pt == 2) { if (inl) PV_I8(8, 2, true); else PV_I8(8, 2, false); }
         else { if (inl) PV_I8(8, 1, true); else PV_I8(8, 1, false); }
     } else {
-        if (pt == 2) { if (inl) PV_I8(4, 2, true); else PV_I8(4, 2, false); }
+        if (pt == 4) { if (inl) PV_I8(4, 4, true); else PV_I8(4, 4, false); }
+        else if (pt == 2) { if (inl) PV_I8(4, 2, true); else PV_I8(4, 2, false); }
         else { if (inl) PV_I8(4, 1, true); else PV_I8(4, 1, false); }
     }
 #undef PV_I8
+#undef PV_I8W
 }
 
 static hipError_t launch_main_scan(const ScanLaunch &L, hipStream_t s) {
@@ -2226,6 +2377,9 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
     const int prec = scan_precision(g, b, o);
     const int nq = b->nq, nqp = b->nqt * kQT, dim = g->dim;
     const uint32_t B = b->topb;
+    /* quantized INT8 scan: the scan keeps the top over x k by code distance */
+    const uint32_t over = scan_overfetch(g, prec, o);
+    const uint32_t Bc = scan_candidates(g, b, prec, o);
     QAux qa = qaux_of(b);
     FvGpuStats &st = g->last;
     memset(&st, 0, sizeof(st));
@@ -2293,7 +2447,7 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
         /* gathered distances are already exact: eps = 0 */
         hipLaunchKernelGGL(k_select, dim3(nq), dim3(256), dim * 4, s,
                            b->d_pool.as<uint64_t>(), b->d_pcnt.as<uint32_t>(),
-                           b->cap_pool, B, (const float *)nullptr,
+                           b->cap_pool, B, 1u, (const float *)nullptr,
                            g->d_vecs, b->d_q, dim, b->d_out.as<FvHit>(),
                            b->out_stride, b->d_cnt.as<uint32_t>(),
                            b->d_flag.as<uint8_t>(), qa.status, skip,
@@ -2308,7 +2462,7 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
     /* ---- full-scan route ---- */
     /* pre-pass samples every stride-th tile (INT8: 8 KB unit of PT tiles) */
     const bool i8 = prec == FV_GPU_PREC_SQ8;
-    const int pt = (int)scan_unit_tiles(prec, dim);
+    const int pt = (int)scan_unit_tiles(g, prec);
     const uint32_t ntiles = (uint32_t)(g->count_pad / 32) / (uint32_t)pt;
     const bool bf = prec == FV_GPU_PREC_BF16;
     const bool f32m = prec == FV_GPU_PREC_FP32;   /* FP32 MFMA scan (8-wave WGs) */
@@ -2320,8 +2474,8 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
     const uint32_t stiles = (ntiles + stride - 1) / stride;
     b->stride = stride;
     b->ngroups = stiles * 2 * (uint32_t)pt;
-    b->cap_pool = std::max<uint64_t>(16384, (uint64_t)8 * B * stride);
-    if (stride_override == 1) b->cap_pool = std::max<uint64_t>(b->cap_pool, (uint64_t)64 * B + 4096);
+    b->cap_pool = std::max<uint64_t>(16384, (uint64_t)8 * Bc * stride);
+    if (stride_override == 1) b->cap_pool = std::max<uint64_t>(b->cap_pool, (uint64_t)64 * Bc + 4096);
     PV_TRY(b->d_gmin.ensure((size_t)nqp * b->ngroups * 8));
     PV_TRY(b->d_pool.ensure((size_t)nq * b->cap_pool * 8));
 
@@ -2363,9 +2517,21 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
         A.qi8 = b->d_qi8.as<int8_t>();
         A.xnorm = g->d_norm8;
         A.qnorm = b->d_qn8.as<float>();
-        A.xn2max = (float)dim * 16384.f;
+        A.xn2max = (float)g->dimp * 16384.f;
     }
-    const size_t shm_i8 = (size_t)2 * (8 / pt) * 64 * 16 + sizeof(CandLds);
+    A.dimp = g->dimp;
+    /* INT8: the queries' code operands, dimp x 64 bytes */
+    const size_t shm_i8 = (size_t)2 * (g->dimp / 32) * 64 * 16 + sizeof(CandLds);
+    if (i8 && shm_i8 > 64 * 1024)
+        for (const void *f : {(const void *)k_scan_i8w<true, 8, false>,
+                              (const void *)k_scan_i8w<false, 8, false>,
+                              (const void *)k_scan_i8w<true, 8, true>,
+                              (const void *)k_scan_i8w<false, 8, true>,
+                              (const void *)k_scan_i8w<true, 4, false>,
+                              (const void *)k_scan_i8w<false, 4, false>,
+                              (const void *)k_scan_i8w<true, 4, true>,
+                              (const void *)k_scan_i8w<false, 4, true>})
+            PV_TRY(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm_i8));
 
     /* 4 waves per WG up to 256 dims (2 WGs per CU), 8 waves above (the
      * query operands then take most of the LDS) */
@@ -2414,6 +2580,7 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
         T.gmin = b->d_gmin.as<uint64_t>();
         T.ngroups = b->ngroups;
         T.B = B;
+        T.over = over;
         T.eps = qa.eps;
         T.tau = qa.tau;
         T.pcnt = b->d_pcnt.as<uint32_t>();
@@ -2453,7 +2620,7 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
     ev_record(g, 4);
     hipLaunchKernelGGL(k_select, dim3(nq), dim3(256), dim * 4, s,
                        b->d_pool.as<uint64_t>(), b->d_pcnt.as<uint32_t>(),
-                       b->cap_pool, B, qa.eps, g->d_vecs, b->d_q, dim,
+                       b->cap_pool, B, over, qa.eps, g->d_vecs, b->d_q, dim,
                        b->d_out.as<FvHit>(), b->out_stride,
                        b->d_cnt.as<uint32_t>(), b->d_flag.as<uint8_t>(), qa.status, skip,
                            b->d_topbq_p);

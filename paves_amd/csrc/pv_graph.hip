@@ -82,7 +82,7 @@ struct GraphArgs {
     uint32_t out_k;       /* hits written per query (stride); 0 = ef */
     uint64_t *spill;     /* [nq][scap + fcap] frontier spill stores */
     uint32_t scap;       /* spilled entries a store accepts */
-    uint32_t *dbg;       /* [nq][4] iterations, visited, results, frontier entries lost */
+    uint32_t *dbg;       /* [nq][4] heads expanded, visited, results, frontier entries lost */
 };
 
 __device__ __forceinline__ uint32_t hash_slot(uint32_t node) {
@@ -502,7 +502,7 @@ void k_graph(GraphArgs A) {
     bool pruned = false;        /* dead frontier entries were discarded */
     uint32_t nO = 0, lost = 0;  /* spilled entries; live entries beyond the store */
     uint64_t omin = ~0ull;      /* smallest spilled key */
-    uint32_t iters = 0;
+    uint32_t iters = 0, heads = 0;
     const int nbs = A.nbr_stride;
 
     for (;;) {
@@ -543,6 +543,7 @@ void k_graph(GraphArgs A) {
             const uint64_t okb = __ballot(ok);
             nh = __ffsll((unsigned long long)~okb) - 1;   /* okb bit 0 is set */
         }
+        heads += (uint32_t)nh;
 
         /* neighbour slot t of the popped heads, then its visited test-and-set */
         uint32_t cand = kNoNode;
@@ -660,7 +661,7 @@ void k_graph(GraphArgs A) {
         A.out_exh[q] = (uint8_t)exh;
         A.status[q] = 0;
         if (A.dbg) {
-            A.dbg[(uint64_t)q * 4 + 0] = iters;
+            A.dbg[(uint64_t)q * 4 + 0] = heads;
             A.dbg[(uint64_t)q * 4 + 1] = (uint32_t)min<uint64_t>(visited, 0xFFFFFFFFull);
             A.dbg[(uint64_t)q * 4 + 2] = (uint32_t)nR;
             A.dbg[(uint64_t)q * 4 + 3] = lost;

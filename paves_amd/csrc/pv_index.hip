@@ -17,6 +17,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <vector>
 #include <mutex>
 
 #include "pv_device.h"
@@ -110,29 +111,77 @@ __global__ __launch_bounds__(256) void k_derive_rows(
     }
 }
 
-/* INT8 copy for SQ8 indexes over integer data: x - centre in the
- * MFMA-tiled int8 layout of v_mfma_i32_32x32x32_i8 — tiles of 32 rows,
- * k-steps of 32 dims as 1 KB chunks, slot l (16 bytes) holding row
- * (l & 31), dims 32 s + 16 (l >> 5) .. +16 — and ||x - centre||^2 (exact:
- * an integer below 2^24). One wave per row. */
+/* float <-> order-preserving uint (atomicMin / atomicMax on floats) */
+__device__ __forceinline__ uint32_t ord_of(float f) {
+    const uint32_t b = __float_as_uint(f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+static inline float float_of_ord(uint32_t u) {
+    const uint32_t b = (u & 0x80000000u) ? (u & 0x7fffffffu) : ~u;
+    float f;
+    memcpy(&f, &b, 4);
+    return f;
+}
+
+/* per-dim minimum and maximum over the rows (order-preserving encodings;
+ * lo starts at all ones, hi at zero). Thread t of a block owns dims t,
+ * t + 256, ... of a contiguous row range. */
+__global__ __launch_bounds__(256) void k_dim_range(const float *__restrict__ v, uint64_t count,
+                                                   int dim, uint64_t rows_per_block,
+                                                   uint32_t *__restrict__ lo,
+                                                   uint32_t *__restrict__ hi) {
+    const uint64_t r0 = (uint64_t)blockIdx.x * rows_per_block;
+    const uint64_t r1 = min(count, r0 + rows_per_block);
+    for (int d = threadIdx.x; d < dim; d += blockDim.x) {
+        float mn = __uint_as_float(0x7f800000u), mx = -mn;
+        for (uint64_t r = r0; r < r1; r++) {
+            const float x = v[r * dim + d];
+            mn = fminf(mn, x);
+            mx = fmaxf(mx, x);
+        }
+        if (r0 < r1) {
+            atomicMin(&lo[d], ord_of(mn));
+            atomicMax(&hi[d], ord_of(mx));
+        }
+    }
+}
+
+/* INT8 copy of an SQ8 index in the MFMA-tiled int8 layout of
+ * v_mfma_i32_32x32x32_i8 — tiles of 32 rows, k-steps of 32 dims as 1 KB
+ * chunks, slot l (16 bytes) holding row (l & 31), dims 32 s + 16 (l >> 5)
+ * .. +16 — over dimp dims (zero codes past dim), and the squared code norm
+ * (an integer below 2^24, exact in FP32). Codes: x - centre (exact mode,
+ * mid == nullptr) or clamp(rint((x - mid[d]) * inv), -127, 127). One wave
+ * per row. */
 __global__ __launch_bounds__(256) void k_derive_i8(const float *__restrict__ v, uint64_t count,
-                                                   uint64_t count_pad, int dim, int centre,
-                                                   int8_t *__restrict__ vi8,
+                                                   uint64_t count_pad, int dim, int dimp,
+                                                   int centre, const float *__restrict__ mid,
+                                                   float inv, int8_t *__restrict__ vi8,
                                                    float *__restrict__ norm8) {
+#pragma clang fp contract(off)
     const int l = threadIdx.x & 63;
     const uint64_t row = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (row >= count_pad) return;
-    const uint64_t tbase = (row >> 5) * 32ull * dim;
-    float ss = 0.f;
-    for (int j = l; j < dim; j += 64) {
-        const float x = (row < count ? v[row * dim + j] : 0.f) - (float)centre;
+    const uint64_t tbase = (row >> 5) * 32ull * dimp;
+    int32_t ss = 0;
+    for (int j = l; j < dimp; j += 64) {
+        int code = 0;
+        if (row < count && j < dim) {
+            const float x = v[row * dim + j];
+            if (mid == nullptr) {
+                code = (int)(x - (float)centre);
+            } else {
+                const float c = rintf((x - mid[j]) * inv);
+                code = (int)fminf(fmaxf(c, -127.f), 127.f);
+            }
+        }
         const int s = j >> 5, g = (j >> 4) & 1;
-        vi8[tbase + (uint64_t)s * 1024 + ((row & 31) + 32 * g) * 16 + (j & 15)] = (int8_t)(int)x;
-        ss = fmaf(x, x, ss);
+        vi8[tbase + (uint64_t)s * 1024 + ((row & 31) + 32 * g) * 16 + (j & 15)] = (int8_t)code;
+        ss += code * code;
     }
 #pragma unroll
-    for (int o = 32; o; o >>= 1) ss += lane_xor(ss, o);
-    if (l == 0) norm8[row] = row < count ? ss : 0.f;
+    for (int o = 32; o; o >>= 1) ss += __shfl_xor(ss, o);
+    if (l == 0) norm8[row] = (float)ss;
 }
 
 /* level-0 lists [cnt, neigh...] (stride l0_stride) -> padded rows of
@@ -183,6 +232,7 @@ extern "C" void fv_gpu_index_free(FvGpuIndex *g) {
     hipFree(g->d_norm);
     hipFree(g->d_vi8);
     hipFree(g->d_norm8);
+    hipFree(g->d_i8mid);
     hipFree(g->d_cols);
     hipFree(g->d_prows);
     hipFree(g->d_nbr);
@@ -311,16 +361,50 @@ static FvGpuIndex *upload_impl(const float *vectors, const double *cols,
     g->xmax_norm = sqrtf(n2max) * (1.0f + 1e-6f);
     g->all_int = hs[2] == 0;
 
-    /* SQ8: an exact INT8 copy when every value is an integer and the values
-     * (0 included) span at most 255, so x - centre fits int8; dot products
-     * of D <= 256 such values and the norms stay below 2^24 (exact in FP32) */
-    if (precision == FV_GPU_PREC_SQ8 && g->all_int && g->vmax - g->vmin <= 255.0f &&
-        (dim == 128 || dim == 256)) {
-        g->i8_center = (int)g->vmin + 128;
-        UP_TRY(hipMalloc(&g->d_vi8, count_pad * (size_t)dim));
+    /* SQ8: an INT8 copy. Exact codes x - centre when every value is an
+     * integer, the values (0 included) span at most 255 and dim <= 256
+     * (dot products and norms stay below 2^24, exact in FP32); otherwise,
+     * up to 1024 dims, quantized codes of the per-dim midrange-centred
+     * values under one scale (the largest half-range maps to 127). */
+    if (precision == FV_GPU_PREC_SQ8 && dim <= 1024) {
+        const bool exact = g->all_int && g->vmax - g->vmin <= 255.0f && dim <= 256;
+        const int dimp = dim <= 64 ? 64 : dim <= 128 ? 128 : (dim + 255) / 256 * 256;
+        g->dimp = dimp;
+        g->i8_mode = exact ? 1 : 2;
+        if (exact) {
+            g->i8_center = (int)g->vmin + 128;
+        } else {
+            uint32_t *d_r = nullptr;
+            UP_TRY(hipMalloc(&d_r, (size_t)dim * 8));
+            UP_TRY(hipMemsetAsync(d_r, 0xFF, (size_t)dim * 4, s));
+            UP_TRY(hipMemsetAsync(d_r + dim, 0, (size_t)dim * 4, s));
+            const uint64_t rpb = (count + 1023) / 1024;
+            hipLaunchKernelGGL(k_dim_range, dim3((unsigned)((count + rpb - 1) / rpb)), dim3(256),
+                               0, s, g->d_vecs, count, dim, rpb, d_r, d_r + dim);
+            std::vector<uint32_t> hr((size_t)dim * 2);
+            hipError_t e = hipGetLastError();
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(hr.data(), d_r, (size_t)dim * 8, hipMemcpyDeviceToHost, s);
+            if (e == hipSuccess) e = hipStreamSynchronize(s);
+            hipFree(d_r);
+            if (e != hipSuccess) UP_FAIL("gpu: SQ8 range: %s", hipGetErrorString(e));
+            std::vector<float> mid((size_t)dimp, 0.f);
+            float hmax = 0.f;
+            for (int d = 0; d < dim; d++) {
+                const float lo = float_of_ord(hr[d]), hi = float_of_ord(hr[dim + d]);
+                mid[d] = (lo + hi) * 0.5f;
+                hmax = fmaxf(hmax, (hi - lo) * 0.5f);
+            }
+            g->i8_inv = hmax > 0.f ? 127.0f / hmax : 1.0f;
+            UP_TRY(hipMalloc(&g->d_i8mid, (size_t)dimp * 4));
+            UP_TRY(hipMemcpyAsync(g->d_i8mid, mid.data(), (size_t)dimp * 4,
+                                  hipMemcpyHostToDevice, s));
+        }
+        UP_TRY(hipMalloc(&g->d_vi8, count_pad * (size_t)dimp));
         UP_TRY(hipMalloc(&g->d_norm8, count_pad * 4));
         hipLaunchKernelGGL(k_derive_i8, dim3((unsigned)((count_pad + 3) / 4)), dim3(256), 0, s,
-                           g->d_vecs, count, count_pad, dim, g->i8_center, g->d_vi8, g->d_norm8);
+                           g->d_vecs, count, count_pad, dim, dimp, g->i8_center, g->d_i8mid,
+                           g->i8_inv, g->d_vi8, g->d_norm8);
         UP_TRY(hipGetLastError());
     }
 

@@ -223,7 +223,8 @@ struct FvGpuBatch {
     bool arena_bound = false; /* buffers carved from an arena: keep no host records */
     void *last_scan = nullptr; /* launch record of the last main scan (timing) */
     bool capturing = false; /* a hipGraph capture of this batch's run is open */
-    bool i8_ok = false;     /* every query is an integer within the INT8 window */
+    bool i8_ok = false;     /* the INT8 scan serves every query (mode 2: always;
+                               mode 1: every query an integer in the window) */
     pv::DevBuf d_qi8;       /* [nqp][dim] int8 queries minus the index centre */
     pv::DevBuf d_qn8;       /* [nqp] ||q - centre||^2 */
     bool query_exact = true;
@@ -302,11 +303,21 @@ struct FvGpuIndex {
     uint32_t *d_prows = nullptr;
     uint32_t *d_nbr = nullptr;
     uint32_t *d_ones = nullptr;  /* all-ones bitmap row [count_pad/32] */
-    /* INT8 copy (SQ8 indexes over integer data spanning <= 255 values):
-     * x - i8_center in the MFMA-tiled int8 layout, and ||x - i8_center||^2 */
+    /* INT8 copy of an SQ8 index, in the MFMA-tiled int8 layout over dimp
+     * dims (dim rounded up to 64, 128 or a multiple of 256; zero codes past
+     * dim), and the squared code norms. i8_mode 1 (exact): integer data
+     * spanning <= 255 values, up to 256 dims — codes x - i8_center, the
+     * scan is exact. i8_mode 2 (quantized): any other data up to 1024 dims
+     * — codes clamp(rint((x - i8_mid[d]) * i8_inv), -127, 127) with the
+     * per-dim midrange i8_mid and one scale; the scan ranks by the code
+     * distance and the top overfetch x k rows are re-ranked exactly. */
     int8_t *d_vi8 = nullptr;
     float *d_norm8 = nullptr;
     int i8_center = 0;
+    int i8_mode = 0;
+    int dimp = 0;
+    float *d_i8mid = nullptr;   /* [dimp] (mode 2) */
+    float i8_inv = 1.f;
 
     float xmax_norm = 0.f;   /* max ||x|| */
     float vmax_abs = 0.f;    /* max |x_i| */

@@ -28,6 +28,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <random>
 #include <vector>
 
@@ -831,7 +832,7 @@ static int setup_view(FvGpuIndex *g, FvGpuBatch *b, char *errbuf, size_t errlen)
     PV_TRY(v->d_qbf.ensure((size_t)nqp * qbf_dim(g) * 2));
     PV_TRY(v->d_qaux.ensure((size_t)nqp * 16));
     if (g->d_vi8 != nullptr) {
-        PV_TRY(v->d_qi8.ensure((size_t)nqp * dim));
+        PV_TRY(v->d_qi8.ensure((size_t)nqp * g->dimp));
         PV_TRY(v->d_qn8.ensure((size_t)nqp * 4));
     }
     v->out_stride = b->topb;
@@ -866,6 +867,9 @@ extern "C" int fv_gpu_batch_run_learned(FvGpuIndex *g, FvGpuBatch *b, FvGpuModel
                 FV_GPU_MAX_EF);
     FvGpuLearnedOpts oc = *o;
     if (o->k > FV_GPU_MAX_EF && o->mode == 0) oc.mode = 3;
+    /* no logit exceeds a bias of +inf: every query takes brute force, so
+     * the run is the brute pipeline alone (no features, model or packing) */
+    if (o->mode == 0 && std::isinf(o->route_bias) && o->route_bias > 0.f) oc.mode = 3;
     o = &oc;
     const bool need_model = o->mode == 0 || o->mode == 1;
     if (need_model && m == nullptr) PV_FAIL("gpu: learned run: no model");

@@ -404,7 +404,7 @@ class GpuIndex:
         if wmax is None and not debug:
             rc = self.lib.fv_gpu_graph_batch(*args, ptr(out), ptr(cnt), ptr(exh), eb, len(eb))
         else:
-            rc = self.lib.fv_gpu_graph_batch_ex(*args, wmax or 4, fcap, ptr(out), ptr(cnt),
+            rc = self.lib.fv_gpu_graph_batch_ex(*args, wmax or 8, fcap, ptr(out), ptr(cnt),
                                                 ptr(exh), ptr(dbg), eb, len(eb))
         if rc != 0:
             raise GpuError(eb.value.decode())

@@ -324,7 +324,7 @@ def calibrate_bias(index, model, queries, sers, entries, truth, k=10, target=0.9
     y = model.predict(features_of(model, index, queries, sers))
     table, best = [], None
     for f in fracs:
-        bias = 1e30 if f == 0.0 else float(np.quantile(y[:, 1], 1.0 - f))
+        bias = float("inf") if f == 0.0 else float(np.quantile(y[:, 1], 1.0 - f))
         for sc in ((scale,) if f == 0.0 or not scales else scales):
             qps, rec, gfrac, _ = run_mode(index, model, queries, sers, entries, truth,
                                           LL.LearnedOpts(k=k, mode=0, ef_scale=sc,
@@ -334,5 +334,5 @@ def calibrate_bias(index, model, queries, sers, entries, truth, k=10, target=0.9
             if rec >= target and (best is None or qps > best[3]):
                 best = table[-1]
     if best is None:
-        return 1e30, scale, table
+        return float("inf"), scale, table
     return best[2], best[1], table

@@ -1,12 +1,13 @@
 """The SQ8 precision: an exact INT8 MFMA scan for integer data.
 
 An index uploaded with FV_GPU_PREC_SQ8 over integer values spanning at most
-255 (and dim 128 or 256) keeps an int8 copy of x - centre; batches whose
+255 (and at most 256 dims) keeps an int8 copy of x - centre; batches whose
 queries are integers inside the same window scan it with
 v_mfma_i32_32x32x32_i8 (exact dot products, exact distances). Every result
 must equal the oracle's exact top-k bit for bit, like the BF16 and FP32
-scans. Batches or indexes outside those conditions take the BF16 scan
-(reported precision BF16) and are still exact.
+scans. Batches whose queries fall outside the window take the BF16 scan
+(reported precision BF16) and are still exact. Data outside the window
+gets the quantized copy instead (test_gpu_sq8_quant.py).
 """
 import numpy as np
 import pytest
@@ -86,9 +87,10 @@ def test_queries_outside_window_take_bf16():
         g.close()
 
 
-def test_wide_range_index_has_no_int8_copy():
-    """Values spanning more than 255 cannot shift into int8: SQ8 requests
-    scan in BF16 (exact after the certified re-rank)."""
+def test_wide_range_index_takes_quantized_codes():
+    """Values spanning more than 255 cannot shift into int8 exactly: the
+    index keeps quantized codes and SQ8 requests equal the oracle's
+    quantized path; BF16 requested on the same index stays exact."""
     n, nq = 30_000, 16
     d, cols, post = _data(n, nq, 128, 401)
     base = d["base"].copy()
@@ -97,10 +99,16 @@ def test_wide_range_index_has_no_int8_copy():
     try:
         b = E.concat([E.serialize(None, n)] * nq, n)
         out, cnt, trunc = g.brute(d["queries"], b, 10, L.BruteOpts(route=1))
+        assert g.stats().precision == L.FV_GPU_PREC_SQ8
+        oh, oc, ot = O.sq8_brute_batch(base, cols, d["queries"], b.qd, b.nodes, b.children,
+                                       b.invals, 10, 4)
+        assert_same(out, cnt, trunc, oh, oc, ot, "wide range, quantized")
+        out, cnt, trunc = g.brute(d["queries"], b, 10,
+                                  L.BruteOpts(route=1, precision=L.FV_GPU_PREC_BF16))
         assert g.stats().precision == L.FV_GPU_PREC_BF16
-        oh, oc, ot, _ = O.brute_batch(base, cols, d["queries"], b.qd, b.nodes, b.children,
+        eh, ec, et, _ = O.brute_batch(base, cols, d["queries"], b.qd, b.nodes, b.children,
                                       b.invals, 10)
-        assert_same(out, cnt, trunc, oh, oc, ot, "wide range")
+        assert_same(out, cnt, trunc, eh, ec, et, "wide range, BF16")
     finally:
         g.close()
 
