@@ -5,7 +5,8 @@ Layout and loss as in include/paves_gpu.h (fv_gpu_model_*): parameters
 W1[64][F] b1[64] W2[64][64] b2[64] W3[2][64] b3[2]; inputs standardised
 x_hat = (x - mean) / std; h1 = relu(W1 x_hat + b1); h2 = relu(W2 h1 + b2);
 y = W3 h2 + b3; per-sample loss valid (y0 - t0)^2 + lambda (softplus(y1)
-- t1 y1); mean over samples. Pinned by central finite differences
+- t1 y1); mean over samples. Pinned by central finite differences and
+hand-worked known answers for forward, loss and features
 (tests/test_learned_oracle.py).
 """
 from __future__ import annotations

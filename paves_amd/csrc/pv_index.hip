@@ -37,31 +37,43 @@ double ev_elapsed_us(FvGpuIndex *g, int a, int b) {
 namespace {
 
 constexpr int kMaxDevices = 32;
-int dev_refcount[kMaxDevices];
-std::mutex dev_mu;
+int indexes_on[kMaxDevices];   /* live indexes per device */
+std::mutex indexes_mu;
 
-/* device with the most free memory above need + 512 MB (gpu_engine.cu:1518-1545).
- * Unlike the CUDA original this does not reset other devices: HIP contexts
- * are per-process primary contexts and resetting one a caller still uses
- * would invalidate its allocations. */
+constexpr size_t kPickMargin = (size_t)512 << 20;   /* working memory kept free */
+
+/* Device for an index uploaded with device = -1: among the devices with
+ * room for `need` plus kPickMargin, the one with the most free HBM (lower
+ * ordinal on a tie) — the ABI's contract for -1 (gpu_engine.cu:1518-1545).
+ * Free memory is read with each device current; the chosen device is left
+ * current, or the caller's when none fits. No device is reset: HIP's
+ * primary contexts are shared with the rest of the process. */
 int pick_device(size_t need) {
-    int n = 0;
-    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return -1;
-    const size_t margin = (size_t)512 << 20;
-    int best = -1;
-    size_t bestfree = 0;
-    for (int i = 0; i < n; i++) {
-        if (hipSetDevice(i) != hipSuccess) continue;
-        size_t f = 0, t = 0;
-        if (hipMemGetInfo(&f, &t) != hipSuccess) continue;
-        if (f > bestfree && f > need + margin) {
-            bestfree = f;
-            best = i;
-        }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return -1;
+    int prev = 0;
+    (void)hipGetDevice(&prev);
+    struct Fit {
+        size_t free;
+        int dev;
+    };
+    std::vector<Fit> fits;
+    for (int d = 0; d < ndev; d++) {
+        size_t fr = 0, tot = 0;
+        if (hipSetDevice(d) == hipSuccess && hipMemGetInfo(&fr, &tot) == hipSuccess &&
+            fr >= need + kPickMargin)
+            fits.push_back({fr, d});
     }
-    if (best >= 0) hipSetDevice(best);
     (void)hipGetLastError();
-    return best;
+    if (fits.empty()) {
+        (void)hipSetDevice(prev);
+        return -1;
+    }
+    const Fit best = *std::max_element(fits.begin(), fits.end(), [](const Fit &a, const Fit &b) {
+        return a.free < b.free || (a.free == b.free && a.dev > b.dev);
+    });
+    (void)hipSetDevice(best.dev);
+    return best.dev;
 }
 
 /* fp32 -> MFMA-tiled bf16 copy and ||x||^2, plus the largest positive and
@@ -247,9 +259,9 @@ extern "C" void fv_gpu_index_free(FvGpuIndex *g) {
     for (hipStream_t bs : g->bstreams)
         if (bs) hipStreamDestroy(bs);
     {
-        std::lock_guard<std::mutex> lk(dev_mu);
-        if (g->device >= 0 && g->device < kMaxDevices && dev_refcount[g->device] > 0)
-            dev_refcount[g->device]--;
+        std::lock_guard<std::mutex> lk(indexes_mu);
+        if (g->device >= 0 && g->device < kMaxDevices && indexes_on[g->device] > 0)
+            indexes_on[g->device]--;
     }
     delete g;
 }
@@ -307,8 +319,8 @@ static FvGpuIndex *upload_impl(const float *vectors, const double *cols,
     g->l0_stride = l0 ? l0_stride : 0;
     UP_TRY(hipSetDevice(device));
     {
-        std::lock_guard<std::mutex> lk(dev_mu);
-        if (device < kMaxDevices) dev_refcount[device]++;
+        std::lock_guard<std::mutex> lk(indexes_mu);
+        if (device < kMaxDevices) indexes_on[device]++;
     }
     {
         hipDeviceProp_t prop;
