@@ -258,12 +258,17 @@ int fv_gpu_brute_batch_ex(FvGpuIndex *g, int nq, const float *queries,
                           uint32_t *out_cnt, uint8_t *truncated, char *errbuf,
                           size_t errlen);
 
-/* fv_gpu_graph_batch with search knobs: wmax = frontier heads expanded per
- * iteration (1 = the CPU cursor's one-at-a-time order), fcap = frontier
- * capacity (64..1024), dbg = optional host array [nq][4] receiving per
- * query the frontier heads expanded (each reads one neighbour list), the
- * nodes visited (each reads one vector row), the results, and the live
- * frontier entries lost (0 while the spill store holds them). */
+/* fv_gpu_graph_batch with search knobs (no reference counterpart): wmax =
+ * frontier heads expanded per iteration (1 = the CPU cursor's
+ * one-at-a-time order), fcap = LDS frontier capacity (64..1024; entries
+ * beyond it go to a per-query spill store in HBM and come back in order, so
+ * no entry the CPU cursor's unbounded heap would pop is lost; a negative
+ * fcap bounds the frontier at -fcap entries with no spill store, dropping
+ * the overflow as the reference GPU kernel does), dbg = optional host array
+ * [nq][4] receiving per query the frontier heads expanded (each reads one
+ * neighbour list), the nodes visited (each reads one vector row), the
+ * results, and the live frontier entries lost. fv_gpu_graph_batch is
+ * wmax 8, fcap 1024. */
 int fv_gpu_graph_batch_ex(FvGpuIndex *g, int nq, const float *queries,
                           const FvGpuQueryDesc *qd, const FvGpuPredNode *nodes,
                           int nnodes, const int32_t *children, int nchildren,
@@ -340,9 +345,13 @@ int fv_gpu_batch_tune(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *base,
  * for it. Fails when the batch takes the span-gather route. */
 int fv_gpu_batch_time_scan(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *opts,
                            int reps, double *us, char *errbuf, size_t errlen);
-/* Enqueue the graph pipeline for a staged batch (asynchronous). */
+/* Enqueue the graph pipeline for a staged batch (asynchronous); _ex with
+ * the knobs of fv_gpu_graph_batch_ex. */
 int fv_gpu_batch_run_graph(FvGpuIndex *g, FvGpuBatch *b, uint32_t ef,
                            uint64_t max_visited, char *errbuf, size_t errlen);
+int fv_gpu_batch_run_graph_ex(FvGpuIndex *g, FvGpuBatch *b, uint32_t ef,
+                              uint64_t max_visited, int wmax, int fcap, char *errbuf,
+                              size_t errlen);
 /* Wait for every stream of the index. */
 int fv_gpu_sync(FvGpuIndex *g, char *errbuf, size_t errlen);
 /* Copy a batch's results to the host (synchronous). */

@@ -51,7 +51,7 @@ EXT_SYMBOLS = ["fv_gpu_set_timing", "fv_gpu_last_stats", "fv_gpu_brute_batch_ex"
                "fv_gpu_model_train", "fv_gpu_model_predict", "fv_gpu_batch_features",
                "fv_gpu_batch_run_learned", "fv_gpu_batch_routes", "fv_gpu_batch_stage_ex",
                "fv_gpu_batch_tune", "fv_gpu_batch_time_scan", "fv_gpu_brute_batch_noalloc",
-               "fv_gpu_batch_restage"]
+               "fv_gpu_batch_restage", "fv_gpu_batch_run_graph_ex"]
 
 
 class BruteOpts(C.Structure):
@@ -152,6 +152,8 @@ def load() -> C.CDLL:
                                       C.POINTER(BruteOpts), C.POINTER(C.c_double),
                                       C.POINTER(C.c_int), C.c_char_p, sz]
     lib.fv_gpu_batch_run_graph.argtypes = [_P, _P, C.c_uint32, C.c_uint64, C.c_char_p, sz]
+    lib.fv_gpu_batch_run_graph_ex.argtypes = [_P, _P, C.c_uint32, C.c_uint64, C.c_int, C.c_int,
+                                              C.c_char_p, sz]
     lib.fv_gpu_sync.argtypes = [_P, C.c_char_p, sz]
     lib.fv_gpu_batch_fetch.argtypes = [_P, _P, _P, _P, _P, C.c_char_p, sz]
     lib.fv_gpu_batch_free.argtypes = [_P]

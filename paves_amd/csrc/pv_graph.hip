@@ -685,6 +685,12 @@ int run_graph_ex(FvGpuIndex *g, FvGpuBatch *b, uint32_t ef, uint64_t max_visited
     if (g->nbr_stride > kMaxNew) PV_FAIL("gpu: graph degree %d too large", g->nbr_stride);
     if (wmax < 1) wmax = 1;
     while (wmax > 1 && wmax * g->nbr_stride > kMaxNew) wmax--;
+    /* fcap < 0: a bounded frontier of -fcap entries with no spill store —
+     * live entries beyond it are dropped (counted in debug word 3), as in
+     * the reference GPU kernel's fixed frontier (gpu_engine.cu:1396-1420);
+     * deep filtered searches then end sooner at some cost in recall */
+    const bool spill = fcap > 0;
+    if (fcap < 0) fcap = -fcap;
     /* the LDS frontier: a power of two in [64, 1024] (its refill sorts up
      * to half of it with a bitonic network) */
     fcap = std::min(1024, std::max(64, fcap));
@@ -698,9 +704,9 @@ int run_graph_ex(FvGpuIndex *g, FvGpuBatch *b, uint32_t ef, uint64_t max_visited
      * min(max_visited, count) entries (plus one iteration's overshoot of
      * max_visited) never overflow; the total is capped at 4 GiB, beyond
      * which entries are lost and counted (debug word 3) */
-    uint64_t scap = std::min<uint64_t>(max_visited, g->count) + kMaxNew;
+    uint64_t scap = spill ? std::min<uint64_t>(max_visited, g->count) + kMaxNew : 0;
     const uint64_t kStoreBytes = (uint64_t)4 << 30;
-    if ((uint64_t)b->nq * (scap + fcap) * 8 > kStoreBytes)
+    if (spill && (uint64_t)b->nq * (scap + fcap) * 8 > kStoreBytes)
         scap = std::max<uint64_t>(kStoreBytes / ((uint64_t)b->nq * 8), (uint64_t)fcap + 64) - fcap;
     PV_TRY(b->d_fspill.ensure((size_t)b->nq * (scap + fcap) * 8));
     const uint32_t ostride = x.out_k ? x.out_k : ef;
@@ -864,6 +870,18 @@ extern "C" int fv_gpu_batch_run_graph(FvGpuIndex *g, FvGpuBatch *b, uint32_t ef,
     if (g == nullptr || b == nullptr || b->owner != g) PV_FAIL("gpu: bad batch");
     try {
         return run_graph(g, b, ef, max_visited, errbuf, errlen);
+    } catch (...) {
+        snprintf(errbuf, errlen, "gpu: internal error");
+        return -1;
+    }
+}
+
+extern "C" int fv_gpu_batch_run_graph_ex(FvGpuIndex *g, FvGpuBatch *b, uint32_t ef,
+                                         uint64_t max_visited, int wmax, int fcap,
+                                         char *errbuf, size_t errlen) {
+    if (g == nullptr || b == nullptr || b->owner != g) PV_FAIL("gpu: bad batch");
+    try {
+        return run_graph_opts(g, b, ef, max_visited, wmax, fcap, nullptr, errbuf, errlen);
     } catch (...) {
         snprintf(errbuf, errlen, "gpu: internal error");
         return -1;

@@ -77,49 +77,61 @@ int pick_device(size_t need) {
 }
 
 /* fp32 -> MFMA-tiled bf16 copy and ||x||^2, plus the largest positive and
- * negative coordinate, max ||x||^2 and an "all integers" flag. One wave per
- * row. The copy has dimb >= dim dims (a multiple of 128); dims past dim
- * hold zeros, which add exactly nothing to a dot product. */
+ * negative coordinate, max ||x||^2 and an "all integers" flag. Each wave
+ * walks rows wave, wave + waves, ... (one row at a time, lanes over its
+ * dims) and folds its rows into wave maxima, so the index-wide statistics
+ * take one set of atomics per wave. The copy has dimb >= dim dims (a
+ * multiple of 128); dims past dim hold zeros, which add exactly nothing to
+ * a dot product. */
 __global__ __launch_bounds__(256) void k_derive_rows(
     const float *__restrict__ v, uint64_t count, uint64_t count_pad, int dim, int dimb,
     uint16_t *__restrict__ vbf, float *__restrict__ norm,
-    unsigned int *__restrict__ stats /* [0]=max|x| bits, [1]=max||x||^2 bits, [2]=nonint */) {
+    unsigned int *__restrict__ stats /* [0]=max x, [1]=max ||x||^2, [2]=nonint, [3]=max -x */) {
     const int l = threadIdx.x & 63;
-    const uint64_t row = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (row >= count_pad) return;
-    float ss = 0.f, pmax = 0.f, nmax = 0.f;
-    int nonint = 0;
+    const uint64_t waves = (uint64_t)gridDim.x * 4;
     /* MFMA-tiled BF16 position of (row, j): tile row/32, k-step
      * (j mod dimb/2)/8, lane (row mod 32) + 32 (j div dimb/2), element j mod 8 */
-    const uint64_t tbase = (row >> 5) * 32ull * dimb;
     const int half = dimb / 2;
     const int jend = vbf ? dimb : dim;
-    for (int j = l; j < jend; j += 64) {
-        float x = (row < count && j < dim) ? v[row * dim + j] : 0.f;
-        if (vbf) {
-            const int g = j / half, w = j % half;
-            vbf[tbase + (uint64_t)(w >> 3) * 512 + ((row & 31) + 32 * g) * 8 + (w & 7)] = f2bf(x);
+    float wp = 0.f, wn = 0.f, wss = 0.f;
+    int wni = 0;
+    for (uint64_t row = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6); row < count_pad;
+         row += waves) {
+        float ss = 0.f, pmax = 0.f, nmax = 0.f;
+        int nonint = 0;
+        const uint64_t tbase = (row >> 5) * 32ull * dimb;
+        for (int j = l; j < jend; j += 64) {
+            float x = (row < count && j < dim) ? v[row * dim + j] : 0.f;
+            if (vbf) {
+                const int g = j / half, w = j % half;
+                vbf[tbase + (uint64_t)(w >> 3) * 512 + ((row & 31) + 32 * g) * 8 + (w & 7)] =
+                    f2bf(x);
+            }
+            ss = fmaf(x, x, ss);
+            pmax = fmaxf(pmax, x);
+            nmax = fmaxf(nmax, -x);
+            nonint |= (x != rintf(x)) ? 1 : 0;
         }
-        ss = fmaf(x, x, ss);
-        pmax = fmaxf(pmax, x);
-        nmax = fmaxf(nmax, -x);
-        nonint |= (x != rintf(x)) ? 1 : 0;
-    }
 #pragma unroll
-    for (int o = 32; o; o >>= 1) {
-        ss += lane_xor(ss, o);
-        pmax = fmaxf(pmax, lane_xor(pmax, o));
-        nmax = fmaxf(nmax, lane_xor(nmax, o));
-        nonint |= lane_xor(nonint, o);
+        for (int o = 32; o; o >>= 1) {
+            ss += lane_xor(ss, o);
+            pmax = fmaxf(pmax, lane_xor(pmax, o));
+            nmax = fmaxf(nmax, lane_xor(nmax, o));
+            nonint |= lane_xor(nonint, o);
+        }
+        if (l == 0) norm[row] = row < count ? ss : 0.f;
+        if (row < count) {
+            wp = fmaxf(wp, pmax);
+            wn = fmaxf(wn, nmax);
+            wss = fmaxf(wss, ss);
+            wni |= nonint;
+        }
     }
-    if (l == 0 && row < count) {
-        norm[row] = ss;
-        atomicMax(&stats[0], __float_as_uint(pmax));
-        atomicMax(&stats[1], __float_as_uint(ss));
-        if (nonint) atomicOr(&stats[2], 1u);
-        atomicMax(&stats[3], __float_as_uint(nmax));
-    } else if (l == 0) {
-        norm[row] = 0.f;
+    if (l == 0) {
+        atomicMax(&stats[0], __float_as_uint(wp));
+        atomicMax(&stats[1], __float_as_uint(wss));
+        if (wni) atomicOr(&stats[2], 1u);
+        atomicMax(&stats[3], __float_as_uint(wn));
     }
 }
 
@@ -347,7 +359,8 @@ static FvGpuIndex *upload_impl(const float *vectors, const double *cols,
     unsigned int *d_stats = nullptr;
     UP_TRY(hipMalloc(&d_stats, 16));
     UP_TRY(hipMemsetAsync(d_stats, 0, 16, s));
-    hipLaunchKernelGGL(k_derive_rows, dim3((unsigned)((count_pad + 3) / 4)), dim3(256), 0, s,
+    hipLaunchKernelGGL(k_derive_rows,
+                       dim3((unsigned)std::min<uint64_t>((count_pad + 3) / 4, 4096)), dim3(256), 0, s,
                        g->d_vecs, count, count_pad, dim, dimb, g->d_vbf, g->d_norm, d_stats);
     {
         hipError_t e = hipGetLastError();

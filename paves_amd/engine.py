@@ -501,9 +501,16 @@ class GpuIndex:
             raise GpuError(eb.value.decode())
         return (route, ef, y) if with_y else (route, ef)
 
-    def run_graph(self, bh, ef: int, max_visited: int):
+    def run_graph(self, bh, ef: int, max_visited: int, wmax: int | None = None,
+                  fcap: int | None = None):
+        """Graph search of a staged batch; wmax / fcap as in graph() (a
+        negative fcap bounds the frontier without a spill store)."""
         eb = L.errbuf()
-        rc = self.lib.fv_gpu_batch_run_graph(self.h, bh, ef, max_visited, eb, len(eb))
+        if wmax is None and fcap is None:
+            rc = self.lib.fv_gpu_batch_run_graph(self.h, bh, ef, max_visited, eb, len(eb))
+        else:
+            rc = self.lib.fv_gpu_batch_run_graph_ex(self.h, bh, ef, max_visited, wmax or 8,
+                                                    1024 if fcap is None else fcap, eb, len(eb))
         if rc != 0:
             raise GpuError(eb.value.decode())
 

@@ -109,3 +109,26 @@ def test_default_recall_per_class(world, ef, nq):
     print("recall", ef, nq, means, flush=True)
     for c, m in means.items():
         assert m >= 0.9, (c, ef, nq, means)
+
+
+@pytest.mark.parametrize("ef", [80, 400])
+def test_bounded_frontier(world, ef):
+    """fcap < 0: a 1024-entry frontier without the spill store (the
+    reference GPU kernel's fixed frontier): the output contract holds, the
+    dropped entries are counted, and recall@10 against exact truth stays
+    >= 0.9 on average."""
+    w = world
+    d = w["d"]
+    nq = 1024
+    b = _sub(w, nq)
+    print(f"case config3 bounded ef={ef}", flush=True)
+    out, cnt, exh, dbg = w["g"].graph(d["queries"][:nq], b, ef, max(200000, 400 * ef),
+                                      wmax=8, fcap=-1024, debug=True)
+    rec = []
+    for q in range(nq):
+        if q % 16 == 0:
+            _contract(w, out, cnt, q, ef)
+        got = out[q, : min(10, int(cnt[q]))]["node"]
+        rec.append(O.recall(got, w["th"][q, : w["tc"][q]]["node"]))
+    print("bounded recall", ef, float(np.mean(rec)), "lost", int(dbg[:, 3].sum()), flush=True)
+    assert float(np.mean(rec)) >= 0.9
