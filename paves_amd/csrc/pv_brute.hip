@@ -1062,6 +1062,400 @@ void k_scan_bf16_mq(ScanArgs A) {
 }
 
 /*
+ * Query-resident BF16 scan for batches of many query tiles (dimb 128, main
+ * scan). A WG of kQrNW waves serves kQrNW query tiles, one per wave, whose
+ * MFMA operands stay in VGPRs for the whole launch. The WG's row chunk
+ * streams through a two-slot LDS ring in stages of kQrTps 32-row tiles:
+ * global_load_lds_dwordx4 copies each 1 KB k-step chunk of a tile straight
+ * into LDS (no VGPR round trip; wave w copies chunk w of every tile of the
+ * stage), dword copies bring the stage's ||x||^2 and each wave's filter
+ * words, and the copy of stage s + 1 runs while stage s is computed. Every
+ * wave reads each tile once from LDS (eight 16-byte reads per lane, already
+ * the MFMA operand layout) and runs its MFMAs against its resident queries,
+ * so one HBM read of a tile feeds kQrNW x 17 MFMAs and no operand is
+ * fetched per MFMA.
+ *
+ * The query operands are held scaled by -2 and the accumulators start from
+ * ||x||^2: acc = ||x||^2 - 2 x.q and d = ||q||^2 + acc. Every scaling by -2
+ * is exact, so d is the value k_scan_bf16 computes as ||q||^2 - 2 (x.q -
+ * ||x||^2 / 2); a lane's 16 elements of a query reduce by min before one
+ * comparison with acc <= tau - ||q||^2 (+ margin).
+ *
+ * Filters as one more MFMA: a wave's queries use at most kQrProgs distinct
+ * predicate programs (slots k). B_pen[k][q] = 1 for the slot of query q's
+ * program (0 elsewhere, all 0 for a query without predicate) and
+ * A_pen[r][k] = 2^127 where row r fails program k (0 where it passes), so
+ * one v_mfma_f32_32x32x16_bf16 started from ||x||^2 adds exactly 0 to the
+ * rows a query's filter passes and 2^127 to every other row, which then
+ * never meets the threshold (held below 2^126). A_pen is built from the
+ * tile's kQrProgs filter words (16 VALU per tile for both query blocks)
+ * instead of a per-element select in each block. A wave whose queries use
+ * more programs, and the index's last (partial) tile, seed per element:
+ * ||x||^2 where the row passes, +inf where it does not.
+ *
+ * Candidates: each lane counts its own slots in a register (the wave owns
+ * its 64 queries; lanes l and l + 32 hold the two row halves of a query and
+ * own half of its kQrSlots slots each), so no LDS atomic is on the scan's
+ * path; past its slots a lane spills to the WG's LDS spill area, then to
+ * the query's global pool segment.
+ */
+constexpr int kQrNW = 8;                 /* waves = query tiles per WG */
+constexpr int kQrTps = 4;                /* 32-row tiles per ring stage */
+constexpr int kQrSlots = 16;             /* LDS candidate slots per query */
+constexpr int kQrProgs = 16;             /* filter slots of the penalty MFMA */
+constexpr uint32_t kQrSpill = 512;       /* WG-wide LDS spill slots */
+constexpr uint16_t kBfBig = 0x7F00;      /* 2^127 in bf16 */
+constexpr uint16_t kBfOne = 0x3F80;      /* 1.0 in bf16 */
+
+struct QrCand {
+    uint32_t ovf;
+    uint32_t pad;
+    uint64_t buf[kQT * kQrSlots];        /* query qi: [qi][half][kQrSlots / 2] */
+};
+struct QrShare {
+    uint32_t nsp;
+    uint32_t spq[kQrSpill];   /* query of the spilled key */
+    uint64_t spk[kQrSpill];
+};
+/* dynamic LDS (candidates); the ring is static */
+constexpr size_t kQrLds = kQrNW * sizeof(QrCand) + sizeof(QrShare);
+static_assert(kQrLds + (size_t)2 * kQrTps * (8192 + 128) + (size_t)2 * kQrNW * kQrTps * kQrProgs * 4 <=
+              160 * 1024, "query-resident scan LDS");
+
+/* acc <= h admits every row with d = ||q||^2 + acc <= tau: the margin
+ * 2^-18 (max ||x||^2 + ||q||^2) covers the rounding of tau - ||q||^2 and
+ * of the two ways of forming d. Held below 2^126 so that filtered rows
+ * (2^127 or +inf) never pass; no real row's acc comes near. */
+__device__ __forceinline__ float qr_threshold(const ScanArgs &A, float qn, float tq) {
+    return fminf((tq - qn) + ldexpf(A.xn2max + qn, -18), 0x1p126f);
+}
+
+/* seeds of a tile: ||x||^2 of the lane's 16 accumulator rows where the
+ * query's filter word passes the row, +inf elsewhere */
+__device__ __forceinline__ void seed_qr(const f32x16 &xn, uint32_t w0, uint32_t w1,
+                                        f32x16 &a0, f32x16 &a1) {
+    const uint32_t v0 = lane_word(w0), v1 = lane_word(w1);
+    constexpr uint32_t kPosInf = 0x7f800000u;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        const uint32_t s = __float_as_uint(xn[i]);
+        const uint32_t m0 = elem_mask(v0, i), m1 = elem_mask(v1, i);
+        a0[i] = __uint_as_float((s & m0) | (kPosInf & ~m0));
+        a1[i] = __uint_as_float((s & m1) | (kPosInf & ~m1));
+    }
+}
+
+template <int NW>
+__global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(2)))
+void k_scan_bf16_qr(ScanArgs A) {
+    static_assert(NW == 8, "one 1 KB chunk of every stage tile per wave");
+    static_assert(kQrTps * kQrProgs == 64, "one dword copy of a wave's stage words");
+    /* the ring's two slots are separate arrays, so the compiler can tell
+     * that reads of one slot do not depend on the copy into the other
+     * (no wait for the copy in flight) */
+    __shared__ __attribute__((aligned(16))) u32x4 ring0[kQrTps * 512], ring1[kQrTps * 512];
+    __shared__ __attribute__((aligned(16))) float nring0[kQrTps * 32], nring1[kQrTps * 32];
+    __shared__ __attribute__((aligned(16))) uint32_t wring0[NW * kQrTps * kQrProgs],
+        wring1[NW * kQrTps * kQrProgs];
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    QrCand *Cs = (QrCand *)smem;                                  /* [NW] */
+    QrShare &S = *(QrShare *)(smem + NW * sizeof(QrCand));
+
+    const int tid = threadIdx.x, l = tid & 63, wave = tid >> 6, half = l >> 5;
+    if (scan_pad(A)) return;
+    const int nqt = (A.nq + kQT - 1) / kQT;
+    bool any = false;
+    for (int k = 0; k < NW; k++) {
+        const int t = (int)scan_qgroup(A) * NW + k;
+        any |= t < nqt && !tile_skipped(A, t * kQT);
+    }
+    if (!any) return;   /* uniform: no query tile of this WG has work */
+    const int qbase = ((int)scan_qgroup(A) * NW + wave) * kQT;
+    const bool on = qbase < nqt * kQT && !tile_skipped(A, qbase);
+
+    const uint32_t nwg = scan_nchunks(A), wg = scan_chunk(A);
+    const uint32_t c0 = (uint32_t)(((uint64_t)A.ntiles * wg) / nwg);
+    const uint32_t c1 = (uint32_t)(((uint64_t)A.ntiles * (wg + 1)) / nwg);
+    const uint32_t nst = (c1 - c0 + kQrTps - 1) / kQrTps;
+    const u32x4 *vb = (const u32x4 *)A.vbf;
+
+    /* the wave's queries: operands x -2 in VGPRs, norms, thresholds (-inf
+     * for padding queries) and bitmap rows */
+    bf16x8 bq[2][8];
+    float tq[2], qn[2], hq[2];
+    const uint32_t *wr[2];
+    int32_t prog[2];
+    load_query_state(A, qbase, tq, qn, wr);
+#pragma unroll
+    for (int qb = 0; qb < 2; qb++) {
+        const int q = qbase + (l & 31) + 32 * qb;
+        const bool v = on && q < A.nq;
+        hq[qb] = v ? qr_threshold(A, qn[qb], tq[qb]) : -__builtin_inff();
+        prog[qb] = v ? A.pofq[q] : -1;
+#pragma unroll
+        for (int s = 0; s < 8; s++) {
+            u32x4 raw = {0u, 0u, 0u, 0u};
+            if (v) raw = *(const u32x4 *)(A.qbf + (uint64_t)q * 128 + (l >> 5) * 64 + 8 * s);
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const float lo = -2.f * __uint_as_float(raw[e] << 16);
+                const float hi = -2.f * __uint_as_float(raw[e] & 0xffff0000u);
+                raw[e] = (__float_as_uint(lo) >> 16) | (__float_as_uint(hi) & 0xffff0000u);
+            }
+            bq[qb][s] = __builtin_bit_cast(bf16x8, raw);
+        }
+    }
+    /* filter slots: the wave's distinct programs in order of first lane;
+     * lane j records the program of slot j % kQrProgs (-1: unused) */
+    int nslot = 0;
+    int32_t myprog = -1;
+    int slot_of[2] = {-1, -1};
+    {
+        int32_t rest0 = prog[0], rest1 = prog[1];
+        while (nslot <= kQrProgs) {
+            const uint64_t m0 = __ballot(rest0 >= 0), m1 = __ballot(rest1 >= 0);
+            if ((m0 | m1) == 0) break;
+            const int32_t p = m0 ? __shfl(rest0, __builtin_ctzll(m0))
+                                 : __shfl(rest1, __builtin_ctzll(m1));
+            if (nslot < kQrProgs) {
+                if (prog[0] == p) slot_of[0] = nslot;
+                if (prog[1] == p) slot_of[1] = nslot;
+                if ((l & (kQrProgs - 1)) == nslot) myprog = p;
+            }
+            if (rest0 == p) rest0 = -1;
+            if (rest1 == p) rest1 = -1;
+            nslot++;
+        }
+    }
+    const bool pen = nslot <= kQrProgs;   /* uniform */
+    /* B_pen of the two query blocks: lane l holds k-slots 8 half .. +8 of
+     * query column l & 31 */
+    bf16x8 bpen[2];
+#pragma unroll
+    for (int qb = 0; qb < 2; qb++) {
+        u32x4 raw = {0u, 0u, 0u, 0u};
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+            if (slot_of[qb] == 8 * half + j) raw[j >> 1] |= (uint32_t)kBfOne << (16 * (j & 1));
+        bpen[qb] = __builtin_bit_cast(bf16x8, raw);
+    }
+    const uint32_t *prow = myprog >= 0 ? A.bitmap + (uint64_t)myprog * A.words : A.ones;
+
+    /* copy of stage st into a ring slot: chunk `wave` of each of the
+     * stage's tiles, the stage's norms (two tiles per dword copy) and the
+     * wave's filter words [tile][slot] */
+    auto load_stage = [&](uint32_t st, u32x4 *ring, float *nring, uint32_t *wring) {
+        const uint32_t t0 = c0 + st * kQrTps;
+#pragma unroll
+        for (int i = 0; i < kQrTps; i++) {
+            const uint32_t t = t0 + (uint32_t)i;
+            if (t < c1)
+                __builtin_amdgcn_global_load_lds(vb + (uint64_t)t * A.stride * 512 + wave * 64 + l,
+                                                 ring + i * 512 + wave * 64, 16, 0, 0);
+        }
+        if (wave < kQrTps / 2) {
+            const uint32_t t = min(t0 + 2u * wave + (uint32_t)(l >> 5), c1 - 1);
+            __builtin_amdgcn_global_load_lds(A.xnorm + (uint64_t)t * A.stride * 32 + (l & 31),
+                                             nring + 2 * wave * 32, 4, 0, 0);
+        }
+        if (pen && on) {
+            const uint32_t t = min(t0 + (uint32_t)(l / kQrProgs), c1 - 1);
+            __builtin_amdgcn_global_load_lds(prow + (uint64_t)t * A.stride,
+                                             wring + wave * kQrTps * kQrProgs, 4, 0, 0);
+        }
+    };
+    /* per-lane filter words of a stage's tiles (waves without the penalty
+     * form), one stage ahead in VGPRs */
+    uint32_t wn[kQrTps][2], wc[kQrTps][2];
+    auto load_words = [&](uint32_t st) {
+        if (pen) return;
+        const uint32_t t0 = c0 + st * kQrTps;
+#pragma unroll
+        for (int i = 0; i < kQrTps; i++) {
+            const uint32_t pt = min(t0 + (uint32_t)i, c1 - 1) * A.stride;
+            wn[i][0] = on ? wr[0][pt] : 0u;
+            wn[i][1] = on ? wr[1][pt] : 0u;
+        }
+    };
+    /* stage top: this wave's copies for the stage have landed; after the
+     * barrier every wave's have, and every wave is done with the slot the
+     * next copy overwrites */
+    auto stage_sync = [&]() {
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        __syncthreads();
+        if (!pen)
+#pragma unroll
+            for (int i = 0; i < kQrTps; i++) {
+                wc[i][0] = wn[i][0];
+                wc[i][1] = wn[i][1];
+            }
+    };
+
+    QrCand &C = Cs[wave];
+    uint32_t cnt[2] = {0u, 0u};
+    /* a candidate of block qb: the lane's own slots, then the WG's spill,
+     * then the query's global pool segment */
+    auto insert = [&](int qb, uint64_t key) {
+        const int qi = (l & 31) + 32 * qb;
+        if (cnt[qb] < (uint32_t)(kQrSlots / 2)) {
+            C.buf[qi * kQrSlots + half * (kQrSlots / 2) + cnt[qb]] = key;
+            cnt[qb]++;
+            return;
+        }
+        const int q = qbase + qi;
+        const uint32_t sp = atomicAdd(&S.nsp, 1u);
+        if (sp < kQrSpill) {
+            S.spk[sp] = key;
+            S.spq[sp] = (uint32_t)q;
+            return;
+        }
+        const uint32_t sh = scan_chunk(A) % kPoolShards;
+        const uint64_t scap = shard_cap(A.cap_pool);
+        const uint32_t gp = atomicAdd(&A.pcnt[pcnt_at(q, sh)], 1u);
+        if (gp < scap)
+            A.pool[(uint64_t)q * A.cap_pool + sh * scap + gp] = key;
+        else
+            C.ovf = 1u;
+    };
+
+    auto compute_stage = [&](uint32_t st, const u32x4 *ring, const float *nring,
+                             const uint32_t *wring) {
+#pragma unroll
+        for (int i = 0; i < kQrTps; i++) {
+            const uint32_t t = c0 + st * kQrTps + (uint32_t)i;
+            if (t >= c1) break;
+            const uint32_t pt = t * A.stride;
+            bf16x8 a[8];
+#pragma unroll
+            for (int j = 0; j < 8; j++) a[j] = __builtin_bit_cast(bf16x8, ring[i * 512 + j * 64 + l]);
+            /* rows g * 8 + (l >> 5) * 4 + 0..3 of accumulator group g */
+            const float4 *nr = (const float4 *)(nring + i * 32);
+            f32x16 xn;
+#pragma unroll
+            for (int g = 0; g < 4; g++) {
+                const float4 v = nr[2 * g + half];
+                xn[4 * g] = v.x;
+                xn[4 * g + 1] = v.y;
+                xn[4 * g + 2] = v.z;
+                xn[4 * g + 3] = v.w;
+            }
+            const uint64_t row0 = (uint64_t)pt * 32;
+            f32x16 acc0, acc1;
+            if (pen && row0 + 32 <= A.count) {
+                /* A_pen: row l & 31, slots 8 half .. +8: 2^127 where the
+                 * slot's word fails the row */
+                const u32x4 *wp = (const u32x4 *)(wring + wave * kQrTps * kQrProgs + i * kQrProgs +
+                                                  8 * half);
+                const u32x4 w0 = wp[0], w1 = wp[1];
+                const int r = l & 31;
+                u32x4 ap;
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const uint32_t lo = (j < 2 ? w0[2 * j] : w1[2 * j - 4]) >> r;
+                    const uint32_t hi = (j < 2 ? w0[2 * j + 1] : w1[2 * j - 3]) >> r;
+                    /* bit 0 of lo into bit 8 (low bf16 exponent field),
+                     * bit 0 of hi into bit 24: pass -> 0, fail -> 0x7F00 */
+                    const uint32_t pass = (lo & 1u) | ((hi & 1u) << 16);
+                    ap[j] = ((pass ^ 0x00010001u) * 0x7F00u);
+                }
+                acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, ap),
+                                                               bpen[0], xn, 0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, ap),
+                                                               bpen[1], xn, 0, 0, 0);
+            } else {
+                const uint32_t valid =
+                    row0 + 32 <= A.count ? 0xFFFFFFFFu
+                    : (A.count > row0 ? (0xFFFFFFFFu >> (32 - (A.count - row0))) : 0u);
+                const uint32_t w0 = pen ? wr[0][pt] : wc[i][0];
+                const uint32_t w1 = pen ? wr[1][pt] : wc[i][1];
+                seed_qr(xn, w0 & valid, w1 & valid, acc0, acc1);
+            }
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[j], bq[0][j], acc0, 0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[j], bq[1][j], acc1, 0, 0, 0);
+            }
+#pragma unroll
+            for (int qb = 0; qb < 2; qb++) {
+                const f32x16 &c = qb ? acc1 : acc0;
+                const float h = hq[qb];
+                /* minima of elements 3g .. 3g + 2 (g < 5) and element 15,
+                 * as three-input mins (v_min3_f32) */
+                float gm[6];
+#pragma unroll
+                for (int g = 0; g < 5; g++) gm[g] = fminf(fminf(c[3 * g], c[3 * g + 1]), c[3 * g + 2]);
+                gm[5] = c[15];
+                const float mn =
+                    fminf(fminf(fminf(fminf(gm[0], gm[1]), gm[2]), fminf(gm[3], gm[4])), gm[5]);
+                if (mn <= h) {
+#pragma unroll
+                    for (int g = 0; g < 6; g++)
+                        if (gm[g] <= h) {
+#pragma unroll
+                            for (int j = 0; j < (g < 5 ? 3 : 1); j++) {
+                                const int e = 3 * g + j;
+                                if (c[e] <= h)
+                                    insert(qb, make_key(fmaxf(qn[qb] + c[e], 0.f),
+                                                        (uint32_t)(row0 + acc_row(e, l))));
+                            }
+                        }
+                }
+            }
+        }
+    };
+
+    if (l == 0) C.ovf = 0;
+    if (tid == 0) S.nsp = 0;
+    if (nst > 0) {
+        load_stage(0, ring0, nring0, wring0);
+        load_words(0);
+    }
+    /* two stages per iteration: even stages in slot 0, odd in slot 1 */
+    for (uint32_t st = 0; st < nst; st += 2) {
+        stage_sync();
+        if (st + 1 < nst) {
+            load_stage(st + 1, ring1, nring1, wring1);
+            load_words(st + 1);
+        }
+        if (on) compute_stage(st, ring0, nring0, wring0);
+        if (st + 1 >= nst) break;
+        stage_sync();
+        if (st + 2 < nst) {
+            load_stage(st + 2, ring0, nring0, wring0);
+            load_words(st + 2);
+        }
+        if (on) compute_stage(st + 1, ring1, nring1, wring1);
+    }
+
+    /* flush: each lane its own slots of its two queries (one reservation
+     * each), then the shared spill, into pool segment chunk % kPoolShards */
+    __syncthreads();
+    const uint32_t sh = scan_chunk(A) % kPoolShards;
+    const uint64_t scap = shard_cap(A.cap_pool);
+    if (on) {
+#pragma unroll
+        for (int qb = 0; qb < 2; qb++) {
+            const int qi = (l & 31) + 32 * qb, q = qbase + qi;
+            const uint32_t n = min(cnt[qb], (uint32_t)(kQrSlots / 2));
+            if (q < A.nq && n > 0) {
+                const uint32_t base = atomicAdd(&A.pcnt[pcnt_at(q, sh)], n);
+                for (uint32_t j = 0; j < n; j++)
+                    if (base + j < scap)
+                        A.pool[(uint64_t)q * A.cap_pool + sh * scap + base + j] =
+                            C.buf[qi * kQrSlots + half * (kQrSlots / 2) + j];
+            }
+        }
+        if (C.ovf && qbase + l < A.nq) atomicOr(&A.status[qbase + l], kStOverflow);
+    }
+    const uint32_t ns = min(S.nsp, kQrSpill);
+    for (uint32_t i = tid; i < ns; i += blockDim.x) {
+        const uint32_t q = S.spq[i];
+        const uint32_t gp = atomicAdd(&A.pcnt[pcnt_at(q, sh)], 1u);
+        if (gp < scap) A.pool[(uint64_t)q * A.cap_pool + sh * scap + gp] = S.spk[i];
+    }
+}
+
+/*
  * INT8 scan (SQ8 indexes; code dims dimp = 64, 128 or 256). x.q of the
  * int8 codes accumulates in int32 with v_mfma_i32_32x32x32_i8 (exact), and
  * with the squared code norms (integers below 2^24) the epilogue sees the
@@ -2154,6 +2548,18 @@ static hipError_t launch_scan_mq(int qtw, bool inl, uint32_t gx, int gy, hipStre
     return hipGetLastError();
 }
 
+/* query-resident BF16 scan launch (main scan, dim 128, bitmap predicates):
+ * kQrNW query tiles per WG, gy = query groups */
+static hipError_t launch_scan_qr(uint32_t gx, int gy, hipStream_t s, const ScanArgs &A0) {
+    ScanArgs A = A0;
+    const void *f = (const void *)k_scan_bf16_qr<kQrNW>;
+    hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kQrLds);
+    if (e != hipSuccess) return e;
+    const dim3 grid = scan_grid(A, gx, (uint32_t)gy);
+    hipLaunchKernelGGL((k_scan_bf16_qr<kQrNW>), grid, dim3(kQrNW * 64), kQrLds, s, A);
+    return hipGetLastError();
+}
+
 /* Phase 1 of a run: predicate bitmaps (when `bitmap`) and the query
  * preparation for scan precision `prec`, one launch on stream s. */
 hipError_t launch_phase1(FvGpuIndex *g, FvGpuBatch *b, int prec, bool bitmap, hipStream_t s,
@@ -2267,11 +2673,34 @@ static ScanShape scan_shape(const FvGpuIndex *g, const FvGpuBatch *b, const FvGp
     /* two query tiles per 4-wave WG measured fastest for multi-tile
      * batches at every WG count and tile order (1024 queries at 1M x 128:
      * 0.35-0.39 ms against 0.43-0.45 ms with four tiles per 8-wave WG) */
-    int qtw = (o && o->qtw > 0) ? o->qtw : (b->nqt >= 2 ? 2 : 1);
+    /* kQrNW (8): the query-resident kernel, kQrNW query tiles per WG, for
+     * batches of at least kQrNW tiles whose predicates are bitmaps */
+    const bool inl_req = o && o->inline_pred && b->inl_ok && b->inl_col >= 0;
+    const bool qr_ok = prec == FV_GPU_PREC_BF16 && g->dimb == 128 && !inl_req;
+    int qtw = (o && o->qtw > 0) ? o->qtw
+                                : (qr_ok && b->nqt >= kQrNW ? kQrNW : (b->nqt >= 2 ? 2 : 1));
     if (prec != FV_GPU_PREC_BF16 || g->dimb != 128 || b->nqt < 2) qtw = 1;
-    qtw = qtw >= 4 ? 4 : (qtw >= 2 ? 2 : 1);
+    qtw = qtw >= kQrNW ? (qr_ok ? kQrNW : 4) : qtw >= 4 ? 4 : (qtw >= 2 ? 2 : 1);
     sh.qtw = qtw;
     sh.gy = (b->nqt + qtw - 1) / qtw;
+    if (qtw == kQrNW) {
+        /* one WG per CU (the LDS ring and candidate slots fill most of a
+         * CU's LDS); every WG's chunk spans many ring stages */
+        sh.nw = kQrNW;
+        sh.wpc = 1;
+        uint32_t x = (uint32_t)std::max(32, g->ncu / std::max(1, sh.gy));
+        if (o && o->wg_per_cu > 0)
+            x = (uint32_t)std::max(8, (std::min(o->wg_per_cu, 4) * g->ncu) / std::max(1, sh.gy));
+        sh.gx = std::min<uint32_t>(x, std::max<uint32_t>(1, ntiles / (4 * kQrTps)));
+        uint32_t stride = stride_override ? stride_override
+                                          : (o && o->sample_stride ? o->sample_stride : 0);
+        if (stride == 0) {
+            stride = std::min<uint32_t>(64, std::max<uint32_t>(1, (uint32_t)kQrSlots / 2 * sh.gx / B));
+            stride = std::max(stride, (ntiles + 131071) / 131072);
+        }
+        sh.stride = std::min(stride, ntiles);
+        return sh;
+    }
     /* 8 waves per WG (one WG per CU) sharing one copy of the query operands
      * and one tile counter measured best for the single-tile BF16 scan;
      * 4 waves for the two-tile variant */
@@ -2347,6 +2776,9 @@ static hipError_t launch_main_scan(const ScanLaunch &L, hipStream_t s) {
         hipLaunchKernelGGL((k_scan_f32m<false, 8>), dim3(L.gx, L.nqt), dim3(512), L.shm, s, L.M);
     } else if (L.i8) {
         launch_scan_i8<false>(L.nw, L.pt, L.inl, L.gx, L.nqt, L.shm, s, L.M);
+    } else if (L.bf && L.qtw == kQrNW) {
+        const hipError_t e = launch_scan_qr(L.gx, L.gy, s, L.M);
+        if (e != hipSuccess) return e;
     } else if (L.bf && L.qtw > 1) {
         const hipError_t e = launch_scan_mq<false>(L.qtw, L.inl, L.gx, L.gy, s, L.M);
         if (e != hipSuccess) return e;
@@ -2566,6 +2998,8 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
              * much as a few tiles */
             const uint32_t gxp = std::min<uint32_t>(gx, std::max<uint32_t>(1, (stiles + nw - 1) / nw));
             if (i8) launch_scan_i8<true>(nw, pt, inl, gxp, b->nqt, shm_i8, s, P);
+            else if (qtw == kQrNW)   /* the query-resident kernel has no pre-pass form */
+                PV_TRY(launch_scan_mq<true>(2, inl, gxp, (b->nqt + 1) / 2, s, P));
             else if (qtw > 1) PV_TRY(launch_scan_mq<true>(qtw, inl, gxp, gy, s, P));
             else launch_scan_bf16<true>(nw, inl, gxp, b->nqt, shm_bf, s, P);
         } else {
