@@ -599,17 +599,29 @@ def measure_batched_scan(g, stagedL, batchesL, bf, args, reps=5):
     full = [bh for bh, (s, e, _) in zip(stagedL, batchesL) if e - s == args.learned_batch]
     if not full:
         return None
-    g.run_brute(full[0], bf)
-    g.sync()
-    us = g.time_scan(full[0], bf, reps)
     nq = args.learned_batch
     flops = 2.0 * args.n * 128 * nq
-    tf = flops / (us * 1e-6) / 1e12
-    log(f"batched scan ({nq} queries): {us:.1f} us -> {tf:.0f} TFLOP/s")
-    return {"kernel": "k_scan_bf16_mq (query tiles per workgroup: library default)",
+
+    def timed(opts):
+        g.run_brute(full[0], opts)
+        g.sync()
+        us = g.time_scan(full[0], opts, reps)
+        return us, flops / (us * 1e-6) / 1e12
+
+    # the library's default shape for a batch of >= 8 query tiles with
+    # predicate bitmaps is the query-resident kernel (qtw 8)
+    us, tf = timed(L.BruteOpts(precision=bf.precision, route=bf.route, qtw=8))
+    us2, tf2 = timed(L.BruteOpts(precision=bf.precision, route=bf.route, qtw=2))
+    log(f"batched scan ({nq} queries): query-resident {us:.1f} us -> {tf:.0f} TFLOP/s; "
+        f"two tiles per WG {us2:.1f} us -> {tf2:.0f} TFLOP/s")
+    return {"kernel": "k_scan_bf16_qr<8> (query-resident: one 64-query tile per wave, "
+                      "the library default for batches of >= 8 query tiles)",
             "queries": nq, "avg_launch_us": round(us, 2), "flops_per_launch": flops,
             "achieved_TFLOPs": round(tf, 1), "peak_TFLOPs": 2500.0,
-            "frac": round(tf / 2500.0, 4)}
+            "frac": round(tf / 2500.0, 4),
+            "two_tiles_per_wg": {"kernel": "k_scan_bf16_mq<false, 2, 4, false>",
+                                 "avg_launch_us": round(us2, 2), "achieved_TFLOPs": round(tf2, 1),
+                                 "frac": round(tf2 / 2500.0, 4)}}
 
 
 def sync_abi(g, d, batches, args):
@@ -939,7 +951,8 @@ def uneven_path(args):
                    "shapes_tried": ntried, "tune_s": round(tune_s, 2)},
         "roofline": {"bound": "mfma", "achieved": round(tflops, 1), "peak": 2500.0,
                      "unit": "TFLOP/s", "frac": round(tflops / 2500.0, 4), "traffic": None,
-                     "kernel": "k_scan_bf16_mq" if best.qtw > 1 else "k_scan_bf16",
+                     "kernel": ("k_scan_bf16_qr" if best.qtw == 8 else
+                                "k_scan_bf16_mq" if best.qtw > 1 else "k_scan_bf16"),
                      "avg_launch_us": round(scan_us, 1), "flops_per_launch": flops,
                      "data_passes": gy, "hbm_GBs": round(hbm_gbs, 1)},
         "prepass_tau_us": round(pre_us, 1), "select_us": round(sel_us, 1),

@@ -194,8 +194,10 @@ typedef struct FvGpuBruteOpts {
                             BF16 scan instead of building bitmaps (batches
                             whose predicates all compare one column) */
     int32_t wg_per_cu;   /* main scan workgroups per CU and query tile (0 = auto) */
-    int32_t qtw;         /* query tiles of 64 served per workgroup (1, 2, 4;
-                            0 = auto; > 1 needs dim 128 and the BF16 scan) */
+    int32_t qtw;         /* query tiles of 64 served per workgroup (1, 2, 4,
+                            or 8: the query-resident kernel, one tile per
+                            wave, predicate bitmaps only; 0 = auto; > 1
+                            needs dims <= 128 and the BF16 scan) */
     int32_t graph;       /* 1: replay the run as a hipGraph captured for this
                             batch and these options (captured on first use) */
     int32_t order;       /* BF16 / INT8 scan tile order: 0 = auto (3 above

@@ -1264,31 +1264,12 @@ void k_scan_bf16_qr(ScanArgs A) {
                                              wring + wave * kQrTps * kQrProgs, 4, 0, 0);
         }
     };
-    /* per-lane filter words of a stage's tiles (waves without the penalty
-     * form), one stage ahead in VGPRs */
-    uint32_t wn[kQrTps][2], wc[kQrTps][2];
-    auto load_words = [&](uint32_t st) {
-        if (pen) return;
-        const uint32_t t0 = c0 + st * kQrTps;
-#pragma unroll
-        for (int i = 0; i < kQrTps; i++) {
-            const uint32_t pt = min(t0 + (uint32_t)i, c1 - 1) * A.stride;
-            wn[i][0] = on ? wr[0][pt] : 0u;
-            wn[i][1] = on ? wr[1][pt] : 0u;
-        }
-    };
     /* stage top: this wave's copies for the stage have landed; after the
      * barrier every wave's have, and every wave is done with the slot the
      * next copy overwrites */
     auto stage_sync = [&]() {
         asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         __syncthreads();
-        if (!pen)
-#pragma unroll
-            for (int i = 0; i < kQrTps; i++) {
-                wc[i][0] = wn[i][0];
-                wc[i][1] = wn[i][1];
-            }
     };
 
     QrCand &C = Cs[wave];
@@ -1366,9 +1347,9 @@ void k_scan_bf16_qr(ScanArgs A) {
                 const uint32_t valid =
                     row0 + 32 <= A.count ? 0xFFFFFFFFu
                     : (A.count > row0 ? (0xFFFFFFFFu >> (32 - (A.count - row0))) : 0u);
-                const uint32_t w0 = pen ? wr[0][pt] : wc[i][0];
-                const uint32_t w1 = pen ? wr[1][pt] : wc[i][1];
-                seed_qr(xn, w0 & valid, w1 & valid, acc0, acc1);
+                /* per-lane filter words, read here: the tile's words (L2
+                 * hits after the first query tile of the chunk) */
+                seed_qr(xn, wr[0][pt] & valid, wr[1][pt] & valid, acc0, acc1);
             }
 #pragma unroll
             for (int j = 0; j < 8; j++) {
@@ -1406,24 +1387,15 @@ void k_scan_bf16_qr(ScanArgs A) {
 
     if (l == 0) C.ovf = 0;
     if (tid == 0) S.nsp = 0;
-    if (nst > 0) {
-        load_stage(0, ring0, nring0, wring0);
-        load_words(0);
-    }
+    if (nst > 0) load_stage(0, ring0, nring0, wring0);
     /* two stages per iteration: even stages in slot 0, odd in slot 1 */
     for (uint32_t st = 0; st < nst; st += 2) {
         stage_sync();
-        if (st + 1 < nst) {
-            load_stage(st + 1, ring1, nring1, wring1);
-            load_words(st + 1);
-        }
+        if (st + 1 < nst) load_stage(st + 1, ring1, nring1, wring1);
         if (on) compute_stage(st, ring0, nring0, wring0);
         if (st + 1 >= nst) break;
         stage_sync();
-        if (st + 2 < nst) {
-            load_stage(st + 2, ring0, nring0, wring0);
-            load_words(st + 2);
-        }
+        if (st + 2 < nst) load_stage(st + 2, ring0, nring0, wring0);
         if (on) compute_stage(st + 1, ring1, nring1, wring1);
     }
 
@@ -2695,7 +2667,15 @@ static ScanShape scan_shape(const FvGpuIndex *g, const FvGpuBatch *b, const FvGp
         uint32_t stride = stride_override ? stride_override
                                           : (o && o->sample_stride ? o->sample_stride : 0);
         if (stride == 0) {
-            stride = std::min<uint32_t>(64, std::max<uint32_t>(1, (uint32_t)kQrSlots / 2 * sh.gx / B));
+            /* about B x stride candidates per query: each costs the scan a
+             * pass through the insertion path, each halving of the stride
+             * doubles the pre-pass. sqrt(2560 / B) balances the two at 1M x
+             * 128 with 1024 queries (measured optimum 16 at B = 10: 317 us
+             * per batch against 365 us at 64 and 337 us at 8), within the
+             * per-WG slots (B x stride / gx per query) */
+            stride = (uint32_t)std::lround(std::sqrt(2560.0 / std::max<uint32_t>(1, B)));
+            stride = std::min<uint32_t>(stride, std::max<uint32_t>(1, (uint32_t)kQrSlots / 2 * sh.gx / B));
+            stride = std::min<uint32_t>(64, std::max<uint32_t>(1, stride));
             stride = std::max(stride, (ntiles + 131071) / 131072);
         }
         sh.stride = std::min(stride, ntiles);
@@ -3491,12 +3471,16 @@ static int tune_batch(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *base, 
     *ntried = 1;
     if (b->last_route != 1) return 0;   /* gather route: no scan shape to tune */
     const bool mq = prec == FV_GPU_PREC_BF16 && g->dimb == 128 && b->nqt >= 2;
-    const int qtws[3] = {1, 2, 4};
+    /* the query-resident kernel: bitmap predicates, at least kQrNW tiles */
+    const bool qr = mq && b->nqt >= kQrNW &&
+                    !(o0.inline_pred && b->inl_ok && b->inl_col >= 0);
+    const int qtws[4] = {1, 2, 4, kQrNW};
     const bool wv = (prec == FV_GPU_PREC_BF16 || prec == FV_GPU_PREC_SQ8) && g->dim <= 256;
-    for (int qi = 0; qi < (mq ? 3 : 1); qi++)
+    for (int qi = 0; qi < (mq ? (qr ? 4 : 3) : 1); qi++)
         for (int waves = 4; waves <= ((qi == 0 && wv) ? 8 : 4); waves += 4)
-        for (int wpc = 1; wpc <= 4; wpc++)
+        for (int wpc = 1; wpc <= (qi == 3 ? 2 : 4); wpc++)
         for (int order = 1; order <= (b->nqt >= 2 ? 3 : 2); order++) {
+            if (qi == 3 && order == 2) continue;   /* no sweep order: same as chunked */
             FvGpuBruteOpts o = o0;
             o.qtw = qtws[qi];
             o.waves = qi == 0 && wv ? waves : 0;
