@@ -439,7 +439,7 @@ def main():
     # ---- SQ8 (exact INT8 MFMA scan of x - centre) on a second index
     sq8 = sq8_path(d, cols, rows, batches, res_bf, args)
     # ---- SQ8 over non-integer data: quantized codes + exact re-rank
-    sq8q = sq8_quant_path(args)
+    sq8q, fp8q = sq8_quant_path(args)
 
     # ---- the reference's synchronous entry point (host buffers in and out:
     # predicate compile, H2D staging, pipeline, D2H), one pass, for the
@@ -550,6 +550,7 @@ def main():
         "sync_abi_breakdown": sync_detail,
         "sq8": sq8,
         "sq8_quantized": sq8q,
+        "fp8": fp8q,
         "ids_bit_exact_bf16_vs_fp32": exact,
         "oracle_sample_ok": sample_ok,
         "overflow_batches": int(overflow),
@@ -1005,60 +1006,80 @@ def sq8_path(d, cols, rows, batches, res_bf, args):
             "achieved_GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
 
 
+def quant_leg(d, cols, rows, batches, prec, args, nq, truth=None):
+    """One reduced-precision index over the float data: throughput of its
+    scan (SQ8 int8 codes or FP8 E4M3 codes, top 4 k re-ranked exactly),
+    recall@k against the exact BF16 results (computed here when not given),
+    and the code scan kernel's rate (10 back-to-back launches per batch)."""
+    g = E.GpuIndex(d["base"], cols, rows, precision=prec, device=0)
+    staged = [g.stage(d["queries"][s:e], b, args.k) for s, e, b in batches]
+    res = {}
+    legs = [("q", prec)] + ([] if truth is not None else [("bf16", L.FV_GPU_PREC_BF16)])
+    for name, pr in legs:
+        o = L.BruteOpts(precision=pr, route=0)
+        run_steps(g, staged, o, max(1, args.warmup))
+        dt = run_steps(g, staged, o, args.steps)
+        res[name] = (args.steps * nq / dt,
+                     [g.fetch(bh, e - s, args.k) for bh, (s, e, _) in zip(staged, batches)])
+    if truth is None:
+        truth = res["bf16"]
+    rec = []
+    for (o8, c8, _, _), (ob, cb, _, _) in zip(res["q"][1], truth[1]):
+        for q in range(len(c8)):
+            tr = set(int(x) for x in ob[q, : cb[q]]["node"])
+            got = set(int(x) for x in o8[q, : c8[q]]["node"])
+            rec.append(len(tr & got) / len(tr) if tr else 1.0)
+    sq = L.BruteOpts(precision=prec, route=0)
+    scan_us, nprog = [], []
+    for bh in staged[:24]:
+        g.run_brute(bh, sq)
+        g.sync()
+        st = g.stats()
+        if st.route == 1 and st.precision == prec:
+            nprog.append(st.nprog)
+            scan_us.append(g.time_scan(bh, sq, 10))
+    for bh in staged:
+        g.free_batch(bh)
+    g.close()
+    us = float(np.mean(scan_us)) if scan_us else None
+    alg = args.n * (128 + 4) + (float(np.mean(nprog)) if nprog else 0.0) * args.n / 8.0
+    gbs = alg / (us * 1e-6) / 1e9 if us else None
+    kern = "k_scan_i8<false, 8, 2, false, true>" if prec == L.FV_GPU_PREC_FP8 else \
+        "k_scan_i8<false, 8, 2, false>"
+    return {"qps": round(res["q"][0], 1), "recall_at_10": round(float(np.mean(rec)), 4),
+            "overfetch": 4, "kernel": kern,
+            "avg_launch_us": round(us, 2) if us else None, "alg_bytes_per_launch": int(alg),
+            "achieved_GBs": round(gbs, 1) if gbs else None,
+            "frac": round(gbs / HBM_PEAK_GBS, 4) if gbs else None}, truth, \
+        (res["bf16"][0] if "bf16" in res else None)
+
+
 def sq8_quant_path(args, nq=4096):
-    """SQ8 over non-integer data: a 1M x 128 unit-norm mixture (the 768-d
-    generator at 128 dims, FP32 values), nq queries of the six filter
-    classes in micro-batches of 64. The index keeps int8 codes of the
-    midrange-centred rows; the INT8 scan ranks by code distance and the top
-    4 k are re-ranked exactly. Throughput and recall@k against the exact
-    BF16 path on the same data and index, and the INT8 scan kernel's rate."""
+    """SQ8 and FP8 indexes over non-integer data: a 1M x 128 unit-norm
+    mixture (the 768-d generator at 128 dims, FP32 values), nq queries of
+    the six filter classes in micro-batches of 64. SQ8 keeps int8 codes of
+    the midrange-centred rows (INT8 MFMA scan), FP8 E4M3 codes of the same
+    (FP8 MFMA scan); both rank by code distance and re-rank the top 4 k
+    exactly. Throughput and recall@k against the exact BF16 path on the same
+    data, and each code scan kernel's rate."""
     t = time.perf_counter()
     d = synth.gmm768(args.n, nq, dim=128, ncent=1024, seed=synth.BASE_SEED + 128)
     cols = synth.columns(d)
     rows, vals = synth.postings(cols)
     post = E.Postings(rows, vals)
     batches = build_batches(d, post, args.n, nq, args.batch, args.k)
-    g8 = E.GpuIndex(d["base"], cols, rows, precision=L.FV_GPU_PREC_SQ8, device=0)
-    staged = [g8.stage(d["queries"][s:e], b, args.k) for s, e, b in batches]
-    log(f"sq8 quantized: {args.n}x128 float data + index in {time.perf_counter() - t:.1f}s")
-    out = {}
-    for name, prec in (("sq8", L.FV_GPU_PREC_SQ8), ("bf16", L.FV_GPU_PREC_BF16)):
-        o = L.BruteOpts(precision=prec, route=0)
-        run_steps(g8, staged, o, max(1, args.warmup))
-        dt = run_steps(g8, staged, o, args.steps)
-        out[name] = (args.steps * nq / dt,
-                     [g8.fetch(bh, e - s, args.k) for bh, (s, e, _) in zip(staged, batches)])
-    rec = []
-    for (o8, c8, _, _), (ob, cb, _, _) in zip(out["sq8"][1], out["bf16"][1]):
-        for q in range(len(c8)):
-            tr = set(int(x) for x in ob[q, : cb[q]]["node"])
-            got = set(int(x) for x in o8[q, : c8[q]]["node"])
-            rec.append(len(tr & got) / len(tr) if tr else 1.0)
-    sq = L.BruteOpts(precision=L.FV_GPU_PREC_SQ8, route=0)
-    scan_us, nprog = [], []
-    for bh in staged[:24]:
-        g8.run_brute(bh, sq)
-        g8.sync()
-        st = g8.stats()
-        if st.route == 1 and st.precision == L.FV_GPU_PREC_SQ8:
-            nprog.append(st.nprog)
-            scan_us.append(g8.time_scan(bh, sq, 10))
-    for bh in staged:
-        g8.free_batch(bh)
-    g8.close()
-    us = float(np.mean(scan_us)) if scan_us else None
-    alg = args.n * (128 + 4) + (float(np.mean(nprog)) if nprog else 0.0) * args.n / 8.0
-    gbs = alg / (us * 1e-6) / 1e9 if us else None
-    log(f"sq8 quantized: {out['sq8'][0]:,.0f} QPS (BF16 exact on the same data "
-        f"{out['bf16'][0]:,.0f}), recall@{args.k} {np.mean(rec):.4f}; scan kernel "
-        f"{us} us -> {gbs} GB/s")
-    return {"qps": round(out["sq8"][0], 1), "bf16_exact_qps": round(out["bf16"][0], 1),
-            "recall_at_10": round(float(np.mean(rec)), 4), "overfetch": 4,
-            "queries": nq, "data": f"{args.n} x 128 unit-norm Gaussian mixture, FP32 values",
-            "kernel": "k_scan_i8<false, 8, 2, false>",
-            "avg_launch_us": round(us, 2) if us else None, "alg_bytes_per_launch": int(alg),
-            "achieved_GBs": round(gbs, 1) if gbs else None,
-            "frac": round(gbs / HBM_PEAK_GBS, 4) if gbs else None}
+    log(f"quantized indexes: {args.n}x128 float data in {time.perf_counter() - t:.1f}s")
+    sq8, truth, bf_qps = quant_leg(d, cols, rows, batches, L.FV_GPU_PREC_SQ8, args, nq)
+    fp8, _, _ = quant_leg(d, cols, rows, batches, L.FV_GPU_PREC_FP8, args, nq, truth)
+    log(f"sq8 quantized: {sq8['qps']:,.0f} QPS (BF16 exact on the same data {bf_qps:,.0f}), "
+        f"recall@{args.k} {sq8['recall_at_10']}; scan {sq8['avg_launch_us']} us -> "
+        f"{sq8['achieved_GBs']} GB/s")
+    log(f"fp8: {fp8['qps']:,.0f} QPS, recall@{args.k} {fp8['recall_at_10']}; scan "
+        f"{fp8['avg_launch_us']} us -> {fp8['achieved_GBs']} GB/s")
+    sq8.update({"bf16_exact_qps": round(bf_qps, 1), "queries": nq,
+                "data": f"{args.n} x 128 unit-norm Gaussian mixture, FP32 values"})
+    fp8.update({"queries": nq, "data": "the same"})
+    return sq8, fp8
 
 
 def learned_path(g, graph, d, post, staged, batches, truth, args, setup):

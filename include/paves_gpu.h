@@ -110,6 +110,14 @@ typedef struct FvGpuQueryDesc {
  * options scans x.q on FP32 MFMA (dim % 8 == 0, dim <= 512; other dims take
  * the VALU scan). Every scan returns the same exact results. */
 #define FV_GPU_PREC_FP32_VALU 3
+/* Index precision (extension; no reference counterpart): FP8 E4M3 codes of
+ * the midrange-centred rows under one scale (the largest half-range maps to
+ * 448), scanned with v_mfma_f32_32x32x16_fp8_fp8 (FP32 accumulation); the
+ * code distance ranks the rows and the top FvGpuBruteOpts.overfetch x B are
+ * re-ranked with the exact FP32 distance, as the quantized SQ8 index. Any
+ * float data up to 1024 dims; returned distances are exact. Asking for
+ * FV_GPU_PREC_SQ8 on an FP8 index scans the FP8 codes. */
+#define FV_GPU_PREC_FP8 4
 
 #ifdef __cplusplus
 #define PV_STATIC_ASSERT static_assert

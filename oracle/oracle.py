@@ -60,6 +60,17 @@ def lib():
         L.or_sq8_params.argtypes = [_P, C.c_uint64, C.c_int, _P, _P]
         L.or_sq8_codes.restype = None
         L.or_sq8_codes.argtypes = [_P, C.c_uint64, C.c_int, _P, C.c_float, _P]
+        L.or_fp8_encode.restype = C.c_uint8
+        L.or_fp8_encode.argtypes = [C.c_float]
+        L.or_fp8_decode.restype = C.c_float
+        L.or_fp8_decode.argtypes = [C.c_uint8]
+        L.or_fp8_params.restype = None
+        L.or_fp8_params.argtypes = [_P, C.c_uint64, C.c_int, _P, _P]
+        L.or_fp8_codes.restype = None
+        L.or_fp8_codes.argtypes = [_P, C.c_uint64, C.c_int, _P, C.c_float, _P]
+        L.or_fp8_brute_batch.restype = C.c_int
+        L.or_fp8_brute_batch.argtypes = [_P, C.c_uint64, C.c_int, _P, C.c_int, _P, _P, _P, _P, _P,
+                                         C.c_uint32, C.c_uint32, _P, _P, _P]
         L.or_sq8_brute_batch.restype = C.c_int
         L.or_sq8_brute_batch.argtypes = [_P, C.c_uint64, C.c_int, _P, C.c_int, _P, _P, _P, _P, _P,
                                          C.c_uint32, C.c_uint32, _P, _P, _P]
@@ -151,6 +162,56 @@ def sq8_brute_batch(vectors, cols, queries, qd, nodes, children, invals, topb, o
                                   over, _ptr(out), _ptr(cnt), _ptr(trunc))
     if rc != 0:
         raise MemoryError("oracle sq8: allocation failed")
+    return out, cnt, trunc
+
+
+def fp8_encode(v):
+    """E4M3 code of one float (round to nearest even, saturating)."""
+    return int(lib().or_fp8_encode(float(v)))
+
+
+def fp8_decode_table():
+    """Value of every E4M3 code 0..255 (float32; 0x7F and 0xFF decode to
+    the bit pattern's formula value, never produced by the encoder)."""
+    return np.array([lib().or_fp8_decode(c) for c in range(256)], np.float32)
+
+
+def fp8_params(vectors):
+    """FP8 quantizer of a row set: (per-dim midrange centre, scale)."""
+    vectors = np.ascontiguousarray(vectors, np.float32)
+    n, dim = vectors.shape
+    mid = np.zeros(dim, np.float32)
+    sc = np.zeros(1, np.float32)
+    lib().or_fp8_params(_ptr(vectors), n, dim, _ptr(mid), _ptr(sc))
+    return mid, np.float32(sc[0])
+
+
+def fp8_codes(x, mid, scale):
+    """E4M3 codes of rows x: encode((x - mid) * scale)."""
+    x = np.ascontiguousarray(np.atleast_2d(x), np.float32)
+    mid = np.ascontiguousarray(mid, np.float32)
+    out = np.zeros(x.shape, np.uint8)
+    lib().or_fp8_codes(_ptr(x), x.shape[0], x.shape[1], _ptr(mid), float(scale), _ptr(out))
+    return out
+
+
+def fp8_brute_batch(vectors, cols, queries, qd, nodes, children, invals, topb, over):
+    """The FP8 index path: top over x topb by exact code distance,
+    re-ranked exactly; returns (hits, cnt, trunc)."""
+    vectors = np.ascontiguousarray(vectors, np.float32)
+    queries = np.ascontiguousarray(queries, np.float32)
+    n, dim = vectors.shape
+    nq = queries.shape[0]
+    cols = None if cols is None else np.ascontiguousarray(cols, np.float64)
+    proots = np.ascontiguousarray(qd["proot"], np.int32)
+    out = np.zeros((nq, topb), dtype=HIT_DT)
+    cnt = np.zeros(nq, np.uint32)
+    trunc = np.zeros(nq, np.uint8)
+    rc = lib().or_fp8_brute_batch(_ptr(vectors), n, dim, _ptr(cols), nq, _ptr(queries),
+                                  _ptr(proots), _ptr(nodes), _ptr(children), _ptr(invals), topb,
+                                  over, _ptr(out), _ptr(cnt), _ptr(trunc))
+    if rc != 0:
+        raise MemoryError("oracle fp8: allocation failed")
     return out, cnt, trunc
 
 

@@ -252,7 +252,17 @@ int or_brute_batch(const float *vectors, uint64_t count, int dim,
  * distance, row) are re-ranked with or_l2sq and sorted by (dist, row); the
  * first min(B, passing) are the result, truncated = (count == B).
  * ------------------------------------------------------------------- */
-void or_sq8_params(const float *vectors, uint64_t count, int dim, float *mid, float *inv) {
+/* (code distance, row) order of the FP8 candidate sort */
+static int dr_cmp(const void *a, const void *b) {
+    const double da = *(const double *)a, db = *(const double *)b;
+    const uint64_t ra = *(const uint64_t *)((const char *)a + 8);
+    const uint64_t rb = *(const uint64_t *)((const char *)b + 8);
+    if (da != db) return da < db ? -1 : 1;
+    return ra < rb ? -1 : (ra > rb ? 1 : 0);
+}
+
+/* per-dim midrange centres, and the largest half-range */
+static float midrange_hmax(const float *vectors, uint64_t count, int dim, float *mid) {
     float hmax = 0.f;
     for (int d = 0; d < dim; d++) {
         float lo = INFINITY, hi = -INFINITY;
@@ -265,6 +275,11 @@ void or_sq8_params(const float *vectors, uint64_t count, int dim, float *mid, fl
         const float h = (hi - lo) * 0.5f;
         hmax = h > hmax ? h : hmax;
     }
+    return hmax;
+}
+
+void or_sq8_params(const float *vectors, uint64_t count, int dim, float *mid, float *inv) {
+    const float hmax = midrange_hmax(vectors, count, dim, mid);
     *inv = hmax > 0.f ? 127.0f / hmax : 1.0f;
 }
 
@@ -339,6 +354,149 @@ int or_sq8_brute_batch(const float *vectors, uint64_t count, int dim, const doub
         free(cand);
     }
     free(mid); free(xc); free(xn);
+    return rc;
+}
+
+/* ---------------------------------------------------------------------
+ * FP8 index (this library's reduced-precision index beside SQ8, in the
+ * reference SQ8's role, gpu_engine.cu:643-760 and 1911-2080): the rows'
+ * midrange-centred values under one scale (the largest half-range maps to
+ * 448) as FP8 E4M3 codes (OCP e4m3fn: bias 7, 3 mantissa bits, largest
+ * finite 448, subnormals m x 2^-9), ranked by the code distance and the top
+ * over x topb re-ranked with the exact FP32 distance.
+ *
+ * Pinning (tests/test_oracle_fp8.py): encoder known answers worked by hand
+ * (ties to even, carries into the next binade, subnormals, saturation);
+ * decode of all 256 codes against the format's definition; encode equals
+ * the nearest code by exhaustive search; with over x topb >= the passing
+ * rows the batch equals the exact brute force bit for bit.
+ * ------------------------------------------------------------------- */
+uint8_t or_fp8_encode(float v) {
+    const float a = fabsf(v);
+    const uint32_t s = v < 0.f ? 0x80u : 0u;
+    if (!(a < 448.f)) return (uint8_t)(s | 0x7Eu);
+    if (a < 0.015625f) {                            /* below 2^-6: m x 2^-9 */
+        const uint32_t m = (uint32_t)rintf(a * 512.f);
+        return (uint8_t)(m == 0 ? 0u : (s | m));
+    }
+    int e;
+    (void)frexpf(a, &e);                            /* a in [2^(e-1), 2^e) */
+    const float frac = ldexpf(a, 1 - e) - 1.f;
+    uint32_t m = (uint32_t)rintf(frac * 8.f);
+    int E = e - 1 + 7;
+    if (m == 8u) {
+        m = 0u;
+        E++;
+    }
+    return (uint8_t)(s | ((uint32_t)E << 3) | m);
+}
+
+float or_fp8_decode(uint8_t c) {
+    const int E = (c >> 3) & 15, m = c & 7;
+    const float a = E == 0 ? ldexpf((float)m, -9) : ldexpf(1.f + (float)m * 0.125f, E - 7);
+    return (c & 0x80u) ? -a : a;
+}
+
+/* SQ8's per-dim midrange and largest half-range, mapped to 448 */
+void or_fp8_params(const float *vectors, uint64_t count, int dim, float *mid, float *scale) {
+    const float hmax = midrange_hmax(vectors, count, dim, mid);
+    *scale = hmax > 0.f ? 448.0f / hmax : 1.0f;
+}
+
+void or_fp8_codes(const float *x, uint64_t n, int dim, const float *mid, float scale,
+                  uint8_t *codes) {
+    for (uint64_t r = 0; r < n; r++)
+        for (int d = 0; d < dim; d++)
+            codes[r * (uint64_t)dim + d] = or_fp8_encode((x[r * (uint64_t)dim + d] - mid[d]) * scale);
+}
+
+int or_fp8_brute_batch(const float *vectors, uint64_t count, int dim, const double *cols,
+                       int nq, const float *queries, const int32_t *proots,
+                       const OrPredNode *nodes, const int32_t *children,
+                       const double *invals, uint32_t topb, uint32_t over, OrHit *out,
+                       uint32_t *out_cnt, uint8_t *truncated) {
+    float *mid = (float *)malloc(sizeof(float) * (size_t)dim);
+    double *xd = (double *)malloc(sizeof(double) * (size_t)count * dim);
+    double *xn = (double *)malloc(sizeof(double) * (size_t)count);
+    if (!mid || !xd || !xn) {
+        free(mid); free(xd); free(xn);
+        return -1;
+    }
+    float scale;
+    or_fp8_params(vectors, count, dim, mid, &scale);
+    /* decoded codes; code distances in FP64 are exact (every product of two
+     * E4M3 values is a multiple of 2^-18 below 2^18, a sum of a few
+     * thousand of them fits in 53 bits) */
+    for (uint64_t r = 0; r < count; r++) {
+        double s = 0.0;
+        for (int d = 0; d < dim; d++) {
+            const double v = or_fp8_decode(or_fp8_encode(
+                (vectors[r * (uint64_t)dim + d] - mid[d]) * scale));
+            xd[r * (uint64_t)dim + d] = v;
+            s += v * v;
+        }
+        xn[r] = s;
+    }
+    int rc = 0;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int q = 0; q < nq; q++) {
+        double *qc = (double *)malloc(sizeof(double) * (size_t)dim);
+        OrHit *cand = (OrHit *)malloc(sizeof(OrHit) * ((size_t)count + 1));
+        double *cd = (double *)malloc(sizeof(double) * ((size_t)count + 1));
+        if (!qc || !cand || !cd) {
+            free(qc); free(cand); free(cd);
+            rc = -1;
+            continue;
+        }
+        const float *qv = queries + (uint64_t)q * dim;
+        double qn = 0.0;
+        for (int d = 0; d < dim; d++) {
+            qc[d] = or_fp8_decode(or_fp8_encode((qv[d] - mid[d]) * scale));
+            qn += qc[d] * qc[d];
+        }
+        uint64_t n = 0;
+        for (uint64_t r = 0; r < count; r++) {
+            if (proots[q] >= 0 &&
+                !or_pred_eval(nodes, children, invals, cols, count, proots[q], (uint32_t)r))
+                continue;
+            const double *x = xd + r * (uint64_t)dim;
+            double dot = 0.0;
+            for (int d = 0; d < dim; d++) dot += x[d] * qc[d];
+            cd[n] = xn[r] + qn - 2.0 * dot;
+            cand[n].node = r;
+            n++;
+        }
+        /* top R by (code distance, row) */
+        typedef struct { double d; uint64_t r; } DR;
+        DR *dr = (DR *)malloc(sizeof(DR) * ((size_t)n + 1));
+        if (!dr) {
+            free(qc); free(cand); free(cd);
+            rc = -1;
+            continue;
+        }
+        for (uint64_t i = 0; i < n; i++) {
+            dr[i].d = cd[i];
+            dr[i].r = cand[i].node;
+        }
+        qsort(dr, n, sizeof(DR), dr_cmp);
+        uint64_t R = (uint64_t)topb * over;
+        if (R > 4096) R = 4096;
+        if (R > n) R = n;
+        for (uint64_t i = 0; i < R; i++) {
+            cand[i].node = dr[i].r;
+            cand[i].dist = or_l2sq(qv, vectors + dr[i].r * (uint64_t)dim, dim);
+        }
+        qsort(cand, R, sizeof(OrHit), hit_cmp);
+        const uint32_t B = (uint32_t)(n < topb ? n : topb);
+        memcpy(out + (uint64_t)q * topb, cand, B * sizeof(OrHit));
+        out_cnt[q] = B;
+        truncated[q] = (B == topb) ? 1 : 0;
+        free(dr);
+        free(qc);
+        free(cand);
+        free(cd);
+    }
+    free(mid); free(xd); free(xn);
     return rc;
 }
 

@@ -47,6 +47,16 @@ int or_sq8_brute_batch(const float *vectors, uint64_t count, int dim, const doub
                        const OrPredNode *nodes, const int32_t *children,
                        const double *invals, uint32_t topb, uint32_t over, OrHit *out,
                        uint32_t *out_cnt, uint8_t *truncated);
+uint8_t or_fp8_encode(float v);
+float or_fp8_decode(uint8_t c);
+void or_fp8_params(const float *vectors, uint64_t count, int dim, float *mid, float *scale);
+void or_fp8_codes(const float *x, uint64_t n, int dim, const float *mid, float scale,
+                  uint8_t *codes);
+int or_fp8_brute_batch(const float *vectors, uint64_t count, int dim, const double *cols,
+                       int nq, const float *queries, const int32_t *proots,
+                       const OrPredNode *nodes, const int32_t *children,
+                       const double *invals, uint32_t topb, uint32_t over, OrHit *out,
+                       uint32_t *out_cnt, uint8_t *truncated);
 uint32_t or_hnsw_search(const float *vectors, uint64_t count, int dim,
                         const uint32_t *l0, uint32_t l0_stride,
                         const double *cols, const OrPredNode *nodes,

@@ -29,6 +29,7 @@ FV_GPU_PREC_FP32 = 0
 FV_GPU_PREC_SQ8 = 1
 FV_GPU_PREC_BF16 = 2
 FV_GPU_PREC_FP32_VALU = 3
+FV_GPU_PREC_FP8 = 4
 FV_GPU_MAX_TOPB = 4096
 FV_GPU_MAX_SPANS = 4096
 FV_GPU_MAX_EF = 512

@@ -42,6 +42,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include "pv_mlp.h"
@@ -81,7 +82,7 @@ __device__ __forceinline__ void prep_query(
     float *__restrict__ qnorm, float *__restrict__ eps, float xmax_norm,
     float vmin, float vmax, int all_int, int mode, int8_t *__restrict__ qi8 = nullptr,
     float *__restrict__ qn8 = nullptr, int centre = 0, uint32_t *__restrict__ status = nullptr,
-    const float *__restrict__ i8mid = nullptr, float i8inv = 1.f, int dimp = 0) {
+    const float *__restrict__ i8mid = nullptr, float i8inv = 1.f, int dimp = 0, int fp8 = 0) {
 #pragma clang fp contract(off)
     const int l = threadIdx.x & 63;
     const float *src = q + (uint64_t)qi * dim;
@@ -100,6 +101,10 @@ __device__ __forceinline__ void prep_query(
                 c = v - (float)centre;
                 out8 |= (c != rintf(c) || c < -128.f || c > 127.f) ? 1 : 0;
                 qi8[(uint64_t)qi * dimp + j] = (int8_t)(int)fminf(fmaxf(c, -128.f), 127.f);
+            } else if (fp8) {
+                const uint32_t code = e4m3_encode((v - i8mid[j]) * i8inv);
+                qi8[(uint64_t)qi * dimp + j] = (int8_t)code;
+                c = e4m3_decode(code);
             } else {
                 c = fminf(fmaxf(rintf((v - i8mid[j]) * i8inv), -127.f), 127.f);
                 qi8[(uint64_t)qi * dimp + j] = (int8_t)(int)c;
@@ -188,6 +193,7 @@ struct Phase1Args {
     float *qn8;
     int centre;
     const float *i8mid;   /* mode 2, quantized codes: per-dim centre, scale */
+    int fp8;              /* mode 2: E4M3 codes (FP8 index) */
     float i8inv;
     int dimp;             /* mode 2: code row length */
     uint32_t *status;
@@ -213,7 +219,8 @@ __global__ __launch_bounds__(256) void k_phase1(Phase1Args A) {
         const int qi = (int)blockIdx.x * 4 + w;
         if (qi >= A.nqp) return;
         prep_query(qi, A.q, A.nq, A.dim, A.dimb, A.qbf, A.qnorm, A.eps, A.xmax_norm, A.vmin, A.vmax,
-                   A.all_int, A.mode, A.qi8, A.qn8, A.centre, A.status, A.i8mid, A.i8inv, A.dimp);
+                   A.all_int, A.mode, A.qi8, A.qn8, A.centre, A.status, A.i8mid, A.i8inv, A.dimp,
+                   A.fp8);
         return;
     }
     /* with router features: one block per padded query; wave 0 preps it,
@@ -225,7 +232,8 @@ __global__ __launch_bounds__(256) void k_phase1(Phase1Args A) {
     if (qi >= A.nqp) return;
     if (w == 0)
         prep_query(qi, A.q, A.nq, A.dim, A.dimb, A.qbf, A.qnorm, A.eps, A.xmax_norm, A.vmin, A.vmax,
-                   A.all_int, A.mode, A.qi8, A.qn8, A.centre, A.status, A.i8mid, A.i8inv, A.dimp);
+                   A.all_int, A.mode, A.qi8, A.qn8, A.centre, A.status, A.i8mid, A.i8inv, A.dimp,
+                   A.fp8);
     if (qi >= A.nq) return; /* block-uniform */
     __shared__ float xq[256];
     __shared__ float wn[4][kNear], wm[kNear];
@@ -335,6 +343,7 @@ __device__ __forceinline__ bool scan_pad(const ScanArgs &A) {
 struct ScanLaunch {
     ScanArgs M;
     bool bf, inl, i8, f32m;
+    bool f8;                /* INT8 layout holding FP8 codes */
     int pt;                 /* INT8: 32-row tiles per 8 KB unit */
     int qtw, nw, nqt, gy;
     uint32_t gx;
@@ -1427,6 +1436,14 @@ void k_scan_bf16_qr(ScanArgs A) {
     }
 }
 
+/* the two 8-byte FP8 operands in a lane's 16 bytes of a k-step */
+__device__ __forceinline__ long f8_lo(const u32x4 &v) {
+    return (long)(((uint64_t)v.y << 32) | v.x);
+}
+__device__ __forceinline__ long f8_hi(const u32x4 &v) {
+    return (long)(((uint64_t)v.w << 32) | v.z);
+}
+
 /*
  * INT8 scan (SQ8 indexes; code dims dimp = 64, 128 or 256). x.q of the
  * int8 codes accumulates in int32 with v_mfma_i32_32x32x32_i8 (exact), and
@@ -1439,8 +1456,12 @@ void k_scan_bf16_qr(ScanArgs A) {
  * 1 KB chunks (slot l: row l & 31, dims 32 s + 16 (l >> 5) .. +16); a unit
  * is 8 chunks = 8 KB = PT tiles (PT = 256 / dimp), the cursor's logical
  * tile. Same pipeline as k_scan_bf16 otherwise.
+ * F8 (FP8 index, i8_mode 3): the same layout holds E4M3 codes; each lane's
+ * 16 bytes of a k-step feed two v_mfma_f32_32x32x16_fp8_fp8 (FP32
+ * accumulation) and the tile takes the BF16 scan's seeds and epilogue over
+ * the code distance ||xc||^2 + ||qc||^2 - 2 xc.qc.
  */
-template <bool PREPASS, int NW, int PT, bool INL>
+template <bool PREPASS, int NW, int PT, bool INL, bool F8 = false>
 __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(2)))
 void k_scan_i8(ScanArgs A) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
@@ -1508,21 +1529,42 @@ void k_scan_i8(ScanArgs A) {
                 w0 = inline_word(cv, s_im, s_iv, 0, 0);
                 w1 = inline_word(cv, s_im, s_iv, 0, 1);
             }
-            i32x16 c0, c1;
-            uint32_t par;
-            seed_i8(in[p].xr, w0 & in[p].valid, w1 & in[p].valid, c0, c1, par);
-#pragma unroll
-            for (int k = 0; k < KS; k++) {
-                const i32x4 av = __builtin_bit_cast(i32x4, a[p * KS + k]);
-                const i32x4 b0 = __builtin_bit_cast(i32x4, qsh[(0 * KS + k) * 64 + l]);
-                const i32x4 b1 = __builtin_bit_cast(i32x4, qsh[(1 * KS + k) * 64 + l]);
-                c0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, b0, c0, 0, 0, 0);
-                c1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, b1, c1, 0, 0, 0);
-            }
             /* group index of the pre-pass: sampled tile x PT + p */
             const uint32_t tg = in[p].t * PT + p;
             const uint32_t pt = u * PT + p;
-            epilogue_i8<PREPASS>(A, qbase, tg, pt, c0, c1, par, qn, th, C);
+            if constexpr (F8) {
+                /* FP8 codes: the BF16 scan's seeds and epilogue over the code
+                 * distance; each lane's 16 bytes of a k-step are two 8-byte
+                 * operands of v_mfma_f32_32x32x16_fp8_fp8 */
+                float sn[16];
+                expand_norms(-0.5f * in[p].xr, sn);
+                f32x16 c0, c1;
+                seed_bf(sn, w0 & in[p].valid, w1 & in[p].valid, c0, c1);
+#pragma unroll
+                for (int k = 0; k < KS; k++) {
+                    const u32x4 av = a[p * KS + k];
+                    const u32x4 b0 = qsh[(0 * KS + k) * 64 + l];
+                    const u32x4 b1 = qsh[(1 * KS + k) * 64 + l];
+                    c0 = __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(f8_lo(av), f8_lo(b0), c0, 0, 0, 0);
+                    c1 = __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(f8_lo(av), f8_lo(b1), c1, 0, 0, 0);
+                    c0 = __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(f8_hi(av), f8_hi(b0), c0, 0, 0, 0);
+                    c1 = __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(f8_hi(av), f8_hi(b1), c1, 0, 0, 0);
+                }
+                epilogue_bf<PREPASS>(A, qbase, tg, pt, c0, c1, qn, tq, C);
+            } else {
+                i32x16 c0, c1;
+                uint32_t par;
+                seed_i8(in[p].xr, w0 & in[p].valid, w1 & in[p].valid, c0, c1, par);
+#pragma unroll
+                for (int k = 0; k < KS; k++) {
+                    const i32x4 av = __builtin_bit_cast(i32x4, a[p * KS + k]);
+                    const i32x4 b0 = __builtin_bit_cast(i32x4, qsh[(0 * KS + k) * 64 + l]);
+                    const i32x4 b1 = __builtin_bit_cast(i32x4, qsh[(1 * KS + k) * 64 + l]);
+                    c0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, b0, c0, 0, 0, 0);
+                    c1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, b1, c1, 0, 0, 0);
+                }
+                epilogue_i8<PREPASS>(A, qbase, tg, pt, c0, c1, par, qn, th, C);
+            }
         }
     };
 
@@ -1545,7 +1587,7 @@ void k_scan_i8(ScanArgs A) {
  * seeds at a tile's first unit, its epilogue after the last), with the
  * int32 accumulators, seeds and epilogue of k_scan_i8.
  */
-template <bool PREPASS, int NW, bool INL>
+template <bool PREPASS, int NW, bool INL, bool F8 = false>
 __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(2)))
 void k_scan_i8w(ScanArgs A) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
@@ -1597,7 +1639,9 @@ void k_scan_i8w(ScanArgs A) {
     __syncthreads();
     cursor_start(cur, &s_next);
     load_in(in0);
-    i32x16 c0 = {}, c1 = {};
+    /* accumulators: int32 (INT8 codes) or FP32 (FP8 codes) */
+    using Acc = std::conditional_t<F8, f32x16, i32x16>;
+    Acc c0 = {}, c1 = {};
     uint32_t par = 0;
     auto do_unit = [&](const u32x4 (&a)[8], const TileIn &in) {
         const uint32_t c = in.c;
@@ -1609,19 +1653,39 @@ void k_scan_i8w(ScanArgs A) {
                 w0 = inline_word(cv, s_im, s_iv, 0, 0);
                 w1 = inline_word(cv, s_im, s_iv, 0, 1);
             }
-            seed_i8(in.xr, w0 & in.valid, w1 & in.valid, c0, c1, par);
+            if constexpr (F8) {
+                float sn[16];
+                expand_norms(-0.5f * in.xr, sn);
+                seed_bf(sn, w0 & in.valid, w1 & in.valid, c0, c1);
+            } else {
+                seed_i8(in.xr, w0 & in.valid, w1 & in.valid, c0, c1, par);
+            }
         }
 #pragma unroll
         for (int j = 0; j < 8; j++) {
             const int s = (int)c * 8 + j;
-            const i32x4 av = __builtin_bit_cast(i32x4, a[j]);
-            const i32x4 b0 = __builtin_bit_cast(i32x4, qsh[(0 * S + s) * 64 + l]);
-            const i32x4 b1 = __builtin_bit_cast(i32x4, qsh[(1 * S + s) * 64 + l]);
-            c0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, b0, c0, 0, 0, 0);
-            c1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, b1, c1, 0, 0, 0);
+            if constexpr (F8) {
+                const u32x4 av = a[j];
+                const u32x4 b0 = qsh[(0 * S + s) * 64 + l];
+                const u32x4 b1 = qsh[(1 * S + s) * 64 + l];
+                c0 = __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(f8_lo(av), f8_lo(b0), c0, 0, 0, 0);
+                c1 = __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(f8_lo(av), f8_lo(b1), c1, 0, 0, 0);
+                c0 = __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(f8_hi(av), f8_hi(b0), c0, 0, 0, 0);
+                c1 = __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(f8_hi(av), f8_hi(b1), c1, 0, 0, 0);
+            } else {
+                const i32x4 av = __builtin_bit_cast(i32x4, a[j]);
+                const i32x4 b0 = __builtin_bit_cast(i32x4, qsh[(0 * S + s) * 64 + l]);
+                const i32x4 b1 = __builtin_bit_cast(i32x4, qsh[(1 * S + s) * 64 + l]);
+                c0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, b0, c0, 0, 0, 0);
+                c1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, b1, c1, 0, 0, 0);
+            }
         }
-        if (c == NCH - 1)
-            epilogue_i8<PREPASS>(A, qbase, in.t, in.t * A.stride, c0, c1, par, qn, th, C);
+        if (c == NCH - 1) {
+            if constexpr (F8)
+                epilogue_bf<PREPASS>(A, qbase, in.t, in.t * A.stride, c0, c1, qn, tq, C);
+            else
+                epilogue_i8<PREPASS>(A, qbase, in.t, in.t * A.stride, c0, c1, par, qn, th, C);
+        }
     };
     while (in0.live) {
         load_rows(a1);
@@ -2466,6 +2530,9 @@ int scan_precision(const FvGpuIndex *g, const FvGpuBatch *b, const FvGpuBruteOpt
     /* an FP32 index is served by the BF16 scan unless FP32 is asked for:
      * the certified re-rank returns the same ids and distances */
     if (!asked && p == FV_GPU_PREC_FP32) p = FV_GPU_PREC_BF16;
+    /* FP8 codes sit in the index's INT8-layout copy (i8_mode 3): the SQ8
+     * pipeline scans them with the FP8 MFMA */
+    if (p == FV_GPU_PREC_FP8) p = FV_GPU_PREC_SQ8;
     if (p == FV_GPU_PREC_SQ8 && !(g->d_vi8 != nullptr && b != nullptr && b->i8_ok))
         p = FV_GPU_PREC_BF16;
     if (p == FV_GPU_PREC_BF16 && (g->d_vbf == nullptr || g->dimb % 128 != 0 || g->dimb > 1024))
@@ -2574,7 +2641,8 @@ hipError_t launch_phase1(FvGpuIndex *g, FvGpuBatch *b, int prec, bool bitmap, hi
         P.qi8 = b->d_qi8.as<int8_t>();
         P.qn8 = b->d_qn8.as<float>();
         P.centre = g->i8_center;
-        P.i8mid = g->i8_mode == 2 ? g->d_i8mid : nullptr;
+        P.i8mid = g->i8_mode >= 2 ? g->d_i8mid : nullptr;
+        P.fp8 = g->i8_mode == 3 ? 1 : 0;
         P.i8inv = g->i8_inv;
         P.dimp = g->dimp;
         P.status = qa.status;
@@ -2622,7 +2690,7 @@ static bool other_batches_in_flight(const FvGpuIndex *g, const FvGpuBatch *b) {
  * scan the overfetch x topb rows that the select stage re-ranks exactly
  * (FvGpuBruteOpts.overfetch, default 4), at most kSelMax */
 static uint32_t scan_overfetch(const FvGpuIndex *g, int prec, const FvGpuBruteOpts *o) {
-    if (!(prec == FV_GPU_PREC_SQ8 && g->i8_mode == 2)) return 1;
+    if (!(prec == FV_GPU_PREC_SQ8 && g->i8_mode >= 2)) return 1;
     const uint32_t f = (o && o->overfetch) ? o->overfetch : 4u;
     return std::min<uint32_t>(f, 64);
 }
@@ -2726,15 +2794,15 @@ static ScanShape scan_shape(const FvGpuIndex *g, const FvGpuBatch *b, const FvGp
 /* INT8 scan launch: waves per WG (4 / 8), tiles per unit (code dims 64:
  * 4, 128: 2, 256: 1; wider rows: the multi-unit kernel) and inline
  * predicates pick the kernel instance */
-template <bool PREPASS>
-static void launch_scan_i8(int nw, int pt, bool inl, uint32_t gx, int nqt, size_t shm,
-                           hipStream_t s, const ScanArgs &A0) {
+template <bool PREPASS, bool F8>
+static void launch_scan_i8_t(int nw, int pt, bool inl, uint32_t gx, int nqt, size_t shm,
+                             hipStream_t s, const ScanArgs &A0) {
     ScanArgs A = A0;
     const dim3 grid = scan_grid(A, gx, (uint32_t)nqt);
 #define PV_I8(NW, PT, INL) \
-    hipLaunchKernelGGL((k_scan_i8<PREPASS, NW, PT, INL>), grid, dim3(NW * 64), shm, s, A)
+    hipLaunchKernelGGL((k_scan_i8<PREPASS, NW, PT, INL, F8>), grid, dim3(NW * 64), shm, s, A)
 #define PV_I8W(NW, INL) \
-    hipLaunchKernelGGL((k_scan_i8w<PREPASS, NW, INL>), grid, dim3(NW * 64), shm, s, A)
+    hipLaunchKernelGGL((k_scan_i8w<PREPASS, NW, INL, F8>), grid, dim3(NW * 64), shm, s, A)
     if (A.dimp > 256) {
         if (nw == 8) { if (inl) PV_I8W(8, true); else PV_I8W(8, false); }
         else { if (inl) PV_I8W(4, true); else PV_I8W(4, false); }
@@ -2750,12 +2818,19 @@ static void launch_scan_i8(int nw, int pt, bool inl, uint32_t gx, int nqt, size_
 #undef PV_I8
 #undef PV_I8W
 }
+/* INT8 or FP8 (code format of the index's INT8-layout copy) */
+template <bool PREPASS>
+static void launch_scan_i8(int nw, int pt, bool inl, bool f8, uint32_t gx, int nqt, size_t shm,
+                           hipStream_t s, const ScanArgs &A0) {
+    if (f8) launch_scan_i8_t<PREPASS, true>(nw, pt, inl, gx, nqt, shm, s, A0);
+    else launch_scan_i8_t<PREPASS, false>(nw, pt, inl, gx, nqt, shm, s, A0);
+}
 
 static hipError_t launch_main_scan(const ScanLaunch &L, hipStream_t s) {
     if (L.f32m) {
         hipLaunchKernelGGL((k_scan_f32m<false, 8>), dim3(L.gx, L.nqt), dim3(512), L.shm, s, L.M);
     } else if (L.i8) {
-        launch_scan_i8<false>(L.nw, L.pt, L.inl, L.gx, L.nqt, L.shm, s, L.M);
+        launch_scan_i8<false>(L.nw, L.pt, L.inl, L.f8, L.gx, L.nqt, L.shm, s, L.M);
     } else if (L.bf && L.qtw == kQrNW) {
         const hipError_t e = launch_scan_qr(L.gx, L.gy, s, L.M);
         if (e != hipSuccess) return e;
@@ -2795,7 +2870,7 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
     QAux qa = qaux_of(b);
     FvGpuStats &st = g->last;
     memset(&st, 0, sizeof(st));
-    st.precision = prec;
+    st.precision = (prec == FV_GPU_PREC_SQ8 && g->i8_mode == 3) ? FV_GPU_PREC_FP8 : prec;
     st.nprog = (uint32_t)b->nprog;
     g->last_batch = b;
 
@@ -2874,6 +2949,7 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
     /* ---- full-scan route ---- */
     /* pre-pass samples every stride-th tile (INT8: 8 KB unit of PT tiles) */
     const bool i8 = prec == FV_GPU_PREC_SQ8;
+    const bool f8 = i8 && g->i8_mode == 3;
     const int pt = (int)scan_unit_tiles(g, prec);
     const uint32_t ntiles = (uint32_t)(g->count_pad / 32) / (uint32_t)pt;
     const bool bf = prec == FV_GPU_PREC_BF16;
@@ -2929,7 +3005,8 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
         A.qi8 = b->d_qi8.as<int8_t>();
         A.xnorm = g->d_norm8;
         A.qnorm = b->d_qn8.as<float>();
-        A.xn2max = (float)g->dimp * 16384.f;
+        /* code norms stay below dimp x (largest code)^2 */
+        A.xn2max = (float)g->dimp * (f8 ? 448.f * 448.f : 16384.f);
     }
     A.dimp = g->dimp;
     /* INT8: the queries' code operands, dimp x 64 bytes */
@@ -2942,7 +3019,15 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
                               (const void *)k_scan_i8w<true, 4, false>,
                               (const void *)k_scan_i8w<false, 4, false>,
                               (const void *)k_scan_i8w<true, 4, true>,
-                              (const void *)k_scan_i8w<false, 4, true>})
+                              (const void *)k_scan_i8w<false, 4, true>,
+                              (const void *)k_scan_i8w<true, 8, false, true>,
+                              (const void *)k_scan_i8w<false, 8, false, true>,
+                              (const void *)k_scan_i8w<true, 8, true, true>,
+                              (const void *)k_scan_i8w<false, 8, true, true>,
+                              (const void *)k_scan_i8w<true, 4, false, true>,
+                              (const void *)k_scan_i8w<false, 4, false, true>,
+                              (const void *)k_scan_i8w<true, 4, true, true>,
+                              (const void *)k_scan_i8w<false, 4, true, true>})
             PV_TRY(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm_i8));
 
     /* 4 waves per WG up to 256 dims (2 WGs per CU), 8 waves above (the
@@ -2977,7 +3062,7 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
              * main scan's grid): staging 64 query operands per WG costs as
              * much as a few tiles */
             const uint32_t gxp = std::min<uint32_t>(gx, std::max<uint32_t>(1, (stiles + nw - 1) / nw));
-            if (i8) launch_scan_i8<true>(nw, pt, inl, gxp, b->nqt, shm_i8, s, P);
+            if (i8) launch_scan_i8<true>(nw, pt, inl, f8, gxp, b->nqt, shm_i8, s, P);
             else if (qtw == kQrNW)   /* the query-resident kernel has no pre-pass form */
                 PV_TRY(launch_scan_mq<true>(2, inl, gxp, (b->nqt + 1) / 2, s, P));
             else if (qtw > 1) PV_TRY(launch_scan_mq<true>(qtw, inl, gxp, gy, s, P));
@@ -3018,6 +3103,7 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
         L.f32m = f32m;
         L.inl = inl;
         L.i8 = i8;
+        L.f8 = f8;
         L.pt = pt;
         L.qtw = qtw;
         L.nw = nw;

@@ -18,6 +18,39 @@ __device__ __forceinline__ float key_dist(uint64_t k) {
 }
 __device__ __forceinline__ uint32_t key_row(uint64_t k) { return (uint32_t)k; }
 
+/* FP8 E4M3 (OCP "e4m3fn": 1 sign, 4 exponent bits with bias 7, 3 mantissa
+ * bits; largest finite 448, subnormals m x 2^-9, no infinity, 0x7F / 0xFF
+ * NaN): the code format of FP8 indexes, the operand format of
+ * v_mfma_f32_32x32x16_fp8_fp8. Encoding rounds to nearest, ties to even,
+ * saturates at +-448 and never writes -0 or NaN. Written out (not the
+ * conversion instruction) so that the codes are a function stated here. */
+__device__ __forceinline__ uint8_t e4m3_encode(float v) {
+    const float a = fabsf(v);
+    const uint32_t s = v < 0.f ? 0x80u : 0u;
+    if (!(a < 448.f)) return (uint8_t)(s | 0x7Eu);   /* saturate (NaN too) */
+    if (a < 0x1p-6f) {
+        /* subnormals: multiples of 2^-9 (8 x 2^-9 is the smallest normal,
+         * whose code is 0x08) */
+        const uint32_t m = (uint32_t)rintf(a * 512.f);
+        return (uint8_t)(m == 0 ? 0u : (s | m));
+    }
+    int e;
+    (void)frexpf(a, &e);                       /* a in [2^(e-1), 2^e) */
+    const float frac = ldexpf(a, 1 - e) - 1.f;  /* [0, 1), exact */
+    uint32_t m = (uint32_t)rintf(frac * 8.f);
+    int E = e - 1 + 7;
+    if (m == 8u) {
+        m = 0u;
+        E++;
+    }
+    return (uint8_t)(s | ((uint32_t)E << 3) | m);
+}
+__device__ __forceinline__ float e4m3_decode(uint32_t c) {
+    const int E = (int)((c >> 3) & 15u), m = (int)(c & 7u);
+    const float a = E == 0 ? ldexpf((float)m, -9) : ldexpf(1.f + (float)m * 0.125f, E - 7);
+    return (c & 0x80u) ? -a : a;
+}
+
 /* float -> bf16 bits, round to nearest even (NaN stays NaN). */
 __device__ __forceinline__ uint16_t f2bf(float x) {
     uint32_t u = __float_as_uint(x);

@@ -176,11 +176,13 @@ __global__ __launch_bounds__(256) void k_dim_range(const float *__restrict__ v, 
  * .. +16 — over dimp dims (zero codes past dim), and the squared code norm
  * (an integer below 2^24, exact in FP32). Codes: x - centre (exact mode,
  * mid == nullptr) or clamp(rint((x - mid[d]) * inv), -127, 127). One wave
- * per row. */
+ * per row. fp8: E4M3 codes e4m3_encode((x - mid[d]) * inv) in the same
+ * layout, and the squared norm of the decoded codes (FP32, each lane's
+ * dims in order, then the butterfly). */
 __global__ __launch_bounds__(256) void k_derive_i8(const float *__restrict__ v, uint64_t count,
                                                    uint64_t count_pad, int dim, int dimp,
                                                    int centre, const float *__restrict__ mid,
-                                                   float inv, int8_t *__restrict__ vi8,
+                                                   float inv, int fp8, int8_t *__restrict__ vi8,
                                                    float *__restrict__ norm8) {
 #pragma clang fp contract(off)
     const int l = threadIdx.x & 63;
@@ -188,11 +190,16 @@ __global__ __launch_bounds__(256) void k_derive_i8(const float *__restrict__ v, 
     if (row >= count_pad) return;
     const uint64_t tbase = (row >> 5) * 32ull * dimp;
     int32_t ss = 0;
+    float sf = 0.f;
     for (int j = l; j < dimp; j += 64) {
         int code = 0;
         if (row < count && j < dim) {
             const float x = v[row * dim + j];
-            if (mid == nullptr) {
+            if (fp8) {
+                code = e4m3_encode((x - mid[j]) * inv);
+                const float dv = e4m3_decode((uint32_t)code);
+                sf = fmaf(dv, dv, sf);
+            } else if (mid == nullptr) {
                 code = (int)(x - (float)centre);
             } else {
                 const float c = rintf((x - mid[j]) * inv);
@@ -204,8 +211,11 @@ __global__ __launch_bounds__(256) void k_derive_i8(const float *__restrict__ v, 
         ss += code * code;
     }
 #pragma unroll
-    for (int o = 32; o; o >>= 1) ss += __shfl_xor(ss, o);
-    if (l == 0) norm8[row] = (float)ss;
+    for (int o = 32; o; o >>= 1) {
+        ss += __shfl_xor(ss, o);
+        sf += __shfl_xor(sf, o);
+    }
+    if (l == 0) norm8[row] = fp8 ? sf : (float)ss;
 }
 
 /* level-0 lists [cnt, neigh...] (stride l0_stride) -> padded rows of
@@ -303,7 +313,8 @@ static FvGpuIndex *upload_impl(const float *vectors, const double *cols,
                  (unsigned long long)count, dim, ncols);
         return nullptr;
     }
-    if (precision < FV_GPU_PREC_FP32 || precision > FV_GPU_PREC_BF16) {
+    if (precision != FV_GPU_PREC_FP32 && precision != FV_GPU_PREC_SQ8 &&
+        precision != FV_GPU_PREC_BF16 && precision != FV_GPU_PREC_FP8) {
         snprintf(errbuf, errlen, "gpu: unknown precision %d", precision);
         return nullptr;
     }
@@ -391,11 +402,14 @@ static FvGpuIndex *upload_impl(const float *vectors, const double *cols,
      * (dot products and norms stay below 2^24, exact in FP32); otherwise,
      * up to 1024 dims, quantized codes of the per-dim midrange-centred
      * values under one scale (the largest half-range maps to 127). */
-    if (precision == FV_GPU_PREC_SQ8 && dim <= 1024) {
-        const bool exact = g->all_int && g->vmax - g->vmin <= 255.0f && dim <= 256;
+    /* FP8: E4M3 codes of the same midrange-centred rows, the largest
+     * half-range scaled to 448 (the format's largest finite value) */
+    const bool fp8 = precision == FV_GPU_PREC_FP8;
+    if ((precision == FV_GPU_PREC_SQ8 || fp8) && dim <= 1024) {
+        const bool exact = !fp8 && g->all_int && g->vmax - g->vmin <= 255.0f && dim <= 256;
         const int dimp = dim <= 64 ? 64 : dim <= 128 ? 128 : (dim + 255) / 256 * 256;
         g->dimp = dimp;
-        g->i8_mode = exact ? 1 : 2;
+        g->i8_mode = exact ? 1 : (fp8 ? 3 : 2);
         if (exact) {
             g->i8_center = (int)g->vmin + 128;
         } else {
@@ -420,7 +434,7 @@ static FvGpuIndex *upload_impl(const float *vectors, const double *cols,
                 mid[d] = (lo + hi) * 0.5f;
                 hmax = fmaxf(hmax, (hi - lo) * 0.5f);
             }
-            g->i8_inv = hmax > 0.f ? 127.0f / hmax : 1.0f;
+            g->i8_inv = hmax > 0.f ? (fp8 ? 448.0f : 127.0f) / hmax : 1.0f;
             UP_TRY(hipMalloc(&g->d_i8mid, (size_t)dimp * 4));
             UP_TRY(hipMemcpyAsync(g->d_i8mid, mid.data(), (size_t)dimp * 4,
                                   hipMemcpyHostToDevice, s));
@@ -429,7 +443,7 @@ static FvGpuIndex *upload_impl(const float *vectors, const double *cols,
         UP_TRY(hipMalloc(&g->d_norm8, count_pad * 4));
         hipLaunchKernelGGL(k_derive_i8, dim3((unsigned)((count_pad + 3) / 4)), dim3(256), 0, s,
                            g->d_vecs, count, count_pad, dim, dimp, g->i8_center, g->d_i8mid,
-                           g->i8_inv, g->d_vi8, g->d_norm8);
+                           g->i8_inv, fp8 ? 1 : 0, g->d_vi8, g->d_norm8);
         UP_TRY(hipGetLastError());
     }
 

@@ -310,13 +310,18 @@ struct FvGpuIndex {
      * scan is exact. i8_mode 2 (quantized): any other data up to 1024 dims
      * — codes clamp(rint((x - i8_mid[d]) * i8_inv), -127, 127) with the
      * per-dim midrange i8_mid and one scale; the scan ranks by the code
-     * distance and the top overfetch x k rows are re-ranked exactly. */
+     * distance and the top overfetch x k rows are re-ranked exactly.
+     * i8_mode 3 (FP8, FV_GPU_PREC_FP8): the same rows as E4M3 codes
+     * e4m3_encode((x - i8_mid[d]) * i8_inv), one byte per dim in the same
+     * tiled layout (v_mfma_f32_32x32x16_fp8_fp8 takes each lane's 16 bytes
+     * as two 8-byte operands); squared code norms from the decoded codes;
+     * ranked and re-ranked as mode 2. */
     int8_t *d_vi8 = nullptr;
     float *d_norm8 = nullptr;
     int i8_center = 0;
     int i8_mode = 0;
     int dimp = 0;
-    float *d_i8mid = nullptr;   /* [dimp] (mode 2) */
+    float *d_i8mid = nullptr;   /* [dimp] (modes 2, 3) */
     float i8_inv = 1.f;
 
     float xmax_norm = 0.f;   /* max ||x|| */
