@@ -305,6 +305,7 @@ struct ScanArgs {
     const uint8_t *skip;      /* optional [nq]: 1 = query routed elsewhere */
     const InlPred *inl;       /* inline predicates [nqp] (INL kernels) */
     const double *icol;       /* their column [count] */
+    const uint32_t *irank;    /* the column's row ranks [count] (query-resident scan) */
     float xn2max;             /* max ||x||^2 over the rows (candidate-test margin) */
     uint32_t nchunk, ngroup;  /* query-major grid: row chunks, query groups */
     uint32_t qmajor;          /* grid (query tiles, row chunks): the query tiles
@@ -1102,6 +1103,12 @@ void k_scan_bf16_mq(ScanArgs A) {
  * more programs, and the index's last (partial) tile, seed per element:
  * ||x||^2 where the row passes, +inf where it does not.
  *
+ * Inline predicates (INL: one comparison per query on one column): the
+ * index holds each row's rank in the column's posting order, staging turns
+ * each comparison into a rank range [lo, hi), and the stage copy brings the
+ * tiles' ranks: a row passes iff rank - lo < hi - lo (unsigned), one
+ * comparison per element instead of a double-precision predicate.
+ *
  * Candidates: each lane counts its own slots in a register (the wave owns
  * its 64 queries; lanes l and l + 32 hold the two row halves of a query and
  * own half of its kQrSlots slots each), so no LDS atomic is on the scan's
@@ -1128,7 +1135,7 @@ struct QrShare {
 };
 /* dynamic LDS (candidates); the ring is static */
 constexpr size_t kQrLds = kQrNW * sizeof(QrCand) + sizeof(QrShare);
-static_assert(kQrLds + (size_t)2 * kQrTps * (8192 + 128) + (size_t)2 * kQrNW * kQrTps * kQrProgs * 4 <=
+static_assert(kQrLds + (size_t)2 * kQrTps * (8192 + 128 + 128) + (size_t)2 * kQrNW * kQrTps * kQrProgs * 4 <=
               160 * 1024, "query-resident scan LDS");
 
 /* acc <= h admits every row with d = ||q||^2 + acc <= tau: the margin
@@ -1154,10 +1161,11 @@ __device__ __forceinline__ void seed_qr(const f32x16 &xn, uint32_t w0, uint32_t 
     }
 }
 
-template <int NW>
+template <int NW, bool INL, bool PREPASS = false>
 __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(2)))
 void k_scan_bf16_qr(ScanArgs A) {
     static_assert(NW == 8, "one 1 KB chunk of every stage tile per wave");
+    static_assert(2 + kQrTps / 2 <= NW, "waves copying the stage's ranks");
     static_assert(kQrTps * kQrProgs == 64, "one dword copy of a wave's stage words");
     /* the ring's two slots are separate arrays, so the compiler can tell
      * that reads of one slot do not depend on the copy into the other
@@ -1166,6 +1174,9 @@ void k_scan_bf16_qr(ScanArgs A) {
     __shared__ __attribute__((aligned(16))) float nring0[kQrTps * 32], nring1[kQrTps * 32];
     __shared__ __attribute__((aligned(16))) uint32_t wring0[NW * kQrTps * kQrProgs],
         wring1[NW * kQrTps * kQrProgs];
+    /* inline predicates: the stage's row ranks in the predicate's column */
+    __shared__ __attribute__((aligned(16))) uint32_t rring0[INL ? kQrTps * 32 : 4],
+        rring1[INL ? kQrTps * 32 : 4];
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     QrCand *Cs = (QrCand *)smem;                                  /* [NW] */
     QrShare &S = *(QrShare *)(smem + NW * sizeof(QrCand));
@@ -1194,13 +1205,19 @@ void k_scan_bf16_qr(ScanArgs A) {
     float tq[2], qn[2], hq[2];
     const uint32_t *wr[2];
     int32_t prog[2];
+    uint32_t rlo[2] = {0u, 0u}, rlen[2] = {0u, 0u};   /* inline: rank range */
     load_query_state(A, qbase, tq, qn, wr);
 #pragma unroll
     for (int qb = 0; qb < 2; qb++) {
         const int q = qbase + (l & 31) + 32 * qb;
         const bool v = on && q < A.nq;
         hq[qb] = v ? qr_threshold(A, qn[qb], tq[qb]) : -__builtin_inff();
-        prog[qb] = v ? A.pofq[q] : -1;
+        prog[qb] = (v && !INL) ? A.pofq[q] : -1;
+        if (INL && v) {
+            const InlPred ip = A.inl[q];
+            rlo[qb] = ip.lo;
+            rlen[qb] = ip.hi - ip.lo;
+        }
 #pragma unroll
         for (int s = 0; s < 8; s++) {
             u32x4 raw = {0u, 0u, 0u, 0u};
@@ -1236,7 +1253,7 @@ void k_scan_bf16_qr(ScanArgs A) {
             nslot++;
         }
     }
-    const bool pen = nslot <= kQrProgs;   /* uniform */
+    const bool pen = !INL && nslot <= kQrProgs;   /* uniform */
     /* B_pen of the two query blocks: lane l holds k-slots 8 half .. +8 of
      * query column l & 31 */
     bf16x8 bpen[2];
@@ -1253,7 +1270,8 @@ void k_scan_bf16_qr(ScanArgs A) {
     /* copy of stage st into a ring slot: chunk `wave` of each of the
      * stage's tiles, the stage's norms (two tiles per dword copy) and the
      * wave's filter words [tile][slot] */
-    auto load_stage = [&](uint32_t st, u32x4 *ring, float *nring, uint32_t *wring) {
+    auto load_stage = [&](uint32_t st, u32x4 *ring, float *nring, uint32_t *wring,
+                          uint32_t *rring) {
         const uint32_t t0 = c0 + st * kQrTps;
 #pragma unroll
         for (int i = 0; i < kQrTps; i++) {
@@ -1271,6 +1289,12 @@ void k_scan_bf16_qr(ScanArgs A) {
             const uint32_t t = min(t0 + (uint32_t)(l / kQrProgs), c1 - 1);
             __builtin_amdgcn_global_load_lds(prow + (uint64_t)t * A.stride,
                                              wring + wave * kQrTps * kQrProgs, 4, 0, 0);
+        }
+        if (INL && wave >= 2 && wave < 2 + kQrTps / 2) {
+            const int h = wave - 2;
+            const uint32_t t = min(t0 + 2u * h + (uint32_t)(l >> 5), c1 - 1);
+            const uint64_t row = min((uint64_t)t * A.stride * 32 + (l & 31), A.count - 1);
+            __builtin_amdgcn_global_load_lds(A.irank + row, rring + 2 * h * 32, 4, 0, 0);
         }
     };
     /* stage top: this wave's copies for the stage have landed; after the
@@ -1309,7 +1333,7 @@ void k_scan_bf16_qr(ScanArgs A) {
     };
 
     auto compute_stage = [&](uint32_t st, const u32x4 *ring, const float *nring,
-                             const uint32_t *wring) {
+                             const uint32_t *wring, const uint32_t *rring) {
 #pragma unroll
         for (int i = 0; i < kQrTps; i++) {
             const uint32_t t = c0 + st * kQrTps + (uint32_t)i;
@@ -1331,7 +1355,30 @@ void k_scan_bf16_qr(ScanArgs A) {
             }
             const uint64_t row0 = (uint64_t)pt * 32;
             f32x16 acc0, acc1;
-            if (pen && row0 + 32 <= A.count) {
+            if (INL) {
+                /* rank range test per element: lo <= rank < lo + len */
+                const u32x4 *rr = (const u32x4 *)(rring + i * 32);
+                uint32_t rk[16];
+#pragma unroll
+                for (int g = 0; g < 4; g++) {
+                    const u32x4 v = rr[2 * g + half];
+                    rk[4 * g] = v.x;
+                    rk[4 * g + 1] = v.y;
+                    rk[4 * g + 2] = v.z;
+                    rk[4 * g + 3] = v.w;
+                }
+                if (row0 + 32 > A.count) {
+                    /* rows past the index never pass */
+#pragma unroll
+                    for (int e = 0; e < 16; e++)
+                        if (row0 + (uint64_t)acc_row(e, l) >= A.count) rk[e] = 0xFFFFFFFFu;
+                }
+#pragma unroll
+                for (int e = 0; e < 16; e++) {
+                    acc0[e] = (rk[e] - rlo[0] < rlen[0]) ? xn[e] : __builtin_inff();
+                    acc1[e] = (rk[e] - rlo[1] < rlen[1]) ? xn[e] : __builtin_inff();
+                }
+            } else if (pen && row0 + 32 <= A.count) {
                 /* A_pen: row l & 31, slots 8 half .. +8: 2^127 where the
                  * slot's word fails the row */
                 const u32x4 *wp = (const u32x4 *)(wring + wave * kQrTps * kQrProgs + i * kQrProgs +
@@ -1377,7 +1424,18 @@ void k_scan_bf16_qr(ScanArgs A) {
                 gm[5] = c[15];
                 const float mn =
                     fminf(fminf(fminf(fminf(gm[0], gm[1]), gm[2]), fminf(gm[3], gm[4])), gm[5]);
-                if (mn <= h) {
+                if (PREPASS) {
+                    /* the minimum over the lane's 16 rows (the tile's row
+                     * group `half`) per query; +inf when every row of the
+                     * group is filtered (2^127 or +inf) */
+                    const int q = qbase + (l & 31) + 32 * qb;
+                    if (q < A.nq) {
+                        const float dmin = mn >= 0x1p126f ? __builtin_inff()
+                                                          : fmaxf(qn[qb] + mn, 0.f);
+                        const uint32_t grp = t * 2 + (uint32_t)half;
+                        A.gmin[(uint64_t)q * A.ngroups + grp] = make_key(dmin, grp);
+                    }
+                } else if (mn <= h) {
 #pragma unroll
                     for (int g = 0; g < 6; g++)
                         if (gm[g] <= h) {
@@ -1396,18 +1454,19 @@ void k_scan_bf16_qr(ScanArgs A) {
 
     if (l == 0) C.ovf = 0;
     if (tid == 0) S.nsp = 0;
-    if (nst > 0) load_stage(0, ring0, nring0, wring0);
+    if (nst > 0) load_stage(0, ring0, nring0, wring0, rring0);
     /* two stages per iteration: even stages in slot 0, odd in slot 1 */
     for (uint32_t st = 0; st < nst; st += 2) {
         stage_sync();
-        if (st + 1 < nst) load_stage(st + 1, ring1, nring1, wring1);
-        if (on) compute_stage(st, ring0, nring0, wring0);
+        if (st + 1 < nst) load_stage(st + 1, ring1, nring1, wring1, rring1);
+        if (on) compute_stage(st, ring0, nring0, wring0, rring0);
         if (st + 1 >= nst) break;
         stage_sync();
-        if (st + 2 < nst) load_stage(st + 2, ring0, nring0, wring0);
-        if (on) compute_stage(st + 1, ring1, nring1, wring1);
+        if (st + 2 < nst) load_stage(st + 2, ring0, nring0, wring0, rring0);
+        if (on) compute_stage(st + 1, ring1, nring1, wring1, rring1);
     }
 
+    if (PREPASS) return;
     /* flush: each lane its own slots of its two queries (one reservation
      * each), then the shared spill, into pool segment chunk % kPoolShards */
     __syncthreads();
@@ -2454,18 +2513,53 @@ int stage_batch(FvGpuIndex *g, FvGpuBatch *b, int nq, const float *queries,
     }
     if (n_sp) memcpy(h + L.o_sp, spans, n_sp * sizeof(FvGpuSpan));
     if (n_so) memcpy(h + L.o_so, spanoff, n_so * 8);
+    b->inl_rank = false;
     if (b->inl_ok) {
         InlPred *ip = (InlPred *)(h + L.o_inl);
+        /* the rank range of a comparison: less than lb, lb .. ub equal,
+         * ub .. nn greater (nn non-NULL values), NULLs from nn on */
+        const std::vector<double> *sv =
+            (g->d_rank != nullptr && b->inl_col >= 0 && b->inl_col < (int)g->h_sorted.size())
+                ? &g->h_sorted[(size_t)b->inl_col] : nullptr;
+        b->inl_rank = sv != nullptr;
         for (int q = 0; q < nqp; q++) {
             const int p = q < nq ? ps.of_query[q] : -1;
+            memset(&ip[q], 0, sizeof(InlPred));
             ip[q].col = -1;
-            ip[q].mask = 0;
-            ip[q].value = 0.0;
+            ip[q].hi = (uint32_t)g->count;
             if (p >= 0) {
                 const PvOp &op = ps.ops[ps.off[p]];
                 ip[q].col = op.col;
                 ip[q].mask = op.b;
                 ip[q].value = op.value;
+                if (sv != nullptr) {
+                    const double c = op.value;
+                    const uint32_t nn = (uint32_t)sv->size();
+                    uint32_t lb = 0, ub = 0;
+                    if (c == c) {
+                        lb = (uint32_t)(std::lower_bound(sv->begin(), sv->end(), c) - sv->begin());
+                        ub = (uint32_t)(std::upper_bound(sv->begin(), sv->end(), c) - sv->begin());
+                    }
+                    uint32_t lo = 0, hi = 0;
+                    bool one = true;
+                    if (c != c) {
+                        /* no order against a NULL constant: only != passes
+                         * (every non-NULL row) */
+                        if ((op.b >> 4) & 1) hi = nn;
+                    } else {
+                        switch (op.b) {
+                            case 0x01: hi = lb; break;                 /* <  */
+                            case 0x03: hi = ub; break;                 /* <= */
+                            case 0x02: lo = lb; hi = ub; break;        /* == */
+                            case 0x06: lo = lb; hi = nn; break;        /* >= */
+                            case 0x04: lo = ub; hi = nn; break;        /* >  */
+                            default: one = false;                      /* != */
+                        }
+                    }
+                    ip[q].lo = lo;
+                    ip[q].hi = hi;
+                    b->inl_rank = b->inl_rank && one;
+                }
             }
         }
     }
@@ -2589,13 +2683,21 @@ static hipError_t launch_scan_mq(int qtw, bool inl, uint32_t gx, int gy, hipStre
 
 /* query-resident BF16 scan launch (main scan, dim 128, bitmap predicates):
  * kQrNW query tiles per WG, gy = query groups */
-static hipError_t launch_scan_qr(uint32_t gx, int gy, hipStream_t s, const ScanArgs &A0) {
+template <bool PREPASS>
+static hipError_t launch_scan_qr(uint32_t gx, int gy, bool inl, hipStream_t s,
+                                 const ScanArgs &A0) {
     ScanArgs A = A0;
-    const void *f = (const void *)k_scan_bf16_qr<kQrNW>;
+    const void *f = inl ? (const void *)k_scan_bf16_qr<kQrNW, true, PREPASS>
+                        : (const void *)k_scan_bf16_qr<kQrNW, false, PREPASS>;
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kQrLds);
     if (e != hipSuccess) return e;
     const dim3 grid = scan_grid(A, gx, (uint32_t)gy);
-    hipLaunchKernelGGL((k_scan_bf16_qr<kQrNW>), grid, dim3(kQrNW * 64), kQrLds, s, A);
+    if (inl)
+        hipLaunchKernelGGL((k_scan_bf16_qr<kQrNW, true, PREPASS>), grid, dim3(kQrNW * 64), kQrLds,
+                           s, A);
+    else
+        hipLaunchKernelGGL((k_scan_bf16_qr<kQrNW, false, PREPASS>), grid, dim3(kQrNW * 64), kQrLds,
+                           s, A);
     return hipGetLastError();
 }
 
@@ -2714,9 +2816,11 @@ static ScanShape scan_shape(const FvGpuIndex *g, const FvGpuBatch *b, const FvGp
      * batches at every WG count and tile order (1024 queries at 1M x 128:
      * 0.35-0.39 ms against 0.43-0.45 ms with four tiles per 8-wave WG) */
     /* kQrNW (8): the query-resident kernel, kQrNW query tiles per WG, for
-     * batches of at least kQrNW tiles whose predicates are bitmaps */
+     * batches of at least kQrNW tiles whose predicates are bitmaps or
+     * inline rank ranges */
     const bool inl_req = o && o->inline_pred && b->inl_ok && b->inl_col >= 0;
-    const bool qr_ok = prec == FV_GPU_PREC_BF16 && g->dimb == 128 && !inl_req;
+    const bool qr_ok = prec == FV_GPU_PREC_BF16 && g->dimb == 128 &&
+                       (!inl_req || (b->inl_rank && g->d_rank != nullptr));
     int qtw = (o && o->qtw > 0) ? o->qtw
                                 : (qr_ok && b->nqt >= kQrNW ? kQrNW : (b->nqt >= 2 ? 2 : 1));
     if (prec != FV_GPU_PREC_BF16 || g->dimb != 128 || b->nqt < 2) qtw = 1;
@@ -2832,7 +2936,7 @@ static hipError_t launch_main_scan(const ScanLaunch &L, hipStream_t s) {
     } else if (L.i8) {
         launch_scan_i8<false>(L.nw, L.pt, L.inl, L.f8, L.gx, L.nqt, L.shm, s, L.M);
     } else if (L.bf && L.qtw == kQrNW) {
-        const hipError_t e = launch_scan_qr(L.gx, L.gy, s, L.M);
+        const hipError_t e = launch_scan_qr<false>(L.gx, L.gy, L.inl, s, L.M);
         if (e != hipSuccess) return e;
     } else if (L.bf && L.qtw > 1) {
         const hipError_t e = launch_scan_mq<false>(L.qtw, L.inl, L.gx, L.gy, s, L.M);
@@ -2993,6 +3097,7 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
     A.skip = skip;
     A.inl = inl ? b->d_inl_p : nullptr;
     A.icol = inl ? g->d_cols + (uint64_t)b->inl_col * g->count : nullptr;
+    A.irank = (inl && g->d_rank != nullptr) ? g->d_rank + (uint64_t)b->inl_col * g->count : nullptr;
     A.sweep = (o && o->order == 2) ? 1u : 0u;
     /* query-major grid: asked for, or by default for rows above 256 dims
      * (several query tiles re-read each chunk: running them together lets
@@ -3063,8 +3168,12 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
              * much as a few tiles */
             const uint32_t gxp = std::min<uint32_t>(gx, std::max<uint32_t>(1, (stiles + nw - 1) / nw));
             if (i8) launch_scan_i8<true>(nw, pt, inl, f8, gxp, b->nqt, shm_i8, s, P);
-            else if (qtw == kQrNW)   /* the query-resident kernel has no pre-pass form */
-                PV_TRY(launch_scan_mq<true>(2, inl, gxp, (b->nqt + 1) / 2, s, P));
+            else if (qtw == kQrNW) {
+                /* chunks of at least a few ring stages of sampled tiles */
+                const uint32_t gq = std::min<uint32_t>(
+                    gx, std::max<uint32_t>(1, stiles / (4 * kQrTps)));
+                PV_TRY(launch_scan_qr<true>(gq, gy, inl, s, P));
+            }
             else if (qtw > 1) PV_TRY(launch_scan_mq<true>(qtw, inl, gxp, gy, s, P));
             else launch_scan_bf16<true>(nw, inl, gxp, b->nqt, shm_bf, s, P);
         } else {
@@ -3388,7 +3497,8 @@ static std::vector<uint64_t> capture_signature(const FvGpuBatch *b) {
             (uint64_t)(uintptr_t)b->d_qd,
             (uint64_t)b->nq, (uint64_t)b->topb, (uint64_t)(b->nprog > 0),
             b->cap_pool, (uint64_t)b->ngroups, (uint64_t)b->stride,
-            (uint64_t)b->inl_ok, (uint64_t)(int64_t)b->inl_col, (uint64_t)b->i8_ok,
+            (uint64_t)b->inl_ok, (uint64_t)(int64_t)b->inl_col, (uint64_t)b->inl_rank,
+            (uint64_t)b->i8_ok,
             (uint64_t)(uintptr_t)b->d_inl_p, (uint64_t)(uintptr_t)b->d_topbq_p,
             (uint64_t)b->last_route, gather ? b->max_total : 0,
             (uint64_t)(gather && b->all_spans), (uint64_t)(gather && b->any_dedup)};
@@ -3559,7 +3669,8 @@ static int tune_batch(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *base, 
     const bool mq = prec == FV_GPU_PREC_BF16 && g->dimb == 128 && b->nqt >= 2;
     /* the query-resident kernel: bitmap predicates, at least kQrNW tiles */
     const bool qr = mq && b->nqt >= kQrNW &&
-                    !(o0.inline_pred && b->inl_ok && b->inl_col >= 0);
+                    (!(o0.inline_pred && b->inl_ok && b->inl_col >= 0) ||
+                     (b->inl_rank && g->d_rank != nullptr));
     const int qtws[4] = {1, 2, 4, kQrNW};
     const bool wv = (prec == FV_GPU_PREC_BF16 || prec == FV_GPU_PREC_SQ8) && g->dim <= 256;
     for (int qi = 0; qi < (mq ? (qr ? 4 : 3) : 1); qi++)

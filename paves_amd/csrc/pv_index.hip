@@ -218,6 +218,16 @@ __global__ __launch_bounds__(256) void k_derive_i8(const float *__restrict__ v, 
     if (l == 0) norm8[row] = fp8 ? sf : (float)ss;
 }
 
+/* rank[c][prows[c][i]] = i: each row's position in its column's posting
+ * order */
+__global__ void k_rank(const uint32_t *__restrict__ prows, uint64_t count, uint64_t nt,
+                       uint32_t *__restrict__ rank) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nt) return;
+    const uint64_t c = i / count;
+    rank[c * count + prows[i]] = (uint32_t)(i - c * count);
+}
+
 /* level-0 lists [cnt, neigh...] (stride l0_stride) -> padded rows of
  * nbr_stride ids, 0xFFFFFFFF after the count; ids >= count are dropped. */
 __global__ void k_pack_l0(const uint32_t *__restrict__ l0, uint32_t l0_stride,
@@ -269,6 +279,7 @@ extern "C" void fv_gpu_index_free(FvGpuIndex *g) {
     hipFree(g->d_i8mid);
     hipFree(g->d_cols);
     hipFree(g->d_prows);
+    hipFree(g->d_rank);
     hipFree(g->d_nbr);
     hipFree(g->d_ones);
     if (g->sync_batch) {
@@ -455,6 +466,34 @@ static FvGpuIndex *upload_impl(const float *vectors, const double *cols,
             UP_TRY(hipMalloc(&g->d_prows, (size_t)ncols * count * 4));
             UP_TRY(hipMemcpyAsync(g->d_prows, post_rows, (size_t)ncols * count * 4,
                                   hipMemcpyHostToDevice, s));
+            /* ranks in posting order, when every column's postings list the
+             * non-NULL values ascending and then the NULLs */
+            bool sorted = count > 0 && count < 0xFFFFFFFFull;
+            std::vector<std::vector<double>> sv((size_t)ncols);
+            for (int c = 0; c < ncols && sorted; c++) {
+                const double *cv = cols + (size_t)c * count;
+                const uint32_t *pr = post_rows + (size_t)c * count;
+                std::vector<double> &v = sv[(size_t)c];
+                v.reserve(count);
+                uint64_t i = 0;
+                for (; i < count; i++) {
+                    if (pr[i] >= count) { sorted = false; break; }
+                    const double x = cv[pr[i]];
+                    if (x != x) break;
+                    if (!v.empty() && x < v.back()) { sorted = false; break; }
+                    v.push_back(x);
+                }
+                for (; i < count && sorted; i++)
+                    if (pr[i] >= count || cv[pr[i]] == cv[pr[i]]) sorted = false;
+            }
+            if (sorted) {
+                UP_TRY(hipMalloc(&g->d_rank, (size_t)ncols * count * 4));
+                const uint64_t nt = (uint64_t)ncols * count;
+                hipLaunchKernelGGL(k_rank, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s,
+                                   g->d_prows, count, nt, g->d_rank);
+                UP_TRY(hipGetLastError());
+                g->h_sorted = std::move(sv);
+            }
         }
     }
     /* bitmap row used by queries without a predicate */

@@ -67,8 +67,13 @@ struct InlPred {
     int32_t col;
     int32_t mask;   /* kCmpPassMask of the comparison */
     double value;
+    /* the same test on the row's rank in the column's posting order (non-NULL
+     * values ascending, NULLs last): pass iff lo <= rank < hi (set when the
+     * index holds ranks and the comparison is one rank range) */
+    uint32_t lo, hi;
+    uint32_t pad[2];
 };
-static_assert(sizeof(InlPred) == 16, "InlPred layout");
+static_assert(sizeof(InlPred) == 32, "InlPred layout");
 
 /* status bits per query */
 constexpr uint32_t kStOverflow = 1u;   /* candidate pool overflowed */
@@ -209,6 +214,8 @@ struct FvGpuBatch {
      * inl_col, evaluated inside the scan instead of through a bitmap */
     bool inl_ok = false;
     int inl_col = -1;
+    /* every inline predicate is one rank range of inl_col (InlPred.lo/hi) */
+    bool inl_rank = false;
     const pv::InlPred *d_inl_p = nullptr;  /* [nqp] in the staging area */
     /* a captured run (FvGpuBruteOpts.graph): replayed while the options
      * it was captured with and every device buffer and size baked into its
@@ -301,6 +308,12 @@ struct FvGpuIndex {
     float *d_norm = nullptr;
     double *d_cols = nullptr;
     uint32_t *d_prows = nullptr;
+    /* rank of every row in its column's posting order [ncols][count] (when
+     * the postings are given and sorted: non-NULL values ascending, NULLs
+     * last), and on the host each column's sorted non-NULL values, to turn
+     * a one-comparison predicate into a rank range at staging */
+    uint32_t *d_rank = nullptr;
+    std::vector<std::vector<double>> h_sorted;
     uint32_t *d_nbr = nullptr;
     uint32_t *d_ones = nullptr;  /* all-ones bitmap row [count_pad/32] */
     /* INT8 copy of an SQ8 index, in the MFMA-tiled int8 layout over dimp

@@ -12,7 +12,10 @@ counts, truncated):
 * non-integer rows at 100 dims (zero-padded to 128; certified re-rank);
 * a batch that is not a multiple of the WG's eight query tiles;
 * the query-major grid, several WGs per CU and hipGraph replays, each equal
-  to the default launch.
+  to the default launch;
+* inline one-comparison predicates as rank ranges of the column's posting
+  order (<, <=, ==, >=, > with ties and NULLs, per-query k), and a batch
+  with != (two ranges) that takes the double-precision inline scan.
 """
 import numpy as np
 import pytest
@@ -147,5 +150,94 @@ def test_float_rows_dim100_partial_group():
             oh, oc, ot, _ = O.brute_batch(d["base"], cols, d["queries"], b.qd, b.nodes,
                                           b.children, b.invals, topb)
             assert_same(out, cnt, trunc, oh, oc, ot, f"dim 100 topb={topb}")
+    finally:
+        g.close()
+
+
+def _inline_world(n, nq, seed, col, nan_frac):
+    d, cols, post = _setup(n, nq, seed=seed, nan_frac=nan_frac)
+    rng = np.random.default_rng(seed + 1)
+    kq = rng.choice(np.array([1, 10, 100], np.uint32), nq)
+    ops = [E.CMP_LT, E.CMP_LE, E.CMP_EQ, E.CMP_GE, E.CMP_GT]
+    vals = cols[col][np.isfinite(cols[col])]
+    preds = []
+    for i in range(nq):
+        if i % 11 == 0:
+            preds.append(None)
+            continue
+        op = ops[i % 5]
+        # constants taken from the column (ties at the boundary) or between
+        # its values; == only on existing values
+        c = float(vals[rng.integers(len(vals))])
+        if op != E.CMP_EQ and i % 3 == 0:
+            c = float(rng.uniform(0, 1000)) if col == synth.COL_PRICE else c + 0.5
+        preds.append(E.cmp(col, op, c))
+    b = E.concat([E.serialize(p, n, post) for p in preds], n)
+    return d, cols, post, kq, b
+
+
+def _check_k(out, cnt, trunc, d, cols, b, kq):
+    for k in np.unique(kq):
+        qs = np.flatnonzero(kq == k)
+        oh, oc, ot, _ = O.brute_batch(d["base"], cols, d["queries"][qs], b.qd[qs], b.nodes,
+                                      b.children, b.invals, int(k))
+        for j, q in enumerate(qs):
+            c = int(cnt[q])
+            assert c == int(oc[j]) and int(trunc[q]) == int(ot[j]), (q, k, c, int(oc[j]))
+            assert np.array_equal(out[q, :c]["node"], oh[j, :c]["node"]), (q, k)
+            assert np.array_equal(out[q, :c]["dist"].view(np.uint32),
+                                  oh[j, :c]["dist"].view(np.uint32)), (q, k)
+
+
+@pytest.mark.parametrize("col,nan_frac", [(synth.COL_PRICE, 0.05), (synth.COL_CATEGORY, 0.0)])
+def test_inline_rank_ranges(col, nan_frac):
+    """One comparison per query (<, <=, ==, >=, >; constants on and between
+    the column's values; NULLs; no predicate) evaluated as rank ranges in the
+    query-resident scan, per-query k in {1, 10, 100}, eager and captured."""
+    n, nq = 20_011, 1000
+    d, cols, post, kq, b = _inline_world(n, nq, 60 + col, col, nan_frac)
+    g = E.GpuIndex(d["base"], cols, post.rows, precision=L.FV_GPU_PREC_BF16)
+    try:
+        bh = g.stage_ex(d["queries"], b, kq)
+        try:
+            ref = None
+            for graph in (0, 1, 1):
+                g.run_brute(bh, L.BruteOpts(route=1, inline_pred=1, qtw=8, graph=graph))
+                g.sync()
+                out, cnt, flag, rc = g.fetch(bh, nq, int(kq.max()))
+                assert rc == 0
+                if ref is None:
+                    _check_k(out, cnt, flag, d, cols, b, kq)
+                    ref = (out, cnt, flag)
+                else:
+                    for x, y in zip((out, cnt, flag), ref):
+                        assert np.array_equal(x, y)
+        finally:
+            g.free_batch(bh)
+    finally:
+        g.close()
+
+
+def test_inline_not_equal_falls_back():
+    """!= is two rank ranges: such a batch takes the double-precision inline
+    scan and still equals the oracle."""
+    n, nq = 12_007, 600
+    d, cols, post = _setup(n, nq, seed=71, nan_frac=0.05)
+    rng = np.random.default_rng(72)
+    kq = np.full(nq, 10, np.uint32)
+    preds = [E.cmp(synth.COL_PRICE, E.CMP_NE if i % 2 else E.CMP_LT, float(rng.uniform(0, 1000)))
+             for i in range(nq)]
+    b = E.concat([E.serialize(p, n, post) for p in preds], n)
+    g = E.GpuIndex(d["base"], cols, post.rows, precision=L.FV_GPU_PREC_BF16)
+    try:
+        bh = g.stage_ex(d["queries"], b, kq)
+        try:
+            g.run_brute(bh, L.BruteOpts(route=1, inline_pred=1, qtw=8))
+            g.sync()
+            out, cnt, flag, rc = g.fetch(bh, nq, 10)
+            assert rc == 0
+            _check_k(out, cnt, flag, d, cols, b, kq)
+        finally:
+            g.free_batch(bh)
     finally:
         g.close()
