@@ -312,6 +312,25 @@ def roofline_only(args):
     print(json.dumps(m), flush=True)
 
 
+def batched_scan_only(args):
+    """The multi-tile scans of 1024-query batches alone (for rocprofv3): the
+    query-resident kernel (library default) and the two-tiles-per-WG kernel,
+    each timed by measure_batched_scan; prints one JSON line."""
+    d = synth.gmm128(args.n, args.learned_batch)
+    cols = synth.columns(d)
+    rows, vals = synth.postings(cols)
+    post = E.Postings(rows, vals)
+    batches = build_batches(d, post, args.n, args.learned_batch, args.learned_batch, args.k)
+    g = E.GpuIndex(d["base"], cols, rows, precision=L.FV_GPU_PREC_BF16, device=0)
+    staged = [g.stage(d["queries"][s:e], b, args.k) for s, e, b in batches]
+    bf = L.BruteOpts(precision=L.FV_GPU_PREC_BF16, route=1)
+    m = measure_batched_scan(g, staged, batches, bf, args, reps=10)
+    for bh in staged:
+        g.free_batch(bh)
+    g.close()
+    print(json.dumps(m), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -344,6 +363,8 @@ def main():
     ap.add_argument("--roofline-only", action="store_true",
                     help="only the isolated scan runs (the rocprofv3 target)")
     ap.add_argument("--copy-calib", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--batched-scan-only", action="store_true",
+                    help="time the multi-tile scans of one 1024-query batch (for rocprofv3)")
     ap.add_argument("--graph-roofline-only", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--graph-file", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--graph-ef", type=int, default=80,
@@ -365,6 +386,9 @@ def main():
         return
     if args.roofline_only:
         roofline_only(args)
+        return
+    if args.batched_scan_only:
+        batched_scan_only(args)
         return
     if args.graph_roofline_only:
         graph_roofline_only(args)
