@@ -83,6 +83,7 @@ struct GraphArgs {
     uint64_t *spill;     /* [nq][scap + fcap] frontier spill stores */
     uint32_t scap;       /* spilled entries a store accepts */
     uint32_t *dbg;       /* [nq][4] heads expanded, visited, results, frontier entries lost */
+    const uint32_t *order; /* optional [nq]: query of workgroup b (heaviest search first) */
 };
 
 __device__ __forceinline__ uint32_t hash_slot(uint32_t node) {
@@ -457,7 +458,8 @@ __device__ __forceinline__ void dist_all(const GraphArgs &A, const float *qsh,
 __global__ __launch_bounds__(kGT) __attribute__((amdgpu_waves_per_eu(3)))
 void k_graph(GraphArgs A) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
-    const int q = blockIdx.x, t = threadIdx.x, w = t >> 6, l = t & 63;
+    const int q = A.order ? (int)A.order[blockIdx.x] : (int)blockIdx.x;
+    const int t = threadIdx.x, w = t >> 6, l = t & 63;
     if (A.route && A.route[q] == 0) return; /* routed to the brute path */
     const uint32_t efq = A.efq ? max(1u, min(A.efq[q], A.ef)) : A.ef;
     const int D = A.dim;
@@ -669,6 +671,59 @@ void k_graph(GraphArgs A) {
     }
 }
 
+/*
+ * Launch order of a graph batch, heaviest search first (the longest-
+ * processing-time rule of the reference worker, worker.c:206-232: one query
+ * is one workgroup and workgroups start in index order, so descending
+ * expected work makes the last wave of workgroups finish together instead
+ * of leaving a long search to start late). Expected work ~ ef / selectivity
+ * (worker.c:221: want * 1000 / sel_permille); the selectivity is measured:
+ * passing rows of each program from its bitmap (k_prog_count).
+ */
+/* popcount of each program's bitmap row, one block per program (pv_learn.hip) */
+__global__ void k_prog_count(const uint32_t *__restrict__ bitmap, uint64_t words,
+                             uint32_t *__restrict__ cnt);
+
+constexpr int kOrderMax = 4096;
+__global__ __launch_bounds__(1024) void k_lpt_order(const int32_t *__restrict__ pofq,
+                                                    const uint32_t *__restrict__ pcnt,
+                                                    const uint32_t *__restrict__ efq, uint32_t ef,
+                                                    uint64_t count, int nq,
+                                                    uint32_t *__restrict__ order) {
+    __shared__ uint64_t key[kOrderMax];
+    int n2 = 1;
+    while (n2 < nq) n2 <<= 1;
+    for (int i = threadIdx.x; i < n2; i += blockDim.x) {
+        uint64_t k = ~0ull;
+        if (i < nq) {
+            const int p = pofq[i];
+            const double pass = p < 0 ? (double)count : (double)max(pcnt[p], 1u);
+            const double e = efq ? (double)max(1u, min(efq[i], ef)) : (double)ef;
+            const float w = (float)(e * (double)count / pass);
+            /* descending work, then ascending query: ascending key order */
+            k = ((uint64_t)(~__float_as_uint(w)) << 32) | (uint32_t)i;
+        }
+        key[i] = k;
+    }
+    __syncthreads();
+    for (int size = 2; size <= n2; size <<= 1)
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int i = threadIdx.x; i < n2; i += blockDim.x) {
+                const int j = i ^ stride;
+                if (j > i) {
+                    const bool up = (i & size) == 0;
+                    const uint64_t a = key[i], b = key[j];
+                    if ((a > b) == up) {
+                        key[i] = b;
+                        key[j] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    for (int i = threadIdx.x; i < nq; i += blockDim.x) order[i] = (uint32_t)key[i];
+}
+
 size_t graph_shmem(int dim, int fcap, uint32_t ef) {
     return (((size_t)dim * 4 + 15) & ~(size_t)15) + (size_t)fcap * 8 + (size_t)ef * 8 +
            2 * (size_t)kGT * 8 + kHashSize * 4 + kMaxNew * 4 + kMaxNew * 4 +
@@ -740,6 +795,7 @@ int run_graph_ex(FvGpuIndex *g, FvGpuBatch *b, uint32_t ef, uint64_t max_visited
     A.spill = b->d_fspill.as<uint64_t>();
     A.scap = (uint32_t)scap;
     A.dbg = dbg;
+    A.order = nullptr;
     if (shm > 64 * 1024)
         PV_TRY(hipFuncSetAttribute((const void *)k_graph,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
@@ -747,6 +803,19 @@ int run_graph_ex(FvGpuIndex *g, FvGpuBatch *b, uint32_t ef, uint64_t max_visited
     g->ev_stream = gs;
     ev_record(g, 0);
     if (!x.bitmap_ready) PV_TRY(launch_bitmap(g, b, gs));
+    /* heaviest searches first when the batch has predicates and more
+     * queries than fill the chip at once */
+    if (b->nprog > 0 && b->nq > 1 && b->nq <= kOrderMax && !b->capturing) {
+        PV_TRY(b->d_order.ensure(((size_t)b->nq + (size_t)b->nprog) * 4));
+        uint32_t *ord = b->d_order.as<uint32_t>();
+        uint32_t *pc = ord + b->nq;
+        hipLaunchKernelGGL(k_prog_count, dim3((unsigned)b->nprog), dim3(256), 0, gs,
+                           b->d_bitmap.as<uint32_t>(), words, pc);
+        hipLaunchKernelGGL(k_lpt_order, dim3(1), dim3(1024), 0, gs, b->d_pofq, pc, x.efq, ef,
+                           g->count, b->nq, ord);
+        PV_TRY(hipGetLastError());
+        A.order = ord;
+    }
     hipLaunchKernelGGL(k_graph, dim3(b->nq), dim3(kGT), shm, gs, A);
     PV_TRY(hipGetLastError());
     ev_record(g, 5);

@@ -273,6 +273,8 @@ struct FvGpuBatch {
     int last_route = 0;
     hipStream_t stream = nullptr; /* one of the index's batch streams */
     pv::DevBuf d_visit;   /* graph: overflow visited bitmaps */
+    pv::DevBuf d_order;   /* graph: launch order [nq] (heaviest search first) and
+                             passing rows per program [nprog] */
     pv::DevBuf d_fspill;  /* graph: frontier spill stores */
     pv::DevBuf d_progcnt; /* [nprog][bitmap blocks] passing rows per program and
                              bitmap block (learned features) */
