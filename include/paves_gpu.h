@@ -235,6 +235,36 @@ typedef struct FvGpuStats {
 void fv_gpu_set_timing(FvGpuIndex *g, int on);
 void fv_gpu_last_stats(const FvGpuIndex *g, FvGpuStats *st);
 
+/* Service-time feedback of an index. No counterpart in the engine: the
+ * table the reference worker keeps per index for the planner
+ * (FvFeedbackEntry.gpu_q_us / gpu_batch_us / gpu_batch_n16, arbiter.h:
+ * 75-104, filled at worker.c:296-333 from host wall time around each
+ * batch), kept here by the library from device event time around every
+ * fv_gpu_brute_batch / fv_gpu_graph_batch call, with each query's
+ * selectivity measured from its predicate bitmap (or its span total). All
+ * values are EWMAs, 0 = no sample yet. Strategy 0 = brute force, 1 =
+ * graph; selectivity buckets (arbiter.h:50-60): >= 20%, 5-20%, 1-5%,
+ * 0.2-1%, < 0.2%. */
+#define FV_GPU_SEL_BUCKETS 5
+typedef struct FvGpuFeedback {
+    double q_us[2][FV_GPU_SEL_BUCKETS]; /* amortized per-query service time, us */
+    double batch_us[2];                 /* whole-batch device time, us */
+    double batch_n[2];                  /* queries per batch */
+    uint64_t batches[2];                /* batches recorded */
+} FvGpuFeedback;
+/* Copy the index's table (0 on success, -1 for a NULL argument). */
+int fv_gpu_feedback(const FvGpuIndex *g, FvGpuFeedback *out);
+/* One batch into a table, the reference worker's rule: per query weight w =
+ * want * 1000 / max(sel_permille, 1) for the graph (expected work), 1 for
+ * brute force; per bucket b with n_b queries the sample (T / max w) x (mean
+ * w over b) enters with weight n_b / (n_b + 64); the batch time and size
+ * with weight 1/8; an empty field takes the sample. want may be NULL for
+ * brute force. */
+void fv_gpu_feedback_update(FvGpuFeedback *fb, int strategy, int n, const int32_t *sel_permille,
+                            const uint32_t *want, double batch_us);
+/* The selectivity bucket of a per-mille selectivity (arbiter.h:50-60). */
+int fv_gpu_sel_bucket(int32_t sel_permille);
+
 /* fv_gpu_brute_batch without any memory allocation. No counterpart in the
  * reference: for callers that cannot allocate (code under HIP graph
  * capture, tight serving loops, harnesses that forbid hipMalloc). Same

@@ -42,7 +42,8 @@ NODE_CMP, NODE_IN, NODE_AND, NODE_OR, NODE_NOT = range(5)
 REF_SYMBOLS = ["fv_gpu_available", "fv_gpu_index_upload", "fv_gpu_index_free",
                "fv_gpu_index_device", "fv_gpu_index_precision", "fv_gpu_brute_batch",
                "fv_gpu_graph_batch", "fv_gpu_tile_variant_qt", "fv_gpu_mem_probe"]
-EXT_SYMBOLS = ["fv_gpu_set_timing", "fv_gpu_last_stats", "fv_gpu_brute_batch_ex",
+EXT_SYMBOLS = ["fv_gpu_set_timing", "fv_gpu_last_stats", "fv_gpu_feedback",
+               "fv_gpu_feedback_update", "fv_gpu_sel_bucket", "fv_gpu_brute_batch_ex",
                "fv_gpu_graph_batch_ex", "fv_gpu_batch_stage", "fv_gpu_batch_run_brute",
                "fv_gpu_batch_run_graph", "fv_gpu_sync", "fv_gpu_batch_fetch",
                "fv_gpu_batch_free", "fv_gpu_copy_bench", "fv_gpu_read_bench", "fv_hnsw_build", "fv_hnsw_info",
@@ -80,6 +81,13 @@ class LearnedOpts(C.Structure):
                  route_bias=0.0, max_visited=0, precision=-1):
         super().__init__(k, mode, ef_fixed, ef_min, ef_max, ef_scale, route_bias, max_visited,
                          precision)
+
+
+class Feedback(C.Structure):
+    """FvGpuFeedback: service-time EWMAs (us) per strategy (0 brute, 1
+    graph) and selectivity bucket, batch time and size, batches recorded."""
+    _fields_ = [("q_us", (C.c_double * 5) * 2), ("batch_us", C.c_double * 2),
+                ("batch_n", C.c_double * 2), ("batches", C.c_uint64 * 2)]
 
 
 class Stats(C.Structure):
@@ -135,6 +143,13 @@ def load() -> C.CDLL:
     lib.fv_gpu_mem_probe.argtypes = [_P, _P]
     lib.fv_gpu_set_timing.argtypes = [_P, C.c_int]
     lib.fv_gpu_last_stats.argtypes = [_P, C.POINTER(Stats)]
+    lib.fv_gpu_feedback.restype = C.c_int
+    lib.fv_gpu_feedback.argtypes = [_P, C.POINTER(Feedback)]
+    lib.fv_gpu_feedback_update.restype = None
+    lib.fv_gpu_feedback_update.argtypes = [C.POINTER(Feedback), C.c_int, C.c_int, _P, _P,
+                                           C.c_double]
+    lib.fv_gpu_sel_bucket.restype = C.c_int
+    lib.fv_gpu_sel_bucket.argtypes = [C.c_int32]
     lib.fv_gpu_batch_stage.restype = _P
     lib.fv_gpu_batch_stage.argtypes = [_P, C.c_int, _P, _P, _P, _P, C.c_int, _P, C.c_int,
                                        _P, C.c_int, _P, C.c_int, _P, C.c_int, C.c_uint32,

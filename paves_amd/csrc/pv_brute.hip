@@ -3308,6 +3308,70 @@ static void fill_stats_times(FvGpuIndex *g) {
 
 using namespace pv;
 
+namespace pv {
+
+/* popcount of each program's bitmap row (pv_learn.hip) */
+__global__ void k_prog_count(const uint32_t *__restrict__ bitmap, uint64_t words,
+                             uint32_t *__restrict__ cnt);
+
+void feedback_begin(FvGpuIndex *g, hipStream_t s) {
+    if (g->fb_ev[0] == nullptr &&
+        (hipEventCreate(&g->fb_ev[0]) != hipSuccess || hipEventCreate(&g->fb_ev[1]) != hipSuccess)) {
+        (void)hipGetLastError();
+        return;
+    }
+    (void)hipEventRecord(g->fb_ev[0], s);
+}
+
+void feedback_end(FvGpuIndex *g, FvGpuBatch *b, hipStream_t s, bool bitmaps) {
+    if (g->fb_ev[1] == nullptr) return;
+    (void)hipEventRecord(g->fb_ev[1], s);
+    if (bitmaps && b->nprog > 0 &&
+        b->d_order.ensure(((size_t)b->nq + (size_t)b->nprog) * 4) == hipSuccess) {
+        hipLaunchKernelGGL(k_prog_count, dim3((unsigned)b->nprog), dim3(256), 0, s,
+                           b->d_bitmap.as<uint32_t>(), g->count_pad / 32,
+                           b->d_order.as<uint32_t>() + b->nq);
+        (void)hipGetLastError();
+    }
+}
+
+void feedback_record(FvGpuIndex *g, FvGpuBatch *b, int strategy, const FvGpuQueryDesc *qd,
+                     uint32_t want, bool bitmaps) {
+    if (g->fb_ev[0] == nullptr || g->count == 0) return;
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, g->fb_ev[0], g->fb_ev[1]) != hipSuccess) {
+        (void)hipGetLastError();
+        return;
+    }
+    const int nq = b->nq;
+    std::vector<uint32_t> pc;
+    if (bitmaps && b->nprog > 0) {
+        pc.resize((size_t)b->nprog);
+        if (hipMemcpy(pc.data(), b->d_order.as<uint32_t>() + nq, pc.size() * 4,
+                      hipMemcpyDeviceToHost) != hipSuccess) {
+            (void)hipGetLastError();
+            return;
+        }
+    }
+    /* program of each query, from the host copy of the staging area */
+    const int32_t *pofq = (const int32_t *)(b->h_stage.as<unsigned char>() + b->lay.o_pofq);
+    std::vector<int32_t> sel((size_t)nq);
+    std::vector<uint32_t> wq((size_t)nq, want);
+    const double n = (double)g->count;
+    for (int q = 0; q < nq; q++) {
+        const int p = pofq[q];
+        double pass = n;
+        if (p >= 0)
+            pass = !pc.empty() ? (double)pc[(size_t)p]
+                               : (double)std::min<uint64_t>(qd[q].total, g->count);
+        sel[(size_t)q] = (int32_t)std::lround(std::min(pass, n) * 1000.0 / n);
+    }
+    std::lock_guard<std::mutex> lk(g->fb_mu);
+    fv_gpu_feedback_update(&g->fb, strategy, nq, sel.data(), wq.data(), (double)ms * 1000.0);
+}
+
+} // namespace pv
+
 static int brute_sync(FvGpuIndex *g, int nq, const float *queries,
                       const FvGpuQueryDesc *qd, const uint8_t *dedup,
                       const FvGpuPredNode *nodes, int nnodes,
@@ -3331,7 +3395,9 @@ static int brute_sync(FvGpuIndex *g, int nq, const float *queries,
                     nchildren, invals, ninvals, spans, nspans_total, spanoff,
                     nspanoff, topb, errbuf, errlen) != 0)
         return -1;
+    feedback_begin(g, b->stream);
     if (run_brute(g, b, opts, 0, errbuf, errlen) != 0) return -1;
+    feedback_end(g, b, b->stream, b->ran_bitmap);
     if (fetch_raw(g, b, errbuf, errlen) != 0) return -1;
     fill_stats_times(g);
     const FvHit *ho;
@@ -3349,6 +3415,7 @@ static int brute_sync(FvGpuIndex *g, int nq, const float *queries,
         for (int q = 0; q < nq; q++) g->last.certified_fail += hs[q] != 0;
         const uint32_t nfail = g->last.certified_fail;
         if (run_brute(g, b, &o2, 1, errbuf, errlen) != 0) return -1;
+        feedback_end(g, b, b->stream, b->ran_bitmap);
         if (fetch_raw(g, b, errbuf, errlen) != 0) return -1;
         fill_stats_times(g);
         g->last.certified_fail = nfail;
@@ -3365,6 +3432,7 @@ static int brute_sync(FvGpuIndex *g, int nq, const float *queries,
         out_cnt[q] = c;
         truncated[q] = hf[q];
     }
+    feedback_record(g, b, 0, qd, topb, b->ran_bitmap);
     return 0;
 }
 

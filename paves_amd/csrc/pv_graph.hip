@@ -870,7 +870,9 @@ static int graph_sync(FvGpuIndex *g, int nq, const float *queries,
         return -1;
     uint32_t *d_dbg = nullptr;
     if (dbg_host) PV_TRY(hipMalloc(&d_dbg, (size_t)nq * 16));
+    feedback_begin(g, b->stream);
     int rc = run_graph_opts(g, b, ef, max_visited, wmax, fcap, d_dbg, errbuf, errlen);
+    if (rc == 0) feedback_end(g, b, b->stream, b->nprog > 0);
     if (rc == 0) {
         b->out_stride = ef;
         std::vector<FvHit> tmp((size_t)nq * ef);
@@ -895,6 +897,7 @@ static int graph_sync(FvGpuIndex *g, int nq, const float *queries,
                 g->last.t_graph_us = ev_elapsed_us(g, 0, 5);
                 g->last.t_total_us = g->last.t_graph_us;
             }
+            feedback_record(g, b, 1, qd2.data(), ef, b->nprog > 0);
         }
     }
     if (d_dbg) hipFree(d_dbg);

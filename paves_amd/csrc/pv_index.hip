@@ -288,6 +288,8 @@ extern "C" void fv_gpu_index_free(FvGpuIndex *g) {
     }
     for (int i = 0; i < 16; i++)
         if (g->ev[i]) hipEventDestroy(g->ev[i]);
+    for (int i = 0; i < 2; i++)
+        if (g->fb_ev[i]) hipEventDestroy(g->fb_ev[i]);
     if (g->stream) hipStreamDestroy(g->stream);
     for (hipStream_t bs : g->bstreams)
         if (bs) hipStreamDestroy(bs);
@@ -556,6 +558,13 @@ extern "C" int fv_gpu_mem_probe(uint64_t *free_b, uint64_t *total_b) {
 
 extern "C" void fv_gpu_set_timing(FvGpuIndex *g, int on) {
     if (g) g->timing = on != 0;
+}
+
+extern "C" int fv_gpu_feedback(const FvGpuIndex *g, FvGpuFeedback *out) {
+    if (g == nullptr || out == nullptr) return -1;
+    std::lock_guard<std::mutex> lk(const_cast<FvGpuIndex *>(g)->fb_mu);
+    *out = g->fb;
+    return 0;
 }
 
 extern "C" void fv_gpu_last_stats(const FvGpuIndex *g, FvGpuStats *st) {

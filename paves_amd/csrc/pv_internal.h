@@ -18,6 +18,7 @@
 #include <stdint.h>
 #include <stdio.h>
 
+#include <mutex>
 #include <vector>
 
 #include "../../include/paves_gpu.h"
@@ -357,6 +358,11 @@ struct FvGpuIndex {
     bool timing = false;
     hipEvent_t ev[16] = {};
     FvGpuStats last = {};
+    /* service-time feedback of the synchronous entry points: the table, its
+     * lock, and the two events bracketing a call's device work */
+    FvGpuFeedback fb = {};
+    std::mutex fb_mu;
+    hipEvent_t fb_ev[2] = {nullptr, nullptr};
 
     FvGpuBatch *sync_batch = nullptr; /* reused by the synchronous ABI */
     FvGpuBatch *last_batch = nullptr; /* batch of the last brute run (stats) */
@@ -418,6 +424,15 @@ struct GraphExtra {
 };
 int run_graph_ex(FvGpuIndex *g, FvGpuBatch *b, uint32_t ef, uint64_t max_visited, int wmax,
                  int fcap, uint32_t *dbg, const GraphExtra &x, char *errbuf, size_t errlen);
+/* pv_brute.hip: service-time feedback of a synchronous call. begin before
+ * its device work on stream s, end after it (also counts the programs'
+ * passing rows when the run built bitmaps), record once the call's results
+ * are on the host (strategy 0 brute, 1 graph; want = hits asked per query).
+ * Failures leave the table as it was; they never fail the call. */
+void feedback_begin(FvGpuIndex *g, hipStream_t s);
+void feedback_end(FvGpuIndex *g, FvGpuBatch *b, hipStream_t s, bool bitmaps);
+void feedback_record(FvGpuIndex *g, FvGpuBatch *b, int strategy, const FvGpuQueryDesc *qd,
+                     uint32_t want, bool bitmaps);
 /* pv_brute.hip: the brute pipeline; skip (device [nq], may be NULL) marks
  * queries it must leave alone; have_phase1 = bitmaps and query prep done;
  * decide (may be NULL): the routing decision, run in the threshold launch,

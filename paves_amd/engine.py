@@ -373,6 +373,12 @@ class GpuIndex:
         self.lib.fv_gpu_last_stats(self.h, C.byref(st))
         return st
 
+    def feedback(self) -> L.Feedback:
+        fb = L.Feedback()
+        if self.lib.fv_gpu_feedback(self.h, C.byref(fb)) != 0:
+            raise GpuError("gpu: feedback")
+        return fb
+
     def brute(self, queries: np.ndarray, b: Batch, topb: int, opts: L.BruteOpts | None = None):
         q = np.ascontiguousarray(queries, dtype=np.float32)
         out = np.zeros((b.nq, topb), dtype=HIT_DT)

@@ -338,3 +338,23 @@ def test_reference_host_glue_drives_engine(tmp_path):
     finally:
         cpu.close(ic)
         gpu.close(ig)
+
+
+def test_feedback_table_fills(data100k, index100k):
+    """fv_gpu_brute_batch and fv_gpu_graph_batch feed the index's service-
+    time table: device time per batch, queries bucketed by the selectivity
+    their bitmaps measure (six classes: buckets 0, 1, 2 and 4)."""
+    d, cols, post = data100k
+    n = d["base"].shape[0]
+    qs = d["queries"][:48]
+    preds = [synth.filter_pred(synth.FILTER_CLASSES[i % 6], d["qcent"][i]) for i in range(48)]
+    b = E.concat([E.serialize(p, n, post) for p in preds], n)
+    before = index100k.feedback()
+    index100k.brute(qs, b, 10, L.BruteOpts(route=1))
+    fb = index100k.feedback()
+    assert fb.batches[0] == before.batches[0] + 1
+    assert fb.batch_us[0] > 0.0 and fb.batch_n[0] > 0.0
+    # all-pass / price<500 (>= 20%), price<100 (10%), category IN / = (3-6%),
+    # price<1 (0.1%)
+    for bk in (0, 1, 2, 4):
+        assert fb.q_us[0][bk] > 0.0, bk
