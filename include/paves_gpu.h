@@ -448,6 +448,47 @@ int fv_hnsw_entries(const float *vectors, int dim, const uint64_t *upper_offs,
 float fv_l2sq(const float *a, const float *b, int dim);
 
 /*
+ * HNSW construction with level 0 on the GPU (the reference builds on the
+ * host only, engine.cpp:698-885; this call replaces fv_hnsw_build for
+ * indexes already uploaded). Host phase: splitmix64 levels, node 0 and
+ * every node with a level above 0 inserted with fv_hnsw_build's semantics
+ * (so the upper levels are complete), then each remaining node's level-0
+ * entry point by greedy descent. Device phase: the remaining nodes in index
+ * order, in batches of min(max_batch, inserted / batch_div) nodes; per
+ * batch, every node's efConstruction search over the level-0 graph as it
+ * stood before the batch (the query cursor's beam, unfiltered), the
+ * heuristic keeping M links, then per target the batch's reverse links in
+ * node order: appended while the list has room, else the list and up to
+ * 64 - cnt incoming links at a time re-pruned to M0 by the heuristic.
+ * Deterministic for threads = 1. M <= 16 (M0 <= 32). g must hold the same
+ * vectors (count x dim FP32 rows); `vectors` is read during the call only.
+ */
+typedef struct FvHnswGpuOpts {
+    int heads;      /* frontier heads per search iteration, 1..8 (1: the
+                       cursor's pop order; default 8) */
+    int exact;      /* 1: unbounded frontier (spill store), 0: 1024 entries */
+    int batch_div;  /* batch = inserted / batch_div (default 8) */
+    int max_batch;  /* default 32768 (also capped by visited-set memory) */
+    int keep;       /* 1: the built level 0 becomes g's graph (default 1);
+                       captured runs of g must then be staged again */
+    int pad0;
+    /* out: seconds of the host phase, of the device phase; batches run */
+    double t_host_s, t_device_s;
+    int batches;
+    int pad;
+} FvHnswGpuOpts;
+FvHnswGraph *fv_hnsw_build_gpu(FvGpuIndex *g, const float *vectors, int M,
+                               int ef_construction, int threads, FvHnswGpuOpts *opts,
+                               char *errbuf, size_t errlen);
+/* The host phase of fv_hnsw_build_gpu alone (the graph its device phase
+ * starts from); with rest_on_host the remaining nodes are then inserted on
+ * the host in index order with fv_hnsw_build's semantics (threads = 1: the
+ * sequential result the device phase reproduces with batches of one). */
+FvHnswGraph *fv_hnsw_build_seeded(const float *vectors, uint64_t count, int dim, int M,
+                                  int ef_construction, int threads, int rest_on_host,
+                                  char *errbuf, size_t errlen);
+
+/*
  * Learned per-query beam width and brute-vs-graph routing (replaces the
  * fixed policy of exec.c:422-424 and the planner's cost race,
  * planner.c:185-346; no counterpart in the reference's GPU object).

@@ -82,6 +82,9 @@ def lib():
                                     C.c_int, _P, _P, _P, C.c_uint32, C.c_uint64, _P, _P, _P]
         L.or_graph_entry.restype = C.c_uint32
         L.or_graph_entry.argtypes = [_P, C.c_int, _P, _P, C.c_uint32, C.c_uint64, C.c_int, _P]
+        L.or_hnsw_batch_insert.restype = C.c_int
+        L.or_hnsw_batch_insert.argtypes = [_P, C.c_uint64, C.c_int, C.c_uint32, C.c_uint32, _P,
+                                           _P, _P, _P, C.c_int]
         _or = L
     return _or
 
@@ -245,6 +248,32 @@ def hnsw_one(vectors, l0, cols, query, nodes, children, invals, proot, entry, ef
                              proot, _ptr(query), entry, ef, max_visited, _ptr(out), _ptr(exh),
                              _ptr(vis))
     return out[:n], int(exh[0]), int(vis[0])
+
+
+def hnsw_batch_insert(vectors, l0, rest, entries, sizes, M, efc):
+    """Level-0 phase of the batched HNSW builder (or_hnsw_batch_insert) from
+    the host phase's lists l0 [n][2M+1]; returns the new lists."""
+    x = np.ascontiguousarray(vectors, dtype=np.float32)
+    out = np.ascontiguousarray(l0, dtype=np.uint32).copy()
+    r = np.ascontiguousarray(rest, dtype=np.uint32)
+    e = np.ascontiguousarray(entries, dtype=np.uint32)
+    sz = np.ascontiguousarray(sizes, dtype=np.int32)
+    rc = lib().or_hnsw_batch_insert(_ptr(x), x.shape[0], x.shape[1], M, efc, _ptr(out), _ptr(r),
+                                    _ptr(e), _ptr(sz), len(sz))
+    assert rc == 0
+    return out
+
+
+def build_schedule(n, nrest, batch_div, max_batch):
+    """Batch sizes of the device builder: min(max_batch, inserted // batch_div),
+    at least 1, over the nodes left after the host phase."""
+    sizes, done, have = [], 0, n - nrest
+    while done < nrest:
+        b = min(nrest - done, max(1, min(have // batch_div, max_batch)))
+        sizes.append(b)
+        done += b
+        have += b
+    return sizes
 
 
 def graph_entry(vectors, upper_offs, upper, M, entry_node, max_level, query):

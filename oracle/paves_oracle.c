@@ -19,6 +19,12 @@
  *                      and an FP64 brute force on tiny inputs.
  *   or_hnsw_search   — compiled reference HnswCursor on the same graph file.
  *   or_graph_entry   — compiled reference fv_index_graph_entry.
+ *   or_hnsw_batch_insert — parity unpinned as a whole (the reference has no
+ *                      batched builder): its search is or_hnsw_search, its
+ *                      heuristic is checked against brute force on tiny
+ *                      inputs and the host builder's reference-identical
+ *                      graph reduces to it with batches of one node
+ *                      (tests/test_oracle_build.py).
  */
 #include <math.h>
 #include <stdint.h>
@@ -621,6 +627,133 @@ int or_hnsw_batch(const float *vectors, uint64_t count, int dim,
                                     queries + (uint64_t)q * dim, entries[q], ef,
                                     max_visited, out + (uint64_t)q * ef,
                                     &exhausted[q], NULL);
+    return 0;
+}
+
+/* ---------------------------------------------------------------------
+ * Level-0 phase of the batched (device) HNSW builder. The pieces are the
+ * reference builder's: the efConstruction beam (HnswBuild's search,
+ * engine.cpp:760-805, whose pop/admit/stop rules are the query cursor's
+ * restated in or_hnsw_search, here unfiltered), the neighbour heuristic
+ * (engine.cpp:807-823: keep a candidate unless a kept node is closer to it
+ * than the base node; stop at m; no pruning at all when there are no more
+ * than m candidates) and the re-pruning of full lists on reverse links
+ * (engine.cpp:825-845). The schedule is the batch one: every node of a
+ * batch searches the level-0 graph as it stood before the batch; each
+ * keeps M links; then each target takes the batch's incoming links in node
+ * order — appended while its count plus the links left fit in M0, else its
+ * list and the next min(left, 64 - count) links are sorted by (distance to
+ * the target, node) and re-pruned to M0, until none are left.
+ * l0 [count][2M+1] ([cnt, ids...]) holds the host phase's lists on entry
+ * and the result on return.
+ * ------------------------------------------------------------------- */
+static int pair_cmp(const void *a, const void *b) {
+    const Pair x = *(const Pair *)a, y = *(const Pair *)b;
+    return p_less(x, y) ? -1 : (p_less(y, x) ? 1 : 0);
+}
+
+static int u64_cmp(const void *a, const void *b) {
+    const uint64_t x = *(const uint64_t *)a, y = *(const uint64_t *)b;
+    return x < y ? -1 : (x > y ? 1 : 0);
+}
+
+/* the heuristic over candidates c[0..n) sorted ascending; kept ids -> keep */
+static uint32_t or_heuristic(const float *X, int dim, const Pair *c, uint32_t n,
+                             uint32_t m, uint32_t *keep) {
+    uint32_t nk = 0;
+    for (uint32_t i = 0; i < n && nk < m; i++) {
+        int ok = 1;
+        for (uint32_t k = 0; k < nk && ok; k++)
+            if (or_l2sq(X + (uint64_t)c[i].n * dim, X + (uint64_t)keep[k] * dim, dim) < c[i].d)
+                ok = 0;
+        if (ok) keep[nk++] = c[i].n;
+    }
+    return nk;
+}
+
+int or_hnsw_batch_insert(const float *X, uint64_t count, int dim, uint32_t M, uint32_t efc,
+                         uint32_t *l0, const uint32_t *rest, const uint32_t *entries,
+                         const int32_t *sizes, int nbatches) {
+    const uint32_t M0 = 2 * M, S = M0 + 1;
+    int32_t bmax = 1;
+    for (int b = 0; b < nbatches; b++) bmax = sizes[b] > bmax ? sizes[b] : bmax;
+    OrHit *hits = (OrHit *)malloc(sizeof(OrHit) * (size_t)bmax * efc);
+    uint32_t *cnt = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)bmax);
+    uint64_t *keys = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)bmax * M);
+    Pair *c = (Pair *)malloc(sizeof(Pair) * (efc > 64 ? efc : 64));
+    uint32_t keep[64];
+    if (!hits || !cnt || !keys || !c) {
+        free(hits);
+        free(cnt);
+        free(keys);
+        free(c);
+        return -1;
+    }
+    uint64_t off = 0;
+    for (int b = 0; b < nbatches; b++) {
+        const int nb = sizes[b];
+#pragma omp parallel for schedule(dynamic, 4)
+        for (int i = 0; i < nb; i++) {
+            uint8_t exh;
+            const uint32_t u = rest[off + i];
+            cnt[i] = or_hnsw_search(X, count, dim, l0, S, NULL, NULL, NULL, NULL, -1,
+                                    X + (uint64_t)u * dim, entries[off + i], efc, count + 1,
+                                    hits + (uint64_t)i * efc, &exh, NULL);
+        }
+        uint64_t nk = 0;
+        for (int i = 0; i < nb; i++) {
+            const uint32_t u = rest[off + i];
+            const OrHit *h = hits + (uint64_t)i * efc;
+            uint32_t n = 0;
+            if (cnt[i] <= M) {
+                for (uint32_t j = 0; j < cnt[i]; j++) keep[n++] = (uint32_t)h[j].node;
+            } else {
+                for (uint32_t j = 0; j < cnt[i]; j++) c[j] = (Pair){h[j].dist, (uint32_t)h[j].node};
+                n = or_heuristic(X, dim, c, cnt[i], M, keep);
+            }
+            uint32_t *L = l0 + (uint64_t)u * S;
+            L[0] = n;
+            for (uint32_t j = 0; j < n; j++) {
+                L[1 + j] = keep[j];
+                keys[nk++] = ((uint64_t)keep[j] << 32) | u;
+            }
+        }
+        qsort(keys, nk, sizeof(uint64_t), u64_cmp);
+        for (uint64_t s = 0; s < nk;) {
+            const uint32_t v = (uint32_t)(keys[s] >> 32);
+            uint64_t e = s;
+            while (e < nk && (uint32_t)(keys[e] >> 32) == v) e++;
+            uint32_t *L = l0 + (uint64_t)v * S;
+            const float *xv = X + (uint64_t)v * dim;
+            uint32_t cc = L[0];
+            uint64_t pos = s;
+            while (pos < e) {
+                if (cc + (e - pos) <= M0) {
+                    while (pos < e) L[1 + cc++] = (uint32_t)keys[pos++];
+                    break;
+                }
+                const uint32_t take = (uint32_t)((e - pos) < 64 - cc ? (e - pos) : 64 - cc);
+                uint32_t n = 0;
+                for (uint32_t j = 0; j < cc; j++)
+                    c[n++] = (Pair){or_l2sq(xv, X + (uint64_t)L[1 + j] * dim, dim), L[1 + j]};
+                for (uint32_t j = 0; j < take; j++) {
+                    const uint32_t x = (uint32_t)keys[pos + j];
+                    c[n++] = (Pair){or_l2sq(xv, X + (uint64_t)x * dim, dim), x};
+                }
+                pos += take;
+                qsort(c, n, sizeof(Pair), pair_cmp);
+                cc = or_heuristic(X, dim, c, n, M0, keep);
+                for (uint32_t j = 0; j < cc; j++) L[1 + j] = keep[j];
+            }
+            L[0] = cc;
+            s = e;
+        }
+        off += (uint64_t)nb;
+    }
+    free(hits);
+    free(cnt);
+    free(keys);
+    free(c);
     return 0;
 }
 

@@ -75,6 +75,9 @@ uint32_t or_graph_entry(const float *vectors, int dim,
                         const uint64_t *upper_offs, const uint32_t *upper,
                         uint32_t M, uint64_t entry_node, int max_level,
                         const float *query);
+int or_hnsw_batch_insert(const float *X, uint64_t count, int dim, uint32_t M, uint32_t efc,
+                         uint32_t *l0, const uint32_t *rest, const uint32_t *entries,
+                         const int32_t *sizes, int nbatches);
 double or_recall(const uint64_t *got, uint32_t ngot, const uint64_t *truth,
                  uint32_t ntruth);
 

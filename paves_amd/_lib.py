@@ -47,7 +47,7 @@ EXT_SYMBOLS = ["fv_gpu_set_timing", "fv_gpu_last_stats", "fv_gpu_feedback",
                "fv_gpu_graph_batch_ex", "fv_gpu_batch_stage", "fv_gpu_batch_run_brute",
                "fv_gpu_batch_run_graph", "fv_gpu_sync", "fv_gpu_batch_fetch",
                "fv_gpu_batch_free", "fv_gpu_copy_bench", "fv_gpu_read_bench", "fv_hnsw_build", "fv_hnsw_info",
-               "fv_hnsw_export", "fv_hnsw_free", "fv_hnsw_entries", "fv_l2sq",
+               "fv_hnsw_export", "fv_hnsw_free", "fv_hnsw_entries", "fv_l2sq", "fv_hnsw_build_gpu", "fv_hnsw_build_seeded",
                "fv_gpu_model_nparams", "fv_gpu_model_create", "fv_gpu_model_free",
                "fv_gpu_model_set_params", "fv_gpu_model_get_params", "fv_gpu_model_grad",
                "fv_gpu_model_train", "fv_gpu_model_predict", "fv_gpu_batch_features",
@@ -88,6 +88,14 @@ class Feedback(C.Structure):
     graph) and selectivity bucket, batch time and size, batches recorded."""
     _fields_ = [("q_us", (C.c_double * 5) * 2), ("batch_us", C.c_double * 2),
                 ("batch_n", C.c_double * 2), ("batches", C.c_uint64 * 2)]
+
+
+class HnswGpuOpts(C.Structure):
+    """FvHnswGpuOpts: device graph build options (in) and timings (out)."""
+    _fields_ = [("heads", C.c_int), ("exact", C.c_int), ("batch_div", C.c_int),
+                ("max_batch", C.c_int), ("keep", C.c_int), ("pad0", C.c_int),
+                ("t_host_s", C.c_double), ("t_device_s", C.c_double), ("batches", C.c_int),
+                ("pad", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -185,6 +193,12 @@ def load() -> C.CDLL:
                                  C.POINTER(C.c_uint64)]
     lib.fv_hnsw_export.argtypes = [_P, _P, _P, _P, _P]
     lib.fv_hnsw_free.argtypes = [_P]
+    lib.fv_hnsw_build_seeded.restype = _P
+    lib.fv_hnsw_build_seeded.argtypes = [_P, C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_int,
+                                         C.c_int, C.c_char_p, C.c_size_t]
+    lib.fv_hnsw_build_gpu.restype = _P
+    lib.fv_hnsw_build_gpu.argtypes = [_P, _P, C.c_int, C.c_int, C.c_int, C.POINTER(HnswGpuOpts),
+                                      C.c_char_p, C.c_size_t]
     lib.fv_hnsw_entries.argtypes = [_P, C.c_int, _P, _P, C.c_uint32, C.c_uint64, C.c_uint32,
                                     C.c_int, _P, _P, C.c_int]
     lib.fv_l2sq.restype = C.c_float

@@ -40,6 +40,7 @@
 #include <vector>
 
 #include "../../../include/paves_gpu.h"
+#include "../pv_hnsw.h"
 
 namespace {
 
@@ -344,66 +345,175 @@ struct FvHnswGraph {
     Graph g;
 };
 
-extern "C" FvHnswGraph *fv_hnsw_build(const float *vectors, uint64_t count, int dim, int M,
-                                      int ef_construction, int threads, char *errbuf,
-                                      size_t errlen) {
+namespace {
+
+/* levels, slab offsets and empty lists for count nodes */
+void init_graph(Graph &g, const float *vectors, uint64_t count, int dim, int M,
+                int ef_construction) {
+    g.X = vectors;
+    g.dim = dim;
+    g.n = count;
+    g.M = (uint32_t)M;
+    g.M0 = 2 * (uint32_t)M;
+    g.efc = (uint32_t)ef_construction;
+    const double mL = 1.0 / log((double)M);
+    g.level.resize(count);
+    g.uoff.resize(count + 1);
+    uint64_t slots = 0;
+    for (uint64_t i = 0; i < count; i++) {
+        g.level[i] = (uint8_t)node_level(i, mL);
+        g.uoff[i] = slots;
+        slots += (uint64_t)g.level[i] * (g.M + 1);
+    }
+    g.uoff[count] = slots;
+    g.up.assign(slots, 0);
+    g.l0.assign(count * (uint64_t)(g.M0 + 1), 0);
+    g.lk.reset(new SpinLock[count]);
+}
+
+/* nodes[0..n) inserted by T threads, each taking the next index */
+void insert_all(Graph &g, const uint32_t *nodes, uint64_t n, int T) {
+    std::atomic<uint64_t> next{0};
+    auto work = [&]() {
+        std::vector<uint32_t> tag(g.n, 0);
+        uint32_t stamp = 0;
+        for (;;) {
+            const uint64_t i = next.fetch_add(1, std::memory_order_relaxed);
+            if (i >= n) break;
+            g.insert(nodes[i], tag, stamp);
+        }
+    };
+    if (T <= 1 || n < 2) {
+        work();
+        return;
+    }
+    std::vector<std::thread> pool;
+    for (int t = 0; t < T; t++) pool.emplace_back(work);
+    for (auto &th : pool) th.join();
+}
+
+bool bad_build_args(const float *vectors, uint64_t count, int dim, int M, int ef_construction,
+                    char *errbuf, size_t errlen) {
     if (vectors == nullptr || count == 0 || count >= 0xFFFFFFFFull || dim <= 0 || M < 2 ||
         M > 32 || ef_construction < 1) {
         snprintf(errbuf, errlen, "hnsw: bad build arguments (count=%llu dim=%d M=%d efc=%d)",
                  (unsigned long long)count, dim, M, ef_construction);
-        return nullptr;
+        return true;
     }
+    return false;
+}
+
+} // namespace
+
+extern "C" FvHnswGraph *fv_hnsw_build(const float *vectors, uint64_t count, int dim, int M,
+                                      int ef_construction, int threads, char *errbuf,
+                                      size_t errlen) {
+    if (bad_build_args(vectors, count, dim, M, ef_construction, errbuf, errlen)) return nullptr;
     try {
         std::unique_ptr<FvHnswGraph> h(new FvHnswGraph());
         Graph &g = h->g;
-        g.X = vectors;
-        g.dim = dim;
-        g.n = count;
-        g.M = (uint32_t)M;
-        g.M0 = 2 * (uint32_t)M;
-        g.efc = (uint32_t)ef_construction;
-        const double mL = 1.0 / log((double)M);
-        g.level.resize(count);
-        g.uoff.resize(count + 1);
-        uint64_t slots = 0;
-        for (uint64_t i = 0; i < count; i++) {
-            g.level[i] = (uint8_t)node_level(i, mL);
-            g.uoff[i] = slots;
-            slots += (uint64_t)g.level[i] * (g.M + 1);
-        }
-        g.uoff[count] = slots;
-        g.up.assign(slots, 0);
-        g.l0.assign(count * (uint64_t)(g.M0 + 1), 0);
-        g.lk.reset(new SpinLock[count]);
+        init_graph(g, vectors, count, dim, M, ef_construction);
         {
             std::vector<uint32_t> tag(count, 0);
             uint32_t stamp = 0;
             g.insert(0, tag, stamp);
         }
-        const int T = std::max(1, threads);
-        std::atomic<uint64_t> next{1};
-        auto work = [&]() {
-            std::vector<uint32_t> tag(count, 0);
-            uint32_t stamp = 0;
-            for (;;) {
-                const uint64_t i = next.fetch_add(1, std::memory_order_relaxed);
-                if (i >= count) break;
-                g.insert(i, tag, stamp);
-            }
-        };
-        if (T == 1) {
-            work();
-        } else {
-            std::vector<std::thread> pool;
-            for (int t = 0; t < T; t++) pool.emplace_back(work);
-            for (auto &th : pool) th.join();
-        }
+        std::vector<uint32_t> rest(count - 1);
+        for (uint64_t i = 1; i < count; i++) rest[i - 1] = (uint32_t)i;
+        insert_all(g, rest.data(), rest.size(), std::max(1, threads));
         g.X = nullptr; /* the caller's array is not kept */
         return h.release();
     } catch (...) {
         snprintf(errbuf, errlen, "hnsw: out of memory");
         return nullptr;
     }
+}
+
+namespace pv {
+
+/* Host phase of the device builder: levels of every node, then node 0 and
+ * every node with a level above 0 inserted with the reference semantics in
+ * index order (their upper levels are then complete: the device phase only
+ * inserts level-0 nodes), then the level-0 entry point of every remaining
+ * node by greedy descent from the top. rest lists the remaining nodes in
+ * index order. */
+FvHnswGraph *hnsw_seed(const float *vectors, uint64_t count, int dim, int M,
+                       int ef_construction, int threads, std::vector<uint32_t> &rest,
+                       std::vector<uint32_t> &entries, char *errbuf, size_t errlen) {
+    if (bad_build_args(vectors, count, dim, M, ef_construction, errbuf, errlen)) return nullptr;
+    try {
+        std::unique_ptr<FvHnswGraph> h(new FvHnswGraph());
+        Graph &g = h->g;
+        init_graph(g, vectors, count, dim, M, ef_construction);
+        {
+            std::vector<uint32_t> tag(count, 0);
+            uint32_t stamp = 0;
+            g.insert(0, tag, stamp);
+        }
+        std::vector<uint32_t> up;
+        rest.clear();
+        for (uint64_t i = 1; i < count; i++) (g.level[i] > 0 ? up : rest).push_back((uint32_t)i);
+        const int T = std::max(1, threads);
+        insert_all(g, up.data(), up.size(), T);
+        entries.assign(rest.size(), 0);
+        const int top = std::max(0, g.top.load());
+        const uint64_t ep0 = g.entry.load();
+        std::atomic<uint64_t> next{0};
+        auto work = [&]() {
+            for (;;) {
+                const uint64_t i0 = next.fetch_add(1024, std::memory_order_relaxed);
+                if (i0 >= rest.size()) break;
+                const uint64_t i1 = std::min<uint64_t>(rest.size(), i0 + 1024);
+                for (uint64_t i = i0; i < i1; i++) {
+                    uint64_t ep = ep0;
+                    for (int l = top; l >= 1; l--) ep = g.descend(g.row(rest[i]), ep, l);
+                    entries[i] = (uint32_t)ep;
+                }
+            }
+        };
+        if (T <= 1 || rest.size() < 4096) {
+            work();
+        } else {
+            std::vector<std::thread> pool;
+            for (int t = 0; t < T; t++) pool.emplace_back(work);
+            for (auto &th : pool) th.join();
+        }
+        return h.release();
+    } catch (...) {
+        snprintf(errbuf, errlen, "hnsw: out of memory");
+        return nullptr;
+    }
+}
+
+/* the graph's level-0 lists [count][M0 + 1] ([cnt, ids...]) */
+uint32_t *hnsw_l0(FvHnswGraph *h, uint32_t *stride) {
+    *stride = h->g.M0 + 1;
+    return h->g.l0.data();
+}
+
+/* forget the caller's vector array (the graph no longer reads it) */
+void hnsw_detach(FvHnswGraph *h) { h->g.X = nullptr; }
+
+} // namespace pv
+
+extern "C" FvHnswGraph *fv_hnsw_build_seeded(const float *vectors, uint64_t count, int dim,
+                                             int M, int ef_construction, int threads,
+                                             int rest_on_host, char *errbuf, size_t errlen) {
+    std::vector<uint32_t> rest, entries;
+    FvHnswGraph *h = pv::hnsw_seed(vectors, count, dim, M, ef_construction, threads, rest,
+                                   entries, errbuf, errlen);
+    if (h == nullptr) return nullptr;
+    if (rest_on_host) {
+        try {
+            insert_all(h->g, rest.data(), rest.size(), std::max(1, threads));
+        } catch (...) {
+            delete h;
+            snprintf(errbuf, errlen, "hnsw: out of memory");
+            return nullptr;
+        }
+    }
+    h->g.X = nullptr;
+    return h;
 }
 
 extern "C" int fv_hnsw_info(const FvHnswGraph *h, uint32_t *M, uint32_t *l0_stride,

@@ -824,6 +824,66 @@ int run_graph_ex(FvGpuIndex *g, FvGpuBatch *b, uint32_t ef, uint64_t max_visited
     return 0;
 }
 
+int build_search(FvGpuIndex *g, BuildSearchBufs &B, const uint32_t *nbr, int nbr_stride,
+                 const float *qf, const FvGpuQueryDesc *qd, const int32_t *pofq, int nq,
+                 int nq_reserve, uint32_t ef, int wmax, bool exact, hipStream_t s,
+                 char *errbuf, size_t errlen) {
+    if (nq < 1) return 0;
+    if (ef < 1 || ef > FV_GPU_MAX_EF) PV_FAIL("hnsw: bad efConstruction %u", ef);
+    if (nbr_stride > kMaxNew) PV_FAIL("hnsw: graph degree %d too large", nbr_stride);
+    while (wmax > 1 && wmax * nbr_stride > kMaxNew) wmax--;
+    if (wmax < 1) wmax = 1;
+    const int fcap = 1024;
+    const size_t shm = graph_shmem(g->dim, fcap, ef);
+    if (shm > 160 * 1024) PV_FAIL("hnsw: search LDS %zu bytes exceeds the CU", shm);
+    const uint64_t nr = (uint64_t)std::max(nq, nq_reserve);
+    const uint64_t words = g->count_pad / 32;
+    const uint64_t vwords = (words + 3) & ~3ull;
+    /* the search never stops on max_visited (visited <= count) */
+    const uint64_t max_visited = g->count + 1;
+    uint64_t scap = exact ? g->count + kMaxNew : 0;
+    const uint64_t kStoreBytes = (uint64_t)4 << 30;
+    if (exact && nr * (scap + fcap) * 8 > kStoreBytes)
+        scap = std::max<uint64_t>(kStoreBytes / (nr * 8), (uint64_t)fcap + 64) - fcap;
+    PV_TRY(B.visit.ensure(nr * vwords * 4));
+    PV_TRY(B.spill.ensure(exact ? nr * (scap + fcap) * 8 : 64));
+    PV_TRY(B.status.ensure(nr * 4));
+    PV_TRY(B.out.ensure(nr * ef * sizeof(FvHit)));
+    PV_TRY(B.cnt.ensure(nr * 4));
+    PV_TRY(B.exh.ensure(nr));
+    GraphArgs A;
+    memset(&A, 0, sizeof(A));
+    A.vecs = g->d_vecs;
+    A.nbr = nbr;
+    A.nbr_stride = nbr_stride;
+    A.count = g->count;
+    A.dim = g->dim;
+    A.qf = qf;
+    A.qd = qd;
+    A.pofq = pofq;
+    A.bitmap = nullptr;
+    A.words = words;
+    A.vwords = vwords;
+    A.ef = ef;
+    A.max_visited = max_visited;
+    A.wmax = wmax;
+    A.fcap = fcap;
+    A.visit = B.visit.as<uint32_t>();
+    A.out = B.out.as<FvHit>();
+    A.out_cnt = B.cnt.as<uint32_t>();
+    A.out_exh = B.exh.as<uint8_t>();
+    A.status = B.status.as<uint32_t>();
+    A.out_k = ef;
+    A.spill = B.spill.as<uint64_t>();
+    A.scap = (uint32_t)scap;
+    if (shm > 64 * 1024)
+        PV_TRY(hipFuncSetAttribute((const void *)k_graph,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+    hipLaunchKernelGGL(k_graph, dim3(nq), dim3(kGT), shm, s, A);
+    PV_TRY(hipGetLastError());
+    return 0;
+}
+
 int run_graph_opts(FvGpuIndex *g, FvGpuBatch *b, uint32_t ef, uint64_t max_visited,
                    int wmax, int fcap, uint32_t *dbg, char *errbuf, size_t errlen) {
     return run_graph_ex(g, b, ef, max_visited, wmax, fcap, dbg, GraphExtra(), errbuf, errlen);

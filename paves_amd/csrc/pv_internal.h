@@ -424,6 +424,26 @@ struct GraphExtra {
 };
 int run_graph_ex(FvGpuIndex *g, FvGpuBatch *b, uint32_t ef, uint64_t max_visited, int wmax,
                  int fcap, uint32_t *dbg, const GraphExtra &x, char *errbuf, size_t errlen);
+/* pv_graph.hip: unfiltered searches of the device graph builder over the
+ * adjacency being built (nbr, nbr_stride ids per node, kNoNode-padded):
+ * query i is qf[i] with entry point qd[i].entry; the top ef hits of query
+ * i land in out[i * ef ...] (ascending (dist, node)), their number in cnt.
+ * Buffers grow to nq_reserve queries on the first call. */
+struct BuildSearchBufs {
+    DevBuf visit, spill, status, out, cnt, exh;
+    void release() {
+        visit.release();
+        spill.release();
+        status.release();
+        out.release();
+        cnt.release();
+        exh.release();
+    }
+};
+int build_search(FvGpuIndex *g, BuildSearchBufs &B, const uint32_t *nbr, int nbr_stride,
+                 const float *qf, const FvGpuQueryDesc *qd, const int32_t *pofq, int nq,
+                 int nq_reserve, uint32_t ef, int wmax, bool exact, hipStream_t s,
+                 char *errbuf, size_t errlen);
 /* pv_brute.hip: service-time feedback of a synchronous call. begin before
  * its device work on stream s, end after it (also counts the programs'
  * passing rows when the run built bitmaps), record once the call's results

@@ -28,6 +28,49 @@ def build(vectors: np.ndarray, M: int = 16, ef_construction: int = 200, threads:
     h = lib.fv_hnsw_build(L.ptr(x), n, dim, M, ef_construction, _threads(threads), eb, len(eb))
     if not h:
         raise RuntimeError(eb.value.decode())
+    return _export(lib, h, n)
+
+
+def build_gpu(index, vectors: np.ndarray, M: int = 16, ef_construction: int = 200,
+              threads: int = 0, heads: int = 8, exact: bool = False, batch_div: int = 8,
+              max_batch: int = 32768, keep: bool = True):
+    """Level 0 built on the GPU of `index` (an engine.GpuIndex over the same
+    vectors); upper levels and entry points on the host (fv_hnsw_build_gpu).
+    Returns build()'s dict plus "stats" (host / device seconds, batches).
+    With keep the built graph also becomes the index's graph."""
+    lib = L.load()
+    x = np.ascontiguousarray(vectors, dtype=np.float32)
+    n, dim = x.shape
+    o = L.HnswGpuOpts()
+    o.heads, o.exact, o.batch_div, o.max_batch = heads, int(exact), batch_div, max_batch
+    o.keep = int(keep)
+    eb = L.errbuf()
+    h = lib.fv_hnsw_build_gpu(index.h, L.ptr(x), M, ef_construction, _threads(threads),
+                              C.byref(o), eb, len(eb))
+    if not h:
+        raise RuntimeError(eb.value.decode())
+    out = _export(lib, h, n)
+    out["stats"] = {"host_s": o.t_host_s, "device_s": o.t_device_s, "batches": o.batches}
+    return out
+
+
+def build_seeded(vectors: np.ndarray, M: int = 16, ef_construction: int = 200,
+                 threads: int = 0, rest_on_host: bool = False):
+    """build_gpu's host phase alone (fv_hnsw_build_seeded): node 0 and the
+    nodes above level 0 inserted, the rest left out — or, with rest_on_host,
+    inserted on the host afterwards in index order."""
+    lib = L.load()
+    x = np.ascontiguousarray(vectors, dtype=np.float32)
+    n, dim = x.shape
+    eb = L.errbuf()
+    h = lib.fv_hnsw_build_seeded(L.ptr(x), n, dim, M, ef_construction, _threads(threads),
+                                 int(rest_on_host), eb, len(eb))
+    if not h:
+        raise RuntimeError(eb.value.decode())
+    return _export(lib, h, n)
+
+
+def _export(lib, h, n):
     h = C.c_void_p(h)
     try:
         m, stride, top = C.c_uint32(), C.c_uint32(), C.c_uint32()
