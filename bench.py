@@ -60,6 +60,13 @@ scan of this workload is MFMA-bound, so its roofline is TFLOP/s.
 (M = 16, efC = 200), ef in {40, 80, 160, 400}, batches of 64 and 1024, the
 exact frontier (spill store) and the bounded one; the best row is gated on
 recall@10 >= 0.9 in every filter class.
+
+`--workload build`: HNSW construction of that graph (1M x 128, M = 16,
+efC = 200) with level 0 on the GPU (fv_hnsw_build_gpu), seconds per build
+(lower is better; one step = one build), against the library's host builder
+on the same threads and the compiled reference builder (cpu_baseline);
+recall@10 of both graphs at ef 40 and 80. Run it with few steps
+(`--steps 2 --warmup 1`).
 """
 from __future__ import annotations
 
@@ -337,10 +344,12 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=None)
-    ap.add_argument("--workload", choices=["filtered", "uneven", "graph"], default="filtered",
+    ap.add_argument("--workload", choices=["filtered", "uneven", "graph", "build"],
+                    default="filtered",
                     help="filtered: the 1M x 128 workload (default); uneven: one 4096-query "
                          "batch with mixed k and selectivity; graph: filtered HNSW beam "
-                         "search, ef sweep with recall@10")
+                         "search, ef sweep with recall@10; build: HNSW construction of the "
+                         "graph workload's graph (level 0 on the GPU)")
     ap.add_argument("--dim", type=int, choices=[128, 768], default=128,
                     help="uneven workload: 1M x 128 or 2M x 768")
     ap.add_argument("--nq", type=int, default=10_000)
@@ -383,6 +392,9 @@ def main():
         return
     if args.workload == "graph":
         graph_path(args)
+        return
+    if args.workload == "build":
+        build_path(args)
         return
     if args.roofline_only:
         roofline_only(args)
@@ -759,6 +771,154 @@ def graph_path(args):
         "sweep": sweep,
         "roofline": {"bound": "hbm", "achieved": roof["achieved_GBs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": roof["frac"], "traffic": None, "detail": roof},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(res), flush=True)
+
+
+def graph_recall(g, d, post, graph, truth, ef, args, nq):
+    """recall@10 per class of 1024-query batches at ef (exact frontier)."""
+    classes = synth.FILTER_CLASSES
+    ent = hnsw.entries(graph, d["base"], d["queries"][:nq], threads=args.threads)
+    batches = build_batches(d, post, args.n, nq, 1024, args.k, ent)
+    per = [[] for _ in classes]
+    mv = max(200_000, ef * 400)
+    for s, e, b in batches:
+        out, cnt, _ = g.graph(d["queries"][s:e], b, ef, mv)
+        for j in range(e - s):
+            got = out[j, : min(int(cnt[j]), args.k)]["node"]
+            per[(s + j) % 6].append(O_recall(got, truth[s + j]))
+    return {classes[i]: round(float(np.mean(per[i])), 4) for i in range(6)}, \
+        round(float(np.mean([x for p in per for x in p])), 4)
+
+
+def reference_build_baseline(x, args, M=16, efc=200):
+    """The reference builder (engine.cpp fv_builder_finish, oracle/_ref) on
+    the host threads: HNSW construction plus the index-file write, vectors
+    only (no columns). Rows are added before the clock starts."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    import ctypes as C
+    try:
+        ref = O.RefEngine()
+    except RuntimeError as e:
+        log(f"cpu baseline skipped: {e}")
+        return None
+    threads = min(16, os.cpu_count() or 1)
+    n, dim = x.shape
+    tmpd = tempfile.mkdtemp(prefix="paves_bench_")
+    path = os.path.join(tmpd, "ref.idx")
+    try:
+        att = np.zeros(1, np.int32)
+        b = ref.L.fv_builder_create(dim, 0, att.ctypes.data, n)
+        sc = np.zeros(1, np.float64)
+        nul = np.zeros(1, np.uint8)
+        xv = np.ascontiguousarray(x, np.float32)
+        for i in range(n):
+            ref.L.fv_builder_add_row(b, xv[i].ctypes.data, i, sc.ctypes.data, nul.ctypes.data)
+        eb = C.create_string_buffer(256)
+        t0 = time.perf_counter()
+        rc = ref.L.fv_builder_finish(b, path.encode(), 1, M, efc, threads, eb, 256)
+        el = time.perf_counter() - t0
+        ref.L.fv_builder_destroy(b)
+        if rc != 0:
+            log(f"cpu baseline failed: {eb.value.decode()}")
+            return None
+        log(f"reference builder: {el:.1f}s on {threads} threads")
+        return {"value": round(el, 2), "unit": "s", "cores": threads, "kind": "reference",
+                **host_info(),
+                "sample": f"the whole workload: reference fv_builder_finish (engine.cpp HNSW "
+                          f"build, M={M}, efC={efc}, {threads} threads) over the same {n} x {dim} "
+                          "rows, including its index-file write"}
+    finally:
+        try:
+            os.unlink(path)
+        except OSError:
+            pass
+        try:
+            os.rmdir(tmpd)
+        except OSError:
+            pass
+
+
+def build_path(args):
+    """HNSW construction of the graph workload's graph (1M x 128, M = 16,
+    efC = 200): host phase (upper levels, entry points) plus level 0 on the
+    GPU, timed end to end from host vectors to the graph on the device and
+    in host memory. Compared with the library's host builder (the
+    reference builder's semantics) on the same threads and with the compiled
+    reference builder (cpu_baseline); both graphs are searched for recall."""
+    nq = min(args.nq, 4096)
+    M, efc = 16, 200
+    d = synth.gmm128(args.n, nq)
+    cols = synth.columns(d)
+    rows, vals = synth.postings(cols)
+    post = E.Postings(rows, vals)
+    x = d["base"]
+    g = E.GpuIndex(x, cols, rows, precision=L.FV_GPU_PREC_FP32, device=0)
+    for _ in range(args.warmup):
+        hnsw.build_gpu(g, x, M, efc, threads=args.threads, keep=False)
+    steps = max(1, args.steps)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        dev = hnsw.build_gpu(g, x, M, efc, threads=args.threads, keep=True)
+    el = (time.perf_counter() - t0) / steps
+    st = dev["stats"]
+    log(f"device build: {el:.2f}s per build ({st})")
+    t1 = time.perf_counter()
+    host = hnsw.build(x, M, efc, threads=args.threads)
+    host_s = time.perf_counter() - t1
+    log(f"host builder: {host_s:.2f}s on {args.threads} threads")
+    # exact truth from the library's brute force (bit-identical to the oracle)
+    classes = synth.FILTER_CLASSES
+    truth = []
+    for s in range(0, nq, 1024):
+        e = min(nq, s + 1024)
+        preds = [synth.filter_pred(classes[i % 6], d["qcent"][i]) for i in range(s, e)]
+        b = E.concat([E.serialize(p, args.n, post) for p in preds], args.n)
+        out, cnt, _ = g.brute(d["queries"][s:e], b, args.k)
+        truth += [out[j, : cnt[j]]["node"].copy() for j in range(e - s)]
+    recall = {}
+    for ef in (40, 80):
+        pd, md = graph_recall(g, d, post, dev, truth, ef, args, nq)
+        recall[f"device_ef{ef}"] = {"mean": md, "per_class": pd}
+    hg = E.GpuIndex(x, cols, rows, l0=host["l0"], precision=L.FV_GPU_PREC_FP32, device=0)
+    try:
+        for ef in (40, 80):
+            ph, mh = graph_recall(hg, d, post, host, truth, ef, args, nq)
+            recall[f"host_ef{ef}"] = {"mean": mh, "per_class": ph}
+    finally:
+        hg.close()
+    log("recall@10: " + ", ".join(f"{k} {v['mean']}" for k, v in recall.items()))
+    # searches of the build: one FP32 row per visited node, one neighbour
+    # line per expanded head, over their event time
+    alg = float(st["visited"]) * 128 * 4 + float(st["heads_expanded"]) * GRAPH_NBR * 4
+    gbs = alg / max(st["search_s"], 1e-9) / 1e9
+    roof = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+            "detail": {"kernel": GRAPH_KERNEL + " (efConstruction searches of the build)",
+                       "nodes_visited": st["visited"], "heads_expanded": st["heads_expanded"],
+                       "alg_bytes": int(alg), "search_s": round(st["search_s"], 4),
+                       "prune_s": round(st["prune_s"], 4), "link_s": round(st["link_s"], 4),
+                       "ns_per_visited_node": round(st["search_s"] * 1e9 /
+                                                    max(1, st["visited"]), 4)}}
+    g.close()
+    cpu = None if args.no_cpu else reference_build_baseline(x, args, M, efc)
+    res = {
+        "metric": "HNSW construction time, 1M x 128, M=16, efC=200 (host vectors to graph), 1 GPU",
+        "value": round(el, 3), "unit": "s", "n_gpus": 1, "steps": steps,
+        "warmup": args.warmup, "ms_per_step": round(el * 1e3, 1), "higher_is_better": False,
+        "scaling": "strong", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic (seeded GMM, see paves_amd/synth.py)",
+        "config": {"workload": f"HNSW build: {args.n} x 128 GMM, M=16 (M0=32), efC=200; host "
+                               f"phase on {args.threads} threads, level 0 on the GPU in "
+                               f"{st['batches']} batches; recall on {nq} queries, six filter "
+                               "classes, batches of 1024",
+                   "n": args.n, "dim": 128, "host_phase_s": round(st["host_s"], 3),
+                   "device_phase_s": round(st["device_s"], 3), "batches": st["batches"],
+                   "host_builder_s": round(host_s, 3), "host_threads": args.threads,
+                   "recall_at_10": recall},
+        "roofline": roof,
         "cpu_baseline": cpu,
     }
     print(json.dumps(res), flush=True)

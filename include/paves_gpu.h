@@ -469,11 +469,17 @@ typedef struct FvHnswGpuOpts {
     int exact;      /* 1: unbounded frontier (spill store), 0: 1024 entries */
     int batch_div;  /* batch = inserted / batch_div (default 8) */
     int max_batch;  /* default 32768 (also capped by visited-set memory) */
-    int keep;       /* 1: the built level 0 becomes g's graph (default 1);
-                       captured runs of g must then be staged again */
+    int keep;       /* 1: the built level 0 becomes g's graph (default 1;
+                       work queued on the device finishes first) */
     int pad0;
-    /* out: seconds of the host phase, of the device phase; batches run */
+    /* out: seconds of the host phase and of the device phase (wall); device
+     * event time of the searches, the prunes and the link merges (0 when
+     * more than 512 batches ran); nodes the searches visited and frontier
+     * heads they expanded; batches */
     double t_host_s, t_device_s;
+    double t_entries_s;  /* of t_host_s: the entry-point descents */
+    double t_search_s, t_prune_s, t_link_s;
+    uint64_t visited, heads_expanded;
     int batches;
     int pad;
 } FvHnswGpuOpts;

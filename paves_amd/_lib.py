@@ -94,7 +94,9 @@ class HnswGpuOpts(C.Structure):
     """FvHnswGpuOpts: device graph build options (in) and timings (out)."""
     _fields_ = [("heads", C.c_int), ("exact", C.c_int), ("batch_div", C.c_int),
                 ("max_batch", C.c_int), ("keep", C.c_int), ("pad0", C.c_int),
-                ("t_host_s", C.c_double), ("t_device_s", C.c_double), ("batches", C.c_int),
+                ("t_host_s", C.c_double), ("t_device_s", C.c_double), ("t_entries_s", C.c_double),
+                ("t_search_s", C.c_double), ("t_prune_s", C.c_double), ("t_link_s", C.c_double),
+                ("visited", C.c_uint64), ("heads_expanded", C.c_uint64), ("batches", C.c_int),
                 ("pad", C.c_int)]
 
 

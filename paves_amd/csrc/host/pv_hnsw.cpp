@@ -32,6 +32,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <memory>
 #include <mutex>
 #include <queue>
@@ -439,7 +440,8 @@ namespace pv {
  * index order. */
 FvHnswGraph *hnsw_seed(const float *vectors, uint64_t count, int dim, int M,
                        int ef_construction, int threads, std::vector<uint32_t> &rest,
-                       std::vector<uint32_t> &entries, char *errbuf, size_t errlen) {
+                       std::vector<uint32_t> &entries, char *errbuf, size_t errlen,
+                       double *t_entries_s) {
     if (bad_build_args(vectors, count, dim, M, ef_construction, errbuf, errlen)) return nullptr;
     try {
         std::unique_ptr<FvHnswGraph> h(new FvHnswGraph());
@@ -455,6 +457,7 @@ FvHnswGraph *hnsw_seed(const float *vectors, uint64_t count, int dim, int M,
         for (uint64_t i = 1; i < count; i++) (g.level[i] > 0 ? up : rest).push_back((uint32_t)i);
         const int T = std::max(1, threads);
         insert_all(g, up.data(), up.size(), T);
+        const auto te = std::chrono::steady_clock::now();
         entries.assign(rest.size(), 0);
         const int top = std::max(0, g.top.load());
         const uint64_t ep0 = g.entry.load();
@@ -478,6 +481,9 @@ FvHnswGraph *hnsw_seed(const float *vectors, uint64_t count, int dim, int M,
             for (int t = 0; t < T; t++) pool.emplace_back(work);
             for (auto &th : pool) th.join();
         }
+        if (t_entries_s)
+            *t_entries_s =
+                std::chrono::duration<double>(std::chrono::steady_clock::now() - te).count();
         return h.release();
     } catch (...) {
         snprintf(errbuf, errlen, "hnsw: out of memory");

@@ -220,6 +220,17 @@ __global__ __launch_bounds__(kBW * 64) void k_link(
     }
 }
 
+/* nodes visited (acc[0]) and heads expanded (acc[1]) by a batch's
+ * searches: debug words 1 and 0 of each query */
+__global__ void k_sum_visited(const uint32_t *__restrict__ dbg, int nq,
+                              unsigned long long *__restrict__ acc) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nq) {
+        atomicAdd(&acc[0], (unsigned long long)dbg[(uint64_t)i * 4 + 1]);
+        atomicAdd(&acc[1], (unsigned long long)dbg[(uint64_t)i * 4 + 0]);
+    }
+}
+
 } // namespace pv
 
 using namespace pv;
@@ -234,11 +245,17 @@ struct BuildBufs {
     int32_t *pofq = nullptr;
     uint64_t *k0 = nullptr, *k1 = nullptr;
     uint32_t *heads = nullptr, *nheads = nullptr;
+    uint32_t *dbg = nullptr;
+    unsigned long long *visited = nullptr;
     void *tmp = nullptr;
     hipStream_t s = nullptr;
     BuildSearchBufs sb;
+    std::vector<hipEvent_t> ev;
     ~BuildBufs() {
         if (s) hipStreamSynchronize(s);
+        for (hipEvent_t e : ev) hipEventDestroy(e);
+        hipFree(dbg);
+        hipFree(visited);
         hipFree(nbr);
         hipFree(ids);
         hipFree(ents);
@@ -262,7 +279,7 @@ double secs_since(std::chrono::steady_clock::time_point t0) {
 /* the device phase; fills l0 ([count][M0 + 1]) from the device lists */
 int device_phase(FvGpuIndex *g, BuildBufs &B, const std::vector<uint32_t> &rest,
                  const std::vector<uint32_t> &entries, uint32_t *l0, uint32_t l0s, int M,
-                 int efc, const FvHnswGpuOpts &o, int *nbatches, char *errbuf, size_t errlen) {
+                 int efc, FvHnswGpuOpts &o, char *errbuf, size_t errlen) {
     const uint64_t n = g->count;
     const int dim = g->dim;
     const int M0 = 2 * M;
@@ -282,7 +299,7 @@ int device_phase(FvGpuIndex *g, BuildBufs &B, const std::vector<uint32_t> &rest,
         have += b;
     }
     const int bpeak = sizes.empty() ? 1 : *std::max_element(sizes.begin(), sizes.end());
-    *nbatches = (int)sizes.size();
+    o.batches = (int)sizes.size();
 
     PV_TRY(hipStreamCreateWithFlags(&B.s, hipStreamNonBlocking));
     /* adjacency: the host phase's level-0 lists, kNoNode-padded */
@@ -313,22 +330,37 @@ int device_phase(FvGpuIndex *g, BuildBufs &B, const std::vector<uint32_t> &rest,
     size_t tmpb = 0;
     PV_TRY(hipcub::DeviceRadixSort::SortKeys(nullptr, tmpb, B.k0, B.k1, (int)pk, 0, 64, B.s));
     PV_TRY(hipMalloc(&B.tmp, std::max<size_t>(tmpb, 256)));
+    PV_TRY(hipMalloc(&B.dbg, (size_t)bpeak * 16));
+    PV_TRY(hipMalloc(&B.visited, 16));
+    PV_TRY(hipMemset(B.visited, 0, 16));
+    /* stage timing: four events per batch while there are few batches */
+    const bool timed = sizes.size() <= 512;
+    if (timed) {
+        B.ev.resize(4 * sizes.size());
+        for (hipEvent_t &e : B.ev) PV_TRY(hipEventCreate(&e));
+    }
 
     const int heads = std::min(8, std::max(1, o.heads));
     uint64_t off = 0;
-    for (const int nb : sizes) {
+    for (size_t bi = 0; bi < sizes.size(); bi++) {
+        const int nb = sizes[bi];
         const uint64_t tot = (uint64_t)nb * dim;
+        if (timed) PV_TRY(hipEventRecord(B.ev[4 * bi], B.s));
         hipLaunchKernelGGL(k_gather_batch, dim3((unsigned)((std::max<uint64_t>(tot, nb) + 255) / 256)),
                            dim3(256), 0, B.s, g->d_vecs, dim, B.ids + off, B.ents + off, nb, B.qf,
                            B.qd);
         PV_TRY(hipGetLastError());
         if (build_search(g, B.sb, B.nbr, nbs, B.qf, B.qd, B.pofq, nb, bpeak, (uint32_t)efc,
-                         heads, o.exact != 0, B.s, errbuf, errlen) != 0)
+                         heads, o.exact != 0, B.dbg, B.s, errbuf, errlen) != 0)
             return -1;
+        hipLaunchKernelGGL(k_sum_visited, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, B.s,
+                           B.dbg, nb, B.visited);
+        if (timed) PV_TRY(hipEventRecord(B.ev[4 * bi + 1], B.s));
         hipLaunchKernelGGL(k_prune, dim3((unsigned)((nb + kBW - 1) / kBW)), dim3(kBW * 64), 0, B.s,
                            g->d_vecs, dim, B.ids + off, nb, B.sb.out.as<FvHit>(),
                            B.sb.cnt.as<uint32_t>(), (uint32_t)efc, M, B.nbr, nbs, B.k0);
         PV_TRY(hipGetLastError());
+        if (timed) PV_TRY(hipEventRecord(B.ev[4 * bi + 2], B.s));
         const uint64_t nk = (uint64_t)nb * M;
         size_t tb = tmpb;
         PV_TRY(hipcub::DeviceRadixSort::SortKeys(B.tmp, tb, B.k0, B.k1, (int)nk, 0, 64, B.s));
@@ -340,9 +372,28 @@ int device_phase(FvGpuIndex *g, BuildBufs &B, const std::vector<uint32_t> &rest,
         hipLaunchKernelGGL(k_link, dim3(lg), dim3(kBW * 64), 0, B.s, g->d_vecs, dim, B.k1, nk,
                            B.heads, B.nheads, M0, B.nbr, nbs);
         PV_TRY(hipGetLastError());
+        if (timed) PV_TRY(hipEventRecord(B.ev[4 * bi + 3], B.s));
         off += nb;
     }
     PV_TRY(hipStreamSynchronize(B.s));
+    if (timed) {
+        double st[3] = {0, 0, 0};
+        for (size_t bi = 0; bi < sizes.size(); bi++)
+            for (int k = 0; k < 3; k++) {
+                float ms = 0.f;
+                PV_TRY(hipEventElapsedTime(&ms, B.ev[4 * bi + k], B.ev[4 * bi + k + 1]));
+                st[k] += ms * 1e-3;
+            }
+        o.t_search_s = st[0];
+        o.t_prune_s = st[1];
+        o.t_link_s = st[2];
+    }
+    {
+        unsigned long long vis[2] = {0, 0};
+        PV_TRY(hipMemcpy(vis, B.visited, 16, hipMemcpyDeviceToHost));
+        o.visited = vis[0];
+        o.heads_expanded = vis[1];
+    }
     /* device lists -> [cnt, ids...] */
     std::vector<uint32_t> pad((size_t)n * nbs);
     PV_TRY(hipMemcpy(pad.data(), B.nbr, pad.size() * 4, hipMemcpyDeviceToHost));
@@ -391,22 +442,26 @@ extern "C" FvHnswGraph *fv_hnsw_build_gpu(FvGpuIndex *g, const float *vectors, i
     }
     const auto t0 = std::chrono::steady_clock::now();
     std::vector<uint32_t> rest, entries;
+    double te = 0.0;
     FvHnswGraph *h = hnsw_seed(vectors, g->count, g->dim, M, ef_construction, threads, rest,
-                               entries, errbuf, errlen);
+                               entries, errbuf, errlen, &te);
     if (h == nullptr) return nullptr;
     const double th = secs_since(t0);
-    if (opts) opts->t_host_s = th;
+    if (opts) {
+        opts->t_host_s = th;
+        opts->t_entries_s = te;
+    }
     const auto t1 = std::chrono::steady_clock::now();
     uint32_t l0s = 0;
     uint32_t *l0 = hnsw_l0(h, &l0s);
-    int nbatches = 0;
     int rc;
     {
         BuildBufs B;
-        rc = device_phase(g, B, rest, entries, l0, l0s, M, ef_construction, o, &nbatches, errbuf,
-                          errlen);
+        rc = device_phase(g, B, rest, entries, l0, l0s, M, ef_construction, o, errbuf, errlen);
         if (rc == 0 && o.keep) {
-            /* the built lists become the index's graph */
+            /* the built lists become the index's graph, once no queued
+             * search can still read the old one */
+            hipDeviceSynchronize();
             hipFree(g->d_nbr);
             g->d_nbr = B.nbr;
             g->nbr_stride = (2 * M + 7) & ~7;
@@ -421,7 +476,12 @@ extern "C" FvHnswGraph *fv_hnsw_build_gpu(FvGpuIndex *g, const float *vectors, i
     }
     if (opts) {
         opts->t_device_s = secs_since(t1);
-        opts->batches = nbatches;
+        opts->t_search_s = o.t_search_s;
+        opts->t_prune_s = o.t_prune_s;
+        opts->t_link_s = o.t_link_s;
+        opts->visited = o.visited;
+        opts->heads_expanded = o.heads_expanded;
+        opts->batches = o.batches;
     }
     return h;
 }

@@ -826,8 +826,8 @@ int run_graph_ex(FvGpuIndex *g, FvGpuBatch *b, uint32_t ef, uint64_t max_visited
 
 int build_search(FvGpuIndex *g, BuildSearchBufs &B, const uint32_t *nbr, int nbr_stride,
                  const float *qf, const FvGpuQueryDesc *qd, const int32_t *pofq, int nq,
-                 int nq_reserve, uint32_t ef, int wmax, bool exact, hipStream_t s,
-                 char *errbuf, size_t errlen) {
+                 int nq_reserve, uint32_t ef, int wmax, bool exact, uint32_t *dbg,
+                 hipStream_t s, char *errbuf, size_t errlen) {
     if (nq < 1) return 0;
     if (ef < 1 || ef > FV_GPU_MAX_EF) PV_FAIL("hnsw: bad efConstruction %u", ef);
     if (nbr_stride > kMaxNew) PV_FAIL("hnsw: graph degree %d too large", nbr_stride);
@@ -876,6 +876,7 @@ int build_search(FvGpuIndex *g, BuildSearchBufs &B, const uint32_t *nbr, int nbr
     A.out_k = ef;
     A.spill = B.spill.as<uint64_t>();
     A.scap = (uint32_t)scap;
+    A.dbg = dbg;
     if (shm > 64 * 1024)
         PV_TRY(hipFuncSetAttribute((const void *)k_graph,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));

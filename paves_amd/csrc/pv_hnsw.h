@@ -16,7 +16,8 @@ namespace pv {
 
 FvHnswGraph *hnsw_seed(const float *vectors, uint64_t count, int dim, int M,
                        int ef_construction, int threads, std::vector<uint32_t> &rest,
-                       std::vector<uint32_t> &entries, char *errbuf, size_t errlen);
+                       std::vector<uint32_t> &entries, char *errbuf, size_t errlen,
+                       double *t_entries_s = nullptr);
 uint32_t *hnsw_l0(FvHnswGraph *h, uint32_t *stride);
 void hnsw_detach(FvHnswGraph *h);
 

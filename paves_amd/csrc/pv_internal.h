@@ -442,8 +442,8 @@ struct BuildSearchBufs {
 };
 int build_search(FvGpuIndex *g, BuildSearchBufs &B, const uint32_t *nbr, int nbr_stride,
                  const float *qf, const FvGpuQueryDesc *qd, const int32_t *pofq, int nq,
-                 int nq_reserve, uint32_t ef, int wmax, bool exact, hipStream_t s,
-                 char *errbuf, size_t errlen);
+                 int nq_reserve, uint32_t ef, int wmax, bool exact, uint32_t *dbg,
+                 hipStream_t s, char *errbuf, size_t errlen);
 /* pv_brute.hip: service-time feedback of a synchronous call. begin before
  * its device work on stream s, end after it (also counts the programs'
  * passing rows when the run built bitmaps), record once the call's results

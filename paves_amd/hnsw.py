@@ -50,7 +50,10 @@ def build_gpu(index, vectors: np.ndarray, M: int = 16, ef_construction: int = 20
     if not h:
         raise RuntimeError(eb.value.decode())
     out = _export(lib, h, n)
-    out["stats"] = {"host_s": o.t_host_s, "device_s": o.t_device_s, "batches": o.batches}
+    out["stats"] = {"host_s": o.t_host_s, "entries_s": o.t_entries_s,
+                    "device_s": o.t_device_s, "batches": o.batches,
+                    "search_s": o.t_search_s, "prune_s": o.t_prune_s, "link_s": o.t_link_s,
+                    "visited": o.visited, "heads_expanded": o.heads_expanded}
     return out
 
 
