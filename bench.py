@@ -25,7 +25,15 @@ kernel's launch over the workload's first 1024 queries (and unfiltered).
 BF16 and FP32 results are checked against each other bit for bit and a
 micro-batch against the CPU oracle.
 
-roofline: the dominant kernel is the BF16 full scan of one 64-query tile,
+roofline: the dominant kernel of the headline's timed region is the
+query-resident BF16 MFMA scan of a routed run of 1024 queries,
+k_scan_bf16_qr<8, false, false> (about 80% of each step): MFMA-bound, so
+`roofline` is its TFLOP/s against the 2.5 PFLOP/s dense BF16 peak (HIP
+events over 10 back-to-back launches on the batch's stream; `python
+bench.py --batched-scan-only` is the rocprofv3 target), with `traffic` from
+FETCH_SIZE + WRITE_SIZE passes over that command scaled by the copy kernel.
+`roofline_microbatch` is the HBM-bound kernel of the micro-batch line: the
+BF16 full scan of one 64-query tile,
 k_scan_bf16<false, 8, false>, when a batch has the chip to itself. Its
 launch duration is measured with HIP events on the batch's stream: for
 each scan-route micro-batch the library re-launches that batch's scan
@@ -95,6 +103,7 @@ HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFS = 157.3    # dense FP32 matrix (MFMA) peak of MI355X
 SCAN_KERNEL = "k_scan_bf16<false, 8, false>"   # main scan: 8 waves per WG, bitmap predicates
 COPY_BYTES = 1 << 28   # 256 MiB device copy used to calibrate the PMC byte counters
+QR_KERNEL = "k_scan_bf16_qr<8, false, false>"  # query-resident scan, batches of 8+ tiles
 GRAPH_KERNEL = "k_graph"   # filtered level-0 beam search, one 4-wave WG per query
 GRAPH_NBR = 32             # neighbour-list slots read per expanded head (M0 = 32)
 
@@ -210,11 +219,11 @@ def pmc_pass(counter, args, child=None, kernel=None):
     return {"scan_kib": float(np.median(scan)), "copy_kib": float(np.median(copy))}
 
 
-def measure_traffic(args):
+def measure_traffic(args, child=None, kernel=None):
     """HBM bytes per scan launch: FETCH_SIZE + WRITE_SIZE, each scaled by
     the counter's reading for the known-size copy (COPY_BYTES each way)."""
-    f = pmc_pass("FETCH_SIZE", args)
-    w = pmc_pass("WRITE_SIZE", args) if f else None
+    f = pmc_pass("FETCH_SIZE", args, child=child, kernel=kernel)
+    w = pmc_pass("WRITE_SIZE", args, child=child, kernel=kernel) if f else None
     if not f or not w:
         return None, None
     kib = COPY_BYTES / 1024.0
@@ -227,15 +236,15 @@ def measure_traffic(args):
     return fetch + write, info
 
 
-def graph_launch(g, queries, b, ef, mv, dim):
+def graph_launch(g, queries, b, ef, mv, dim, fcap=1024):
     """One beam-search launch over a batch: its kernel duration (HIP events
     around the search kernel) and its algorithmic bytes from the kernel's
     own counters — one neighbour list (GRAPH_NBR ids) per expanded head,
     one FP32 vector row per visited node and, for a filtered query, one
     predicate-bitmap word per visited node."""
-    g.graph(queries, b, ef, mv)          # warm-up (buffers, code)
+    g.graph(queries, b, ef, mv, wmax=8, fcap=fcap)   # warm-up (buffers, code)
     g.set_timing(True)
-    _, _, _, dbg = g.graph(queries, b, ef, mv, debug=True)
+    _, _, _, dbg = g.graph(queries, b, ef, mv, wmax=8, fcap=fcap, debug=True)
     us = g.stats().t_graph_us
     g.set_timing(False)
     heads = dbg[:, 0].astype(np.float64)
@@ -244,6 +253,7 @@ def graph_launch(g, queries, b, ef, mv, dim):
     alg = float((heads * GRAPH_NBR * 4).sum() + (vis * (dim * 4 + 4 * filt)).sum())
     gbs = alg / (us * 1e-6) / 1e9
     return {"kernel": GRAPH_KERNEL, "queries": len(queries), "ef": ef,
+            "frontier": "exact" if fcap > 0 else "bounded",
             "heads_expanded": int(heads.sum()), "nodes_visited": int(vis.sum()),
             "alg_bytes_per_launch": int(alg), "avg_launch_us": round(us, 2),
             "achieved_GBs": round(gbs, 1), "peak_GBs": HBM_PEAK_GBS,
@@ -332,6 +342,8 @@ def batched_scan_only(args):
     staged = [g.stage(d["queries"][s:e], b, args.k) for s, e, b in batches]
     bf = L.BruteOpts(precision=L.FV_GPU_PREC_BF16, route=1)
     m = measure_batched_scan(g, staged, batches, bf, args, reps=10)
+    if args.copy_calib:
+        copy_calibration(reps=3)
     for bh in staged:
         g.free_batch(bh)
     g.close()
@@ -408,10 +420,19 @@ def main():
     learned_on = args.train_queries > 0
 
     traffic, traffic_info = (None, None)
+    traffic_qr, traffic_qr_info = (None, None)
     if not args.no_traffic:
         t = time.perf_counter()
         traffic, traffic_info = measure_traffic(args)
         log(f"pmc traffic: {traffic_info} ({time.perf_counter() - t:.0f}s)")
+        if learned_on:
+            t = time.perf_counter()
+            traffic_qr, traffic_qr_info = measure_traffic(
+                args, child=["--batched-scan-only", "--copy-calib", "--n", str(args.n),
+                             "--learned-batch", str(args.learned_batch), "--k", str(args.k)],
+                kernel=QR_KERNEL)
+            log(f"pmc traffic (query-resident scan): {traffic_qr_info} "
+                f"({time.perf_counter() - t:.0f}s)")
 
     t = time.perf_counter()
     d = synth.gmm128(args.n, args.nq)
@@ -421,20 +442,23 @@ def main():
     log(f"data: {args.n}x128 + {args.nq} queries in {time.perf_counter() - t:.1f}s")
 
     graph, ent, setup = None, None, {}
+    g = E.GpuIndex(d["base"], cols, rows, precision=L.FV_GPU_PREC_BF16, device=0)
     if learned_on:
+        # the graph, built with level 0 on this GPU and kept there
         t = time.perf_counter()
-        graph = hnsw.build(d["base"], M=16, ef_construction=200, threads=args.threads)
+        graph = hnsw.build_gpu(g, d["base"], M=16, ef_construction=200, threads=args.threads,
+                               keep=True)
         setup["graph_build_s"] = round(time.perf_counter() - t, 2)
+        setup["graph_builder"] = "fv_hnsw_build_gpu (level 0 on the GPU)"
         t = time.perf_counter()
         ent = hnsw.entries(graph, d["base"], d["queries"], threads=args.threads)
         setup["entry_points_us_per_query"] = round((time.perf_counter() - t) / args.nq * 1e6, 3)
-        log(f"graph: M=16 efC=200 built in {setup['graph_build_s']}s ({args.threads} threads), "
-            f"entry points {setup['entry_points_us_per_query']} us/query")
+        log(f"graph: M=16 efC=200 built in {setup['graph_build_s']}s (level 0 on the GPU, "
+            f"{args.threads} host threads), entry points "
+            f"{setup['entry_points_us_per_query']} us/query")
     batches = build_batches(d, post, args.n, args.nq, args.batch, args.k, ent)
     batchesL = build_batches(d, post, args.n, args.nq, args.learned_batch, args.k, ent)
 
-    g = E.GpuIndex(d["base"], cols, rows, l0=None if graph is None else graph["l0"],
-                   precision=L.FV_GPU_PREC_BF16, device=0)
     staged = [g.stage(d["queries"][s:e], b, args.k) for s, e, b in batches]
     stagedL = [g.stage(d["queries"][s:e], b, args.k) for s, e, b in batchesL]
     log(f"staged {len(staged)} micro-batches of {args.batch} and {len(stagedL)} batches of "
@@ -536,6 +560,31 @@ def main():
     else:
         value, steps_ms, rec = qps_bf, dt / args.steps * 1e3, (1.0 if exact else None)
     plain = max(qps32, qps32L)
+    roof_main = None
+    if lrn is not None and batched_scan is not None:
+        # the dominant kernel of the timed region: the learned path's routed
+        # runs spend most of each step in the query-resident scan
+        runs = len([1 for s, e, _ in batchesL])
+        share = batched_scan["avg_launch_us"] * runs / (steps_ms * 1e3)
+        qr_alg = args.n * (128 * 2 + 4)
+        roof_main = {"bound": "mfma", "achieved": batched_scan["achieved_TFLOPs"],
+                     "peak": 2500.0, "unit": "TFLOP/s", "frac": batched_scan["frac"],
+                     "traffic": int(traffic_qr) if traffic_qr else None,
+                     "kernel": QR_KERNEL, "avg_launch_us": batched_scan["avg_launch_us"],
+                     "flops_per_launch": batched_scan["flops_per_launch"],
+                     "alg_bytes_per_launch": int(qr_alg),
+                     "traffic_detail": traffic_qr_info,
+                     "share_of_step": round(share, 3),
+                     "note": f"the query-resident BF16 MFMA scan of one routed run of "
+                             f"{args.learned_batch} queries (2 x N x 128 FLOP per query; "
+                             "HIP events over 10 back-to-back launches on the batch's stream); "
+                             "algorithmic bytes = N x (256 B BF16 row + 4 B norm); the "
+                             "predicate bitmaps (N/8 B per distinct program) are left out"}
+    if roof_main is None:
+        roof_main = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": int(traffic) if traffic else None, "kernel": SCAN_KERNEL,
+                     "avg_launch_us": round(avg_scan, 2)}
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "queries/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(steps_ms, 3),
@@ -551,7 +600,9 @@ def main():
                    "k": args.k, "topb": args.k,
                    "graph": "HNSW M=16 (M0=32) efC=200" if lrn else None,
                    "train_queries": args.train_queries if lrn else 0},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+        "roofline": roof_main,
+        "roofline_microbatch": {
+                     "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": int(traffic) if traffic else None, "kernel": SCAN_KERNEL,
                      "avg_launch_us": round(avg_scan, 2),
@@ -749,9 +800,12 @@ def graph_path(args):
                 best = row
         for bh in staged:
             g.free_batch(bh)
+    # roofline of the launch shape of the reported row (its ef and frontier)
     b0 = build_batches(d, post, args.n, min(nq, 1024), 1024, args.k, ent)[0][2]
-    roof = graph_launch(g, d["queries"][:min(nq, 1024)], b0, args.graph_ef,
-                        max(200_000, 400 * args.graph_ef), 128)
+    ref_ef = best["ef"] if best else args.graph_ef
+    ref_fcap = -1024 if best and best["frontier"] == "bounded" else 1024
+    roof = graph_launch(g, d["queries"][:min(nq, 1024)], b0, ref_ef,
+                        max(200_000, 400 * ref_ef), 128, fcap=ref_fcap)
     log(f"graph kernel roofline: {roof}")
     cpu = None if args.no_cpu else graph_cpu_baseline(d, cols, rows, vals, graph, ent, truth, args)
     g.close()
