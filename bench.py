@@ -505,6 +505,9 @@ def main():
     # predicate compile, H2D staging, pipeline, D2H), one pass, for the
     # PCIe-inclusive rate, with its per-stage breakdown
     qps_sync, sync_detail = sync_abi(g, d, batches, args)
+    # the same entry point at the reference's largest batch (FV_GPU_MAX_BATCH)
+    qps_syncL, sync_detailL = sync_abi(g, d, batchesL, args)
+    sync_detail["batches_of_1024"] = dict(sync_detailL, qps=round(qps_syncL, 1))
 
     # ---- correctness: BF16 == FP32 (both exact), batch sizes agree, sample == oracle
     exact = same_results(res_bf, res_fp) and same_results(
@@ -728,8 +731,9 @@ def sync_abi(g, d, batches, args):
     g.set_timing(False)
     per_call = el / len(batches) * 1e6
     dev_us = float(np.mean(dev))
-    log(f"fv_gpu_brute_batch (host buffers, synchronous): {qps:,.0f} QPS; {per_call:.1f} us "
-        f"per call, of which {dev_us:.1f} us device pipeline")
+    log(f"fv_gpu_brute_batch (host buffers, synchronous, {batches[0][1] - batches[0][0]} "
+        f"queries per call): {qps:,.0f} QPS; {per_call:.1f} us per call, of which "
+        f"{dev_us:.1f} us device pipeline")
     return qps, {"us_per_call": round(per_call, 1), "device_pipeline_us": round(dev_us, 1),
                  "host_and_transfer_us": round(per_call - dev_us, 1),
                  "note": "per call: predicate compile and packing on the host, one H2D copy, "

@@ -542,6 +542,10 @@ typedef struct FvGpuLearnedOpts {
                               the run is then the brute pipeline alone) */
     uint64_t max_visited;  /* 0 = max(200000, 400 ef) */
     int32_t precision;     /* brute scan precision, -1 = index default */
+    int32_t graph_cus;     /* reserved, must be 0: the graph-routed searches
+                              share the chip with the packed brute scan (a
+                              split of the CUs between the two is not
+                              available; non-zero fails the call) */
 } FvGpuLearnedOpts;
 int fv_gpu_batch_run_learned(FvGpuIndex *g, FvGpuBatch *b, FvGpuModel *m,
                              const FvGpuLearnedOpts *o, char *errbuf, size_t errlen);

@@ -75,12 +75,12 @@ class LearnedOpts(C.Structure):
     _fields_ = [("k", C.c_uint32), ("mode", C.c_int32), ("ef_fixed", C.c_uint32),
                 ("ef_min", C.c_uint32), ("ef_max", C.c_uint32), ("ef_scale", C.c_float),
                 ("route_bias", C.c_float), ("max_visited", C.c_uint64),
-                ("precision", C.c_int32)]
+                ("precision", C.c_int32), ("graph_cus", C.c_int32)]
 
     def __init__(self, k=10, mode=0, ef_fixed=400, ef_min=10, ef_max=512, ef_scale=1.0,
-                 route_bias=0.0, max_visited=0, precision=-1):
+                 route_bias=0.0, max_visited=0, precision=-1, graph_cus=0):
         super().__init__(k, mode, ef_fixed, ef_min, ef_max, ef_scale, route_bias, max_visited,
-                         precision)
+                         precision, graph_cus)
 
 
 class Feedback(C.Structure):

@@ -857,6 +857,9 @@ extern "C" int fv_gpu_batch_run_learned(FvGpuIndex *g, FvGpuBatch *b, FvGpuModel
     if (g == nullptr || b == nullptr || b->owner != g || o == nullptr)
         PV_FAIL("gpu: learned run: bad arguments");
     if (o->mode < 0 || o->mode > 3) PV_FAIL("gpu: learned run: bad mode %d", o->mode);
+    if (o->graph_cus != 0)
+        PV_FAIL("gpu: learned run: graph_cus %d: splitting the chip between the graph "
+                "searches and the scan is not available (must be 0)", o->graph_cus);
     if (o->k != b->topb) PV_FAIL("gpu: learned run: k %u != staged topb %u", o->k, b->topb);
     /* the graph search returns at most FV_GPU_MAX_EF hits: with a larger k
      * the graph cannot answer (the reference falls back to the CPU cursor,

@@ -175,3 +175,78 @@ def test_routed_run_decisions_match_model():
         g.free_batch(bh)
         m.close()
         g.close()
+
+
+@pytest.mark.skip(reason="splitting the chip between a routed run's graph searches and its "
+                  "scan hung on the GPU (persistent graph grid) and crashed the host process "
+                  "(CU-masked stream); cause not found, so the split is not available and "
+                  "graph_cus != 0 fails the call")
+def test_routed_run_persistent_graph_grid_equals_shared():
+    """A routed run over many query tiles with the chip split between the
+    graph searches (graph_cus persistent workgroups, one per CU, taking the
+    graph-routed queries in turn) and the packed brute scan (its grid sized
+    to the other CUs) returns what the shared run returns: same routes,
+    hits, counts and flags; brute-routed queries equal the exact brute
+    force."""
+    n, nq = 40_000, 1024
+    d = synth.gmm128(n, nq, seed=19)
+    cols = synth.columns(d)
+    rows, vals = synth.postings(cols)
+    post = E.Postings(rows, vals)
+    preds = [synth.filter_pred(synth.FILTER_CLASSES[i % 6], d["qcent"][i]) for i in range(nq)]
+    from paves_amd import hnsw
+    g = E.GpuIndex(d["base"], cols, rows, precision=L.FV_GPU_PREC_BF16)
+    gr = hnsw.build_gpu(g, d["base"], M=8, ef_construction=48, threads=8, keep=True)
+    ent = hnsw.entries(gr, d["base"], d["queries"])
+    b = E.concat([E.serialize(p, n, post) for p in preds], n, entries=ent)
+    rng = np.random.default_rng(3)
+    m = learned.Model(128, learned.kmeans(d["base"], k=64, iters=2, sample=4000))
+    F = R.nfeat(128)
+    p = (rng.standard_normal(R.nparams(F)) * 0.1).astype(np.float32)
+    bh = g.stage(d["queries"], b, 10)
+    try:
+        f = m.batch_features(g, bh, nq)
+        m.set_params(p, f.mean(0), f.std(0) + 1e-3)
+        ref = None
+        for cus in (0, 32, 64, 32):
+            print(f"case routed run, persistent graph grid of {cus} CUs", flush=True)
+            g.run_learned(bh, m, L.LearnedOpts(k=10, mode=0, ef_min=10, ef_max=64,
+                                               max_visited=20_000, graph_cus=cus))
+            g.sync()
+            res = g.fetch(bh, nq, 10)
+            assert res[3] == 0
+            route, ef = g.routes(bh, nq)
+            if ref is None:
+                ref = (res, route, ef)
+                assert 0 < int(route.sum()) < nq
+                continue
+            assert np.array_equal(route, ref[1]) and np.array_equal(ef, ref[2])
+            for x, y in zip(res[:3], ref[0][:3]):
+                assert np.array_equal(x, y), cus
+        out, cnt, _, _ = ref[0]
+        bo, bc, _ = g.brute(d["queries"], E.concat([E.serialize(q, n, post) for q in preds], n), 10)
+        for q in np.flatnonzero(ref[1] == 0):
+            assert cnt[q] == bc[q] and np.array_equal(out[q, : cnt[q]]["node"], bo[q, : bc[q]]["node"])
+    finally:
+        g.free_batch(bh)
+        m.close()
+        g.close()
+
+
+def test_graph_cus_rejected():
+    """graph_cus != 0 (a split of the chip) fails the call before any
+    device work, with a message; 0 runs."""
+    n, nq = 5_000, 128
+    d = synth.gmm128(n, nq, seed=23)
+    g = E.GpuIndex(d["base"], None, None, precision=L.FV_GPU_PREC_BF16)
+    b = E.concat([E.serialize(None, n) for _ in range(nq)], n)
+    bh = g.stage(d["queries"], b, 10)
+    try:
+        with pytest.raises(E.GpuError, match="graph_cus"):
+            g.run_learned(bh, None, L.LearnedOpts(k=10, mode=3, graph_cus=32))
+        g.run_learned(bh, None, L.LearnedOpts(k=10, mode=3))
+        g.sync()
+        assert g.fetch(bh, nq, 10)[3] == 0
+    finally:
+        g.free_batch(bh)
+        g.close()
