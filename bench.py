@@ -497,7 +497,7 @@ def main():
     fp32_scan = measure_fp32_scan(g, staged, fp, args.n)
 
     # ---- SQ8 (exact INT8 MFMA scan of x - centre) on a second index
-    sq8 = sq8_path(d, cols, rows, batches, res_bf, args)
+    sq8 = sq8_path(d, cols, rows, batches, res_bf, args, batchesL)
     # ---- SQ8 over non-integer data: quantized codes + exact re-rank
     sq8q, fp8q = sq8_quant_path(args)
 
@@ -1206,7 +1206,7 @@ def uneven_path(args):
     print(json.dumps(res), flush=True)
 
 
-def sq8_path(d, cols, rows, batches, res_bf, args):
+def sq8_path(d, cols, rows, batches, res_bf, args, batchesL=None):
     """The same micro-batches on an index uploaded with FV_GPU_PREC_SQ8:
     integer data spanning <= 255 values scan as int8 (x - centre) with
     v_mfma_i32_32x32x32_i8 — exact, half the bytes of BF16. Throughput,
@@ -1233,6 +1233,25 @@ def sq8_path(d, cols, rows, batches, res_bf, args):
             scan_us.append(g8.time_scan(bh, sq, 10))
     for bh in staged8:
         g8.free_batch(bh)
+    # batches of 1024 on the SQ8 index: the library's default (the exact
+    # query-resident BF16 scan) and the INT8 code scan asked for
+    batched = None
+    if batchesL:
+        stagedL8 = [g8.stage(d["queries"][s:e], b, args.k) for s, e, b in batchesL]
+        batched = {}
+        for tag, o in (("default", L.BruteOpts(route=1)),
+                       ("int8_asked", L.BruteOpts(route=1, precision=L.FV_GPU_PREC_SQ8))):
+            run_steps(g8, stagedL8, o, max(1, args.warmup))
+            dtb = run_steps(g8, stagedL8, o, max(2, args.steps // 4))
+            g8.run_brute(stagedL8[0], o)
+            g8.sync()
+            batched[tag] = {"qps": round(max(2, args.steps // 4) * args.nq / dtb, 1),
+                            "precision": int(g8.stats().precision)}
+        log(f"sq8 index, batches of {args.learned_batch}: default (query-resident BF16, "
+            f"exact) {batched['default']['qps']:,.0f} QPS; INT8 scan asked "
+            f"{batched['int8_asked']['qps']:,.0f} QPS")
+        for bh in stagedL8:
+            g8.free_batch(bh)
     g8.close()
     if not scan_us:
         log("sq8: the INT8 scan was not taken")
@@ -1243,6 +1262,7 @@ def sq8_path(d, cols, rows, batches, res_bf, args):
     log(f"sq8 (INT8 scan): {qps8:,.0f} QPS, results equal BF16: {same}; scan kernel "
         f"{us:.1f} us -> {gbs:.0f} GB/s")
     return {"qps": round(qps8, 1), "int8_scan_taken": True, "batches_int8": used,
+            "batches_of_1024": batched,
             "exact_vs_bf16": same, "kernel": "k_scan_i8<false, 8, 2, false>",
             "avg_launch_us": round(us, 2), "alg_bytes_per_launch": int(alg),
             "achieved_GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}

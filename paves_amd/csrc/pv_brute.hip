@@ -2629,6 +2629,15 @@ int scan_precision(const FvGpuIndex *g, const FvGpuBatch *b, const FvGpuBruteOpt
     if (p == FV_GPU_PREC_FP8) p = FV_GPU_PREC_SQ8;
     if (p == FV_GPU_PREC_SQ8 && !(g->d_vi8 != nullptr && b != nullptr && b->i8_ok))
         p = FV_GPU_PREC_BF16;
+    /* a batch of kQrNW or more query tiles at 128 dims that the call does
+     * not pin to a precision takes the query-resident BF16 scan also on an
+     * SQ8 / FP8 index: exact results, and faster there than the code scan,
+     * which re-reads the code rows once per query tile */
+    if (!asked && p == FV_GPU_PREC_SQ8 && b != nullptr && b->nqt >= kQrNW &&
+        g->d_vbf != nullptr && g->dimb == 128) {
+        const bool inl_req = o && o->inline_pred && b->inl_ok && b->inl_col >= 0;
+        if (!inl_req || (b->inl_rank && g->d_rank != nullptr)) p = FV_GPU_PREC_BF16;
+    }
     if (p == FV_GPU_PREC_BF16 && (g->d_vbf == nullptr || g->dimb % 128 != 0 || g->dimb > 1024))
         p = FV_GPU_PREC_FP32;
     if (p == FV_GPU_PREC_FP32 && (g->dim % 8 != 0 || g->dim > 512)) p = FV_GPU_PREC_FP32_VALU;

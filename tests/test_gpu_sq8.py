@@ -144,3 +144,35 @@ def test_int8_inline_uneven_batch():
             g.free_batch(bh)
     finally:
         g.close()
+
+
+@pytest.mark.parametrize("float_rows", [False, True])
+def test_large_batch_takes_exact_query_resident_scan(float_rows):
+    """An SQ8 index served a batch of 16 query tiles without a precision
+    asked: the query-resident BF16 scan runs (reported BF16) and the result
+    is the exact top-k bit for bit — on integer rows (exact INT8 copy) and
+    on float rows (quantized copy). Asking for SQ8 keeps the code scan."""
+    n, nq = 30_011, 1024
+    d, cols, post = _data(n, nq, 128, 611)
+    base = d["base"]
+    if float_rows:
+        base = (base + np.random.default_rng(7).random(base.shape, dtype=np.float32) * 0.5)
+        base = np.ascontiguousarray(base.astype(np.float32))
+    preds = [synth.filter_pred(synth.FILTER_CLASSES[i % 6], d["qcent"][i]) for i in range(nq)]
+    b = E.concat([E.serialize(p, n, post) for p in preds], n)
+    g = E.GpuIndex(base, cols, post.rows, precision=L.FV_GPU_PREC_SQ8, device=0)
+    try:
+        print(f"case sq8 large batch float_rows={float_rows}", flush=True)
+        out, cnt, trunc = g.brute(d["queries"], b, 10, L.BruteOpts(route=1))
+        assert g.stats().precision == L.FV_GPU_PREC_BF16
+        oh, oc, ot, _ = O.brute_batch(base, cols, d["queries"], b.qd, b.nodes, b.children,
+                                      b.invals, 10)
+        assert_same(out, cnt, trunc, oh, oc, ot, f"sq8 index, {nq} queries")
+        # the synchronous reference entry point gets the same
+        o2, c2, t2 = g.brute(d["queries"], b, 10)
+        for x, y in zip((o2, c2, t2), (out, cnt, trunc)):
+            assert np.array_equal(x, y)
+        g.brute(d["queries"], b, 10, L.BruteOpts(route=1, precision=L.FV_GPU_PREC_SQ8))
+        assert g.stats().precision == L.FV_GPU_PREC_SQ8
+    finally:
+        g.close()
