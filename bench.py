@@ -11,12 +11,12 @@ once, then a run per batch). Batches are spread over the library's batch
 streams, so one batch's small kernels overlap another batch's scan.
 
 One step = one pass over all 10,000 queries. `value` = queries per second
-of the learned path: routed runs of 1024 queries (fv_gpu_batch_run_learned,
+of the learned path: routed runs of 4096 queries (fv_gpu_batch_run_learned,
 mode 0) whose route threshold and ef factor are calibrated on held-out
 training queries to the fastest setting with recall@10 >= 0.9 in every
 filter class (a threshold of +inf when no graph share pays: the run is the
 BF16 brute pipeline). Beside it, on the same staged batches: BF16 brute
-force at micro-batches of 64 and batches of 1024, the plain FP32 path (FP32
+force at micro-batches of 64 and batches of 4096, the plain FP32 path (FP32
 MFMA scan + the same certified re-rank) at both sizes, SQ8 on integer data
 (exact INT8) and on a float copy of the workload (quantized codes +
 exact re-rank of the top 4 k), the reference's synchronous entry point with
@@ -26,7 +26,7 @@ BF16 and FP32 results are checked against each other bit for bit and a
 micro-batch against the CPU oracle.
 
 roofline: the dominant kernel of the headline's timed region is the
-query-resident BF16 MFMA scan of a routed run of 1024 queries,
+query-resident BF16 MFMA scan of a routed run of 4096 queries,
 k_scan_bf16_qr<8, false, false> (about 80% of each step): MFMA-bound, so
 `roofline` is its TFLOP/s against the 2.5 PFLOP/s dense BF16 peak (HIP
 events over 10 back-to-back launches on the batch's stream; `python
@@ -44,7 +44,7 @@ Algorithmic bytes per launch = N x (2*D bytes of BF16 row + 4 bytes of
 WRITE_SIZE) over the isolated runs (`--roofline-only`), each scaled by what
 the counter reads for a device copy of known size in the same pass; `python
 bench.py --roofline-only` alone is the command whose rocprofv3 kernel trace
-gives the kernel's average duration. The batches of 1024 run the multi-tile
+gives the kernel's average duration. The batches of 4096 run the multi-tile
 scan, reported as `batched_scan` (TFLOP/s of the MFMA peak). The graph
 kernel's launch is accounted from its own counters (neighbour lists of the
 expanded heads, rows and bitmap words of the visited nodes), with a
@@ -330,9 +330,10 @@ def roofline_only(args):
 
 
 def batched_scan_only(args):
-    """The multi-tile scans of 1024-query batches alone (for rocprofv3): the
-    query-resident kernel (library default) and the two-tiles-per-WG kernel,
-    each timed by measure_batched_scan; prints one JSON line."""
+    """The multi-tile scans of one batch of --learned-batch queries alone
+    (for rocprofv3): the query-resident kernel (library default) and the
+    two-tiles-per-WG kernel, each timed by measure_batched_scan; prints one
+    JSON line."""
     d = synth.gmm128(args.n, args.learned_batch)
     cols = synth.columns(d)
     rows, vals = synth.postings(cols)
@@ -368,7 +369,7 @@ def main():
     ap.add_argument("--stream-batches", type=int, default=4,
                     help="uneven workload: distinct batches restaged through one capture")
     ap.add_argument("--batch", type=int, default=64, help="micro-batch of the brute baselines")
-    ap.add_argument("--learned-batch", type=int, default=1024,
+    ap.add_argument("--learned-batch", type=int, default=4096,
                     help="queries per routed run of the learned path (packed brute tiles)")
     ap.add_argument("--k", type=int, default=10)
     ap.add_argument("--train-queries", type=int, default=200_000,
@@ -385,7 +386,7 @@ def main():
                     help="only the isolated scan runs (the rocprofv3 target)")
     ap.add_argument("--copy-calib", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--batched-scan-only", action="store_true",
-                    help="time the multi-tile scans of one 1024-query batch (for rocprofv3)")
+                    help="time the multi-tile scans of one --learned-batch batch (for rocprofv3)")
     ap.add_argument("--graph-roofline-only", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--graph-file", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--graph-ef", type=int, default=80,
@@ -466,7 +467,7 @@ def main():
     bf = L.BruteOpts(precision=L.FV_GPU_PREC_BF16, route=0)
     fp = L.BruteOpts(precision=L.FV_GPU_PREC_FP32, route=0)
 
-    # ---- BF16 brute path: micro-batches of 64, then batches of 1024
+    # ---- BF16 brute path: micro-batches of 64, then batches of --learned-batch
     run_steps(g, staged, bf, args.warmup)
     g.sync()
     dt = run_steps(g, staged, bf, args.steps)
@@ -506,7 +507,9 @@ def main():
     # PCIe-inclusive rate, with its per-stage breakdown
     qps_sync, sync_detail = sync_abi(g, d, batches, args)
     # the same entry point at the reference's largest batch (FV_GPU_MAX_BATCH)
-    qps_syncL, sync_detailL = sync_abi(g, d, batchesL, args)
+    batches1k = batchesL if args.learned_batch == 1024 else build_batches(
+        d, post, args.n, args.nq, 1024, args.k, ent)
+    qps_syncL, sync_detailL = sync_abi(g, d, batches1k, args)
     sync_detail["batches_of_1024"] = dict(sync_detailL, qps=round(qps_syncL, 1))
 
     # ---- correctness: BF16 == FP32 (both exact), batch sizes agree, sample == oracle
@@ -567,7 +570,8 @@ def main():
     if lrn is not None and batched_scan is not None:
         # the dominant kernel of the timed region: the learned path's routed
         # runs spend most of each step in the query-resident scan
-        runs = len([1 for s, e, _ in batchesL])
+        # full-batch scan time, scaled by each run's share of a full batch
+        runs = sum((e - s) / args.learned_batch for s, e, _ in batchesL)
         share = batched_scan["avg_launch_us"] * runs / (steps_ms * 1e3)
         qr_alg = args.n * (128 * 2 + 4)
         roof_main = {"bound": "mfma", "achieved": batched_scan["achieved_TFLOPs"],
@@ -1413,6 +1417,12 @@ def learned_path(g, graph, d, post, staged, batches, truth, args, setup):
     # the beam-search kernel on its roofline: one launch over the first
     # batch of the workload (six classes) at ef = graph_ef
     s0, e0, b0 = batches[0]
+    if e0 - s0 > 1024:
+        # the synchronous graph entry point takes at most FV_GPU_MAX_BATCH
+        e0 = s0 + 1024
+        b0 = E.concat([E.serialize(synth.filter_pred(classes[i % 6], d["qcent"][i]), args.n, post)
+                       for i in range(s0, e0)], args.n,
+                      entries=np.asarray(batches[0][2].qd["entry"])[:1024])
     mvg = max(200_000, 400 * args.graph_ef)
     gk = graph_launch(g, d["queries"][s0:e0], b0, args.graph_ef, mvg, 128)
     gk["traffic"] = None if args.no_traffic else graph_traffic(args, graph,

@@ -322,7 +322,8 @@ int fv_gpu_graph_batch_ex(FvGpuIndex *g, int nq, const float *queries,
  * any number of times with no host<->device copy inside the run. Outputs
  * stay on the device until fv_gpu_batch_fetch. A batch belongs to one
  * index. Used by throughput measurement and by callers that keep queries
- * on the GPU.
+ * on the GPU. A staged batch holds up to 4096 queries (the synchronous
+ * entry points above keep FV_GPU_MAX_BATCH).
  */
 typedef struct FvGpuBatch FvGpuBatch;
 FvGpuBatch *fv_gpu_batch_stage(FvGpuIndex *g, int nq, const float *queries,

@@ -3504,9 +3504,11 @@ extern "C" FvGpuBatch *fv_gpu_batch_stage(
             return nullptr;
         }
         FvGpuBatch *b = new FvGpuBatch();
+        /* device-resident batches take up to kMaxBatchEx queries (the
+         * synchronous entry points keep the reference's FV_GPU_MAX_BATCH) */
         if (stage_batch(g, b, nq, queries, qd, dedup, nodes, nnodes, children,
                         nchildren, invals, ninvals, spans, nspans_total, spanoff,
-                        nspanoff, topb, errbuf, errlen) != 0 ||
+                        nspanoff, topb, errbuf, errlen, nullptr, kMaxBatchEx) != 0 ||
             hipStreamSynchronize(b->stream) != hipSuccess) {
             free_batch_buffers(b);
             delete b;

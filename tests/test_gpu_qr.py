@@ -241,3 +241,31 @@ def test_inline_not_equal_falls_back():
             g.free_batch(bh)
     finally:
         g.close()
+
+
+def test_staged_batch_of_4096():
+    """A device-resident batch of 4096 queries (uniform k): the
+    query-resident scan over 64 query tiles equals the oracle, and a
+    routed run in brute mode returns the same."""
+    n, nq = 20_011, 4096
+    d, cols, post = _setup(n, nq, seed=81)
+    preds = [synth.filter_pred(synth.FILTER_CLASSES[i % 6], d["qcent"][i]) for i in range(nq)]
+    b = E.concat([E.serialize(p, n, post) for p in preds], n)
+    g = E.GpuIndex(d["base"], cols, post.rows, precision=L.FV_GPU_PREC_BF16)
+    bh = g.stage(d["queries"], b, 10)
+    try:
+        g.run_brute(bh, L.BruteOpts(route=1))
+        g.sync()
+        out, cnt, trunc, rc = g.fetch(bh, nq, 10)
+        assert rc == 0
+        oh, oc, ot, _ = O.brute_batch(d["base"], cols, d["queries"], b.qd, b.nodes, b.children,
+                                      b.invals, 10)
+        assert_same(out, cnt, trunc, oh, oc, ot, "4096 staged")
+        g.run_learned(bh, None, L.LearnedOpts(k=10, mode=3))
+        g.sync()
+        r = g.fetch(bh, nq, 10)
+        for x, y in zip(r[:3], (out, cnt, trunc)):
+            assert np.array_equal(x, y)
+    finally:
+        g.free_batch(bh)
+        g.close()
