@@ -482,6 +482,16 @@ def main():
     qps_bfL = args.steps * args.nq / dtL
     log(f"bf16 brute, batches of {args.learned_batch}: {qps_bfL:,.0f} QPS")
     res_bfL = [g.fetch(bh, e - s, args.k) for bh, (s, e, _) in zip(stagedL, batchesL)]
+    # the hits the reference worker asks for: max(4 k, 64) (exec.c:286)
+    wk = max(4 * args.k, 64)
+    stagedW = [g.stage(d["queries"][s:e], b, wk) for s, e, b in batchesL]
+    run_steps(g, stagedW, bf, max(1, args.warmup))
+    dtW = run_steps(g, stagedW, bf, max(2, args.steps // 4))
+    qps_bfW = max(2, args.steps // 4) * args.nq / dtW
+    log(f"bf16 brute, batches of {args.learned_batch}, top-{wk} (the worker's request): "
+        f"{qps_bfW:,.0f} QPS")
+    for bh in stagedW:
+        g.free_batch(bh)
 
     # ---- plain FP32 path (FP32 MFMA x.q + certified exact re-rank) on the
     # same staged batches, at both batch sizes
@@ -635,6 +645,7 @@ def main():
         "recall_at_10": round(rec, 4) if rec is not None else None,
         "bf16_brute_qps": round(qps_bf, 1),
         "bf16_brute_qps_batched": round(qps_bfL, 1),
+        "bf16_brute_qps_batched_worker_topb": {"topb": wk, "qps": round(qps_bfW, 1)},
         "plain_fp32_qps": round(plain, 1),
         "plain_fp32": {"kind": "FP32 MFMA x.q (v_mfma_f32_32x32x2_f32) + certified exact "
                                "FP32 re-rank", "qps_micro_batches": round(qps32, 1),

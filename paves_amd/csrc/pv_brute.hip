@@ -298,7 +298,7 @@ struct ScanArgs {
     uint64_t *pool;
     uint32_t *pcnt;
     uint64_t cap_pool;
-    uint64_t *gmin;
+    uint32_t *gmin;       /* [nq][ngroups] pre-pass minima (FP32 bits, >= 0) */
     uint32_t ngroups;
     const uint32_t *ones;     /* all-ones bitmap row (queries without predicate) */
     uint32_t *status;         /* per-query status bits */
@@ -605,7 +605,7 @@ __device__ __forceinline__ void tile_epilogue(const ScanArgs &A, int qbase, uint
             for (int i = 0; i < 16; i++)
                 if ((word >> acc_row(i, l)) & 1u) m = fminf(m, acc[qb][i]);
             const uint32_t grp = t * 2 + (l >> 5);
-            A.gmin[(uint64_t)q * A.ngroups + grp] = make_key(m, grp);
+            A.gmin[(uint64_t)q * A.ngroups + grp] = __float_as_uint(m);
         } else {
             const float tau = tq[qb];
 #pragma unroll
@@ -724,7 +724,7 @@ __device__ __forceinline__ void epilogue_bf(const ScanArgs &A, int qbase, uint32
             /* all rows filtered: mx = -inf, d = +inf */
             const float d = fmaxf(fmaf(-2.f, mx, qn[qb]), 0.f);
             const uint32_t grp = t * 2 + (l >> 5);
-            A.gmin[(uint64_t)q * A.ngroups + grp] = make_key(d, grp);
+            A.gmin[(uint64_t)q * A.ngroups + grp] = __float_as_uint(d);
         } else {
             const float h = bf_threshold(A, qn[qb], tq[qb]);
             /* lane-divergent: the wave skips the insertion code when no
@@ -764,7 +764,7 @@ __device__ __forceinline__ void epilogue_i8(const ScanArgs &A, int qbase, uint32
             const float d = mx > kI8Off / 2 ? (float)(qni + 1 - 2 * mx)
                                              : __uint_as_float(0x7f800000u);
             const uint32_t grp = t * 2 + (l >> 5);
-            A.gmin[(uint64_t)q * A.ngroups + grp] = make_key(d, grp);
+            A.gmin[(uint64_t)q * A.ngroups + grp] = __float_as_uint(d);
         } else if (mx >= th[qb]) {
 #pragma unroll
             for (int i = 0; i < 16; i++)
@@ -1433,7 +1433,7 @@ void k_scan_bf16_qr(ScanArgs A) {
                         const float dmin = mn >= 0x1p126f ? __builtin_inff()
                                                           : fmaxf(qn[qb] + mn, 0.f);
                         const uint32_t grp = t * 2 + (uint32_t)half;
-                        A.gmin[(uint64_t)q * A.ngroups + grp] = make_key(dmin, grp);
+                        A.gmin[(uint64_t)q * A.ngroups + grp] = __float_as_uint(dmin);
                     }
                 } else if (mn <= h) {
 #pragma unroll
@@ -2027,7 +2027,7 @@ __device__ uint64_t block_kth_small(Get get, uint32_t n, uint32_t B, uint64_t *s
  * widened by 2 eps and a relative 2^-20 slack; also clears the pool
  * counter and status (a fill, so the pipeline can be graph-captured). */
 struct TauArgs {
-    const uint64_t *gmin;
+    const uint32_t *gmin;   /* FP32 bits of the group minima */
     uint32_t ngroups, B;
     uint32_t over;          /* candidates kept per query: over x k (at most kSelMax) */
     const float *eps;
@@ -2042,7 +2042,7 @@ __device__ __forceinline__ void tau_block(const TauArgs &A, int q) {
     __shared__ uint32_t hist[256], tmp[8], bc[8];
     __shared__ uint64_t sel[1024], wb[4];
     __shared__ uint32_t nsel;
-    const uint64_t *__restrict__ gmin = A.gmin;
+    const uint32_t *__restrict__ gmin = A.gmin;
     const uint32_t ngroups = A.ngroups;
     uint32_t B = A.B;
     const float *__restrict__ eps = A.eps;
@@ -2061,14 +2061,16 @@ __device__ __forceinline__ void tau_block(const TauArgs &A, int q) {
         }
         return;
     }
-    const uint64_t *g = gmin + (uint64_t)q * ngroups;
+    const uint32_t *g = gmin + (uint64_t)q * ngroups;
+    /* (distance, group) keys: distinct, ordered by distance first (the
+     * minima are non-negative, so their bits order as the values) */
+    auto key = [&](uint32_t i) { return ((uint64_t)g[i] << 32) | i; };
     float tv = __uint_as_float(0x7f800000u);
     if (ngroups >= B) {
         uint32_t m;
         const uint64_t kth =
-            B <= 64 ? block_kth_small([&](uint32_t i) { return g[i]; }, ngroups, B, sel, 1024,
-                                      &nsel, wb, hist, tmp, bc, &m)
-                    : block_kth([&](uint32_t i) { return g[i]; }, ngroups, B, hist, tmp, bc);
+            B <= 64 ? block_kth_small(key, ngroups, B, sel, 1024, &nsel, wb, hist, tmp, bc, &m)
+                    : block_kth(key, ngroups, B, hist, tmp, bc);
         const float d = key_dist(kth);
         if (d < __uint_as_float(0x7f800000u)) {
             const double w = (double)d * (1.0 + ldexp(1.0, -20)) +
@@ -3077,7 +3079,7 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
     b->ngroups = stiles * 2 * (uint32_t)pt;
     b->cap_pool = std::max<uint64_t>(16384, (uint64_t)8 * Bc * stride);
     if (stride_override == 1) b->cap_pool = std::max<uint64_t>(b->cap_pool, (uint64_t)64 * Bc + 4096);
-    PV_TRY(b->d_gmin.ensure((size_t)nqp * b->ngroups * 8));
+    PV_TRY(b->d_gmin.ensure((size_t)nqp * b->ngroups * 4));
     PV_TRY(b->d_pool.ensure((size_t)nq * b->cap_pool * 8));
 
     ScanArgs A = {};
@@ -3099,7 +3101,7 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
     A.pool = b->d_pool.as<uint64_t>();
     A.pcnt = b->d_pcnt.as<uint32_t>();
     A.cap_pool = b->cap_pool;
-    A.gmin = b->d_gmin.as<uint64_t>();
+    A.gmin = b->d_gmin.as<uint32_t>();
     A.ngroups = b->ngroups;
     A.ones = g->d_ones;
     A.status = qa.status;
@@ -3194,7 +3196,7 @@ int run_brute_ex(FvGpuIndex *g, FvGpuBatch *b, const FvGpuBruteOpts *o,
     ev_record(g, 2);
     {
         TauArgs T;
-        T.gmin = b->d_gmin.as<uint64_t>();
+        T.gmin = b->d_gmin.as<uint32_t>();
         T.ngroups = b->ngroups;
         T.B = B;
         T.over = over;

@@ -258,7 +258,7 @@ struct FvGpuBatch {
     pv::DevBuf d_qbf;     /* [nqt*64][dim] bf16 */
     pv::DevBuf d_qaux;    /* qnorm f32 [nq_pad], eps f32 [nq_pad], tau f32 [nq_pad] */
     pv::DevBuf d_bitmap;  /* [nprog][words] u32 */
-    pv::DevBuf d_gmin;    /* prepass group minima [nq][ngroups] u64 */
+    pv::DevBuf d_gmin;    /* prepass group minima [nq][ngroups] (FP32 bits) */
     pv::DevBuf d_pool;    /* candidate pool [nq][cap_pool] u64 */
     pv::DevBuf d_pcnt;    /* [nq] u32 pool counts */
     uint64_t cap_pool = 0;
