@@ -732,7 +732,11 @@ def measure_batched_scan(g, stagedL, batchesL, bf, args, reps=5):
 
 def sync_abi(g, d, batches, args):
     """fv_gpu_brute_batch (host buffers in and out) over every micro-batch,
-    once; with event timing, the device time of its pipeline per call."""
+    once, after one untimed call that sizes the library's buffers for this
+    batch shape (a worker calls it repeatedly); with event timing, the
+    device time of its pipeline per call."""
+    s, e, b = batches[0]
+    g.brute(d["queries"][s:e], b, args.k)
     t0 = time.perf_counter()
     for s, e, b in batches:
         g.brute(d["queries"][s:e], b, args.k)

@@ -17,7 +17,6 @@
  */
 #include <string.h>
 
-#include <unordered_map>
 
 #include "pv_device.h"
 
@@ -139,7 +138,12 @@ int compile_programs(int nq, const FvGpuQueryDesc *qd,
     ps->off.clear();
     ps->len.clear();
     ps->of_query.assign(nq, -1);
-    std::unordered_map<uint64_t, std::vector<int32_t>> seen;
+    /* distinct programs by content hash: a flat open-addressed table of
+     * program indices, at most half full (at most nq programs) */
+    size_t cap = 16;
+    while (cap < 2 * (size_t)(nq > 0 ? nq : 1)) cap <<= 1;
+    std::vector<int32_t> slot(cap, -1);
+    std::vector<uint64_t> phash;
     std::vector<PvOp> tmp;
     for (int q = 0; q < nq; q++) {
         if (qd[q].proot < 0) continue;
@@ -163,19 +167,23 @@ int compile_programs(int nq, const FvGpuQueryDesc *qd,
             if (op.kind == OP_IN) h = fnv(invals + op.a, 8 * (size_t)op.b, fnv(&op.b, 4, h));
         }
         int32_t found = -1;
-        auto it = seen.find(h);
-        if (it != seen.end())
-            for (int32_t p : it->second)
-                if (same_prog(*ps, ps->off[p], ps->len[p], tmp, 0, (int32_t)tmp.size(), invals)) {
-                    found = p;
-                    break;
-                }
+        size_t i = (size_t)(h ^ (h >> 29)) & (cap - 1);
+        for (; slot[i] >= 0; i = (i + 1) & (cap - 1)) {
+            const int32_t p = slot[i];
+            if (phash[(size_t)p] == h &&
+                same_prog(*ps, ps->off[p], ps->len[p], tmp, 0, (int32_t)tmp.size(), invals)) {
+                found = p;
+                break;
+            }
+        }
         if (found < 0) {
+            /* programs are numbered in order of first use */
             found = (int32_t)ps->off.size();
             ps->off.push_back((int32_t)ps->ops.size());
             ps->len.push_back((int32_t)tmp.size());
             ps->ops.insert(ps->ops.end(), tmp.begin(), tmp.end());
-            seen[h].push_back(found);
+            phash.push_back(h);
+            slot[i] = found;
         }
         ps->of_query[q] = found;
     }
