@@ -2855,9 +2855,13 @@ static ScanShape scan_shape(const FvGpuIndex *g, const FvGpuBatch *b, const FvGp
              * doubles the pre-pass. sqrt(2560 / B) balances the two at 1M x
              * 128 with 1024 queries (measured optimum 16 at B = 10: 317 us
              * per batch against 365 us at 64 and 337 us at 8), within the
-             * per-WG slots (B x stride / gx per query) */
+             * per-WG slots: about B x stride / gx candidates per query per
+             * WG against the query's kQrSlots (its two lane halves split
+             * the rows evenly). At B = 64 with 32 chunks that allows 6
+             * (2.74M QPS on 4096-query batches at 1M x 128, against 2.47M
+             * at 4 and 2.67M at 8); past the slots a WG spills, exactly */
             stride = (uint32_t)std::lround(std::sqrt(2560.0 / std::max<uint32_t>(1, B)));
-            stride = std::min<uint32_t>(stride, std::max<uint32_t>(1, (uint32_t)kQrSlots / 2 * sh.gx / B));
+            stride = std::min<uint32_t>(stride, std::max<uint32_t>(1, (uint32_t)kQrSlots * sh.gx / B));
             stride = std::min<uint32_t>(64, std::max<uint32_t>(1, stride));
             stride = std::max(stride, (ntiles + 131071) / 131072);
         }
