@@ -269,3 +269,28 @@ def test_staged_batch_of_4096():
     finally:
         g.free_batch(bh)
         g.close()
+
+
+def test_staged_batch_of_4096_worker_top64():
+    """The reference worker's request, top-64 (max(4k, 64), exec.c:286), on
+    a 4096-query batch: the default shape (pre-pass stride 6 at B = 64 with
+    32 row chunks, about 12 candidates per query per WG against its 16
+    slots) equals the oracle, and so does a run at stride 1."""
+    n, nq, k = 20_011, 4096, 64
+    d, cols, post = _setup(n, nq, seed=83)
+    preds = [synth.filter_pred(synth.FILTER_CLASSES[i % 6], d["qcent"][i]) for i in range(nq)]
+    b = E.concat([E.serialize(p, n, post) for p in preds], n)
+    g = E.GpuIndex(d["base"], cols, post.rows, precision=L.FV_GPU_PREC_BF16)
+    bh = g.stage(d["queries"], b, k)
+    try:
+        oh, oc, ot, _ = O.brute_batch(d["base"], cols, d["queries"], b.qd, b.nodes, b.children,
+                                      b.invals, k)
+        for o in (L.BruteOpts(route=1), L.BruteOpts(route=1, sample_stride=1)):
+            g.run_brute(bh, o)
+            g.sync()
+            out, cnt, trunc, rc = g.fetch(bh, nq, k)
+            assert rc == 0
+            assert_same(out, cnt, trunc, oh, oc, ot, f"4096 staged top-{k} stride {o.sample_stride}")
+    finally:
+        g.free_batch(bh)
+        g.close()
