@@ -667,18 +667,31 @@ __device__ __forceinline__ void seed_bf(const float (&sn)[16], uint32_t w0, uint
 
 /* INT8 seeds from xr = ||x||^2 of the lane's own row (lane & 31, an
  * integer): -floor(xr / 2) is formed once per lane and spread to the 16
- * accumulator rows; par gets bit r = ||x_r||^2 is odd for rows 0..31 (one
- * ballot), read only when a candidate is emitted */
+ * accumulator rows through the wave's 32-entry LDS slot (one store per
+ * row lane, four 16-byte reads of rows g * 8 + (l >> 5) * 4 .. +3; a
+ * wave's LDS operations run in order); par gets bit r = ||x_r||^2 is odd
+ * for rows 0..31 (one ballot), read only when a candidate is emitted */
 __device__ __forceinline__ void seed_i8(float xr, uint32_t w0, uint32_t w1,
-                                        i32x16 &c0, i32x16 &c1, uint32_t &par) {
+                                        i32x16 &c0, i32x16 &c1, uint32_t &par,
+                                        int32_t *slot) {
     const int l = threadIdx.x & 63;
     const int32_t xi = (int32_t)xr;
     const int32_t half = -(xi >> 1);
     par = (uint32_t)__ballot(xi & 1);
     const uint32_t v0 = lane_word(w0), v1 = lane_word(w1);
+    if (l < 32) slot[l] = half;
+    int32_t hs[16];
+#pragma unroll
+    for (int g = 0; g < 4; g++) {
+        const int4 v = *(const int4 *)(slot + g * 8 + (l >> 5) * 4);
+        hs[4 * g] = v.x;
+        hs[4 * g + 1] = v.y;
+        hs[4 * g + 2] = v.z;
+        hs[4 * g + 3] = v.w;
+    }
 #pragma unroll
     for (int i = 0; i < 16; i++) {
-        const uint32_t s = (uint32_t)__shfl(half, acc_row(i, l));
+        const uint32_t s = (uint32_t)hs[i];
         const uint32_t m0 = elem_mask(v0, i), m1 = elem_mask(v1, i);
         c0[i] = (int32_t)((s & m0) | ((uint32_t)kI8Off & ~m0));
         c1[i] = (int32_t)((s & m1) | ((uint32_t)kI8Off & ~m1));
@@ -1534,6 +1547,7 @@ void k_scan_i8(ScanArgs A) {
     if (tile_skipped(A, qbase)) return;
 
     __shared__ uint32_t s_next;
+    __shared__ __attribute__((aligned(16))) int32_t s_half[NW][32]; /* seed spread */
     UnitCursor cur = cursor_init<NW>(A, wave, &s_next);
     const u32x4 *vb = (const u32x4 *)A.vi8;
     /* physical unit of a logical tile (units past the range load a valid one) */
@@ -1613,7 +1627,8 @@ void k_scan_i8(ScanArgs A) {
             } else {
                 i32x16 c0, c1;
                 uint32_t par;
-                seed_i8(in[p].xr, w0 & in[p].valid, w1 & in[p].valid, c0, c1, par);
+                seed_i8(in[p].xr, w0 & in[p].valid, w1 & in[p].valid, c0, c1, par,
+                        s_half[wave]);
 #pragma unroll
                 for (int k = 0; k < KS; k++) {
                     const i32x4 av = __builtin_bit_cast(i32x4, a[p * KS + k]);
@@ -1661,6 +1676,7 @@ void k_scan_i8w(ScanArgs A) {
     if (tile_skipped(A, qbase)) return;
 
     __shared__ uint32_t s_next;
+    __shared__ __attribute__((aligned(16))) int32_t s_half[NW][32]; /* seed spread */
     UnitCursor cur = cursor_init<NW>(A, wave, &s_next);
     const u32x4 *vb = (const u32x4 *)A.vi8;
     const uint32_t tlast = cursor_last(cur, A.ntiles);
@@ -1717,7 +1733,7 @@ void k_scan_i8w(ScanArgs A) {
                 expand_norms(-0.5f * in.xr, sn);
                 seed_bf(sn, w0 & in.valid, w1 & in.valid, c0, c1);
             } else {
-                seed_i8(in.xr, w0 & in.valid, w1 & in.valid, c0, c1, par);
+                seed_i8(in.xr, w0 & in.valid, w1 & in.valid, c0, c1, par, s_half[wave]);
             }
         }
 #pragma unroll
