@@ -523,6 +523,22 @@ __device__ __forceinline__ void expand_norms(float xr, float (&xn)[16]) {
     for (int i = 0; i < 16; i++) xn[i] = __shfl(xr, acc_row(i, l));
 }
 
+/* expand_norms through the wave's 32-entry LDS slot: one store per row
+ * lane, four 16-byte reads of rows g * 8 + (l >> 5) * 4 .. +3 (a wave's
+ * LDS operations run in order) */
+__device__ __forceinline__ void expand_norms_lds(float xr, float (&xn)[16], int32_t *slot) {
+    const int l = threadIdx.x & 63;
+    if (l < 32) slot[l] = __float_as_int(xr);
+#pragma unroll
+    for (int g = 0; g < 4; g++) {
+        const float4 v = *(const float4 *)(slot + g * 8 + (l >> 5) * 4);
+        xn[4 * g] = v.x;
+        xn[4 * g + 1] = v.y;
+        xn[4 * g + 2] = v.z;
+        xn[4 * g + 3] = v.w;
+    }
+}
+
 /* inline predicates: the column values of the lane's 16 accumulator rows,
  * gathered from the lanes that loaded them (one value per row per lane) */
 __device__ __forceinline__ void expand_rows(double rv, double (&cv)[16]) {
@@ -1610,7 +1626,7 @@ void k_scan_i8(ScanArgs A) {
                  * distance; each lane's 16 bytes of a k-step are two 8-byte
                  * operands of v_mfma_f32_32x32x16_fp8_fp8 */
                 float sn[16];
-                expand_norms(-0.5f * in[p].xr, sn);
+                expand_norms_lds(-0.5f * in[p].xr, sn, s_half[wave]);
                 f32x16 c0, c1;
                 seed_bf(sn, w0 & in[p].valid, w1 & in[p].valid, c0, c1);
 #pragma unroll
@@ -1730,7 +1746,7 @@ void k_scan_i8w(ScanArgs A) {
             }
             if constexpr (F8) {
                 float sn[16];
-                expand_norms(-0.5f * in.xr, sn);
+                expand_norms_lds(-0.5f * in.xr, sn, s_half[wave]);
                 seed_bf(sn, w0 & in.valid, w1 & in.valid, c0, c1);
             } else {
                 seed_i8(in.xr, w0 & in.valid, w1 & in.valid, c0, c1, par, s_half[wave]);
